@@ -1,0 +1,246 @@
+"""Synthetic treatment-plan fixtures (CT, beams, spots) for the pencil-beam dose path.
+
+Every random draw comes from a counter-based splitmix64 stream written here, so the
+same seed gives the same phantom and spot weights on every machine and numpy version.
+
+The beam dicts use the key names of the reference Python binding
+(src/bindings/raytracedicom_pybind.cpp:1712-1821); the energy dict is
+:func:`subsecondpbs_amd.lut.load_energy_tables`.
+
+Conventions (mm throughout, so the reference's length heuristic picks lenToMm = 1,
+src/core/raytracedicom_wrapper.cu:2728-2729):
+
+* CT and dose volumes are x-fastest float32 ``[nz][ny][nx]``; corners are voxel centres.
+* ``spot_positions`` are gantry-frame (x, y) coordinates on the isocentre plane, which is
+  how the reference's spot-lattice path consumes them (wrapper:1366-1456, 3044-3102).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .lut import load_energy_tables
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """Counter-based RNG: draw ``i`` of stream (seed, name) is splitmix64(key + i)."""
+
+    def __init__(self, seed: int, name: str):
+        h = 1469598103934665603
+        for ch in name.encode():
+            h = ((h ^ ch) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+        base = np.array([(seed * 0x9E3779B97F4A7C15 + h) & 0xFFFFFFFFFFFFFFFF], dtype=np.uint64)
+        self.key = _splitmix64(base)[0]
+        self.counter = 0
+
+    def _raw(self, n: int) -> np.ndarray:
+        idx = np.arange(self.counter, self.counter + n, dtype=np.uint64)
+        self.counter += n
+        with np.errstate(over="ignore"):
+            return _splitmix64(idx * np.uint64(0x9E3779B97F4A7C15) + self.key)
+
+    def uniform(self, n: int) -> np.ndarray:
+        """Doubles in [0, 1) with 53 random bits."""
+        return (self._raw(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n: int) -> np.ndarray:
+        """Box-Muller standard normals."""
+        u1 = 1.0 - self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def _grid_centres(dims, res, corner):
+    xs = corner[0] + res[0] * np.arange(dims[0], dtype=np.float64)
+    ys = corner[1] + res[1] * np.arange(dims[1], dtype=np.float64)
+    zs = corner[2] + res[2] * np.arange(dims[2], dtype=np.float64)
+    return xs, ys, zs
+
+
+def subspot_lut(num_layers: int, sigma_sub: float | np.ndarray, offsets_mm=(-4.0, -2.0, 0.0, 2.0, 4.0),
+                var_weight: float = 12.0) -> np.ndarray:
+    """[L][S][5] subspot table (dx, dy, w, sigma_x, sigma_y); weights ~ exp(-|d|^2/(2 var)), normalised."""
+    offs = np.asarray(offsets_mm, dtype=np.float64)
+    dx, dy = np.meshgrid(offs, offs, indexing="xy")
+    dx = dx.ravel()
+    dy = dy.ravel()
+    w = np.exp(-(dx * dx + dy * dy) / (2.0 * var_weight))
+    w /= w.sum()
+    sig = np.broadcast_to(np.asarray(sigma_sub, dtype=np.float64), (num_layers,))
+    out = np.zeros((num_layers, dx.size, 5), dtype=np.float32)
+    out[:, :, 0] = dx
+    out[:, :, 1] = dy
+    out[:, :, 2] = w
+    out[:, :, 3] = sig[:, None]
+    out[:, :, 4] = sig[:, None]
+    return out
+
+
+def single_subspot_lut(spot_sigmas: np.ndarray) -> np.ndarray:
+    """[L][1][5] table holding one centred subspot per layer with the layer's spot sigma."""
+    L = spot_sigmas.shape[0]
+    out = np.zeros((L, 1, 5), dtype=np.float32)
+    out[:, 0, 2] = 1.0
+    out[:, 0, 3] = spot_sigmas[:, 0]
+    out[:, 0, 4] = spot_sigmas[:, 1]
+    return out
+
+
+def beam_basis(gantry_deg: float, couch_deg: float = 0.0):
+    """Beam direction and lateral axes for a coplanar gantry angle (IEC-like, about world z).
+
+    Gantry 0 points the beam along +z? No: gantry rotates in the x-y plane here. Gantry 90
+    gives a beam along +x with bmX = +y and bmY = +z, so that bmX x bmY = beam direction.
+    """
+    a = math.radians(gantry_deg)
+    bdir = np.array([math.sin(a), -math.cos(a), 0.0])   # 90 deg -> +x ; 0 deg -> -y
+    bmy = np.array([0.0, 0.0, 1.0])
+    bmx = np.cross(bmy, bdir)  # so that cross(bmx, bmy) = bdir
+    bmx /= np.linalg.norm(bmx)
+    return bdir, bmx, bmy
+
+
+def make_beam(energies, spot_xy_per_layer, weights_per_layer, spot_sigmas, *, beam_dir, bm_x, bm_y,
+              sad: float, iso=(0.0, 0.0, 0.0), ray_spacing=(1.0, 1.0), subspots=None) -> dict:
+    energies = np.asarray(energies, dtype=np.float32)
+    L = energies.size
+    counts = np.array([len(w) for w in weights_per_layer], dtype=np.int32)
+    pos = np.concatenate([np.asarray(p, dtype=np.float32).reshape(-1, 2) for p in spot_xy_per_layer])
+    wts = np.concatenate([np.asarray(w, dtype=np.float32) for w in weights_per_layer])
+    spot_sigmas = np.asarray(spot_sigmas, dtype=np.float32).reshape(L, 2)
+    beam_dir = np.asarray(beam_dir, dtype=np.float64)
+    src = np.asarray(iso, dtype=np.float64) - beam_dir * sad
+    if subspots is None:
+        subspots = single_subspot_lut(spot_sigmas)
+    return {
+        "energies": energies,
+        "spot_sigmas": spot_sigmas,
+        "ray_spacing": np.asarray(ray_spacing, dtype=np.float32),
+        "steps": 0,
+        "source_dist": np.array([sad, sad], dtype=np.float32),
+        "spot_offset": np.zeros(3, np.float32),
+        "spot_delta": np.zeros(3, np.float32),
+        "beam_direction": beam_dir.astype(np.float32),
+        "beam_xdir": np.asarray(bm_x, dtype=np.float32),
+        "beam_ydir": np.asarray(bm_y, dtype=np.float32),
+        "sad": np.float32(sad),
+        "source_position": src.astype(np.float32),
+        "ref_plane_z": np.float32(0.0),
+        "layer_spot_counts": counts,
+        "spot_positions": pos.ravel(),
+        "spot_weights": wts,
+        "subspot_data": np.ascontiguousarray(subspots, dtype=np.float32),
+    }
+
+
+def make_config1(seed: int = 1) -> dict:
+    """Water phantom 64^3 at 2 mm with a 20 mm bone-like slab, one beam along +z.
+
+    HU+1000 is supplied directly (water 1000, slab 2000): the reference's CT-type
+    auto-detection reads a non-negative volume as HU+1000 (wrapper:2327-2335).
+    5 layers 100..140 MeV, 11x11 spots at 5 mm, weights U[0.5,1.5], sigma 4 mm,
+    a 5x5 subspot table (offsets -4..4 mm, sigma_sub 2 mm).
+    """
+    dims = (64, 64, 64)
+    res = (2.0, 2.0, 2.0)
+    corner = (-63.0, -63.0, -63.0)
+    ct = np.full((dims[2], dims[1], dims[0]), 1000.0, dtype=np.float32)
+    ct[20:30, :, :] = 2000.0
+    energies = np.array([100.0, 110.0, 120.0, 130.0, 140.0], dtype=np.float32)
+    g = np.arange(-25.0, 25.0 + 1e-6, 5.0)
+    gx, gy = np.meshgrid(g, g, indexing="xy")
+    xy = np.stack([gx.ravel(), gy.ravel()], axis=1)
+    rng = Stream(seed, "config1.weights")
+    weights = [0.5 + rng.uniform(xy.shape[0]) for _ in energies]
+    sig = np.full((energies.size, 2), 4.0, dtype=np.float32)
+    beam = make_beam(energies, [xy] * energies.size, weights, sig,
+                     beam_dir=(0.0, 0.0, 1.0), bm_x=(1.0, 0.0, 0.0), bm_y=(0.0, 1.0, 0.0), sad=2000.0,
+                     subspots=subspot_lut(energies.size, 2.0))
+    return {
+        "name": "config1-water-slab-64",
+        "ct": ct, "ct_dims": dims, "ct_resolution": res, "ct_corner": corner,
+        "dose_dims": dims, "dose_resolution": res, "dose_corner": corner,
+        "beams": [beam], "energy": load_energy_tables(),
+    }
+
+
+def hex_lattice(pitch: float, radius: float):
+    h = pitch * math.sqrt(3.0) / 2.0
+    nmax = int(math.floor(radius / h)) + 1
+    mmax = int(math.floor(radius / pitch)) + 2
+    pts = []
+    for n in range(-nmax, nmax + 1):
+        y = n * h
+        shift = 0.5 * pitch if (n % 2) else 0.0
+        for m in range(-mmax, mmax + 1):
+            x = m * pitch + shift
+            if x * x + y * y <= radius * radius + 1e-9:
+                pts.append((x, y))
+    return np.array(pts, dtype=np.float64)
+
+
+def head_phantom(seed: int = 2):
+    """256x256x160 CT at 1x1x2 mm (HU): ellipsoidal body N(0,20), two bones U[700,1200],
+    an air cavity, air outside."""
+    dims = (256, 256, 160)
+    res = (1.0, 1.0, 2.0)
+    corner = (-127.5, -127.5, -159.0)
+    xs, ys, zs = _grid_centres(dims, res, corner)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    body = (X / 80.0) ** 2 + (Y / 100.0) ** 2 + (Z / 70.0) ** 2 <= 1.0
+    n = X.size
+    ct = np.full(X.shape, -1000.0, dtype=np.float64)
+    noise = Stream(seed, "head.body").normal(n).reshape(X.shape) * 20.0
+    ct[body] = noise[body]
+    ub = Stream(seed, "head.bone").uniform(n).reshape(X.shape)
+    for cx in (-40.0, 40.0):
+        bone = ((X - cx) / 12.0) ** 2 + ((Y - 20.0) / 15.0) ** 2 + (Z / 25.0) ** 2 <= 1.0
+        ct[bone] = 700.0 + 500.0 * ub[bone]
+    air = (X ** 2 + (Y + 35.0) ** 2 + (Z - 20.0) ** 2) <= 144.0
+    ct[air] = -1000.0
+    return ct.astype(np.float32), dims, res, corner
+
+
+def head_beam(seed: int = 2, gantry_deg: float = 90.0, n_layers: int = 30,
+              e_min: float = 70.0, e_max: float = 180.0, radius: float = 26.5) -> dict:
+    """Head-size proton field: layers evenly spaced e_min..e_max MeV, 5 mm hexagonal spot
+    lattice clipped to a disc, log-normal(0, 0.5) weights, sigma = 6 - 3 (E - 70) / 110 mm."""
+    energies = np.linspace(e_min, e_max, n_layers).astype(np.float32)
+    xy = hex_lattice(5.0, radius)
+    rng = Stream(seed, f"head.weights.{gantry_deg:.3f}")
+    weights = [np.exp(0.5 * rng.normal(xy.shape[0])) for _ in energies]
+    s = (6.0 - 3.0 * (energies.astype(np.float64) - 70.0) / 110.0).astype(np.float32)
+    sig = np.stack([s, s], axis=1)
+    bdir, bmx, bmy = beam_basis(gantry_deg)
+    return make_beam(energies, [xy] * energies.size, weights, sig,
+                     beam_dir=bdir, bm_x=bmx, bm_y=bmy, sad=2000.0)
+
+
+def make_config2(seed: int = 2, gantry_deg: float = 90.0) -> dict:
+    """Head-size phantom with one 30-layer proton beam at gantry 90 deg (along +x)."""
+    ct, dims, res, corner = head_phantom(seed)
+    return {
+        "name": "config2-head-256x256x160-30L",
+        "ct": ct, "ct_dims": dims, "ct_resolution": res, "ct_corner": corner,
+        "dose_dims": dims, "dose_resolution": res, "dose_corner": corner,
+        "beams": [head_beam(seed, gantry_deg)], "energy": load_energy_tables(),
+    }
+
+
+def make_plan_4field(seed: int = 2) -> dict:
+    """Config-2 phantom with four coplanar fields (gantry 0/90/180/270): the <100 ms plan."""
+    base = make_config2(seed)
+    base["beams"] = [head_beam(seed, g) for g in (0.0, 90.0, 180.0, 270.0)]
+    base["name"] = "plan-4field-head"
+    return base

@@ -1,0 +1,399 @@
+"""Pins the FP64 oracle against known answers, closed forms and brute force (CPU only).
+
+Each check is chosen so that a plausible slip in the restatement (a dropped factor of 1/2,
+a wrong sign, an off-by-one step, a swapped axis) changes its outcome.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erf
+
+import oracle as O
+from subsecondpbs_amd import fixtures as F
+from subsecondpbs_amd import lut
+
+E = lut.load_energy_tables()
+
+
+# ---------------------------------------------------------------- LUTs --------------------
+def test_lut_token_counts_and_invariants():
+    assert E["n_energies"] == 147 and E["n_energy_samples"] == 1024
+    assert E["cidd_matrix"].shape == (147, 1024)
+    # peak depth x samples-per-mm is the same sample index for every energy row
+    prod = E["peak_depths"].astype(np.float64) * E["scale_facts"]
+    assert np.all(np.abs(prod - 819.0) < 0.005)
+    # HU+1000 = 1000 is water, 2000 is the dense-bone end of the tables
+    assert E["density_vector"][1000] == pytest.approx(1.0)
+    assert E["sp_vector"][1000] == pytest.approx(1.0)
+    assert E["density_vector"][2000] == pytest.approx(1.61022, abs=1e-5)
+    assert E["sp_vector"][2000] == pytest.approx(1.38507, abs=1e-5)
+    # 1/X0 vs 1000*rho: radiation length of unit-density material
+    assert E["rrl_vector"][1000] == pytest.approx(0.00259113, rel=1e-5)
+    assert E["rrl_scale_fact"] == 1000.0
+
+
+@pytest.mark.parametrize("energy,peak", [
+    (100.0, 74.209), (110.0, 88.480), (120.0, 103.755), (130.0, 119.958), (140.0, 137.073),
+    (70.0, 37.687), (160.0, 173.919), (180.0, 214.054)])
+def test_peak_depth_known_answers(energy, peak):
+    """Peak depth at an energy, via findDecimalOrdered + vectorInterpolate; the 120/160/180 MeV
+    values match the reference's own Bragg-peak report (103.75/173.92/214.05 mm)."""
+    L = O.lib()
+    en = np.ascontiguousarray(E["energies_per_u"], np.float32)
+    pk = np.ascontiguousarray(E["peak_depths"], np.float32)
+    idx = L.orc_find_decimal_ordered(O._fp(en), en.size, energy)
+    got = L.orc_vector_interpolate(O._fp(pk), pk.size, idx)
+    assert got == pytest.approx(peak, abs=1.5e-3)
+
+
+def test_find_first_larger_bruteforce():
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        v = np.sort(rng.integers(0, 20, size=rng.integers(2, 30))).astype(np.float64)
+        t = float(rng.integers(-2, 22))
+        got = L.orc_find_first_larger_ordered(O._dp(v), v.size, t)
+        above = np.nonzero(v > t)[0]
+        want = v.size - 1 if v[-1] <= t else (0 if v[0] > t else int(above[0]))
+        assert got == want
+
+
+# ---------------------------------------------------------------- planning ----------------
+def test_config_grids_hand_computed():
+    p1 = O.plan_to_dict(O.plan_beam(F.make_config1()))
+    # spots -25..25 mm, margin 3*4 + 0.5 = 12.5 -> steps -12..62 (75) -> 96 x 80 rays from -37 mm
+    assert p1["rayDims"] == (96, 80) and p1["cpbCorner"][:2] == (-37.0, -37.0)
+    # CT gantry depth -63..63 mm at 2 mm -> 64 steps starting at +63
+    assert p1["steps"] == 64 and p1["fanCorner"][2] == 63.0 and p1["fanDelta"][2] == -2.0
+    assert p1["spotDims"] == (11, 11) and p1["spotDelta"] == (5.0, 5.0)
+    c2 = F.make_config1()
+    c2["beams"] = [F.head_beam(2, 90.0)]
+    c2["ct_dims"] = (256, 256, 160)
+    c2["ct_resolution"] = (1.0, 1.0, 2.0)
+    c2["ct_corner"] = (-127.5, -127.5, -159.0)
+    c2["dose_dims"], c2["dose_resolution"], c2["dose_corner"] = c2["ct_dims"], c2["ct_resolution"], c2["ct_corner"]
+    p2 = O.plan_to_dict(O.plan_beam(c2))
+    # hex lattice: x step 2.5 (21 columns), y step 4.330127 (13 rows); margin 18.5 -> 87 x 89 -> 96 x 96
+    assert p2["spotDims"] == (21, 13)
+    assert p2["rayDims"] == (96, 96)
+    # beam along +x: CT x extent -127.5..127.5 at 2 mm steps (ctRes.z) -> ceil(255/2)+1 = 129
+    assert p2["steps"] == 129
+
+
+def test_align_grid_examples():
+    L = O.lib()
+    o = np.zeros(1, np.float32)
+    n, f, l = (np.zeros(1, np.int32) for _ in range(3))
+    assert L.orc_align_grid(-37.5, 37.5, -25.0, 1.0, 32, O._fp(o), O._ip(n), O._ip(f), O._ip(l)) == 1
+    assert (o[0], n[0], f[0], l[0]) == (-37.0, 96, -12, 62)
+    # exact multiples land on the lattice (the -1e-4 tolerance keeps the boundary step)
+    assert L.orc_align_grid(-10.0, 10.0, 0.0, 2.0, 1, O._fp(o), O._ip(n), O._ip(f), O._ip(l)) == 1
+    assert (o[0], n[0], f[0], l[0]) == (-10.0, 11, -5, 5)
+
+
+# ---------------------------------------------------------------- trace -------------------
+def _water_case(n=(32, 32, 40), res=(2.0, 2.0, 2.0), hu=1000.0, slab=None, sigma=3.0, energies=(100.0,)):
+    nx, ny, nz = n
+    corner = tuple(-(d - 1) * r / 2.0 for d, r in zip(n, res))
+    ct = np.full((nz, ny, nx), hu, np.float32)
+    if slab is not None:
+        z0, z1, v = slab
+        ct[z0:z1] = v
+    xy = np.array([[0.0, 0.0]])
+    L = len(energies)
+    beam = F.make_beam(np.array(energies, np.float32), [xy] * L, [np.ones(1)] * L,
+                       np.full((L, 2), sigma, np.float32), beam_dir=(0, 0, 1), bm_x=(1, 0, 0), bm_y=(0, 1, 0),
+                       sad=2000.0)
+    return {"ct": ct, "ct_dims": n, "ct_resolution": res, "ct_corner": corner,
+            "dose_dims": n, "dose_resolution": res, "dose_corner": corner, "beams": [beam], "energy": E}
+
+
+def _trace(case):
+    keep = O._Keep()
+    en = O.make_energy(case["energy"], keep)
+    bm = O.make_beam(case["beams"][0], keep)
+    p = O.plan_beam(case)
+    rx, ry, K = p.rayDims[0], p.rayDims[1], p.steps
+    dens = np.zeros((K, ry, rx))
+    wepl = np.zeros((K, ry, rx))
+    fi = np.zeros((ry, rx), np.int32)
+    fo = np.zeros((ry, rx), np.int32)
+    ct = np.ascontiguousarray(case["ct"], np.float32)
+    dims = np.array(case["ct_dims"], np.int32)
+    O.lib().orc_trace(C.byref(p), O._fp(ct), O._ip(dims), C.byref(en), O._dp(dens), O._dp(wepl),
+                      O._ip(fi), O._ip(fo))
+    return p, dens, wepl, fi, fo, keep, en, bm
+
+
+def test_trace_water_closed_form():
+    case = _water_case()
+    p, dens, wepl, fi, fo, *_ = _trace(case)
+    # the central ray: fan x of ray i is corner + i*1mm -> find the ray on the axis
+    i0 = int(round(-p.cpbCorner[0] / p.cpbRes[0]))
+    j0 = int(round(-p.cpbCorner[1] / p.cpbRes[1]))
+    K = p.steps
+    # water everywhere inside: WEPL(k) = (k+1) * stepLen * SP(water)=1, density 1
+    np.testing.assert_allclose(wepl[:, j0, i0], 2.0 * np.arange(1, K + 1), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(dens[:, j0, i0], 1.0, atol=1e-12)
+    # first sample already inside: the cumulative-HU flag leaves beforeFirstInside at 0 -> 1
+    assert fi[j0, i0] == 1
+    assert fo[j0, i0] == K  # last inside sample is the last step
+    # an off-axis ray has step length |dz| sqrt(1 + (x/d)^2 + (y/d)^2)
+    i1 = i0 + 10
+    x = p.fanCorner[0] + i1 * p.fanDelta[0]
+    sl = 2.0 * math.sqrt(1.0 + (x / 2000.0) ** 2 + ((p.fanCorner[1] + j0) / 2000.0) ** 2)
+    assert wepl[0, j0, i1] == pytest.approx(sl, rel=1e-12)
+
+
+def test_trace_slab_wepl_and_entry():
+    # 10 air voxels, then water with a 10-voxel (20 mm) dense slab
+    case = _water_case(n=(16, 16, 40), slab=(15, 25, 2000.0))
+    case["ct"][:10] = 0.0
+    p, dens, wepl, fi, fo, *_ = _trace(case)
+    i0 = int(round(-p.cpbCorner[0]))
+    j0 = int(round(-p.cpbCorner[1]))
+    w = wepl[:, j0, i0]
+    sp_air = E["sp_vector"][0]
+    # samples 0..9 are air, sample 10 is water, 15..24 are slab: increment over the slab
+    assert w[24] - w[14] == pytest.approx(20.0 * float(E["sp_vector"][2000]), rel=1e-6)
+    assert w[9] == pytest.approx(20.0 * float(sp_air), rel=1e-6)
+    # cumulative HU stays below 150 through the air -> beforeFirstInside = 9 -> firstInside 10
+    assert fi[j0, i0] == 10
+
+
+def test_sample3d_voxel_centres_and_border():
+    L = O.lib()
+    rng = np.random.default_rng(1)
+    vol = rng.random((5, 6, 7)).astype(np.float32)
+    dims = np.array([7, 6, 5], np.int32)
+    for _ in range(20):
+        x, y, z = rng.integers(0, 7), rng.integers(0, 6), rng.integers(0, 5)
+        assert L.orc_sample3d_border(O._fp(vol), O._ip(dims), x, y, z) == pytest.approx(vol[z, y, x])
+    # half a voxel outside: interpolates with the zero border
+    assert L.orc_sample3d_border(O._fp(vol), O._ip(dims), -0.5, 2, 2) == pytest.approx(0.5 * vol[2, 2, 0])
+    # midpoint of two voxels along x
+    assert L.orc_sample3d_border(O._fp(vol), O._ip(dims), 2.5, 1, 3) == pytest.approx(0.5 * (vol[3, 1, 2] + vol[3, 1, 3]))
+
+
+# ---------------------------------------------------------------- convolution -------------
+def test_conv_single_spot_telescopes():
+    """One unit spot: the x and y tap sums telescope to 1/2[erf(hi)-erf(lo)] each."""
+    case = _water_case(sigma=3.0)
+    p, dens, wepl, fi, fo, keep, en, bm = _trace(case)
+    wmin = np.zeros(p.steps)
+    O.lib().orc_beam_cutoffs(C.byref(p), C.byref(bm), C.byref(en), O._dp(wepl), O._ip(fi), O._ip(fo), O._dp(wmin))
+    rx, ry = p.rayDims[0], p.rayDims[1]
+    rw = np.zeros((1, ry, rx))
+    O.lib().orc_conv_lattice(C.byref(p), C.byref(bm), C.byref(en), O._dp(rw))
+    sig = p.entrySigma[0][0]
+    entryZ = np.float32(p.beamFirstInside) * np.float32(p.fanDelta[2]) + np.float32(p.fanCorner[2])
+    px = np.float32(1.0) - entryZ / np.float32(p.sourceDist[0])
+    s_eff = sig / (p.cpbRes[0] * float(px))
+    r = 1.0 / (math.sqrt(2.0) * s_eff)
+    # spot sits at ray coordinate off = (0 - corner)/1; taps cover |dist| < 3 s_eff + 1/2
+    off = (0.0 - p.cpbCorner[0])
+    reach = 3.0 * s_eff + 0.5
+    outs = [o for o in range(rx) if abs(off - o) < reach]
+    d = [off - o for o in outs]
+    mass_x = 0.5 * (erf((max(d) + 0.5) * r) - erf((min(d) - 0.5) * r))
+    offy = (0.0 - p.cpbCorner[1])
+    outs_y = [o for o in range(ry) if abs(offy - o) < reach]
+    dy = [offy - o for o in outs_y]
+    mass_y = 0.5 * (erf((max(dy) + 0.5) * r) - erf((min(dy) - 0.5) * r))
+    # entrySigma is reported rounded to float, hence 1e-7
+    assert rw.sum() == pytest.approx(mass_x * mass_y, rel=1e-7)
+    # and the total is close to the 3-sigma mass of a unit Gaussian
+    assert 0.99 < rw.sum() < 1.0
+    # entry sigma = sqrt(a z^2 + b z + sigma^2) with (a, b) = sigmaSqAirCoefs(peak)
+    peak = p.convPeakDepth[0]
+    a, b = 0.0027 / (peak - 4.5), -4.39 / (peak - 3.86)
+    assert sig == pytest.approx(math.sqrt(a * entryZ ** 2 + b * entryZ + 9.0), rel=1e-5)
+
+
+def test_subspot_to_cpb_brute_force():
+    """2D erf cell integral of a subspot = product of the two 1D Gaussian cell masses."""
+    ss = np.zeros((1, 2, 5), np.float32)
+    ss[0, 0] = (1.3, -0.7, 2.0, 1.5, 2.5)
+    ss[0, 1] = (-2.0, 1.0, 0.5, 1.0, 1.0)
+    cc = np.array([-8.0, -8.0, 0.0], np.float32)
+    cr = np.array([0.5, 0.5, 1.0], np.float32)
+    cd = np.array([32, 32, 1], np.int32)
+    cpb = np.zeros((1, 32, 32))
+    bd = np.array([0, 0, 1], np.float32)
+    bx = np.array([1, 0, 0], np.float32)
+    by = np.array([0, 1, 0], np.float32)
+    src = np.array([0, 0, -1000], np.float32)
+    O.lib().orc_subspot_to_cpb(O._fp(ss), 1, 2, O._fp(cc), O._fp(cr), O._ip(cd), O._fp(bd), O._fp(bx), O._fp(by),
+                               O._fp(src), 1000.0, O._dp(cpb))
+    want = np.zeros((32, 32))
+    for dx, dy, w, sx, sy in ss[0].astype(np.float64):
+        xs = -8.0 + (np.arange(32) + 0.5) * 0.5
+        gx = 0.5 * (erf((xs + 0.25 - dx) / (math.sqrt(2) * sx)) - erf((xs - 0.25 - dx) / (math.sqrt(2) * sx)))
+        gy = 0.5 * (erf((xs + 0.25 - dy) / (math.sqrt(2) * sy)) - erf((xs - 0.25 - dy) / (math.sqrt(2) * sy)))
+        # 3-sigma window in cells: floor/ceil of the (pos -/+ 3 sigma) edges
+        x0 = max(0, math.floor((dx - 3 * sx + 8) / 0.5)); x1 = min(31, math.ceil((dx + 3 * sx + 8) / 0.5))
+        y0 = max(0, math.floor((dy - 3 * sy + 8) / 0.5)); y1 = min(31, math.ceil((dy + 3 * sy + 8) / 0.5))
+        m = np.zeros((32, 32))
+        m[y0:y1 + 1, x0:x1 + 1] = 1
+        want += w * np.outer(gy, gx) * m
+    np.testing.assert_allclose(cpb[0], want, rtol=1e-12, atol=1e-15)
+
+
+# ---------------------------------------------------------------- IDD / sigma -------------
+def _run_fill(case, layer=0):
+    p, dens, wepl, fi, fo, keep, en, bm = _trace(case)
+    K = p.steps
+    wmin = np.zeros(K)
+    L = O.lib()
+    L.orc_beam_cutoffs(C.byref(p), C.byref(bm), C.byref(en), O._dp(wepl), O._ip(fi), O._ip(fo), O._dp(wmin))
+    rx, ry = p.rayDims[0], p.rayDims[1]
+    nl = len(case["beams"][0]["energies"])
+    rw = np.zeros((nl, ry, rx))
+    L.orc_conv_lattice(C.byref(p), C.byref(bm), C.byref(en), O._dp(rw))
+    L.orc_layer_params(C.byref(p), C.byref(bm), C.byref(en), O._dp(wmin))
+    idd = np.zeros((K, ry, rx))
+    rs = np.zeros((K, ry, rx))
+    fp = np.zeros((ry, rx), np.int32)
+    L.orc_fill(C.byref(p), layer, C.byref(en), O._dp(dens), O._dp(wepl), O._dp(rw[layer]), O._ip(fi), O._ip(fo),
+               O._dp(idd), O._dp(rs), O._ip(fp))
+    return p, dens, wepl, rw, idd, rs, fp, fi
+
+
+def _cidd(wepl_mm, scale, eidx):
+    """Bilinear CIDD lookup written independently with numpy interpolation."""
+    m = E["cidd_matrix"].astype(np.float64)
+    e0 = int(math.floor(eidx)); a = eidx - e0
+    e1 = min(e0 + 1, m.shape[0] - 1)
+    xs = np.arange(m.shape[1])
+    r0 = np.interp(wepl_mm * scale, xs, m[e0])
+    r1 = np.interp(wepl_mm * scale, xs, m[e1])
+    return (1 - a) * r0 + a * r1
+
+
+def test_fill_energy_telescopes():
+    """sum_k res_k * mass_k = w * CIDD(WEPL at the last deposited step): the first live step
+    takes the whole cumulative dose from zero depth (cumulDoseOld starts at 0)."""
+    case = _water_case(energies=(110.0,))
+    p, dens, wepl, rw, idd, rs, fp, fi = _run_fill(case)
+    i0 = int(round(-p.cpbCorner[0])); j0 = int(round(-p.cpbCorner[1]))
+    first = p.beamFirstInside
+    stop = fp[j0, i0]
+    ks = np.arange(first, stop)
+    vol = np.float64(p.volConst) + ks * np.float64(p.volLin) + ks.astype(np.float64) ** 2 * np.float64(p.volSq)
+    dep = (idd[ks, j0, i0] * dens[ks, j0, i0] * vol).sum()
+    w = rw[0, j0, i0]
+    want = w * _cidd(wepl[stop - 1, j0, i0], p.scaleFact[0], p.energyIdx[0])
+    assert dep == pytest.approx(want, rel=1e-9)
+    # the stop step is the first one past 1.05 x peak (or the layer cut-off)
+    assert wepl[stop, j0, i0] > 1.05 * p.peakDepth[0] or stop == p.afterLast[0]
+    assert wepl[stop - 1, j0, i0] <= 1.05 * np.float32(p.peakDepth[0])
+
+
+def test_fill_sigma_air_closed_form():
+    """With scattering switched off (1/X0 = 0), sigma^2 at step k is the air-divergence
+    polynomial a z^2 + b z between the entry step and step k (fan z)."""
+    e2 = dict(E)
+    e2["rrl_vector"] = np.zeros_like(E["rrl_vector"])
+    case = _water_case(energies=(120.0,))
+    case["energy"] = e2
+    p, dens, wepl, rw, idd, rs, fp, fi = _run_fill(case)
+    i0 = int(round(-p.cpbCorner[0])); j0 = int(round(-p.cpbCorner[1]))
+    peak = np.float32(p.peakDepth[0])
+    a = np.float32(0.0027) / (peak - np.float32(4.5))
+    b = np.float32(-4.39) / (peak - np.float32(3.86))
+    first = p.beamFirstInside
+    for k in range(first + 1, fp[j0, i0]):
+        if wepl[k, j0, i0] >= p.peakDepth[0]:
+            break
+        z = np.float64(p.fanCorner[2]) + k * np.float64(p.fanDelta[2])
+        zf = np.float64(p.fanCorner[2]) + first * np.float64(p.fanDelta[2])
+        s2 = float(a) * (z * z - zf * zf) + float(b) * (z - zf)
+        vw = np.float64(p.fanDelta[0]) * (1 - z / p.sourceDist[0])
+        want = vw / (math.sqrt(2.0) * (math.sqrt(s2) + 0.21))
+        assert rs[k, j0, i0] == pytest.approx(want, rel=2e-6), k
+
+
+# ---------------------------------------------------------------- bins / superposition ----
+def test_tile_radius_formula_and_nan():
+    L = O.lib()
+    assert L.orc_tile_radius_f32(float("inf")) == 0
+    assert L.orc_tile_radius_f32(float("nan")) == 0
+    for r in [0.05, 0.1, 0.2, 0.33, 1.0, 2.0]:
+        assert L.orc_tile_radius_f32(r) == min(int(np.float32(3.0) / (np.float32(math.sqrt(2.0)) * np.float32(r)) + np.float32(0.5)), 33)
+    assert L.orc_tile_radius_f32(0.01) == 33
+
+
+def test_batch_bins_hand_examples():
+    L = O.lib()
+
+    def run(hist):
+        h = np.zeros(34, np.int32); h[:len(hist)] = hist
+        bt = np.zeros(33, np.int32); eff = np.zeros(34, np.int32)
+        L.orc_batch_bins(O._ip(h), O._ip(bt), O._ip(eff))
+        return bt, eff
+
+    # bins 5:10, 4:10 -> batch at 5 closes with 20 tiles; bin 3:2, 2:1, 1:1 -> batch at 3 (4 tiles)
+    bt, eff = run([7, 1, 1, 2, 10, 10])
+    assert bt[0] == 7 and bt[5] == 20 and bt[3] == 4 and bt.sum() == 31
+    assert list(eff[1:6]) == [3, 3, 3, 5, 5]
+    # overflow bin 33 is clamped into 32
+    h = [0] * 34; h[33] = 3; h[32] = 20; h[10] = 1
+    bt, eff = run(h)
+    assert bt[32] == 23 and bt[31] == 1 and eff[10] == 31
+
+
+def test_tile_radii_bruteforce_min():
+    rng = np.random.default_rng(3)
+    rs = (0.05 + rng.random((3, 16, 64))).astype(np.float32)
+    rs[1, 0:8, 0:32] = np.inf
+    rs[2, 3, 5] = np.nan
+    rad = np.zeros((3, 2, 2), np.int32)
+    O.lib().orc_tile_radii_f32(O._fp(rs), 64, 16, 0, 3, O._ip(rad))
+    for k in range(3):
+        for ty in range(2):
+            for tx in range(2):
+                blk = rs[k, ty * 8:(ty + 1) * 8, tx * 32:(tx + 1) * 32]
+                m = np.nanmin(blk)
+                want = min(int(np.float32(3.0) / (np.float32(math.sqrt(2.0)) * m) + np.float32(0.5)), 33) if np.isfinite(m) else 0
+                assert rad[k, ty, tx] == want
+
+
+def test_superposition_single_pixel_mass():
+    """One live pixel: BEV total = d * erf(r (R + 1/2))^2 and the footprint is (2R+1)^2."""
+    case = _water_case()
+    p = O.plan_beam(case)
+    sx, sy = p.superpDims[0], p.superpDims[1]
+    K = p.steps
+    p.beamFirstInside = 2
+    p.layerPassive[0] = 3
+    p.guaranteedPassive = 4
+    idd = np.zeros((K, sy, sx))
+    rs = np.full((K, sy, sx), np.inf)
+    idd[2, 3, 20] = 2.5
+    rs[2, 3, 20] = 0.3
+    bev = np.zeros((4, sy + 64, sx + 64))
+    O.lib().orc_superpose(C.byref(p), 0, O._dp(idd), O._dp(rs), O._dp(bev))
+    R = min(int(np.float32(3.0) / (np.float32(math.sqrt(2.0)) * np.float32(0.3)) + np.float32(0.5)), 33)
+    assert bev[2].sum() == pytest.approx(2.5 * erf(0.3 * (R + 0.5)) ** 2, rel=1e-12)
+    nz = np.nonzero(bev[2])
+    assert nz[0].min() == 3 + 32 - R and nz[0].max() == 3 + 32 + R
+    assert nz[1].min() == 20 + 32 - R and nz[1].max() == 20 + 32 + R
+    # separable profile: the centre row is d * e0 * e[|i|]
+    e0 = erf(0.15)
+    e1 = 0.5 * (erf(0.45) - erf(0.15))
+    assert bev[2, 35, 52] == pytest.approx(2.5 * e0 * e0)
+    assert bev[2, 35, 53] == pytest.approx(2.5 * e0 * e1)
+
+
+# ---------------------------------------------------------------- end to end ---------------
+def test_config1_oracle_runs_and_is_finite():
+    case = F.make_config1()
+    dose, p = O.beam_dose(case)
+    d = O.plan_to_dict(p)
+    assert np.isfinite(dose).all() and dose.max() > 0
+    assert d["beamFirstInside"] == 1
+    # distal cut-offs grow with energy; the 140 MeV layer reaches the far CT edge
+    assert list(d["afterLast"]) == sorted(d["afterLast"])
+    assert d["afterLast"][-1] <= d["guaranteedPassive"]
