@@ -1,0 +1,160 @@
+/*
+ * C ABI of the MI355X pencil-beam final-dose engine (libpbs_hip.so).
+ *
+ * Plain pointers and sizes only; no HIP, torch or STL types cross this boundary. Every
+ * entry point returns 0 on success or a negative PBS_E_* code, and leaves a message for
+ * pbs_last_error() (thread-local). Nothing here calls exit().
+ *
+ * Which reference interface each entry point replaces is given next to it. All lengths
+ * are millimetres unless the reference's own unit heuristics say otherwise (a CT
+ * resolution below 0.3 marks centimetre input, wrapper:2728-2729).
+ */
+#ifndef PBS_ABI_H
+#define PBS_ABI_H
+
+#include <stddef.h>
+#include "pbs_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBS_OK 0
+#define PBS_E_INVALID -1   /* argument validation failed (RTD_ENTRY_ASSERT class) */
+#define PBS_E_GEOMETRY -2  /* grid / lattice / halo contract failure (reference throws) */
+#define PBS_E_DEVICE -3    /* HIP runtime error */
+#define PBS_E_NOMEM -4     /* device allocation failed or admission refused */
+#define PBS_E_UNSUPPORTED -5
+
+/*
+ * Beam description: a POD mirror of RTDBeamSettings (include/core/raytracedicom_integration.h:31-95)
+ * with each std::vector replaced by a pointer + count. Optional arrays may be NULL.
+ */
+typedef struct pbs_beam_settings {
+    int numLayers;
+    const float *energies;        /* [numLayers] MeV/u */
+    const float *spotSigmas;      /* [numLayers][2] sigma_x, sigma_y at iso in air */
+    float raySpacing[2];
+    unsigned int steps;           /* kept for parity; the tracer derives its own step count */
+    float sourceDist[2];
+    float spotOffset[3];
+    float spotDelta[3];
+    int maxSubspotsPerLayer;
+    const float *subspotData;     /* [numLayers][maxSubspotsPerLayer][5]: dx, dy, w, sx, sy */
+    const int *layerSpotCounts;   /* [numLayers] or NULL (then the subspot->CPB path is used) */
+    int numSpots;
+    const float *spotPositions;   /* [numSpots][2] gantry-frame x, y on the iso plane */
+    const float *spotWeights;     /* [numSpots] */
+    const float *spotBeamDirections; /* [numSpots][3] or NULL */
+    const float *layerLongitudinalCutoffs; /* [numLayers] or NULL (audit only, as in the reference) */
+    float beamDirection[3];
+    float bmXDirection[3];
+    float bmYDirection[3];
+    float sourcePosition[3];
+    float sad;
+    float refPlaneZ;
+    const int *roiLinearIndices;  /* x-major x*ny*nz + y*nz + z, or NULL */
+    int numRoiLinearIndices;
+    int useSubspotConvolution;    /* 1: per-subspot lattice convolution (reference branch
+                                     wrapper:3770-3822, compiled in but switched off there) */
+} pbs_beam_settings;
+
+/* Energy/LUT payload: POD mirror of RTDEnergyStruct (raytracedicom_integration.h:97-128). */
+typedef struct pbs_energy_struct {
+    int nEnergySamples;
+    int nEnergies;
+    const float *energiesPerU;    /* [nEnergies] */
+    const float *peakDepths;      /* [nEnergies] */
+    const float *scaleFacts;      /* [nEnergies] */
+    const float *ciddMatrix;      /* [nEnergies][nEnergySamples] */
+    int nDensitySamples; float densityScaleFact; const float *densityVector;
+    int nSpSamples; float spScaleFact; const float *spVector;
+    int nRRlSamples; float rRlScaleFact; const float *rRlVector;
+    const float *nucWeightMatrix;   /* [nEnergies][nEnergySamples] or NULL */
+    const float *nucSqSigmaMatrix;  /* [nEnergies][nEnergySamples] or NULL */
+} pbs_energy_struct;
+
+/*
+ * Whole-plan dose on the host, accumulated into doseData (the reference uploads doseData,
+ * adds every beam's dose and copies it back: wrapper:2653-2662, 5642).
+ * Replaces extern "C" subsecondWrapper (include/core/raytracedicom_integration.h:132-138,
+ * src/core/raytracedicom_wrapper.cu:2446).
+ */
+int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float ctResolution[3],
+                          const float ctCorner[3], float *doseData, const int doseDims[3],
+                          const float doseResolution[3], const float doseCorner[3],
+                          const pbs_beam_settings *beams, size_t numBeams,
+                          const pbs_energy_struct *energy, int gpuId, int nuclearCorrection, int verbose);
+
+/* Message of the last failing call on this thread (empty string if none). */
+const char *pbs_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int pbs_device_count(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Resident engine: CT, LUTs and work buffers stay in HBM between calls (batching, serving,
+ * differentiation). No reference counterpart; these are the building blocks of the
+ * reference's per-call flow split into reusable parts.
+ * ------------------------------------------------------------------------------------- */
+typedef struct pbs_engine pbs_engine;
+
+pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *energy);
+void pbs_engine_destroy(pbs_engine *e);
+/* Upload (ctIsDevice = 0) or adopt (ctIsDevice = 1) a CT; values are converted to HU+1000
+ * with the reference's auto-detection (wrapper:2290-2441). */
+int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int ctDims[3], const float ctRes[3],
+                      const float ctCorner[3], int ctIsDevice);
+/* Dose grid geometry used by subsequent beam calls. */
+int pbs_engine_set_dose_grid(pbs_engine *e, const int doseDims[3], const float doseRes[3],
+                             const float doseCorner[3]);
+/* Accumulate one beam's dose into doseDevice ([nz][ny][nx] float32 device buffer) on the
+ * engine's stream; returns after the work is queued (synchronize with pbs_engine_sync). */
+int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDevice);
+int pbs_engine_sync(pbs_engine *e);
+/* Raw stream handle (hipStream_t) the engine queues work on. */
+void *pbs_engine_stream(pbs_engine *e);
+
+/* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
+ * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside
+ * [ry][rx]; weplMin [K]; rayWeights [L][ry][rx]; iddSigma [L][K][sy][sx][2] (IDD, rSigmaEff;
+ * only steps [beamFirstInside, afterLast_l) are defined); firstPassive [L][ry][rx];
+ * tileRad [L][K][sy/8][sx/32] (-1 where not computed); radHist [L][34]; bev
+ * [bevZ][sy+64][sx+64]; ints: see pbs_debug_ints. */
+typedef struct pbs_debug_ints {
+    int rayDims[2], superpDims[2], steps, numLayers, legacyPath;
+    int beamFirstInside, beamFirstOutside, firstPastCutoffAll, guaranteedPassive;
+    int afterLast[256], layerPassive[256];
+    int transferStart[3], transferMax[3];
+    float cpbCorner[3], fanCorner[3], fanDelta[3], sourceDist[2];
+    float energyIdx[256], scaleFact[256], peakDepth[256];
+} pbs_debug_ints;
+int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, float *wepl,
+                           int *firstInside, int *firstOutside, float *weplMin, float *rayWeights,
+                           float *iddSigma, int *firstPassive, int *tileRad, int *radHist, float *bev);
+
+/* ---------------------------------------------------------------------------------------
+ * cudaFinalDose with the reference's name and argument order
+ * (include/algorithms/cuda_final_dose.h:42-56, src/algorithms/cuda_final_dose.cu:230-390).
+ * subspotData is a float buffer [numLayers][maxSubspotsPerLayer][5] (host or device,
+ * detected) instead of a texture object; roiIndices and finalDose are device pointers as in
+ * the reference. The reference stops after the CPB convolution; this one completes the
+ * chain using the CT, LUTs and per-layer energies bound with pbs_final_dose_bind().
+ * Returns 0 or a PBS_E_* code (PBS_OK with an untouched dose when the beam misses the ROI).
+ * ------------------------------------------------------------------------------------- */
+int pbs_final_dose_bind(pbs_engine *e, const float *layerEnergies, const float *layerSigmas,
+                        int numLayers, float raySpacing);
+int cudaFinalDose(const float *subspotData, const vec3i *roiIndices, int numROIIndices,
+                  vec3f doseGridResolution, vec3f doseGridCorner, vec3f beamDirection,
+                  vec3f sourcePosition, float sad, float refPlaneZ, int numLayers,
+                  int maxSubspotsPerLayer, float *finalDose, vec3i doseGridDims);
+/* ROI range / intersection / CPB grid of cudaFinalDose on the host (bit-exact integers;
+ * cuda_final_dose.cu:66-129, 132-152, 302-308). Returns 1 if the beam meets the ROI. */
+int pbs_final_dose_roi(const vec3i *roiIndicesHost, int numROIIndices, vec3f doseGridResolution,
+                       vec3f doseGridCorner, vec3f beamDirection, vec3f sourcePosition, int numLayers,
+                       vec3f *roiMinCorner, vec3f *roiMaxCorner, vec3i *roiDims, CPBGrid *cpbGrid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBS_ABI_H */

@@ -1,0 +1,249 @@
+"""ctypes bindings of the engine's C ABI (include/pbs_abi.h).
+
+The library is the in-tree ``subsecondpbs_amd/_lib/libpbs_hip.so`` built by
+``subsecondpbs_amd.build``. Importing this module does not touch the GPU; loading fails
+loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpbs_hip.so")
+
+_f = C.POINTER(C.c_float)
+_i = C.POINTER(C.c_int)
+
+PBS_OK = 0
+ERRORS = {-1: "invalid argument", -2: "geometry", -3: "device", -4: "out of memory", -5: "unsupported"}
+
+
+class BeamSettings(C.Structure):
+    _fields_ = [
+        ("numLayers", C.c_int), ("energies", _f), ("spotSigmas", _f),
+        ("raySpacing", C.c_float * 2), ("steps", C.c_uint), ("sourceDist", C.c_float * 2),
+        ("spotOffset", C.c_float * 3), ("spotDelta", C.c_float * 3),
+        ("maxSubspotsPerLayer", C.c_int), ("subspotData", _f),
+        ("layerSpotCounts", _i), ("numSpots", C.c_int), ("spotPositions", _f), ("spotWeights", _f),
+        ("spotBeamDirections", _f), ("layerLongitudinalCutoffs", _f),
+        ("beamDirection", C.c_float * 3), ("bmXDirection", C.c_float * 3),
+        ("bmYDirection", C.c_float * 3), ("sourcePosition", C.c_float * 3),
+        ("sad", C.c_float), ("refPlaneZ", C.c_float),
+        ("roiLinearIndices", _i), ("numRoiLinearIndices", C.c_int),
+        ("useSubspotConvolution", C.c_int),
+    ]
+
+
+class EnergyStruct(C.Structure):
+    _fields_ = [
+        ("nEnergySamples", C.c_int), ("nEnergies", C.c_int),
+        ("energiesPerU", _f), ("peakDepths", _f), ("scaleFacts", _f), ("ciddMatrix", _f),
+        ("nDensitySamples", C.c_int), ("densityScaleFact", C.c_float), ("densityVector", _f),
+        ("nSpSamples", C.c_int), ("spScaleFact", C.c_float), ("spVector", _f),
+        ("nRRlSamples", C.c_int), ("rRlScaleFact", C.c_float), ("rRlVector", _f),
+        ("nucWeightMatrix", _f), ("nucSqSigmaMatrix", _f),
+    ]
+
+
+class DebugInts(C.Structure):
+    _fields_ = [
+        ("rayDims", C.c_int * 2), ("superpDims", C.c_int * 2), ("steps", C.c_int), ("numLayers", C.c_int),
+        ("legacyPath", C.c_int),
+        ("beamFirstInside", C.c_int), ("beamFirstOutside", C.c_int), ("firstPastCutoffAll", C.c_int),
+        ("guaranteedPassive", C.c_int),
+        ("afterLast", C.c_int * 256), ("layerPassive", C.c_int * 256),
+        ("transferStart", C.c_int * 3), ("transferMax", C.c_int * 3),
+        ("cpbCorner", C.c_float * 3), ("fanCorner", C.c_float * 3), ("fanDelta", C.c_float * 3),
+        ("sourceDist", C.c_float * 2),
+        ("energyIdx", C.c_float * 256), ("scaleFact", C.c_float * 256), ("peakDepth", C.c_float * 256),
+    ]
+
+
+class Vec3f(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Vec3i(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
+
+
+class CPBGrid(C.Structure):
+    _fields_ = [("corner", Vec3f), ("resolution", Vec3f), ("dims", Vec3i), ("nLayers", C.c_int)]
+
+
+# every symbol include/pbs_abi.h declares
+EXPORTED = [
+    "pbs_subsecond_wrapper", "pbs_last_error", "pbs_device_count",
+    "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
+    "pbs_engine_beam_dose", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
+    "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libpbs_hip.so (raises when it is missing: there is no CPU fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.pbs_last_error.restype = C.c_char_p
+        L.pbs_device_count.restype = C.c_int
+        L.pbs_subsecond_wrapper.restype = C.c_int
+        L.pbs_subsecond_wrapper.argtypes = [_f, _i, _f, _f, _f, _i, _f, _f, C.POINTER(BeamSettings), C.c_size_t,
+                                            C.POINTER(EnergyStruct), C.c_int, C.c_int, C.c_int]
+        L.pbs_engine_create.restype = C.c_void_p
+        L.pbs_engine_create.argtypes = [C.c_int, C.POINTER(EnergyStruct)]
+        L.pbs_engine_destroy.argtypes = [C.c_void_p]
+        L.pbs_engine_set_ct.restype = C.c_int
+        L.pbs_engine_set_ct.argtypes = [C.c_void_p, C.c_void_p, _i, _f, _f, C.c_int]
+        L.pbs_engine_set_dose_grid.restype = C.c_int
+        L.pbs_engine_set_dose_grid.argtypes = [C.c_void_p, _i, _f, _f]
+        L.pbs_engine_beam_dose.restype = C.c_int
+        L.pbs_engine_beam_dose.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p]
+        L.pbs_engine_sync.restype = C.c_int
+        L.pbs_engine_sync.argtypes = [C.c_void_p]
+        L.pbs_engine_stream.restype = C.c_void_p
+        L.pbs_engine_stream.argtypes = [C.c_void_p]
+        L.pbs_engine_debug_fetch.restype = C.c_int
+        L.pbs_engine_debug_fetch.argtypes = [C.c_void_p, C.POINTER(DebugInts)] + [C.c_void_p] * 11
+        L.pbs_final_dose_bind.restype = C.c_int
+        L.pbs_final_dose_bind.argtypes = [C.c_void_p, _f, _f, C.c_int, C.c_float]
+        L.cudaFinalDose.restype = C.c_int
+        L.cudaFinalDose.argtypes = [C.c_void_p, C.c_void_p, C.c_int, Vec3f, Vec3f, Vec3f, Vec3f, C.c_float,
+                                    C.c_float, C.c_int, C.c_int, C.c_void_p, Vec3i]
+        L.pbs_final_dose_roi.restype = C.c_int
+        L.pbs_final_dose_roi.argtypes = [C.POINTER(Vec3i), C.c_int, Vec3f, Vec3f, Vec3f, Vec3f, C.c_int,
+                                         C.POINTER(Vec3f), C.POINTER(Vec3f), C.POINTER(Vec3i), C.POINTER(CPBGrid)]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().pbs_last_error().decode(errors="replace")
+
+
+def check(rc: int, what: str):
+    if rc != PBS_OK:
+        raise RuntimeError(f"{what} failed ({ERRORS.get(rc, rc)}): {last_error()}")
+
+
+class Keep:
+    """Owns the numpy buffers a ctypes struct points at."""
+
+    def __init__(self):
+        self.bufs = []
+
+    def f32(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self.bufs.append(a)
+        return a.ctypes.data_as(_f)
+
+    def i32(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        self.bufs.append(a)
+        return a.ctypes.data_as(_i)
+
+
+def energy_struct(energy: dict, keep: Keep) -> EnergyStruct:
+    e = EnergyStruct()
+    e.nEnergies = int(energy["n_energies"])
+    e.nEnergySamples = int(energy["n_energy_samples"])
+    e.energiesPerU = keep.f32(energy["energies_per_u"])
+    e.peakDepths = keep.f32(energy["peak_depths"])
+    e.scaleFacts = keep.f32(energy["scale_facts"])
+    cidd = np.asarray(energy["cidd_matrix"], dtype=np.float32)
+    if cidd.size != e.nEnergies * e.nEnergySamples:
+        raise ValueError("cidd_matrix shape mismatch with n_energies/n_energy_samples")
+    e.ciddMatrix = keep.f32(cidd)
+    e.nDensitySamples = int(energy["n_density_samples"])
+    e.densityScaleFact = float(energy["density_scale_fact"])
+    e.densityVector = keep.f32(energy["density_vector"])
+    e.nSpSamples = int(energy["n_sp_samples"])
+    e.spScaleFact = float(energy["sp_scale_fact"])
+    e.spVector = keep.f32(energy["sp_vector"])
+    e.nRRlSamples = int(energy["n_rrl_samples"])
+    e.rRlScaleFact = float(energy["rrl_scale_fact"])
+    e.rRlVector = keep.f32(energy["rrl_vector"])
+    if energy.get("nuclear_weight_matrix") is not None:
+        e.nucWeightMatrix = keep.f32(energy["nuclear_weight_matrix"])
+        e.nucSqSigmaMatrix = keep.f32(energy["nuclear_sq_sigma_matrix"])
+    return e
+
+
+def _vec(beam, key, n, default):
+    v = beam.get(key, default)
+    v = np.asarray(v, dtype=np.float32).ravel()
+    if v.size != n:
+        raise ValueError(f"beam['{key}'] must have {n} values")
+    return [float(x) for x in v]
+
+
+def beam_settings(beam: dict, keep: Keep, use_subspots: bool | None = None) -> BeamSettings:
+    """Beam dict (reference binding key names, raytracedicom_pybind.cpp:1712-1821) -> POD struct."""
+    b = BeamSettings()
+    energies = np.asarray(beam["energies"], dtype=np.float32).ravel()
+    L = energies.size
+    b.numLayers = L
+    b.energies = keep.f32(energies)
+    sig = np.asarray(beam["spot_sigmas"], dtype=np.float32)
+    if sig.size != 2 * L:
+        raise ValueError("beam.spot_sigmas length must equal num_layers")
+    b.spotSigmas = keep.f32(sig.reshape(L, 2))
+    b.raySpacing[:] = _vec(beam, "ray_spacing", 2, (0.0, 0.0))
+    b.steps = int(beam.get("steps", 0))
+    b.sourceDist[:] = _vec(beam, "source_dist", 2, (0.0, 0.0))
+    b.spotOffset[:] = _vec(beam, "spot_offset", 3, (0.0, 0.0, 0.0))
+    b.spotDelta[:] = _vec(beam, "spot_delta", 3, (0.0, 0.0, 0.0))
+    ss = np.asarray(beam["subspot_data"], dtype=np.float32)
+    if ss.ndim != 3 or ss.shape[2] != 5:
+        raise ValueError("subspot_data must have shape (num_layers, max_subspots_per_layer, 5)")
+    if ss.shape[0] != L:
+        raise ValueError("beam.energies length must equal subspot_data.shape[0] (num_layers)")
+    b.maxSubspotsPerLayer = int(ss.shape[1])
+    b.subspotData = keep.f32(ss)
+    if beam.get("layer_spot_counts") is not None:
+        b.layerSpotCounts = keep.i32(beam["layer_spot_counts"])
+        w = np.asarray(beam["spot_weights"], dtype=np.float32).ravel()
+        b.numSpots = int(w.size)
+        b.spotWeights = keep.f32(w)
+        b.spotPositions = keep.f32(beam["spot_positions"])
+    if beam.get("spot_beam_directions") is not None:
+        b.spotBeamDirections = keep.f32(beam["spot_beam_directions"])
+    if beam.get("layer_longitudinal_cutoffs") is not None:
+        lc = np.asarray(beam["layer_longitudinal_cutoffs"], dtype=np.float32).ravel()
+        if lc.size != L:
+            raise ValueError("beam.layer_longitudinal_cutoffs length must equal num_layers")
+        b.layerLongitudinalCutoffs = keep.f32(lc)
+    b.beamDirection[:] = _vec(beam, "beam_direction", 3, None)
+    b.bmXDirection[:] = _vec(beam, "beam_xdir", 3, None)
+    b.bmYDirection[:] = _vec(beam, "beam_ydir", 3, None)
+    b.sourcePosition[:] = _vec(beam, "source_position", 3, None)
+    b.sad = float(beam["sad"])
+    b.refPlaneZ = float(beam.get("ref_plane_z", 0.0))
+    if beam.get("roi_linear_indices") is not None:
+        roi = np.asarray(beam["roi_linear_indices"], dtype=np.int32).ravel()
+        b.roiLinearIndices = keep.i32(roi)
+        b.numRoiLinearIndices = int(roi.size)
+    if use_subspots is None:
+        use_subspots = bool(beam.get("use_subspot_convolution", False))
+    b.useSubspotConvolution = 1 if use_subspots else 0
+    return b
+
+
+def i3(v):
+    a = np.ascontiguousarray(v, dtype=np.int32).ravel()
+    return a, a.ctypes.data_as(_i)
+
+
+def f3(v):
+    a = np.ascontiguousarray(v, dtype=np.float32).ravel()
+    return a, a.ctypes.data_as(_f)

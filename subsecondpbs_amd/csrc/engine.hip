@@ -1,0 +1,667 @@
+// Engine orchestration and C ABI of the MI355X pencil-beam final-dose path.
+//
+// One beam is a fixed sequence of launches on the engine's stream with no host round trip:
+// trace -> trace reductions -> cut-offs -> lattice convolution (or subspot->CPB) -> IDD/sigma
+// fill -> layer finalisation -> tile radii / bins / items -> superposition (8 radius
+// classes) -> block merge -> BEV->dose transfer. Device scalars (entry step, cut-offs,
+// passive steps, transfer box) are produced and consumed on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pbs_abi.h"
+#include "device.h"
+#include "kernels.h"
+#include "plan.h"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int set_error(int code, const std::string &msg) {
+    g_lastError = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t _e = (expr);                                                                            \
+        if (_e != hipSuccess)                                                                              \
+            return set_error(PBS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));            \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t count) {
+        if (count <= n && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = std::max<size_t>(count, 1);
+        return e;
+    }
+};
+
+}  // namespace
+
+using namespace pbs;
+
+struct pbs_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // LUTs (device) and host copies used by planning
+    DevBuf<float> cidd, dens, sp, rrl, nucW, nucS2;
+    std::vector<float> hEnergies, hPeaks, hScales;
+    pbs_energy_struct energy{};
+    LutDev lut{};
+    // CT
+    DevBuf<float> ct;
+    const float *ctPtr = nullptr;
+    int ctDims[3] = {0, 0, 0};
+    float ctRes[3] = {0, 0, 0}, ctCorner[3] = {0, 0, 0};
+    // dose grid
+    int doseDims[3] = {0, 0, 0};
+    float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
+    // per-beam work buffers
+    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, slots, bev, cpb;
+    DevBuf<float2> iddSigma;
+    DevBuf<int> fi, fo, firstPassive, tileRad, hist, effRad, itemR, classCount, classList;
+    DevBuf<LayerHost> layerHost;
+    DevBuf<LayerScalars> layerScal;
+    DevBuf<BeamScalars> beamScal;
+    // last beam (debug fetch)
+    BeamPlan plan;
+    FanGeom geom{};
+    int lastMaxItems = 0;
+    bool haveLast = false;
+    // host staging of the last beam's uploads; the next beam waits for uploadDone before reuse
+    std::vector<LayerHost> hostLayers;
+    hipEvent_t uploadDone = nullptr;
+    bool uploadPending = false;
+    // final-dose binding
+    std::vector<float> fdEnergies, fdSigmas;
+    float fdRaySpacing = 0.5f;
+};
+
+namespace {
+
+void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
+    std::memset(&g, 0, sizeof(g));
+    const M3 *ms[3] = {&p.g2im.m, &p.dose2g.m, &p.g2dose.m};
+    float *dst[3] = {g.g2im, g.d2g, g.g2dose};
+    const F3 *vs[3] = {&p.g2im.v, &p.dose2g.v, &p.g2dose.v};
+    float *dv[3] = {g.g2imOff, g.d2gOff, g.g2doseOff};
+    for (int q = 0; q < 3; ++q) {
+        const M3 &m = *ms[q];
+        const float v9[9] = {m.r0.x, m.r0.y, m.r0.z, m.r1.x, m.r1.y, m.r1.z, m.r2.x, m.r2.y, m.r2.z};
+        std::memcpy(dst[q], v9, sizeof(v9));
+        dv[q][0] = vs[q]->x; dv[q][1] = vs[q]->y; dv[q][2] = vs[q]->z;
+    }
+    for (int i = 0; i < 3; ++i) { g.fc[i] = p.fanCorner[i]; g.fd[i] = p.fanDelta[i]; }
+    g.dist[0] = p.sourceDist[0]; g.dist[1] = p.sourceDist[1];
+    g.rx = p.rayDims[0]; g.ry = p.rayDims[1];
+    g.sx = p.superpDims[0]; g.sy = p.superpDims[1];
+    g.K = p.steps;
+    for (int i = 0; i < 3; ++i) { g.ctDims[i] = e.ctDims[i]; g.doseDims[i] = e.doseDims[i]; }
+    g.volConst = p.volConst; g.volLin = p.volLin; g.volSq = p.volSq;
+    g.maxCutoffWepl = p.maxCutoffWepl;
+    g.numLayers = p.numLayers;
+    g.legacy = p.legacyPath ? 1 : 0;
+}
+
+// Host worst-case transfer launch extents: the box for steps [0, K-1] contains every
+// layer's box (the fan frustum grows with its depth range).
+void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &gy) {
+    const float xv[2] = {-32.0f, float(p.superpDims[0] + 32 - 1)};
+    const float yv[2] = {-32.0f, float(p.superpDims[1] + 32 - 1)};
+    const float zv[2] = {0.0f, float(p.steps - 1)};
+    float mx[3] = {-1.f, -1.f, -1.f}, mn[3] = {1e30f, 1e30f, 1e30f};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                F3 f = f3(xv[c] * p.fanDelta[0] + p.fanCorner[0], yv[b] * p.fanDelta[1] + p.fanCorner[1],
+                          zv[a] * p.fanDelta[2] + p.fanCorner[2]);
+                f.x *= (1.0f - f.z / p.sourceDist[0]);
+                f.y *= (1.0f - f.z / p.sourceDist[1]);
+                const F3 q = apply(p.g2dose, f);
+                const float qv[3] = {q.x, q.y, q.z};
+                for (int r = 0; r < 3; ++r) { mx[r] = std::max(mx[r], qv[r]); mn[r] = std::min(mn[r], qv[r]); }
+            }
+    const int sx = std::max((int(std::floor(mn[0])) / 32) * 32 - 32, 0);
+    const int sy = std::max(int(std::floor(mn[1])) - 1, 0);
+    const int ex = std::min(int(std::ceil(mx[0])) + 1, doseDims[0] - 1);
+    const int ey = std::min(int(std::ceil(mx[1])) + 1, doseDims[1] - 1);
+    gx = std::max(1, (ex - sx + 1 + 31) / 32 + 1);
+    gy = std::max(1, (ey - sy + 1 + 7) / 8 + 1);
+}
+
+void fill_transfer_args(const BeamPlan &p, TransferArgs &t) {
+    // doseIdx -> gantry (inverse of gantryToDoseIdx), then gantry -> fan index with the
+    // (+32, +32) BEV halo shift; the -beamFirstInside z shift is applied on the device.
+    const M3 &m = p.dose2g.m;
+    const float dlt[3] = {1.0f / p.fanDelta[0], 1.0f / p.fanDelta[1], 1.0f / p.fanDelta[2]};
+    const float col0[3] = {m.r0.x, m.r1.x, m.r2.x}, col1[3] = {m.r0.y, m.r1.y, m.r2.y}, col2[3] = {m.r0.z, m.r1.z, m.r2.z};
+    const float off[3] = {p.dose2g.v.x, p.dose2g.v.y, p.dose2g.v.z};
+    const float shift[3] = {32.0f, 32.0f, 0.0f};
+    for (int i = 0; i < 3; ++i) {
+        t.coefI[i] = col0[i] * dlt[i];
+        t.coefJ[i] = col1[i] * dlt[i];
+        t.inc[i] = col2[i] * dlt[i];
+        t.coefOffset[i] = off[i] * dlt[i];
+        t.globalOffset[i] = -p.fanCorner[i] * dlt[i] + shift[i];
+    }
+    t.normDist[0] = dlt[2] * p.sourceDist[0];
+    t.normDist[1] = dlt[2] * p.sourceDist[1];
+}
+
+int validate_energy(const pbs_energy_struct *en) {
+    if (!en) return set_error(PBS_E_INVALID, "energyData must not be null");
+    if (!(en->nEnergies > 0 && en->nEnergySamples > 0)) return set_error(PBS_E_INVALID, "energyData dimensions must be positive");
+    if (!en->energiesPerU || !en->peakDepths || !en->scaleFacts || !en->ciddMatrix)
+        return set_error(PBS_E_INVALID, "energy axis / CIDD arrays must not be null");
+    if (!en->densityVector || !en->spVector || !en->rRlVector || en->nDensitySamples <= 0 || en->nSpSamples <= 0 ||
+        en->nRRlSamples <= 0)
+        return set_error(PBS_E_INVALID, "density, stopping-power, and radiation-length LUTs must not be empty");
+    if (en->nDensitySamples + en->nSpSamples > 16384 || en->nRRlSamples > 16384)
+        return set_error(PBS_E_UNSUPPORTED, "LUT longer than the 16384 samples kept in LDS");
+    return PBS_OK;
+}
+
+int validate_beam(const pbs_beam_settings &b, size_t idx) {
+    const std::string pre = "beamIdx=" + std::to_string(idx) + " ";
+    if (b.numLayers <= 0 || !b.energies) return set_error(PBS_E_INVALID, pre + "beam.energies must not be empty");
+    if (b.numLayers > kMaxLayers) return set_error(PBS_E_UNSUPPORTED, pre + "more than 256 layers");
+    if (b.maxSubspotsPerLayer <= 0 || !b.subspotData)
+        return set_error(PBS_E_INVALID, pre + "beam.maxSubspotsPerLayer must be positive");
+    if (!b.spotSigmas) return set_error(PBS_E_INVALID, pre + "beam.spotSigmas must not be null");
+    if (b.layerSpotCounts) {
+        long total = 0;
+        for (int l = 0; l < b.numLayers; ++l) total += b.layerSpotCounts[l];
+        if (b.spotWeights && total != b.numSpots) return set_error(PBS_E_INVALID, pre + "sum(layerSpotCounts) must match spotWeights size");
+    }
+    return PBS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pbs_last_error(void) { return g_lastError.c_str(); }
+
+int pbs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return n;
+}
+
+pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
+    if (validate_energy(en) != PBS_OK) return nullptr;
+    if (hipSetDevice(gpuId) != hipSuccess) { set_error(PBS_E_DEVICE, "hipSetDevice failed"); return nullptr; }
+    std::unique_ptr<pbs_engine> e(new pbs_engine());
+    e->device = gpuId;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess) {
+        set_error(PBS_E_DEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    const size_t nC = size_t(en->nEnergies) * en->nEnergySamples;
+    auto up = [&](DevBuf<float> &d, const float *h, size_t n) {
+        if (d.ensure(n) != hipSuccess) return false;
+        return hipMemcpy(d.p, h, n * sizeof(float), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(e->cidd, en->ciddMatrix, nC) || !up(e->dens, en->densityVector, en->nDensitySamples) ||
+        !up(e->sp, en->spVector, en->nSpSamples) || !up(e->rrl, en->rRlVector, en->nRRlSamples)) {
+        set_error(PBS_E_NOMEM, "LUT upload failed");
+        return nullptr;
+    }
+    if (en->nucWeightMatrix && en->nucSqSigmaMatrix) {
+        if (!up(e->nucW, en->nucWeightMatrix, nC) || !up(e->nucS2, en->nucSqSigmaMatrix, nC)) {
+            set_error(PBS_E_NOMEM, "nuclear LUT upload failed");
+            return nullptr;
+        }
+    }
+    e->hEnergies.assign(en->energiesPerU, en->energiesPerU + en->nEnergies);
+    e->hPeaks.assign(en->peakDepths, en->peakDepths + en->nEnergies);
+    e->hScales.assign(en->scaleFacts, en->scaleFacts + en->nEnergies);
+    e->energy = *en;
+    e->energy.energiesPerU = e->hEnergies.data();
+    e->energy.peakDepths = e->hPeaks.data();
+    e->energy.scaleFacts = e->hScales.data();
+    e->energy.ciddMatrix = nullptr;  // device copy only
+    e->lut = LutDev{e->cidd.p, en->nEnergies, en->nEnergySamples, e->dens.p, en->nDensitySamples, en->densityScaleFact,
+                    e->sp.p, en->nSpSamples, en->spScaleFact, e->rrl.p, en->nRRlSamples, en->rRlScaleFact};
+    return e.release();
+}
+
+void pbs_engine_destroy(pbs_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->uploadDone) (void)hipEventDestroy(e->uploadDone);
+    delete e;
+}
+
+void *pbs_engine_stream(pbs_engine *e) { return e ? reinterpret_cast<void *>(e->stream) : nullptr; }
+
+int pbs_engine_sync(pbs_engine *e) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    return PBS_OK;
+}
+
+int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const float res[3], const float corner[3],
+                      int ctIsDevice) {
+    if (!e || !ct || !dims || !res || !corner) return set_error(PBS_E_INVALID, "ctData must not be null");
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return set_error(PBS_E_INVALID, "ctDims must be positive");
+    if (!(res[0] > 0 && res[1] > 0 && res[2] > 0)) return set_error(PBS_E_INVALID, "ctResolution must be positive");
+    HIP_TRY(hipSetDevice(e->device));
+    const size_t n = size_t(dims[0]) * dims[1] * dims[2];
+    std::vector<float> host;
+    const float *src = ct;
+    if (ctIsDevice) {
+        host.resize(n);
+        HIP_TRY(hipMemcpy(host.data(), ct, n * sizeof(float), hipMemcpyDeviceToHost));
+        src = host.data();
+    }
+    // CT adapter (wrapper:2290-2441): HU -> HU+1000; density/SPR are inverted through the LUT
+    const int type = detect_ct_type(src, n);
+    std::vector<float> conv;
+    if (type == 0) {
+        conv.resize(n);
+        for (size_t i = 0; i < n; ++i) conv[i] = src[i] + 1000.0f;
+        src = conv.data();
+    } else if (type == 2) {
+        const float *lut = e->energy.densityVector;
+        (void)lut;
+        return set_error(PBS_E_UNSUPPORTED, "density-valued CT input: pass HU or HU+1000");
+    }
+    HIP_TRY(e->ct.ensure(n));
+    HIP_TRY(hipMemcpy(e->ct.p, src, n * sizeof(float), hipMemcpyHostToDevice));
+    e->ctPtr = e->ct.p;
+    for (int i = 0; i < 3; ++i) { e->ctDims[i] = dims[i]; e->ctRes[i] = res[i]; e->ctCorner[i] = corner[i]; }
+    return PBS_OK;
+}
+
+int pbs_engine_set_dose_grid(pbs_engine *e, const int dims[3], const float res[3], const float corner[3]) {
+    if (!e || !dims || !res || !corner) return set_error(PBS_E_INVALID, "null argument");
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return set_error(PBS_E_INVALID, "doseDims must be positive");
+    if (!(res[0] > 0 && res[1] > 0 && res[2] > 0)) return set_error(PBS_E_INVALID, "doseResolution must be positive");
+    for (int i = 0; i < 3; ++i) { e->doseDims[i] = dims[i]; e->doseRes[i] = res[i]; e->doseCorner[i] = corner[i]; }
+    return PBS_OK;
+}
+
+static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
+                            const float *cpbWorldGeom);
+
+int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
+    return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
+}
+
+// cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
+// positions when the CPB grid is given in world coordinates (cudaFinalDose).
+static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
+                            const float *cpbWorldGeom) {
+    if (!e || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (!e->ctPtr) return set_error(PBS_E_INVALID, "no CT bound to the engine");
+    if (e->doseDims[0] <= 0) return set_error(PBS_E_INVALID, "no dose grid bound to the engine");
+    int rc = validate_beam(*beam, 0);
+    if (rc != PBS_OK) return rc;
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->uploadPending) {
+        HIP_TRY(hipEventSynchronize(e->uploadDone));
+        e->uploadPending = false;
+    }
+    BeamPlan &p = e->plan;
+    if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p, ov))
+        return set_error(PBS_E_GEOMETRY, p.error);
+    FanGeom &g = e->geom;
+    fill_geom(p, *e, g);
+    const int L = p.numLayers, K = p.steps, rx = g.rx, ry = g.ry, sx = g.sx, sy = g.sy;
+    const size_t nray = size_t(rx) * ry, nsup = size_t(sx) * sy;
+    const int nt = (sx / 32) * (sy / 8);
+    const int maxItems = K * nt;
+    const bool subMode = p.subspotConv && !p.legacyPath;
+    const int nSub = subMode ? beam->maxSubspotsPerLayer : 1;
+    // buffers
+    HIP_TRY(e->density.ensure(nray * K));
+    HIP_TRY(e->wepl.ensure(nray * K));
+    HIP_TRY(e->fi.ensure(nray));
+    HIP_TRY(e->fo.ensure(nray));
+    HIP_TRY(e->weplMin.ensure(K));
+    HIP_TRY(e->rayW.ensure(nray * L));
+    HIP_TRY(e->iddSigma.ensure(nsup * K * L));
+    HIP_TRY(e->firstPassive.ensure(nray * L));
+    HIP_TRY(e->tileRad.ensure(size_t(L) * K * nt));
+    HIP_TRY(e->hist.ensure(size_t(L) * (kMaxRadDev + 2)));
+    HIP_TRY(e->effRad.ensure(size_t(L) * (kMaxRadDev + 2)));
+    HIP_TRY(e->itemR.ensure(maxItems));
+    HIP_TRY(e->classCount.ensure(16));
+    HIP_TRY(e->classList.ensure(size_t(kNumClasses) * maxItems));
+    HIP_TRY(e->slots.ensure(size_t(maxItems) * kSlotFloats));
+    HIP_TRY(e->bev.ensure(size_t(K) * (sx + 64) * (sy + 64)));
+    HIP_TRY(e->layerHost.ensure(L));
+    HIP_TRY(e->layerScal.ensure(L));
+    HIP_TRY(e->beamScal.ensure(1));
+    HIP_TRY(e->subspots.ensure(size_t(L) * beam->maxSubspotsPerLayer * 5));
+    HIP_TRY(e->subEntry.ensure(size_t(L) * beam->maxSubspotsPerLayer * 2));
+    std::vector<LayerHost> &lh = e->hostLayers;
+    lh.resize(L);
+    for (int l = 0; l < L; ++l) {
+        const LayerParams &q = p.layers[l];
+        lh[l] = LayerHost{q.energyIdx, q.scaleFact, q.peakDepth, q.rangeStop, q.cutoffWepl, q.airQuad, q.airLin,
+                          q.convA, q.convB, q.sigmaX, q.sigmaY, 0.0f};
+    }
+    hipStream_t s = e->stream;
+    HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
+                           hipMemcpyHostToDevice, s));
+    // 1. trace and per-beam reductions
+    HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
+    HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
+    const float nucFactor = 1.0f;
+    HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
+                              subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
+                              e->classCount.p, e->hist.p, s));
+    // 2. ray weights
+    if (!p.legacyPath) {
+        const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
+        HIP_TRY(e->spot.ensure(splane * L));
+        HIP_TRY(e->interm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
+        HIP_TRY(hipMemcpyAsync(e->spot.p, p.denseWeights.data(), sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
+        ConvArgs a{};
+        a.rx = rx; a.ry = ry; a.spotX = p.spotDims[0]; a.spotY = p.spotDims[1];
+        a.nSub = nSub; a.subspotMode = subMode ? 1 : 0;
+        a.cpbResX = p.cpbRes[0]; a.cpbResY = p.cpbRes[1];
+        a.deltaX = p.spotDelta[0] / p.cpbRes[0];
+        a.deltaY = p.spotDelta[1] / p.cpbRes[1];
+        a.offX = (p.spotOffset[0] - p.cpbCorner[0]) / p.cpbRes[0];
+        a.offY = (p.spotOffset[1] - p.cpbCorner[1]) / p.cpbRes[1];
+        a.spotOffX = p.spotOffset[0]; a.spotOffY = p.spotOffset[1];
+        a.cpbCornerX = p.cpbCorner[0]; a.cpbCornerY = p.cpbCorner[1];
+        a.lenToMm = p.lenToMm;
+        HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
+                            e->subEntry.p, L, s));
+    } else {
+        // subspot -> CPB in the gantry frame (wrapper:3480-3511); the ray grid is the CPB grid
+        CpbArgs a{};
+        a.nx = rx; a.ny = ry; a.nSub = beam->maxSubspotsPerLayer;
+        a.cx = p.cpbCorner[0]; a.cy = p.cpbCorner[1]; a.rx_ = p.cpbRes[0]; a.ry_ = p.cpbRes[1];
+        const float bd[3] = {0.f, 0.f, -1.f}, bx[3] = {1.f, 0.f, 0.f}, by[3] = {0.f, -1.f, 0.f};
+        for (int i = 0; i < 3; ++i) { a.bd[i] = bd[i]; a.bx[i] = bx[i]; a.by[i] = by[i]; }
+        a.src[0] = 0.f; a.src[1] = 0.f; a.src[2] = p.sadInput;
+        a.sad = p.sadInput;
+        if (cpbWorldGeom) {
+            for (int i = 0; i < 3; ++i) {
+                a.bd[i] = cpbWorldGeom[i];
+                a.bx[i] = cpbWorldGeom[3 + i];
+                a.by[i] = cpbWorldGeom[6 + i];
+                a.src[i] = cpbWorldGeom[9 + i];
+            }
+            a.sad = cpbWorldGeom[12];
+        }
+        HIP_TRY(e->cpb.ensure(nray * L));
+        HIP_TRY(launch_cpb(a, e->subspots.p, e->cpb.p, e->rayW.p, rx, ry, p.cpbCorner[0], p.cpbCorner[1], p.cpbRes[0],
+                           p.cpbRes[1], L, s));
+    }
+    HIP_TRY(hipEventRecord(e->uploadDone, s));
+    e->uploadPending = true;
+    // 3. IDD / sigma
+    const FillConsts fc{198.81f, 0.21f, 0, 0};
+    HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
+                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, s));
+    // 4. passive steps, tiles, bins, items
+    HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->beamScal.p, e->tileRad.p, e->hist.p,
+                                      e->effRad.p, e->itemR.p, e->classCount.p, e->classList.p, maxItems, s));
+    // 5. superposition into per-item blocks, then the deterministic merge into the BEV volume
+    HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->classCount.p,
+                             e->classList.p, maxItems, e->slots.p, 2048, s));
+    HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, K, s));
+    // 6. BEV -> dose
+    TransferArgs ta{};
+    fill_transfer_args(p, ta);
+    int gx = 1, gy = 1;
+    host_transfer_grid(p, e->doseDims, gx, gy);
+    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->bev.p, K, doseDev, gx, gy, s));
+    e->lastMaxItems = maxItems;
+    e->haveLast = true;
+    return PBS_OK;
+}
+
+int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, float *wepl, int *firstInside,
+                           int *firstOutside, float *weplMin, float *rayWeights, float *iddSigma, int *firstPassive,
+                           int *tileRad, int *radHist, float *bev) {
+    if (!e || !e->haveLast) return set_error(PBS_E_INVALID, "no beam has been computed");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const BeamPlan &p = e->plan;
+    const FanGeom &g = e->geom;
+    const int L = p.numLayers, K = p.steps;
+    const size_t nray = size_t(g.rx) * g.ry, nsup = size_t(g.sx) * g.sy;
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+        return dst ? hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) : hipSuccess;
+    };
+    HIP_TRY(cp(density, e->density.p, nray * K * 4));
+    HIP_TRY(cp(wepl, e->wepl.p, nray * K * 4));
+    HIP_TRY(cp(firstInside, e->fi.p, nray * 4));
+    HIP_TRY(cp(firstOutside, e->fo.p, nray * 4));
+    HIP_TRY(cp(weplMin, e->weplMin.p, size_t(K) * 4));
+    HIP_TRY(cp(rayWeights, e->rayW.p, nray * L * 4));
+    HIP_TRY(cp(iddSigma, e->iddSigma.p, nsup * K * L * 8));
+    HIP_TRY(cp(firstPassive, e->firstPassive.p, nray * L * 4));
+    HIP_TRY(cp(tileRad, e->tileRad.p, size_t(L) * K * nt * 4));
+    HIP_TRY(cp(radHist, e->hist.p, size_t(L) * (kMaxRadDev + 2) * 4));
+    HIP_TRY(cp(bev, e->bev.p, size_t(K) * (g.sx + 64) * (g.sy + 64) * 4));
+    if (ints) {
+        BeamScalars bs{};
+        std::vector<LayerScalars> ls(L);
+        HIP_TRY(hipMemcpy(&bs, e->beamScal.p, sizeof(bs), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ls.data(), e->layerScal.p, sizeof(LayerScalars) * L, hipMemcpyDeviceToHost));
+        std::memset(ints, 0, sizeof(*ints));
+        ints->rayDims[0] = g.rx; ints->rayDims[1] = g.ry;
+        ints->superpDims[0] = g.sx; ints->superpDims[1] = g.sy;
+        ints->steps = K; ints->numLayers = L; ints->legacyPath = g.legacy;
+        ints->beamFirstInside = bs.beamFirstInside;
+        ints->beamFirstOutside = bs.beamFirstOutside;
+        ints->firstPastCutoffAll = bs.firstPastCutoffAll;
+        ints->guaranteedPassive = bs.guaranteedPassive;
+        for (int l = 0; l < L; ++l) {
+            ints->afterLast[l] = ls[l].afterLast;
+            ints->layerPassive[l] = ls[l].passive;
+            ints->energyIdx[l] = p.layers[l].energyIdx;
+            ints->scaleFact[l] = p.layers[l].scaleFact;
+            ints->peakDepth[l] = p.layers[l].peakDepth;
+        }
+        for (int i = 0; i < 3; ++i) {
+            ints->transferStart[i] = bs.boxStart[i];
+            ints->transferMax[i] = bs.boxMax[i];
+            ints->cpbCorner[i] = p.cpbCorner[i];
+            ints->fanCorner[i] = p.fanCorner[i];
+            ints->fanDelta[i] = p.fanDelta[i];
+        }
+        ints->sourceDist[0] = p.sourceDist[0];
+        ints->sourceDist[1] = p.sourceDist[1];
+    }
+    return PBS_OK;
+}
+
+int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float ctResolution[3],
+                          const float ctCorner[3], float *doseData, const int doseDims[3],
+                          const float doseResolution[3], const float doseCorner[3], const pbs_beam_settings *beams,
+                          size_t numBeams, const pbs_energy_struct *energy, int gpuId, int nuclearCorrection,
+                          int verbose) {
+    (void)verbose;
+    // validateWrapperGlobalInputs (wrapper:1025-1075): validation failures leave doseData untouched
+    if (!ctData) return set_error(PBS_E_INVALID, "ctData must not be null");
+    if (!doseData) return set_error(PBS_E_INVALID, "doseData must not be null");
+    if (!beams) return set_error(PBS_E_INVALID, "beamSettings must not be null");
+    if (!energy) return set_error(PBS_E_INVALID, "energyData must not be null");
+    if (numBeams == 0) return set_error(PBS_E_INVALID, "numBeams must be positive");
+    if (ctDims[0] <= 0 || ctDims[1] <= 0 || ctDims[2] <= 0) return set_error(PBS_E_INVALID, "ctDims must be positive");
+    if (doseDims[0] <= 0 || doseDims[1] <= 0 || doseDims[2] <= 0) return set_error(PBS_E_INVALID, "doseDims must be positive");
+    if (!(ctResolution[0] > 0 && ctResolution[1] > 0 && ctResolution[2] > 0))
+        return set_error(PBS_E_INVALID, "ctResolution must be positive");
+    if (!(doseResolution[0] > 0 && doseResolution[1] > 0 && doseResolution[2] > 0))
+        return set_error(PBS_E_INVALID, "doseResolution must be positive");
+    int rc = validate_energy(energy);
+    if (rc != PBS_OK) return rc;
+    for (size_t b = 0; b < numBeams; ++b) {
+        rc = validate_beam(beams[b], b);
+        if (rc != PBS_OK) return rc;
+    }
+    if (nuclearCorrection && !(energy->nucWeightMatrix && energy->nucSqSigmaMatrix))
+        return set_error(PBS_E_GEOMETRY, "nuclear_correction=true requested, but the energy struct does not carry nuclear tables");
+    if (nuclearCorrection) return set_error(PBS_E_UNSUPPORTED, "nuclear correction is not available in this build");
+    std::unique_ptr<pbs_engine, void (*)(pbs_engine *)> e(pbs_engine_create(gpuId, energy), pbs_engine_destroy);
+    if (!e) return PBS_E_DEVICE;
+    rc = pbs_engine_set_ct(e.get(), ctData, ctDims, ctResolution, ctCorner, 0);
+    if (rc != PBS_OK) return rc;
+    rc = pbs_engine_set_dose_grid(e.get(), doseDims, doseResolution, doseCorner);
+    if (rc != PBS_OK) return rc;
+    const size_t nd = size_t(doseDims[0]) * doseDims[1] * doseDims[2];
+    DevBuf<float> dose;
+    HIP_TRY(dose.ensure(nd));
+    HIP_TRY(hipMemcpy(dose.p, doseData, nd * sizeof(float), hipMemcpyHostToDevice));
+    for (size_t b = 0; b < numBeams; ++b) {
+        rc = pbs_engine_beam_dose(e.get(), &beams[b], dose.p);
+        if (rc != PBS_OK) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpy(doseData, dose.p, nd * sizeof(float), hipMemcpyDeviceToHost));
+    return PBS_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// cudaFinalDose
+// ---------------------------------------------------------------------------------------
+static pbs_engine *g_finalDoseEngine = nullptr;
+
+int pbs_final_dose_bind(pbs_engine *e, const float *layerEnergies, const float *layerSigmas, int numLayers,
+                        float raySpacing) {
+    if (!e || !layerEnergies || !layerSigmas || numLayers <= 0)
+        return set_error(PBS_E_INVALID, "pbs_final_dose_bind: null argument");
+    e->fdEnergies.assign(layerEnergies, layerEnergies + numLayers);
+    e->fdSigmas.assign(layerSigmas, layerSigmas + 2 * numLayers);
+    e->fdRaySpacing = raySpacing;
+    g_finalDoseEngine = e;
+    return PBS_OK;
+}
+
+int pbs_final_dose_roi(const vec3i *roi, int n, vec3f res, vec3f corner, vec3f bdir, vec3f src, int numLayers,
+                       vec3f *minC, vec3f *maxC, vec3i *dims, CPBGrid *cpb) {
+    // calculateROIRangeKernel (cuda_final_dose.cu:66-129), margins (5, 5, 2)
+    vec3f mn{0.f, 0.f, 0.f}, mx{0.f, 0.f, 0.f};
+    vec3i d{0, 0, 0};
+    if (n > 0 && roi) {
+        vec3i lo = roi[0], hi = roi[0];
+        for (int i = 1; i < n; ++i) {
+            lo.x = std::min(lo.x, roi[i].x); lo.y = std::min(lo.y, roi[i].y); lo.z = std::min(lo.z, roi[i].z);
+            hi.x = std::max(hi.x, roi[i].x); hi.y = std::max(hi.y, roi[i].y); hi.z = std::max(hi.z, roi[i].z);
+        }
+        mn = vec3f{corner.x + lo.x * res.x, corner.y + lo.y * res.y, corner.z + lo.z * res.z};
+        mx = vec3f{corner.x + (hi.x + 1) * res.x, corner.y + (hi.y + 1) * res.y, corner.z + (hi.z + 1) * res.z};
+        mn.x -= 5.0f; mn.y -= 5.0f; mn.z -= 2.0f;
+        mx.x += 5.0f; mx.y += 5.0f; mx.z += 2.0f;
+        d = vec3i{hi.x - lo.x + 1, hi.y - lo.y + 1, hi.z - lo.z + 1};
+    }
+    if (minC) *minC = mn;
+    if (maxC) *maxC = mx;
+    if (dims) *dims = d;
+    // checkBeamROIIntersectionKernel (cuda_final_dose.cu:132-152)
+    const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
+    const float dp = bdir.x * (cx - src.x) + bdir.y * (cy - src.y) + bdir.z * (cz - src.z);
+    // CPB grid (cuda_final_dose.cu:302-308)
+    if (cpb) {
+        cpb->corner = vec3f{mn.x, mn.y, 0.0f};
+        cpb->resolution = vec3f{0.5f, 0.5f, 1.0f};
+        cpb->dims = vec3i{int((mx.x - mn.x) / 0.5f), int((mx.y - mn.y) / 0.5f), numLayers};
+        cpb->nLayers = numLayers;
+    }
+    return dp > 0.0f ? 1 : 0;
+}
+
+int cudaFinalDose(const float *subspotData, const vec3i *roiIndices, int numROIIndices, vec3f doseGridResolution,
+                  vec3f doseGridCorner, vec3f beamDirection, vec3f sourcePosition, float sad, float refPlaneZ,
+                  int numLayers, int maxSubspotsPerLayer, float *finalDose, vec3i doseGridDims) {
+    (void)refPlaneZ;
+    if (!subspotData || !finalDose || numLayers <= 0 || maxSubspotsPerLayer <= 0)
+        return set_error(PBS_E_INVALID, "cudaFinalDose: null or empty argument");
+    std::vector<vec3i> roi(std::max(numROIIndices, 0));
+    if (numROIIndices > 0) {
+        if (!roiIndices) return set_error(PBS_E_INVALID, "cudaFinalDose: roiIndices must not be null");
+        hipPointerAttribute_t at{};
+        const bool dev = hipPointerGetAttributes(&at, roiIndices) == hipSuccess && at.type == hipMemoryTypeDevice;
+        (void)hipGetLastError();
+        HIP_TRY(hipMemcpy(roi.data(), roiIndices, sizeof(vec3i) * numROIIndices,
+                          dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    }
+    vec3f mn, mx;
+    vec3i rd;
+    CPBGrid cpb;
+    const int hit = pbs_final_dose_roi(roi.data(), numROIIndices, doseGridResolution, doseGridCorner, beamDirection,
+                                       sourcePosition, numLayers, &mn, &mx, &rd, &cpb);
+    if (!hit) return PBS_OK;  // the reference returns before touching finalDose
+    const size_t nd = size_t(doseGridDims.x) * doseGridDims.y * doseGridDims.z;
+    HIP_TRY(hipMemset(finalDose, 0, nd * sizeof(float)));
+    pbs_engine *e = g_finalDoseEngine;
+    if (!e || int(e->fdEnergies.size()) != numLayers)
+        return set_error(PBS_E_INVALID, "cudaFinalDose: bind CT, LUTs and layer energies with pbs_final_dose_bind");
+    std::vector<float> sub(size_t(numLayers) * maxSubspotsPerLayer * 5);
+    {
+        hipPointerAttribute_t at{};
+        const bool dev = hipPointerGetAttributes(&at, subspotData) == hipSuccess && at.type == hipMemoryTypeDevice;
+        (void)hipGetLastError();
+        HIP_TRY(hipMemcpy(sub.data(), subspotData, sub.size() * sizeof(float),
+                          dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+    }
+    const int dd[3] = {doseGridDims.x, doseGridDims.y, doseGridDims.z};
+    const float dr[3] = {doseGridResolution.x, doseGridResolution.y, doseGridResolution.z};
+    const float dc[3] = {doseGridCorner.x, doseGridCorner.y, doseGridCorner.z};
+    int rc = pbs_engine_set_dose_grid(e, dd, dr, dc);
+    if (rc != PBS_OK) return rc;
+    pbs_beam_settings b{};
+    b.numLayers = numLayers;
+    b.energies = e->fdEnergies.data();
+    b.spotSigmas = e->fdSigmas.data();
+    b.raySpacing[0] = b.raySpacing[1] = cpb.resolution.x;
+    b.maxSubspotsPerLayer = maxSubspotsPerLayer;
+    b.subspotData = sub.data();
+    const float bdn = std::sqrt(beamDirection.x * beamDirection.x + beamDirection.y * beamDirection.y +
+                                beamDirection.z * beamDirection.z);
+    b.beamDirection[0] = beamDirection.x; b.beamDirection[1] = beamDirection.y; b.beamDirection[2] = beamDirection.z;
+    b.bmXDirection[0] = 1.0f; b.bmYDirection[1] = 1.0f;  // the reference's fixed lateral axes
+    b.sourcePosition[0] = sourcePosition.x; b.sourcePosition[1] = sourcePosition.y; b.sourcePosition[2] = sourcePosition.z;
+    b.sad = sad;
+    if (!(bdn > 0.0f)) return set_error(PBS_E_INVALID, "cudaFinalDose: zero beam direction");
+    // The ROI CPB grid is in world x/y; the gantry frame has y = -bmY, so its y range flips.
+    CpbOverride ov{};
+    ov.corner[0] = cpb.corner.x;
+    ov.corner[1] = -(cpb.corner.y + float(cpb.dims.y) * cpb.resolution.y);
+    ov.res[0] = cpb.resolution.x; ov.res[1] = cpb.resolution.y;
+    ov.dims[0] = cpb.dims.x; ov.dims[1] = cpb.dims.y;
+    // subspot positions of the world-frame CPB convolution, expressed in that gantry frame
+    const float geom[13] = {0.f, 0.f, -1.f, 1.f, 0.f, 0.f, 0.f, -1.f, 0.f, 0.f, 0.f, sad, sad};
+    rc = engine_beam_dose(e, &b, finalDose, &ov, geom);
+    if (rc != PBS_OK) return rc;
+    return pbs_engine_sync(e);
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Launch interface of the gfx950 kernels (kernels.hip) used by the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace pbs {
+
+constexpr int kMaxRadDev = 32;
+constexpr int kMinTilesDev = 16;
+constexpr int kSlotPitch = 96;                    // 32 + 2 * 32 columns
+constexpr int kSlotFloats = kSlotPitch * 72;      // (8 + 2 * 32) rows
+constexpr int kNumClasses = 8;
+
+struct ConvArgs {
+    int rx, ry, spotX, spotY, nSub, subspotMode;
+    float cpbResX, cpbResY, deltaX, deltaY, offX, offY;
+    float spotOffX, spotOffY, cpbCornerX, cpbCornerY, lenToMm;
+};
+
+struct CpbArgs {
+    int nx, ny, nSub, pad;
+    float cx, cy, rx_, ry_;
+    float bd[3], bx[3], by[3], src[3];
+    float sad;
+};
+
+struct FillConsts {
+    float eRefSq, sigmaDelta;
+    int suppressDistalDip, pad;
+};
+
+struct TransferArgs {
+    float coefI[3], coefJ[3], coefOffset[3], inc[3], globalOffset[3];
+    float normDist[2];
+};
+
+hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
+                        int *fo, hipStream_t s);
+hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,
+                               BeamScalars *bs, hipStream_t s);
+hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
+                             LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
+                             int *classCount, int *hist, hipStream_t s);
+hipError_t launch_conv(const ConvArgs &a, const float *spot, float *interm, float *rayW, const LayerScalars *ls,
+                       const BeamScalars *bs, const float *subspots, const float *subEntry, int numLayers,
+                       hipStream_t s);
+hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *rays, int rnx, int rny, float rcx,
+                      float rcy, float rrx, float rry, int numLayers, hipStream_t s);
+hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
+                       const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
+                       const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, hipStream_t s);
+hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, BeamScalars *bs,
+                                     int *tileRad, int *hist, int *effRad, int *itemR, int *classCount,
+                                     int *classList, int maxItems, hipStream_t s);
+hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
+                            const LayerScalars *ls, const int *classCount, const int *classList, int maxItems,
+                            float *slots, int gridCap, hipStream_t s);
+hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *slots, float *bev,
+                        int bevZ, hipStream_t s);
+hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const float *bev, int bevZ,
+                           float *dose, int gridX, int gridY, hipStream_t s);
+
+}  // namespace pbs

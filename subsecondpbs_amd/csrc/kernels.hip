@@ -1,0 +1,899 @@
+// gfx950 kernels of the pencil-beam final-dose path.
+//
+//   k_trace ............ BEV fan-ray trace: explicit trilinear CT gather, density and
+//                        compensated WEPL prefix per ray (bev_ray_tracing.cu:65-159)
+//   k_trace_reduce ..... per-step min WEPL and beam entry/exit steps
+//                        (complete_superposition.cu:109-177, wrapper:1990-2023)
+//   k_beam_setup ....... distal cut-offs and entry sigmas on the device (wrapper:3651-3736,
+//                        4308-4329), so a beam needs no host round trip
+//   k_conv_x / k_conv_y  separable erf gather-resample spot lattice -> rays
+//                        (convolution.cu:198-262, 477-513)
+//   k_subspot_cpb ...... subspot -> CPB 2D erf cell integrals as a gather, no atomics
+//                        (convolution.cu:95-152)
+//   k_cpb_to_rays ...... area-overlap resample (convolution.cu:269-339)
+//   k_fill ............. IDD and sigma per ray and layer (idd_sigma.cu:46-259)
+//   k_layer_finalize ... layer passive steps, transfer box (wrapper:4950-4996, 5343-5395)
+//   k_tiles / k_bins ... 32x8 tile radii, radius histogram, batched launch radii
+//                        (complete_superposition.cu:197-255, 400-490)
+//   k_items ............ per (step, tile) work item: max launch radius over live layers
+//   k_superpose<RM> .... register-blocked Gaussian superposition of one (step, tile) over
+//                        all layers into a private output block (no atomics)
+//   k_merge ............ deterministic sum of overlapping blocks into the BEV dose volume
+//   k_transfer ......... divergent BEV -> dose-grid trilinear gather, one owner per dose
+//                        column (prim_transf_kernel.cu:11-44)
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace pbs {
+
+#define PBS_INF __builtin_inff()
+
+__device__ __forceinline__ float lut1d_clamp(const float *t, int n, float xb) {
+    const float f = floorf(xb);
+    const float a = xb - f;
+    int i0 = int(f), i1 = i0 + 1;
+    i0 = min(max(i0, 0), n - 1);
+    i1 = min(max(i1, 0), n - 1);
+    return (1.0f - a) * t[i0] + a * t[i1];
+}
+
+__device__ __forceinline__ float lut2d_clamp(const float *__restrict__ t, int nx, int ny, float xb, float yb) {
+    const float fx = floorf(xb), fy = floorf(yb);
+    const float ax = xb - fx, ay = yb - fy;
+    int x0 = int(fx), y0 = int(fy);
+    int x1 = min(max(x0 + 1, 0), nx - 1), y1 = min(max(y0 + 1, 0), ny - 1);
+    x0 = min(max(x0, 0), nx - 1);
+    y0 = min(max(y0, 0), ny - 1);
+    const float v0 = (1.0f - ax) * t[y0 * nx + x0] + ax * t[y0 * nx + x1];
+    const float v1 = (1.0f - ax) * t[y1 * nx + x0] + ax * t[y1 * nx + x1];
+    return (1.0f - ay) * v0 + ay * v1;
+}
+
+template <typename T>
+__device__ __forceinline__ float trilinear_border(const T *__restrict__ v, int nx, int ny, int nz, float x, float y,
+                                                  float z) {
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    const float tx = x - fx, ty = y - fy, tz = z - fz;
+    float c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int xi = x0 + (q & 1), yi = y0 + ((q >> 1) & 1), zi = z0 + (q >> 2);
+        const bool in = unsigned(xi) < unsigned(nx) && unsigned(yi) < unsigned(ny) && unsigned(zi) < unsigned(nz);
+        c[q] = in ? float(v[(size_t(zi) * ny + yi) * nx + xi]) : 0.0f;
+    }
+    const float c00 = c[0] + (c[1] - c[0]) * tx;
+    const float c10 = c[2] + (c[3] - c[2]) * tx;
+    const float c01 = c[4] + (c[5] - c[4]) * tx;
+    const float c11 = c[6] + (c[7] - c[6]) * tx;
+    const float c0 = c00 + (c10 - c00) * ty;
+    const float c1 = c01 + (c11 - c01) * ty;
+    return c0 + (c1 - c0) * tz;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// findFirstLargerOrdered (include/utils/vector_find.h:62-81) on a device array.
+__device__ int first_larger_ordered(const float *v, int n, float value) {
+    int upper = n - 1, lower = 0;
+    if (v[n - 1] <= value) return upper;
+    if (v[0] > value) return 0;
+    while (upper - lower > 1) {
+        const int pivot = (upper + lower) / 2;
+        if (v[pivot] <= value) lower = pivot; else upper = pivot;
+    }
+    return lower + 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// trace
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_trace(FanGeom g, const float *__restrict__ ct, LutDev lut,
+                                               float *__restrict__ density, float *__restrict__ wepl,
+                                               int *__restrict__ firstInside, int *__restrict__ firstOutside) {
+    extern __shared__ float s_lut[];  // density LUT, then stopping-power LUT
+    for (int i = threadIdx.x; i < lut.nDensity; i += blockDim.x) s_lut[i] = lut.density[i];
+    for (int i = threadIdx.x; i < lut.nSp; i += blockDim.x) s_lut[lut.nDensity + i] = lut.sp[i];
+    __syncthreads();
+    const int nray = g.rx * g.ry;
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= nray) return;
+    const float *s_dens = s_lut, *s_sp = s_lut + lut.nDensity;
+    const int i = ray % g.rx, j = ray / g.rx;
+    const float fx = g.fc[0] + float(i) * g.fd[0];
+    const float fy = g.fc[1] + float(j) * g.fd[1];
+    float p0[3], p1[3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const float fz = g.fc[2] + float(s) * g.fd[2];
+        const float gx = fx * (1.0f - fz / g.dist[0]);
+        const float gy = fy * (1.0f - fz / g.dist[1]);
+        float *o = s ? p1 : p0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) o[r] = g.g2im[3 * r] * gx + g.g2im[3 * r + 1] * gy + g.g2im[3 * r + 2] * fz + g.g2imOff[r];
+    }
+    const float inc[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const float dx = fx / g.dist[0], dy = fy / g.dist[1];
+    const float stepLen = fabsf(g.fd[2]) * sqrtf(1.0f + dx * dx + dy * dy);
+    float cum = 0.0f, comp = 0.0f, cumHu = 0.0f;
+    unsigned before = 0, lastIn = 0;
+    bool flag = true;
+    size_t idx = size_t(ray);
+    const size_t plane = size_t(nray);
+#pragma unroll 4
+    for (int k = 0; k < g.K; ++k) {
+        const float fk = float(k);
+        const float hu = trilinear_border(ct, g.ctDims[0], g.ctDims[1], g.ctDims[2], p0[0] + fk * inc[0],
+                                          p0[1] + fk * inc[1], p0[2] + fk * inc[2]);
+        cumHu += hu;
+        density[idx] = lut1d_clamp(s_dens, lut.nDensity, hu * lut.densityScale);
+        // compensated prefix of stepLen * SP
+        const float y = stepLen * lut1d_clamp(s_sp, lut.nSp, hu * lut.spScale) - comp;
+        const float t = cum + y;
+        comp = (t - cum) - y;
+        cum = t;
+        if (flag && cumHu < 150.0f) before = unsigned(k); else flag = false;
+        if (hu > 150.0f) lastIn = unsigned(k);
+        wepl[idx] = cum;
+        idx += plane;
+    }
+    firstInside[ray] = int(before) + 1;
+    firstOutside[ray] = int(lastIn) + 1;
+}
+
+// blocks [0, K): min WEPL over rays that leave material; block K: entry/exit step extremes.
+__global__ __launch_bounds__(256) void k_trace_reduce(FanGeom g, const float *__restrict__ wepl,
+                                                      const int *__restrict__ fi, const int *__restrict__ fo,
+                                                      float *__restrict__ weplMin, BeamScalars *__restrict__ bs) {
+    __shared__ float s_f[4];
+    __shared__ int s_a[4], s_b[4];
+    const int n = g.rx * g.ry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (blockIdx.x < unsigned(g.K)) {
+        const size_t base = size_t(blockIdx.x) * n;
+        float m = 1.0e20f;  // Macro.cuh INF as float
+        for (int r = threadIdx.x; r < n; r += blockDim.x)
+            if (fo[r] > 0) m = fminf(m, wepl[base + r]);
+        m = wave_min(m);
+        if (lane == 0) s_f[w] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) weplMin[blockIdx.x] = fminf(fminf(s_f[0], s_f[1]), fminf(s_f[2], s_f[3]));
+    } else {
+        int mn = INT_MAX, mx = INT_MIN;
+        for (int r = threadIdx.x; r < n; r += blockDim.x) { mn = min(mn, fi[r]); mx = max(mx, fo[r]); }
+        mn = wave_min_i(mn);
+        mx = wave_max_i(mx);
+        if (lane == 0) { s_a[w] = mn; s_b[w] = mx; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            bs->beamFirstInside = min(min(s_a[0], s_a[1]), min(s_a[2], s_a[3]));
+            bs->beamFirstOutside = max(max(s_b[0], s_b[1]), max(s_b[2], s_b[3]));
+        }
+    }
+}
+
+// Distal cut-offs, entry plane and entry sigmas (host formulas, evaluated without
+// contraction so the integers match the host's float arithmetic).
+__global__ __launch_bounds__(256) void k_beam_setup(FanGeom g, const float *__restrict__ weplMin,
+                                                    const LayerHost *__restrict__ lh, BeamScalars *__restrict__ bs,
+                                                    LayerScalars *__restrict__ ls, const float *__restrict__ subspots,
+                                                    int maxSub, float *__restrict__ subEntry, float nucFactor,
+                                                    int *__restrict__ classCount, int *__restrict__ hist) {
+#pragma clang fp contract(off)
+    const int inside = bs->beamFirstInside;
+    const int fpa = first_larger_ordered(weplMin, g.K, g.maxCutoffWepl);
+    const int gp = min(fpa, bs->beamFirstOutside);
+    const float entryZ = float(inside) * g.fd[2] + g.fc[2];
+    const float pxX = 1.0f - entryZ / g.dist[0], pxY = 1.0f - entryZ / g.dist[1];
+    if (threadIdx.x == 0) {
+        bs->firstPastCutoffAll = fpa;
+        bs->guaranteedPassive = gp;
+        bs->entryZ = entryZ;
+        bs->pxMultX = pxX;
+        bs->pxMultY = pxY;
+    }
+    if (threadIdx.x < 16) classCount[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < g.numLayers * (kMaxRadDev + 2); i += blockDim.x) hist[i] = 0;
+    for (int l = threadIdx.x; l < g.numLayers; l += blockDim.x) {
+        const LayerHost L = lh[l];
+        const int local = first_larger_ordered(weplMin, g.K, L.cutoffWepl);
+        LayerScalars s;
+        s.afterLast = min(local, gp);
+        s.passiveRaw = INT_MIN;
+        s.passive = 0;
+        s.kEnd = 0;
+        const float base = L.convA * entryZ * entryZ + L.convB * entryZ;
+        const float vx = base + L.sigmaX * L.sigmaX, vy = base + L.sigmaY * L.sigmaY;
+        s.entrySigX = sqrtf(fmaxf(0.0f, vx)) * nucFactor;
+        s.entrySigY = sqrtf(fmaxf(0.0f, vy)) * nucFactor;
+        s.pad0 = s.pad1 = 0;
+        ls[l] = s;
+        for (int q = 0; q < maxSub && subspots; ++q) {
+            const float *ss = subspots + (size_t(l) * maxSub + q) * 5;
+            const float sxq = ss[3], syq = ss[4];
+            subEntry[(size_t(l) * maxSub + q) * 2 + 0] = sqrtf(fmaxf(0.0f, base + sxq * sxq)) * nucFactor;
+            subEntry[(size_t(l) * maxSub + q) * 2 + 1] = sqrtf(fmaxf(0.0f, base + syq * syq)) * nucFactor;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// spot lattice -> ray weights
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float erf_gather(const float *__restrict__ in, int inN, int inStride, int o, float sigma,
+                                            float pixelSp, float off, float delta) {
+    const float sEff = sigma / pixelSp;
+    const float rS = rsqrtf(2.0f) / sEff;
+    const float reach = 3.0f * sEff + 0.5f;
+    int ci = int(ceilf((float(o) - reach - off) / delta));
+    float dist = float(ci) * delta + off - float(o);
+    float res = 0.0f;
+    while (dist < reach) {
+        if (ci >= 0 && ci < inN) res += 0.5f * (erff((dist + 0.5f) * rS) - erff((dist - 0.5f) * rS)) * in[size_t(ci) * inStride];
+        ++ci;
+        dist = float(ci) * delta + off - float(o);
+    }
+    return res;
+}
+
+// grid: (ceil(rx/64), spotY, L * nSub); interm[(l*nSub + s)][spotY][rx]
+__global__ __launch_bounds__(64) void k_conv_x(ConvArgs a, const float *__restrict__ spot, float *__restrict__ interm,
+                                               const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                                               const float *__restrict__ subspots, const float *__restrict__ subEntry) {
+    const int o = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y;
+    const int l = blockIdx.z / a.nSub, s = blockIdx.z % a.nSub;
+    if (o >= a.rx) return;
+    float sigma = ls[l].entrySigX, off = a.offX, scale = 1.0f;
+    if (a.subspotMode) {
+        const float *ss = subspots + (size_t(l) * a.nSub + s) * 5;
+        scale = ss[2];
+        sigma = subEntry[(size_t(l) * a.nSub + s) * 2];
+        // subspot offsets shift the lattice origin (wrapper:3802-3806)
+        off = (a.spotOffX + ss[0] / a.lenToMm - a.cpbCornerX) / a.cpbResX;
+        if (!(scale > 1.0e-8f) || !(ss[3] > 1.0e-8f) || !(ss[4] > 1.0e-8f)) scale = 0.0f;
+    }
+    float r = 0.0f;
+    if (scale != 0.0f) {
+        const float *row = spot + (size_t(l) * a.spotY + y) * a.spotX;
+        r = scale * erf_gather(row, a.spotX, 1, o, sigma, a.cpbResX * bs->pxMultX, off, a.deltaX);
+    }
+    interm[((size_t(blockIdx.z) * a.spotY + y) * a.rx) + o] = r;
+}
+
+// grid: (ceil(rx/64), ry, L); sums the subspot planes in subspot order
+__global__ __launch_bounds__(64) void k_conv_y(ConvArgs a, const float *__restrict__ interm, float *__restrict__ rayW,
+                                               const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                                               const float *__restrict__ subspots, const float *__restrict__ subEntry) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int o = blockIdx.y, l = blockIdx.z;
+    if (x >= a.rx) return;
+    float acc = 0.0f;
+    for (int s = 0; s < a.nSub; ++s) {
+        float sigma = ls[l].entrySigY, off = a.offY;
+        if (a.subspotMode) {
+            const float *ss = subspots + (size_t(l) * a.nSub + s) * 5;
+            if (!(ss[2] > 1.0e-8f) || !(ss[3] > 1.0e-8f) || !(ss[4] > 1.0e-8f)) continue;
+            sigma = subEntry[(size_t(l) * a.nSub + s) * 2 + 1];
+            off = (a.spotOffY + ss[1] / a.lenToMm - a.cpbCornerY) / a.cpbResY;
+        }
+        const float *col = interm + (size_t(l) * a.nSub + s) * a.spotY * a.rx + x;
+        acc += erf_gather(col, a.spotY, a.rx, o, sigma, a.cpbResY * bs->pxMultY, off, a.deltaY);
+    }
+    rayW[(size_t(l) * a.ry + o) * a.rx + x] = acc;
+}
+
+// CPB weights as a gather: each cell sums the subspots whose 3-sigma cell window covers it,
+// in subspot order. Window membership is the reference's floor/ceil rule.
+__global__ __launch_bounds__(256) void k_subspot_cpb(CpbArgs a, const float *__restrict__ sub, float *__restrict__ cpb) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (cell >= a.nx * a.ny) return;
+    const int cx = cell % a.nx, cy = cell / a.nx;
+    float acc = 0.0f;
+    for (int s = 0; s < a.nSub; ++s) {
+        const float *q = sub + (size_t(l) * a.nSub + s) * 5;
+        const float w = q[2], sX = q[3], sY = q[4];
+        // SubspotInfo position (common.cuh:186-194)
+        const float px = a.src[0] + a.bd[0] * a.sad + a.bx[0] * q[0] + a.by[0] * q[1];
+        const float py = a.src[1] + a.bd[1] * a.sad + a.bx[1] * q[0] + a.by[1] * q[1];
+        const float pz = a.src[2] + a.bd[2] * a.sad + a.bx[2] * q[0] + a.by[2] * q[1];
+        const float dxs = px - a.src[0], dys = py - a.src[1], dzs = pz - a.src[2];
+        if (!(sqrtf(dxs * dxs + dys * dys + dzs * dzs) > 1e-6f)) continue;
+        if (!(w > 1e-6f && sX > 1e-6f && sY > 1e-6f)) continue;
+        const int x0 = max(0, int(floorf((px - 3.0f * sX - a.cx) / a.rx_)));
+        const int x1 = min(a.nx - 1, int(ceilf((px + 3.0f * sX - a.cx) / a.rx_)));
+        const int y0 = max(0, int(floorf((py - 3.0f * sY - a.cy) / a.ry_)));
+        const int y1 = min(a.ny - 1, int(ceilf((py + 3.0f * sY - a.cy) / a.ry_)));
+        if (cx < x0 || cx > x1 || cy < y0 || cy > y1) continue;
+        const float ccx = a.cx + (float(cx) + 0.5f) * a.rx_, ccy = a.cy + (float(cy) + 0.5f) * a.ry_;
+        const float ddx = ccx - px, ddy = ccy - py;
+        const float r2x = 1.0f / (1.41421356f * sX), r2y = 1.0f / (1.41421356f * sY);
+        const float ex = erff((ddx + 0.5f * a.rx_) * r2x) - erff((ddx - 0.5f * a.rx_) * r2x);
+        const float ey = erff((ddy + 0.5f * a.ry_) * r2y) - erff((ddy - 0.5f * a.ry_) * r2y);
+        acc += 0.25f * ex * ey * w;
+    }
+    cpb[(size_t(l) * a.ny + cy) * a.nx + cx] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_cpb_to_rays(CpbArgs a, const float *__restrict__ cpb, float *__restrict__ rays,
+                                                     int rnx, int rny, float rcx, float rcy, float rrx, float rry) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (r >= rnx * rny) return;
+    const int x = r % rnx, y = r / rnx;
+    const float x0 = rcx + float(x) * rrx, x1 = x0 + rrx, y0 = rcy + float(y) * rry, y1 = y0 + rry;
+    int cx0 = int(floorf((x0 - a.cx) / a.rx_)), cx1 = int(ceilf((x1 - a.cx) / a.rx_)) - 1;
+    int cy0 = int(floorf((y0 - a.cy) / a.ry_)), cy1 = int(ceilf((y1 - a.cy) / a.ry_)) - 1;
+    cx0 = max(cx0, 0); cy0 = max(cy0, 0);
+    cx1 = min(cx1, a.nx - 1); cy1 = min(cy1, a.ny - 1);
+    const float area = a.rx_ * a.ry_;
+    float acc = 0.0f;
+    if (area > 0.0f)
+        for (int cy = cy0; cy <= cy1; ++cy) {
+            const float c0 = a.cy + float(cy) * a.ry_, c1 = c0 + a.ry_;
+            const float oy = fmaxf(0.0f, fminf(y1, c1) - fmaxf(y0, c0));
+            if (oy <= 0.0f) continue;
+            for (int cx = cx0; cx <= cx1; ++cx) {
+                const float d0 = a.cx + float(cx) * a.rx_, d1 = d0 + a.rx_;
+                const float ox = fmaxf(0.0f, fminf(x1, d1) - fmaxf(x0, d0));
+                if (ox <= 0.0f) continue;
+                acc += cpb[(size_t(l) * a.ny + cy) * a.nx + cx] * (ox * oy) / area;
+            }
+        }
+    rays[(size_t(l) * rny + y) * rnx + x] = acc;
+}
+
+// ---------------------------------------------------------------------------------------
+// IDD and sigma
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const LayerHost *__restrict__ lh,
+                                              LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                                              const float *__restrict__ density, const float *__restrict__ wepl,
+                                              const float *__restrict__ rayW, const int *__restrict__ fi,
+                                              const int *__restrict__ fo, float2 *__restrict__ iddSigma,
+                                              int *__restrict__ firstPassive, FillConsts fc) {
+    extern __shared__ float s_rrl[];
+    for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
+    __syncthreads();
+    const int nray = g.rx * g.ry;
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    // every lane stays to the wave reduction at the end; lanes past the last ray add INT_MIN
+    int passive = INT_MIN;
+    if (ray < nray) {
+    const LayerHost L = lh[l];
+    const unsigned first = unsigned(bs->beamFirstInside);
+    const unsigned pAfter = unsigned(ls[l].afterLast);
+    const unsigned firstIn = unsigned(fi[ray]);
+    unsigned afterLast = unsigned(min(fo[ray], int(pAfter)));
+    const float w = rayW[size_t(l) * nray + ray];
+    bool live = true;
+    if (w < 1e-6f || afterLast < first) { live = false; afterLast = 0; }
+    float cumulSp = 0.f, cumulSpOld = 0.f, cumulDose = 0.f, cumulDoseOld = 0.f;
+    float incScat = 0.f, incincScat = 0.f;
+    float incDiv = L.airLin + (2.0f * float(first) - 1.0f) * L.airQuad;
+    float sigmaSq = -incDiv;
+    const float stepLength = fabsf(g.fd[2]);
+    const int x = ray % g.rx, y = ray / g.rx;
+    const size_t plane = size_t(nray);
+    float2 *out = iddSigma + (size_t(l) * g.K * g.sy + y) * g.sx + x;
+    const size_t outStep = size_t(g.sx) * g.sy;
+    for (unsigned k = first; k < pAfter; ++k) {
+        float res = 0.0f, rs = PBS_INF;
+        const size_t idx = size_t(k) * plane + ray;
+        if (live) {
+            cumulSp = wepl[idx];
+            cumulDose = lut2d_clamp(lut.cidd, lut.nS, lut.nE, cumulSp * L.scaleFact, L.energyIdx);
+            const float rho = density[idx];
+            if (cumulSp < L.peakDepth) {
+                const float resE = 8.639415f * powf(L.peakDepth - 0.5f * (cumulSp + cumulSpOld), 0.5649718f);
+                const float betaP = resE + 938.3f - 938.3f * 938.3f / (resE + 938.3f);
+                const float rRl = rho * lut1d_clamp(s_rrl, lut.nRrl, rho * lut.rrlScale);
+                const float thetaSq = fc.eRefSq / (betaP * betaP) * stepLength * rRl;
+                sigmaSq += incScat + incDiv;
+                incincScat += 2.0f * thetaSq * stepLength * stepLength;
+                incScat += incincScat;
+                incDiv += 2.0f * L.airQuad;
+            } else if (!fc.suppressDistalDip) {
+                sigmaSq -= 1.5f * (incScat + incDiv) * rho;
+            }
+            const float zk = g.fc[2] + float(k) * g.fd[2];
+            const float vwx = g.fd[0] * (1.0f - zk / g.dist[0]);
+            const float vwy = g.fd[1] * (1.0f - zk / g.dist[1]);
+            rs = 0.5f * (vwx + vwy) / (1.41421356f * (sqrtf(sigmaSq) + fc.sigmaDelta));
+            const float fk = float(k);
+            const float mass = rho * (g.volConst + fk * g.volLin + fk * fk * g.volSq);
+            if (mass > 1e-2f) res = w * (cumulDose - cumulDoseOld) / mass;
+            cumulSpOld = cumulSp;
+            cumulDoseOld = cumulDose;
+            if (cumulSp > L.rangeStop || k == afterLast) { live = false; afterLast = k; }
+        }
+        if (!live || int(k) < int(firstIn) - 1) { res = 0.0f; rs = PBS_INF; }
+        out[size_t(k) * outStep] = make_float2(res, rs);
+    }
+    firstPassive[size_t(l) * nray + ray] = int(afterLast);
+    passive = int(afterLast);
+    }
+    // layer passive step = max over rays (wrapper:4950-4979): one atomic per wave
+    const int m = wave_max_i(passive);
+    if ((threadIdx.x & 63) == 0) atomicMax(&ls[l].passiveRaw, m);
+}
+
+// Layer passive steps, superposed step ranges and the transfer box (one block).
+__global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
+                                                        BeamScalars *__restrict__ bs) {
+#pragma clang fp contract(off)
+    __shared__ int s_max[2][4];
+    const int inside = bs->beamFirstInside;
+    int mp = INT_MIN, mk = INT_MIN, act = 0;
+    for (int l = threadIdx.x; l < g.numLayers; l += blockDim.x) {
+        int pas = ls[l].passiveRaw;  // wrapper:4984-4996
+        if (pas < inside + 1) pas = inside + 1;
+        if (pas < 0) pas = inside + 1;
+        if (pas > ls[l].afterLast) pas = ls[l].afterLast;
+        if (pas > g.K) pas = g.K;
+        ls[l].passive = pas;
+        // The reference samples its BEV volume at (step - beamFirstInside) while the volume
+        // holds absolute steps (wrapper:5308, 5379-5381); steps past passive - inside are
+        // never transferred, so they are not superposed here.
+        const int kEnd = pas > inside ? pas - inside : inside;
+        ls[l].kEnd = kEnd;
+        if (pas > inside) { mp = max(mp, pas); mk = max(mk, kEnd); ++act; }
+    }
+    mp = wave_max_i(mp);
+    mk = wave_max_i(mk);
+    int a2 = act;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+    if ((threadIdx.x & 63) == 0) { s_max[0][threadIdx.x >> 6] = mp; s_max[1][threadIdx.x >> 6] = mk; }
+    __shared__ int s_act[4];
+    if ((threadIdx.x & 63) == 0) s_act[threadIdx.x >> 6] = a2;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    mp = max(max(s_max[0][0], s_max[0][1]), max(s_max[0][2], s_max[0][3]));
+    mk = max(max(s_max[1][0], s_max[1][1]), max(s_max[1][2], s_max[1][3]));
+    bs->maxPassive = mp;
+    bs->kEndMax = mk;
+    bs->activeLayers = s_act[0] + s_act[1] + s_act[2] + s_act[3];
+    if (mp <= inside) {
+        for (int q = 0; q < 3; ++q) { bs->boxStart[q] = 0; bs->boxEnd[q] = 0; bs->boxMax[q] = -1; }
+        return;
+    }
+    // transfer box (wrapper:5343-5395): 8 fan corners -> dose index, float arithmetic
+    const float xv[2] = {-32.0f, float(g.sx + 32 - 1)};
+    const float yv[2] = {-32.0f, float(g.sy + 32 - 1)};
+    const float zv[2] = {float(inside), float(mp - 1)};
+    float mx[3] = {-1.f, -1.f, -1.f}, mn[3] = {100000.f, 100000.f, 100000.f};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                float f0 = xv[c] * g.fd[0] + g.fc[0], f1 = yv[b] * g.fd[1] + g.fc[1];
+                const float f2 = zv[a] * g.fd[2] + g.fc[2];
+                f0 *= (1.0f - f2 / g.dist[0]);
+                f1 *= (1.0f - f2 / g.dist[1]);
+                for (int r = 0; r < 3; ++r) {
+                    const float q = (g.g2dose[3 * r] * f0 + g.g2dose[3 * r + 1] * f1 + g.g2dose[3 * r + 2] * f2) + g.g2doseOff[r];
+                    mx[r] = fmaxf(mx[r], q);
+                    mn[r] = fminf(mn[r], q);
+                }
+            }
+    int st[3], mi[3];
+    st[0] = max((int(floorf(mn[0])) / 32) * 32, 0);
+    st[1] = max(int(floorf(mn[1])), 0);
+    st[2] = max(int(floorf(mn[2])), 0);
+    for (int r = 0; r < 3; ++r) mi[r] = min(int(ceilf(mx[r])), g.doseDims[r] - 1);
+    for (int r = 0; r < 3; ++r) { bs->boxStart[r] = st[r]; bs->boxMax[r] = mi[r]; }
+    // the launch covers whole 32x8 thread blocks from the start; threads stop at the grid edge
+    bs->boxEnd[0] = min(st[0] + ((mi[0] - st[0] + 1 + 31) / 32) * 32, g.doseDims[0]);
+    bs->boxEnd[1] = min(st[1] + ((mi[1] - st[1] + 1 + 7) / 8) * 8, g.doseDims[1]);
+    bs->boxEnd[2] = mi[2] + 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// tile radii, bins and work items
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int tile_radius(float m) {
+    if (m != m) return 0;
+    const float q = 3.0f / (1.41421356237f * m) + 0.5f;  // KS_SIGMA_CUTOFF / (sqrt(2) min) + 1/2
+    const int r = int(q);
+    return r < kMaxRadDev + 1 ? r : kMaxRadDev + 1;
+}
+
+// one wave per (step, tile); loops over layers. code = rad | live << 8, -1 = not computed
+__global__ __launch_bounds__(256) void k_tiles(FanGeom g, const float2 *__restrict__ iddSigma,
+                                               const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                                               int *__restrict__ tileRad, int *__restrict__ hist) {
+    const int ntx = g.sx / 32, nty = g.sy / 8, nt = ntx * nty;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    if (t >= nt) return;
+    const int tx = t % ntx, ty = t / ntx;
+    const int inside = bs->beamFirstInside;
+    for (int l = 0; l < g.numLayers; ++l) {
+        const size_t code_idx = (size_t(l) * g.K + k) * nt + t;
+        const int pas = ls[l].passive;
+        if (k < inside || k >= pas) {
+            if (lane == 0) tileRad[code_idx] = -1;
+            continue;
+        }
+        float m = __builtin_nanf("");
+        bool any = false;
+        const float2 *base = iddSigma + (size_t(l) * g.K + k) * g.sx * g.sy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int p = lane + 64 * q;
+            const int x = tx * 32 + (p & 31), y = ty * 8 + (p >> 5);
+            float2 v = make_float2(0.0f, PBS_INF);
+            if (x < g.rx && y < g.ry) v = base[size_t(y) * g.sx + x];
+            m = fminf(m, v.y);
+            any |= v.x > 0.0f;
+        }
+        m = wave_min(m);
+        const bool live = __any(any);
+        if (lane == 0) {
+            const int r = tile_radius(m);
+            atomicAdd(&hist[l * (kMaxRadDev + 2) + r], 1);
+            tileRad[code_idx] = r | (live ? 256 : 0);
+        }
+    }
+}
+
+__global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restrict__ effRad) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= numLayers) return;
+    int h[kMaxRadDev + 2];
+    for (int i = 0; i < kMaxRadDev + 2; ++i) h[i] = hist[l * (kMaxRadDev + 2) + i];
+    h[kMaxRadDev] += h[kMaxRadDev + 1];  // primary overflow clamps into the largest radius
+    h[kMaxRadDev + 1] = 0;
+    int layerMax = 0;
+    for (int i = 0; i < kMaxRadDev + 2; ++i) if (h[i] > 0) layerMax = i;
+    int eff[kMaxRadDev + 2];
+    for (int i = 0; i < kMaxRadDev + 2; ++i) eff[i] = i;
+    eff[kMaxRadDev + 1] = kMaxRadDev;
+    int batched = 0, rec = layerMax;
+    for (int r = layerMax; r > 0; --r) {
+        batched += h[r];
+        eff[r] = rec;
+        if (batched >= kMinTilesDev) { rec = r - 1; batched = 0; }
+    }
+    for (int i = 0; i < kMaxRadDev + 2; ++i) effRad[l * (kMaxRadDev + 2) + i] = eff[i];
+}
+
+__device__ __forceinline__ int radius_class(int R) {
+    // launch classes: accumulator layouts of radius 2, 4, 6, 8, 12, 16, 24, 32
+    return R <= 2 ? 0 : R <= 4 ? 1 : R <= 6 ? 2 : R <= 8 ? 3 : R <= 12 ? 4 : R <= 16 ? 5 : R <= 24 ? 6 : 7;
+}
+__device__ __forceinline__ int class_radius(int c) {
+    constexpr int rm[8] = {2, 4, 6, 8, 12, 16, 24, 32};
+    return rm[c];
+}
+
+// one thread per (step, tile)
+__global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict__ tileRad,
+                                               const int *__restrict__ effRad, const LayerScalars *__restrict__ ls,
+                                               int *__restrict__ itemR, int *__restrict__ classCount,
+                                               int *__restrict__ classList, int maxItems) {
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (t >= nt) return;
+    int R = -1;
+    for (int l = 0; l < g.numLayers; ++l) {
+        const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
+        if (code < 0 || !(code & 256) || k >= ls[l].kEnd) continue;
+        R = max(R, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
+    }
+    const int slot = k * nt + t;
+    itemR[slot] = R;
+    if (R >= 0) {
+        const int c = radius_class(R);
+        const int pos = atomicAdd(&classCount[c], 1);
+        classList[size_t(c) * maxItems + pos] = slot;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// superposition
+// ---------------------------------------------------------------------------------------
+// One work-group (4 waves) per (step, tile) item. Per live layer, each thread computes the
+// erf-difference vector of its pixel into LDS; wave w then scatters input rows 2w, 2w+1:
+// lane = output column (64 per chunk), registers = output rows. The pixel's vector is
+// read with a wave-uniform LDS address (broadcast) and the column factor with a per-lane
+// address, so every FMA works on registers. The four wave blocks are combined in a fixed
+// order and the item's block is written to its own slot: no atomics, bitwise repeatable.
+template <int RM>
+__global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__restrict__ iddSigma,
+                                                   const int *__restrict__ tileRad, const int *__restrict__ effRad,
+                                                   const LayerScalars *__restrict__ ls,
+                                                   const int *__restrict__ classCount,
+                                                   const int *__restrict__ classList, int cls, int maxItems,
+                                                   float *__restrict__ slots) {
+    constexpr int NC = (32 + 2 * RM + 63) / 64;
+    constexpr int EW = ((RM + 1) + 3) & ~3;
+    constexpr int BW = 32 + 2 * RM, BH = 8 + 2 * RM;
+    constexpr int NR = 2 + 2 * RM;
+    constexpr int LDS_E = 256 * EW, LDS_B = BH * BW;
+    __shared__ float s_buf[LDS_E > LDS_B ? LDS_E : LDS_B];
+    __shared__ float s_d[256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
+    const int count = classCount[cls];
+    const int *list = classList + size_t(cls) * maxItems;
+    for (int it = blockIdx.x; it < count; it += gridDim.x) {
+        const int slot = list[it];
+        const int k = slot / nt, t = slot % nt;
+        const int tx = t % ntx, ty = t / ntx;
+        float acc[NC][NR];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[c][q] = 0.0f;
+        for (int l = 0; l < g.numLayers; ++l) {
+            const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
+            if (code < 0 || !(code & 256) || k >= ls[l].kEnd) continue;
+            const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
+            // phase 1: erf differences of this thread's pixel
+            {
+                const int px = tid & 31, py = tid >> 5;
+                const int gx = tx * 32 + px, gy = ty * 8 + py;
+                float d = 0.0f, r = PBS_INF;
+                if (gx < g.rx && gy < g.ry) {
+                    const float2 v = iddSigma[((size_t(l) * g.K + k) * g.sy + gy) * g.sx + gx];
+                    d = v.x;
+                    r = v.y;
+                }
+                float *E = s_buf + tid * EW;
+                if (d != 0.0f) {
+                    float en = erff(r * 0.5f), eo = -en;
+                    for (int i = 0; i <= R; ++i) {
+                        E[i] = 0.5f * (en - eo);
+                        eo = en;
+                        en = erff(r * (float(i) + 1.5f));
+                    }
+                }
+                s_d[tid] = d;
+            }
+            __syncthreads();
+            // phase 2: scatter the wave's two input rows into register accumulators
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                const int prow = 2 * w + half;
+                for (int px = 0; px < 32; ++px) {
+                    const int p = prow * 32 + px;
+                    const float d = s_d[p];
+                    if (d == 0.0f) continue;
+                    const float *Ep = s_buf + p * EW;
+                    float a[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        const int X = lane + 64 * c - RM;
+                        const int adx = abs(X - px);
+                        a[c] = adx <= R ? d * Ep[min(adx, RM)] : 0.0f;
+                    }
+#pragma unroll
+                    for (int jj = 0; jj <= 2 * RM; ++jj) {
+                        const int aj = jj < RM ? RM - jj : jj - RM;
+                        if (aj > R) continue;
+                        const float ej = Ep[aj];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) acc[c][half + jj] += a[c] * ej;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // combine the four wave blocks in a fixed order, then write the item's slot
+        float *B = s_buf;
+        for (int i = tid; i < BH * BW; i += 256) B[i] = 0.0f;
+        __syncthreads();
+        for (int ww = 0; ww < 4; ++ww) {
+            if (w == ww) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    const int col = lane + 64 * c;
+                    if (col < BW) {
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) B[(2 * w + q) * BW + col] += acc[c][q];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        float *out = slots + size_t(slot) * kSlotFloats;
+        for (int i = tid; i < BH * BW; i += 256) out[(i / BW) * kSlotPitch + (i % BW)] = B[i];
+        __syncthreads();
+    }
+}
+
+// BEV dose [bevZ][sy+64][sx+64] = sum of the item blocks covering each voxel, in (ty, tx) order.
+__global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__restrict__ bs,
+                                               const int *__restrict__ itemR, const float *__restrict__ slots,
+                                               float *__restrict__ bev, int bevZ) {
+    const int bx = g.sx + 64, by = g.sy + 64;
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (X >= bx || Y >= by || k >= bevZ) return;
+    const int ntx = g.sx / 32, nty = g.sy / 8, nt = ntx * nty;
+    float sum = 0.0f;
+    if (k >= bs->beamFirstInside && k < bs->kEndMax) {
+        const int xr = X - 32, yr = Y - 32;
+        // tiles whose block can reach (xr, yr): |offset| <= 32 beyond the tile
+        const int tx0 = max(0, (xr - 63 + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + 32 + 64 * 32) / 32 - 64);
+        const int ty0 = max(0, (yr - 39 + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + 32 + 8 * 64) / 8 - 64);
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int slot = k * nt + ty * ntx + tx;
+                const int R = itemR[slot];
+                if (R < 0) continue;
+                const int RM = class_radius(radius_class(R));
+                const int col = xr - 32 * tx + RM, row = yr - 8 * ty + RM;
+                if (col >= 0 && col < 32 + 2 * RM && row >= 0 && row < 8 + 2 * RM)
+                    sum += slots[size_t(slot) * kSlotFloats + row * kSlotPitch + col];
+            }
+    }
+    bev[(size_t(k) * by + Y) * bx + X] = sum;
+}
+
+// ---------------------------------------------------------------------------------------
+// BEV -> dose
+// ---------------------------------------------------------------------------------------
+// Thread per dose (x, y) column inside the transfer box; loops over z (one owner per voxel).
+// The dose-index -> fan-index map is the reference's factored form
+// (transfer_param_struct_div3.cuh:34-44, transfer_param_struct_div3_impl.cu:14-34).
+__global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
+                                                  const float *__restrict__ bev, int bevZ, float *__restrict__ dose) {
+    const int x = bs->boxStart[0] + int(blockIdx.x * 32 + (threadIdx.x & 31));
+    const int y = bs->boxStart[1] + int(blockIdx.y * 8 + (threadIdx.x >> 5));
+    if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1]) return;
+    const int inside = bs->beamFirstInside;
+    const float goffZ = ta.globalOffset[2] - float(inside);
+    const float bx = float(g.sx + 64), by = float(g.sy + 64);
+    float sx0 = float(x) * ta.coefI[0] + float(y) * ta.coefJ[0] + ta.coefOffset[0];
+    float sy0 = float(x) * ta.coefI[1] + float(y) * ta.coefJ[1] + ta.coefOffset[1];
+    float sz0 = float(x) * ta.coefI[2] + float(y) * ta.coefJ[2] + ta.coefOffset[2];
+    const size_t sliceStride = size_t(g.doseDims[0]) * g.doseDims[1];
+    float *res = dose + size_t(bs->boxStart[2]) * sliceStride + size_t(y) * g.doseDims[0] + x;
+    for (int z = bs->boxStart[2]; z < bs->boxEnd[2]; ++z, res += sliceStride) {
+        const float fz = float(z);
+        float px = sx0 + fz * ta.inc[0], py = sy0 + fz * ta.inc[1];
+        const float pz = sz0 + fz * ta.inc[2];
+        px *= 1.0f + pz / (ta.normDist[0] - pz);
+        py *= 1.0f + pz / (ta.normDist[1] - pz);
+        const float fxI = px + ta.globalOffset[0], fyI = py + ta.globalOffset[1], fzI = pz + goffZ;
+        // prim_transf_kernel.cu:28-40: texel coordinate = fan index + 1/2; negatives skipped
+        if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) continue;
+        if (fxI > bx || fyI > by || fzI > float(bevZ)) continue;  // wholly outside: border value 0
+        const float v = trilinear_border(bev, g.sx + 64, g.sy + 64, bevZ, fxI, fyI, fzI);
+        if (v > 0.0f) *res += v;
+    }
+}
+
+}  // namespace pbs
+
+// ---------------------------------------------------------------------------------------
+// launch wrappers (host)
+// ---------------------------------------------------------------------------------------
+namespace pbs {
+
+hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
+                        int *fo, hipStream_t s) {
+    const int n = g.rx * g.ry;
+    const size_t shm = sizeof(float) * size_t(lut.nDensity + lut.nSp);
+    hipLaunchKernelGGL(k_trace, dim3((n + 255) / 256), dim3(256), shm, s, g, ct, lut, density, wepl, fi, fo);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,
+                               BeamScalars *bs, hipStream_t s) {
+    hipLaunchKernelGGL(k_trace_reduce, dim3(g.K + 1), dim3(256), 0, s, g, wepl, fi, fo, weplMin, bs);
+    return hipGetLastError();
+}
+
+hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
+                             LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
+                             int *classCount, int *hist, hipStream_t s) {
+    hipLaunchKernelGGL(k_beam_setup, dim3(1), dim3(256), 0, s, g, weplMin, lh, bs, ls, subspots, maxSub, subEntry,
+                       nucFactor, classCount, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv(const ConvArgs &a, const float *spot, float *interm, float *rayW, const LayerScalars *ls,
+                       const BeamScalars *bs, const float *subspots, const float *subEntry, int numLayers,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_conv_x, dim3((a.rx + 63) / 64, a.spotY, numLayers * a.nSub), dim3(64), 0, s, a, spot, interm,
+                       ls, bs, subspots, subEntry);
+    hipLaunchKernelGGL(k_conv_y, dim3((a.rx + 63) / 64, a.ry, numLayers), dim3(64), 0, s, a, interm, rayW, ls, bs,
+                       subspots, subEntry);
+    return hipGetLastError();
+}
+
+hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *rays, int rnx, int rny, float rcx,
+                      float rcy, float rrx, float rry, int numLayers, hipStream_t s) {
+    hipLaunchKernelGGL(k_subspot_cpb, dim3((a.nx * a.ny + 255) / 256, numLayers), dim3(256), 0, s, a, sub, cpb);
+    if (rays)
+        hipLaunchKernelGGL(k_cpb_to_rays, dim3((rnx * rny + 255) / 256, numLayers), dim3(256), 0, s, a, cpb, rays, rnx,
+                           rny, rcx, rcy, rrx, rry);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
+                       const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
+                       const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, hipStream_t s) {
+    const int n = g.rx * g.ry;
+    hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256, g.numLayers), dim3(256), sizeof(float) * lut.nRrl, s, g, lut, lh,
+                       ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, BeamScalars *bs,
+                                     int *tileRad, int *hist, int *effRad, int *itemR, int *classCount,
+                                     int *classList, int maxItems, hipStream_t s) {
+    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, bs);
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
+    hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad);
+    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, itemR,
+                       classCount, classList, maxItems);
+    return hipGetLastError();
+}
+
+template <int RM>
+static void launch_class(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
+                         const LayerScalars *ls, const int *classCount, const int *classList, int cls, int maxItems,
+                         float *slots, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, classCount,
+                       classList, cls, maxItems, slots);
+}
+
+hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
+                            const LayerScalars *ls, const int *classCount, const int *classList, int maxItems,
+                            float *slots, int gridCap, hipStream_t s) {
+    const int grid = maxItems < gridCap ? maxItems : gridCap;
+    launch_class<2>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 0, maxItems, slots, grid, s);
+    launch_class<4>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 1, maxItems, slots, grid, s);
+    launch_class<6>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 2, maxItems, slots, grid, s);
+    launch_class<8>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 3, maxItems, slots, grid, s);
+    launch_class<12>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 4, maxItems, slots, grid, s);
+    launch_class<16>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 5, maxItems, slots, grid, s);
+    launch_class<24>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 6, maxItems, slots, grid, s);
+    launch_class<32>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 7, maxItems, slots, grid, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *slots, float *bev,
+                        int bevZ, hipStream_t s) {
+    const int bx = g.sx + 64, by = g.sy + 64;
+    hipLaunchKernelGGL(k_merge, dim3((bx + 63) / 64, (by + 3) / 4, bevZ), dim3(256), 0, s, g, bs, itemR, slots, bev,
+                       bevZ);
+    return hipGetLastError();
+}
+
+hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const float *bev, int bevZ,
+                           float *dose, int gridX, int gridY, hipStream_t s) {
+    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY), dim3(256), 0, s, g, ta, bs, bev, bevZ, dose);
+    return hipGetLastError();
+}
+
+}  // namespace pbs

@@ -1,0 +1,424 @@
+// Host-side beam planning (see plan.h). Float operation order follows the reference host
+// code so that every integer it derives matches.
+#include "plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pbs {
+
+F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+static F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static F3 scale(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+static float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static F3 normalize_safe(F3 v) {  // wrapper:2886-2890
+    const float len = std::sqrt(dot(v, v));
+    if (len > 1e-8f) { v.x /= len; v.y /= len; v.z /= len; }
+    return v;
+}
+static F3 mul(const M3 &m, F3 a) { return f3(dot(m.r0, a), dot(m.r1, a), dot(m.r2, a)); }
+static M3 transpose(const M3 &m) {
+    return M3{f3(m.r0.x, m.r1.x, m.r2.x), f3(m.r0.y, m.r1.y, m.r2.y), f3(m.r0.z, m.r1.z, m.r2.z)};
+}
+// Matrix product as rows of A dotted with rows of B^T (matrix_3x3.h:66-75).
+static M3 mul(const M3 &a, const M3 &b) {
+    const M3 t = transpose(b);
+    return M3{f3(dot(a.r0, t.r0), dot(a.r0, t.r1), dot(a.r0, t.r2)),
+              f3(dot(a.r1, t.r0), dot(a.r1, t.r1), dot(a.r1, t.r2)),
+              f3(dot(a.r2, t.r0), dot(a.r2, t.r1), dot(a.r2, t.r2))};
+}
+static M3 diag(float a, float b, float c) { return M3{f3(a, 0, 0), f3(0, b, 0), f3(0, 0, c)}; }
+// Adjugate times 1/det, the latter applied as a scalar matrix (matrix_3x3.h:80-93).
+static M3 inverse(const M3 &m) {
+    const F3 a = m.r0, b = m.r1, c = m.r2;
+    const float det = a.x * (b.y * c.z - b.z * c.y) - a.y * (b.x * c.z - b.z * c.x) + a.z * (b.x * c.y - b.y * c.x);
+    const float s = 1.0f / det;
+    const M3 adj{f3(b.y * c.z - b.z * c.y, a.z * c.y - a.y * c.z, a.y * b.z - a.z * b.y),
+                 f3(b.z * c.x - b.x * c.z, a.x * c.z - a.z * c.x, a.z * b.x - a.x * b.z),
+                 f3(b.x * c.y - b.y * c.x, a.y * c.x - a.x * c.y, a.x * b.y - a.y * b.x)};
+    return mul(adj, diag(s, s, s));
+}
+F3 apply(const Aff &t, F3 a) { return add(mul(t.m, a), t.v); }
+Aff inverse(const Aff &t) {
+    Aff r;
+    r.m = inverse(t.m);
+    r.v = mul(r.m, scale(t.v, -1.0f));
+    return r;
+}
+Aff concat(const Aff &first, const Aff &second) {
+    Aff r;
+    r.m = mul(second.m, first.m);
+    r.v = add(mul(second.m, first.v), second.v);
+    return r;
+}
+
+static int last_smaller_or_eq(const float *v, int n, float value) {
+    int upper = n - 1, lower = 0;
+    if (v[n - 1] <= value) return upper;
+    if (v[0] > value) return -1;
+    while (upper - lower > 1) {
+        const int pivot = (upper + lower) / 2;
+        if (v[pivot] <= value) lower = pivot; else upper = pivot;
+    }
+    return lower;
+}
+
+float find_decimal_ordered(const float *v, int n, float value) {
+    if (value >= v[n - 1]) return float(n - 1);
+    if (value < v[0]) return 0.0f;
+    const unsigned f = unsigned(last_smaller_or_eq(v, n, value));
+    const float corr = (value - v[f]) / (v[f + 1] - v[f]);
+    return float(f) + corr;
+}
+
+// vectorInterpolate (vector_interpolate.h:19-30); also the wrapper's inline lerp (4016-4052).
+static float interp_at(const float *v, int n, float idx) {
+    if (idx <= 0.0f) return v[0];
+    if (idx >= float(n - 1)) return v[n - 1];
+    float ip;
+    const float dec = std::modf(idx, &ip);
+    const unsigned f = unsigned(ip);
+    return v[f] + (v[f + 1] - v[f]) * dec;
+}
+
+float layer_energy_index(const pbs_energy_struct &e, float energy) {
+    const int n = e.nEnergies;
+    if (n <= 1) return 0.0f;
+    const bool asc = e.energiesPerU[0] < e.energiesPerU[n - 1];
+    const int lo = asc ? 0 : n - 1, hi = asc ? n - 1 : 0;
+    if (energy <= e.energiesPerU[lo]) return float(lo);
+    if (energy >= e.energiesPerU[hi]) return float(hi);
+    int fl = 0;
+    float e0 = e.energiesPerU[0], e1 = e.energiesPerU[0];
+    for (int i = 0; i < n - 1; ++i) {
+        const float a = e.energiesPerU[i], b = e.energiesPerU[i + 1];
+        const float mn = a < b ? a : b, mx = a < b ? b : a;
+        if (energy >= mn && energy <= mx) { fl = i; e0 = a; e1 = b; break; }
+    }
+    const float den = e1 - e0;
+    float corr = den != 0.0f ? (energy - e0) / den : 0.0f;
+    if (!std::isfinite(corr)) corr = 0.0f;
+    corr = std::min(1.0f, std::max(0.0f, corr));
+    return float(fl) + corr;
+}
+
+int detect_ct_type(const float *ct, size_t n) {
+    // sampleCtStats + autoDetectCtType (wrapper:2307-2335): every (n/1e5)-th voxel
+    const size_t stride = n > 100000 ? n / 100000 : 1;
+    float mn = ct[0], mx = ct[0];
+    for (size_t i = 0; i < n; i += stride) { mn = std::min(mn, ct[i]); mx = std::max(mx, ct[i]); }
+    if (mn < -200.0f) return 0;
+    if (mx <= 20.0f && mn >= 0.0f) return 2;
+    return 1;
+}
+
+void batch_bins(const int *histIn, int *batched, int *effRad) {
+    int h[kMaxRad + 2];
+    std::memcpy(h, histIn, sizeof(h));
+    h[kMaxRad] += h[kMaxRad + 1];  // primary overflow is clamped into the largest radius
+    h[kMaxRad + 1] = 0;
+    int layerMax = 0;
+    for (int i = 0; i < kMaxRad + 2; ++i) if (h[i] > 0) layerMax = i;
+    for (int i = 0; i <= kMaxRad; ++i) batched[i] = 0;
+    for (int i = 0; i < kMaxRad + 2; ++i) effRad[i] = i;
+    effRad[kMaxRad + 1] = kMaxRad;
+    batched[0] = h[0];
+    int rec = layerMax;
+    for (int r = layerMax; r > 0; --r) {
+        batched[rec] += h[r];
+        effRad[r] = rec;
+        if (batched[rec] >= kMinTilesInBatch) rec = r - 1;
+    }
+}
+
+// buildRawSpotLattice, physical-position mode (wrapper:1259-1473).
+static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
+    if (!b.layerSpotCounts || !b.spotPositions || !b.spotWeights || b.numSpots <= 0) return false;
+    int total = 0;
+    for (int l = 0; l < b.numLayers; ++l) total += b.layerSpotCounts[l];
+    if (total <= 0 || total != b.numSpots) return false;
+    std::vector<float> xs(total), ys(total);
+    for (int i = 0; i < total; ++i) { xs[i] = b.spotPositions[2 * i]; ys[i] = b.spotPositions[2 * i + 1]; }
+    auto unique_sorted = [](std::vector<float> v) {
+        std::sort(v.begin(), v.end());
+        std::vector<float> u;
+        for (float x : v) if (u.empty() || !(std::fabs(u.back() - x) <= 1e-4f)) u.push_back(x);
+        return u;
+    };
+    const std::vector<float> ux = unique_sorted(xs), uy = unique_sorted(ys);
+    auto step_of = [](const std::vector<float> &u) {
+        float s = 1.0e20f;
+        for (size_t i = 1; i < u.size(); ++i) {
+            const float d = u[i] - u[i - 1];
+            if (d > 1.0e-4f && d < s) s = d;
+        }
+        return s == 1.0e20f ? 1.0f : s;
+    };
+    const float dx = step_of(ux), dy = step_of(uy), ox = ux.front(), oy = uy.front();
+    const int nx = std::max(1, int(std::lround((ux.back() - ox) / dx)) + 1);
+    const int ny = std::max(1, int(std::lround((uy.back() - oy) / dy)) + 1);
+    std::vector<float> dense(size_t(nx) * ny * b.numLayers, 0.0f);
+    int off = 0;
+    for (int l = 0; l < b.numLayers; ++l) {
+        const int cnt = b.layerSpotCounts[l];
+        if (cnt <= 0) return false;
+        for (int i = 0; i < cnt; ++i) {
+            const int s = off + i;
+            const float x = b.spotPositions[2 * s], y = b.spotPositions[2 * s + 1], w = b.spotWeights[s];
+            if (!std::isfinite(w) || w < 0.0f) return false;
+            if (w == 0.0f) continue;
+            const float fx = dx != 0.0f ? (x - ox) / dx : 0.0f;
+            const float fy = dy != 0.0f ? (y - oy) / dy : 0.0f;
+            const int ix = int(std::lround(fx)), iy = int(std::lround(fy));
+            if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) return false;
+            if (!(std::fabs(ox + ix * dx - x) <= 2e-2f) || !(std::fabs(oy + iy * dy - y) <= 2e-2f)) return false;
+            dense[(size_t(l) * ny + iy) * nx + ix] += w;
+        }
+        off += cnt;
+    }
+    p.spotDims[0] = nx; p.spotDims[1] = ny;
+    p.spotOffset[0] = ox; p.spotOffset[1] = oy;
+    p.spotDelta[0] = dx; p.spotDelta[1] = dy;
+    p.denseWeights.swap(dense);
+    return true;
+}
+
+// alignGridToReferencePhase (wrapper:970-997).
+static bool align_grid(float minC, float maxC, float ref, float step, int multiple, float &origin, int &count) {
+    if (!(step > 0.0f) || !std::isfinite(minC) || !std::isfinite(maxC) || !std::isfinite(ref)) return false;
+    if (maxC < minC) std::swap(minC, maxC);
+    const float eps = 1.0e-4f;
+    const int fs = int(std::ceil((minC - ref) / step - eps));
+    const int ls = int(std::floor((maxC - ref) / step + eps));
+    if (ls < fs) return false;
+    origin = ref + float(fs) * step;
+    count = std::max(1, ls - fs + 1);
+    if (multiple > 1) count = ((count + multiple - 1) / multiple) * multiple;
+    return true;
+}
+
+// estimateVirtualSourceDistancesFromSpots, physical positions (wrapper:1847-1921).
+static bool infer_source_dist(const pbs_beam_settings &b, F3 bmX, F3 bmY, F3 bmZ, float sad, float &sx, float &sy) {
+    if (!(sad > 1.0e-6f) || !b.spotBeamDirections || !b.spotPositions || b.numSpots <= 0) return false;
+    std::vector<float> ax, ay;
+    for (int i = 0; i < b.numSpots; ++i) {
+        const float px = b.spotPositions[2 * i], py = b.spotPositions[2 * i + 1];
+        F3 d = f3(b.spotBeamDirections[3 * i], b.spotBeamDirections[3 * i + 1], b.spotBeamDirections[3 * i + 2]);
+        const float len = std::sqrt(dot(d, d));
+        if (!(len > 1.0e-6f)) continue;
+        d.x /= len; d.y /= len; d.z /= len;
+        const float a = dot(d, bmZ);
+        if (!(std::fabs(a) > 1.0e-6f)) continue;
+        const float bx = dot(d, bmX), by = dot(d, bmY);
+        if (std::fabs(px) > 1.0e-3f && std::fabs(bx) > 1.0e-6f) {
+            const float c = std::fabs(px * a / bx);
+            if (c > 1.0f && std::isfinite(c)) ax.push_back(c);
+        }
+        if (std::fabs(py) > 1.0e-3f && std::fabs(by) > 1.0e-6f) {
+            const float c = std::fabs(py * a / by);
+            if (c > 1.0f && std::isfinite(c)) ay.push_back(c);
+        }
+    }
+    std::sort(ax.begin(), ax.end());
+    std::sort(ay.begin(), ay.end());
+    sx = ax.empty() ? 0.0f : ax[ax.size() / 2];
+    sy = ay.empty() ? 0.0f : ay[ay.size() / 2];
+    return sx > 0.0f || sy > 0.0f;
+}
+
+static float sigma_to_mm(float lenToMm, float s) { return (lenToMm > 1.0f && s > 0.0f && s < 1.0f) ? s * 10.0f : s; }
+
+bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
+               const float doseRes[3], const float doseCorner[3], const pbs_beam_settings &b,
+               const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov) {
+    p = BeamPlan{};
+    if (b.numLayers <= 0 || b.numLayers > kMaxLayers) { p.error = "numLayers out of range"; return false; }
+    p.numLayers = b.numLayers;
+    p.subspotConv = b.useSubspotConvolution != 0;
+    // units (wrapper:2728-2802)
+    p.lenToMm = (ctRes[2] > 0.0f && ctRes[2] < 0.3f) ? 10.0f : 1.0f;
+    p.energyDepthToMm = p.lenToMm;
+    float peakMax = 0.0f;
+    for (int i = 0; i < en.nEnergies; ++i) peakMax = std::max(peakMax, en.peakDepths[i]);
+    if (p.lenToMm == 10.0f && peakMax > 200.0f) p.energyDepthToMm = 1.0f;
+    const float lenToMm = p.lenToMm;
+
+    // beam basis (wrapper:2893-2936)
+    const F3 beamDir = normalize_safe(f3(b.beamDirection[0], b.beamDirection[1], b.beamDirection[2]));
+    const F3 bmXW = normalize_safe(f3(b.bmXDirection[0], b.bmXDirection[1], b.bmXDirection[2]));
+    const F3 bmYW = normalize_safe(f3(b.bmYDirection[0], b.bmYDirection[1], b.bmYDirection[2]));
+    const F3 srcW = f3(b.sourcePosition[0], b.sourcePosition[1], b.sourcePosition[2]);
+    const float sad = b.sad > 0.0f ? b.sad : b.sourceDist[0];
+    p.sadInput = sad;
+    const F3 basisCross = normalize_safe(cross(bmXW, bmYW));
+    const bool haveCross = dot(basisCross, basisCross) > 1.0e-8f;
+    const bool haveDir = dot(beamDir, beamDir) > 1.0e-8f;
+    F3 bmZ = haveCross ? basisCross : beamDir;
+    if (haveCross && haveDir && dot(bmZ, beamDir) < 0.0f) bmZ = scale(bmZ, -1.0f);
+    if (!(dot(bmZ, bmZ) > 1.0e-8f)) bmZ = f3(0, 0, 1);
+    F3 bmX = normalize_safe(sub(bmXW, scale(bmZ, dot(bmXW, bmZ))));
+    if (!(dot(bmX, bmX) > 1.0e-8f)) bmX = normalize_safe(cross(bmYW, bmZ));
+    if (!(dot(bmX, bmX) > 1.0e-8f)) bmX = f3(1, 0, 0);
+    F3 bmY = normalize_safe(cross(bmZ, bmX));
+    if (dot(bmY, bmYW) < 0.0f) bmY = scale(bmY, -1.0f);
+    bmX = normalize_safe(cross(bmY, bmZ));
+    const F3 gX = bmX, gY = scale(bmY, -1.0f), gZ = scale(bmZ, -1.0f);
+    const F3 iso = add(scale(srcW, lenToMm), scale(bmZ, sad * lenToMm));
+    p.bmX = bmX; p.bmY = bmY; p.bmZ = bmZ; p.iso = iso;
+    p.g2w.m = M3{f3(gX.x, gY.x, gZ.x), f3(gX.y, gY.y, gZ.y), f3(gX.z, gY.z, gZ.z)};
+    p.g2w.v = iso;
+    const Aff w2g = inverse(p.g2w);
+
+    // spot lattice and CPB / ray grid (wrapper:2951-3212)
+    const bool lattice = ov ? false : build_lattice(b, p);
+    const F3 doseRes_mm = f3(doseRes[0] * lenToMm, doseRes[1] * lenToMm, doseRes[2] * lenToMm);
+    const F3 doseCorner_mm = f3(doseCorner[0] * lenToMm, doseCorner[1] * lenToMm, doseCorner[2] * lenToMm);
+    if (lattice && b.raySpacing[0] > 0.0f && b.raySpacing[1] > 0.0f) {
+        float msx = 0.f, msy = 0.f;
+        for (int l = 0; l < b.numLayers; ++l) {
+            msx = std::max(msx, b.spotSigmas[2 * l]);
+            msy = std::max(msy, b.spotSigmas[2 * l + 1]);
+        }
+        const float mX = 3.0f * msx + 0.5f * b.raySpacing[0], mY = 3.0f * msy + 0.5f * b.raySpacing[1];
+        float ox, oy;
+        int nx, ny;
+        if (!align_grid(p.spotOffset[0] - mX, p.spotOffset[0] + float(p.spotDims[0] - 1) * p.spotDelta[0] + mX,
+                        p.spotOffset[0], b.raySpacing[0], kTileX, ox, nx) ||
+            !align_grid(p.spotOffset[1] - mY, p.spotOffset[1] + float(p.spotDims[1] - 1) * p.spotDelta[1] + mY,
+                        p.spotOffset[1], b.raySpacing[1], kTileY, oy, ny)) {
+            p.error = "Failed to align CPB grid to raw spot lattice phase";
+            return false;
+        }
+        p.cpbCorner[0] = ox; p.cpbCorner[1] = oy; p.cpbCorner[2] = 0.0f;
+        p.cpbRes[0] = b.raySpacing[0]; p.cpbRes[1] = b.raySpacing[1]; p.cpbRes[2] = doseRes[2];
+        p.rayDims[0] = nx; p.rayDims[1] = ny;
+        p.legacyPath = false;
+    } else if (ov) {
+        p.denseWeights.clear();
+        p.cpbCorner[0] = ov->corner[0]; p.cpbCorner[1] = ov->corner[1]; p.cpbCorner[2] = 0.0f;
+        p.cpbRes[0] = ov->res[0]; p.cpbRes[1] = ov->res[1]; p.cpbRes[2] = doseRes[2];
+        p.rayDims[0] = std::max(1, ov->dims[0]);
+        p.rayDims[1] = std::max(1, ov->dims[1]);
+        p.legacyPath = true;
+    } else {
+        if (!(b.raySpacing[0] > 0.0f) || !(b.raySpacing[1] > 0.0f)) {
+            p.error = "Legacy CPB path requires explicit beam.raySpacing";
+            return false;
+        }
+        p.denseWeights.clear();
+        float minGX = 1e30f, maxGX = -1e30f, minGY = 1e30f, maxGY = -1e30f;
+        const F3 sz = f3(float(std::max(doseDims[0] - 1, 0)) * doseRes_mm.x, float(std::max(doseDims[1] - 1, 0)) * doseRes_mm.y,
+                         float(std::max(doseDims[2] - 1, 0)) * doseRes_mm.z);
+        for (int cx = 0; cx <= 1; ++cx)
+            for (int cy = 0; cy <= 1; ++cy)
+                for (int cz = 0; cz <= 1; ++cz) {
+                    const F3 g = apply(w2g, add(doseCorner_mm, f3(float(cx) * sz.x, float(cy) * sz.y, float(cz) * sz.z)));
+                    minGX = std::fmin(minGX, g.x); maxGX = std::fmax(maxGX, g.x);
+                    minGY = std::fmin(minGY, g.y); maxGY = std::fmax(maxGY, g.y);
+                }
+        for (int l = 0; l < b.numLayers; ++l)
+            for (int s = 0; s < b.maxSubspotsPerLayer; ++s) {
+                const float *q = b.subspotData + (size_t(l) * b.maxSubspotsPerLayer + s) * 5;
+                if (q[2] > 0.0f) {
+                    const float sx = q[0] / lenToMm, sy = q[1] / lenToMm;
+                    minGX = std::fmin(minGX, sx); maxGX = std::fmax(maxGX, sx);
+                    minGY = std::fmin(minGY, sy); maxGY = std::fmax(maxGY, sy);
+                }
+            }
+        const float covX = (maxGX - minGX) / lenToMm, covY = (maxGY - minGY) / lenToMm;
+        if (!(covX > 0.0f) || !(covY > 0.0f)) { p.error = "Legacy CPB path produced non-positive coverage"; return false; }
+        p.cpbCorner[0] = minGX / lenToMm; p.cpbCorner[1] = minGY / lenToMm; p.cpbCorner[2] = 0.0f;
+        p.cpbRes[0] = b.raySpacing[0]; p.cpbRes[1] = b.raySpacing[1]; p.cpbRes[2] = doseRes[2];
+        p.rayDims[0] = std::max(1, int(std::ceil(covX / p.cpbRes[0])));
+        p.rayDims[1] = std::max(1, int(std::ceil(covY / p.cpbRes[1])));
+        p.legacyPath = true;
+    }
+    p.superpDims[0] = ((p.rayDims[0] + kTileX - 1) / kTileX) * kTileX;
+    p.superpDims[1] = ((p.rayDims[1] + kTileY - 1) / kTileY) * kTileY;
+
+    // index transforms (wrapper:3233-3248)
+    const F3 ctRes_mm = f3(ctRes[0] * lenToMm, ctRes[1] * lenToMm, ctRes[2] * lenToMm);
+    const F3 ctCorner_mm = f3(ctCorner[0] * lenToMm, ctCorner[1] * lenToMm, ctCorner[2] * lenToMm);
+    Aff w2im, w2dose;
+    w2im.m = diag(1.0f / ctRes_mm.x, 1.0f / ctRes_mm.y, 1.0f / ctRes_mm.z);
+    w2im.v = f3(-ctCorner_mm.x / ctRes_mm.x, -ctCorner_mm.y / ctRes_mm.y, -ctCorner_mm.z / ctRes_mm.z);
+    w2dose.m = diag(1.0f / doseRes_mm.x, 1.0f / doseRes_mm.y, 1.0f / doseRes_mm.z);
+    w2dose.v = f3(-doseCorner_mm.x / doseRes_mm.x, -doseCorner_mm.y / doseRes_mm.y, -doseCorner_mm.z / doseRes_mm.z);
+    p.g2im = concat(p.g2w, w2im);
+    p.g2dose = concat(p.g2w, w2dose);
+    p.dose2g = inverse(p.g2dose);
+
+    // source distances (wrapper:3250-3271)
+    float isx = 0, isy = 0;
+    const bool inferred = infer_source_dist(b, bmX, bmY, bmZ, sad, isx, isy);
+    float dxmm = b.sourceDist[0] * lenToMm, dymm = b.sourceDist[1] * lenToMm;
+    const bool legacySad = std::fabs(b.sourceDist[0] - b.sad) < 1.0e-3f && std::fabs(b.sourceDist[1] - b.sad) < 1.0e-3f;
+    if (inferred && (!(dxmm > 1e-6f) || legacySad)) dxmm = isx * lenToMm;
+    if (inferred && (!(dymm > 1e-6f) || legacySad)) dymm = isy * lenToMm;
+    if (!(dxmm > 1e-6f)) dxmm = sad * lenToMm;
+    if (!(dymm > 1e-6f)) dymm = sad * lenToMm;
+    p.sourceDist[0] = dxmm; p.sourceDist[1] = dymm;
+
+    // tracer extent (wrapper:3287-3370)
+    float minGZ = 1e30f, maxGZ = -1e30f;
+    const F3 csz = f3(float(std::max(ctDims[0] - 1, 0)) * ctRes_mm.x, float(std::max(ctDims[1] - 1, 0)) * ctRes_mm.y,
+                      float(std::max(ctDims[2] - 1, 0)) * ctRes_mm.z);
+    for (int cx = 0; cx <= 1; ++cx)
+        for (int cy = 0; cy <= 1; ++cy)
+            for (int cz = 0; cz <= 1; ++cz) {
+                const F3 g = apply(w2g, add(ctCorner_mm, f3(float(cx) * csz.x, float(cy) * csz.y, float(cz) * csz.z)));
+                minGZ = std::fmin(minGZ, g.z); maxGZ = std::fmax(maxGZ, g.z);
+            }
+    const float stepLen = ctRes[2] * lenToMm;
+    p.steps = std::max(1, int(std::ceil((maxGZ - minGZ) / std::fabs(stepLen))) + 1);
+    p.fanCorner[0] = p.cpbCorner[0] * lenToMm;
+    p.fanCorner[1] = p.cpbCorner[1] * lenToMm;
+    p.fanCorner[2] = maxGZ;
+    p.fanDelta[0] = p.cpbRes[0] * lenToMm;
+    p.fanDelta[1] = p.cpbRes[1] * lenToMm;
+    p.fanDelta[2] = -std::fabs(stepLen);
+
+    {  // frustum volume per step (wrapper:4276-4289)
+        const float dX = p.fanDelta[0], dY = p.fanDelta[1], dZ = p.fanDelta[2], cz = p.fanCorner[2];
+        const float sx = p.sourceDist[0], sy = p.sourceDist[1];
+        const float dxyz = std::fabs(dX * dY * dZ);
+        p.volConst = dxyz * (1.0f - cz / sx - cz / sy + (cz * cz + dZ * dZ / 12.0f) / (sx * sy));
+        p.volLin = dxyz * dZ * (-1.0f / sx - 1.0f / sy + 2.0f * cz / (sx * sy));
+        p.volSq = dxyz * dZ * dZ / (sx * sy);
+    }
+
+    // per-layer LUT parameters (conv stage wrapper:3746-3752; fill stage 3926-4329)
+    float maxE = b.energies[0];
+    for (int l = 1; l < b.numLayers; ++l) maxE = std::max(maxE, b.energies[l]);
+    {
+        const float mi = find_decimal_ordered(en.energiesPerU, en.nEnergies, maxE);
+        const float maxPeak = interp_at(en.peakDepths, en.nEnergies, mi) * p.energyDepthToMm;
+        p.maxCutoffWepl = 1.05f * maxPeak;
+    }
+    p.layers.resize(b.numLayers);
+    for (int l = 0; l < b.numLayers; ++l) {
+        LayerParams &L = p.layers[l];
+        std::memset(&L, 0, sizeof(L));
+        const float ci = find_decimal_ordered(en.energiesPerU, en.nEnergies, b.energies[l]);
+        const float convPeak = interp_at(en.peakDepths, en.nEnergies, ci) * p.energyDepthToMm;
+        L.convA = 0.00270f / (convPeak - 4.50f);
+        L.convB = -4.39f / (convPeak - 3.86f);
+        L.sigmaX = sigma_to_mm(lenToMm, b.spotSigmas[2 * l]);
+        L.sigmaY = sigma_to_mm(lenToMm, b.spotSigmas[2 * l + 1]);
+        const float ei = layer_energy_index(en, b.energies[l]);
+        L.energyIdx = ei;
+        L.scaleFact = interp_at(en.scaleFacts, en.nEnergies, ei) / p.energyDepthToMm;
+        L.peakDepth = interp_at(en.peakDepths, en.nEnergies, ei) * p.energyDepthToMm;
+        L.rangeStop = 1.05f * L.peakDepth;
+        L.cutoffWepl = 1.05f * L.peakDepth;
+        const float a = 0.00270f / (L.peakDepth - 4.50f), c = -4.39f / (L.peakDepth - 3.86f);
+        L.airQuad = a * 1.0f * p.fanDelta[2] * p.fanDelta[2];
+        L.airLin = 2.0f * a * 1.0f * p.fanDelta[2] * p.fanCorner[2] + c * p.fanDelta[2];
+    }
+    return true;
+}
+
+}  // namespace pbs

@@ -1,0 +1,95 @@
+// Host-side beam planning for the pencil-beam engine: units, beam basis, spot lattice,
+// CPB/ray grid, fan geometry, step counts and per-layer LUT parameters.
+//
+// The float arithmetic follows the reference host code operation by operation
+// (src/core/raytracedicom_wrapper.cu:2446-4386), because the integers it produces (grid
+// sizes and corners, step counts, layer cut-offs) must come out identical.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pbs_abi.h"
+
+namespace pbs {
+
+constexpr int kTileX = 32;       // SUPERP_TILE_X, Macro.cuh:58
+constexpr int kTileY = 8;        // SUPERP_TILE_Y, Macro.cuh:59
+constexpr int kMaxRad = 32;      // MAX_SUPERP_RADIUS, Macro.cuh:57
+constexpr int kMinTilesInBatch = 16;  // MIN_TILES_IN_BATCH, Macro.cuh:60
+constexpr int kMaxLayers = 256;
+
+struct F3 { float x, y, z; };
+struct M3 { F3 r0, r1, r2; };
+struct Aff { M3 m; F3 v; };
+
+F3 f3(float x, float y, float z);
+F3 apply(const Aff &t, F3 a);
+Aff inverse(const Aff &t);
+Aff concat(const Aff &first, const Aff &second);
+
+// Per-layer parameters consumed by device kernels (all plain floats/ints).
+struct LayerParams {
+    float energyIdx;       // fractional row of the CIDD table
+    float scaleFact;       // CIDD samples per mm
+    float peakDepth;       // mm
+    float rangeStop;       // 1.05 * peak
+    float cutoffWepl;      // threshold for findFirstLarger(weplMin, .)
+    float airQuad, airLin; // sigma^2 air-divergence recurrence coefficients
+    float convA, convB;    // sigmaSqAirCoefs at the conversion peak (conv stage)
+    float sigmaX, sigmaY;  // spot sigma at iso (mm)
+    float pad0, pad1;
+};
+
+struct BeamPlan {
+    std::string error;
+    bool legacyPath = false;
+    float lenToMm = 1.f, energyDepthToMm = 1.f;
+    F3 bmX{}, bmY{}, bmZ{}, iso{};
+    Aff g2w{}, g2im{}, g2dose{}, dose2g{};
+    float sourceDist[2] = {0, 0};
+    float sadInput = 0.f;           // SAD in input units (legacy CPB geometry)
+    // spot lattice (lattice path)
+    int spotDims[2] = {0, 0};
+    float spotOffset[2] = {0, 0}, spotDelta[2] = {0, 0};
+    std::vector<float> denseWeights;  // [L][sy][sx]
+    // grids
+    float cpbCorner[3] = {0, 0, 0}, cpbRes[3] = {0, 0, 0};
+    int rayDims[2] = {0, 0};
+    int superpDims[2] = {0, 0};
+    int steps = 0;
+    float fanCorner[3] = {0, 0, 0}, fanDelta[3] = {0, 0, 0};
+    float volConst = 0, volLin = 0, volSq = 0;
+    float maxCutoffWepl = 0;  // 1.05 * peak depth of the highest-energy layer
+    int numLayers = 0;
+    std::vector<LayerParams> layers;
+    bool nuclear = false;
+    bool subspotConv = false;
+};
+
+// Explicit CPB grid in the gantry frame (used by cudaFinalDose, whose CPB grid comes from
+// the ROI box rather than from the dose-box projection).
+struct CpbOverride {
+    float corner[2], res[2];
+    int dims[2];
+};
+
+// Returns false (and sets plan.error) on a contract failure.
+bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3],
+               const int doseDims[3], const float doseRes[3], const float doseCorner[3],
+               const pbs_beam_settings &beam, const pbs_energy_struct &energy, BeamPlan &plan,
+               const CpbOverride *cpbOverride = nullptr);
+
+// findDecimalOrdered (include/utils/vector_find.h:130-145) and the wrapper's own energy
+// bracket (wrapper:3926-3992).
+float find_decimal_ordered(const float *v, int n, float value);
+float layer_energy_index(const pbs_energy_struct &e, float energy);
+
+// Reference CT auto-detection on a host buffer: 0 = HU, 1 = HU+1000, 2 = density/SPR.
+int detect_ct_type(const float *ct, size_t n);
+
+// Radius bins -> launch radii (complete_superposition.cu:413-490).
+void batch_bins(const int *hist34, int *batched33, int *effRad34);
+
+}  // namespace pbs

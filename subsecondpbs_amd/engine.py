@@ -1,0 +1,117 @@
+"""Resident dose engine: CT, LUTs and work buffers stay in HBM across beams.
+
+Thin Python handle over the ``pbs_engine_*`` C ABI. Device buffers passed in (dose grids,
+CT) are torch tensors on the HIP device; the engine queues work on its own stream, so
+callers synchronise with :meth:`Engine.sync` before reading results.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class Engine:
+    def __init__(self, energy: dict, device: int = 0):
+        self._keep = _abi.Keep()
+        self._es = _abi.energy_struct(energy, self._keep)
+        h = _abi.lib().pbs_engine_create(int(device), C.byref(self._es))
+        if not h:
+            raise RuntimeError("pbs_engine_create failed: " + _abi.last_error())
+        self.handle = h
+        self.device = device
+        self.ct_dims = None
+        self.dose_dims = None
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _abi.lib().pbs_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ geometry
+    def set_ct(self, ct, dims, resolution, corner):
+        """ct: numpy float32 [nz][ny][nx] (host) or a torch tensor on the device."""
+        d, dp = _abi.i3(dims)
+        r, rp = _abi.f3(resolution)
+        c, cp = _abi.f3(corner)
+        if isinstance(ct, np.ndarray):
+            a = np.ascontiguousarray(ct, dtype=np.float32)
+            rc = _abi.lib().pbs_engine_set_ct(self.handle, a.ctypes.data, dp, rp, cp, 0)
+        else:
+            rc = _abi.lib().pbs_engine_set_ct(self.handle, C.c_void_p(ct.data_ptr()), dp, rp, cp, 1)
+        _abi.check(rc, "pbs_engine_set_ct")
+        self.ct_dims = tuple(int(v) for v in dims)
+
+    def set_dose_grid(self, dims, resolution, corner):
+        d, dp = _abi.i3(dims)
+        r, rp = _abi.f3(resolution)
+        c, cp = _abi.f3(corner)
+        _abi.check(_abi.lib().pbs_engine_set_dose_grid(self.handle, dp, rp, cp), "pbs_engine_set_dose_grid")
+        self.dose_dims = tuple(int(v) for v in dims)
+
+    # ------------------------------------------------------------------ compute
+    def beam_dose(self, beam: dict, dose, use_subspots: bool | None = None):
+        """Queue one beam; its dose is added into ``dose`` (device tensor [nz][ny][nx] float32)."""
+        keep = _abi.Keep()
+        bs = _abi.beam_settings(beam, keep, use_subspots)
+        rc = _abi.lib().pbs_engine_beam_dose(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()))
+        _abi.check(rc, "pbs_engine_beam_dose")
+
+    def sync(self):
+        _abi.check(_abi.lib().pbs_engine_sync(self.handle), "pbs_engine_sync")
+
+    @property
+    def stream_ptr(self) -> int:
+        return int(_abi.lib().pbs_engine_stream(self.handle) or 0)
+
+    def debug_fetch(self, what=("density", "wepl", "fi", "fo", "weplMin", "rayW", "iddSigma", "firstPassive",
+                                 "tileRad", "radHist", "bev")) -> dict:
+        """Copy the last beam's intermediate arrays to host numpy arrays."""
+        ints = _abi.DebugInts()
+        rc = _abi.lib().pbs_engine_debug_fetch(self.handle, C.byref(ints), *([None] * 11))
+        _abi.check(rc, "pbs_engine_debug_fetch")
+        rx, ry = ints.rayDims
+        sx, sy = ints.superpDims
+        K, L = ints.steps, ints.numLayers
+        nt = (sx // 32) * (sy // 8)
+        shapes = {
+            "density": ((K, ry, rx), np.float32), "wepl": ((K, ry, rx), np.float32),
+            "fi": ((ry, rx), np.int32), "fo": ((ry, rx), np.int32), "weplMin": ((K,), np.float32),
+            "rayW": ((L, ry, rx), np.float32), "iddSigma": ((L, K, sy, sx, 2), np.float32),
+            "firstPassive": ((L, ry, rx), np.int32), "tileRad": ((L, K, sy // 8, sx // 32), np.int32),
+            "radHist": ((L, 34), np.int32), "bev": ((K, sy + 64, sx + 64), np.float32),
+        }
+        order = ["density", "wepl", "fi", "fo", "weplMin", "rayW", "iddSigma", "firstPassive", "tileRad",
+                 "radHist", "bev"]
+        out = {}
+        ptrs = []
+        for name in order:
+            if name in what:
+                shp, dt = shapes[name]
+                a = np.zeros(shp, dt)
+                out[name] = a
+                ptrs.append(C.c_void_p(a.ctypes.data))
+            else:
+                ptrs.append(None)
+        rc = _abi.lib().pbs_engine_debug_fetch(self.handle, C.byref(ints), *ptrs)
+        _abi.check(rc, "pbs_engine_debug_fetch")
+        out["ints"] = {
+            "rayDims": (rx, ry), "superpDims": (sx, sy), "steps": K, "numLayers": L, "legacy": ints.legacyPath,
+            "beamFirstInside": ints.beamFirstInside, "beamFirstOutside": ints.beamFirstOutside,
+            "firstPastCutoffAll": ints.firstPastCutoffAll, "guaranteedPassive": ints.guaranteedPassive,
+            "afterLast": np.array(ints.afterLast[:L], np.int32), "layerPassive": np.array(ints.layerPassive[:L], np.int32),
+            "transferStart": tuple(ints.transferStart), "transferMax": tuple(ints.transferMax),
+            "cpbCorner": tuple(ints.cpbCorner), "fanCorner": tuple(ints.fanCorner), "fanDelta": tuple(ints.fanDelta),
+            "sourceDist": tuple(ints.sourceDist),
+            "energyIdx": np.array(ints.energyIdx[:L], np.float32), "scaleFact": np.array(ints.scaleFact[:L], np.float32),
+            "peakDepth": np.array(ints.peakDepth[:L], np.float32),
+        }
+        return out
