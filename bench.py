@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X pencil-beam final-dose engine.
+
+Workload (one "step" = one beam's full final dose): the head-size proton beam, plain GPU
+form - synthetic CT 256x256x160 at 1x1x2 mm (ellipsoidal body, two bones, air cavity),
+one beam at gantry 90 deg, 30 energy layers 70-180 MeV, 109 spots per layer on a 5 mm
+hexagonal lattice (3270 spots), FP32 dose on the CT grid. CT, LUTs and the dose grid are
+resident in HBM before timing; each step queues the beam and waits for it (one request at
+a time), so ms_per_step is the per-beam latency.
+
+Prints one JSON line. Extra fields: p50/p99 ms per beam, the max dose error of one beam
+against the FP64 oracle, per-stage device times (HIP events on the engine stream), the
+roofline of the dominant stage and the CPU baseline (the FP64 oracle on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "beams/sec & p50/p99 ms per beam at ≤0.5% Dmax err vs FP64; served vs plain"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def algorithmic_work(dbg: dict, case: dict) -> dict:
+    """Per-beam algorithmic work of each stage, from the beam's own intermediate arrays."""
+    from subsecondpbs_amd.plan_util import batch_bins
+
+    ints = dbg["ints"]
+    L, K = ints["numLayers"], ints["steps"]
+    rx, ry = ints["rayDims"]
+    sx, sy = ints["superpDims"]
+    first = ints["beamFirstInside"]
+    nray = rx * ry
+    # superposition: 2 (2R+1)^2 flops per live pixel of every live (layer, step, tile)
+    flops = 0.0
+    live_px = 0
+    for l in range(L):
+        pas = int(ints["layerPassive"][l])
+        kend = pas - first if pas > first else first
+        eff = batch_bins(dbg["radHist"][l])
+        for k in range(first, kend):
+            codes = dbg["tileRad"][l, k]
+            d = dbg["iddSigma"][l, k, :, :, 0]
+            for ty in range(codes.shape[0]):
+                for tx in range(codes.shape[1]):
+                    c = int(codes[ty, tx])
+                    if c < 0 or not (c & 256):
+                        continue
+                    R = int(eff[c & 255])
+                    n = int(np.count_nonzero(d[ty * 8:(ty + 1) * 8, tx * 32:(tx + 1) * 32] > 0))
+                    live_px += n
+                    flops += n * 2.0 * (2 * R + 1) ** 2
+    # IDD/sigma fill: read density + WEPL, write (IDD, sigma) per ray per computed step
+    steps_sum = sum(max(int(ints["afterLast"][l]) - first, 0) for l in range(L))
+    fill_bytes = 16.0 * nray * steps_sum
+    ct_bytes = 4.0 * float(np.prod(case["ct_dims"]))
+    trace_bytes = ct_bytes + 8.0 * nray * K
+    ts, tm = ints["transferStart"], ints["transferMax"]
+    nbox = max(tm[0] - ts[0] + 1, 0) * max(tm[1] - ts[1] + 1, 0) * max(tm[2] - ts[2] + 1, 0)
+    bev_bytes = 4.0 * (sx + 64) * (sy + 64) * K
+    return {
+        "superpose": {"flops": flops, "live_pixels": live_px},
+        "fill": {"bytes": fill_bytes},
+        "trace": {"bytes": trace_bytes},
+        "transfer": {"bytes": 8.0 * nbox + bev_bytes},
+        "merge": {"bytes": 2.0 * bev_bytes},
+    }
+
+
+def cpu_baseline(case: dict, runs: int) -> dict:
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        oracle.beam_dose(case)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"value": 1.0 / med, "unit": "beams/s", "cores": threads, "kind": "port",
+            "sample": f"{runs} full head-size beams (30 layers, 3270 spots) through the FP64 oracle, "
+                      f"median {med:.2f} s per beam"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU per process; multi-GPU runs are not part of "
+              "this build (beams would shard across GPUs with one dose reduction).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import oracle
+    from subsecondpbs_amd import fixtures
+    from subsecondpbs_amd.engine import Engine
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    case = fixtures.make_config2()
+    beam = case["beams"][0]
+    eng = Engine(case["energy"], device=0)
+    eng.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    eng.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    nx, ny, nz = case["dose_dims"]
+    dose = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+
+    # accuracy of one beam against the FP64 oracle (outside the timed region)
+    eng.beam_dose(beam, dose)
+    eng.sync()
+    one = dose.cpu().numpy().astype(np.float64)
+    dbg = eng.debug_fetch(("iddSigma", "tileRad", "radHist"))
+    t0 = time.perf_counter()
+    ref, _ = oracle.beam_dose(case)
+    oracle_s = time.perf_counter() - t0
+    err = float(np.abs(one - ref).max() / ref.max())
+    work = algorithmic_work(dbg, case)
+
+    for _ in range(args.warmup):
+        eng.beam_dose(beam, dose)
+    eng.sync()
+    torch.cuda.synchronize()
+
+    eng.set_profiling(True)
+    eng.stage_times()  # reset
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        eng.beam_dose(beam, dose)
+        eng.sync()
+        lat.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    stages, nb = eng.stage_times()
+    eng.set_profiling(False)
+    per_beam = {k: v / max(nb, 1) for k, v in stages.items()}
+
+    # roofline of the dominant stage
+    dom = max(per_beam, key=per_beam.get)
+    t_dom = per_beam[dom] * 1e-3
+    if dom == "superpose":
+        ach = work["superpose"]["flops"] / t_dom / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_superpose<R> (8 radius classes)",
+                "compute": "fp32 VALU (no MFMA; peak quoted is the fp32 dense figure)"}
+    else:
+        b = work.get(dom, {}).get("bytes", 0.0)
+        ach = b / t_dom / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "kernel": dom}
+    roof["stage_ms"] = per_beam[dom]
+
+    out = {
+        "metric": METRIC,
+        "value": args.steps / total,
+        "unit": "beams/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "head-size proton beam, plain GPU form: CT 256x256x160 @1x1x2 mm, gantry 90, "
+                               "30 layers 70-180 MeV, 3270 spots (5 mm hex), dose on the CT grid",
+                   "form": "plain", "beams_per_step": 1},
+        "p50_ms_per_beam": float(np.percentile(lat, 50)),
+        "p99_ms_per_beam": float(np.percentile(lat, 99)),
+        "max_dose_err_vs_fp64": err,
+        "stage_ms_per_beam": per_beam,
+        "algorithmic": {"superpose_gflop": work["superpose"]["flops"] / 1e9,
+                        "fill_mb": work["fill"]["bytes"] / 1e6, "trace_mb": work["trace"]["bytes"] / 1e6},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(case, max(args.cpu_runs, 1))
+    else:
+        out["cpu_baseline"] = {"value": 1.0 / oracle_s, "unit": "beams/s", "cores": os.cpu_count(), "kind": "port",
+                               "sample": "1 head-size beam through the FP64 oracle"}
+    eng.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -115,6 +115,13 @@ int pbs_engine_sync(pbs_engine *e);
 /* Raw stream handle (hipStream_t) the engine queues work on. */
 void *pbs_engine_stream(pbs_engine *e);
 
+/* Per-stage device time, measured with HIP events on the engine stream while profiling is
+ * on: msOut[7] = trace (+ reductions, cut-offs), ray weights, IDD/sigma fill, tile radii /
+ * bins / items, superposition, block merge, BEV->dose transfer, summed over the beams queued
+ * since the previous call (their count in *beams). */
+int pbs_engine_set_profiling(pbs_engine *e, int enable);
+int pbs_engine_stage_times(pbs_engine *e, float *msOut, int *beams);
+
 /* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
  * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside
  * [ry][rx]; weplMin [K]; rayWeights [L][ry][rx]; iddSigma [L][K][sy][sx][2] (IDD, rSigmaEff;

@@ -94,12 +94,29 @@ struct pbs_engine {
     std::vector<LayerHost> hostLayers;
     hipEvent_t uploadDone = nullptr;
     bool uploadPending = false;
+    // optional per-stage timing: kStages + 1 events per beam, resolved by pbs_engine_stage_times
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::vector<hipEvent_t>> evBeams;
     // final-dose binding
     std::vector<float> fdEnergies, fdSigmas;
     float fdRaySpacing = 0.5f;
 };
 
 namespace {
+
+constexpr int kStages = 7;  // trace, conv, fill, tiles, superpose, merge, transfer
+
+hipEvent_t take_event(pbs_engine *e) {
+    if (!e->evPool.empty()) {
+        hipEvent_t ev = e->evPool.back();
+        e->evPool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    (void)hipEventCreate(&ev);
+    return ev;
+}
 
 void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
     std::memset(&g, 0, sizeof(g));
@@ -255,6 +272,9 @@ void pbs_engine_destroy(pbs_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->uploadDone) (void)hipEventDestroy(e->uploadDone);
+    for (auto &m : e->evBeams)
+        for (hipEvent_t ev : m) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : e->evPool) (void)hipEventDestroy(ev);
     delete e;
 }
 
@@ -373,7 +393,14 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
                            hipMemcpyHostToDevice, s));
+    std::vector<hipEvent_t> marks;
+    auto mark = [&]() {
+        if (!e->profiling) return hipSuccess;
+        marks.push_back(take_event(e));
+        return hipEventRecord(marks.back(), s);
+    };
     // 1. trace and per-beam reductions
+    HIP_TRY(mark());
     HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     const float nucFactor = 1.0f;
@@ -381,6 +408,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
                               e->classCount.p, e->hist.p, s));
     // 2. ray weights
+    HIP_TRY(mark());
     if (!p.legacyPath) {
         const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
         HIP_TRY(e->spot.ensure(splane * L));
@@ -424,24 +452,57 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(hipEventRecord(e->uploadDone, s));
     e->uploadPending = true;
     // 3. IDD / sigma
+    HIP_TRY(mark());
     const FillConsts fc{198.81f, 0.21f, 0, 0};
     HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
                         e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, s));
     // 4. passive steps, tiles, bins, items
+    HIP_TRY(mark());
     HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->beamScal.p, e->tileRad.p, e->hist.p,
                                       e->effRad.p, e->itemR.p, e->classCount.p, e->classList.p, maxItems, s));
     // 5. superposition into per-item blocks, then the deterministic merge into the BEV volume
+    HIP_TRY(mark());
     HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->classCount.p,
                              e->classList.p, maxItems, e->slots.p, 2048, s));
+    HIP_TRY(mark());
     HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, K, s));
     // 6. BEV -> dose
     TransferArgs ta{};
     fill_transfer_args(p, ta);
     int gx = 1, gy = 1;
     host_transfer_grid(p, e->doseDims, gx, gy);
+    HIP_TRY(mark());
     HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->bev.p, K, doseDev, gx, gy, s));
+    HIP_TRY(mark());
+    if (e->profiling) e->evBeams.push_back(std::move(marks));
     e->lastMaxItems = maxItems;
     e->haveLast = true;
+    return PBS_OK;
+}
+
+int pbs_engine_set_profiling(pbs_engine *e, int enable) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    e->profiling = enable != 0;
+    return PBS_OK;
+}
+
+int pbs_engine_stage_times(pbs_engine *e, float *msOut, int *beams) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    std::vector<double> acc(kStages, 0.0);
+    for (auto &m : e->evBeams) {
+        for (int i = 0; i < kStages && i + 1 < int(m.size()); ++i) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, m[i], m[i + 1]));
+            acc[i] += ms;
+        }
+        for (hipEvent_t ev : m) e->evPool.push_back(ev);
+    }
+    if (beams) *beams = int(e->evBeams.size());
+    e->evBeams.clear();
+    if (msOut)
+        for (int i = 0; i < kStages; ++i) msOut[i] = float(acc[i]);
     return PBS_OK;
 }
 

@@ -68,6 +68,18 @@ class Engine:
     def sync(self):
         _abi.check(_abi.lib().pbs_engine_sync(self.handle), "pbs_engine_sync")
 
+    STAGES = ("trace", "conv", "fill", "tiles", "superpose", "merge", "transfer")
+
+    def set_profiling(self, on: bool = True):
+        _abi.check(_abi.lib().pbs_engine_set_profiling(self.handle, int(bool(on))), "pbs_engine_set_profiling")
+
+    def stage_times(self) -> tuple[dict, int]:
+        """Device ms per stage summed over the beams queued since the last call, and their count."""
+        ms = (C.c_float * 7)()
+        n = C.c_int(0)
+        _abi.check(_abi.lib().pbs_engine_stage_times(self.handle, ms, C.byref(n)), "pbs_engine_stage_times")
+        return {k: float(v) for k, v in zip(self.STAGES, ms)}, int(n.value)
+
     @property
     def stream_ptr(self) -> int:
         return int(_abi.lib().pbs_engine_stream(self.handle) or 0)

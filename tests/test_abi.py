@@ -59,3 +59,11 @@ def test_final_dose_roi_integers():
     hit = L.pbs_final_dose_roi(roi, 3, res, corner, _abi.Vec3f(0, 0, -1), _abi.Vec3f(0, 0, -1000), 4,
                                None, None, None, None)
     assert hit == 0
+
+
+def test_cxx_dropin_symbols_exported():
+    """include/raytracedicom_integration.h: the reference's original C++ entry points."""
+    lib = C.CDLL(_abi.LIB_PATH)
+    for name in ("subsecondWrapper", "createRTDBeamSettings", "destroyRTDBeamSettings",
+                 "createRTDEnergyStruct", "destroyRTDEnergyStruct"):
+        assert hasattr(lib, name), name
