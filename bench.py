@@ -46,9 +46,15 @@ def algorithmic_work(dbg: dict, case: dict) -> dict:
     live_px = 0
     for l in range(L):
         pas = int(ints["layerPassive"][l])
-        kend = pas - first if pas > first else first
+        g = int(ints["layerGroup"][l])
+        # steps the beam superposes for this layer: main BEV below mainEnd, then the
+        # layer group's tail planes (only steps some dose voxel samples)
+        ks = list(range(first, int(ints["mainEnd"][l])))
+        if g >= 0:
+            t0, tc = int(ints["groupTailStart"][g]), int(ints["groupTailCount"][g])
+            ks += list(range(t0, min(t0 + tc, pas)))
         eff = batch_bins(dbg["radHist"][l])
-        for k in range(first, kend):
+        for k in ks:
             codes = dbg["tileRad"][l, k]
             d = dbg["iddSigma"][l, k, :, :, 0]
             for ty in range(codes.shape[0]):

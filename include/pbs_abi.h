@@ -135,6 +135,11 @@ typedef struct pbs_debug_ints {
     int transferStart[3], transferMax[3];
     float cpbCorner[3], fanCorner[3], fanDelta[3], sourceDist[2];
     float energyIdx[256], scaleFact[256], peakDepth[256];
+    /* layer groups (layers sharing a passive step share a transfer box): superposed into
+     * the main BEV below mainEnd; group g's tail planes [tailStart, tailStart + tailCount) */
+    int numGroups, tailPlanes, tailOverflow, mainEndMax;
+    int mainEnd[256], layerGroup[256];
+    int groupPassive[256], groupTailStart[256], groupTailCount[256];
 } pbs_debug_ints;
 int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, float *wepl,
                            int *firstInside, int *firstOutside, float *weplMin, float *rayWeights,

@@ -58,6 +58,9 @@ class DebugInts(C.Structure):
         ("cpbCorner", C.c_float * 3), ("fanCorner", C.c_float * 3), ("fanDelta", C.c_float * 3),
         ("sourceDist", C.c_float * 2),
         ("energyIdx", C.c_float * 256), ("scaleFact", C.c_float * 256), ("peakDepth", C.c_float * 256),
+        ("numGroups", C.c_int), ("tailPlanes", C.c_int), ("tailOverflow", C.c_int), ("mainEndMax", C.c_int),
+        ("mainEnd", C.c_int * 256), ("layerGroup", C.c_int * 256),
+        ("groupPassive", C.c_int * 256), ("groupTailStart", C.c_int * 256), ("groupTailCount", C.c_int * 256),
     ]
 
 

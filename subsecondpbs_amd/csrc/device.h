@@ -41,15 +41,25 @@ struct LayerHost {
 struct BeamScalars {
     int beamFirstInside, beamFirstOutside, firstPastCutoffAll, guaranteedPassive;
     float entryZ, pxMultX, pxMultY, pad0;
-    int maxPassive, kEndMax, activeLayers, pad1;
-    int boxStart[3], boxEnd[3];  // transfer box [start, end) in dose index space (x, y from grid rounding)
-    int boxMax[3], pad2;
+    int maxPassive, mainEndMax, activeLayers, numGroups;
+    int tailPlanes, overflow, pad1, pad2;
+    int boxStart[3], boxEnd[3];  // union of the group transfer boxes, [start, end)
+    int boxMax[3], pad3;
 };
 
 struct LayerScalars {
-    int afterLast, passiveRaw, passive, kEnd;  // kEnd: last superposed step + 1
+    int afterLast, passiveRaw, passive, mainEnd;  // mainEnd: superposed into the main BEV below this step
     float entrySigX, entrySigY;
-    int pad0, pad1;
+    int group, pad0;
+};
+
+// Layers with the same passive step share a transfer box (a "group"). The main BEV holds
+// every layer's steps below its group's tailStart; the group's own tail BEV holds steps
+// [tailStart, tailStart + tailCount), sampled only by dose voxels inside the group's box.
+struct GroupScalars {
+    int passive, tailStart, tailCount, tailBase;
+    int boxStart[3], boxEnd[3];
+    int boxMax[3], pad;
 };
 
 }  // namespace pbs

@@ -80,15 +80,18 @@ struct pbs_engine {
     float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
     // per-beam work buffers
     DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, slots, bev, cpb;
+    DevBuf<float> tailSlots, tailBev;
     DevBuf<float2> iddSigma;
     DevBuf<int> fi, fo, firstPassive, tileRad, hist, effRad, itemR, classCount, classList;
+    DevBuf<int> tailPlaneGroup, tailPlaneK, tailItemR;
+    DevBuf<GroupScalars> groupScal;
     DevBuf<LayerHost> layerHost;
     DevBuf<LayerScalars> layerScal;
     DevBuf<BeamScalars> beamScal;
     // last beam (debug fetch)
     BeamPlan plan;
     FanGeom geom{};
-    int lastMaxItems = 0;
+    int lastMaxItems = 0, lastTailCap = 0;
     bool haveLast = false;
     // host staging of the last beam's uploads; the next beam waits for uploadDone before reuse
     std::vector<LayerHost> hostLayers;
@@ -358,6 +361,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     const size_t nray = size_t(rx) * ry, nsup = size_t(sx) * sy;
     const int nt = (sx / 32) * (sy / 8);
     const int maxItems = K * nt;
+    // tail planes: a group's tail is at most min(beamFirstInside + 1, K - beamFirstInside)
+    // <= (K + 1) / 2 steps long, and there are at most min(L, K) groups
+    const int tailCap = std::min(L, K) * std::max(1, (K + 1) / 2);
+    const size_t bevPlane = size_t(sx + 64) * (sy + 64);
     const bool subMode = p.subspotConv && !p.legacyPath;
     const int nSub = subMode ? beam->maxSubspotsPerLayer : 1;
     // buffers
@@ -376,7 +383,13 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(e->classCount.ensure(16));
     HIP_TRY(e->classList.ensure(size_t(kNumClasses) * maxItems));
     HIP_TRY(e->slots.ensure(size_t(maxItems) * kSlotFloats));
-    HIP_TRY(e->bev.ensure(size_t(K) * (sx + 64) * (sy + 64)));
+    HIP_TRY(e->bev.ensure(size_t(K) * bevPlane));
+    HIP_TRY(e->groupScal.ensure(L));
+    HIP_TRY(e->tailPlaneGroup.ensure(tailCap));
+    HIP_TRY(e->tailPlaneK.ensure(tailCap));
+    HIP_TRY(e->tailItemR.ensure(size_t(tailCap) * nt));
+    HIP_TRY(e->tailSlots.ensure(size_t(tailCap) * nt * kSlotFloats));
+    HIP_TRY(e->tailBev.ensure(size_t(tailCap) * bevPlane));
     HIP_TRY(e->layerHost.ensure(L));
     HIP_TRY(e->layerScal.ensure(L));
     HIP_TRY(e->beamScal.ensure(1));
@@ -458,24 +471,29 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                         e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, s));
     // 4. passive steps, tiles, bins, items
     HIP_TRY(mark());
-    HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->beamScal.p, e->tileRad.p, e->hist.p,
-                                      e->effRad.p, e->itemR.p, e->classCount.p, e->classList.p, maxItems, s));
+    HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->groupScal.p, e->beamScal.p,
+                                      e->tailPlaneGroup.p, e->tailPlaneK.p, tailCap, e->tileRad.p, e->hist.p,
+                                      e->effRad.p, e->itemR.p, e->tailItemR.p, e->classCount.p, e->classList.p,
+                                      maxItems, s));
     // 5. superposition into per-item blocks, then the deterministic merge into the BEV volume
     HIP_TRY(mark());
-    HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->classCount.p,
-                             e->classList.p, maxItems, e->slots.p, 2048, s));
+    HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->groupScal.p,
+                             e->beamScal.p, e->classCount.p, e->classList.p, maxItems, e->slots.p, e->tailSlots.p,
+                             2048, s));
     HIP_TRY(mark());
-    HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, K, s));
+    HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, e->tailItemR.p, e->tailSlots.p,
+                         e->tailBev.p, tailCap, s));
     // 6. BEV -> dose
     TransferArgs ta{};
     fill_transfer_args(p, ta);
     int gx = 1, gy = 1;
     host_transfer_grid(p, e->doseDims, gx, gy);
     HIP_TRY(mark());
-    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->bev.p, K, doseDev, gx, gy, s));
+    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->groupScal.p, e->bev.p, e->tailBev.p, doseDev, gx, gy, s));
     HIP_TRY(mark());
     if (e->profiling) e->evBeams.push_back(std::move(marks));
     e->lastMaxItems = maxItems;
+    e->lastTailCap = tailCap;
     e->haveLast = true;
     return PBS_OK;
 }
@@ -560,6 +578,22 @@ int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, 
         }
         ints->sourceDist[0] = p.sourceDist[0];
         ints->sourceDist[1] = p.sourceDist[1];
+        ints->numGroups = bs.numGroups;
+        ints->tailPlanes = bs.tailPlanes;
+        ints->tailOverflow = bs.overflow;
+        ints->mainEndMax = bs.mainEndMax;
+        std::vector<GroupScalars> gs(std::max(bs.numGroups, 1));
+        if (bs.numGroups > 0)
+            HIP_TRY(hipMemcpy(gs.data(), e->groupScal.p, sizeof(GroupScalars) * bs.numGroups, hipMemcpyDeviceToHost));
+        for (int l = 0; l < L; ++l) {
+            ints->mainEnd[l] = ls[l].mainEnd;
+            ints->layerGroup[l] = ls[l].group;
+        }
+        for (int j = 0; j < bs.numGroups && j < 256; ++j) {
+            ints->groupPassive[j] = gs[j].passive;
+            ints->groupTailStart[j] = gs[j].tailStart;
+            ints->groupTailCount[j] = gs[j].tailCount;
+        }
     }
     return PBS_OK;
 }

@@ -12,6 +12,7 @@ constexpr int kMinTilesDev = 16;
 constexpr int kSlotPitch = 96;                    // 32 + 2 * 32 columns
 constexpr int kSlotFloats = kSlotPitch * 72;      // (8 + 2 * 32) rows
 constexpr int kNumClasses = 8;
+constexpr int kMaxLayersDev = 256;
 
 struct ConvArgs {
     int rx, ry, spotX, spotY, nSub, subspotMode;
@@ -51,15 +52,17 @@ hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *ray
 hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
                        const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, hipStream_t s);
-hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, BeamScalars *bs,
-                                     int *tileRad, int *hist, int *effRad, int *itemR, int *classCount,
+hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
+                                     BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
+                                     int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int *classList, int maxItems, hipStream_t s);
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
-                            const LayerScalars *ls, const int *classCount, const int *classList, int maxItems,
-                            float *slots, int gridCap, hipStream_t s);
-hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *slots, float *bev,
-                        int bevZ, hipStream_t s);
-hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const float *bev, int bevZ,
-                           float *dose, int gridX, int gridY, hipStream_t s);
+                            const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
+                            const int *classCount, const int *classList, int maxItems, float *mainSlots,
+                            float *tailSlots, int gridCap, hipStream_t s);
+hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
+                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, hipStream_t s);
+hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
+                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, hipStream_t s);
 
 }  // namespace pbs

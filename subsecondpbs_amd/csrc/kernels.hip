@@ -12,15 +12,20 @@
 //                        (convolution.cu:95-152)
 //   k_cpb_to_rays ...... area-overlap resample (convolution.cu:269-339)
 //   k_fill ............. IDD and sigma per ray and layer (idd_sigma.cu:46-259)
-//   k_layer_finalize ... layer passive steps, transfer box (wrapper:4950-4996, 5343-5395)
+//   k_layer_finalize ... layer passive steps, layer groups (layers sharing a passive step
+//                        share a transfer box), group boxes and tail planes
+//                        (wrapper:4950-4996, 5343-5395)
 //   k_tiles / k_bins ... 32x8 tile radii, radius histogram, batched launch radii
 //                        (complete_superposition.cu:197-255, 400-490)
 //   k_items ............ per (step, tile) work item: max launch radius over live layers
 //   k_superpose<RM> .... register-blocked Gaussian superposition of one (step, tile) over
-//                        all layers into a private output block (no atomics)
-//   k_merge ............ deterministic sum of overlapping blocks into the BEV dose volume
+//                        all layers into private output blocks (main + group tails; no
+//                        atomics)
+//   k_merge ............ deterministic sum of overlapping blocks into the main BEV volume
+//                        and the group tail planes
 //   k_transfer ......... divergent BEV -> dose-grid trilinear gather, one owner per dose
-//                        column (prim_transf_kernel.cu:11-44)
+//                        column (prim_transf_kernel.cu:11-44); voxels inside a group's
+//                        transfer box also sample that group's tail planes
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -121,17 +126,22 @@ __global__ __launch_bounds__(256) void k_trace(FanGeom g, const float *__restric
     const int i = ray % g.rx, j = ray / g.rx;
     const float fx = g.fc[0] + float(i) * g.fd[0];
     const float fy = g.fc[1] + float(j) * g.fd[1];
-    float p0[3], p1[3];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-        const float fz = g.fc[2] + float(s) * g.fd[2];
+    // ray start and per-step increment in CT index space; the increment is the exact
+    // derivative of the divergent fan point (not a difference of two rounded points, whose
+    // error would grow with the step index)
+    float p0[3], inc[3];
+    {
+        const float fz = g.fc[2];
         const float gx = fx * (1.0f - fz / g.dist[0]);
         const float gy = fy * (1.0f - fz / g.dist[1]);
-        float *o = s ? p1 : p0;
+        const float dgx = -fx * (g.fd[2] / g.dist[0]);
+        const float dgy = -fy * (g.fd[2] / g.dist[1]);
 #pragma unroll
-        for (int r = 0; r < 3; ++r) o[r] = g.g2im[3 * r] * gx + g.g2im[3 * r + 1] * gy + g.g2im[3 * r + 2] * fz + g.g2imOff[r];
+        for (int r = 0; r < 3; ++r) {
+            p0[r] = g.g2im[3 * r] * gx + g.g2im[3 * r + 1] * gy + g.g2im[3 * r + 2] * fz + g.g2imOff[r];
+            inc[r] = g.g2im[3 * r] * dgx + g.g2im[3 * r + 1] * dgy + g.g2im[3 * r + 2] * g.fd[2];
+        }
     }
-    const float inc[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
     const float dx = fx / g.dist[0], dy = fy / g.dist[1];
     const float stepLen = fabsf(g.fd[2]) * sqrtf(1.0f + dx * dx + dy * dy);
     float cum = 0.0f, comp = 0.0f, cumHu = 0.0f;
@@ -220,12 +230,13 @@ __global__ __launch_bounds__(256) void k_beam_setup(FanGeom g, const float *__re
         s.afterLast = min(local, gp);
         s.passiveRaw = INT_MIN;
         s.passive = 0;
-        s.kEnd = 0;
+        s.mainEnd = 0;
+        s.group = -1;
         const float base = L.convA * entryZ * entryZ + L.convB * entryZ;
         const float vx = base + L.sigmaX * L.sigmaX, vy = base + L.sigmaY * L.sigmaY;
         s.entrySigX = sqrtf(fmaxf(0.0f, vx)) * nucFactor;
         s.entrySigY = sqrtf(fmaxf(0.0f, vy)) * nucFactor;
-        s.pad0 = s.pad1 = 0;
+        s.pad0 = 0;
         ls[l] = s;
         for (int q = 0; q < maxSub && subspots; ++q) {
             const float *ss = subspots + (size_t(l) * maxSub + q) * 5;
@@ -439,50 +450,14 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
     if ((threadIdx.x & 63) == 0) atomicMax(&ls[l].passiveRaw, m);
 }
 
-// Layer passive steps, superposed step ranges and the transfer box (one block).
-__global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
-                                                        BeamScalars *__restrict__ bs) {
+// Transfer box of one passive step (wrapper:5343-5395): 8 fan corners -> dose index in float,
+// floor/ceil, x start rounded down to 32; the launch covers whole 32x8 thread blocks from the
+// start, so the box end is rounded up to those blocks and clipped to the grid.
+__device__ void transfer_box(const FanGeom &g, int inside, int passive, int st[3], int en[3], int mi[3]) {
 #pragma clang fp contract(off)
-    __shared__ int s_max[2][4];
-    const int inside = bs->beamFirstInside;
-    int mp = INT_MIN, mk = INT_MIN, act = 0;
-    for (int l = threadIdx.x; l < g.numLayers; l += blockDim.x) {
-        int pas = ls[l].passiveRaw;  // wrapper:4984-4996
-        if (pas < inside + 1) pas = inside + 1;
-        if (pas < 0) pas = inside + 1;
-        if (pas > ls[l].afterLast) pas = ls[l].afterLast;
-        if (pas > g.K) pas = g.K;
-        ls[l].passive = pas;
-        // The reference samples its BEV volume at (step - beamFirstInside) while the volume
-        // holds absolute steps (wrapper:5308, 5379-5381); steps past passive - inside are
-        // never transferred, so they are not superposed here.
-        const int kEnd = pas > inside ? pas - inside : inside;
-        ls[l].kEnd = kEnd;
-        if (pas > inside) { mp = max(mp, pas); mk = max(mk, kEnd); ++act; }
-    }
-    mp = wave_max_i(mp);
-    mk = wave_max_i(mk);
-    int a2 = act;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) a2 += __shfl_xor(a2, o, 64);
-    if ((threadIdx.x & 63) == 0) { s_max[0][threadIdx.x >> 6] = mp; s_max[1][threadIdx.x >> 6] = mk; }
-    __shared__ int s_act[4];
-    if ((threadIdx.x & 63) == 0) s_act[threadIdx.x >> 6] = a2;
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    mp = max(max(s_max[0][0], s_max[0][1]), max(s_max[0][2], s_max[0][3]));
-    mk = max(max(s_max[1][0], s_max[1][1]), max(s_max[1][2], s_max[1][3]));
-    bs->maxPassive = mp;
-    bs->kEndMax = mk;
-    bs->activeLayers = s_act[0] + s_act[1] + s_act[2] + s_act[3];
-    if (mp <= inside) {
-        for (int q = 0; q < 3; ++q) { bs->boxStart[q] = 0; bs->boxEnd[q] = 0; bs->boxMax[q] = -1; }
-        return;
-    }
-    // transfer box (wrapper:5343-5395): 8 fan corners -> dose index, float arithmetic
     const float xv[2] = {-32.0f, float(g.sx + 32 - 1)};
     const float yv[2] = {-32.0f, float(g.sy + 32 - 1)};
-    const float zv[2] = {float(inside), float(mp - 1)};
+    const float zv[2] = {float(inside), float(passive - 1)};
     float mx[3] = {-1.f, -1.f, -1.f}, mn[3] = {100000.f, 100000.f, 100000.f};
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
@@ -497,16 +472,114 @@ __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars 
                     mn[r] = fminf(mn[r], q);
                 }
             }
-    int st[3], mi[3];
     st[0] = max((int(floorf(mn[0])) / 32) * 32, 0);
     st[1] = max(int(floorf(mn[1])), 0);
     st[2] = max(int(floorf(mn[2])), 0);
     for (int r = 0; r < 3; ++r) mi[r] = min(int(ceilf(mx[r])), g.doseDims[r] - 1);
-    for (int r = 0; r < 3; ++r) { bs->boxStart[r] = st[r]; bs->boxMax[r] = mi[r]; }
-    // the launch covers whole 32x8 thread blocks from the start; threads stop at the grid edge
-    bs->boxEnd[0] = min(st[0] + ((mi[0] - st[0] + 1 + 31) / 32) * 32, g.doseDims[0]);
-    bs->boxEnd[1] = min(st[1] + ((mi[1] - st[1] + 1 + 7) / 8) * 8, g.doseDims[1]);
-    bs->boxEnd[2] = mi[2] + 1;
+    en[0] = min(st[0] + ((mi[0] - st[0] + 1 + 31) / 32) * 32, g.doseDims[0]);
+    en[1] = min(st[1] + ((mi[1] - st[1] + 1 + 7) / 8) * 8, g.doseDims[1]);
+    en[2] = mi[2] + 1;
+    for (int r = 0; r < 3; ++r) if (en[r] < st[r]) en[r] = st[r];
+}
+
+// BEV plane sampled by dose voxel (x, y, z): the fan step minus beamFirstInside.
+__device__ __forceinline__ float sampled_plane(const FanGeom &g, int inside, float x, float y, float z) {
+    const float gz = g.d2g[6] * x + g.d2g[7] * y + g.d2g[8] * z + g.d2gOff[2];
+    return (gz - g.fc[2]) / g.fd[2] - float(inside);
+}
+
+// Layer passive steps, layer groups, group boxes and tail planes (one block of 256 threads).
+__global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
+                                                        GroupScalars *__restrict__ gs, BeamScalars *__restrict__ bs,
+                                                        int *__restrict__ tailPlaneGroup, int *__restrict__ tailPlaneK,
+                                                        int tailCap) {
+    __shared__ int s_pas[kMaxLayersDev];
+    __shared__ int s_first[kMaxLayersDev];
+    const int inside = bs->beamFirstInside;
+    const int L = g.numLayers;
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        int pas = ls[l].passiveRaw;  // wrapper:4984-4996
+        if (pas < inside + 1) pas = inside + 1;
+        if (pas < 0) pas = inside + 1;
+        if (pas > ls[l].afterLast) pas = ls[l].afterLast;
+        if (pas > g.K) pas = g.K;
+        ls[l].passive = pas;
+        s_pas[l] = pas > inside ? pas : -1;  // layers with no superposed step take no part
+    }
+    __syncthreads();
+    // groups = distinct passive steps in ascending order; group of a layer = rank of its step
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        int first = s_pas[l] >= 0 ? 1 : 0;
+        for (int q = 0; q < l && first; ++q) first = (s_pas[q] != s_pas[l]);
+        s_first[l] = first;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        int rank = -1;
+        if (s_pas[l] >= 0) {
+            rank = 0;
+            for (int q = 0; q < L; ++q) rank += (s_first[q] && s_pas[q] < s_pas[l]) ? 1 : 0;
+            if (s_first[l]) gs[rank].passive = s_pas[l];
+        }
+        ls[l].group = rank;
+    }
+    __syncthreads();
+    int m = 0;
+    for (int q = 0; q < L; ++q) m += s_first[q];
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        GroupScalars G = gs[j];
+        transfer_box(g, inside, G.passive, G.boxStart, G.boxEnd, G.boxMax);
+        // main BEV keeps steps below passive - inside - 1: no voxel outside the group's box
+        // samples them; the tail covers the rest of what voxels inside the box can sample.
+        G.tailStart = max(inside, G.passive - inside - 1);
+        float tmax = -1.0e30f;
+        for (int c = 0; c < 8; ++c) {
+            const float x = float((c & 1) ? G.boxEnd[0] - 1 : G.boxStart[0]);
+            const float y = float((c & 2) ? G.boxEnd[1] - 1 : G.boxStart[1]);
+            const float z = float((c & 4) ? G.boxEnd[2] - 1 : G.boxStart[2]);
+            tmax = fmaxf(tmax, sampled_plane(g, inside, x, y, z));
+        }
+        const int tailEnd = min(G.passive, int(floorf(tmax)) + 2);
+        G.tailCount = max(0, tailEnd - G.tailStart);
+        G.tailBase = 0;
+        gs[j] = G;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int base = 0, mp = INT_MIN, me = inside;
+        int st[3] = {INT_MAX, INT_MAX, INT_MAX}, en[3] = {INT_MIN, INT_MIN, INT_MIN}, mi[3] = {-1, -1, -1};
+        for (int j = 0; j < m; ++j) {
+            gs[j].tailBase = base;
+            for (int q = 0; q < gs[j].tailCount && base + q < tailCap; ++q) {
+                tailPlaneGroup[base + q] = j;
+                tailPlaneK[base + q] = gs[j].tailStart + q;
+            }
+            base += gs[j].tailCount;
+            mp = max(mp, gs[j].passive);
+            me = max(me, gs[j].tailStart);
+            for (int r = 0; r < 3; ++r) {
+                st[r] = min(st[r], gs[j].boxStart[r]);
+                en[r] = max(en[r], gs[j].boxEnd[r]);
+                mi[r] = max(mi[r], gs[j].boxMax[r]);
+            }
+        }
+        bs->numGroups = m;
+        bs->tailPlanes = min(base, tailCap);
+        bs->overflow = base > tailCap ? 1 : 0;
+        bs->maxPassive = mp;
+        bs->mainEndMax = me;
+        bs->activeLayers = m > 0 ? 1 : 0;
+        for (int r = 0; r < 3; ++r) {
+            bs->boxStart[r] = m > 0 ? st[r] : 0;
+            bs->boxEnd[r] = m > 0 ? en[r] : 0;
+            bs->boxMax[r] = mi[r];
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        const int j = ls[l].group;
+        ls[l].mainEnd = j >= 0 ? gs[j].tailStart : inside;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -589,25 +662,51 @@ __device__ __forceinline__ int class_radius(int c) {
     return rm[c];
 }
 
-// one thread per (step, tile)
+__device__ __forceinline__ bool in_tail(const GroupScalars &G, int k) {
+    return k >= G.tailStart && k < G.tailStart + G.tailCount;
+}
+
+// One thread per (step, tile): the item's launch class from the largest launch radius of its
+// live layers; which of its blocks (main, group tails) exist, with the block layout radius.
 __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict__ tileRad,
                                                const int *__restrict__ effRad, const LayerScalars *__restrict__ ls,
-                                               int *__restrict__ itemR, int *__restrict__ classCount,
-                                               int *__restrict__ classList, int maxItems) {
+                                               const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
+                                               int *__restrict__ itemR, int *__restrict__ tailItemR,
+                                               int *__restrict__ classCount, int *__restrict__ classList, int maxItems) {
     const int nt = (g.sx / 32) * (g.sy / 8);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
     if (t >= nt) return;
-    int R = -1;
+    int Rmax = -1;
+    bool mainLive = false;
     for (int l = 0; l < g.numLayers; ++l) {
         const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
-        if (code < 0 || !(code & 256) || k >= ls[l].kEnd) continue;
-        R = max(R, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
+        if (code < 0 || !(code & 256)) continue;
+        const int j = ls[l].group;
+        const bool isMain = k < ls[l].mainEnd;
+        if (!isMain && !(j >= 0 && in_tail(gs[j], k))) continue;
+        mainLive |= isMain;
+        Rmax = max(Rmax, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
     }
+    const int RM = Rmax >= 0 ? class_radius(radius_class(Rmax)) : -1;
     const int slot = k * nt + t;
-    itemR[slot] = R;
-    if (R >= 0) {
-        const int c = radius_class(R);
+    itemR[slot] = mainLive ? RM : -1;
+    const int m = bs->numGroups;
+    for (int j = 0; j < m; ++j) {
+        const GroupScalars G = gs[j];
+        if (!in_tail(G, k)) continue;
+        const int q = G.tailBase + (k - G.tailStart);
+        if (q >= bs->tailPlanes) continue;
+        bool live = false;
+        for (int l = 0; l < g.numLayers && !live; ++l) {
+            if (ls[l].group != j) continue;
+            const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
+            live = code >= 0 && (code & 256);
+        }
+        tailItemR[size_t(q) * nt + t] = live ? RM : -1;
+    }
+    if (Rmax >= 0) {
+        const int c = radius_class(Rmax);
         const int pos = atomicAdd(&classCount[c], 1);
         classList[size_t(c) * maxItems + pos] = slot;
     }
@@ -616,163 +715,217 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
 // ---------------------------------------------------------------------------------------
 // superposition
 // ---------------------------------------------------------------------------------------
-// One work-group (4 waves) per (step, tile) item. Per live layer, each thread computes the
-// erf-difference vector of its pixel into LDS; wave w then scatters input rows 2w, 2w+1:
-// lane = output column (64 per chunk), registers = output rows. The pixel's vector is
-// read with a wave-uniform LDS address (broadcast) and the column factor with a per-lane
-// address, so every FMA works on registers. The four wave blocks are combined in a fixed
-// order and the item's block is written to its own slot: no atomics, bitwise repeatable.
+// One work-group (4 waves) per (step, tile) item; wave w owns input rows 2w, 2w+1 (its 64
+// lanes are its 64 pixels). Per live layer each lane computes the erf-difference vector of
+// its pixel in registers. The wave then walks its live pixels (ballot mask, scalar loop):
+// the pixel's vector is broadcast with readlane into scalar registers, the column factor
+// d * e[|X - px|] is read per lane from the wave's own LDS rows, and lane X accumulates
+// output column X for all 2R+1 output rows in registers. Waves never share LDS inside a
+// block pass, so there is no block barrier per layer. A block (main, or one group tail) is
+// combined across the four waves in a fixed order and written to its own slot.
 template <int RM>
 __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__restrict__ iddSigma,
                                                    const int *__restrict__ tileRad, const int *__restrict__ effRad,
                                                    const LayerScalars *__restrict__ ls,
+                                                   const GroupScalars *__restrict__ gs,
+                                                   const BeamScalars *__restrict__ bs,
                                                    const int *__restrict__ classCount,
                                                    const int *__restrict__ classList, int cls, int maxItems,
-                                                   float *__restrict__ slots) {
+                                                   float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
     constexpr int NC = (32 + 2 * RM + 63) / 64;
-    constexpr int EW = ((RM + 1) + 3) & ~3;
+    constexpr int EW = RM + 1;
     constexpr int BW = 32 + 2 * RM, BH = 8 + 2 * RM;
     constexpr int NR = 2 + 2 * RM;
-    constexpr int LDS_E = 256 * EW, LDS_B = BH * BW;
-    __shared__ float s_buf[LDS_E > LDS_B ? LDS_E : LDS_B];
-    __shared__ float s_d[256];
+    __shared__ float s_e[4][64 * EW];
+    __shared__ float s_B[BH * BW];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
     const int count = classCount[cls];
     const int *list = classList + size_t(cls) * maxItems;
+    const int m = bs->numGroups;
+    float *Ew = s_e[w];
     for (int it = blockIdx.x; it < count; it += gridDim.x) {
         const int slot = list[it];
         const int k = slot / nt, t = slot % nt;
         const int tx = t % ntx, ty = t / ntx;
-        float acc[NC][NR];
+        const int gx = tx * 32 + (lane & 31), gy = ty * 8 + 2 * w + (lane >> 5);
+        const bool pixIn = gx < g.rx && gy < g.ry;
+        // blocks: b = -1 main, b = j tail of group j
+        for (int b = -1; b < m; ++b) {
+            if (b >= 0 && !in_tail(gs[b], k)) continue;
+            float acc[NC][NR];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+            for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int q = 0; q < NR; ++q) acc[c][q] = 0.0f;
-        for (int l = 0; l < g.numLayers; ++l) {
-            const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
-            if (code < 0 || !(code & 256) || k >= ls[l].kEnd) continue;
-            const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
-            // phase 1: erf differences of this thread's pixel
-            {
-                const int px = tid & 31, py = tid >> 5;
-                const int gx = tx * 32 + px, gy = ty * 8 + py;
+                for (int q = 0; q < NR; ++q) acc[c][q] = 0.0f;
+            bool any = false;
+            for (int l = 0; l < g.numLayers; ++l) {
+                const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
+                if (code < 0 || !(code & 256)) continue;
+                const bool isMain = k < ls[l].mainEnd;
+                if (b < 0 ? !isMain : (isMain || ls[l].group != b)) continue;
+                const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
+                any = true;
                 float d = 0.0f, r = PBS_INF;
-                if (gx < g.rx && gy < g.ry) {
+                if (pixIn) {
                     const float2 v = iddSigma[((size_t(l) * g.K + k) * g.sy + gy) * g.sx + gx];
                     d = v.x;
                     r = v.y;
                 }
-                float *E = s_buf + tid * EW;
-                if (d != 0.0f) {
+                // erf differences of this lane's pixel (complete_superposition.cu:298-307),
+                // written to the lane's LDS row, then held in registers for readlane
+                {
                     float en = erff(r * 0.5f), eo = -en;
-                    for (int i = 0; i <= R; ++i) {
-                        E[i] = 0.5f * (en - eo);
-                        eo = en;
-                        en = erff(r * (float(i) + 1.5f));
+                    for (int i = 0; i <= RM; ++i) {
+                        float v = 0.0f;
+                        if (i <= R) {
+                            v = 0.5f * (en - eo);
+                            eo = en;
+                            if (i < R) en = erff(r * (float(i) + 1.5f));
+                        }
+                        Ew[lane * EW + i] = v;
                     }
                 }
-                s_d[tid] = d;
-            }
-            __syncthreads();
-            // phase 2: scatter the wave's two input rows into register accumulators
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                float e[RM + 1];
 #pragma unroll
-            for (int half = 0; half < 2; ++half) {
-                const int prow = 2 * w + half;
-                for (int px = 0; px < 32; ++px) {
-                    const int p = prow * 32 + px;
-                    const float d = s_d[p];
-                    if (d == 0.0f) continue;
-                    const float *Ep = s_buf + p * EW;
-                    float a[NC];
+                for (int i = 0; i <= RM; ++i) e[i] = Ew[lane * EW + i];
+                const uint64_t live = __ballot(d > 0.0f || d < 0.0f);
+                const int dbits = __float_as_int(d);
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    uint64_t msk = half ? (live >> 32) : (live & 0xffffffffull);
+                    while (msk) {
+                        const int px = __builtin_ctzll(msk);
+                        msk &= msk - 1;
+                        const int p = px + 32 * half;
+                        const float dp = __int_as_float(__builtin_amdgcn_readlane(dbits, p));
+                        float a[NC];
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            const int adx = abs(lane + 64 * c - RM - px);
+                            a[c] = adx <= R ? dp * Ew[p * EW + min(adx, RM)] : 0.0f;
+                        }
+#pragma unroll
+                        for (int i = 0; i <= RM; ++i) {
+                            if (i <= R) {
+                                const float ei =
+                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e[i]), p));
+#pragma unroll
+                                for (int c = 0; c < NC; ++c) {
+                                    acc[c][half + RM + i] += a[c] * ei;
+                                    if (i) acc[c][half + RM - i] += a[c] * ei;
+                                }
+                            }
+                        }
+                    }
+                }
+                // the next layer rewrites this wave's rows: keep its stores after these loads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            if (!any) continue;  // uniform across the block
+            // combine the four wave blocks in a fixed order, then write the block's slot
+            for (int i = tid; i < BH * BW; i += 256) s_B[i] = 0.0f;
+            __syncthreads();
+            for (int ww = 0; ww < 4; ++ww) {
+                if (w == ww) {
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
-                        const int X = lane + 64 * c - RM;
-                        const int adx = abs(X - px);
-                        a[c] = adx <= R ? d * Ep[min(adx, RM)] : 0.0f;
-                    }
+                        const int col = lane + 64 * c;
+                        if (col < BW) {
 #pragma unroll
-                    for (int jj = 0; jj <= 2 * RM; ++jj) {
-                        const int aj = jj < RM ? RM - jj : jj - RM;
-                        if (aj > R) continue;
-                        const float ej = Ep[aj];
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) acc[c][half + jj] += a[c] * ej;
+                            for (int q = 0; q < NR; ++q) s_B[(2 * w + q) * BW + col] += acc[c][q];
+                        }
                     }
                 }
+                __syncthreads();
             }
+            float *out;
+            if (b < 0) {
+                out = mainSlots + size_t(slot) * kSlotFloats;
+            } else {
+                const int q = gs[b].tailBase + (k - gs[b].tailStart);
+                out = (q < bs->tailPlanes) ? tailSlots + (size_t(q) * nt + t) * kSlotFloats : nullptr;
+            }
+            if (out)
+                for (int i = tid; i < BH * BW; i += 256) out[(i / BW) * kSlotPitch + (i % BW)] = s_B[i];
             __syncthreads();
         }
-        // combine the four wave blocks in a fixed order, then write the item's slot
-        float *B = s_buf;
-        for (int i = tid; i < BH * BW; i += 256) B[i] = 0.0f;
-        __syncthreads();
-        for (int ww = 0; ww < 4; ++ww) {
-            if (w == ww) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    const int col = lane + 64 * c;
-                    if (col < BW) {
-#pragma unroll
-                        for (int q = 0; q < NR; ++q) B[(2 * w + q) * BW + col] += acc[c][q];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        float *out = slots + size_t(slot) * kSlotFloats;
-        for (int i = tid; i < BH * BW; i += 256) out[(i / BW) * kSlotPitch + (i % BW)] = B[i];
-        __syncthreads();
     }
 }
 
-// BEV dose [bevZ][sy+64][sx+64] = sum of the item blocks covering each voxel, in (ty, tx) order.
+// Sum of the item blocks covering BEV pixel (X, Y) of one plane, in (ty, tx) order.
+__device__ __forceinline__ float gather_blocks(const FanGeom &g, int X, int Y, const int *__restrict__ layoutR,
+                                               const float *__restrict__ slots) {
+    const int ntx = g.sx / 32, nty = g.sy / 8;
+    const int xr = X - 32, yr = Y - 32;
+    const int tx0 = max(0, (xr - 63 + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + 32 + 64 * 32) / 32 - 64);
+    const int ty0 = max(0, (yr - 39 + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + 32 + 8 * 64) / 8 - 64);
+    float sum = 0.0f;
+    for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) {
+            const int tt = ty * ntx + tx;
+            const int RM = layoutR[tt];
+            if (RM < 0) continue;
+            const int col = xr - 32 * tx + RM, row = yr - 8 * ty + RM;
+            if (col >= 0 && col < 32 + 2 * RM && row >= 0 && row < 8 + 2 * RM)
+                sum += slots[size_t(tt) * kSlotFloats + row * kSlotPitch + col];
+        }
+    return sum;
+}
+
+// Main BEV [K][sy+64][sx+64] and tail BEV planes [tailCap][sy+64][sx+64].
 __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__restrict__ bs,
-                                               const int *__restrict__ itemR, const float *__restrict__ slots,
-                                               float *__restrict__ bev, int bevZ) {
+                                               const int *__restrict__ itemR, const float *__restrict__ mainSlots,
+                                               float *__restrict__ bev, const int *__restrict__ tailItemR,
+                                               const float *__restrict__ tailSlots, float *__restrict__ tailBev) {
     const int bx = g.sx + 64, by = g.sy + 64;
     const int X = blockIdx.x * 64 + (threadIdx.x & 63);
     const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (X >= bx || Y >= by || k >= bevZ) return;
-    const int ntx = g.sx / 32, nty = g.sy / 8, nt = ntx * nty;
-    float sum = 0.0f;
-    if (k >= bs->beamFirstInside && k < bs->kEndMax) {
-        const int xr = X - 32, yr = Y - 32;
-        // tiles whose block can reach (xr, yr): |offset| <= 32 beyond the tile
-        const int tx0 = max(0, (xr - 63 + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + 32 + 64 * 32) / 32 - 64);
-        const int ty0 = max(0, (yr - 39 + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + 32 + 8 * 64) / 8 - 64);
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                const int slot = k * nt + ty * ntx + tx;
-                const int R = itemR[slot];
-                if (R < 0) continue;
-                const int RM = class_radius(radius_class(R));
-                const int col = xr - 32 * tx + RM, row = yr - 8 * ty + RM;
-                if (col >= 0 && col < 32 + 2 * RM && row >= 0 && row < 8 + 2 * RM)
-                    sum += slots[size_t(slot) * kSlotFloats + row * kSlotPitch + col];
-            }
+    const int z = blockIdx.z;
+    if (X >= bx || Y >= by) return;
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    if (z < g.K) {
+        float sum = 0.0f;
+        if (z >= bs->beamFirstInside && z < bs->mainEndMax)
+            sum = gather_blocks(g, X, Y, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats);
+        bev[(size_t(z) * by + Y) * bx + X] = sum;
+    } else {
+        // tail planes, strided over the gridDim.z - K tail block rows
+        const int tz = int(gridDim.z) - g.K;
+        for (int q = z - g.K; q < bs->tailPlanes; q += tz) {
+            const float sum =
+                gather_blocks(g, X, Y, tailItemR + size_t(q) * nt, tailSlots + size_t(q) * nt * kSlotFloats);
+            tailBev[(size_t(q) * by + Y) * bx + X] = sum;
+        }
     }
-    bev[(size_t(k) * by + Y) * bx + X] = sum;
 }
 
 // ---------------------------------------------------------------------------------------
 // BEV -> dose
 // ---------------------------------------------------------------------------------------
-// Thread per dose (x, y) column inside the transfer box; loops over z (one owner per voxel).
+// Thread per dose (x, y) column inside the union box; loops over z (one owner per voxel).
 // The dose-index -> fan-index map is the reference's factored form
-// (transfer_param_struct_div3.cuh:34-44, transfer_param_struct_div3_impl.cu:14-34).
+// (transfer_param_struct_div3.cuh:34-44, transfer_param_struct_div3_impl.cu:14-34). Every
+// voxel samples the main BEV; a voxel inside a group's box also samples that group's tail.
 __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
-                                                  const float *__restrict__ bev, int bevZ, float *__restrict__ dose) {
+                                                  const GroupScalars *__restrict__ gs, const float *__restrict__ bev,
+                                                  const float *__restrict__ tailBev, float *__restrict__ dose) {
     const int x = bs->boxStart[0] + int(blockIdx.x * 32 + (threadIdx.x & 31));
     const int y = bs->boxStart[1] + int(blockIdx.y * 8 + (threadIdx.x >> 5));
     if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1]) return;
     const int inside = bs->beamFirstInside;
+    const int m = bs->numGroups;
     const float goffZ = ta.globalOffset[2] - float(inside);
-    const float bx = float(g.sx + 64), by = float(g.sy + 64);
-    float sx0 = float(x) * ta.coefI[0] + float(y) * ta.coefJ[0] + ta.coefOffset[0];
-    float sy0 = float(x) * ta.coefI[1] + float(y) * ta.coefJ[1] + ta.coefOffset[1];
-    float sz0 = float(x) * ta.coefI[2] + float(y) * ta.coefJ[2] + ta.coefOffset[2];
+    const int bx = g.sx + 64, by = g.sy + 64;
+    const float fbx = float(bx), fby = float(by);
+    const float sx0 = float(x) * ta.coefI[0] + float(y) * ta.coefJ[0] + ta.coefOffset[0];
+    const float sy0 = float(x) * ta.coefI[1] + float(y) * ta.coefJ[1] + ta.coefOffset[1];
+    const float sz0 = float(x) * ta.coefI[2] + float(y) * ta.coefJ[2] + ta.coefOffset[2];
     const size_t sliceStride = size_t(g.doseDims[0]) * g.doseDims[1];
     float *res = dose + size_t(bs->boxStart[2]) * sliceStride + size_t(y) * g.doseDims[0] + x;
     for (int z = bs->boxStart[2]; z < bs->boxEnd[2]; ++z, res += sliceStride) {
@@ -784,8 +937,19 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
         const float fxI = px + ta.globalOffset[0], fyI = py + ta.globalOffset[1], fzI = pz + goffZ;
         // prim_transf_kernel.cu:28-40: texel coordinate = fan index + 1/2; negatives skipped
         if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) continue;
-        if (fxI > bx || fyI > by || fzI > float(bevZ)) continue;  // wholly outside: border value 0
-        const float v = trilinear_border(bev, g.sx + 64, g.sy + 64, bevZ, fxI, fyI, fzI);
+        if (fxI > fbx || fyI > fby) continue;  // wholly outside the BEV: border value 0
+        float v = 0.0f;
+        if (fzI <= float(g.K)) v = trilinear_border(bev, bx, by, g.K, fxI, fyI, fzI);
+        for (int j = 0; j < m; ++j) {
+            const GroupScalars &G = gs[j];
+            if (G.tailCount <= 0) continue;
+            const float zl = fzI - float(G.tailStart);
+            if (!(zl > -1.0f && zl < float(G.tailCount))) continue;
+            if (x < G.boxStart[0] || x >= G.boxEnd[0] || y < G.boxStart[1] || y >= G.boxEnd[1] ||
+                z < G.boxStart[2] || z >= G.boxEnd[2])
+                continue;
+            v += trilinear_border(tailBev + size_t(G.tailBase) * bx * by, bx, by, G.tailCount, fxI, fyI, zl);
+        }
         if (v > 0.0f) *res += v;
     }
 }
@@ -847,52 +1011,60 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, BeamScalars *bs,
-                                     int *tileRad, int *hist, int *effRad, int *itemR, int *classCount,
+hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
+                                     BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
+                                     int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int *classList, int maxItems, hipStream_t s) {
-    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, bs);
+    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap);
     const int nt = (g.sx / 32) * (g.sy / 8);
     hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad);
-    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, itemR,
-                       classCount, classList, maxItems);
+    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
+                       tailItemR, classCount, classList, maxItems);
     return hipGetLastError();
 }
 
 template <int RM>
 static void launch_class(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
-                         const LayerScalars *ls, const int *classCount, const int *classList, int cls, int maxItems,
-                         float *slots, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, classCount,
-                       classList, cls, maxItems, slots);
+                         const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs, const int *classCount,
+                         const int *classList, int cls, int maxItems, float *mainSlots, float *tailSlots, int grid,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, gs, bs,
+                       classCount, classList, cls, maxItems, mainSlots, tailSlots);
 }
 
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
-                            const LayerScalars *ls, const int *classCount, const int *classList, int maxItems,
-                            float *slots, int gridCap, hipStream_t s) {
+                            const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
+                            const int *classCount, const int *classList, int maxItems, float *mainSlots,
+                            float *tailSlots, int gridCap, hipStream_t s) {
     const int grid = maxItems < gridCap ? maxItems : gridCap;
-    launch_class<2>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 0, maxItems, slots, grid, s);
-    launch_class<4>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 1, maxItems, slots, grid, s);
-    launch_class<6>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 2, maxItems, slots, grid, s);
-    launch_class<8>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 3, maxItems, slots, grid, s);
-    launch_class<12>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 4, maxItems, slots, grid, s);
-    launch_class<16>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 5, maxItems, slots, grid, s);
-    launch_class<24>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 6, maxItems, slots, grid, s);
-    launch_class<32>(g, iddSigma, tileRad, effRad, ls, classCount, classList, 7, maxItems, slots, grid, s);
+#define PBS_CLASS(RMV, C)                                                                                          \
+    launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxItems, mainSlots,    \
+                      tailSlots, grid, s)
+    PBS_CLASS(2, 0);
+    PBS_CLASS(4, 1);
+    PBS_CLASS(6, 2);
+    PBS_CLASS(8, 3);
+    PBS_CLASS(12, 4);
+    PBS_CLASS(16, 5);
+    PBS_CLASS(24, 6);
+    PBS_CLASS(32, 7);
+#undef PBS_CLASS
     return hipGetLastError();
 }
 
-hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *slots, float *bev,
-                        int bevZ, hipStream_t s) {
+hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
+                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, hipStream_t s) {
     const int bx = g.sx + 64, by = g.sy + 64;
-    hipLaunchKernelGGL(k_merge, dim3((bx + 63) / 64, (by + 3) / 4, bevZ), dim3(256), 0, s, g, bs, itemR, slots, bev,
-                       bevZ);
+    const int tz = tailCap < 64 ? tailCap : 64;
+    hipLaunchKernelGGL(k_merge, dim3((bx + 63) / 64, (by + 3) / 4, g.K + tz), dim3(256), 0, s, g, bs, itemR,
+                       mainSlots, bev, tailItemR, tailSlots, tailBev);
     return hipGetLastError();
 }
 
-hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const float *bev, int bevZ,
-                           float *dose, int gridX, int gridY, hipStream_t s) {
-    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY), dim3(256), 0, s, g, ta, bs, bev, bevZ, dose);
+hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
+                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, hipStream_t s) {
+    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY), dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose);
     return hipGetLastError();
 }
 

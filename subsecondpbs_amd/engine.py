@@ -125,5 +125,11 @@ class Engine:
             "sourceDist": tuple(ints.sourceDist),
             "energyIdx": np.array(ints.energyIdx[:L], np.float32), "scaleFact": np.array(ints.scaleFact[:L], np.float32),
             "peakDepth": np.array(ints.peakDepth[:L], np.float32),
+            "numGroups": ints.numGroups, "tailPlanes": ints.tailPlanes, "tailOverflow": ints.tailOverflow,
+            "mainEndMax": ints.mainEndMax,
+            "mainEnd": np.array(ints.mainEnd[:L], np.int32), "layerGroup": np.array(ints.layerGroup[:L], np.int32),
+            "groupPassive": np.array(ints.groupPassive[:ints.numGroups], np.int32),
+            "groupTailStart": np.array(ints.groupTailStart[:ints.numGroups], np.int32),
+            "groupTailCount": np.array(ints.groupTailCount[:ints.numGroups], np.int32),
         }
         return out
