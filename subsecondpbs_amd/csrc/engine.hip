@@ -82,7 +82,8 @@ struct pbs_engine {
     DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, slots, bev, cpb;
     DevBuf<float> tailSlots, tailBev;
     DevBuf<float2> iddSigma;
-    DevBuf<int> fi, fo, firstPassive, tileRad, hist, effRad, itemR, classCount, classList;
+    DevBuf<int> fi, fo, firstPassive, tileRad, hist, effRad, itemR, classCount;
+    DevBuf<int2> classList;
     DevBuf<int> tailPlaneGroup, tailPlaneK, tailItemR;
     DevBuf<GroupScalars> groupScal;
     DevBuf<LayerHost> layerHost;
@@ -96,6 +97,10 @@ struct pbs_engine {
     // host staging of the last beam's uploads; the next beam waits for uploadDone before reuse
     std::vector<LayerHost> hostLayers;
     hipEvent_t uploadDone = nullptr;
+    // radius classes of the superposition run concurrently on the engine stream and these
+    // side streams (fork after the work lists, join before the merge)
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t forkEv = nullptr, joinEv[kSideStreams] = {};
     bool uploadPending = false;
     // optional per-stage timing: kStages + 1 events per beam, resolved by pbs_engine_stage_times
     bool profiling = false;
@@ -147,7 +152,7 @@ void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
 
 // Host worst-case transfer launch extents: the box for steps [0, K-1] contains every
 // layer's box (the fan frustum grows with its depth range).
-void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &gy) {
+void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &gy, int &gz) {
     const float xv[2] = {-32.0f, float(p.superpDims[0] + 32 - 1)};
     const float yv[2] = {-32.0f, float(p.superpDims[1] + 32 - 1)};
     const float zv[2] = {0.0f, float(p.steps - 1)};
@@ -169,6 +174,9 @@ void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &
     const int ey = std::min(int(std::ceil(mx[1])) + 1, doseDims[1] - 1);
     gx = std::max(1, (ex - sx + 1 + 31) / 32 + 1);
     gy = std::max(1, (ey - sy + 1 + 7) / 8 + 1);
+    const int sz = std::max(int(std::floor(mn[2])) - 1, 0);
+    const int ez = std::min(int(std::ceil(mx[2])) + 1, doseDims[2] - 1);
+    gz = std::max(1, ez - sz + 1);
 }
 
 void fill_transfer_args(const BeamPlan &p, TransferArgs &t) {
@@ -236,9 +244,17 @@ pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
     std::unique_ptr<pbs_engine> e(new pbs_engine());
     e->device = gpuId;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->forkEv, hipEventDisableTiming) != hipSuccess) {
         set_error(PBS_E_DEVICE, "hipStreamCreate failed");
         return nullptr;
+    }
+    for (int i = 0; i < kSideStreams; ++i) {
+        if (hipStreamCreateWithFlags(&e->side[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->joinEv[i], hipEventDisableTiming) != hipSuccess) {
+            set_error(PBS_E_DEVICE, "hipStreamCreate failed");
+            return nullptr;
+        }
     }
     const size_t nC = size_t(en->nEnergies) * en->nEnergySamples;
     auto up = [&](DevBuf<float> &d, const float *h, size_t n) {
@@ -275,6 +291,14 @@ void pbs_engine_destroy(pbs_engine *e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     if (e->uploadDone) (void)hipEventDestroy(e->uploadDone);
+    for (int i = 0; i < kSideStreams; ++i) {
+        if (e->side[i]) {
+            (void)hipStreamSynchronize(e->side[i]);
+            (void)hipStreamDestroy(e->side[i]);
+        }
+        if (e->joinEv[i]) (void)hipEventDestroy(e->joinEv[i]);
+    }
+    if (e->forkEv) (void)hipEventDestroy(e->forkEv);
     for (auto &m : e->evBeams)
         for (hipEvent_t ev : m) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : e->evPool) (void)hipEventDestroy(ev);
@@ -365,6 +389,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     // <= (K + 1) / 2 steps long, and there are at most min(L, K) groups
     const int tailCap = std::min(L, K) * std::max(1, (K + 1) / 2);
     const size_t bevPlane = size_t(sx + 64) * (sy + 64);
+    const int maxUnits = maxItems + tailCap * nt;  // main blocks + tail blocks
     const bool subMode = p.subspotConv && !p.legacyPath;
     const int nSub = subMode ? beam->maxSubspotsPerLayer : 1;
     // buffers
@@ -381,7 +406,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(e->effRad.ensure(size_t(L) * (kMaxRadDev + 2)));
     HIP_TRY(e->itemR.ensure(maxItems));
     HIP_TRY(e->classCount.ensure(16));
-    HIP_TRY(e->classList.ensure(size_t(kNumClasses) * maxItems));
+    HIP_TRY(e->classList.ensure(size_t(kNumClasses) * maxUnits));
     HIP_TRY(e->slots.ensure(size_t(maxItems) * kSlotFloats));
     HIP_TRY(e->bev.ensure(size_t(K) * bevPlane));
     HIP_TRY(e->groupScal.ensure(L));
@@ -474,22 +499,32 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->groupScal.p, e->beamScal.p,
                                       e->tailPlaneGroup.p, e->tailPlaneK.p, tailCap, e->tileRad.p, e->hist.p,
                                       e->effRad.p, e->itemR.p, e->tailItemR.p, e->classCount.p, e->classList.p,
-                                      maxItems, s));
+                                      maxUnits, s));
     // 5. superposition into per-item blocks, then the deterministic merge into the BEV volume
     HIP_TRY(mark());
+    HIP_TRY(hipEventRecord(e->forkEv, s));
+    hipStream_t streams[1 + kSideStreams] = {s};
+    for (int i = 0; i < kSideStreams; ++i) {
+        HIP_TRY(hipStreamWaitEvent(e->side[i], e->forkEv, 0));
+        streams[1 + i] = e->side[i];
+    }
     HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->groupScal.p,
-                             e->beamScal.p, e->classCount.p, e->classList.p, maxItems, e->slots.p, e->tailSlots.p,
-                             2048, s));
+                             e->beamScal.p, e->classCount.p, e->classList.p, maxUnits, e->slots.p, e->tailSlots.p,
+                             4096, streams));
+    for (int i = 0; i < kSideStreams; ++i) {
+        HIP_TRY(hipEventRecord(e->joinEv[i], e->side[i]));
+        HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
+    }
     HIP_TRY(mark());
     HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, e->tailItemR.p, e->tailSlots.p,
                          e->tailBev.p, tailCap, s));
     // 6. BEV -> dose
     TransferArgs ta{};
     fill_transfer_args(p, ta);
-    int gx = 1, gy = 1;
-    host_transfer_grid(p, e->doseDims, gx, gy);
+    int gx = 1, gy = 1, gz = 1;
+    host_transfer_grid(p, e->doseDims, gx, gy, gz);
     HIP_TRY(mark());
-    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->groupScal.p, e->bev.p, e->tailBev.p, doseDev, gx, gy, s));
+    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->groupScal.p, e->bev.p, e->tailBev.p, doseDev, gx, gy, gz, s));
     HIP_TRY(mark());
     if (e->profiling) e->evBeams.push_back(std::move(marks));
     e->lastMaxItems = maxItems;

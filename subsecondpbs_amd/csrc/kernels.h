@@ -13,6 +13,7 @@ constexpr int kSlotPitch = 96;                    // 32 + 2 * 32 columns
 constexpr int kSlotFloats = kSlotPitch * 72;      // (8 + 2 * 32) rows
 constexpr int kNumClasses = 8;
 constexpr int kMaxLayersDev = 256;
+constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
 
 struct ConvArgs {
     int rx, ry, spotX, spotY, nSub, subspotMode;
@@ -55,14 +56,15 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
-                                     int *classList, int maxItems, hipStream_t s);
+                                     int2 *classList, int maxUnits, hipStream_t s);
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
-                            const int *classCount, const int *classList, int maxItems, float *mainSlots,
-                            float *tailSlots, int gridCap, hipStream_t s);
+                            const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,
+                            float *tailSlots, int gridCap, const hipStream_t streams[1 + kSideStreams]);
 hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
                         const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, hipStream_t s);
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
-                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, hipStream_t s);
+                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, int gridZ,
+                           hipStream_t s);
 
 }  // namespace pbs

@@ -666,13 +666,36 @@ __device__ __forceinline__ bool in_tail(const GroupScalars &G, int k) {
     return k >= G.tailStart && k < G.tailStart + G.tailCount;
 }
 
+// Appends a work unit to its class list; lanes of a wave that append to the same class share
+// one counter update.
+__device__ __forceinline__ void append_unit(int *__restrict__ classCount, int2 *__restrict__ classList, int maxUnits,
+                                            int c, int2 unit) {
+    const uint64_t active = __ballot(1);
+    for (int cc = 0; cc < kNumClasses; ++cc) {
+        const uint64_t peers = __ballot(c == cc);
+        if (!peers) continue;
+        const int leader = __builtin_ctzll(peers);
+        const int lane = __lane_id();
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&classCount[cc], __popcll(peers));
+        base = __shfl(base, leader);
+        if (c == cc) {
+            const int pos = base + __popcll(peers & ((uint64_t(1) << lane) - 1));
+            if (pos < maxUnits) classList[size_t(cc) * maxUnits + pos] = unit;
+        }
+    }
+    (void)active;
+}
+
 // One thread per (step, tile): the item's launch class from the largest launch radius of its
-// live layers; which of its blocks (main, group tails) exist, with the block layout radius.
+// live layers, the block layout radius of its main block and group tail blocks (-1 when a
+// block is empty), and one work unit (slot, block) per non-empty block in its class list.
 __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict__ tileRad,
                                                const int *__restrict__ effRad, const LayerScalars *__restrict__ ls,
                                                const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
                                                int *__restrict__ itemR, int *__restrict__ tailItemR,
-                                               int *__restrict__ classCount, int *__restrict__ classList, int maxItems) {
+                                               int *__restrict__ classCount, int2 *__restrict__ classList,
+                                               int maxUnits) {
     const int nt = (g.sx / 32) * (g.sy / 8);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -688,9 +711,11 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
         mainLive |= isMain;
         Rmax = max(Rmax, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
     }
-    const int RM = Rmax >= 0 ? class_radius(radius_class(Rmax)) : -1;
+    const int c = Rmax >= 0 ? radius_class(Rmax) : 0;
+    const int RM = Rmax >= 0 ? class_radius(c) : -1;
     const int slot = k * nt + t;
     itemR[slot] = mainLive ? RM : -1;
+    if (mainLive) append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
     const int m = bs->numGroups;
     for (int j = 0; j < m; ++j) {
         const GroupScalars G = gs[j];
@@ -704,25 +729,91 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
             live = code >= 0 && (code & 256);
         }
         tailItemR[size_t(q) * nt + t] = live ? RM : -1;
-    }
-    if (Rmax >= 0) {
-        const int c = radius_class(Rmax);
-        const int pos = atomicAdd(&classCount[c], 1);
-        classList[size_t(c) * maxItems + pos] = slot;
+        if (live) append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
     }
 }
 
 // ---------------------------------------------------------------------------------------
 // superposition
 // ---------------------------------------------------------------------------------------
-// One work-group (4 waves) per (step, tile) item; wave w owns input rows 2w, 2w+1 (its 64
-// lanes are its 64 pixels). Per live layer each lane computes the erf-difference vector of
-// its pixel in registers. The wave then walks its live pixels (ballot mask, scalar loop):
-// the pixel's vector is broadcast with readlane into scalar registers, the column factor
-// d * e[|X - px|] is read per lane from the wave's own LDS rows, and lane X accumulates
-// output column X for all 2R+1 output rows in registers. Waves never share LDS inside a
-// block pass, so there is no block barrier per layer. A block (main, or one group tail) is
-// combined across the four waves in a fixed order and written to its own slot.
+// Superposition of one item block by one wave.
+//
+// Lane c*64 + lane owns output column X = c*64 + lane - RM of the block (tile coordinates;
+// the block spans X in [-RM, 32 + RM)) and keeps every output row of that column in
+// registers. Sources (the tile's 32x8 pixels of one layer) are staged in LDS as
+// [d, e_0 .. e_R] per pixel. A sweep visits each source row in 2R+1 phases; in phase phi a
+// lane takes the one source x_s = phi (mod 2R+1) of the row with |X - x_s| <= R, so the
+// sources of one phase have disjoint footprints and all of them are applied by one pass of
+// 2R+1 FMAs per lane (d e[|X - x_s|] e[|dy|] into rows y_s + dy). Every output is a sum in a
+// fixed order: the result is bitwise repeatable.
+constexpr int pad4(int n) { return (n + 3) / 4 * 4; }
+
+template <int R, int RM, int Y0>
+__device__ __forceinline__ void sweep_rows(float (&acc)[(32 + 2 * RM + 63) / 64][8 + 2 * RM],
+                                           const float *__restrict__ tab, unsigned rowLive, int lane) {
+    constexpr int NC = (32 + 2 * RM + 63) / 64;
+    constexpr int EWp = pad4(RM + 2);
+    constexpr int S = 2 * R + 1;
+    constexpr int NE4 = (R + 2 + 3) / 4;  // float4 reads covering [d, e_0 .. e_R]
+    int rem[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int u = c * 64 + lane - RM + R + S * 64;  // X + R, made positive
+        rem[c] = u % S;
+    }
+#pragma unroll 1
+    for (int phi = 0; phi < S; ++phi) {
+        int base[NC], adx[NC];
+        bool valid[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int xs = c * 64 + lane - RM + R - rem[c];
+            valid[c] = xs >= 0 && xs < 32;
+            base[c] = (valid[c] ? xs : 0) * EWp;
+            adx[c] = rem[c] > R ? rem[c] - R : R - rem[c];
+        }
+#pragma unroll
+        for (int yy = 0; yy < 4; ++yy) {
+            if (!(rowLive & (1u << yy))) continue;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const float *src = tab + yy * 32 * EWp + base[c];
+                float E[NE4 * 4];
+#pragma unroll
+                for (int q = 0; q < NE4; ++q) {
+                    const float4 v = *reinterpret_cast<const float4 *>(src + 4 * q);
+                    E[4 * q] = v.x; E[4 * q + 1] = v.y; E[4 * q + 2] = v.z; E[4 * q + 3] = v.w;
+                }
+                const float a = valid[c] ? E[0] * src[1 + adx[c]] : 0.0f;
+#pragma unroll
+                for (int i = -R; i <= R; ++i) acc[c][Y0 + yy + RM + i] += a * E[1 + (i < 0 ? -i : i)];
+            }
+            // one source row's weights live at a time (register budget, occupancy)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rem[c] = rem[c] == 0 ? S - 1 : rem[c] - 1;
+    }
+}
+
+template <int R, int RM, int Y0>
+__device__ __forceinline__ void sweep_dispatch(int r, float (&acc)[(32 + 2 * RM + 63) / 64][8 + 2 * RM],
+                                               const float *tab, unsigned rowLive, int lane) {
+    if constexpr (R <= RM) {
+        if (r == R)
+            sweep_rows<R, RM, Y0>(acc, tab, rowLive, lane);
+        else
+            sweep_dispatch<R + 1, RM, Y0>(r, acc, tab, rowLive, lane);
+    }
+}
+
+// One work-group of 4 waves per work unit (step, tile, block) of radius class RM; the block
+// is the main BEV block or one layer group's tail block. The unit's live layers are dealt
+// to the waves round-robin in layer order; each wave sums its layers in registers
+// (complete_superposition.cu:280-330: per-pixel erf differences e0 = erf(r/2),
+// e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product d e_x e_y), then the four partial
+// blocks are added in wave order in LDS and written to the unit's slot as a
+// (8 + 2RM) x (32 + 2RM) block. Fixed orders throughout: bitwise repeatable.
 template <int RM>
 __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__restrict__ iddSigma,
                                                    const int *__restrict__ tileRad, const int *__restrict__ effRad,
@@ -730,131 +821,133 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
                                                    const GroupScalars *__restrict__ gs,
                                                    const BeamScalars *__restrict__ bs,
                                                    const int *__restrict__ classCount,
-                                                   const int *__restrict__ classList, int cls, int maxItems,
+                                                   const int2 *__restrict__ classList, int cls, int maxUnits,
                                                    float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
     constexpr int NC = (32 + 2 * RM + 63) / 64;
-    constexpr int EW = RM + 1;
+    constexpr int EWp = pad4(RM + 2);
     constexpr int BW = 32 + 2 * RM, BH = 8 + 2 * RM;
-    constexpr int NR = 2 + 2 * RM;
-    __shared__ float s_e[4][64 * EW];
+    constexpr int LJ = kMaxLayersDev / 64;
+    __shared__ __align__(16) float s_tab[4][128 * EWp];  // per wave: 4 source rows x 32 pixels
     __shared__ float s_B[BH * BW];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float *tab = s_tab[w];
     const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
-    const int count = classCount[cls];
-    const int *list = classList + size_t(cls) * maxItems;
-    const int m = bs->numGroups;
-    float *Ew = s_e[w];
-    for (int it = blockIdx.x; it < count; it += gridDim.x) {
-        const int slot = list[it];
+    const int count = min(classCount[cls], maxUnits);
+    const int2 *list = classList + size_t(cls) * maxUnits;
+    for (int u = blockIdx.x; u < count; u += gridDim.x) {
+        const int2 unit = list[u];
+        const int slot = unit.x, b = unit.y;
         const int k = slot / nt, t = slot % nt;
         const int tx = t % ntx, ty = t / ntx;
-        const int gx = tx * 32 + (lane & 31), gy = ty * 8 + 2 * w + (lane >> 5);
-        const bool pixIn = gx < g.rx && gy < g.ry;
-        // blocks: b = -1 main, b = j tail of group j
-        for (int b = -1; b < m; ++b) {
-            if (b >= 0 && !in_tail(gs[b], k)) continue;
-            float acc[NC][NR];
+        // this block's layers, one layer per lane, loaded together
+        uint64_t lmask[LJ];
+        int radv[LJ];
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int q = 0; q < NR; ++q) acc[c][q] = 0.0f;
-            bool any = false;
-            for (int l = 0; l < g.numLayers; ++l) {
+        for (int j = 0; j < LJ; ++j) {
+            const int l = 64 * j + lane;
+            bool in = false;
+            int rad = 0;
+            if (l < g.numLayers) {
                 const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
-                if (code < 0 || !(code & 256)) continue;
-                const bool isMain = k < ls[l].mainEnd;
-                if (b < 0 ? !isMain : (isMain || ls[l].group != b)) continue;
-                const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
-                any = true;
-                float d = 0.0f, r = PBS_INF;
-                if (pixIn) {
-                    const float2 v = iddSigma[((size_t(l) * g.K + k) * g.sy + gy) * g.sx + gx];
-                    d = v.x;
-                    r = v.y;
+                if (code >= 0 && (code & 256)) {
+                    const LayerScalars L = ls[l];
+                    in = b < 0 ? (k < L.mainEnd) : (k >= L.mainEnd && L.group == b);
+                    if (in) rad = effRad[l * (kMaxRadDev + 2) + (code & 255)];
                 }
-                // erf differences of this lane's pixel (complete_superposition.cu:298-307),
-                // written to the lane's LDS row, then held in registers for readlane
-                {
-                    float en = erff(r * 0.5f), eo = -en;
-                    for (int i = 0; i <= RM; ++i) {
-                        float v = 0.0f;
-                        if (i <= R) {
-                            v = 0.5f * (en - eo);
+            }
+            lmask[j] = __ballot(in);
+            radv[j] = rad;
+        }
+        float acc[NC][BH];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < BH; ++q) acc[c][q] = 0.0f;
+        int rank = 0;
+#pragma unroll
+        for (int j = 0; j < LJ; ++j) {
+            uint64_t msk = lmask[j];
+            while (msk) {
+                const int ln = __builtin_ctzll(msk);
+                msk &= msk - 1;
+                if ((rank++ & 3) != w) continue;
+                const int l = 64 * j + ln;
+                const int R = __builtin_amdgcn_readlane(radv[j], ln);
+                const float2 *plane = iddSigma + (size_t(l) * g.K + k) * g.sy * g.sx;
+                // all four pixels of this lane for the layer, loads issued together
+                float dv[4], rv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int gx = tx * 32 + (lane & 31), gy = ty * 8 + 2 * q + (lane >> 5);
+                    dv[q] = 0.0f;
+                    rv[q] = PBS_INF;
+                    if (gx < g.rx && gy < g.ry) {
+                        const float2 v = plane[size_t(gy) * g.sx + gx];
+                        dv[q] = v.x;
+                        rv[q] = v.y;
+                    }
+                }
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    unsigned rowLive = 0;
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const float d = dv[2 * half + p], r = rv[2 * half + p];
+                        const uint64_t live = __ballot(d != 0.0f);
+                        rowLive |= (unsigned(uint32_t(live) != 0u) << (2 * p)) |
+                                   (unsigned(uint32_t(live >> 32) != 0u) << (2 * p + 1));
+                        float *e = tab + ((2 * p + (lane >> 5)) * 32 + (lane & 31)) * EWp;
+                        e[0] = d;
+                        float en = erff(r * 0.5f), eo = -en;
+                        for (int i = 0; i <= R; ++i) {
+                            e[1 + i] = 0.5f * (en - eo);
                             eo = en;
                             if (i < R) en = erff(r * (float(i) + 1.5f));
                         }
-                        Ew[lane * EW + i] = v;
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (rowLive) {
+                        if (half == 0)
+                            sweep_dispatch<0, RM, 0>(R, acc, tab, rowLive, lane);
+                        else
+                            sweep_dispatch<0, RM, 4>(R, acc, tab, rowLive, lane);
+                    }
+                    // the next staging overwrites the table: keep it after these reads
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                float e[RM + 1];
+            }
+        }
+        // partial blocks added in wave order
+        for (int ww = 0; ww < 4; ++ww) {
+            if (w == ww) {
 #pragma unroll
-                for (int i = 0; i <= RM; ++i) e[i] = Ew[lane * EW + i];
-                const uint64_t live = __ballot(d > 0.0f || d < 0.0f);
-                const int dbits = __float_as_int(d);
+                for (int c = 0; c < NC; ++c) {
+                    const int col = c * 64 + lane;
+                    if (col < BW) {
 #pragma unroll
-                for (int half = 0; half < 2; ++half) {
-                    uint64_t msk = half ? (live >> 32) : (live & 0xffffffffull);
-                    while (msk) {
-                        const int px = __builtin_ctzll(msk);
-                        msk &= msk - 1;
-                        const int p = px + 32 * half;
-                        const float dp = __int_as_float(__builtin_amdgcn_readlane(dbits, p));
-                        float a[NC];
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            const int adx = abs(lane + 64 * c - RM - px);
-                            a[c] = adx <= R ? dp * Ew[p * EW + min(adx, RM)] : 0.0f;
-                        }
-#pragma unroll
-                        for (int i = 0; i <= RM; ++i) {
-                            if (i <= R) {
-                                const float ei =
-                                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(e[i]), p));
-#pragma unroll
-                                for (int c = 0; c < NC; ++c) {
-                                    acc[c][half + RM + i] += a[c] * ei;
-                                    if (i) acc[c][half + RM - i] += a[c] * ei;
-                                }
-                            }
+                        for (int q = 0; q < BH; ++q) {
+                            float *o = s_B + q * BW + col;
+                            *o = ww == 0 ? acc[c][q] : *o + acc[c][q];
                         }
                     }
                 }
-                // the next layer rewrites this wave's rows: keep its stores after these loads
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            if (!any) continue;  // uniform across the block
-            // combine the four wave blocks in a fixed order, then write the block's slot
-            for (int i = tid; i < BH * BW; i += 256) s_B[i] = 0.0f;
-            __syncthreads();
-            for (int ww = 0; ww < 4; ++ww) {
-                if (w == ww) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        const int col = lane + 64 * c;
-                        if (col < BW) {
-#pragma unroll
-                            for (int q = 0; q < NR; ++q) s_B[(2 * w + q) * BW + col] += acc[c][q];
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            float *out;
-            if (b < 0) {
-                out = mainSlots + size_t(slot) * kSlotFloats;
-            } else {
-                const int q = gs[b].tailBase + (k - gs[b].tailStart);
-                out = (q < bs->tailPlanes) ? tailSlots + (size_t(q) * nt + t) * kSlotFloats : nullptr;
-            }
-            if (out)
-                for (int i = tid; i < BH * BW; i += 256) out[(i / BW) * kSlotPitch + (i % BW)] = s_B[i];
             __syncthreads();
         }
+        float *out;
+        if (b < 0) {
+            out = mainSlots + size_t(slot) * kSlotFloats;
+        } else {
+            const int q = gs[b].tailBase + (k - gs[b].tailStart);
+            out = (q < bs->tailPlanes) ? tailSlots + (size_t(q) * nt + t) * kSlotFloats : nullptr;
+        }
+        if (out)
+            for (int i = threadIdx.x; i < BH * BW; i += 256) out[(i / BW) * kSlotPitch + (i % BW)] = s_B[i];
+        __syncthreads();
     }
 }
 
@@ -908,8 +1001,8 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
 // ---------------------------------------------------------------------------------------
 // BEV -> dose
 // ---------------------------------------------------------------------------------------
-// Thread per dose (x, y) column inside the union box; loops over z (one owner per voxel).
-// The dose-index -> fan-index map is the reference's factored form
+// Thread per dose voxel inside the union box (one owner per voxel, no atomics). The
+// dose-index -> fan-index map is the reference's factored form
 // (transfer_param_struct_div3.cuh:34-44, transfer_param_struct_div3_impl.cu:14-34). Every
 // voxel samples the main BEV; a voxel inside a group's box also samples that group's tail.
 __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
@@ -917,40 +1010,37 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
                                                   const float *__restrict__ tailBev, float *__restrict__ dose) {
     const int x = bs->boxStart[0] + int(blockIdx.x * 32 + (threadIdx.x & 31));
     const int y = bs->boxStart[1] + int(blockIdx.y * 8 + (threadIdx.x >> 5));
-    if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1]) return;
+    const int z = bs->boxStart[2] + int(blockIdx.z);
+    if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1] || z >= bs->boxEnd[2]) return;
     const int inside = bs->beamFirstInside;
     const int m = bs->numGroups;
-    const float goffZ = ta.globalOffset[2] - float(inside);
     const int bx = g.sx + 64, by = g.sy + 64;
-    const float fbx = float(bx), fby = float(by);
-    const float sx0 = float(x) * ta.coefI[0] + float(y) * ta.coefJ[0] + ta.coefOffset[0];
-    const float sy0 = float(x) * ta.coefI[1] + float(y) * ta.coefJ[1] + ta.coefOffset[1];
-    const float sz0 = float(x) * ta.coefI[2] + float(y) * ta.coefJ[2] + ta.coefOffset[2];
-    const size_t sliceStride = size_t(g.doseDims[0]) * g.doseDims[1];
-    float *res = dose + size_t(bs->boxStart[2]) * sliceStride + size_t(y) * g.doseDims[0] + x;
-    for (int z = bs->boxStart[2]; z < bs->boxEnd[2]; ++z, res += sliceStride) {
-        const float fz = float(z);
-        float px = sx0 + fz * ta.inc[0], py = sy0 + fz * ta.inc[1];
-        const float pz = sz0 + fz * ta.inc[2];
-        px *= 1.0f + pz / (ta.normDist[0] - pz);
-        py *= 1.0f + pz / (ta.normDist[1] - pz);
-        const float fxI = px + ta.globalOffset[0], fyI = py + ta.globalOffset[1], fzI = pz + goffZ;
-        // prim_transf_kernel.cu:28-40: texel coordinate = fan index + 1/2; negatives skipped
-        if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) continue;
-        if (fxI > fbx || fyI > fby) continue;  // wholly outside the BEV: border value 0
-        float v = 0.0f;
-        if (fzI <= float(g.K)) v = trilinear_border(bev, bx, by, g.K, fxI, fyI, fzI);
-        for (int j = 0; j < m; ++j) {
-            const GroupScalars &G = gs[j];
-            if (G.tailCount <= 0) continue;
-            const float zl = fzI - float(G.tailStart);
-            if (!(zl > -1.0f && zl < float(G.tailCount))) continue;
-            if (x < G.boxStart[0] || x >= G.boxEnd[0] || y < G.boxStart[1] || y >= G.boxEnd[1] ||
-                z < G.boxStart[2] || z >= G.boxEnd[2])
-                continue;
-            v += trilinear_border(tailBev + size_t(G.tailBase) * bx * by, bx, by, G.tailCount, fxI, fyI, zl);
-        }
-        if (v > 0.0f) *res += v;
+    const float fx = float(x), fy = float(y), fz = float(z);
+    float px = fx * ta.coefI[0] + fy * ta.coefJ[0] + ta.coefOffset[0] + fz * ta.inc[0];
+    float py = fx * ta.coefI[1] + fy * ta.coefJ[1] + ta.coefOffset[1] + fz * ta.inc[1];
+    const float pz = fx * ta.coefI[2] + fy * ta.coefJ[2] + ta.coefOffset[2] + fz * ta.inc[2];
+    px *= 1.0f + pz / (ta.normDist[0] - pz);
+    py *= 1.0f + pz / (ta.normDist[1] - pz);
+    const float fxI = px + ta.globalOffset[0], fyI = py + ta.globalOffset[1];
+    const float fzI = pz + (ta.globalOffset[2] - float(inside));
+    // prim_transf_kernel.cu:28-40: texel coordinate = fan index + 1/2; negatives skipped
+    if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return;
+    if (fxI > float(bx) || fyI > float(by)) return;  // wholly outside the BEV: border value 0
+    float v = 0.0f;
+    if (fzI <= float(g.K)) v = trilinear_border(bev, bx, by, g.K, fxI, fyI, fzI);
+    for (int j = 0; j < m; ++j) {
+        const GroupScalars &G = gs[j];
+        if (G.tailCount <= 0) continue;
+        const float zl = fzI - float(G.tailStart);
+        if (!(zl > -1.0f && zl < float(G.tailCount))) continue;
+        if (x < G.boxStart[0] || x >= G.boxEnd[0] || y < G.boxStart[1] || y >= G.boxEnd[1] || z < G.boxStart[2] ||
+            z >= G.boxEnd[2])
+            continue;
+        v += trilinear_border(tailBev + size_t(G.tailBase) * bx * by, bx, by, G.tailCount, fxI, fyI, zl);
+    }
+    if (v > 0.0f) {
+        float *res = dose + (size_t(z) * g.doseDims[1] + y) * g.doseDims[0] + x;
+        *res += v;
     }
 }
 
@@ -1014,41 +1104,42 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
-                                     int *classList, int maxItems, hipStream_t s) {
+                                     int2 *classList, int maxUnits, hipStream_t s) {
     hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap);
     const int nt = (g.sx / 32) * (g.sy / 8);
     hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad);
     hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
-                       tailItemR, classCount, classList, maxItems);
+                       tailItemR, classCount, classList, maxUnits);
     return hipGetLastError();
 }
 
 template <int RM>
 static void launch_class(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                          const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs, const int *classCount,
-                         const int *classList, int cls, int maxItems, float *mainSlots, float *tailSlots, int grid,
+                         const int2 *classList, int cls, int maxUnits, float *mainSlots, float *tailSlots, int grid,
                          hipStream_t s) {
     hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, gs, bs,
-                       classCount, classList, cls, maxItems, mainSlots, tailSlots);
+                       classCount, classList, cls, maxUnits, mainSlots, tailSlots);
 }
 
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
-                            const int *classCount, const int *classList, int maxItems, float *mainSlots,
-                            float *tailSlots, int gridCap, hipStream_t s) {
-    const int grid = maxItems < gridCap ? maxItems : gridCap;
-#define PBS_CLASS(RMV, C)                                                                                          \
-    launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxItems, mainSlots,    \
-                      tailSlots, grid, s)
-    PBS_CLASS(2, 0);
-    PBS_CLASS(4, 1);
-    PBS_CLASS(6, 2);
-    PBS_CLASS(8, 3);
-    PBS_CLASS(12, 4);
-    PBS_CLASS(16, 5);
-    PBS_CLASS(24, 6);
-    PBS_CLASS(32, 7);
+                            const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,
+                            float *tailSlots, int gridCap, const hipStream_t streams[1 + kSideStreams]) {
+    const int grid = maxUnits < gridCap ? maxUnits : gridCap;
+    // the heavy middle classes on separate streams; light ones share
+#define PBS_CLASS(RMV, C, S)                                                                                       \
+    launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxUnits, mainSlots,    \
+                      tailSlots, grid, streams[S])
+    PBS_CLASS(12, 4, 0);
+    PBS_CLASS(16, 5, 1);
+    PBS_CLASS(8, 3, 2);
+    PBS_CLASS(6, 2, 3);
+    PBS_CLASS(4, 1, 1);
+    PBS_CLASS(2, 0, 2);
+    PBS_CLASS(24, 6, 3);
+    PBS_CLASS(32, 7, 3);
 #undef PBS_CLASS
     return hipGetLastError();
 }
@@ -1063,8 +1154,9 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
 }
 
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
-                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, hipStream_t s) {
-    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY), dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose);
+                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, int gridZ,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY, gridZ), dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose);
     return hipGetLastError();
 }
 
