@@ -83,6 +83,17 @@ def algorithmic_work(dbg: dict, case: dict) -> dict:
     }
 
 
+def measured_traffic() -> dict | None:
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
+    workload (scaled by a copy kernel of known size), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "head_beam_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(case: dict, runs: int) -> dict:
     import oracle
 
@@ -105,6 +116,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB 5 times (a known-size kernel "
+                         "for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU per process; multi-GPU runs are not part of "
@@ -163,11 +177,14 @@ def main() -> int:
     # roofline of the dominant stage
     dom = max(per_beam, key=per_beam.get)
     t_dom = per_beam[dom] * 1e-3
+    traffic = measured_traffic()
     if dom == "superpose":
         ach = work["superpose"]["flops"] / t_dom / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_superpose<R> (8 radius classes)",
-                "compute": "fp32 VALU (no MFMA; peak quoted is the fp32 dense figure)"}
+                "frac": ach / FP32_PEAK_TFLOPS,
+                "traffic": traffic.get("superpose_stage_bytes") if traffic else None,
+                "kernel": "k_superpose<RM>: 8 radius classes on 4 streams, one stage",
+                "compute": "fp32 VALU FMA (no matrix cores; peak is the fp32 dense figure)"}
     else:
         b = work.get(dom, {}).get("bytes", 0.0)
         ach = b / t_dom / 1e9
@@ -205,6 +222,14 @@ def main() -> int:
         out["cpu_baseline"] = {"value": 1.0 / oracle_s, "unit": "beams/s", "cores": os.cpu_count(), "kind": "port",
                                "sample": "1 head-size beam through the FP64 oracle"}
     eng.close()
+    if args.copy_mb > 0:
+        n = args.copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration"] = {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}
     print(json.dumps(out))
     return 0
 

@@ -397,3 +397,103 @@ def test_config1_oracle_runs_and_is_finite():
     # distal cut-offs grow with energy; the 140 MeV layer reaches the far CT edge
     assert list(d["afterLast"]) == sorted(d["afterLast"])
     assert d["afterLast"][-1] <= d["guaranteedPassive"]
+
+
+# ---------------------------------------------------------------- cut-offs, resample, transfer
+
+
+def test_beam_cutoffs_water_closed_form():
+    """Homogeneous water wider than the ray field: weplMin(k) is the axial ray's (k+1) * 2 mm;
+    the all-layer cut-off is the first step past 1.05 x the deepest peak (KAT 120 MeV ->
+    103.755 mm). (A ray that never enters keeps firstOutside = 1, bev_ray_tracing.cu:121-158,
+    and would count in weplMin, so the phantom covers every ray.)"""
+    case = _water_case(n=(64, 64, 72), energies=(100.0, 120.0))
+    p, dens, wepl, fi, fo, keep, en, bm = _trace(case)
+    K = p.steps
+    wmin = np.zeros(K)
+    O.lib().orc_beam_cutoffs(C.byref(p), C.byref(bm), C.byref(en), O._dp(wepl), O._ip(fi), O._ip(fo),
+                             O._dp(wmin))
+    assert p.beamFirstInside == 1 and p.beamFirstOutside == K
+    np.testing.assert_allclose(wmin, 2.0 * np.arange(1, K + 1), atol=1e-9)
+    assert p.maxPeakDepth == pytest.approx(103.755, abs=2e-3)
+    want = next(k for k in range(K) if 2.0 * (k + 1) > 1.05 * 103.755)
+    assert want == 54
+    assert p.firstPastCutoffAll == want
+    assert p.guaranteedPassive == min(want, K)
+
+
+def test_cpb_to_rays_identity_and_mass():
+    rng = np.random.default_rng(3)
+    cpb = rng.random((2, 8, 12))
+    cd = np.array([12, 8, 2], np.int32)
+    cc = np.array([-3.0, -2.0, 0.0], np.float32)
+    cr = np.array([0.5, 0.5, 1.0], np.float32)
+    L = O.lib()
+    # same grid: identity
+    rays = np.zeros((8, 12))
+    L.orc_cpb_to_rays(O._dp(cpb), O._ip(cd), O._fp(cc), O._fp(cr), O._dp(rays), O._ip(np.array([12, 8], np.int32)),
+                      O._fp(cc), O._fp(cr), 1)
+    np.testing.assert_allclose(rays, cpb[1], rtol=0, atol=1e-15)
+    # 1 mm rays over the same extent: each ray is the sum of its 2x2 cells; total mass kept
+    rays2 = np.zeros((4, 6))
+    L.orc_cpb_to_rays(O._dp(cpb), O._ip(cd), O._fp(cc), O._fp(cr), O._dp(rays2), O._ip(np.array([6, 4], np.int32)),
+                      O._fp(cc), O._fp(np.array([1.0, 1.0, 1.0], np.float32)), 0)
+    np.testing.assert_allclose(rays2, cpb[0].reshape(4, 2, 6, 2).sum(axis=(1, 3)), atol=1e-12)
+    assert rays2.sum() == pytest.approx(cpb[0].sum(), rel=1e-12)
+
+
+def _identity_transfer_plan(passive=6, dist=1.0e30):
+    """Fan index == dose index (x, y), fan step == dose z; BEV halo of 32 in x and y."""
+    p = O.Plan()
+    p.superpDims[0], p.superpDims[1] = 32, 8
+    p.beamFirstInside = 0
+    p.layerPassive[0] = passive
+    p.guaranteedPassive = passive
+    for i, v in enumerate([1, 0, 0, 0, 1, 0, 0, 0, -1]):
+        p.g2dose[i] = v
+    p.g2doseOff[0], p.g2doseOff[1], p.g2doseOff[2] = 16.0, 4.0, 0.0
+    p.fanCorner[0], p.fanCorner[1], p.fanCorner[2] = -16.0, -4.0, 0.0
+    p.fanDelta[0], p.fanDelta[1], p.fanDelta[2] = 1.0, 1.0, -1.0
+    p.sourceDist[0] = p.sourceDist[1] = dist
+    return p
+
+
+def test_transfer_identity_geometry():
+    """With no divergence and unit fan steps on the dose grid, every voxel of the launch box
+    reads its own BEV texel: dose[z, y, x] = bev[z, y + 32, x + 32]; voxels past the
+    passive step are outside the box and stay zero."""
+    P = 6
+    p = _identity_transfer_plan(P)
+    rng = np.random.default_rng(4)
+    bev = rng.random((P, 8 + 64, 32 + 64)) + 0.1
+    dims = np.array([40, 12, 10], np.int32)
+    dose = np.zeros((10, 12, 40))
+    O.lib().orc_transfer(C.byref(p), 0, O._dp(bev), O._dp(dose), O._ip(dims))
+    np.testing.assert_allclose(dose[:P], bev[:, 32:44, 32:72], rtol=0, atol=1e-12)
+    assert not dose[P:].any()
+
+
+def test_transfer_linear_in_the_bev():
+    P = 6
+    p = _identity_transfer_plan(P, dist=300.0)
+    p.fanCorner[0], p.fanCorner[1] = -16.3, -4.6  # sub-voxel offsets: trilinear weights in play
+    rng = np.random.default_rng(5)
+    b1 = rng.random((P, 72, 96))
+    b2 = rng.random((P, 72, 96))
+    dims = np.array([40, 12, 10], np.int32)
+    out = []
+    for b in (b1, b2, 2.0 * b1 + 3.0 * b2):
+        d = np.zeros((10, 12, 40))
+        O.lib().orc_transfer(C.byref(p), 0, O._dp(np.ascontiguousarray(b)), O._dp(d), O._ip(dims))
+        out.append(d)
+    assert out[0].any()
+    np.testing.assert_allclose(out[2], 2.0 * out[0] + 3.0 * out[1], rtol=1e-12, atol=1e-14)
+
+
+def test_trace_ray_that_never_enters():
+    """A ray outside the CT: the cumulative-HU flag never drops (firstInside = K) and
+    lastInside keeps its initial 0 (firstOutside = 1), bev_ray_tracing.cu:117-158."""
+    case = _water_case(n=(8, 8, 40))
+    p, dens, wepl, fi, fo, *_ = _trace(case)
+    assert fi[0, 0] == p.steps and fo[0, 0] == 1
+    assert dens[:, 0, 0].max() < 0.01
