@@ -1,0 +1,156 @@
+"""The drop-in entry points on the GPU: the Python contract of raytracedicom_wrapper, the
+host-buffer C form of subsecondWrapper (accumulation, validation), the legacy subspot -> CPB
+path, and cudaFinalDose. Dose tolerance: max |dD| <= 0.5 % of Dmax against the FP64 oracle.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from subsecondpbs_amd import _abi, fixtures as F, rtd
+
+gpu = pytest.mark.gpu
+DOSE_TOL = 5e-3
+
+
+def _need_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def config1():
+    case = F.make_config1()
+    ref, _ = O.beam_dose(case)
+    return case, ref
+
+
+@gpu
+def test_raytracedicom_wrapper_config1(config1):
+    _need_gpu()
+    case, ref = config1
+    d = rtd.raytracedicom_wrapper(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"],
+                                  case["dose_dims"], case["dose_resolution"], case["dose_corner"],
+                                  case["beams"][0], case["energy"])
+    assert d.shape == tuple(reversed(case["dose_dims"])) and d.dtype == np.float32
+    err = _rel(d, ref)
+    print(f"raytracedicom_wrapper config1 max|dD|/Dmax = {err:.3e}")
+    assert err <= DOSE_TOL
+
+
+@gpu
+def test_subsecond_wrapper_accumulates_and_validates(config1):
+    _need_gpu()
+    case, ref = config1
+    args = (case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    grid = (case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    dose = np.zeros(tuple(reversed(case["dose_dims"])), np.float32)
+    rtd.subsecond_wrapper(*args, dose, *grid, case["beams"], case["energy"])
+    once = dose.copy()
+    rtd.subsecond_wrapper(*args, dose, *grid, case["beams"], case["energy"])
+    np.testing.assert_allclose(dose, 2.0 * once, rtol=1e-6, atol=1e-12 * once.max())
+    # a failing validation leaves the dose untouched
+    bad = dict(case["beams"][0])
+    bad["spot_sigmas"] = np.asarray(bad["spot_sigmas"])[:-1]
+    before = dose.copy()
+    with pytest.raises((RuntimeError, ValueError)):
+        rtd.subsecond_wrapper(*args, dose, *grid, [bad], case["energy"])
+    assert np.array_equal(dose, before)
+
+
+def _legacy_beam(case):
+    """The config-1 spots as absolute subspots (no spot lattice): the CPB path."""
+    b = dict(case["beams"][0])
+    L = len(b["energies"])
+    counts = np.asarray(b["layer_spot_counts"])
+    pos = np.asarray(b["spot_positions"], np.float32).reshape(-1, 2)
+    w = np.asarray(b["spot_weights"], np.float32)
+    S = int(counts.max())
+    sub = np.zeros((L, S, 5), np.float32)
+    o = 0
+    for l in range(L):
+        n = int(counts[l])
+        sub[l, :n, 0:2] = pos[o:o + n]
+        sub[l, :n, 2] = w[o:o + n]
+        sub[l, :n, 3:5] = np.asarray(b["spot_sigmas"], np.float32).reshape(L, 2)[l]
+        o += n
+    for k in ("layer_spot_counts", "spot_positions", "spot_weights"):
+        b.pop(k)
+    b["subspot_data"] = sub
+    return b
+
+
+@gpu
+def test_legacy_cpb_path_config1(config1):
+    torch = _need_gpu()
+    case, _ = config1
+    case = dict(case)
+    case["beams"] = [_legacy_beam(case)]
+    ref, plan = O.beam_dose(case)
+    assert plan.legacyPath == 1
+    from subsecondpbs_amd.engine import Engine
+    eng = Engine(case["energy"])
+    eng.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    eng.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    dose = torch.zeros(tuple(reversed(case["dose_dims"])), dtype=torch.float32, device="cuda")
+    eng.beam_dose(case["beams"][0], dose)
+    eng.sync()
+    ints = eng.debug_fetch(())["ints"]
+    eng.close()
+    assert ints["legacy"] == 1
+    assert ints["rayDims"] == tuple(plan.rayDims)
+    err = _rel(dose.cpu().numpy(), ref)
+    print(f"legacy CPB path config1 max|dD|/Dmax = {err:.3e}")
+    assert err <= DOSE_TOL
+
+
+@gpu
+def test_cuda_final_dose_roi_miss_and_hit(config1):
+    torch = _need_gpu()
+    case, _ = config1
+    from subsecondpbs_amd.engine import Engine
+    eng = Engine(case["energy"])
+    eng.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    eng.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    b = _legacy_beam(case)
+    L = len(b["energies"])
+    energies = np.asarray(b["energies"], np.float32)
+    sig = np.asarray(b["spot_sigmas"], np.float32).reshape(L, 2)
+    lib = _abi.lib()
+    _abi.check(lib.pbs_final_dose_bind(C.c_void_p(eng.handle), energies.ctypes.data_as(_abi._f),
+                                       sig.ctypes.data_as(_abi._f), L, C.c_float(0.5)), "pbs_final_dose_bind")
+    nx, ny, nz = case["dose_dims"]
+    res = _abi.Vec3f(*case["dose_resolution"])
+    corner = _abi.Vec3f(*case["dose_corner"])
+    dims = _abi.Vec3i(nx, ny, nz)
+    sub = torch.from_numpy(np.ascontiguousarray(b["subspot_data"])).cuda()
+    # ROI: a 20x20x20-voxel cube in the middle of the phantom
+    g = np.arange(22, 42)
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    roi = torch.from_numpy(np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.int32)).cuda()
+    dose = torch.full((nz, ny, nx), 7.0, dtype=torch.float32, device="cuda")
+    # beam pointing away from the ROI: untouched
+    rc = lib.cudaFinalDose(C.c_void_p(sub.data_ptr()), C.c_void_p(roi.data_ptr()), roi.shape[0], res, corner,
+                           _abi.Vec3f(0, 0, -1), _abi.Vec3f(0, 0, -2000), C.c_float(2000.0), C.c_float(0.0), L,
+                           int(sub.shape[1]), C.c_void_p(dose.data_ptr()), dims)
+    assert rc == 0
+    assert bool((dose == 7.0).all())
+    # beam along +z from z = -2000 through the ROI: the dose grid is overwritten with the dose
+    rc = lib.cudaFinalDose(C.c_void_p(sub.data_ptr()), C.c_void_p(roi.data_ptr()), roi.shape[0], res, corner,
+                           _abi.Vec3f(0, 0, 1), _abi.Vec3f(0, 0, -2000), C.c_float(2000.0), C.c_float(0.0), L,
+                           int(sub.shape[1]), C.c_void_p(dose.data_ptr()), dims)
+    _abi.check(rc, "cudaFinalDose")
+    d = dose.cpu().numpy()
+    eng.close()
+    assert np.isfinite(d).all() and d.min() >= 0.0 and d.max() > 0.0
+    assert not (d == 7.0).any()
+    # dose is deposited along +z inside the lateral extent of the spots
+    prof = d.sum(axis=(1, 2))
+    assert prof[:5].sum() > 0 and prof.argmax() > 5
