@@ -72,6 +72,8 @@ class Plan(C.Structure):
         ("batchRad", (C.c_int * (MAX_RAD + 1)) * MAX_LAYERS),
         ("transferStart", C.c_int * 3), ("transferMax", C.c_int * 3),
         ("numLayers", C.c_int), ("error", C.c_char * 256),
+        ("nucDims", C.c_int * 2), ("spotDist", C.c_float), ("nucRadHist", (C.c_int * 34) * 256),
+        ("entrySpotSigmaX", C.c_float * 256),
     ]
 
 
@@ -114,6 +116,10 @@ def lib():
         L.orc_cpb_to_rays.argtypes = [_d, _i, _f, _f, _d, _i, _f, _f, C.c_int]
         L.orc_layer_params.argtypes = [C.POINTER(Plan), C.POINTER(Beam), C.POINTER(Energy), _d]
         L.orc_fill.argtypes = [C.POINTER(Plan), C.c_int, C.POINTER(Energy), _d, _d, _d, _i, _i, _d, _d, _i]
+        L.orc_fill_halo.argtypes = [C.POINTER(Plan), C.c_int, C.POINTER(Energy), _d, _d, _d, _i, _i, _d, _d, _i,
+                                    C.c_int, _i, _d, _d, _d]
+        L.orc_halo_plan.restype = C.c_int
+        L.orc_halo_plan.argtypes = [C.POINTER(Plan), C.POINTER(Beam), _i, _d]
         L.orc_superpose.argtypes = [C.POINTER(Plan), C.c_int, _d, _d, _d]
         L.orc_transfer.argtypes = [C.POINTER(Plan), C.c_int, _d, _d, _i]
         L.orc_beam_dose.restype = C.c_int
@@ -246,11 +252,13 @@ def plan_beam(case: dict, beam_index: int = 0, use_subspots: bool = False) -> Pl
 
 
 def beam_dose(case: dict, beam_index: int = 0, dose: np.ndarray | None = None,
-              use_subspots: bool = False, ct_hu_plus_1000: np.ndarray | None = None):
-    """FP64 dose of one beam accumulated into ``dose`` ([nz][ny][nx] float64); returns (dose, plan)."""
+              use_subspots: bool = False, ct_hu_plus_1000: np.ndarray | None = None, nuclear: int = 0):
+    """FP64 dose of one beam accumulated into ``dose`` ([nz][ny][nx] float64); returns (dose, plan).
+    ``nuclear``: halo branch off (0) or its model: 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA."""
     keep = _Keep()
     en = make_energy(case["energy"], keep)
     bm = make_beam(case["beams"][beam_index], keep, use_subspots)
+    bm.nuclear = int(nuclear)
     ct_dims, ct_res, ct_corner, dose_dims, dose_res, dose_corner = case_args(case)
     ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
     ct = np.ascontiguousarray(ct, dtype=np.float32)

@@ -828,6 +828,7 @@ void orc_layer_params(orc_plan *p, const orc_beam *b, const orc_energy *en, cons
         p->energyIdx[l] = eIdx;
         p->scaleFact[l] = sf;
         p->peakDepth[l] = pk;
+        p->entrySpotSigmaX[l] = sigma_to_mm(p->lenToMm, b->spotSigmas[2 * l]);
         float a, c;
         air_coefs(pk, &a, &c);
         /* initStepAndAirDiv (fill_idd_and_sigma_params.cuh:66-73) */
@@ -841,17 +842,42 @@ void orc_layer_params(orc_plan *p, const orc_beam *b, const orc_energy *en, cons
 /* ------------------------------------------------------------------------------------
  * IDD and sigma (idd_sigma.cu:46-259)
  * ---------------------------------------------------------------------------------- */
-void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *density, const double *wepl,
-              const double *rayWeights, const int *fi, const int *fo, double *idd, double *rsigma,
-              int *firstPassive) {
+/* Scattering constants of the halo models (idd_sigma.cu:108-121): eRefSq, sigmaDelta. */
+static void halo_model_consts(int model, double *eRefSq, double *sigmaDelta, int *suppressDip) {
+    *eRefSq = 198.81; *sigmaDelta = 0.21; *suppressDip = 0;
+    if (model == 1) { *eRefSq = 169.00; *sigmaDelta = 0.06; *suppressDip = 1; } /* GAUSS_FIT */
+    if (model == 2) { *eRefSq = 190.44; *sigmaDelta = 0.0; }                     /* SOUKUP */
+    if (model == 3) { *eRefSq = 216.09; *sigmaDelta = 0.08; }                    /* FLUKA */
+}
+
+void orc_fill_halo(const orc_plan *p, int l, const orc_energy *en, const double *density, const double *wepl,
+                   const double *rayWeights, const int *fi, const int *fo, double *idd, double *rsigma,
+                   int *firstPassive, int model, const int *rayToNuc, const double *nucW, double *nucIdd,
+                   double *nucRs) {
     const int rx = p->rayDims[0], ry = p->rayDims[1];
     const size_t plane = (size_t)rx * ry;
+    const size_t nucPlane = (size_t)p->nucDims[0] * p->nucDims[1];
     const unsigned first = (unsigned)p->beamFirstInside, pAfter = (unsigned)p->afterLast[l];
     const double peak = p->peakDepth[l], rangeStop = 1.05f * p->peakDepth[l];
     const double scale = p->scaleFact[l], eIdx = p->energyIdx[l];
     const double stepLength = fabs((double)p->fanDelta[2]);
     const double quad = p->sigmaSqAirQuad[l], lin = p->sigmaSqAirLin[l];
-    const double eRefSq = 198.81, sigmaDelta = 0.21;
+    double eRefSq, sigmaDelta;
+    int suppressDip;
+    halo_model_consts(model, &eRefSq, &sigmaDelta, &suppressDip);
+    const int halo = model != 0 && rayToNuc && nucW && nucIdd && nucRs && en->nucWeight && en->nucSqSigma &&
+                     nucPlane > 0 && p->spotDist > 0.0f;
+    /* entry sigma^2 of the halo: sigmaSqAirCoefs(peak) at the entry plane + spot sigma_x^2
+     * (wrapper:4324-4329), float as on the host */
+    float entrySigmaSq = 0.0f;
+    if (halo) {
+        float a, c;
+        air_coefs(p->peakDepth[l], &a, &c);
+        const float entryZ = (float)p->beamFirstInside * p->fanDelta[2] + p->fanCorner[2];
+        const float sgx = p->entrySpotSigmaX[l];
+        entrySigmaSq = a * entryZ * entryZ + c * entryZ + sgx * sgx;
+    }
+    const double spotDist = p->spotDist;
 #pragma omp parallel for schedule(static)
     for (int r = 0; r < (int)plane; ++r) {
         const unsigned firstIn = (unsigned)fi[r];
@@ -860,12 +886,14 @@ void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *dens
         const double w = rayWeights[r];
         int live = 1;
         if (w < 1e-6 || afterLast < first) { live = 0; afterLast = 0; }
+        const int cell = halo ? rayToNuc[r] : -1;
+        const double nucRayW = cell >= 0 ? nucW[cell] : 0.0;
         double cumulSp = 0, cumulSpOld = 0, cumulDose = 0, cumulDoseOld = 0;
         double incScat = 0, incincScat = 0;
         double incDiv = lin + (2.0 * (double)first - 1.0) * quad;
         double sigmaSq = -incDiv;
         for (unsigned k = first; k < pAfter; ++k) {
-            double res = 0.0, rs = INFINITY;
+            double res = 0.0, rs = INFINITY, nres = 0.0, nrs = INFINITY;
             const size_t idx = (size_t)k * plane + r;
             if (live) {
                 cumulSp = wepl[idx];
@@ -880,7 +908,7 @@ void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *dens
                     incincScat += 2.0 * thetaSq * stepLength * stepLength;
                     incScat += incincScat;
                     incDiv += 2.0 * quad;
-                } else {
+                } else if (!suppressDip) {
                     sigmaSq -= 1.5 * (incScat + incDiv) * rho;
                 }
                 double zk = (double)p->fanCorner[2] + (double)k * p->fanDelta[2];
@@ -889,31 +917,108 @@ void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *dens
                 rs = 0.5 * (vwx + vwy) / (sqrt(2.0) * (sqrt(sigmaSq) + sigmaDelta));
                 double vol = (double)p->volConst + (double)k * p->volLin + (double)k * (double)k * p->volSq;
                 double mass = rho * vol;
-                if (mass > 1e-2) res = w * (cumulDose - cumulDoseOld) / mass;
+                const double depthMid = 0.5 * (cumulSp + cumulSpOld) * scale;
+                if (mass > 1e-2) {
+                    if (halo) {
+                        const double nw = lut2d_clamp(en->nucWeight, en->nSamples, en->nEnergies, depthMid, eIdx);
+                        res = (1.0 - nw) * w * (cumulDose - cumulDoseOld) / mass;
+                        if (cell >= 0) nres = nw * nucRayW * (cumulDose - cumulDoseOld) / (mass * spotDist * spotDist);
+                    } else {
+                        res = w * (cumulDose - cumulDoseOld) / mass;
+                    }
+                }
+                if (cell >= 0) {
+                    const double s2 = lut2d_clamp(en->nucSqSigma, en->nSamples, en->nEnergies, depthMid, eIdx);
+                    nrs = 0.5 * spotDist * (vwx + vwy) / (sqrt(2.0) * sqrt(sigmaSq + s2 + (double)entrySigmaSq));
+                }
                 cumulSpOld = cumulSp;
                 cumulDoseOld = cumulDose;
                 if (cumulSp > rangeStop || k == afterLast) { live = 0; afterLast = k; }
             }
-            if (!live || (int)k < (int)firstIn - 1) { res = 0.0; rs = INFINITY; }
+            if (!live || (int)k < (int)firstIn - 1) { res = 0.0; rs = INFINITY; nres = 0.0; nrs = INFINITY; }
             idd[idx] = res;
             rsigma[idx] = rs;
+            if (cell >= 0) {
+                nucIdd[(size_t)k * nucPlane + cell] = nres;
+                nucRs[(size_t)k * nucPlane + cell] = nrs;
+            }
         }
         firstPassive[r] = (int)afterLast;
     }
 }
 
+void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *density, const double *wepl,
+              const double *rayWeights, const int *fi, const int *fo, double *idd, double *rsigma,
+              int *firstPassive) {
+    orc_fill_halo(p, l, en, density, wepl, rayWeights, fi, fo, idd, rsigma, firstPassive, 0, NULL, NULL, NULL, NULL);
+}
+
+int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeights) {
+    float *dense = build_lattice(b, p);
+    if (!dense) {
+        snprintf(p->error, sizeof(p->error), "nuclear_correction=true requires raw spot lattice inputs");
+        return -1;
+    }
+    const int rawX = p->spotDims[0], rawY = p->spotDims[1];
+    const int nucX = (rawX + ORC_TILE_X - 1) / ORC_TILE_X * ORC_TILE_X;
+    const int nucY = (rawY + ORC_TILE_Y - 1) / ORC_TILE_Y * ORC_TILE_Y;
+    p->nucDims[0] = nucX;
+    p->nucDims[1] = nucY;
+    const int rx = p->rayDims[0], ry = p->rayDims[1];
+    for (int i = 0; i < rx * ry; ++i) rayToNuc[i] = -1;
+    for (int j = 0; j < rawY; ++j) {
+        const float gy = p->spotOffset[1] + (float)j * p->spotDelta[1];
+        const int rayY = (int)lroundf((gy - p->cpbCorner[1]) / p->cpbRes[1]);
+        if (rayY < 0 || rayY >= ry) {
+            snprintf(p->error, sizeof(p->error), "dense halo lattice row mapped outside active ray grid");
+            free(dense);
+            return -1;
+        }
+        for (int i = 0; i < rawX; ++i) {
+            const float gx = p->spotOffset[0] + (float)i * p->spotDelta[0];
+            const int rayX = (int)lroundf((gx - p->cpbCorner[0]) / p->cpbRes[0]);
+            if (rayX < 0 || rayX >= rx) {
+                snprintf(p->error, sizeof(p->error), "dense halo lattice column mapped outside active ray grid");
+                free(dense);
+                return -1;
+            }
+            rayToNuc[rayY * rx + rayX] = j * nucX + i;
+        }
+    }
+    if (nucWeights) {
+        for (int l = 0; l < b->numLayers; ++l)
+            for (int j = 0; j < nucY; ++j)
+                for (int i = 0; i < nucX; ++i)
+                    nucWeights[((size_t)l * nucY + j) * nucX + i] =
+                        (i < rawX && j < rawY) ? dense[((size_t)l * rawY + j) * rawX + i] : 0.0;
+    }
+    free(dense);
+    /* spot spacing: the beam's spotDelta when set (> 1.5 rays), else the lattice's */
+    const int beamDelta = b->spotDelta[0] > b->raySpacing[0] * 1.5f;
+    const float spacing = beamDelta ? b->spotDelta[0] : p->spotDelta[0];
+    if (b->raySpacing[0] > 0.0f && spacing / b->raySpacing[0] < 1.5f) {
+        snprintf(p->error, sizeof(p->error), "nuclear superposition gate: spotDistInRays < 1.5");
+        return -1;
+    }
+    p->spotDist = fabsf(p->fanDelta[0]) > 1e-6f ? fabsf(spacing / p->fanDelta[0]) : 1.0f;
+    return 0;
+}
+
 /* ------------------------------------------------------------------------------------
  * superposition (complete_superposition.cu:197-544)
  * ---------------------------------------------------------------------------------- */
-void orc_superpose(orc_plan *p, int l, const double *idd, const double *rsigma, double *bev) {
-    const int sx = p->superpDims[0], sy = p->superpDims[1];
+/* Superposition of steps [k0, k1) of a [K][sy][sx] field into a BEV [bevZ][sy+64][sx+64].
+ * Radius histogram into hist[34] and batching into batch[33]. overflowIsError: the halo
+ * grid's policy (complete_superposition.cu:558-620: a tile past the largest radius is an
+ * error); the primary grid clamps it into the largest radius (413-463). Returns -1 on an
+ * overflow that is an error, else 0. */
+static int superpose_grid(int k0, int k1, int sx, int sy, const double *idd, const double *rsigma, double *bev,
+                          int *hist, int *batch, int overflowIsError) {
     const int ntx = sx / ORC_TILE_X, nty = sy / ORC_TILE_Y;
-    const int k0 = p->beamFirstInside, k1 = p->layerPassive[l];
     const int bx = sx + 2 * ORC_MAX_RAD, by = sy + 2 * ORC_MAX_RAD;
-    int *hist = p->radHist[l];
     memset(hist, 0, sizeof(int) * (ORC_MAX_RAD + 2));
-    memset(p->batchRad[l], 0, sizeof(int) * (ORC_MAX_RAD + 1));
-    if (k1 <= k0) return;
+    memset(batch, 0, sizeof(int) * (ORC_MAX_RAD + 1));
+    if (k1 <= k0) return 0;
     const int nk = k1 - k0;
     int *rad = (int *)malloc(sizeof(int) * (size_t)nk * ntx * nty);
     for (int k = k0; k < k1; ++k)
@@ -927,8 +1032,12 @@ void orc_superpose(orc_plan *p, int l, const double *idd, const double *rsigma, 
                 rad[((size_t)(k - k0) * nty + ty) * ntx + tx] = r;
                 hist[r]++;
             }
+    if (overflowIsError && hist[ORC_MAX_RAD + 1] > 0) {
+        free(rad);
+        return -1;
+    }
     int effRad[ORC_MAX_RAD + 2];
-    orc_batch_bins(hist, p->batchRad[l], effRad);
+    orc_batch_bins(hist, batch, effRad);
     for (int k = k0; k < k1; ++k) {
         double *plane = bev + (size_t)k * bx * by;
         /* a tile's block reaches at most 32 columns past its edges, so tile columns three
@@ -964,13 +1073,31 @@ void orc_superpose(orc_plan *p, int l, const double *idd, const double *rsigma, 
         }
     }
     free(rad);
+    return 0;
+}
+
+void orc_superpose(orc_plan *p, int l, const double *idd, const double *rsigma, double *bev) {
+    (void)superpose_grid(p->beamFirstInside, p->layerPassive[l], p->superpDims[0], p->superpDims[1], idd, rsigma, bev,
+                         p->radHist[l], p->batchRad[l], 0);
 }
 
 /* ------------------------------------------------------------------------------------
  * BEV -> dose transfer (wrapper:5343-5505, transfer_param_struct_div3*, prim_transf_kernel.cu)
  * ---------------------------------------------------------------------------------- */
-static void transfer_box(const orc_plan *p, int passive, int start[3], int maxI[3], const int doseDims[3]) {
-    const int sx = p->superpDims[0], sy = p->superpDims[1];
+/* A fan grid: cell (i, j) of step k is the fan point (fc + (i, j) * fd, fanCorner.z +
+ * k * fanDelta.z). The primary grid's cells are the rays; the halo grid's cells are the
+ * spots (nucIdxToFan, wrapper:5527-5535). */
+typedef struct { int sx, sy; float fc[2], fd[2]; } fan_grid;
+
+static fan_grid primary_grid(const orc_plan *p) {
+    fan_grid g = {p->superpDims[0], p->superpDims[1], {p->fanCorner[0], p->fanCorner[1]},
+                  {p->fanDelta[0], p->fanDelta[1]}};
+    return g;
+}
+
+static void transfer_box_g(const orc_plan *p, const fan_grid *gr, int passive, int start[3], int maxI[3],
+                           const int doseDims[3]) {
+    const int sx = gr->sx, sy = gr->sy;
     /* rayIdx -> doseIdx in float, as the host does (Float3FromFanTransform::transformPoint) */
     const float xv[2] = {-(float)ORC_MAX_RAD, (float)(sx + ORC_MAX_RAD - 1)};
     const float yv[2] = {-(float)ORC_MAX_RAD, (float)(sy + ORC_MAX_RAD - 1)};
@@ -981,7 +1108,7 @@ static void transfer_box(const orc_plan *p, int passive, int start[3], int maxI[
     for (int a = 0; a < 2; ++a)
         for (int bb = 0; bb < 2; ++bb)
             for (int c = 0; c < 2; ++c) {
-                f3 f = mk3(xv[c] * p->fanDelta[0] + p->fanCorner[0], yv[bb] * p->fanDelta[1] + p->fanCorner[1],
+                f3 f = mk3(xv[c] * gr->fd[0] + gr->fc[0], yv[bb] * gr->fd[1] + gr->fc[1],
                            zv[a] * p->fanDelta[2] + p->fanCorner[2]);
                 f.x *= (1.0f - f.z / p->sourceDist[0]);
                 f.y *= (1.0f - f.z / p->sourceDist[1]);
@@ -998,11 +1125,17 @@ static void transfer_box(const orc_plan *p, int passive, int start[3], int maxI[
     maxI[2] = (int)ceilf(mx.z) < doseDims[2] - 1 ? (int)ceilf(mx.z) : doseDims[2] - 1;
 }
 
-void orc_transfer(orc_plan *p, int l, const double *bev, double *dose, const int doseDims[3]) {
-    const int passive = p->layerPassive[l];
+static void transfer_box(const orc_plan *p, int passive, int start[3], int maxI[3], const int doseDims[3]) {
+    fan_grid g = primary_grid(p);
+    transfer_box_g(p, &g, passive, start, maxI, doseDims);
+}
+
+/* One layer's BEV of grid gr -> dose (primTransfDiv / nucTransfDiv, prim_transf_kernel.cu:11-78). */
+static void transfer_g(orc_plan *p, const fan_grid *gr, int passive, const double *bev, double *dose,
+                       const int doseDims[3]) {
     if (passive <= p->beamFirstInside) return;
     int st[3], mxI[3];
-    transfer_box(p, passive, st, mxI, doseDims);
+    transfer_box_g(p, gr, passive, st, mxI, doseDims);
     if (!(st[0] <= mxI[0] && st[1] <= mxI[1] && st[2] <= mxI[2])) return;
     /* the launch covers whole 32x8 blocks from the box start; threads stop only at the grid edge */
     int xEnd = st[0] + ((mxI[0] - st[0] + 1 + 31) / 32) * 32;
@@ -1019,8 +1152,7 @@ void orc_transfer(orc_plan *p, int l, const double *bev, double *dose, const int
     m3_to9(d2g.m, t9);
     for (int i = 0; i < 9; ++i) M[i] = t9[i];
     O[0] = d2g.v.x; O[1] = d2g.v.y; O[2] = d2g.v.z;
-    const int bevDims[3] = {p->superpDims[0] + 2 * ORC_MAX_RAD, p->superpDims[1] + 2 * ORC_MAX_RAD,
-                            p->guaranteedPassive};
+    const int bevDims[3] = {gr->sx + 2 * ORC_MAX_RAD, gr->sy + 2 * ORC_MAX_RAD, p->guaranteedPassive};
     const double shift[3] = {ORC_MAX_RAD, ORC_MAX_RAD, -(double)p->beamFirstInside};
 #pragma omp parallel for schedule(static)
     for (int y = st[1]; y < yEnd; ++y)
@@ -1030,14 +1162,19 @@ void orc_transfer(orc_plan *p, int l, const double *bev, double *dose, const int
                 for (int r = 0; r < 3; ++r) g[r] = M[3 * r] * x + M[3 * r + 1] * y + M[3 * r + 2] * z + O[r];
                 double fx = g[0] / (1.0 - g[2] / p->sourceDist[0]);
                 double fy = g[1] / (1.0 - g[2] / p->sourceDist[1]);
-                double f[3] = {(fx - p->fanCorner[0]) / p->fanDelta[0] + shift[0],
-                               (fy - p->fanCorner[1]) / p->fanDelta[1] + shift[1],
+                double f[3] = {(fx - gr->fc[0]) / gr->fd[0] + shift[0],
+                               (fy - gr->fc[1]) / gr->fd[1] + shift[1],
                                (g[2] - p->fanCorner[2]) / p->fanDelta[2] + shift[2]};
                 /* prim_transf_kernel.cu:28-40: pos = fanIdx + 0.5; skip negatives; add if > 0 */
                 if (f[0] + 0.5 < 0.0 || f[1] + 0.5 < 0.0 || f[2] + 0.5 < 0.0) continue;
                 double v = sample3d_border_d(bev, bevDims, f[0], f[1], f[2]);
                 if (v > 0.0) dose[((size_t)z * doseDims[1] + y) * doseDims[0] + x] += v;
             }
+}
+
+void orc_transfer(orc_plan *p, int l, const double *bev, double *dose, const int doseDims[3]) {
+    fan_grid g = primary_grid(p);
+    transfer_g(p, &g, p->layerPassive[l], bev, dose, doseDims);
 }
 
 /* ------------------------------------------------------------------------------------
@@ -1085,8 +1222,39 @@ int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], co
         iddS = (double *)malloc(sizeof(double) * splane * K);
         rsS = (double *)malloc(sizeof(double) * splane * K);
     }
-    for (int l = 0; l < L; ++l) {
-        orc_fill(p, l, en, density, wepl, rw + (size_t)l * rplane, fi, fo, idd, rs, fp);
+    /* halo branch (wrapper:3413-3450, 4062-4470, 5102-5215, 5510-5600) */
+    int *rayToNuc = NULL;
+    double *nucW = NULL, *nucIdd = NULL, *nucRs = NULL, *nucBev = NULL;
+    size_t nucPlane = 0, nucBevN = 0;
+    int haloFail = 0;
+    if (b->nuclear) {
+        if (p->legacyPath || !en->nucWeight || !en->nucSqSigma) {
+            snprintf(p->error, sizeof(p->error), p->legacyPath
+                         ? "nuclear_correction=true requires raw spot lattice inputs; legacy CPB fallback cannot drive halo mode"
+                         : "nuclear_correction=true requested, but the energy struct does not carry nuclear tables");
+            haloFail = 1;
+        } else {
+            rayToNuc = (int *)malloc(sizeof(int) * rplane);
+            nucW = (double *)malloc(sizeof(double) * (size_t)L * (((size_t)p->spotDims[0] + 31) / 32 * 32) *
+                                    (((size_t)p->spotDims[1] + 7) / 8 * 8));
+            if (orc_halo_plan(p, b, rayToNuc, nucW) != 0) haloFail = 1;
+            nucPlane = (size_t)p->nucDims[0] * p->nucDims[1];
+            nucIdd = (double *)malloc(sizeof(double) * nucPlane * K);
+            nucRs = (double *)malloc(sizeof(double) * nucPlane * K);
+            nucBevN = (size_t)(p->nucDims[0] + 64) * (p->nucDims[1] + 64) * (p->guaranteedPassive > 0 ? p->guaranteedPassive : 1);
+            nucBev = (double *)malloc(sizeof(double) * nucBevN);
+        }
+    }
+    fan_grid nucGrid = {p->nucDims[0], p->nucDims[1], {p->spotOffset[0], p->spotOffset[1]},
+                        {p->spotDelta[0], p->spotDelta[1]}};
+    for (int l = 0; l < L && !haloFail; ++l) {
+        if (b->nuclear) {
+            for (size_t i = 0; i < nucPlane * K; ++i) { nucIdd[i] = 0.0; nucRs[i] = INFINITY; }
+            orc_fill_halo(p, l, en, density, wepl, rw + (size_t)l * rplane, fi, fo, idd, rs, fp, b->nuclear, rayToNuc,
+                          nucW + (size_t)l * nucPlane, nucIdd, nucRs);
+        } else {
+            orc_fill(p, l, en, density, wepl, rw + (size_t)l * rplane, fi, fo, idd, rs, fp);
+        }
         int mxp = INT32_MIN;
         for (size_t r = 0; r < rplane; ++r) if (fp[r] > mxp) mxp = fp[r];
         /* wrapper:4984-4996 */
@@ -1113,7 +1281,20 @@ int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], co
         memset(bev, 0, sizeof(double) * bevN);
         orc_superpose(p, l, iddUse, rsUse, bev);
         orc_transfer(p, l, bev, dose, doseDims);
+        if (b->nuclear) {
+            int batch[ORC_MAX_RAD + 1];
+            memset(nucBev, 0, sizeof(double) * nucBevN);
+            if (superpose_grid(p->beamFirstInside, pas, p->nucDims[0], p->nucDims[1], nucIdd, nucRs, nucBev,
+                               p->nucRadHist[l], batch, 1) != 0) {
+                snprintf(p->error, sizeof(p->error), "nuclear superposition kernel radius exceeded %d (layer %d)",
+                         ORC_MAX_RAD, l);
+                haloFail = 1;
+                break;
+            }
+            transfer_g(p, &nucGrid, pas, nucBev, dose, doseDims);
+        }
     }
+    free(rayToNuc); free(nucW); free(nucIdd); free(nucRs); free(nucBev);
     /* union transfer box over layers, reported for tests */
     {
         int maxPas = p->beamFirstInside;
@@ -1123,5 +1304,5 @@ int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], co
     free(density); free(wepl); free(fi); free(fo); free(weplMin); free(rw);
     free(idd); free(rs); free(fp); free(bev); free(iddS); free(rsS);
     if (!planOut) free(p);
-    return 0;
+    return haloFail ? -1 : 0;
 }

@@ -58,7 +58,7 @@ typedef struct {
     float beamDirection[3], bmXDirection[3], bmYDirection[3], sourcePosition[3];
     float sad;
     int useSubspotConvolution; /* 0 = per-layer spot sigma (reference default) */
-    int nuclear;               /* 1 = halo branch (GAUSS_FIT constants) */
+    int nuclear;               /* halo branch: 0 off, 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA constants */
 } orc_beam;
 
 /* Beam plan: everything the host computes before and between kernels. */
@@ -97,6 +97,11 @@ typedef struct {
     int transferStart[3], transferMax[3];         /* union box over layers */
     int numLayers;
     char error[256];
+    /* halo grid (nuclear branch): spot lattice rounded up to 32x8, spot spacing in rays */
+    int nucDims[2];
+    float spotDist;
+    int nucRadHist[ORC_MAX_LAYERS][ORC_MAX_RAD + 2];
+    float entrySpotSigmaX[ORC_MAX_LAYERS];  /* spot sigma_x in mm (halo entry sigma) */
 } orc_plan;
 
 /* ---- helpers exposed for known-answer tests ---- */
@@ -156,6 +161,22 @@ void orc_fill(const orc_plan *plan, int layer, const orc_energy *en, const doubl
               const double *wepl, const double *rayWeights, const int *firstInside,
               const int *firstOutside, double *idd, double *rsigma, int *firstPassive);
 
+/* Halo lattice plan (wrapper:804-929; spot spacing wrapper:4062-4100, 4149-4172, 4338-4350):
+ * plan->nucDims = spot lattice dims rounded up to 32x8, plan->spotDist = spot spacing / fan
+ * ray spacing; rayToNuc[ray] = halo cell of the spot whose gantry position rounds to that ray
+ * (else -1); nucWeights[L][nucY][nucX] = the layer lattice weights (may be NULL).
+ * Returns 0, or -1 with plan->error (a spot outside the ray grid, spotDist < 1.5, no
+ * lattice). Pinned: hand-computed config-1 plan (11x11 spots at 5 rays). */
+int orc_halo_plan(orc_plan *plan, const orc_beam *beam, int *rayToNuc, double *nucWeights);
+/* IDD / sigma fill with the halo branch (idd_sigma.cu:46-259 under NUCLEAR_CORR): model 0 is
+ * exactly orc_fill; otherwise the primary takes (1 - w_nuc) of the energy and the model's
+ * scattering constants, and each spot's central ray writes the halo IDD and rSigmaEff into
+ * nucIdd / nucRsigma [K][nucY][nucX] (nucW = this layer's halo weights [nucY][nucX]).
+ * Pinned: energy split sum_k (res + nucRes spotDist^2) mass = w CIDD telescoping. */
+void orc_fill_halo(const orc_plan *plan, int layer, const orc_energy *en, const double *density,
+                   const double *wepl, const double *rayWeights, const int *firstInside,
+                   const int *firstOutside, double *idd, double *rsigma, int *firstPassive, int model,
+                   const int *rayToNuc, const double *nucW, double *nucIdd, double *nucRsigma);
 /* Superposition of one layer into a BEV volume [bevZ][ry+64][rx+64]
  * (complete_superposition.cu:197-544). Also records radius histogram and batching.
  * Pinned: per-pixel kernel mass erf(r(R+1/2))^2, brute-force scatter. */

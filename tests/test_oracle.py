@@ -497,3 +497,130 @@ def test_trace_ray_that_never_enters():
     p, dens, wepl, fi, fo, *_ = _trace(case)
     assert fi[0, 0] == p.steps and fo[0, 0] == 1
     assert dens[:, 0, 0].max() < 0.01
+
+
+# ---------------------------------------------------------------- halo (nuclear) branch ---
+EN = lut.load_energy_tables(nuclear=True)
+
+
+def test_halo_plan_config1():
+    """11x11 spots at 5 mm on 1 mm rays: halo grid 32x16, spotDist 5, each spot's cell mapped
+    from the ray at its position, cells carry the layer weights."""
+    case = F.make_config1()
+    keep = O._Keep()
+    bm = O.make_beam(case["beams"][0], keep)
+    p = O.plan_beam(case)
+    rx, ry = p.rayDims[0], p.rayDims[1]
+    r2n = np.zeros(rx * ry, np.int32)
+    L = len(case["beams"][0]["energies"])
+    nw = np.zeros((L, 16, 32))
+    assert O.lib().orc_halo_plan(C.byref(p), C.byref(bm), O._ip(r2n), O._dp(nw)) == 0, p.error
+    assert tuple(p.nucDims) == (32, 16)
+    assert p.spotDist == pytest.approx(5.0)
+    assert (r2n >= 0).sum() == 121
+    cx, cy = -int(round(p.cpbCorner[0])), -int(round(p.cpbCorner[1]))  # ray of x = 0, y = 0
+    w = np.asarray(case["beams"][0]["spot_weights"], np.float32).reshape(L, 11, 11)
+    for j in range(11):
+        for i in range(11):
+            ray = (cy - 25 + 5 * j) * rx + (cx - 25 + 5 * i)
+            assert r2n[ray] == j * 32 + i
+    np.testing.assert_array_equal(nw[:, :11, :11], w.astype(np.float64))
+    assert not nw[:, 11:, :].any() and not nw[:, :, 11:].any()
+
+
+def _halo_fill(case, model, spot_dist=5.0):
+    p, dens, wepl, fi, fo, keep, en, bm = _trace(case)
+    K = p.steps
+    wmin = np.zeros(K)
+    L = O.lib()
+    L.orc_beam_cutoffs(C.byref(p), C.byref(bm), C.byref(en), O._dp(wepl), O._ip(fi), O._ip(fo), O._dp(wmin))
+    rx, ry = p.rayDims[0], p.rayDims[1]
+    rw = np.zeros((1, ry, rx))
+    L.orc_conv_lattice(C.byref(p), C.byref(bm), C.byref(en), O._dp(rw))
+    L.orc_layer_params(C.byref(p), C.byref(bm), C.byref(en), O._dp(wmin))
+    i0 = int(round(-p.cpbCorner[0])); j0 = int(round(-p.cpbCorner[1]))
+    p.nucDims[0], p.nucDims[1] = 32, 8
+    p.spotDist = spot_dist
+    r2n = np.full(rx * ry, -1, np.int32)
+    r2n[j0 * rx + i0] = 0
+    nucw = np.zeros(32 * 8)
+    nucw[0] = rw[0, j0, i0]  # the central ray carries its own weight into the halo cell
+    idd = np.zeros((K, ry, rx)); rs = np.zeros((K, ry, rx)); fp = np.zeros((ry, rx), np.int32)
+    nidd = np.zeros((K, 8, 32)); nrs = np.full((K, 8, 32), np.inf)
+    L.orc_fill_halo(C.byref(p), 0, C.byref(en), O._dp(dens), O._dp(wepl), O._dp(rw[0]), O._ip(fi), O._ip(fo),
+                    O._dp(idd), O._dp(rs), O._ip(fp), model, O._ip(r2n), O._dp(nucw), O._dp(nidd), O._dp(nrs))
+    return p, dens, wepl, rw, idd, rs, fp, nidd, nrs, (i0, j0)
+
+
+def test_fill_halo_model_off_is_the_plain_fill():
+    case = _water_case(energies=(110.0,))
+    case["energy"] = EN
+    p, dens, wepl, rw, idd0, rs0, fp0, fi = _run_fill(case)
+    _, _, _, _, idd1, rs1, fp1, nidd, nrs, _ = _halo_fill(case, 0)
+    np.testing.assert_array_equal(idd0, idd1)
+    np.testing.assert_array_equal(rs0, rs1)
+    np.testing.assert_array_equal(fp0, fp1)
+    assert not nidd.any()
+
+
+def test_fill_halo_energy_split_telescopes():
+    """Primary and halo share each step's energy: sum_k (res + nucRes spotDist^2) mass = w CIDD,
+    and the halo part is sum_k w_nuc(mid-depth) dCIDD with w_nuc from the nuclear table at the
+    step's mid WEPL."""
+    case = _water_case(energies=(110.0,))
+    case["energy"] = EN
+    p, dens, wepl, rw, idd, rs, fp, nidd, nrs, (i0, j0) = _halo_fill(case, 1)
+    first, stop = p.beamFirstInside, fp[j0, i0]
+    ks = np.arange(first, stop)
+    vol = np.float64(p.volConst) + ks * np.float64(p.volLin) + ks.astype(np.float64) ** 2 * np.float64(p.volSq)
+    mass = dens[ks, j0, i0] * vol
+    w = rw[0, j0, i0]
+    prim = (idd[ks, j0, i0] * mass).sum()
+    halo = (nidd[ks, 0, 0] * mass).sum() * 25.0
+    assert prim + halo == pytest.approx(w * _cidd(wepl[stop - 1, j0, i0], p.scaleFact[0], p.energyIdx[0]), rel=1e-9)
+    # halo fraction per step from an independent bilinear lookup of the nuclear weight table
+    m = EN["nuclear_weight_matrix"].astype(np.float64)
+    e = p.energyIdx[0]; e0 = int(math.floor(e)); a = e - e0
+    xs = np.arange(m.shape[1])
+    wp = wepl[ks, j0, i0]
+    wprev = np.concatenate([[0.0], wp[:-1]])
+    mid = 0.5 * (wp + wprev) * p.scaleFact[0]
+    nw = (1 - a) * np.interp(mid, xs, m[e0]) + a * np.interp(mid, xs, m[e0 + 1])
+    dD = np.array([_cidd(x, p.scaleFact[0], e) for x in wp]) - np.array([_cidd(x, p.scaleFact[0], e) if x > 0 else 0.0
+                                                                          for x in wprev])
+    np.testing.assert_allclose(nidd[ks, 0, 0] * mass * 25.0, w * nw * dD, rtol=1e-9, atol=1e-18)
+    assert halo > 0.0 and halo < 0.3 * (prim + halo)
+
+
+@pytest.mark.parametrize("model,delta", [(1, 0.06), (2, 0.0), (3, 0.08)])
+def test_fill_halo_model_sigma_delta(model, delta):
+    """Without scattering the primary sigma is the air polynomial; each halo model adds its
+    own sigmaDelta (idd_sigma.cu:108-121)."""
+    e2 = dict(EN)
+    e2["rrl_vector"] = np.zeros_like(EN["rrl_vector"])
+    case = _water_case(energies=(120.0,))
+    case["energy"] = e2
+    p, dens, wepl, rw, idd, rs, fp, nidd, nrs, (i0, j0) = _halo_fill(case, model)
+    peak = np.float32(p.peakDepth[0])
+    a = np.float32(0.0027) / (peak - np.float32(4.5))
+    b = np.float32(-4.39) / (peak - np.float32(3.86))
+    first = p.beamFirstInside
+    k = first + 10
+    assert wepl[k, j0, i0] < p.peakDepth[0]
+    z = np.float64(p.fanCorner[2]) + k * np.float64(p.fanDelta[2])
+    zf = np.float64(p.fanCorner[2]) + first * np.float64(p.fanDelta[2])
+    s2 = float(a) * (z * z - zf * zf) + float(b) * (z - zf)
+    vw = np.float64(p.fanDelta[0]) * (1 - z / p.sourceDist[0])
+    assert rs[k, j0, i0] == pytest.approx(vw / (math.sqrt(2.0) * (math.sqrt(s2) + delta)), rel=2e-6)
+
+
+def test_halo_beam_conserves_integral_dose():
+    """The halo redistributes energy laterally: the integral dose of config 1 stays within 1 %
+    and the peak dose drops."""
+    case = F.make_config1()
+    case["energy"] = EN
+    d0, _ = O.beam_dose(case)
+    d1, p1 = O.beam_dose(case, nuclear=1)
+    assert d1.sum() == pytest.approx(d0.sum(), rel=1e-2)
+    assert d1.max() < d0.max()
+    assert sum(p1.nucRadHist[0]) > 0 and p1.nucRadHist[0][33] == 0
