@@ -120,7 +120,21 @@ void *pbs_engine_stream(pbs_engine *e);
  * bins / items, superposition, block merge, BEV->dose transfer, summed over the beams queued
  * since the previous call (their count in *beams). */
 int pbs_engine_set_profiling(pbs_engine *e, int enable);
+/* Device time per stage summed over the beams queued since the last call (profiling on):
+ * msOut[8] = trace, conv, fill, tiles, superpose, merge, transfer, halo. */
 int pbs_engine_stage_times(pbs_engine *e, float *msOut, int *beams);
+
+/* Nuclear (halo) branch for the following beams: 0 off, 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA
+ * (the reference's NUCLEAR_CORR build modes, idd_sigma.cu:108-121). Requires the energy
+ * struct's nuclear tables. The halo of each spot is computed on the spot lattice
+ * (raytracedicom_wrapper.cu:804-929, 4062-4470), superposed on its own grid with radius
+ * overflow as an error (complete_superposition.cu:558-708, reported by pbs_engine_sync) and
+ * transferred with its own fan geometry (prim_transf_kernel.cu:46-78, wrapper:5510-5600). */
+int pbs_engine_set_nuclear(pbs_engine *e, int model);
+/* Halo grid of the last beam: nucDims[2], spotDist, (IDD, rSigmaEff) [L][K][nucY][nucX][2],
+ * radius histograms [L][34], sticky overflow flag. Any pointer may be NULL. */
+int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float *haloIddSigma,
+                          int *haloRadHist, int *radOverflow);
 
 /* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
  * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside

@@ -701,7 +701,7 @@ void orc_conv_lattice(orc_plan *p, const orc_beam *b, const orc_energy *en, doub
                 float vx = ca * entryZ * entryZ + cb * entryZ + sgx * sgx;
                 float vy = ca * entryZ * entryZ + cb * entryZ + sgy * sgy;
                 double ex = sqrt((double)(vx > 0.f ? vx : 0.f)), ey = sqrt((double)(vy > 0.f ? vy : 0.f));
-                if (b->nuclear) { ex *= 0.97; ey *= 0.97; }
+                if (b->nuclear == 1) { ex *= 0.97; ey *= 0.97; } /* GAUSS_FIT only, wrapper:3788-3795 */
                 conv_plane(spot, sx, sy, out, rx, ry, ex, ey, p,
                            p->spotOffset[0] + ss[0] / p->lenToMm, p->spotOffset[1] + ss[1] / p->lenToMm,
                            pxX, pxY, (double)ss[2], done);
@@ -714,7 +714,7 @@ void orc_conv_lattice(orc_plan *p, const orc_beam *b, const orc_energy *en, doub
             float vx = ca * entryZ * entryZ + cb * entryZ + sgx * sgx;
             float vy = ca * entryZ * entryZ + cb * entryZ + sgy * sgy;
             double ex = sqrt((double)(vx > 0.f ? vx : 0.f)), ey = sqrt((double)(vy > 0.f ? vy : 0.f));
-            if (b->nuclear) { ex *= 0.97; ey *= 0.97; }
+            if (b->nuclear == 1) { ex *= 0.97; ey *= 0.97; } /* GAUSS_FIT only, wrapper:3788-3795 */
             p->entrySigma[l][0] = (float)ex; p->entrySigma[l][1] = (float)ey;
             conv_plane(spot, sx, sy, out, rx, ry, ex, ey, p, p->spotOffset[0], p->spotOffset[1], pxX, pxY, 1.0, 0);
         }

@@ -81,7 +81,7 @@ EXPORTED = [
     "pbs_subsecond_wrapper", "pbs_last_error", "pbs_device_count",
     "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
     "pbs_engine_beam_dose", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
-    "pbs_engine_set_profiling", "pbs_engine_stage_times",
+    "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_nuclear", "pbs_engine_debug_halo",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
 ]
 
@@ -119,6 +119,10 @@ def lib():
         L.pbs_engine_set_profiling.argtypes = [C.c_void_p, C.c_int]
         L.pbs_engine_stage_times.restype = C.c_int
         L.pbs_engine_stage_times.argtypes = [C.c_void_p, _f, _i]
+        L.pbs_engine_set_nuclear.restype = C.c_int
+        L.pbs_engine_set_nuclear.argtypes = [C.c_void_p, C.c_int]
+        L.pbs_engine_debug_halo.restype = C.c_int
+        L.pbs_engine_debug_halo.argtypes = [C.c_void_p, _i, _f, C.c_void_p, C.c_void_p, _i]
         L.pbs_final_dose_bind.restype = C.c_int
         L.pbs_final_dose_bind.argtypes = [C.c_void_p, _f, _f, C.c_int, C.c_float]
         L.cudaFinalDose.restype = C.c_int

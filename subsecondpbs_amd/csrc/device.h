@@ -35,6 +35,19 @@ struct LutDev {
 struct LayerHost {
     float energyIdx, scaleFact, peakDepth, rangeStop, cutoffWepl, airQuad, airLin;
     float convA, convB, sigmaX, sigmaY, pad;
+    float airA, airC;  // sigmaSqAirCoefs(peakDepth): entry sigma^2 of the halo
+    float pad1, pad2;
+};
+
+// Halo (nuclear) branch inputs and outputs of the fill; rayToNuc == nullptr switches it off.
+struct HaloFill {
+    const int *rayToNuc;       // [ry][rx]: halo cell of the spot centred on the ray, or -1
+    const float *weights;      // [L][nucPlane] spot weights
+    float2 *out;               // [L][K][nucPlane] (IDD, rSigmaEff)
+    const float *nucWeightLut; // [nE][nS] fraction of the energy in the halo
+    const float *nucSqSigmaLut;// [nE][nS] halo sigma^2
+    int nucPlane;
+    float spotDist;            // spot spacing in rays
 };
 
 // Per-beam values produced on the device.
@@ -42,7 +55,7 @@ struct BeamScalars {
     int beamFirstInside, beamFirstOutside, firstPastCutoffAll, guaranteedPassive;
     float entryZ, pxMultX, pxMultY, pad0;
     int maxPassive, mainEndMax, activeLayers, numGroups;
-    int tailPlanes, overflow, pad1, pad2;
+    int tailPlanes, overflow, radOverflow, pad2;  // radOverflow: a tile past the largest radius
     int boxStart[3], boxEnd[3];  // union of the group transfer boxes, [start, end)
     int boxMax[3], pad3;
 };

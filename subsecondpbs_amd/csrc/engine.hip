@@ -62,6 +62,40 @@ struct DevBuf {
 
 using namespace pbs;
 
+// Buffers of one superposition grid (tiles, work lists, item blocks, BEV planes, groups).
+struct GridWork {
+    DevBuf<int> tileRad, hist, effRad, itemR, classCount, tailPlaneGroup, tailPlaneK, tailItemR;
+    DevBuf<int2> classList;
+    DevBuf<float> slots, tailSlots, bev, tailBev;
+    DevBuf<GroupScalars> groupScal;
+    int maxItems = 0, maxUnits = 0, tailCap = 0, nt = 0;
+
+    hipError_t ensure(int L, int K, int sx, int sy) {
+        nt = (sx / 32) * (sy / 8);
+        maxItems = K * nt;
+        // a group's tail is at most min(beamFirstInside + 1, K - beamFirstInside) <= (K + 1) / 2
+        // steps long, and there are at most min(L, K) groups
+        tailCap = std::min(L, K) * std::max(1, (K + 1) / 2);
+        maxUnits = maxItems + tailCap * nt;
+        const size_t bevPlane = size_t(sx + 64) * (sy + 64);
+        hipError_t r;
+        if ((r = tileRad.ensure(size_t(L) * K * nt)) != hipSuccess) return r;
+        if ((r = hist.ensure(size_t(L) * (kMaxRadDev + 2))) != hipSuccess) return r;
+        if ((r = effRad.ensure(size_t(L) * (kMaxRadDev + 2))) != hipSuccess) return r;
+        if ((r = itemR.ensure(maxItems)) != hipSuccess) return r;
+        if ((r = classCount.ensure(16)) != hipSuccess) return r;
+        if ((r = classList.ensure(size_t(kNumClasses) * maxUnits)) != hipSuccess) return r;
+        if ((r = slots.ensure(size_t(maxItems) * kSlotFloats)) != hipSuccess) return r;
+        if ((r = bev.ensure(size_t(K) * bevPlane)) != hipSuccess) return r;
+        if ((r = groupScal.ensure(L)) != hipSuccess) return r;
+        if ((r = tailPlaneGroup.ensure(tailCap)) != hipSuccess) return r;
+        if ((r = tailPlaneK.ensure(tailCap)) != hipSuccess) return r;
+        if ((r = tailItemR.ensure(size_t(tailCap) * nt)) != hipSuccess) return r;
+        if ((r = tailSlots.ensure(size_t(tailCap) * nt * kSlotFloats)) != hipSuccess) return r;
+        return tailBev.ensure(size_t(tailCap) * bevPlane);
+    }
+};
+
 struct pbs_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -79,20 +113,27 @@ struct pbs_engine {
     int doseDims[3] = {0, 0, 0};
     float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
     // per-beam work buffers
-    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, slots, bev, cpb;
-    DevBuf<float> tailSlots, tailBev;
+    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, cpb;
     DevBuf<float2> iddSigma;
-    DevBuf<int> fi, fo, firstPassive, tileRad, hist, effRad, itemR, classCount;
-    DevBuf<int2> classList;
-    DevBuf<int> tailPlaneGroup, tailPlaneK, tailItemR;
-    DevBuf<GroupScalars> groupScal;
+    DevBuf<int> fi, fo, firstPassive;
+    // superposition -> merge -> transfer state of the primary (ray) grid and the halo grid
+    GridWork prim, halo;
+    // halo branch: spot-centre map, halo weights and fields, its own layer/beam scalars
+    int nuclearModel = 0;  // 0 off, 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA
+    HaloPlan haloPlan;
+    DevBuf<int> rayToNuc, haloErr;
+    DevBuf<float> haloW;
+    DevBuf<float2> haloIddSigma;
+    DevBuf<LayerScalars> haloLayerScal;
+    DevBuf<BeamScalars> haloBeamScal;
+    FanGeom haloGeom{};
+    bool lastHalo = false, haloUsed = false;
     DevBuf<LayerHost> layerHost;
     DevBuf<LayerScalars> layerScal;
     DevBuf<BeamScalars> beamScal;
     // last beam (debug fetch)
     BeamPlan plan;
     FanGeom geom{};
-    int lastMaxItems = 0, lastTailCap = 0;
     bool haveLast = false;
     // host staging of the last beam's uploads; the next beam waits for uploadDone before reuse
     std::vector<LayerHost> hostLayers;
@@ -113,7 +154,7 @@ struct pbs_engine {
 
 namespace {
 
-constexpr int kStages = 7;  // trace, conv, fill, tiles, superpose, merge, transfer
+constexpr int kStages = 8;  // trace, conv, fill, tiles, superpose, merge, transfer, halo
 
 hipEvent_t take_event(pbs_engine *e) {
     if (!e->evPool.empty()) {
@@ -152,15 +193,15 @@ void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
 
 // Host worst-case transfer launch extents: the box for steps [0, K-1] contains every
 // layer's box (the fan frustum grows with its depth range).
-void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &gy, int &gz) {
-    const float xv[2] = {-32.0f, float(p.superpDims[0] + 32 - 1)};
-    const float yv[2] = {-32.0f, float(p.superpDims[1] + 32 - 1)};
+void host_transfer_grid(const BeamPlan &p, const FanGeom &g, const int doseDims[3], int &gx, int &gy, int &gz) {
+    const float xv[2] = {-32.0f, float(g.sx + 32 - 1)};
+    const float yv[2] = {-32.0f, float(g.sy + 32 - 1)};
     const float zv[2] = {0.0f, float(p.steps - 1)};
     float mx[3] = {-1.f, -1.f, -1.f}, mn[3] = {1e30f, 1e30f, 1e30f};
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int c = 0; c < 2; ++c) {
-                F3 f = f3(xv[c] * p.fanDelta[0] + p.fanCorner[0], yv[b] * p.fanDelta[1] + p.fanCorner[1],
+                F3 f = f3(xv[c] * g.fd[0] + g.fc[0], yv[b] * g.fd[1] + g.fc[1],
                           zv[a] * p.fanDelta[2] + p.fanCorner[2]);
                 f.x *= (1.0f - f.z / p.sourceDist[0]);
                 f.y *= (1.0f - f.z / p.sourceDist[1]);
@@ -179,11 +220,11 @@ void host_transfer_grid(const BeamPlan &p, const int doseDims[3], int &gx, int &
     gz = std::max(1, ez - sz + 1);
 }
 
-void fill_transfer_args(const BeamPlan &p, TransferArgs &t) {
-    // doseIdx -> gantry (inverse of gantryToDoseIdx), then gantry -> fan index with the
-    // (+32, +32) BEV halo shift; the -beamFirstInside z shift is applied on the device.
+void fill_transfer_args(const BeamPlan &p, const FanGeom &g, TransferArgs &t) {
+    // doseIdx -> gantry (inverse of gantryToDoseIdx), then gantry -> cell index of grid g with
+    // the (+32, +32) BEV halo shift; the -beamFirstInside z shift is applied on the device.
     const M3 &m = p.dose2g.m;
-    const float dlt[3] = {1.0f / p.fanDelta[0], 1.0f / p.fanDelta[1], 1.0f / p.fanDelta[2]};
+    const float dlt[3] = {1.0f / g.fd[0], 1.0f / g.fd[1], 1.0f / p.fanDelta[2]};
     const float col0[3] = {m.r0.x, m.r1.x, m.r2.x}, col1[3] = {m.r0.y, m.r1.y, m.r2.y}, col2[3] = {m.r0.z, m.r1.z, m.r2.z};
     const float off[3] = {p.dose2g.v.x, p.dose2g.v.y, p.dose2g.v.z};
     const float shift[3] = {32.0f, 32.0f, 0.0f};
@@ -192,7 +233,8 @@ void fill_transfer_args(const BeamPlan &p, TransferArgs &t) {
         t.coefJ[i] = col1[i] * dlt[i];
         t.inc[i] = col2[i] * dlt[i];
         t.coefOffset[i] = off[i] * dlt[i];
-        t.globalOffset[i] = -p.fanCorner[i] * dlt[i] + shift[i];
+        const float corner = i < 2 ? g.fc[i] : p.fanCorner[2];
+        t.globalOffset[i] = -corner * dlt[i] + shift[i];
     }
     t.normDist[0] = dlt[2] * p.sourceDist[0];
     t.normDist[1] = dlt[2] * p.sourceDist[1];
@@ -247,6 +289,10 @@ pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
         hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&e->forkEv, hipEventDisableTiming) != hipSuccess) {
         set_error(PBS_E_DEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    if (e->haloErr.ensure(1) != hipSuccess || hipMemset(e->haloErr.p, 0, sizeof(int)) != hipSuccess) {
+        set_error(PBS_E_NOMEM, "hipMalloc failed");
         return nullptr;
     }
     for (int i = 0; i < kSideStreams; ++i) {
@@ -311,6 +357,46 @@ int pbs_engine_sync(pbs_engine *e) {
     if (!e) return set_error(PBS_E_INVALID, "null engine");
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    if (e->haloUsed) {
+        // halo radius overflow is an error (complete_superposition.cu:558-620), raised here
+        e->haloUsed = false;
+        int flag = 0;
+        HIP_TRY(hipMemcpy(&flag, e->haloErr.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (flag) {
+            const int zero = 0;
+            HIP_TRY(hipMemcpy(e->haloErr.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+            return set_error(PBS_E_GEOMETRY, "nuclear superposition kernel radius exceeded 32: the halo of a queued "
+                                             "beam is wider than the largest superposition radius; its dose is invalid");
+        }
+    }
+    return PBS_OK;
+}
+
+int pbs_engine_set_nuclear(pbs_engine *e, int model) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    if (model < 0 || model > 3) return set_error(PBS_E_INVALID, "nuclear model must be 0 (off), 1 (GAUSS_FIT), 2 (SOUKUP) or 3 (FLUKA)");
+    if (model && !(e->nucW.p && e->nucS2.p))
+        return set_error(PBS_E_GEOMETRY, "nuclear_correction=true requested, but the energy struct does not carry nuclear tables");
+    e->nuclearModel = model;
+    return PBS_OK;
+}
+
+int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float *haloIddSigma, int *haloRadHist,
+                          int *radOverflow) {
+    if (!e || !e->haveLast) return set_error(PBS_E_INVALID, "no beam has been computed");
+    if (!e->lastHalo) return set_error(PBS_E_INVALID, "the last beam ran without the halo branch");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const HaloPlan &hp = e->haloPlan;
+    const int L = e->plan.numLayers, K = e->plan.steps;
+    if (nucDims) { nucDims[0] = hp.nucDims[0]; nucDims[1] = hp.nucDims[1]; }
+    if (spotDist) *spotDist = hp.spotDist;
+    const size_t nucPlane = size_t(hp.nucDims[0]) * hp.nucDims[1];
+    if (haloIddSigma)
+        HIP_TRY(hipMemcpy(haloIddSigma, e->haloIddSigma.p, nucPlane * K * L * 8, hipMemcpyDeviceToHost));
+    if (haloRadHist)
+        HIP_TRY(hipMemcpy(haloRadHist, e->halo.hist.p, size_t(L) * (kMaxRadDev + 2) * 4, hipMemcpyDeviceToHost));
+    if (radOverflow) HIP_TRY(hipMemcpy(radOverflow, e->haloErr.p, sizeof(int), hipMemcpyDeviceToHost));
     return PBS_OK;
 }
 
@@ -362,6 +448,43 @@ int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *do
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
 }
 
+// One grid's stages after the fill: layer groups and boxes, tile radii, bins, work units;
+// the radius classes on the engine and side streams; merge; transfer into the dose.
+// `mark` (optional) records a stage boundary on the engine stream.
+static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork &w, const float2 *iddSigma,
+                    LayerScalars *ls, BeamScalars *bs, int *overflowFlag, float *doseDev,
+                    const hipStream_t streams[1 + kSideStreams], std::vector<hipEvent_t> *marks) {
+    hipStream_t s = streams[0];
+    auto mk = [&]() {
+        if (!marks || !e->profiling) return hipSuccess;
+        marks->push_back(take_event(e));
+        return hipEventRecord(marks->back(), s);
+    };
+    HIP_TRY(mk());
+    HIP_TRY(launch_finalize_and_tiles(g, iddSigma, ls, w.groupScal.p, bs, w.tailPlaneGroup.p, w.tailPlaneK.p,
+                                      w.tailCap, w.tileRad.p, w.hist.p, w.effRad.p, w.itemR.p, w.tailItemR.p,
+                                      w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, s));
+    HIP_TRY(mk());
+    HIP_TRY(hipEventRecord(e->forkEv, s));
+    for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
+    HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
+                             w.classList.p, w.maxUnits, w.slots.p, w.tailSlots.p, 4096, streams));
+    for (int i = 0; i < kSideStreams; ++i) {
+        HIP_TRY(hipEventRecord(e->joinEv[i], streams[1 + i]));
+        HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
+    }
+    HIP_TRY(mk());
+    HIP_TRY(launch_merge(g, bs, w.itemR.p, w.slots.p, w.bev.p, w.tailItemR.p, w.tailSlots.p, w.tailBev.p, w.tailCap,
+                         s));
+    TransferArgs ta{};
+    fill_transfer_args(p, g, ta);
+    int gx = 1, gy = 1, gz = 1;
+    host_transfer_grid(p, g, e->doseDims, gx, gy, gz);
+    HIP_TRY(mk());
+    HIP_TRY(launch_transfer(g, ta, bs, w.groupScal.p, w.bev.p, w.tailBev.p, doseDev, gx, gy, gz, s));
+    return PBS_OK;
+}
+
 // cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
@@ -383,15 +506,16 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     fill_geom(p, *e, g);
     const int L = p.numLayers, K = p.steps, rx = g.rx, ry = g.ry, sx = g.sx, sy = g.sy;
     const size_t nray = size_t(rx) * ry, nsup = size_t(sx) * sy;
-    const int nt = (sx / 32) * (sy / 8);
-    const int maxItems = K * nt;
-    // tail planes: a group's tail is at most min(beamFirstInside + 1, K - beamFirstInside)
-    // <= (K + 1) / 2 steps long, and there are at most min(L, K) groups
-    const int tailCap = std::min(L, K) * std::max(1, (K + 1) / 2);
-    const size_t bevPlane = size_t(sx + 64) * (sy + 64);
-    const int maxUnits = maxItems + tailCap * nt;  // main blocks + tail blocks
     const bool subMode = p.subspotConv && !p.legacyPath;
     const int nSub = subMode ? beam->maxSubspotsPerLayer : 1;
+    const bool halo = e->nuclearModel != 0;
+    if (halo) {
+        if (!e->nucW.p || !e->nucS2.p)
+            return set_error(PBS_E_GEOMETRY,
+                             "nuclear_correction=true requested, but the energy struct does not carry nuclear tables");
+        std::string herr;
+        if (!plan_halo(p, *beam, e->haloPlan, herr)) return set_error(PBS_E_GEOMETRY, herr);
+    }
     // buffers
     HIP_TRY(e->density.ensure(nray * K));
     HIP_TRY(e->wepl.ensure(nray * K));
@@ -401,34 +525,52 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(e->rayW.ensure(nray * L));
     HIP_TRY(e->iddSigma.ensure(nsup * K * L));
     HIP_TRY(e->firstPassive.ensure(nray * L));
-    HIP_TRY(e->tileRad.ensure(size_t(L) * K * nt));
-    HIP_TRY(e->hist.ensure(size_t(L) * (kMaxRadDev + 2)));
-    HIP_TRY(e->effRad.ensure(size_t(L) * (kMaxRadDev + 2)));
-    HIP_TRY(e->itemR.ensure(maxItems));
-    HIP_TRY(e->classCount.ensure(16));
-    HIP_TRY(e->classList.ensure(size_t(kNumClasses) * maxUnits));
-    HIP_TRY(e->slots.ensure(size_t(maxItems) * kSlotFloats));
-    HIP_TRY(e->bev.ensure(size_t(K) * bevPlane));
-    HIP_TRY(e->groupScal.ensure(L));
-    HIP_TRY(e->tailPlaneGroup.ensure(tailCap));
-    HIP_TRY(e->tailPlaneK.ensure(tailCap));
-    HIP_TRY(e->tailItemR.ensure(size_t(tailCap) * nt));
-    HIP_TRY(e->tailSlots.ensure(size_t(tailCap) * nt * kSlotFloats));
-    HIP_TRY(e->tailBev.ensure(size_t(tailCap) * bevPlane));
+    HIP_TRY(e->prim.ensure(L, K, sx, sy));
     HIP_TRY(e->layerHost.ensure(L));
     HIP_TRY(e->layerScal.ensure(L));
     HIP_TRY(e->beamScal.ensure(1));
     HIP_TRY(e->subspots.ensure(size_t(L) * beam->maxSubspotsPerLayer * 5));
     HIP_TRY(e->subEntry.ensure(size_t(L) * beam->maxSubspotsPerLayer * 2));
+    FanGeom &gh = e->haloGeom;
+    HaloFill hf{};
+    if (halo) {
+        const HaloPlan &hp = e->haloPlan;
+        gh = g;  // the halo grid: cells are spots (nucIdxToFan, wrapper:5527-5535)
+        gh.sx = gh.rx = hp.nucDims[0];
+        gh.sy = gh.ry = hp.nucDims[1];
+        gh.fc[0] = p.spotOffset[0]; gh.fc[1] = p.spotOffset[1];
+        gh.fd[0] = p.spotDelta[0]; gh.fd[1] = p.spotDelta[1];
+        const size_t nucPlane = size_t(hp.nucDims[0]) * hp.nucDims[1];
+        HIP_TRY(e->halo.ensure(L, K, gh.sx, gh.sy));
+        HIP_TRY(e->rayToNuc.ensure(nray));
+        HIP_TRY(e->haloW.ensure(nucPlane * L));
+        HIP_TRY(e->haloIddSigma.ensure(nucPlane * K * L));
+        HIP_TRY(e->haloLayerScal.ensure(L));
+        HIP_TRY(e->haloBeamScal.ensure(1));
+        hf.rayToNuc = e->rayToNuc.p;
+        hf.weights = e->haloW.p;
+        hf.out = e->haloIddSigma.p;
+        hf.nucWeightLut = e->nucW.p;
+        hf.nucSqSigmaLut = e->nucS2.p;
+        hf.nucPlane = int(nucPlane);
+        hf.spotDist = hp.spotDist;
+    }
     std::vector<LayerHost> &lh = e->hostLayers;
     lh.resize(L);
     for (int l = 0; l < L; ++l) {
         const LayerParams &q = p.layers[l];
         lh[l] = LayerHost{q.energyIdx, q.scaleFact, q.peakDepth, q.rangeStop, q.cutoffWepl, q.airQuad, q.airLin,
-                          q.convA, q.convB, q.sigmaX, q.sigmaY, 0.0f};
+                          q.convA, q.convB, q.sigmaX, q.sigmaY, 0.0f, q.airA, q.airC, 0.0f, 0.0f};
     }
     hipStream_t s = e->stream;
     HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    if (halo) {
+        const HaloPlan &hp = e->haloPlan;
+        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, hp.rayToNuc.data(), sizeof(int) * nray, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(e->haloW.p, hp.weights.data(), sizeof(float) * hp.weights.size(),
+                               hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
+    }
     HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
                            hipMemcpyHostToDevice, s));
     std::vector<hipEvent_t> marks;
@@ -441,10 +583,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(mark());
     HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
-    const float nucFactor = 1.0f;
+    const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
-                              e->classCount.p, e->hist.p, s));
+                              e->prim.classCount.p, e->prim.hist.p, s));
     // 2. ray weights
     HIP_TRY(mark());
     if (!p.legacyPath) {
@@ -489,46 +631,33 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     }
     HIP_TRY(hipEventRecord(e->uploadDone, s));
     e->uploadPending = true;
-    // 3. IDD / sigma
+    // 3. IDD / sigma (+ the halo's share when the halo branch is on; idd_sigma.cu:108-121)
     HIP_TRY(mark());
-    const FillConsts fc{198.81f, 0.21f, 0, 0};
+    FillConsts fc{198.81f, 0.21f, 0, 0};
+    if (e->nuclearModel == 1) fc = FillConsts{169.00f, 0.06f, 1, 0};  // GAUSS_FIT
+    if (e->nuclearModel == 2) fc = FillConsts{190.44f, 0.0f, 0, 0};   // SOUKUP
+    if (e->nuclearModel == 3) fc = FillConsts{216.09f, 0.08f, 0, 0};  // FLUKA
     HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
-                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, s));
-    // 4. passive steps, tiles, bins, items
-    HIP_TRY(mark());
-    HIP_TRY(launch_finalize_and_tiles(g, e->iddSigma.p, e->layerScal.p, e->groupScal.p, e->beamScal.p,
-                                      e->tailPlaneGroup.p, e->tailPlaneK.p, tailCap, e->tileRad.p, e->hist.p,
-                                      e->effRad.p, e->itemR.p, e->tailItemR.p, e->classCount.p, e->classList.p,
-                                      maxUnits, s));
-    // 5. superposition into per-item blocks, then the deterministic merge into the BEV volume
-    HIP_TRY(mark());
-    HIP_TRY(hipEventRecord(e->forkEv, s));
+                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, s));
+    // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
-    for (int i = 0; i < kSideStreams; ++i) {
-        HIP_TRY(hipStreamWaitEvent(e->side[i], e->forkEv, 0));
-        streams[1 + i] = e->side[i];
-    }
-    HIP_TRY(launch_superpose(g, e->iddSigma.p, e->tileRad.p, e->effRad.p, e->layerScal.p, e->groupScal.p,
-                             e->beamScal.p, e->classCount.p, e->classList.p, maxUnits, e->slots.p, e->tailSlots.p,
-                             4096, streams));
-    for (int i = 0; i < kSideStreams; ++i) {
-        HIP_TRY(hipEventRecord(e->joinEv[i], e->side[i]));
-        HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
-    }
+    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];
+    int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, doseDev, streams,
+                       &marks);
+    if (rc2 != PBS_OK) return rc2;
+    // 7. halo grid through the same stages
     HIP_TRY(mark());
-    HIP_TRY(launch_merge(g, e->beamScal.p, e->itemR.p, e->slots.p, e->bev.p, e->tailItemR.p, e->tailSlots.p,
-                         e->tailBev.p, tailCap, s));
-    // 6. BEV -> dose
-    TransferArgs ta{};
-    fill_transfer_args(p, ta);
-    int gx = 1, gy = 1, gz = 1;
-    host_transfer_grid(p, e->doseDims, gx, gy, gz);
-    HIP_TRY(mark());
-    HIP_TRY(launch_transfer(g, ta, e->beamScal.p, e->groupScal.p, e->bev.p, e->tailBev.p, doseDev, gx, gy, gz, s));
+    if (halo) {
+        HIP_TRY(launch_seed_grid(L, e->layerScal.p, e->beamScal.p, e->haloLayerScal.p, e->haloBeamScal.p,
+                                 e->halo.hist.p, e->halo.classCount.p, s));
+        rc2 = run_grid(e, p, gh, e->halo, e->haloIddSigma.p, e->haloLayerScal.p, e->haloBeamScal.p, e->haloErr.p,
+                       doseDev, streams, nullptr);
+        if (rc2 != PBS_OK) return rc2;
+    }
+    e->lastHalo = halo;
+    e->haloUsed |= halo;
     HIP_TRY(mark());
     if (e->profiling) e->evBeams.push_back(std::move(marks));
-    e->lastMaxItems = maxItems;
-    e->lastTailCap = tailCap;
     e->haveLast = true;
     return PBS_OK;
 }
@@ -581,9 +710,9 @@ int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, 
     HIP_TRY(cp(rayWeights, e->rayW.p, nray * L * 4));
     HIP_TRY(cp(iddSigma, e->iddSigma.p, nsup * K * L * 8));
     HIP_TRY(cp(firstPassive, e->firstPassive.p, nray * L * 4));
-    HIP_TRY(cp(tileRad, e->tileRad.p, size_t(L) * K * nt * 4));
-    HIP_TRY(cp(radHist, e->hist.p, size_t(L) * (kMaxRadDev + 2) * 4));
-    HIP_TRY(cp(bev, e->bev.p, size_t(K) * (g.sx + 64) * (g.sy + 64) * 4));
+    HIP_TRY(cp(tileRad, e->prim.tileRad.p, size_t(L) * K * nt * 4));
+    HIP_TRY(cp(radHist, e->prim.hist.p, size_t(L) * (kMaxRadDev + 2) * 4));
+    HIP_TRY(cp(bev, e->prim.bev.p, size_t(K) * (g.sx + 64) * (g.sy + 64) * 4));
     if (ints) {
         BeamScalars bs{};
         std::vector<LayerScalars> ls(L);
@@ -619,7 +748,8 @@ int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, 
         ints->mainEndMax = bs.mainEndMax;
         std::vector<GroupScalars> gs(std::max(bs.numGroups, 1));
         if (bs.numGroups > 0)
-            HIP_TRY(hipMemcpy(gs.data(), e->groupScal.p, sizeof(GroupScalars) * bs.numGroups, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(gs.data(), e->prim.groupScal.p, sizeof(GroupScalars) * bs.numGroups,
+                              hipMemcpyDeviceToHost));
         for (int l = 0; l < L; ++l) {
             ints->mainEnd[l] = ls[l].mainEnd;
             ints->layerGroup[l] = ls[l].group;
@@ -659,12 +789,15 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
     }
     if (nuclearCorrection && !(energy->nucWeightMatrix && energy->nucSqSigmaMatrix))
         return set_error(PBS_E_GEOMETRY, "nuclear_correction=true requested, but the energy struct does not carry nuclear tables");
-    if (nuclearCorrection) return set_error(PBS_E_UNSUPPORTED, "nuclear correction is not available in this build");
+    if (nuclearCorrection < 0 || nuclearCorrection > 3)
+        return set_error(PBS_E_INVALID, "nuclearCorrection must be 0, 1 (GAUSS_FIT), 2 (SOUKUP) or 3 (FLUKA)");
     std::unique_ptr<pbs_engine, void (*)(pbs_engine *)> e(pbs_engine_create(gpuId, energy), pbs_engine_destroy);
     if (!e) return PBS_E_DEVICE;
     rc = pbs_engine_set_ct(e.get(), ctData, ctDims, ctResolution, ctCorner, 0);
     if (rc != PBS_OK) return rc;
     rc = pbs_engine_set_dose_grid(e.get(), doseDims, doseResolution, doseCorner);
+    if (rc != PBS_OK) return rc;
+    rc = pbs_engine_set_nuclear(e.get(), nuclearCorrection);
     if (rc != PBS_OK) return rc;
     const size_t nd = size_t(doseDims[0]) * doseDims[1] * doseDims[2];
     DevBuf<float> dose;
@@ -674,7 +807,8 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
         rc = pbs_engine_beam_dose(e.get(), &beams[b], dose.p);
         if (rc != PBS_OK) return rc;
     }
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    rc = pbs_engine_sync(e.get());  // reports a halo radius overflow before the dose is copied back
+    if (rc != PBS_OK) return rc;
     HIP_TRY(hipMemcpy(doseData, dose.p, nd * sizeof(float), hipMemcpyDeviceToHost));
     return PBS_OK;
 }

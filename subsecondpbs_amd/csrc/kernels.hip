@@ -382,7 +382,7 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
                                               const float *__restrict__ density, const float *__restrict__ wepl,
                                               const float *__restrict__ rayW, const int *__restrict__ fi,
                                               const int *__restrict__ fo, float2 *__restrict__ iddSigma,
-                                              int *__restrict__ firstPassive, FillConsts fc) {
+                                              int *__restrict__ firstPassive, FillConsts fc, HaloFill hf) {
     extern __shared__ float s_rrl[];
     for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
     __syncthreads();
@@ -409,8 +409,22 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
     const size_t plane = size_t(nray);
     float2 *out = iddSigma + (size_t(l) * g.K * g.sy + y) * g.sx + x;
     const size_t outStep = size_t(g.sx) * g.sy;
+    // halo cell of this ray (the spot it centres) and the halo entry sigma^2 (wrapper:4324-4329)
+    const int cell = hf.rayToNuc ? hf.rayToNuc[ray] : -1;
+    const bool halo = hf.rayToNuc != nullptr;
+    float nucRayW = 0.0f, entrySigmaSq = 0.0f;
+    float2 *nout = nullptr;
+    if (cell >= 0) {
+        nucRayW = hf.weights[size_t(l) * hf.nucPlane + cell];
+        nout = hf.out + size_t(l) * g.K * hf.nucPlane + cell;
+        {
+#pragma clang fp contract(off)
+            const float ez = bs->entryZ;
+            entrySigmaSq = L.airA * ez * ez + L.airC * ez + L.sigmaX * L.sigmaX;
+        }
+    }
     for (unsigned k = first; k < pAfter; ++k) {
-        float res = 0.0f, rs = PBS_INF;
+        float res = 0.0f, rs = PBS_INF, nres = 0.0f, nrs = PBS_INF;
         const size_t idx = size_t(k) * plane + ray;
         if (live) {
             cumulSp = wepl[idx];
@@ -434,13 +448,29 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
             rs = 0.5f * (vwx + vwy) / (1.41421356f * (sqrtf(sigmaSq) + fc.sigmaDelta));
             const float fk = float(k);
             const float mass = rho * (g.volConst + fk * g.volLin + fk * fk * g.volSq);
-            if (mass > 1e-2f) res = w * (cumulDose - cumulDoseOld) / mass;
+            const float depthMid = 0.5f * (cumulSp + cumulSpOld) * L.scaleFact;
+            if (mass > 1e-2f) {
+                if (halo) {
+                    // idd_sigma.cu:194-213: the halo takes w_nuc of the step's energy
+                    const float nw = lut2d_clamp(hf.nucWeightLut, lut.nS, lut.nE, depthMid, L.energyIdx);
+                    res = (1.0f - nw) * w * (cumulDose - cumulDoseOld) / mass;
+                    if (cell >= 0)
+                        nres = nw * nucRayW * (cumulDose - cumulDoseOld) / (mass * hf.spotDist * hf.spotDist);
+                } else {
+                    res = w * (cumulDose - cumulDoseOld) / mass;
+                }
+            }
+            if (cell >= 0) {
+                const float s2 = lut2d_clamp(hf.nucSqSigmaLut, lut.nS, lut.nE, depthMid, L.energyIdx);
+                nrs = 0.5f * hf.spotDist * (vwx + vwy) / (1.41421356f * sqrtf(sigmaSq + s2 + entrySigmaSq));
+            }
             cumulSpOld = cumulSp;
             cumulDoseOld = cumulDose;
             if (cumulSp > L.rangeStop || k == afterLast) { live = false; afterLast = k; }
         }
-        if (!live || int(k) < int(firstIn) - 1) { res = 0.0f; rs = PBS_INF; }
+        if (!live || int(k) < int(firstIn) - 1) { res = 0.0f; rs = PBS_INF; nres = 0.0f; nrs = PBS_INF; }
         out[size_t(k) * outStep] = make_float2(res, rs);
+        if (nout) nout[size_t(k) * hf.nucPlane] = make_float2(nres, nrs);
     }
     firstPassive[size_t(l) * nray + ray] = int(afterLast);
     passive = int(afterLast);
@@ -582,6 +612,27 @@ __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars 
     }
 }
 
+// The halo grid reuses the beam's passive steps; its groups, boxes and tails are its own.
+// Copies the primary scalars and clears the grid's radius histograms and class counters.
+__global__ void k_seed_grid(int numLayers, const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                            LayerScalars *__restrict__ lsOut, BeamScalars *__restrict__ bsOut, int *__restrict__ hist,
+                            int *__restrict__ classCount) {
+    for (int l = threadIdx.x; l < numLayers; l += blockDim.x) lsOut[l] = ls[l];
+    for (int i = threadIdx.x; i < numLayers * (kMaxRadDev + 2); i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x < 16) classCount[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        BeamScalars b = *bs;
+        b.radOverflow = 0;
+        *bsOut = b;
+    }
+}
+
+// (0, +inf) over a halo (IDD, rSigmaEff) field: cells no spot ray writes stay empty.
+__global__ void k_clear_pairs(float2 *__restrict__ v, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        v[i] = make_float2(0.0f, PBS_INF);
+}
+
 // ---------------------------------------------------------------------------------------
 // tile radii, bins and work items
 // ---------------------------------------------------------------------------------------
@@ -632,11 +683,16 @@ __global__ __launch_bounds__(256) void k_tiles(FanGeom g, const float2 *__restri
     }
 }
 
-__global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restrict__ effRad) {
+// overflowFlag (halo grid, complete_superposition.cu:558-620): a tile past the largest radius
+// sets the sticky flag the host reports as an error; the primary grid (nullptr) only clamps
+// it (413-463). Both clamp, so every launch stays within the largest radius.
+__global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restrict__ effRad,
+                       int *__restrict__ overflowFlag) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= numLayers) return;
     int h[kMaxRadDev + 2];
     for (int i = 0; i < kMaxRadDev + 2; ++i) h[i] = hist[l * (kMaxRadDev + 2) + i];
+    if (overflowFlag && h[kMaxRadDev + 1] > 0) *overflowFlag = 1;
     h[kMaxRadDev] += h[kMaxRadDev + 1];  // primary overflow clamps into the largest radius
     h[kMaxRadDev + 1] = 0;
     int layerMax = 0;
@@ -1094,21 +1150,35 @@ hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *ray
 
 hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
                        const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
-                       const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, hipStream_t s) {
+                       const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
+                       hipStream_t s) {
     const int n = g.rx * g.ry;
     hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256, g.numLayers), dim3(256), sizeof(float) * lut.nRrl, s, g, lut, lh,
-                       ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc);
+                       ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
+                            BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s) {
+    hipLaunchKernelGGL(k_seed_grid, dim3(1), dim3(256), 0, s, numLayers, ls, bs, lsOut, bsOut, hist, classCount);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s) {
+    const size_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_clear_pairs, dim3(unsigned(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, v, n);
     return hipGetLastError();
 }
 
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
-                                     int2 *classList, int maxUnits, hipStream_t s) {
+                                     int2 *classList, int maxUnits, int *overflowFlag, hipStream_t s) {
     hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap);
     const int nt = (g.sx / 32) * (g.sy / 8);
     hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
-    hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad);
+    hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
+                       overflowFlag);
     hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
                        tailItemR, classCount, classList, maxUnits);
     return hipGetLastError();

@@ -415,9 +415,55 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
         L.rangeStop = 1.05f * L.peakDepth;
         L.cutoffWepl = 1.05f * L.peakDepth;
         const float a = 0.00270f / (L.peakDepth - 4.50f), c = -4.39f / (L.peakDepth - 3.86f);
+        L.airA = a;
+        L.airC = c;
         L.airQuad = a * 1.0f * p.fanDelta[2] * p.fanDelta[2];
         L.airLin = 2.0f * a * 1.0f * p.fanDelta[2] * p.fanCorner[2] + c * p.fanDelta[2];
     }
+    return true;
+}
+
+bool plan_halo(const BeamPlan &p, const pbs_beam_settings &b, HaloPlan &h, std::string &error) {
+    if (p.legacyPath || p.spotDims[0] <= 0 || p.spotDims[1] <= 0) {
+        error = "nuclear_correction=true requires raw spot lattice inputs; legacy CPB fallback cannot drive halo mode";
+        return false;
+    }
+    const int rawX = p.spotDims[0], rawY = p.spotDims[1];
+    const int nucX = (rawX + 31) / 32 * 32, nucY = (rawY + 7) / 8 * 8;
+    h.nucDims[0] = nucX;
+    h.nucDims[1] = nucY;
+    const int rx = p.rayDims[0], ry = p.rayDims[1];
+    h.rayToNuc.assign(size_t(rx) * ry, -1);
+    for (int j = 0; j < rawY; ++j) {
+        const float gy = p.spotOffset[1] + float(j) * p.spotDelta[1];
+        const int rayY = int(std::lround((gy - p.cpbCorner[1]) / p.cpbRes[1]));
+        if (rayY < 0 || rayY >= ry) {
+            error = "dense halo lattice row mapped outside active ray grid";
+            return false;
+        }
+        for (int i = 0; i < rawX; ++i) {
+            const float gx = p.spotOffset[0] + float(i) * p.spotDelta[0];
+            const int rayX = int(std::lround((gx - p.cpbCorner[0]) / p.cpbRes[0]));
+            if (rayX < 0 || rayX >= rx) {
+                error = "dense halo lattice column mapped outside active ray grid";
+                return false;
+            }
+            h.rayToNuc[size_t(rayY) * rx + rayX] = j * nucX + i;
+        }
+    }
+    const int L = p.numLayers;
+    h.weights.assign(size_t(L) * nucX * nucY, 0.0f);
+    for (int l = 0; l < L; ++l)
+        for (int j = 0; j < rawY; ++j)
+            for (int i = 0; i < rawX; ++i)
+                h.weights[(size_t(l) * nucY + j) * nucX + i] = p.denseWeights[(size_t(l) * rawY + j) * rawX + i];
+    const bool beamDelta = b.spotDelta[0] > b.raySpacing[0] * 1.5f;
+    const float spacing = beamDelta ? b.spotDelta[0] : p.spotDelta[0];
+    if (b.raySpacing[0] > 0.0f && spacing / b.raySpacing[0] < 1.5f) {
+        error = "nuclear superposition gate: spotDistInRays < 1.5 (the halo radius would exceed 32)";
+        return false;
+    }
+    h.spotDist = std::fabs(p.fanDelta[0]) > 1e-6f ? std::fabs(spacing / p.fanDelta[0]) : 1.0f;
     return true;
 }
 

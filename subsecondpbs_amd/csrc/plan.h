@@ -39,6 +39,7 @@ struct LayerParams {
     float airQuad, airLin; // sigma^2 air-divergence recurrence coefficients
     float convA, convB;    // sigmaSqAirCoefs at the conversion peak (conv stage)
     float sigmaX, sigmaY;  // spot sigma at iso (mm)
+    float airA, airC;      // sigmaSqAirCoefs(peakDepth) (halo entry sigma)
     float pad0, pad1;
 };
 
@@ -74,6 +75,18 @@ struct CpbOverride {
     float corner[2], res[2];
     int dims[2];
 };
+
+// Halo (nuclear) lattice plan of a beam (wrapper:804-929; spot spacing wrapper:4062-4100,
+// 4149-4172, 4338-4350): the halo grid is the spot lattice rounded up to 32x8; the ray at a
+// spot's gantry position carries that spot's halo; spotDist = spot spacing / fan ray spacing.
+struct HaloPlan {
+    int nucDims[2] = {0, 0};
+    float spotDist = 0.f;
+    std::vector<int> rayToNuc;   // [ry][rx], -1 = no spot centred on the ray
+    std::vector<float> weights;  // [L][nucY][nucX]
+};
+// Returns false with `error` set when the beam cannot drive the halo branch.
+bool plan_halo(const BeamPlan &plan, const pbs_beam_settings &beam, HaloPlan &halo, std::string &error);
 
 // Returns false (and sets plan.error) on a contract failure.
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3],

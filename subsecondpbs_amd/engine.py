@@ -68,14 +68,40 @@ class Engine:
     def sync(self):
         _abi.check(_abi.lib().pbs_engine_sync(self.handle), "pbs_engine_sync")
 
-    STAGES = ("trace", "conv", "fill", "tiles", "superpose", "merge", "transfer")
+    STAGES = ("trace", "conv", "fill", "tiles", "superpose", "merge", "transfer", "halo")
+
+    NUCLEAR_MODELS = {"off": 0, "gauss_fit": 1, "soukup": 2, "fluka": 3}
+
+    def set_nuclear(self, model: int | str | bool):
+        """Halo branch for the following beams: 0/False off, 1/True GAUSS_FIT, 2 SOUKUP, 3 FLUKA."""
+        if isinstance(model, str):
+            model = self.NUCLEAR_MODELS[model.lower()]
+        _abi.check(_abi.lib().pbs_engine_set_nuclear(self.handle, int(model)), "pbs_engine_set_nuclear")
+
+    def debug_halo(self) -> dict:
+        """The last beam's halo grid: dims, spotDist, (IDD, rSigmaEff) fields, radius histograms."""
+        dims = (C.c_int * 2)()
+        sd = C.c_float(0.0)
+        rc = _abi.lib().pbs_engine_debug_halo(self.handle, dims, C.byref(sd), None, None, None)
+        _abi.check(rc, "pbs_engine_debug_halo")
+        ints = self.debug_fetch(())["ints"]
+        L, K = ints["numLayers"], ints["steps"]
+        nx, ny = dims[0], dims[1]
+        f = np.zeros((L, K, ny, nx, 2), np.float32)
+        h = np.zeros((L, 34), np.int32)
+        ov = C.c_int(0)
+        rc = _abi.lib().pbs_engine_debug_halo(self.handle, dims, C.byref(sd), C.c_void_p(f.ctypes.data),
+                                              C.c_void_p(h.ctypes.data), C.byref(ov))
+        _abi.check(rc, "pbs_engine_debug_halo")
+        return {"nucDims": (nx, ny), "spotDist": float(sd.value), "iddSigma": f, "radHist": h,
+                "overflow": int(ov.value)}
 
     def set_profiling(self, on: bool = True):
         _abi.check(_abi.lib().pbs_engine_set_profiling(self.handle, int(bool(on))), "pbs_engine_set_profiling")
 
     def stage_times(self) -> tuple[dict, int]:
         """Device ms per stage summed over the beams queued since the last call, and their count."""
-        ms = (C.c_float * 7)()
+        ms = (C.c_float * len(self.STAGES))()
         n = C.c_int(0)
         _abi.check(_abi.lib().pbs_engine_stage_times(self.handle, ms, C.byref(n)), "pbs_engine_stage_times")
         return {k: float(v) for k, v in zip(self.STAGES, ms)}, int(n.value)

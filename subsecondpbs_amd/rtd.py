@@ -33,6 +33,44 @@ def _complete_energy(energy: dict, tables_dir: str | None) -> dict:
     return out
 
 
+def _complete_nuclear(energy: dict, tables_dir: str | None) -> dict:
+    """Nuclear halo tables on the energy dict's primary axis when the dict lacks them
+    (maybe_fill_nuclear_tables, pybind:796-837; alignNuclearTablesToPrimaryAxis,
+    include/utils/nuclear_table_utils.h:126-189): copied when the axes agree, otherwise each
+    primary energy row is resampled at the same depths from the bracketing reference rows."""
+    if "nuclear_weight_matrix" in energy and "nuclear_sq_sigma_matrix" in energy:
+        return energy
+    ref = load_energy_tables(tables_dir or TABLES_DIR, nuclear=True)
+    out = dict(energy)
+    e_out = np.asarray(energy["energies_per_u"], np.float32)
+    same = (int(energy["n_energies"]) == ref["n_energies"] and int(energy["n_energy_samples"]) == ref["n_energy_samples"]
+            and np.allclose(e_out, ref["energies_per_u"]) and np.allclose(energy["peak_depths"], ref["peak_depths"])
+            and np.allclose(energy["scale_facts"], ref["scale_facts"]))
+    if same:
+        out["nuclear_weight_matrix"] = ref["nuclear_weight_matrix"]
+        out["nuclear_sq_sigma_matrix"] = ref["nuclear_sq_sigma_matrix"]
+        return out
+    nE, nS = int(energy["n_energies"]), int(energy["n_energy_samples"])
+    e_ref = np.asarray(ref["energies_per_u"], np.float64)
+    sc_ref = np.asarray(ref["scale_facts"], np.float64)
+    sc_out = np.asarray(energy["scale_facts"], np.float64)
+    w = np.zeros((nE, nS), np.float32)
+    s2 = np.zeros((nE, nS), np.float32)
+    xs = np.arange(ref["n_energy_samples"])
+    for e in range(nE):
+        idx = float(np.interp(e_out[e], e_ref, np.arange(e_ref.size)))
+        e0 = int(math.floor(idx)); fr = idx - e0
+        e1 = min(e0 + (1 if fr > 0 else 0), e_ref.size - 1)
+        depth = np.arange(nS) / sc_out[e] if sc_out[e] > 0 else np.zeros(nS)
+        for m_ref, m_out in ((ref["nuclear_weight_matrix"], w), (ref["nuclear_sq_sigma_matrix"], s2)):
+            a = np.interp(depth * sc_ref[e0], xs, m_ref[e0])
+            b = np.interp(depth * sc_ref[e1], xs, m_ref[e1])
+            m_out[e] = a + (b - a) * fr
+    out["nuclear_weight_matrix"] = w
+    out["nuclear_sq_sigma_matrix"] = s2
+    return out
+
+
 def _check_dose(dose: np.ndarray, beam: dict, where: str):
     # require_nonempty_final_dose (pybind:1069-1081)
     if not np.isfinite(dose).all():
@@ -43,7 +81,8 @@ def _check_dose(dose: np.ndarray, beam: dict, where: str):
 
 def subsecond_wrapper(ct, ct_dims, ct_resolution, ct_corner, dose, dose_dims, dose_resolution, dose_corner,
                       beams, energy, gpu_id: int = 0, nuclear_correction: bool = False, verbose: int = 0):
-    """Accumulate the dose of ``beams`` (list of beam dicts) into ``dose`` (float32, x fastest)."""
+    """Accumulate the dose of ``beams`` (list of beam dicts) into ``dose`` (float32, x fastest).
+    ``nuclear_correction``: False/0 off, True/1 halo branch with GAUSS_FIT constants, 2 SOUKUP, 3 FLUKA."""
     keep = _abi.Keep()
     es = _abi.energy_struct(energy, keep)
     arr = (_abi.BeamSettings * len(beams))()
@@ -61,7 +100,7 @@ def subsecond_wrapper(ct, ct_dims, ct_resolution, ct_corner, dose, dose_dims, do
     dr, drp = _abi.f3(dose_resolution)
     dc, dcp = _abi.f3(dose_corner)
     rc = _abi.lib().pbs_subsecond_wrapper(ct.ctypes.data_as(_abi._f), cdp, crp, ccp, dose.ctypes.data_as(_abi._f),
-                                          ddp, drp, dcp, arr, len(beams), es, int(gpu_id), int(bool(nuclear_correction)),
+                                          ddp, drp, dcp, arr, len(beams), es, int(gpu_id), int(nuclear_correction),
                                           int(verbose))
     _abi.check(rc, "subsecondWrapper")
     return dose
@@ -71,6 +110,8 @@ def raytracedicom_wrapper(ct, ct_dims, ct_resolution, ct_corner, dose_dims, dose
                           beam: dict, energy: dict, gpu_id: int = 0, nuclear_correction: bool = False,
                           verbose: int = 0, tables_dir: str | None = None) -> np.ndarray:
     energy = _complete_energy(energy, tables_dir)
+    if nuclear_correction:
+        energy = _complete_nuclear(energy, tables_dir)
     for key in ("energies", "spot_sigmas", "steps", "subspot_data", "beam_direction", "beam_xdir", "beam_ydir",
                 "sad", "source_position"):
         if key not in beam:

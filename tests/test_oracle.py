@@ -624,3 +624,22 @@ def test_halo_beam_conserves_integral_dose():
     assert d1.sum() == pytest.approx(d0.sum(), rel=1e-2)
     assert d1.max() < d0.max()
     assert sum(p1.nucRadHist[0]) > 0 and p1.nucRadHist[0][33] == 0
+
+
+@pytest.mark.parametrize("model,factor", [(0, 1.0), (1, 0.97), (2, 1.0), (3, 1.0)])
+def test_conv_entry_sigma_halo_factor(model, factor):
+    """Only the GAUSS_FIT halo model narrows the primary entry sigma by 0.97 (wrapper:3788-3795)."""
+    case = _water_case(energies=(110.0,))
+    case["energy"] = EN
+    p, dens, wepl, fi, fo, keep, en, bm = _trace(case)
+    K = p.steps
+    wmin = np.zeros(K)
+    L = O.lib()
+    L.orc_beam_cutoffs(C.byref(p), C.byref(bm), C.byref(en), O._dp(wepl), O._ip(fi), O._ip(fo), O._dp(wmin))
+    rw = np.zeros((1, p.rayDims[1], p.rayDims[0]))
+    bm.nuclear = 0
+    L.orc_conv_lattice(C.byref(p), C.byref(bm), C.byref(en), O._dp(rw))
+    s0 = p.entrySigma[0][0]
+    bm.nuclear = model
+    L.orc_conv_lattice(C.byref(p), C.byref(bm), C.byref(en), O._dp(rw))
+    assert p.entrySigma[0][0] == pytest.approx(factor * s0, rel=1e-6)
