@@ -101,8 +101,10 @@ typedef struct pbs_engine pbs_engine;
 
 pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *energy);
 void pbs_engine_destroy(pbs_engine *e);
-/* Upload (ctIsDevice = 0) or adopt (ctIsDevice = 1) a CT; values are converted to HU+1000
- * with the reference's auto-detection (wrapper:2290-2441). */
+/* Upload a host CT (ctIsDevice = 0) or copy a device CT (ctIsDevice = 1); values are
+ * converted to HU+1000 with the reference's auto-detection (wrapper:2290-2441).
+ * ctIsDevice = 2 borrows a device buffer that already holds HU+1000 (no copy; it must outlive
+ * the beams queued on it), which lets several engines share one resident CT. */
 int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int ctDims[3], const float ctRes[3],
                       const float ctCorner[3], int ctIsDevice);
 /* Dose grid geometry used by subsequent beam calls. */
@@ -115,10 +117,7 @@ int pbs_engine_sync(pbs_engine *e);
 /* Raw stream handle (hipStream_t) the engine queues work on. */
 void *pbs_engine_stream(pbs_engine *e);
 
-/* Per-stage device time, measured with HIP events on the engine stream while profiling is
- * on: msOut[7] = trace (+ reductions, cut-offs), ray weights, IDD/sigma fill, tile radii /
- * bins / items, superposition, block merge, BEV->dose transfer, summed over the beams queued
- * since the previous call (their count in *beams). */
+/* Per-stage device timing with HIP events on the engine stream (off by default). */
 int pbs_engine_set_profiling(pbs_engine *e, int enable);
 /* Device time per stage summed over the beams queued since the last call (profiling on):
  * msOut[8] = trace, conv, fill, tiles, superpose, merge, transfer, halo. */
@@ -158,6 +157,34 @@ typedef struct pbs_debug_ints {
 int pbs_engine_debug_fetch(pbs_engine *e, pbs_debug_ints *ints, float *density, float *wepl,
                            int *firstInside, int *firstOutside, float *weplMin, float *rayWeights,
                            float *iddSigma, int *firstPassive, int *tileRad, int *radHist, float *bev);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched and served forms (one GPU). A batch owns `lanes` engines (own stream and
+ * workspaces) that share one resident CT and LUT set; beams are dealt to lanes in submission
+ * order and their kernels overlap on the device. No counterpart in the reference, which
+ * computes one beam at a time on the default stream.
+ * ------------------------------------------------------------------------------------- */
+typedef struct pbs_batch pbs_batch;
+pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lanes);
+void pbs_batch_destroy(pbs_batch *b);
+int pbs_batch_set_ct(pbs_batch *b, const float *ct, const int ctDims[3], const float ctRes[3],
+                     const float ctCorner[3], int ctIsDevice);
+int pbs_batch_set_dose_grid(pbs_batch *b, const int doseDims[3], const float doseRes[3],
+                            const float doseCorner[3]);
+int pbs_batch_set_nuclear(pbs_batch *b, int model);
+/* Sum of the doses of `numBeams` beams, accumulated into doseDevice: lane i takes beams
+ * i, i + lanes, ...; each lane accumulates into its own buffer and the lanes are added in
+ * lane order, so the result depends only on the beam list and the lane count. Returns when
+ * the work is queued; pbs_batch_sync waits. */
+int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDevice);
+/* Serving: queue one beam whose dose is accumulated into its own doseDevice buffer, on the
+ * next lane (round robin). *ticket identifies the request for pbs_batch_query / wait. */
+int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDevice, long long *ticket);
+/* 1 when the request's work has finished on the device, 0 while it runs. */
+int pbs_batch_query(pbs_batch *b, long long ticket);
+int pbs_batch_wait(pbs_batch *b, long long ticket);
+int pbs_batch_sync(pbs_batch *b);
+int pbs_batch_lanes(pbs_batch *b);
 
 /* ---------------------------------------------------------------------------------------
  * cudaFinalDose with the reference's name and argument order

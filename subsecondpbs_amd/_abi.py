@@ -83,6 +83,8 @@ EXPORTED = [
     "pbs_engine_beam_dose", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_nuclear", "pbs_engine_debug_halo",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
+    "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
+    "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
 ]
 
 _lib = None
@@ -123,6 +125,27 @@ def lib():
         L.pbs_engine_set_nuclear.argtypes = [C.c_void_p, C.c_int]
         L.pbs_engine_debug_halo.restype = C.c_int
         L.pbs_engine_debug_halo.argtypes = [C.c_void_p, _i, _f, C.c_void_p, C.c_void_p, _i]
+        L.pbs_batch_create.restype = C.c_void_p
+        L.pbs_batch_create.argtypes = [C.c_int, C.POINTER(EnergyStruct), C.c_int]
+        L.pbs_batch_destroy.argtypes = [C.c_void_p]
+        L.pbs_batch_set_ct.restype = C.c_int
+        L.pbs_batch_set_ct.argtypes = [C.c_void_p, C.c_void_p, _i, _f, _f, C.c_int]
+        L.pbs_batch_set_dose_grid.restype = C.c_int
+        L.pbs_batch_set_dose_grid.argtypes = [C.c_void_p, _i, _f, _f]
+        L.pbs_batch_set_nuclear.restype = C.c_int
+        L.pbs_batch_set_nuclear.argtypes = [C.c_void_p, C.c_int]
+        L.pbs_batch_dose.restype = C.c_int
+        L.pbs_batch_dose.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_size_t, C.c_void_p]
+        L.pbs_batch_submit.restype = C.c_int
+        L.pbs_batch_submit.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p, C.POINTER(C.c_longlong)]
+        L.pbs_batch_query.restype = C.c_int
+        L.pbs_batch_query.argtypes = [C.c_void_p, C.c_longlong]
+        L.pbs_batch_wait.restype = C.c_int
+        L.pbs_batch_wait.argtypes = [C.c_void_p, C.c_longlong]
+        L.pbs_batch_sync.restype = C.c_int
+        L.pbs_batch_sync.argtypes = [C.c_void_p]
+        L.pbs_batch_lanes.restype = C.c_int
+        L.pbs_batch_lanes.argtypes = [C.c_void_p]
         L.pbs_final_dose_bind.restype = C.c_int
         L.pbs_final_dose_bind.argtypes = [C.c_void_p, _f, _f, C.c_int, C.c_float]
         L.cudaFinalDose.restype = C.c_int

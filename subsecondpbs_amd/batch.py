@@ -1,0 +1,93 @@
+"""Batched and served forms of the final-dose engine on one GPU.
+
+``Batch`` owns several engine lanes (own HIP stream and workspaces) sharing one resident CT
+and LUT set; the beams of a batch are dealt to the lanes and overlap on the device, and the
+lanes' doses are added in a fixed order (the result depends only on the beam list and the
+lane count). ``submit`` is the served form: each request gets its own dose buffer, so a
+request's result is the same alone and among others.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class Batch:
+    def __init__(self, energy: dict, lanes: int = 4, device: int = 0):
+        self._keep = _abi.Keep()
+        es = _abi.energy_struct(energy, self._keep)
+        self._es = es
+        h = _abi.lib().pbs_batch_create(device, C.byref(es), int(lanes))
+        if not h:
+            raise RuntimeError("pbs_batch_create: " + _abi.last_error())
+        self.handle = h
+        self.lanes = int(lanes)
+        self._beam_keep = {}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _abi.lib().pbs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ct(self, ct, dims, resolution, corner):
+        import torch
+        if isinstance(ct, torch.Tensor):
+            if not (ct.is_cuda and ct.dtype == torch.float32 and ct.is_contiguous()):
+                raise ValueError("device CT must be a contiguous float32 tensor")
+            ptr, mode = C.c_void_p(ct.data_ptr()), 1
+        else:
+            a = np.ascontiguousarray(ct, dtype=np.float32)
+            self._ct_host = a
+            ptr, mode = C.c_void_p(a.ctypes.data), 0
+        d, dp = _abi.i3(dims)
+        r, rp = _abi.f3(resolution)
+        c, cp = _abi.f3(corner)
+        _abi.check(_abi.lib().pbs_batch_set_ct(self.handle, ptr, dp, rp, cp, mode), "pbs_batch_set_ct")
+
+    def set_dose_grid(self, dims, resolution, corner):
+        d, dp = _abi.i3(dims)
+        r, rp = _abi.f3(resolution)
+        c, cp = _abi.f3(corner)
+        _abi.check(_abi.lib().pbs_batch_set_dose_grid(self.handle, dp, rp, cp), "pbs_batch_set_dose_grid")
+
+    def set_nuclear(self, model: int):
+        _abi.check(_abi.lib().pbs_batch_set_nuclear(self.handle, int(model)), "pbs_batch_set_nuclear")
+
+    def dose(self, beams: list[dict], dose):
+        """Queue the sum of the beams' doses into ``dose`` (device tensor [nz][ny][nx])."""
+        keep = _abi.Keep()
+        arr = (_abi.BeamSettings * len(beams))()
+        for i, b in enumerate(beams):
+            arr[i] = _abi.beam_settings(b, keep)
+        _abi.check(_abi.lib().pbs_batch_dose(self.handle, arr, len(beams), C.c_void_p(dose.data_ptr())),
+                   "pbs_batch_dose")
+
+    def submit(self, beam: dict, dose) -> int:
+        """Queue one beam into its own ``dose`` buffer; returns a ticket."""
+        keep = _abi.Keep()
+        bs = _abi.beam_settings(beam, keep)
+        t = C.c_longlong(0)
+        _abi.check(_abi.lib().pbs_batch_submit(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()), C.byref(t)),
+                   "pbs_batch_submit")
+        return int(t.value)
+
+    def query(self, ticket: int) -> bool:
+        rc = _abi.lib().pbs_batch_query(self.handle, C.c_longlong(ticket))
+        if rc < 0:
+            _abi.check(rc, "pbs_batch_query")
+        return rc == 1
+
+    def wait(self, ticket: int):
+        _abi.check(_abi.lib().pbs_batch_wait(self.handle, C.c_longlong(ticket)), "pbs_batch_wait")
+
+    def sync(self):
+        _abi.check(_abi.lib().pbs_batch_sync(self.handle), "pbs_batch_sync")

@@ -407,6 +407,11 @@ int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const f
     if (!(res[0] > 0 && res[1] > 0 && res[2] > 0)) return set_error(PBS_E_INVALID, "ctResolution must be positive");
     HIP_TRY(hipSetDevice(e->device));
     const size_t n = size_t(dims[0]) * dims[1] * dims[2];
+    if (ctIsDevice == 2) {  // borrowed device HU+1000 volume
+        e->ctPtr = ct;
+        for (int i = 0; i < 3; ++i) { e->ctDims[i] = dims[i]; e->ctRes[i] = res[i]; e->ctCorner[i] = corner[i]; }
+        return PBS_OK;
+    }
     std::vector<float> host;
     const float *src = ct;
     if (ctIsDevice) {
@@ -928,4 +933,203 @@ int cudaFinalDose(const float *subspotData, const vec3i *roiIndices, int numROII
     return pbs_engine_sync(e);
 }
 
+// ---------------------------------------------------------------------------------------
+// batched / served forms
+// ---------------------------------------------------------------------------------------
 }  // extern "C"
+
+struct pbs_batch {
+    int device = 0;
+    std::vector<pbs_engine *> lanes;
+    pbs_engine *owner = nullptr;  // holds the CT every lane borrows
+    DevBuf<float> laneDose;  // [lanes][dose voxels] per-lane accumulators of pbs_batch_dose
+    size_t doseN = 0;
+    size_t next = 0;               // round-robin cursor of the served form
+    long long tickets = 0;
+    std::vector<std::pair<long long, hipEvent_t>> pending;
+    std::vector<hipEvent_t> freeEvents;
+    hipEvent_t laneDone[16] = {};
+};
+
+namespace {
+
+__global__ void k_add_lanes(float *__restrict__ dose, const float *__restrict__ lanes, int numLanes, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        float s = 0.0f;
+        for (int l = 0; l < numLanes; ++l) s += lanes[size_t(l) * n + i];
+        dose[i] += s;
+    }
+}
+
+__global__ void k_zero(float *__restrict__ v, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        v[i] = 0.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lanes) {
+    if (lanes < 1 || lanes > 16) {
+        set_error(PBS_E_INVALID, "lanes must be 1..16");
+        return nullptr;
+    }
+    std::unique_ptr<pbs_batch> b(new pbs_batch());
+    b->device = gpuId;
+    b->owner = pbs_engine_create(gpuId, energy);
+    if (!b->owner) return nullptr;
+    for (int i = 0; i < lanes; ++i) {
+        pbs_engine *e = pbs_engine_create(gpuId, energy);
+        if (!e) {
+            for (pbs_engine *x : b->lanes) pbs_engine_destroy(x);
+            pbs_engine_destroy(b->owner);
+            return nullptr;
+        }
+        b->lanes.push_back(e);
+        if (hipEventCreateWithFlags(&b->laneDone[i], hipEventDisableTiming) != hipSuccess) {
+            set_error(PBS_E_DEVICE, "hipEventCreate failed");
+            return nullptr;
+        }
+    }
+    return b.release();
+}
+
+void pbs_batch_destroy(pbs_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    for (pbs_engine *e : b->lanes) pbs_engine_destroy(e);
+    pbs_engine_destroy(b->owner);
+    for (auto &p : b->pending) (void)hipEventDestroy(p.second);
+    for (hipEvent_t ev : b->freeEvents) (void)hipEventDestroy(ev);
+    for (int i = 0; i < 16; ++i)
+        if (b->laneDone[i]) (void)hipEventDestroy(b->laneDone[i]);
+    delete b;
+}
+
+int pbs_batch_lanes(pbs_batch *b) { return b ? int(b->lanes.size()) : 0; }
+
+int pbs_batch_set_ct(pbs_batch *b, const float *ct, const int dims[3], const float res[3], const float corner[3],
+                     int ctIsDevice) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    int rc = pbs_engine_set_ct(b->owner, ct, dims, res, corner, ctIsDevice);
+    if (rc != PBS_OK) return rc;
+    for (pbs_engine *e : b->lanes) {
+        rc = pbs_engine_set_ct(e, b->owner->ctPtr, dims, res, corner, 2);
+        if (rc != PBS_OK) return rc;
+    }
+    return PBS_OK;
+}
+
+int pbs_batch_set_dose_grid(pbs_batch *b, const int dims[3], const float res[3], const float corner[3]) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    for (pbs_engine *e : b->lanes) {
+        const int rc = pbs_engine_set_dose_grid(e, dims, res, corner);
+        if (rc != PBS_OK) return rc;
+    }
+    b->doseN = size_t(dims[0]) * dims[1] * dims[2];
+    return PBS_OK;
+}
+
+int pbs_batch_set_nuclear(pbs_batch *b, int model) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    for (pbs_engine *e : b->lanes) {
+        const int rc = pbs_engine_set_nuclear(e, model);
+        if (rc != PBS_OK) return rc;
+    }
+    return PBS_OK;
+}
+
+int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDev) {
+    if (!b || !beams || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (b->doseN == 0) return set_error(PBS_E_INVALID, "no dose grid bound to the batch");
+    HIP_TRY(hipSetDevice(b->device));
+    const int L = int(b->lanes.size());
+    const size_t n = b->doseN;
+    HIP_TRY(b->laneDose.ensure(n * L));
+    float *acc = b->laneDose.p;
+    hipStream_t s0 = b->lanes[0]->stream;
+    // lanes start from zero once every lane's earlier work on the buffer has finished
+    for (int l = 0; l < L; ++l) {
+        HIP_TRY(hipEventRecord(b->laneDone[l], b->lanes[l]->stream));
+        HIP_TRY(hipStreamWaitEvent(s0, b->laneDone[l], 0));
+    }
+    hipLaunchKernelGGL(k_zero, dim3(2048), dim3(256), 0, s0, acc, n * L);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(b->laneDone[0], s0));
+    for (int l = 1; l < L; ++l) HIP_TRY(hipStreamWaitEvent(b->lanes[l]->stream, b->laneDone[0], 0));
+    for (size_t i = 0; i < numBeams; ++i) {
+        const int l = int(i % size_t(L));
+        const int rc = pbs_engine_beam_dose(b->lanes[l], &beams[i], acc + size_t(l) * n);
+        if (rc != PBS_OK) return rc;
+    }
+    // join the lanes on lane 0 and add them in lane order
+    for (int l = 1; l < L; ++l) {
+        HIP_TRY(hipEventRecord(b->laneDone[l], b->lanes[l]->stream));
+        HIP_TRY(hipStreamWaitEvent(s0, b->laneDone[l], 0));
+    }
+    hipLaunchKernelGGL(k_add_lanes, dim3(2048), dim3(256), 0, s0, doseDev, acc, L, n);
+    HIP_TRY(hipGetLastError());
+    return PBS_OK;
+}
+
+int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev, long long *ticket) {
+    if (!b || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    const int l = int(b->next++ % b->lanes.size());
+    pbs_engine *e = b->lanes[l];
+    const int rc = pbs_engine_beam_dose(e, beam, doseDev);
+    if (rc != PBS_OK) return rc;
+    hipEvent_t ev;
+    if (!b->freeEvents.empty()) {
+        ev = b->freeEvents.back();
+        b->freeEvents.pop_back();
+    } else {
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(ev, e->stream));
+    const long long t = ++b->tickets;
+    b->pending.emplace_back(t, ev);
+    if (ticket) *ticket = t;
+    return PBS_OK;
+}
+
+int pbs_batch_query(pbs_batch *b, long long ticket) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    for (size_t i = 0; i < b->pending.size(); ++i) {
+        if (b->pending[i].first != ticket) continue;
+        const hipError_t q = hipEventQuery(b->pending[i].second);
+        if (q == hipErrorNotReady) return 0;
+        HIP_TRY(q);
+        b->freeEvents.push_back(b->pending[i].second);
+        b->pending.erase(b->pending.begin() + long(i));
+        return 1;
+    }
+    return 1;  // unknown or already collected: done
+}
+
+int pbs_batch_wait(pbs_batch *b, long long ticket) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    for (size_t i = 0; i < b->pending.size(); ++i) {
+        if (b->pending[i].first != ticket) continue;
+        HIP_TRY(hipEventSynchronize(b->pending[i].second));
+        b->freeEvents.push_back(b->pending[i].second);
+        b->pending.erase(b->pending.begin() + long(i));
+        break;
+    }
+    return PBS_OK;
+}
+
+int pbs_batch_sync(pbs_batch *b) {
+    if (!b) return set_error(PBS_E_INVALID, "null batch");
+    int rc = PBS_OK;
+    for (pbs_engine *e : b->lanes) {
+        const int r = pbs_engine_sync(e);
+        if (r != PBS_OK && rc == PBS_OK) rc = r;
+    }
+    for (auto &p : b->pending) b->freeEvents.push_back(p.second);
+    b->pending.clear();
+    return rc;
+}
+
+}  // extern "C\"

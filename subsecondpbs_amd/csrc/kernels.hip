@@ -792,22 +792,29 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
 // ---------------------------------------------------------------------------------------
 // superposition
 // ---------------------------------------------------------------------------------------
-// Superposition of one item block by one wave.
+// Superposition of a 2-row slice of a tile by one wave.
 //
-// Lane c*64 + lane owns output column X = c*64 + lane - RM of the block (tile coordinates;
-// the block spans X in [-RM, 32 + RM)) and keeps every output row of that column in
-// registers. Sources (the tile's 32x8 pixels of one layer) are staged in LDS as
-// [d, e_0 .. e_R] per pixel. A sweep visits each source row in 2R+1 phases; in phase phi a
-// lane takes the one source x_s = phi (mod 2R+1) of the row with |X - x_s| <= R, so the
-// sources of one phase have disjoint footprints and all of them are applied by one pass of
-// 2R+1 FMAs per lane (d e[|X - x_s|] e[|dy|] into rows y_s + dy). Every output is a sum in a
-// fixed order: the result is bitwise repeatable.
+// Lane c*64 + lane owns output column X = c*64 + lane - RM (tile coordinates; the block spans
+// X in [-RM, 32 + RM)) and keeps the 2 + 2RM output rows its slice reaches in registers.
+// The slice's 64 source pixels are staged in the wave's LDS table as [d, e_0 .. e_R]. A sweep
+// visits each source row in 2R+1 phases; in phase phi a lane takes the one source
+// x_s = phi (mod 2R+1) of the row with |X - x_s| <= R, so the sources of one phase have
+// disjoint footprints and all of them are applied by one pass of 2R+1 FMAs per lane
+// (d e[|X - x_s|] e[|dy|] into row y_s + dy). Every output is a sum in a fixed order: the
+// result is bitwise repeatable.
 constexpr int pad4(int n) { return (n + 3) / 4 * 4; }
 
-template <int R, int RM, int Y0>
-__device__ __forceinline__ void sweep_rows(float (&acc)[(32 + 2 * RM + 63) / 64][8 + 2 * RM],
-                                           const float *__restrict__ tab, unsigned rowLive, int lane) {
-    constexpr int NC = (32 + 2 * RM + 63) / 64;
+template <int RM>
+struct SliceAcc {
+    static constexpr int NC = (32 + 2 * RM + 63) / 64;
+    static constexpr int NR = 2 + 2 * RM;
+    float v[NC][NR];
+};
+
+template <int R, int RM>
+__device__ __forceinline__ void sweep_slice(SliceAcc<RM> &acc, const float *__restrict__ tab, unsigned rowLive,
+                                            int lane) {
+    constexpr int NC = SliceAcc<RM>::NC;
     constexpr int EWp = pad4(RM + 2);
     constexpr int S = 2 * R + 1;
     constexpr int NE4 = (R + 2 + 3) / 4;  // float4 reads covering [d, e_0 .. e_R]
@@ -829,7 +836,7 @@ __device__ __forceinline__ void sweep_rows(float (&acc)[(32 + 2 * RM + 63) / 64]
             adx[c] = rem[c] > R ? rem[c] - R : R - rem[c];
         }
 #pragma unroll
-        for (int yy = 0; yy < 4; ++yy) {
+        for (int yy = 0; yy < 2; ++yy) {
             if (!(rowLive & (1u << yy))) continue;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -842,34 +849,32 @@ __device__ __forceinline__ void sweep_rows(float (&acc)[(32 + 2 * RM + 63) / 64]
                 }
                 const float a = valid[c] ? E[0] * src[1 + adx[c]] : 0.0f;
 #pragma unroll
-                for (int i = -R; i <= R; ++i) acc[c][Y0 + yy + RM + i] += a * E[1 + (i < 0 ? -i : i)];
+                for (int i = -R; i <= R; ++i) acc.v[c][yy + RM + i] += a * E[1 + (i < 0 ? -i : i)];
             }
-            // one source row's weights live at a time (register budget, occupancy)
-            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) rem[c] = rem[c] == 0 ? S - 1 : rem[c] - 1;
     }
 }
 
-template <int R, int RM, int Y0>
-__device__ __forceinline__ void sweep_dispatch(int r, float (&acc)[(32 + 2 * RM + 63) / 64][8 + 2 * RM],
-                                               const float *tab, unsigned rowLive, int lane) {
+template <int R, int RM>
+__device__ __forceinline__ void sweep_dispatch(int r, SliceAcc<RM> &acc, const float *tab, unsigned rowLive,
+                                               int lane) {
     if constexpr (R <= RM) {
         if (r == R)
-            sweep_rows<R, RM, Y0>(acc, tab, rowLive, lane);
+            sweep_slice<R, RM>(acc, tab, rowLive, lane);
         else
-            sweep_dispatch<R + 1, RM, Y0>(r, acc, tab, rowLive, lane);
+            sweep_dispatch<R + 1, RM>(r, acc, tab, rowLive, lane);
     }
 }
 
 // One work-group of 4 waves per work unit (step, tile, block) of radius class RM; the block
-// is the main BEV block or one layer group's tail block. The unit's live layers are dealt
-// to the waves round-robin in layer order; each wave sums its layers in registers
-// (complete_superposition.cu:280-330: per-pixel erf differences e0 = erf(r/2),
-// e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product d e_x e_y), then the four partial
-// blocks are added in wave order in LDS and written to the unit's slot as a
-// (8 + 2RM) x (32 + 2RM) block. Fixed orders throughout: bitwise repeatable.
+// is the main BEV block or one layer group's tail block. Wave w takes source rows 2w, 2w+1
+// of every live layer of the block (complete_superposition.cu:280-330: per-pixel erf
+// differences e0 = erf(r/2), e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product
+// d e_x e_y), so all waves work on every layer; the four partial blocks are added in wave
+// order in LDS and written to the unit's slot as a (8 + 2RM) x (32 + 2RM) block. Fixed
+// orders throughout: bitwise repeatable.
 template <int RM>
 __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__restrict__ iddSigma,
                                                    const int *__restrict__ tileRad, const int *__restrict__ effRad,
@@ -879,11 +884,11 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
                                                    const int *__restrict__ classCount,
                                                    const int2 *__restrict__ classList, int cls, int maxUnits,
                                                    float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
-    constexpr int NC = (32 + 2 * RM + 63) / 64;
+    constexpr int NC = SliceAcc<RM>::NC, NR = SliceAcc<RM>::NR;
     constexpr int EWp = pad4(RM + 2);
     constexpr int BW = 32 + 2 * RM, BH = 8 + 2 * RM;
     constexpr int LJ = kMaxLayersDev / 64;
-    __shared__ __align__(16) float s_tab[4][128 * EWp];  // per wave: 4 source rows x 32 pixels
+    __shared__ __align__(16) float s_tab[4][64 * EWp];  // per wave: its 2 source rows x 32 pixels
     __shared__ float s_B[BH * BW];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     float *tab = s_tab[w];
@@ -914,70 +919,55 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
             lmask[j] = __ballot(in);
             radv[j] = rad;
         }
-        float acc[NC][BH];
+        SliceAcc<RM> acc;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int q = 0; q < BH; ++q) acc[c][q] = 0.0f;
-        int rank = 0;
+            for (int q = 0; q < NR; ++q) acc.v[c][q] = 0.0f;
+        // this lane's source pixel: column lane & 31 of row 2w + (lane >> 5)
+        const int gx = tx * 32 + (lane & 31), gy = ty * 8 + 2 * w + (lane >> 5);
+        const bool pixIn = gx < g.rx && gy < g.ry;
+        const size_t pixOff = size_t(gy) * g.sx + gx;
+        float *e = tab + ((lane >> 5) * 32 + (lane & 31)) * EWp;
 #pragma unroll
         for (int j = 0; j < LJ; ++j) {
             uint64_t msk = lmask[j];
             while (msk) {
                 const int ln = __builtin_ctzll(msk);
                 msk &= msk - 1;
-                if ((rank++ & 3) != w) continue;
                 const int l = 64 * j + ln;
                 const int R = __builtin_amdgcn_readlane(radv[j], ln);
-                const float2 *plane = iddSigma + (size_t(l) * g.K + k) * g.sy * g.sx;
-                // all four pixels of this lane for the layer, loads issued together
-                float dv[4], rv[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int gx = tx * 32 + (lane & 31), gy = ty * 8 + 2 * q + (lane >> 5);
-                    dv[q] = 0.0f;
-                    rv[q] = PBS_INF;
-                    if (gx < g.rx && gy < g.ry) {
-                        const float2 v = plane[size_t(gy) * g.sx + gx];
-                        dv[q] = v.x;
-                        rv[q] = v.y;
+                float d = 0.0f, r = PBS_INF;
+                if (pixIn) {
+                    const float2 v = iddSigma[(size_t(l) * g.K + k) * g.sy * g.sx + pixOff];
+                    d = v.x;
+                    r = v.y;
+                }
+                const uint64_t live = __ballot(d != 0.0f);
+                const unsigned rowLive = (unsigned(uint32_t(live) != 0u)) | (unsigned(uint32_t(live >> 32) != 0u) << 1);
+                if (!rowLive) continue;  // uniform
+                e[0] = d;
+                {
+                    float en = erff(r * 0.5f), eo = -en;
+                    for (int i = 0; i <= R; ++i) {
+                        e[1 + i] = 0.5f * (en - eo);
+                        eo = en;
+                        if (i < R) en = erff(r * (float(i) + 1.5f));
                     }
                 }
-#pragma unroll
-                for (int half = 0; half < 2; ++half) {
-                    unsigned rowLive = 0;
-#pragma unroll
-                    for (int p = 0; p < 2; ++p) {
-                        const float d = dv[2 * half + p], r = rv[2 * half + p];
-                        const uint64_t live = __ballot(d != 0.0f);
-                        rowLive |= (unsigned(uint32_t(live) != 0u) << (2 * p)) |
-                                   (unsigned(uint32_t(live >> 32) != 0u) << (2 * p + 1));
-                        float *e = tab + ((2 * p + (lane >> 5)) * 32 + (lane & 31)) * EWp;
-                        e[0] = d;
-                        float en = erff(r * 0.5f), eo = -en;
-                        for (int i = 0; i <= R; ++i) {
-                            e[1 + i] = 0.5f * (en - eo);
-                            eo = en;
-                            if (i < R) en = erff(r * (float(i) + 1.5f));
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (rowLive) {
-                        if (half == 0)
-                            sweep_dispatch<0, RM, 0>(R, acc, tab, rowLive, lane);
-                        else
-                            sweep_dispatch<0, RM, 4>(R, acc, tab, rowLive, lane);
-                    }
-                    // the next staging overwrites the table: keep it after these reads
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                sweep_dispatch<0, RM>(R, acc, tab, rowLive, lane);
+                // the next staging overwrites the table: keep it after these reads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
-        // partial blocks added in wave order
+        // partial blocks added in wave order (wave w's rows start at block row 2w)
+        for (int i = threadIdx.x; i < BH * BW; i += 256) s_B[i] = 0.0f;
+        __syncthreads();
         for (int ww = 0; ww < 4; ++ww) {
             if (w == ww) {
 #pragma unroll
@@ -985,10 +975,7 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
                     const int col = c * 64 + lane;
                     if (col < BW) {
 #pragma unroll
-                        for (int q = 0; q < BH; ++q) {
-                            float *o = s_B + q * BW + col;
-                            *o = ww == 0 ? acc[c][q] : *o + acc[c][q];
-                        }
+                        for (int q = 0; q < NR; ++q) s_B[(2 * w + q) * BW + col] += acc.v[c][q];
                     }
                 }
             }
