@@ -457,7 +457,7 @@ int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *do
 // the radius classes on the engine and side streams; merge; transfer into the dose.
 // `mark` (optional) records a stage boundary on the engine stream.
 static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork &w, const float2 *iddSigma,
-                    LayerScalars *ls, BeamScalars *bs, int *overflowFlag, float *doseDev,
+                    LayerScalars *ls, BeamScalars *bs, int *overflowFlag, bool codesFromFill, float *doseDev,
                     const hipStream_t streams[1 + kSideStreams], std::vector<hipEvent_t> *marks) {
     hipStream_t s = streams[0];
     auto mk = [&]() {
@@ -468,7 +468,8 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     HIP_TRY(mk());
     HIP_TRY(launch_finalize_and_tiles(g, iddSigma, ls, w.groupScal.p, bs, w.tailPlaneGroup.p, w.tailPlaneK.p,
                                       w.tailCap, w.tileRad.p, w.hist.p, w.effRad.p, w.itemR.p, w.tailItemR.p,
-                                      w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, s));
+                                      w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, codesFromFill ? 1 : 0,
+                                      s));
     HIP_TRY(mk());
     HIP_TRY(hipEventRecord(e->forkEv, s));
     for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
@@ -643,12 +644,12 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (e->nuclearModel == 2) fc = FillConsts{190.44f, 0.0f, 0, 0};   // SOUKUP
     if (e->nuclearModel == 3) fc = FillConsts{216.09f, 0.08f, 0, 0};  // FLUKA
     HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
-                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, s));
+                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p, s));
     // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
     for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];
-    int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, doseDev, streams,
-                       &marks);
+    int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev,
+                       streams, &marks);
     if (rc2 != PBS_OK) return rc2;
     // 7. halo grid through the same stages
     HIP_TRY(mark());
@@ -656,7 +657,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(launch_seed_grid(L, e->layerScal.p, e->beamScal.p, e->haloLayerScal.p, e->haloBeamScal.p,
                                  e->halo.hist.p, e->halo.classCount.p, s));
         rc2 = run_grid(e, p, gh, e->halo, e->haloIddSigma.p, e->haloLayerScal.p, e->haloBeamScal.p, e->haloErr.p,
-                       doseDev, streams, nullptr);
+                       false, doseDev, streams, nullptr);
         if (rc2 != PBS_OK) return rc2;
     }
     e->lastHalo = halo;

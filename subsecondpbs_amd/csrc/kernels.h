@@ -53,14 +53,15 @@ hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *ray
 hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
                        const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
-                       hipStream_t s);
+                       int *tileCode, hipStream_t s);
 hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
                             BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s);
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s);
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
-                                     int2 *classList, int maxUnits, int *overflowFlag, hipStream_t s);
+                                     int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
+                                     hipStream_t s);
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
                             const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,

@@ -374,43 +374,58 @@ __global__ __launch_bounds__(256) void k_cpb_to_rays(CpbArgs a, const float *__r
     rays[(size_t(l) * rny + y) * rnx + x] = acc;
 }
 
+// Tile radius of a 32x8 tile from its smallest rSigmaEff (complete_superposition.cu:242):
+// min(int(KS_SIGMA_CUTOFF / (sqrt(2) min) + 1/2), MAX_RAD + 1); NaN minimum -> 0.
+__device__ __forceinline__ int tile_radius(float m) {
+    if (m != m) return 0;
+    const float q = 3.0f / (1.41421356237f * m) + 0.5f;  // KS_SIGMA_CUTOFF / (sqrt(2) min) + 1/2
+    const int r = int(q);
+    return r < kMaxRadDev + 1 ? r : kMaxRadDev + 1;
+}
+
 // ---------------------------------------------------------------------------------------
 // IDD and sigma
 // ---------------------------------------------------------------------------------------
+// One block per (32x8 tile, layer); thread per ray of the tile. Besides IDD / rSigmaEff per
+// step it reduces each step's tile: code = radius | live << 8 (the tileRadCalc of
+// complete_superposition.cu:197-255, fused so the field is not read back).
 __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const LayerHost *__restrict__ lh,
                                               LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                               const float *__restrict__ density, const float *__restrict__ wepl,
                                               const float *__restrict__ rayW, const int *__restrict__ fi,
                                               const int *__restrict__ fo, float2 *__restrict__ iddSigma,
-                                              int *__restrict__ firstPassive, FillConsts fc, HaloFill hf) {
+                                              int *__restrict__ firstPassive, FillConsts fc, HaloFill hf,
+                                              int *__restrict__ tileCode) {
     extern __shared__ float s_rrl[];
+    __shared__ float s_min[2][4];
+    __shared__ int s_live[2][4];
     for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
-    __syncthreads();
+    const int ntx = g.sx / 32, nt = ntx * (g.sy / 8);
+    const int t = blockIdx.x, l = blockIdx.y;
+    const int x = (t % ntx) * 32 + (threadIdx.x & 31), y = (t / ntx) * 8 + (threadIdx.x >> 5);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool rayIn = x < g.rx && y < g.ry;
     const int nray = g.rx * g.ry;
-    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
-    // every lane stays to the wave reduction at the end; lanes past the last ray add INT_MIN
-    int passive = INT_MIN;
-    if (ray < nray) {
+    const int ray = rayIn ? y * g.rx + x : 0;
     const LayerHost L = lh[l];
     const unsigned first = unsigned(bs->beamFirstInside);
     const unsigned pAfter = unsigned(ls[l].afterLast);
-    const unsigned firstIn = unsigned(fi[ray]);
-    unsigned afterLast = unsigned(min(fo[ray], int(pAfter)));
-    const float w = rayW[size_t(l) * nray + ray];
-    bool live = true;
-    if (w < 1e-6f || afterLast < first) { live = false; afterLast = 0; }
+    const unsigned firstIn = rayIn ? unsigned(fi[ray]) : 0u;
+    unsigned afterLast = rayIn ? unsigned(min(fo[ray], int(pAfter))) : 0u;
+    const float w0 = rayIn ? rayW[size_t(l) * nray + ray] : 0.0f;
+    bool live = rayIn;
+    if (w0 < 1e-6f || afterLast < first) { live = false; afterLast = 0; }
     float cumulSp = 0.f, cumulSpOld = 0.f, cumulDose = 0.f, cumulDoseOld = 0.f;
     float incScat = 0.f, incincScat = 0.f;
     float incDiv = L.airLin + (2.0f * float(first) - 1.0f) * L.airQuad;
     float sigmaSq = -incDiv;
     const float stepLength = fabsf(g.fd[2]);
-    const int x = ray % g.rx, y = ray / g.rx;
     const size_t plane = size_t(nray);
     float2 *out = iddSigma + (size_t(l) * g.K * g.sy + y) * g.sx + x;
     const size_t outStep = size_t(g.sx) * g.sy;
+    int *codes = tileCode + size_t(l) * g.K * nt + t;
     // halo cell of this ray (the spot it centres) and the halo entry sigma^2 (wrapper:4324-4329)
-    const int cell = hf.rayToNuc ? hf.rayToNuc[ray] : -1;
+    const int cell = (hf.rayToNuc && rayIn) ? hf.rayToNuc[ray] : -1;
     const bool halo = hf.rayToNuc != nullptr;
     float nucRayW = 0.0f, entrySigmaSq = 0.0f;
     float2 *nout = nullptr;
@@ -423,13 +438,16 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
             entrySigmaSq = L.airA * ez * ez + L.airC * ez + L.sigmaX * L.sigmaX;
         }
     }
+    __syncthreads();
     for (unsigned k = first; k < pAfter; ++k) {
         float res = 0.0f, rs = PBS_INF, nres = 0.0f, nrs = PBS_INF;
         const size_t idx = size_t(k) * plane + ray;
+        // loads do not depend on the recurrence: issued every step so they can run ahead
+        const float spNow = wepl[idx];
+        const float rho = density[idx];
         if (live) {
-            cumulSp = wepl[idx];
+            cumulSp = spNow;
             cumulDose = lut2d_clamp(lut.cidd, lut.nS, lut.nE, cumulSp * L.scaleFact, L.energyIdx);
-            const float rho = density[idx];
             if (cumulSp < L.peakDepth) {
                 const float resE = 8.639415f * powf(L.peakDepth - 0.5f * (cumulSp + cumulSpOld), 0.5649718f);
                 const float betaP = resE + 938.3f - 938.3f * 938.3f / (resE + 938.3f);
@@ -453,11 +471,11 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
                 if (halo) {
                     // idd_sigma.cu:194-213: the halo takes w_nuc of the step's energy
                     const float nw = lut2d_clamp(hf.nucWeightLut, lut.nS, lut.nE, depthMid, L.energyIdx);
-                    res = (1.0f - nw) * w * (cumulDose - cumulDoseOld) / mass;
+                    res = (1.0f - nw) * w0 * (cumulDose - cumulDoseOld) / mass;
                     if (cell >= 0)
                         nres = nw * nucRayW * (cumulDose - cumulDoseOld) / (mass * hf.spotDist * hf.spotDist);
                 } else {
-                    res = w * (cumulDose - cumulDoseOld) / mass;
+                    res = w0 * (cumulDose - cumulDoseOld) / mass;
                 }
             }
             if (cell >= 0) {
@@ -471,13 +489,47 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
         if (!live || int(k) < int(firstIn) - 1) { res = 0.0f; rs = PBS_INF; nres = 0.0f; nrs = PBS_INF; }
         out[size_t(k) * outStep] = make_float2(res, rs);
         if (nout) nout[size_t(k) * hf.nucPlane] = make_float2(nres, nrs);
+        // the step's tile code: minimum rSigmaEff (NaN ignored, fminf) and any live pixel
+        const float m = wave_min(rs);
+        const bool any = __any(res > 0.0f);
+        const int buf = int(k & 1u);
+        if (lane == 0) {
+            s_min[buf][w] = m;
+            s_live[buf][w] = any ? 1 : 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float mm = fminf(fminf(s_min[buf][0], s_min[buf][1]), fminf(s_min[buf][2], s_min[buf][3]));
+            const int lv = s_live[buf][0] | s_live[buf][1] | s_live[buf][2] | s_live[buf][3];
+            codes[size_t(k) * nt] = tile_radius(mm) | (lv ? 256 : 0);
+        }
     }
-    firstPassive[size_t(l) * nray + ray] = int(afterLast);
-    passive = int(afterLast);
+    int passive = INT_MIN;
+    if (rayIn) {
+        firstPassive[size_t(l) * nray + ray] = int(afterLast);
+        passive = int(afterLast);
     }
     // layer passive step = max over rays (wrapper:4950-4979): one atomic per wave
-    const int m = wave_max_i(passive);
-    if ((threadIdx.x & 63) == 0) atomicMax(&ls[l].passiveRaw, m);
+    const int mx = wave_max_i(passive);
+    if (lane == 0) atomicMax(&ls[l].passiveRaw, mx);
+}
+
+// Radius histogram of one layer from its tile codes over the superposed steps
+// [beamFirstInside, passive) (complete_superposition.cu:197-255, 400-463); one block per layer.
+__global__ __launch_bounds__(256) void k_tile_hist(FanGeom g, const int *__restrict__ tileCode,
+                                                   const LayerScalars *__restrict__ ls,
+                                                   const BeamScalars *__restrict__ bs, int *__restrict__ hist) {
+    __shared__ int s_h[kMaxRadDev + 2];
+    const int l = blockIdx.x;
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    if (threadIdx.x < kMaxRadDev + 2) s_h[threadIdx.x] = 0;
+    __syncthreads();
+    const int k0 = bs->beamFirstInside, k1 = ls[l].passive;
+    const int n = k1 > k0 ? (k1 - k0) * nt : 0;
+    const int *codes = tileCode + (size_t(l) * g.K + k0) * nt;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&s_h[codes[i] & 255], 1);
+    __syncthreads();
+    if (threadIdx.x < kMaxRadDev + 2) hist[l * (kMaxRadDev + 2) + threadIdx.x] = s_h[threadIdx.x];
 }
 
 // Transfer box of one passive step (wrapper:5343-5395): 8 fan corners -> dose index in float,
@@ -518,13 +570,17 @@ __device__ __forceinline__ float sampled_plane(const FanGeom &g, int inside, flo
     return (gz - g.fc[2]) / g.fd[2] - float(inside);
 }
 
-// Layer passive steps, layer groups, group boxes and tail planes (one block of 256 threads).
+// Layer passive steps, layer groups, group boxes and tail planes (one block of 256 threads;
+// the groups are built in LDS and written out once).
 __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
                                                         GroupScalars *__restrict__ gs, BeamScalars *__restrict__ bs,
                                                         int *__restrict__ tailPlaneGroup, int *__restrict__ tailPlaneK,
                                                         int tailCap) {
     __shared__ int s_pas[kMaxLayersDev];
     __shared__ int s_first[kMaxLayersDev];
+    __shared__ int s_group[kMaxLayersDev];
+    __shared__ GroupScalars s_gs[kMaxLayersDev];
+    __shared__ int s_m, s_total;
     const int inside = bs->beamFirstInside;
     const int L = g.numLayers;
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
@@ -549,15 +605,19 @@ __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars 
         if (s_pas[l] >= 0) {
             rank = 0;
             for (int q = 0; q < L; ++q) rank += (s_first[q] && s_pas[q] < s_pas[l]) ? 1 : 0;
-            if (s_first[l]) gs[rank].passive = s_pas[l];
+            if (s_first[l]) s_gs[rank].passive = s_pas[l];
         }
-        ls[l].group = rank;
+        s_group[l] = rank;
+    }
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int q = 0; q < L; ++q) m += s_first[q];
+        s_m = m;
     }
     __syncthreads();
-    int m = 0;
-    for (int q = 0; q < L; ++q) m += s_first[q];
+    const int m = s_m;
     for (int j = threadIdx.x; j < m; j += blockDim.x) {
-        GroupScalars G = gs[j];
+        GroupScalars G = s_gs[j];
         transfer_box(g, inside, G.passive, G.boxStart, G.boxEnd, G.boxMax);
         // main BEV keeps steps below passive - inside - 1: no voxel outside the group's box
         // samples them; the tail covers the rest of what voxels inside the box can sample.
@@ -571,44 +631,62 @@ __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars 
         }
         const int tailEnd = min(G.passive, int(floorf(tmax)) + 2);
         G.tailCount = max(0, tailEnd - G.tailStart);
-        G.tailBase = 0;
-        gs[j] = G;
+        G.pad = 0;
+        s_gs[j] = G;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int base = 0, mp = INT_MIN, me = inside;
-        int st[3] = {INT_MAX, INT_MAX, INT_MAX}, en[3] = {INT_MIN, INT_MIN, INT_MIN}, mi[3] = {-1, -1, -1};
+        int base = 0;
         for (int j = 0; j < m; ++j) {
-            gs[j].tailBase = base;
-            for (int q = 0; q < gs[j].tailCount && base + q < tailCap; ++q) {
-                tailPlaneGroup[base + q] = j;
-                tailPlaneK[base + q] = gs[j].tailStart + q;
-            }
-            base += gs[j].tailCount;
-            mp = max(mp, gs[j].passive);
-            me = max(me, gs[j].tailStart);
-            for (int r = 0; r < 3; ++r) {
-                st[r] = min(st[r], gs[j].boxStart[r]);
-                en[r] = max(en[r], gs[j].boxEnd[r]);
-                mi[r] = max(mi[r], gs[j].boxMax[r]);
-            }
+            s_gs[j].tailBase = base;
+            base += s_gs[j].tailCount;
         }
-        bs->numGroups = m;
-        bs->tailPlanes = min(base, tailCap);
-        bs->overflow = base > tailCap ? 1 : 0;
-        bs->maxPassive = mp;
-        bs->mainEndMax = me;
-        bs->activeLayers = m > 0 ? 1 : 0;
-        for (int r = 0; r < 3; ++r) {
-            bs->boxStart[r] = m > 0 ? st[r] : 0;
-            bs->boxEnd[r] = m > 0 ? en[r] : 0;
-            bs->boxMax[r] = mi[r];
-        }
+        s_total = base;
     }
     __syncthreads();
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        const GroupScalars G = s_gs[j];
+        for (int q = 0; q < G.tailCount && G.tailBase + q < tailCap; ++q) {
+            tailPlaneGroup[G.tailBase + q] = j;
+            tailPlaneK[G.tailBase + q] = G.tailStart + q;
+        }
+        gs[j] = G;
+    }
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
-        const int j = ls[l].group;
-        ls[l].mainEnd = j >= 0 ? gs[j].tailStart : inside;
+        const int j = s_group[l];
+        ls[l].group = j;
+        ls[l].mainEnd = j >= 0 ? s_gs[j].tailStart : inside;
+    }
+    // union box and tail totals: one wave reduces over the groups
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int mp = INT_MIN, me = inside;
+        int st0 = INT_MAX, st1 = INT_MAX, st2 = INT_MAX, en0 = INT_MIN, en1 = INT_MIN, en2 = INT_MIN;
+        int mi0 = -1, mi1 = -1, mi2 = -1;
+        for (int j = lane; j < m; j += 64) {
+            const GroupScalars &G = s_gs[j];
+            mp = max(mp, G.passive);
+            me = max(me, G.tailStart);
+            st0 = min(st0, G.boxStart[0]); st1 = min(st1, G.boxStart[1]); st2 = min(st2, G.boxStart[2]);
+            en0 = max(en0, G.boxEnd[0]); en1 = max(en1, G.boxEnd[1]); en2 = max(en2, G.boxEnd[2]);
+            mi0 = max(mi0, G.boxMax[0]); mi1 = max(mi1, G.boxMax[1]); mi2 = max(mi2, G.boxMax[2]);
+        }
+        mp = wave_max_i(mp); me = wave_max_i(me);
+        st0 = wave_min_i(st0); st1 = wave_min_i(st1); st2 = wave_min_i(st2);
+        en0 = wave_max_i(en0); en1 = wave_max_i(en1); en2 = wave_max_i(en2);
+        mi0 = wave_max_i(mi0); mi1 = wave_max_i(mi1); mi2 = wave_max_i(mi2);
+        if (lane == 0) {
+            const int total = s_total;
+            bs->numGroups = m;
+            bs->tailPlanes = min(total, tailCap);
+            bs->overflow = total > tailCap ? 1 : 0;
+            bs->maxPassive = mp;
+            bs->mainEndMax = me;
+            bs->activeLayers = m > 0 ? 1 : 0;
+            bs->boxStart[0] = m > 0 ? st0 : 0; bs->boxStart[1] = m > 0 ? st1 : 0; bs->boxStart[2] = m > 0 ? st2 : 0;
+            bs->boxEnd[0] = m > 0 ? en0 : 0; bs->boxEnd[1] = m > 0 ? en1 : 0; bs->boxEnd[2] = m > 0 ? en2 : 0;
+            bs->boxMax[0] = mi0; bs->boxMax[1] = mi1; bs->boxMax[2] = mi2;
+        }
     }
 }
 
@@ -636,13 +714,6 @@ __global__ void k_clear_pairs(float2 *__restrict__ v, size_t n) {
 // ---------------------------------------------------------------------------------------
 // tile radii, bins and work items
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ int tile_radius(float m) {
-    if (m != m) return 0;
-    const float q = 3.0f / (1.41421356237f * m) + 0.5f;  // KS_SIGMA_CUTOFF / (sqrt(2) min) + 1/2
-    const int r = int(q);
-    return r < kMaxRadDev + 1 ? r : kMaxRadDev + 1;
-}
-
 // one wave per (step, tile); loops over layers. code = rad | live << 8, -1 = not computed
 __global__ __launch_bounds__(256) void k_tiles(FanGeom g, const float2 *__restrict__ iddSigma,
                                                const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
@@ -758,13 +829,17 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
     if (t >= nt) return;
     int Rmax = -1;
     bool mainLive = false;
+    uint64_t tailLive[kMaxLayersDev / 64] = {0, 0, 0, 0};  // groups with a live tail layer here
+    const int inside = bs->beamFirstInside;
     for (int l = 0; l < g.numLayers; ++l) {
+        if (k < inside || k >= ls[l].passive) continue;  // steps the layer does not superpose
         const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
         if (code < 0 || !(code & 256)) continue;
         const int j = ls[l].group;
         const bool isMain = k < ls[l].mainEnd;
         if (!isMain && !(j >= 0 && in_tail(gs[j], k))) continue;
         mainLive |= isMain;
+        if (!isMain) tailLive[j >> 6] |= uint64_t(1) << (j & 63);
         Rmax = max(Rmax, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
     }
     const int c = Rmax >= 0 ? radius_class(Rmax) : 0;
@@ -778,12 +853,7 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
         if (!in_tail(G, k)) continue;
         const int q = G.tailBase + (k - G.tailStart);
         if (q >= bs->tailPlanes) continue;
-        bool live = false;
-        for (int l = 0; l < g.numLayers && !live; ++l) {
-            if (ls[l].group != j) continue;
-            const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
-            live = code >= 0 && (code & 256);
-        }
+        const bool live = (tailLive[j >> 6] >> (j & 63)) & 1;
         tailItemR[size_t(q) * nt + t] = live ? RM : -1;
         if (live) append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
     }
@@ -908,7 +978,7 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
             const int l = 64 * j + lane;
             bool in = false;
             int rad = 0;
-            if (l < g.numLayers) {
+            if (l < g.numLayers && k >= bs->beamFirstInside && k < ls[l].passive) {
                 const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
                 if (code >= 0 && (code & 256)) {
                     const LayerScalars L = ls[l];
@@ -1138,10 +1208,10 @@ hipError_t launch_cpb(const CpbArgs &a, const float *sub, float *cpb, float *ray
 hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh, LayerScalars *ls,
                        const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
-                       hipStream_t s) {
-    const int n = g.rx * g.ry;
-    hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256, g.numLayers), dim3(256), sizeof(float) * lut.nRrl, s, g, lut, lh,
-                       ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf);
+                       int *tileCode, hipStream_t s) {
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    hipLaunchKernelGGL(k_fill, dim3(nt, g.numLayers), dim3(256), sizeof(float) * lut.nRrl, s, g, lut, lh, ls, bs,
+                       density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
     return hipGetLastError();
 }
 
@@ -1160,10 +1230,14 @@ hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s) {
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
-                                     int2 *classList, int maxUnits, int *overflowFlag, hipStream_t s) {
+                                     int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
+                                     hipStream_t s) {
     hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap);
     const int nt = (g.sx / 32) * (g.sy / 8);
-    hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
+    if (codesFromFill)
+        hipLaunchKernelGGL(k_tile_hist, dim3(g.numLayers), dim3(256), 0, s, g, tileRad, ls, bs, hist);
+    else
+        hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
                        overflowFlag);
     hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
