@@ -64,7 +64,7 @@ using namespace pbs;
 
 // Buffers of one superposition grid (tiles, work lists, item blocks, BEV planes, groups).
 struct GridWork {
-    DevBuf<int> tileRad, hist, effRad, itemR, classCount, tailPlaneGroup, tailPlaneK, tailItemR;
+    DevBuf<int> tileRad, hist, effRad, itemR, classCount, tailPlaneGroup, tailPlaneK, tailItemR, planeRM;
     DevBuf<int2> classList;
     DevBuf<float> slots, tailSlots, bev, tailBev;
     DevBuf<GroupScalars> groupScal;
@@ -91,6 +91,7 @@ struct GridWork {
         if ((r = tailPlaneGroup.ensure(tailCap)) != hipSuccess) return r;
         if ((r = tailPlaneK.ensure(tailCap)) != hipSuccess) return r;
         if ((r = tailItemR.ensure(size_t(tailCap) * nt)) != hipSuccess) return r;
+        if ((r = planeRM.ensure(size_t(K) + tailCap)) != hipSuccess) return r;
         if ((r = tailSlots.ensure(size_t(tailCap) * nt * kSlotFloats)) != hipSuccess) return r;
         return tailBev.ensure(size_t(tailCap) * bevPlane);
     }
@@ -113,7 +114,7 @@ struct pbs_engine {
     int doseDims[3] = {0, 0, 0};
     float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
     // per-beam work buffers
-    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, cpb;
+    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, cpb, traceScratch;
     DevBuf<float2> iddSigma;
     DevBuf<int> fi, fo, firstPassive;
     // superposition -> merge -> transfer state of the primary (ray) grid and the halo grid
@@ -469,7 +470,7 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     HIP_TRY(launch_finalize_and_tiles(g, iddSigma, ls, w.groupScal.p, bs, w.tailPlaneGroup.p, w.tailPlaneK.p,
                                       w.tailCap, w.tileRad.p, w.hist.p, w.effRad.p, w.itemR.p, w.tailItemR.p,
                                       w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, codesFromFill ? 1 : 0,
-                                      s));
+                                      w.planeRM.p, s));
     HIP_TRY(mk());
     HIP_TRY(hipEventRecord(e->forkEv, s));
     for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
@@ -481,13 +482,18 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     }
     HIP_TRY(mk());
     HIP_TRY(launch_merge(g, bs, w.itemR.p, w.slots.p, w.bev.p, w.tailItemR.p, w.tailSlots.p, w.tailBev.p, w.tailCap,
-                         s));
+                         w.planeRM.p, s));
     TransferArgs ta{};
     fill_transfer_args(p, g, ta);
     int gx = 1, gy = 1, gz = 1;
     host_transfer_grid(p, g, e->doseDims, gx, gy, gz);
     HIP_TRY(mk());
-    HIP_TRY(launch_transfer(g, ta, bs, w.groupScal.p, w.bev.p, w.tailBev.p, doseDev, gx, gy, gz, s));
+    // dose axis along which the fan depth changes fastest (row 2 of doseIdx -> gantry)
+    int depthAxis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (std::fabs(g.d2g[6 + a]) > std::fabs(g.d2g[6 + depthAxis])) depthAxis = a;
+    HIP_TRY(launch_transfer(g, ta, bs, w.groupScal.p, w.bev.p, w.tailBev.p, doseDev, gx * 32, gy * 8, gz, depthAxis,
+                            s));
     return PBS_OK;
 }
 
@@ -524,6 +530,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     }
     // buffers
     HIP_TRY(e->density.ensure(nray * K));
+    HIP_TRY(e->traceScratch.ensure(trace_scratch_floats(g)));
     HIP_TRY(e->wepl.ensure(nray * K));
     HIP_TRY(e->fi.ensure(nray));
     HIP_TRY(e->fo.ensure(nray));
@@ -587,7 +594,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     };
     // 1. trace and per-beam reductions
     HIP_TRY(mark());
-    HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
+    HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p, s));
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,

@@ -38,8 +38,10 @@ struct TransferArgs {
     float normDist[2];
 };
 
+// Scratch floats the trace needs (the segment prefixes and carries), 8-byte aligned start.
+size_t trace_scratch_floats(const FanGeom &g);
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
-                        int *fo, hipStream_t s);
+                        int *fo, float *scratch, hipStream_t s);
 hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,
                                BeamScalars *bs, hipStream_t s);
 hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
@@ -61,15 +63,16 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
-                                     hipStream_t s);
+                                     int *planeRM, hipStream_t s);
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
                             const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,
                             float *tailSlots, int gridCap, const hipStream_t streams[1 + kSideStreams]);
 hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
-                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, hipStream_t s);
+                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, const int *planeRM,
+                        hipStream_t s);
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
-                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, int gridZ,
-                           hipStream_t s);
+                           const float *bev, const float *tailBev, float *dose, int extentX, int extentY, int extentZ,
+                           int depthAxis, hipStream_t s);
 
 }  // namespace pbs

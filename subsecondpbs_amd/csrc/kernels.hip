@@ -112,15 +112,26 @@ __device__ int first_larger_ordered(const float *v, int n, float value) {
 // ---------------------------------------------------------------------------------------
 // trace
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_trace(FanGeom g, const float *__restrict__ ct, LutDev lut,
-                                               float *__restrict__ density, float *__restrict__ wepl,
-                                               int *__restrict__ firstInside, int *__restrict__ firstOutside) {
+// The trace runs as (ray, 16-step segment) threads so that every ray's K steps are spread
+// over many waves: pass 1 samples the CT and keeps segment-local prefixes, pass 2 turns the
+// segment totals into carries and the entry / exit steps, pass 3 adds the carries.
+constexpr int kTraceSeg = 16;
+
+// Pass 1 (bev_ray_tracing.cu:65-159 per segment): explicit trilinear CT sample at fan step k
+// (texel centre -1/2, border 0), density and SP from the LUTs in LDS, the segment's Kahan
+// prefix of stepLen * SP (written to wepl), the segment's HU prefix, its totals and its last
+// step with HU+1000 > 150.
+__global__ __launch_bounds__(256) void k_trace_seg(FanGeom g, const float *__restrict__ ct, LutDev lut,
+                                                   float *__restrict__ density, float *__restrict__ wepl,
+                                                   float *__restrict__ huPrefix, float2 *__restrict__ segSp,
+                                                   float *__restrict__ segHu, int *__restrict__ segLast) {
     extern __shared__ float s_lut[];  // density LUT, then stopping-power LUT
     for (int i = threadIdx.x; i < lut.nDensity; i += blockDim.x) s_lut[i] = lut.density[i];
     for (int i = threadIdx.x; i < lut.nSp; i += blockDim.x) s_lut[lut.nDensity + i] = lut.sp[i];
     __syncthreads();
     const int nray = g.rx * g.ry;
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const int seg = blockIdx.y;
     if (ray >= nray) return;
     const float *s_dens = s_lut, *s_sp = s_lut + lut.nDensity;
     const int i = ray % g.rx, j = ray / g.rx;
@@ -144,30 +155,77 @@ __global__ __launch_bounds__(256) void k_trace(FanGeom g, const float *__restric
     }
     const float dx = fx / g.dist[0], dy = fy / g.dist[1];
     const float stepLen = fabsf(g.fd[2]) * sqrtf(1.0f + dx * dx + dy * dy);
+    const int k0 = seg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
     float cum = 0.0f, comp = 0.0f, cumHu = 0.0f;
-    unsigned before = 0, lastIn = 0;
-    bool flag = true;
-    size_t idx = size_t(ray);
-    const size_t plane = size_t(nray);
+    int lastIn = -1;
+    size_t idx = size_t(k0) * nray + ray;
 #pragma unroll 4
-    for (int k = 0; k < g.K; ++k) {
+    for (int k = k0; k < k1; ++k) {
         const float fk = float(k);
         const float hu = trilinear_border(ct, g.ctDims[0], g.ctDims[1], g.ctDims[2], p0[0] + fk * inc[0],
                                           p0[1] + fk * inc[1], p0[2] + fk * inc[2]);
         cumHu += hu;
         density[idx] = lut1d_clamp(s_dens, lut.nDensity, hu * lut.densityScale);
-        // compensated prefix of stepLen * SP
         const float y = stepLen * lut1d_clamp(s_sp, lut.nSp, hu * lut.spScale) - comp;
         const float t = cum + y;
         comp = (t - cum) - y;
         cum = t;
-        if (flag && cumHu < 150.0f) before = unsigned(k); else flag = false;
-        if (hu > 150.0f) lastIn = unsigned(k);
+        if (hu > 150.0f) lastIn = k;
         wepl[idx] = cum;
-        idx += plane;
+        huPrefix[idx] = cumHu;
+        idx += nray;
     }
-    firstInside[ray] = int(before) + 1;
-    firstOutside[ray] = int(lastIn) + 1;
+    const size_t o = size_t(seg) * nray + ray;
+    segSp[o] = make_float2(cum, comp);
+    segHu[o] = cumHu;
+    segLast[o] = lastIn;
+}
+
+// Pass 2, thread per ray: segment carries (double), entry and exit steps. The cumulative HU
+// never decreases (HU+1000 >= 0), so the reference's flag loop (bev_ray_tracing.cu:141-158)
+// reduces to: firstInside = max(#steps with cumulative HU < 150, 1); firstOutside =
+// last step with HU+1000 > 150 (0 if none) + 1.
+__global__ __launch_bounds__(256) void k_trace_carry(FanGeom g, const float *__restrict__ huPrefix,
+                                                     const float2 *__restrict__ segSp,
+                                                     const float *__restrict__ segHu, const int *__restrict__ segLast,
+                                                     double *__restrict__ segCarry, int *__restrict__ firstInside,
+                                                     int *__restrict__ firstOutside) {
+    const int nray = g.rx * g.ry;
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= nray) return;
+    const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
+    double carrySp = 0.0, carryHu = 0.0;
+    int count = -1, last = -1;
+    for (int sg = 0; sg < S; ++sg) {
+        const size_t o = size_t(sg) * nray + ray;
+        segCarry[o] = carrySp;
+        const float2 sp = segSp[o];
+        const double hu = double(segHu[o]);
+        if (count < 0 && carryHu + hu >= 150.0) {
+            // the threshold is crossed in this segment: find the step
+            const int k0 = sg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
+            count = k1;
+            for (int k = k0; k < k1; ++k)
+                if (carryHu + double(huPrefix[size_t(k) * nray + ray]) >= 150.0) { count = k; break; }
+        }
+        carrySp += double(sp.x) - double(sp.y);
+        carryHu += hu;
+        last = max(last, segLast[o]);
+    }
+    if (count < 0) count = g.K;
+    firstInside[ray] = max(count, 1);
+    firstOutside[ray] = max(last, 0) + 1;
+}
+
+// Pass 3, thread per (step, ray): the segment's carry added to its local WEPL prefix.
+__global__ __launch_bounds__(256) void k_trace_fix(FanGeom g, const double *__restrict__ segCarry,
+                                                   float *__restrict__ wepl) {
+    const int nray = g.rx * g.ry;
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y + kTraceSeg;  // segment 0 has no carry
+    if (ray >= nray || k >= g.K) return;
+    const size_t idx = size_t(k) * nray + ray;
+    wepl[idx] = float(segCarry[size_t(k / kTraceSeg) * nray + ray] + double(wepl[idx]));
 }
 
 // blocks [0, K): min WEPL over rays that leave material; block K: entry/exit step extremes.
@@ -386,6 +444,20 @@ __device__ __forceinline__ int tile_radius(float m) {
 // ---------------------------------------------------------------------------------------
 // IDD and sigma
 // ---------------------------------------------------------------------------------------
+// Bilinear lookup in a table row pair staged in LDS: the same arithmetic as lut2d_clamp for
+// a fixed row coordinate (rows r0 = row y0, r1 = row y1, fraction ay).
+__device__ __forceinline__ float lut_rows(const float *__restrict__ r0, const float *__restrict__ r1, int nx,
+                                          float ay, float xb) {
+    const float fx = floorf(xb);
+    const float ax = xb - fx;
+    int x0 = int(fx);
+    const int x1 = min(max(x0 + 1, 0), nx - 1);
+    x0 = min(max(x0, 0), nx - 1);
+    const float v0 = (1.0f - ax) * r0[x0] + ax * r0[x1];
+    const float v1 = (1.0f - ax) * r1[x0] + ax * r1[x1];
+    return (1.0f - ay) * v0 + ay * v1;
+}
+
 // One block per (32x8 tile, layer); thread per ray of the tile. Besides IDD / rSigmaEff per
 // step it reduces each step's tile: code = radius | live << 8 (the tileRadCalc of
 // complete_superposition.cu:197-255, fused so the field is not read back).
@@ -396,12 +468,30 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
                                               const int *__restrict__ fo, float2 *__restrict__ iddSigma,
                                               int *__restrict__ firstPassive, FillConsts fc, HaloFill hf,
                                               int *__restrict__ tileCode) {
-    extern __shared__ float s_rrl[];
+    // LDS: 1/X0 LUT, then the layer's two CIDD rows, then (halo) its nuclear table rows
+    extern __shared__ float s_lds[];
     __shared__ float s_min[2][4];
     __shared__ int s_live[2][4];
-    for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
     const int ntx = g.sx / 32, nt = ntx * (g.sy / 8);
     const int t = blockIdx.x, l = blockIdx.y;
+    const float eIdx = lh[l].energyIdx;
+    const float eFl = floorf(eIdx);
+    const float ay = eIdx - eFl;
+    const int ey0 = min(max(int(eFl), 0), lut.nE - 1), ey1 = min(max(int(eFl) + 1, 0), lut.nE - 1);
+    float *s_rrl = s_lds;
+    float *c0 = s_lds + lut.nRrl, *c1 = c0 + lut.nS;
+    float *nw0 = c1 + lut.nS, *nw1 = nw0 + lut.nS, *ns0 = nw1 + lut.nS, *ns1 = ns0 + lut.nS;
+    for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
+    for (int i = threadIdx.x; i < lut.nS; i += blockDim.x) {
+        c0[i] = lut.cidd[size_t(ey0) * lut.nS + i];
+        c1[i] = lut.cidd[size_t(ey1) * lut.nS + i];
+        if (hf.rayToNuc) {
+            nw0[i] = hf.nucWeightLut[size_t(ey0) * lut.nS + i];
+            nw1[i] = hf.nucWeightLut[size_t(ey1) * lut.nS + i];
+            ns0[i] = hf.nucSqSigmaLut[size_t(ey0) * lut.nS + i];
+            ns1[i] = hf.nucSqSigmaLut[size_t(ey1) * lut.nS + i];
+        }
+    }
     const int x = (t % ntx) * 32 + (threadIdx.x & 31), y = (t / ntx) * 8 + (threadIdx.x >> 5);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const bool rayIn = x < g.rx && y < g.ry;
@@ -439,15 +529,25 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
         }
     }
     __syncthreads();
+    // the next step's WEPL and density are loaded one step ahead (they do not depend on the
+    // recurrence), so the memory latency overlaps the step's arithmetic
+    float spNext = 0.0f, rhoNext = 0.0f;
+    if (first < pAfter) {
+        spNext = wepl[size_t(first) * plane + ray];
+        rhoNext = density[size_t(first) * plane + ray];
+    }
     for (unsigned k = first; k < pAfter; ++k) {
         float res = 0.0f, rs = PBS_INF, nres = 0.0f, nrs = PBS_INF;
         const size_t idx = size_t(k) * plane + ray;
-        // loads do not depend on the recurrence: issued every step so they can run ahead
-        const float spNow = wepl[idx];
-        const float rho = density[idx];
+        const float spNow = spNext;
+        const float rho = rhoNext;
+        if (k + 1 < pAfter) {
+            spNext = wepl[idx + plane];
+            rhoNext = density[idx + plane];
+        }
         if (live) {
             cumulSp = spNow;
-            cumulDose = lut2d_clamp(lut.cidd, lut.nS, lut.nE, cumulSp * L.scaleFact, L.energyIdx);
+            cumulDose = lut_rows(c0, c1, lut.nS, ay, cumulSp * L.scaleFact);
             if (cumulSp < L.peakDepth) {
                 const float resE = 8.639415f * powf(L.peakDepth - 0.5f * (cumulSp + cumulSpOld), 0.5649718f);
                 const float betaP = resE + 938.3f - 938.3f * 938.3f / (resE + 938.3f);
@@ -470,7 +570,7 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
             if (mass > 1e-2f) {
                 if (halo) {
                     // idd_sigma.cu:194-213: the halo takes w_nuc of the step's energy
-                    const float nw = lut2d_clamp(hf.nucWeightLut, lut.nS, lut.nE, depthMid, L.energyIdx);
+                    const float nw = lut_rows(nw0, nw1, lut.nS, ay, depthMid);
                     res = (1.0f - nw) * w0 * (cumulDose - cumulDoseOld) / mass;
                     if (cell >= 0)
                         nres = nw * nucRayW * (cumulDose - cumulDoseOld) / (mass * hf.spotDist * hf.spotDist);
@@ -479,7 +579,7 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
                 }
             }
             if (cell >= 0) {
-                const float s2 = lut2d_clamp(hf.nucSqSigmaLut, lut.nS, lut.nE, depthMid, L.energyIdx);
+                const float s2 = lut_rows(ns0, ns1, lut.nS, ay, depthMid);
                 nrs = 0.5f * hf.spotDist * (vwx + vwy) / (1.41421356f * sqrtf(sigmaSq + s2 + entrySigmaSq));
             }
             cumulSpOld = cumulSp;
@@ -575,7 +675,8 @@ __device__ __forceinline__ float sampled_plane(const FanGeom &g, int inside, flo
 __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
                                                         GroupScalars *__restrict__ gs, BeamScalars *__restrict__ bs,
                                                         int *__restrict__ tailPlaneGroup, int *__restrict__ tailPlaneK,
-                                                        int tailCap) {
+                                                        int tailCap, int *__restrict__ planeRM) {
+    for (int i = threadIdx.x; i < g.K + tailCap; i += blockDim.x) planeRM[i] = -1;  // per-plane max layout radius
     __shared__ int s_pas[kMaxLayersDev];
     __shared__ int s_first[kMaxLayersDev];
     __shared__ int s_group[kMaxLayersDev];
@@ -822,7 +923,7 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
                                                const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
                                                int *__restrict__ itemR, int *__restrict__ tailItemR,
                                                int *__restrict__ classCount, int2 *__restrict__ classList,
-                                               int maxUnits) {
+                                               int maxUnits, int *__restrict__ planeRM) {
     const int nt = (g.sx / 32) * (g.sy / 8);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -846,7 +947,10 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
     const int RM = Rmax >= 0 ? class_radius(c) : -1;
     const int slot = k * nt + t;
     itemR[slot] = mainLive ? RM : -1;
-    if (mainLive) append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
+    if (mainLive) {
+        append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
+        atomicMax(&planeRM[k], RM);
+    }
     const int m = bs->numGroups;
     for (int j = 0; j < m; ++j) {
         const GroupScalars G = gs[j];
@@ -855,7 +959,10 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
         if (q >= bs->tailPlanes) continue;
         const bool live = (tailLive[j >> 6] >> (j & 63)) & 1;
         tailItemR[size_t(q) * nt + t] = live ? RM : -1;
-        if (live) append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
+        if (live) {
+            append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
+            atomicMax(&planeRM[g.K + q], RM);
+        }
     }
 }
 
@@ -1065,12 +1172,15 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
 }
 
 // Sum of the item blocks covering BEV pixel (X, Y) of one plane, in (ty, tx) order.
+// RMmax: the largest block layout radius of the plane, which bounds the tiles that can reach
+// the pixel (a tile's block spans 32 + 2RM columns and 8 + 2RM rows around it).
 __device__ __forceinline__ float gather_blocks(const FanGeom &g, int X, int Y, const int *__restrict__ layoutR,
-                                               const float *__restrict__ slots) {
+                                               const float *__restrict__ slots, int RMmax) {
     const int ntx = g.sx / 32, nty = g.sy / 8;
     const int xr = X - 32, yr = Y - 32;
-    const int tx0 = max(0, (xr - 63 + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + 32 + 64 * 32) / 32 - 64);
-    const int ty0 = max(0, (yr - 39 + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + 32 + 8 * 64) / 8 - 64);
+    // floor divisions of possibly negative values (xr, yr >= -32, RMmax <= 32)
+    const int tx0 = max(0, (xr - 31 - RMmax + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + RMmax + 64 * 32) / 32 - 64);
+    const int ty0 = max(0, (yr - 7 - RMmax + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + RMmax + 8 * 64) / 8 - 64);
     float sum = 0.0f;
     for (int ty = ty0; ty <= ty1; ++ty)
         for (int tx = tx0; tx <= tx1; ++tx) {
@@ -1088,7 +1198,8 @@ __device__ __forceinline__ float gather_blocks(const FanGeom &g, int X, int Y, c
 __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__restrict__ bs,
                                                const int *__restrict__ itemR, const float *__restrict__ mainSlots,
                                                float *__restrict__ bev, const int *__restrict__ tailItemR,
-                                               const float *__restrict__ tailSlots, float *__restrict__ tailBev) {
+                                               const float *__restrict__ tailSlots, float *__restrict__ tailBev,
+                                               const int *__restrict__ planeRM) {
     const int bx = g.sx + 64, by = g.sy + 64;
     const int X = blockIdx.x * 64 + (threadIdx.x & 63);
     const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -1097,15 +1208,18 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (z < g.K) {
         float sum = 0.0f;
-        if (z >= bs->beamFirstInside && z < bs->mainEndMax)
-            sum = gather_blocks(g, X, Y, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats);
+        const int RMmax = planeRM[z];
+        if (z >= bs->beamFirstInside && z < bs->mainEndMax && RMmax >= 0)
+            sum = gather_blocks(g, X, Y, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats, RMmax);
         bev[(size_t(z) * by + Y) * bx + X] = sum;
     } else {
         // tail planes, strided over the gridDim.z - K tail block rows
         const int tz = int(gridDim.z) - g.K;
         for (int q = z - g.K; q < bs->tailPlanes; q += tz) {
-            const float sum =
-                gather_blocks(g, X, Y, tailItemR + size_t(q) * nt, tailSlots + size_t(q) * nt * kSlotFloats);
+            const int RMmax = planeRM[g.K + q];
+            const float sum = RMmax < 0 ? 0.0f
+                                        : gather_blocks(g, X, Y, tailItemR + size_t(q) * nt,
+                                                        tailSlots + size_t(q) * nt * kSlotFloats, RMmax);
             tailBev[(size_t(q) * by + Y) * bx + X] = sum;
         }
     }
@@ -1120,10 +1234,14 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
 // voxel samples the main BEV; a voxel inside a group's box also samples that group's tail.
 __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
                                                   const GroupScalars *__restrict__ gs, const float *__restrict__ bev,
-                                                  const float *__restrict__ tailBev, float *__restrict__ dose) {
-    const int x = bs->boxStart[0] + int(blockIdx.x * 32 + (threadIdx.x & 31));
-    const int y = bs->boxStart[1] + int(blockIdx.y * 8 + (threadIdx.x >> 5));
-    const int z = bs->boxStart[2] + int(blockIdx.z);
+                                                  const float *__restrict__ tailBev, float *__restrict__ dose,
+                                                  int tx, int ty, int tz) {
+    // a block is a tx x ty x tz voxel brick (256 voxels), short along the beam's depth axis so
+    // its samples stay within a few BEV planes
+    const int lx = int(threadIdx.x) % tx, ly = (int(threadIdx.x) / tx) % ty, lz = int(threadIdx.x) / (tx * ty);
+    const int x = bs->boxStart[0] + int(blockIdx.x) * tx + lx;
+    const int y = bs->boxStart[1] + int(blockIdx.y) * ty + ly;
+    const int z = bs->boxStart[2] + int(blockIdx.z) * tz + lz;
     if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1] || z >= bs->boxEnd[2]) return;
     const int inside = bs->beamFirstInside;
     const int m = bs->numGroups;
@@ -1164,11 +1282,44 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
 // ---------------------------------------------------------------------------------------
 namespace pbs {
 
+// Sub-buffer offsets of the trace scratch in floats, each rounded up to 64 floats (256 B) so
+// the float2 and double arrays are aligned whatever nray * K is.
+struct TraceScratch {
+    size_t hu, sp, shu, last, carry, total;
+};
+static TraceScratch trace_layout(const FanGeom &g) {
+    auto up = [](size_t n) { return (n + 63) / 64 * 64; };
+    const size_t nray = size_t(g.rx) * g.ry;
+    const size_t S = size_t((g.K + kTraceSeg - 1) / kTraceSeg);
+    TraceScratch t;
+    t.hu = 0;
+    t.sp = up(nray * g.K);            // huPrefix [K][nray]
+    t.shu = t.sp + up(2 * nray * S);  // segSp float2 [S][nray]
+    t.last = t.shu + up(nray * S);    // segHu [S][nray]
+    t.carry = t.last + up(nray * S);  // segLast int [S][nray]
+    t.total = t.carry + up(2 * nray * S);  // segCarry double [S][nray]
+    return t;
+}
+
+size_t trace_scratch_floats(const FanGeom &g) { return trace_layout(g).total; }
+
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
-                        int *fo, hipStream_t s) {
+                        int *fo, float *scratch, hipStream_t s) {
     const int n = g.rx * g.ry;
+    const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
+    const TraceScratch L = trace_layout(g);
+    float *huPrefix = scratch + L.hu;
+    float2 *segSp = reinterpret_cast<float2 *>(scratch + L.sp);
+    float *segHu = scratch + L.shu;
+    int *segLast = reinterpret_cast<int *>(scratch + L.last);
+    double *segCarry = reinterpret_cast<double *>(scratch + L.carry);
     const size_t shm = sizeof(float) * size_t(lut.nDensity + lut.nSp);
-    hipLaunchKernelGGL(k_trace, dim3((n + 255) / 256), dim3(256), shm, s, g, ct, lut, density, wepl, fi, fo);
+    hipLaunchKernelGGL(k_trace_seg, dim3((n + 255) / 256, S), dim3(256), shm, s, g, ct, lut, density, wepl, huPrefix,
+                       segSp, segHu, segLast);
+    hipLaunchKernelGGL(k_trace_carry, dim3((n + 255) / 256), dim3(256), 0, s, g, huPrefix, segSp, segHu, segLast,
+                       segCarry, fi, fo);
+    if (g.K > kTraceSeg)
+        hipLaunchKernelGGL(k_trace_fix, dim3((n + 255) / 256, g.K - kTraceSeg), dim3(256), 0, s, g, segCarry, wepl);
     return hipGetLastError();
 }
 
@@ -1210,8 +1361,9 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
                        int *tileCode, hipStream_t s) {
     const int nt = (g.sx / 32) * (g.sy / 8);
-    hipLaunchKernelGGL(k_fill, dim3(nt, g.numLayers), dim3(256), sizeof(float) * lut.nRrl, s, g, lut, lh, ls, bs,
-                       density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
+    const size_t lds = sizeof(float) * (size_t(lut.nRrl) + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
+    hipLaunchKernelGGL(k_fill, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl, rayW, fi,
+                       fo, iddSigma, firstPassive, fc, hf, tileCode);
     return hipGetLastError();
 }
 
@@ -1231,8 +1383,9 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
-                                     hipStream_t s) {
-    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap);
+                                     int *planeRM, hipStream_t s) {
+    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap,
+                       planeRM);
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (codesFromFill)
         hipLaunchKernelGGL(k_tile_hist, dim3(g.numLayers), dim3(256), 0, s, g, tileRad, ls, bs, hist);
@@ -1241,7 +1394,7 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
                        overflowFlag);
     hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
-                       tailItemR, classCount, classList, maxUnits);
+                       tailItemR, classCount, classList, maxUnits, planeRM);
     return hipGetLastError();
 }
 
@@ -1276,18 +1429,26 @@ hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int 
 }
 
 hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
-                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, hipStream_t s) {
+                        const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, const int *planeRM,
+                        hipStream_t s) {
     const int bx = g.sx + 64, by = g.sy + 64;
     const int tz = tailCap < 64 ? tailCap : 64;
     hipLaunchKernelGGL(k_merge, dim3((bx + 63) / 64, (by + 3) / 4, g.K + tz), dim3(256), 0, s, g, bs, itemR,
-                       mainSlots, bev, tailItemR, tailSlots, tailBev);
+                       mainSlots, bev, tailItemR, tailSlots, tailBev, planeRM);
     return hipGetLastError();
 }
 
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
-                           const float *bev, const float *tailBev, float *dose, int gridX, int gridY, int gridZ,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_transfer, dim3(gridX, gridY, gridZ), dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose);
+                           const float *bev, const float *tailBev, float *dose, int extentX, int extentY, int extentZ,
+                           int depthAxis, hipStream_t s) {
+    int t[3] = {8, 8, 8};
+    t[depthAxis] = 4;
+    const int e[3] = {extentX, extentY, extentZ};
+    dim3 grid;
+    grid.x = unsigned((e[0] + t[0] - 1) / t[0]);
+    grid.y = unsigned((e[1] + t[1] - 1) / t[1]);
+    grid.z = unsigned((e[2] + t[2] - 1) / t[2]);
+    hipLaunchKernelGGL(k_transfer, grid, dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose, t[0], t[1], t[2]);
     return hipGetLastError();
 }
 
