@@ -48,6 +48,7 @@ struct HaloFill {
     const float *nucSqSigmaLut;// [nE][nS] halo sigma^2
     int nucPlane;
     float spotDist;            // spot spacing in rays
+    float rSpotDistSq;         // 1 / spotDist^2
 };
 
 // Per-beam values produced on the device.

@@ -64,7 +64,7 @@ using namespace pbs;
 
 // Buffers of one superposition grid (tiles, work lists, item blocks, BEV planes, groups).
 struct GridWork {
-    DevBuf<int> tileRad, hist, effRad, itemR, classCount, tailPlaneGroup, tailPlaneK, tailItemR, planeRM;
+    DevBuf<int> tileRad, hist, effRad, itemR, classCount, tailPlaneGroup, tailPlaneK, tailItemR, planeRM, slotLayers;
     DevBuf<int2> classList;
     DevBuf<float> slots, tailSlots, bev, tailBev;
     DevBuf<GroupScalars> groupScal;
@@ -80,6 +80,7 @@ struct GridWork {
         const size_t bevPlane = size_t(sx + 64) * (sy + 64);
         hipError_t r;
         if ((r = tileRad.ensure(size_t(L) * K * nt)) != hipSuccess) return r;
+        if ((r = slotLayers.ensure(size_t(L) * K * nt)) != hipSuccess) return r;
         if ((r = hist.ensure(size_t(L) * (kMaxRadDev + 2))) != hipSuccess) return r;
         if ((r = effRad.ensure(size_t(L) * (kMaxRadDev + 2))) != hipSuccess) return r;
         if ((r = itemR.ensure(maxItems)) != hipSuccess) return r;
@@ -470,12 +471,12 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     HIP_TRY(launch_finalize_and_tiles(g, iddSigma, ls, w.groupScal.p, bs, w.tailPlaneGroup.p, w.tailPlaneK.p,
                                       w.tailCap, w.tileRad.p, w.hist.p, w.effRad.p, w.itemR.p, w.tailItemR.p,
                                       w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, codesFromFill ? 1 : 0,
-                                      w.planeRM.p, s));
+                                      w.planeRM.p, w.slotLayers.p, s));
     HIP_TRY(mk());
     HIP_TRY(hipEventRecord(e->forkEv, s));
     for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
     HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
-                             w.classList.p, w.maxUnits, w.slots.p, w.tailSlots.p, 4096, streams));
+                             w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, streams));
     for (int i = 0; i < kSideStreams; ++i) {
         HIP_TRY(hipEventRecord(e->joinEv[i], streams[1 + i]));
         HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
@@ -567,6 +568,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         hf.nucSqSigmaLut = e->nucS2.p;
         hf.nucPlane = int(nucPlane);
         hf.spotDist = hp.spotDist;
+        hf.rSpotDistSq = 1.0f / (hp.spotDist * hp.spotDist);
     }
     std::vector<LayerHost> &lh = e->hostLayers;
     lh.resize(L);

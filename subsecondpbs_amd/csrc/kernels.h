@@ -11,7 +11,7 @@ constexpr int kMaxRadDev = 32;
 constexpr int kMinTilesDev = 16;
 constexpr int kSlotPitch = 96;                    // 32 + 2 * 32 columns
 constexpr int kSlotFloats = kSlotPitch * 72;      // (8 + 2 * 32) rows
-constexpr int kNumClasses = 8;
+constexpr int kNumClasses = 5;
 constexpr int kMaxLayersDev = 256;
 constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
 
@@ -63,11 +63,12 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
-                                     int *planeRM, hipStream_t s);
+                                     int *planeRM, int *slotLayers, hipStream_t s);
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
-                            const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,
-                            float *tailSlots, int gridCap, const hipStream_t streams[1 + kSideStreams]);
+                            int *classCount, const int2 *classList, int maxUnits, const int *slotLayers,
+                            float *mainSlots, float *tailSlots, int gridCap,
+                            const hipStream_t streams[1 + kSideStreams]);
 hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *itemR, const float *mainSlots, float *bev,
                         const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, const int *planeRM,
                         hipStream_t s);

@@ -444,6 +444,13 @@ __device__ __forceinline__ int tile_radius(float m) {
 // ---------------------------------------------------------------------------------------
 // IDD and sigma
 // ---------------------------------------------------------------------------------------
+// Hardware transcendentals (about 1 ulp): the fill's step is VALU-issue bound, and the
+// correctly rounded library forms cost 10-80 instructions each.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
+// x^e for x > 0 as 2^(e log2 x)
+__device__ __forceinline__ float pow_hw(float x, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x)); }
+
 // Bilinear lookup in a table row pair staged in LDS: the same arithmetic as lut2d_clamp for
 // a fixed row coordinate (rows r0 = row y0, r1 = row y1, fraction ay).
 __device__ __forceinline__ float lut_rows(const float *__restrict__ r0, const float *__restrict__ r1, int nx,
@@ -461,6 +468,16 @@ __device__ __forceinline__ float lut_rows(const float *__restrict__ r0, const fl
 // One block per (32x8 tile, layer); thread per ray of the tile. Besides IDD / rSigmaEff per
 // step it reduces each step's tile: code = radius | live << 8 (the tileRadCalc of
 // complete_superposition.cu:197-255, fused so the field is not read back).
+//
+// The recurrence (idd_sigma.cu:130-230) carries only sigma^2 and its three increments from step
+// to step; WEPL, density, the cumulative dose and the stopping power are functions of the loaded
+// WEPL / density. So each step is written branch-free (selects on the ray's live flag), WEPL and
+// density are loaded a chunk of kFillChunk steps ahead, and the tile codes of a chunk are
+// written after one barrier per chunk: the compiler can then overlap the independent parts of
+// the steps of a chunk, which is what bounds the longest layer's blocks.
+constexpr int kFillChunk = 4;
+
+template <bool HALO>
 __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const LayerHost *__restrict__ lh,
                                               LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                               const float *__restrict__ density, const float *__restrict__ wepl,
@@ -468,29 +485,41 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
                                               const int *__restrict__ fo, float2 *__restrict__ iddSigma,
                                               int *__restrict__ firstPassive, FillConsts fc, HaloFill hf,
                                               int *__restrict__ tileCode) {
-    // LDS: 1/X0 LUT, then the layer's two CIDD rows, then (halo) its nuclear table rows
+    // LDS: 1/X0 LUT, per-step beam geometry (ray spacing factor, voxel mass factor), the layer's
+    // two CIDD rows, then (halo) its nuclear table rows
     extern __shared__ float s_lds[];
-    __shared__ float s_min[2][4];
-    __shared__ int s_live[2][4];
+    __shared__ float s_min[2][kFillChunk][4];
+    __shared__ int s_live[2][kFillChunk][4];
     const int ntx = g.sx / 32, nt = ntx * (g.sy / 8);
     const int t = blockIdx.x, l = blockIdx.y;
     const float eIdx = lh[l].energyIdx;
     const float eFl = floorf(eIdx);
     const float ay = eIdx - eFl;
     const int ey0 = min(max(int(eFl), 0), lut.nE - 1), ey1 = min(max(int(eFl) + 1, 0), lut.nE - 1);
+    const int nS = lut.nS, nRrl = lut.nRrl;
     float *s_rrl = s_lds;
-    float *c0 = s_lds + lut.nRrl, *c1 = c0 + lut.nS;
-    float *nw0 = c1 + lut.nS, *nw1 = nw0 + lut.nS, *ns0 = nw1 + lut.nS, *ns1 = ns0 + lut.nS;
-    for (int i = threadIdx.x; i < lut.nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
-    for (int i = threadIdx.x; i < lut.nS; i += blockDim.x) {
-        c0[i] = lut.cidd[size_t(ey0) * lut.nS + i];
-        c1[i] = lut.cidd[size_t(ey1) * lut.nS + i];
-        if (hf.rayToNuc) {
-            nw0[i] = hf.nucWeightLut[size_t(ey0) * lut.nS + i];
-            nw1[i] = hf.nucWeightLut[size_t(ey1) * lut.nS + i];
-            ns0[i] = hf.nucSqSigmaLut[size_t(ey0) * lut.nS + i];
-            ns1[i] = hf.nucSqSigmaLut[size_t(ey1) * lut.nS + i];
+    float *s_vw = s_rrl + nRrl, *s_mp = s_vw + g.K;
+    float *c0 = s_mp + g.K, *c1 = c0 + nS;
+    float *nw0 = c1 + nS, *nw1 = nw0 + nS, *ns0 = nw1 + nS, *ns1 = ns0 + nS;
+    for (int i = threadIdx.x; i < nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
+    for (int i = threadIdx.x; i < nS; i += blockDim.x) {
+        c0[i] = lut.cidd[size_t(ey0) * nS + i];
+        c1[i] = lut.cidd[size_t(ey1) * nS + i];
+        if (HALO) {
+            nw0[i] = hf.nucWeightLut[size_t(ey0) * nS + i];
+            nw1[i] = hf.nucWeightLut[size_t(ey1) * nS + i];
+            ns0[i] = hf.nucSqSigmaLut[size_t(ey0) * nS + i];
+            ns1[i] = hf.nucSqSigmaLut[size_t(ey1) * nS + i];
         }
+    }
+    for (int k = threadIdx.x; k < g.K; k += blockDim.x) {
+        // half the summed fan ray spacings at the step's depth, and the voxel volume at step k
+        const float zk = g.fc[2] + float(k) * g.fd[2];
+        const float vwx = g.fd[0] * (1.0f - zk / g.dist[0]);
+        const float vwy = g.fd[1] * (1.0f - zk / g.dist[1]);
+        s_vw[k] = 0.5f * (vwx + vwy);
+        const float fk = float(k);
+        s_mp[k] = g.volConst + fk * g.volLin + fk * fk * g.volSq;
     }
     const int x = (t % ntx) * 32 + (threadIdx.x & 31), y = (t / ntx) * 8 + (threadIdx.x >> 5);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -505,21 +534,21 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
     const float w0 = rayIn ? rayW[size_t(l) * nray + ray] : 0.0f;
     bool live = rayIn;
     if (w0 < 1e-6f || afterLast < first) { live = false; afterLast = 0; }
-    float cumulSp = 0.f, cumulSpOld = 0.f, cumulDose = 0.f, cumulDoseOld = 0.f;
+    float spOld = 0.f, cdOld = 0.f;
     float incScat = 0.f, incincScat = 0.f;
     float incDiv = L.airLin + (2.0f * float(first) - 1.0f) * L.airQuad;
     float sigmaSq = -incDiv;
     const float stepLength = fabsf(g.fd[2]);
+    const float scatStep = 2.0f * stepLength * stepLength;
     const size_t plane = size_t(nray);
     float2 *out = iddSigma + (size_t(l) * g.K * g.sy + y) * g.sx + x;
     const size_t outStep = size_t(g.sx) * g.sy;
     int *codes = tileCode + size_t(l) * g.K * nt + t;
     // halo cell of this ray (the spot it centres) and the halo entry sigma^2 (wrapper:4324-4329)
-    const int cell = (hf.rayToNuc && rayIn) ? hf.rayToNuc[ray] : -1;
-    const bool halo = hf.rayToNuc != nullptr;
+    const int cell = (HALO && rayIn) ? hf.rayToNuc[ray] : -1;
     float nucRayW = 0.0f, entrySigmaSq = 0.0f;
     float2 *nout = nullptr;
-    if (cell >= 0) {
+    if (HALO && cell >= 0) {
         nucRayW = hf.weights[size_t(l) * hf.nucPlane + cell];
         nout = hf.out + size_t(l) * g.K * hf.nucPlane + cell;
         {
@@ -529,80 +558,137 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
         }
     }
     __syncthreads();
-    // the next step's WEPL and density are loaded one step ahead (they do not depend on the
-    // recurrence), so the memory latency overlaps the step's arithmetic
-    float spNext = 0.0f, rhoNext = 0.0f;
-    if (first < pAfter) {
-        spNext = wepl[size_t(first) * plane + ray];
-        rhoNext = density[size_t(first) * plane + ray];
-    }
-    for (unsigned k = first; k < pAfter; ++k) {
-        float res = 0.0f, rs = PBS_INF, nres = 0.0f, nrs = PBS_INF;
-        const size_t idx = size_t(k) * plane + ray;
-        const float spNow = spNext;
-        const float rho = rhoNext;
-        if (k + 1 < pAfter) {
-            spNext = wepl[idx + plane];
-            rhoNext = density[idx + plane];
+
+    // one step of the recurrence for this ray (idd_sigma.cu:130-230); outputs are zero / inf
+    // once the ray has stopped and before its first inside step
+    auto step = [&](unsigned k, float sp, float rho, float &res, float &rs, float &nres, float &nrs) {
+        const float cd = lut_rows(c0, c1, nS, ay, sp * L.scaleFact);
+        const float mid = 0.5f * (sp + spOld);
+        const bool pre = sp < L.peakDepth;
+        const float resE = 8.639415f * pow_hw(L.peakDepth - mid, 0.5649718f);
+        const float betaP = resE + 938.3f - 938.3f * 938.3f * rcp_hw(resE + 938.3f);
+        const float rRl = rho * lut1d_clamp(s_rrl, nRrl, rho * lut.rrlScale);
+        const float thetaSq = fc.eRefSq * rcp_hw(betaP * betaP) * stepLength * rRl;
+        const float incSum = incScat + incDiv;
+        const float iis = incincScat + thetaSq * scatStep;
+        const float sgPost = fc.suppressDistalDip ? sigmaSq : sigmaSq - 1.5f * incSum * rho;
+        const bool adv = live && pre;
+        sigmaSq = live ? (pre ? sigmaSq + incSum : sgPost) : sigmaSq;
+        incincScat = adv ? iis : incincScat;
+        incScat = adv ? incScat + iis : incScat;
+        incDiv = adv ? incDiv + 2.0f * L.airQuad : incDiv;
+        const float vw = s_vw[k];
+        const float rsv = vw * rcp_hw(1.41421356f * (sqrt_hw(sigmaSq) + fc.sigmaDelta));
+        const float mass = rho * s_mp[k];
+        const float rMass = rcp_hw(mass);
+        const float dd = cd - cdOld;
+        const bool heavy = mass > 1e-2f;
+        float r, nr = 0.0f, nrsv = PBS_INF;
+        if (HALO) {
+            // idd_sigma.cu:194-213: the halo takes w_nuc of the step's energy
+            const float depthMid = mid * L.scaleFact;
+            const float nw = lut_rows(nw0, nw1, nS, ay, depthMid);
+            r = (1.0f - nw) * w0 * dd * rMass;
+            nr = nw * nucRayW * dd * rMass * hf.rSpotDistSq;
+            const float s2 = lut_rows(ns0, ns1, nS, ay, depthMid);
+            nrsv = hf.spotDist * vw * rcp_hw(1.41421356f * sqrt_hw(sigmaSq + s2 + entrySigmaSq));
+            nr = (heavy && cell >= 0) ? nr : 0.0f;
+            nrsv = cell >= 0 ? nrsv : PBS_INF;
+        } else {
+            r = w0 * dd * rMass;
         }
-        if (live) {
-            cumulSp = spNow;
-            cumulDose = lut_rows(c0, c1, lut.nS, ay, cumulSp * L.scaleFact);
-            if (cumulSp < L.peakDepth) {
-                const float resE = 8.639415f * powf(L.peakDepth - 0.5f * (cumulSp + cumulSpOld), 0.5649718f);
-                const float betaP = resE + 938.3f - 938.3f * 938.3f / (resE + 938.3f);
-                const float rRl = rho * lut1d_clamp(s_rrl, lut.nRrl, rho * lut.rrlScale);
-                const float thetaSq = fc.eRefSq / (betaP * betaP) * stepLength * rRl;
-                sigmaSq += incScat + incDiv;
-                incincScat += 2.0f * thetaSq * stepLength * stepLength;
-                incScat += incincScat;
-                incDiv += 2.0f * L.airQuad;
-            } else if (!fc.suppressDistalDip) {
-                sigmaSq -= 1.5f * (incScat + incDiv) * rho;
-            }
-            const float zk = g.fc[2] + float(k) * g.fd[2];
-            const float vwx = g.fd[0] * (1.0f - zk / g.dist[0]);
-            const float vwy = g.fd[1] * (1.0f - zk / g.dist[1]);
-            rs = 0.5f * (vwx + vwy) / (1.41421356f * (sqrtf(sigmaSq) + fc.sigmaDelta));
-            const float fk = float(k);
-            const float mass = rho * (g.volConst + fk * g.volLin + fk * fk * g.volSq);
-            const float depthMid = 0.5f * (cumulSp + cumulSpOld) * L.scaleFact;
-            if (mass > 1e-2f) {
-                if (halo) {
-                    // idd_sigma.cu:194-213: the halo takes w_nuc of the step's energy
-                    const float nw = lut_rows(nw0, nw1, lut.nS, ay, depthMid);
-                    res = (1.0f - nw) * w0 * (cumulDose - cumulDoseOld) / mass;
-                    if (cell >= 0)
-                        nres = nw * nucRayW * (cumulDose - cumulDoseOld) / (mass * hf.spotDist * hf.spotDist);
-                } else {
-                    res = w0 * (cumulDose - cumulDoseOld) / mass;
-                }
-            }
-            if (cell >= 0) {
-                const float s2 = lut_rows(ns0, ns1, lut.nS, ay, depthMid);
-                nrs = 0.5f * hf.spotDist * (vwx + vwy) / (1.41421356f * sqrtf(sigmaSq + s2 + entrySigmaSq));
-            }
-            cumulSpOld = cumulSp;
-            cumulDoseOld = cumulDose;
-            if (cumulSp > L.rangeStop || k == afterLast) { live = false; afterLast = k; }
-        }
-        if (!live || int(k) < int(firstIn) - 1) { res = 0.0f; rs = PBS_INF; nres = 0.0f; nrs = PBS_INF; }
+        r = heavy ? r : 0.0f;
+        spOld = sp;
+        cdOld = cd;
+        const bool stop = sp > L.rangeStop || k == afterLast;
+        afterLast = (live && stop) ? k : afterLast;
+        live = live && !stop;
+        const bool zero = !live || int(k) < int(firstIn) - 1;
+        res = zero ? 0.0f : r;
+        rs = zero ? PBS_INF : rsv;
+        nres = zero ? 0.0f : nr;
+        nrs = zero ? PBS_INF : nrsv;
+    };
+    // store a step and stage its per-wave tile reduction: minimum rSigmaEff (NaN ignored,
+    // fminf) and whether any pixel is live
+    auto emit = [&](unsigned k, int u, int cb, float res, float rs, float nres, float nrs) {
         out[size_t(k) * outStep] = make_float2(res, rs);
-        if (nout) nout[size_t(k) * hf.nucPlane] = make_float2(nres, nrs);
-        // the step's tile code: minimum rSigmaEff (NaN ignored, fminf) and any live pixel
+        if (HALO && cell >= 0) nout[size_t(k) * hf.nucPlane] = make_float2(nres, nrs);
         const float m = wave_min(rs);
         const bool any = __any(res > 0.0f);
-        const int buf = int(k & 1u);
         if (lane == 0) {
-            s_min[buf][w] = m;
-            s_live[buf][w] = any ? 1 : 0;
+            s_min[cb][u][w] = m;
+            s_live[cb][u][w] = any ? 1 : 0;
         }
+    };
+    // tile codes of the n steps of a chunk from the four waves' reductions
+    auto flush = [&](unsigned kc, int n, int cb) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const float mm = fminf(fminf(s_min[buf][0], s_min[buf][1]), fminf(s_min[buf][2], s_min[buf][3]));
-            const int lv = s_live[buf][0] | s_live[buf][1] | s_live[buf][2] | s_live[buf][3];
-            codes[size_t(k) * nt] = tile_radius(mm) | (lv ? 256 : 0);
+        if (int(threadIdx.x) < n) {
+            const int u = threadIdx.x;
+            const float mm = fminf(fminf(s_min[cb][u][0], s_min[cb][u][1]), fminf(s_min[cb][u][2], s_min[cb][u][3]));
+            const int lv = s_live[cb][u][0] | s_live[cb][u][1] | s_live[cb][u][2] | s_live[cb][u][3];
+            codes[size_t(kc + u) * nt] = tile_radius(mm) | (lv ? 256 : 0);
         }
+    };
+
+    // WEPL and density of the next chunk are loaded while the current one is computed; loads
+    // past the layer's last step are clamped to it (their values are never used)
+    const unsigned kLast = pAfter > 0 ? pAfter - 1 : 0;
+    float spBuf[kFillChunk], rhoBuf[kFillChunk];
+#pragma unroll
+    for (int u = 0; u < kFillChunk; ++u) {
+        const size_t k = min(first + unsigned(u), kLast);
+        spBuf[u] = wepl[k * plane + ray];
+        rhoBuf[u] = density[k * plane + ray];
+    }
+    unsigned kc = first;
+    int cb = 0;
+    for (; kc + kFillChunk <= pAfter; kc += kFillChunk, cb ^= 1) {
+        float spNx[kFillChunk], rhoNx[kFillChunk];
+#pragma unroll
+        for (int u = 0; u < kFillChunk; ++u) {
+            const size_t k = min(kc + kFillChunk + unsigned(u), kLast);
+            spNx[u] = wepl[k * plane + ray];
+            rhoNx[u] = density[k * plane + ray];
+        }
+        if (__any(live)) {
+#pragma unroll
+            for (int u = 0; u < kFillChunk; ++u) {
+                float res, rs, nres, nrs;
+                step(kc + u, spBuf[u], rhoBuf[u], res, rs, nres, nrs);
+                emit(kc + u, u, cb, res, rs, nres, nrs);
+            }
+        } else {
+            // every ray of the wave has stopped: zero outputs, no live pixel
+#pragma unroll
+            for (int u = 0; u < kFillChunk; ++u) {
+                out[size_t(kc + u) * outStep] = make_float2(0.0f, PBS_INF);
+                if (HALO && cell >= 0) nout[size_t(kc + u) * hf.nucPlane] = make_float2(0.0f, PBS_INF);
+                if (lane == 0) {
+                    s_min[cb][u][w] = PBS_INF;
+                    s_live[cb][u][w] = 0;
+                }
+            }
+        }
+        flush(kc, kFillChunk, cb);
+#pragma unroll
+        for (int u = 0; u < kFillChunk; ++u) {
+            spBuf[u] = spNx[u];
+            rhoBuf[u] = rhoNx[u];
+        }
+    }
+    const int rem = int(pAfter) - int(kc);
+    if (rem > 0) {
+#pragma unroll
+        for (int u = 0; u < kFillChunk; ++u) {
+            if (u < rem) {
+                float res, rs, nres, nrs;
+                step(kc + u, spBuf[u], rhoBuf[u], res, rs, nres, nrs);
+                emit(kc + u, u, cb, res, rs, nres, nrs);
+            }
+        }
+        flush(kc, rem, cb);
     }
     int passive = INT_MIN;
     if (rayIn) {
@@ -882,11 +968,12 @@ __global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restr
 }
 
 __device__ __forceinline__ int radius_class(int R) {
-    // launch classes: accumulator layouts of radius 2, 4, 6, 8, 12, 16, 24, 32
-    return R <= 2 ? 0 : R <= 4 ? 1 : R <= 6 ? 2 : R <= 8 ? 3 : R <= 12 ? 4 : R <= 16 ? 5 : R <= 24 ? 6 : 7;
+    // launch classes: accumulator layouts of radius 7, 11, 15 (2RM + 1 lanes fill a group of
+    // 16, 32, 32), then 24, 32
+    return R <= 7 ? 0 : R <= 11 ? 1 : R <= 15 ? 2 : R <= 24 ? 3 : 4;
 }
 __device__ __forceinline__ int class_radius(int c) {
-    constexpr int rm[8] = {2, 4, 6, 8, 12, 16, 24, 32};
+    constexpr int rm[kNumClasses] = {7, 11, 15, 24, 32};
     return rm[c];
 }
 
@@ -923,7 +1010,8 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
                                                const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
                                                int *__restrict__ itemR, int *__restrict__ tailItemR,
                                                int *__restrict__ classCount, int2 *__restrict__ classList,
-                                               int maxUnits, int *__restrict__ planeRM) {
+                                               int maxUnits, int *__restrict__ planeRM,
+                                               int *__restrict__ slotLayers) {
     const int nt = (g.sx / 32) * (g.sy / 8);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
@@ -932,6 +1020,11 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
     bool mainLive = false;
     uint64_t tailLive[kMaxLayersDev / 64] = {0, 0, 0, 0};  // groups with a live tail layer here
     const int inside = bs->beamFirstInside;
+    const int slot = k * nt + t;
+    // the slot's live layers in layer order, each l | R << 8 | (block + 1) << 16 (block -1 =
+    // main BEV, j = layer group j's tail), terminated by -1 when fewer than numLayers
+    int *list = slotLayers + size_t(slot) * g.numLayers;
+    int n = 0;
     for (int l = 0; l < g.numLayers; ++l) {
         if (k < inside || k >= ls[l].passive) continue;  // steps the layer does not superpose
         const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
@@ -941,11 +1034,13 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
         if (!isMain && !(j >= 0 && in_tail(gs[j], k))) continue;
         mainLive |= isMain;
         if (!isMain) tailLive[j >> 6] |= uint64_t(1) << (j & 63);
-        Rmax = max(Rmax, effRad[l * (kMaxRadDev + 2) + (code & 255)]);
+        const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
+        Rmax = max(Rmax, R);
+        list[n++] = l | (R << 8) | ((isMain ? 0 : j + 1) << 16);
     }
+    if (n < g.numLayers) list[n] = -1;
     const int c = Rmax >= 0 ? radius_class(Rmax) : 0;
     const int RM = Rmax >= 0 ? class_radius(c) : -1;
-    const int slot = k * nt + t;
     itemR[slot] = mainLive ? RM : -1;
     if (mainLive) {
         append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
@@ -1171,6 +1266,241 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Superposition with lanes on output rows (radius classes up to 16)
+// ---------------------------------------------------------------------------------------
+// erf(x) for x >= 0 by Abramowitz & Stegun 7.1.26 (Hastings): |error| <= 1.5e-7.
+__device__ __forceinline__ float erf_as(float x) {
+    const float t = rcp_hw(1.0f + 0.3275911f * x);
+    const float p =
+        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    return 1.0f - p * __builtin_amdgcn_exp2f(-1.44269504f * (x * x));
+}
+
+// Work-group barrier for LDS hand-offs only: waits for this wave's LDS accesses, not for its
+// global loads, so loads issued ahead (the next layer's pixels) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt left at their maximum
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// Launch shape of the row-lane superposition of class RM: a group of G >= 2RM + 1 lanes holds
+// the output rows y_s - RM .. y_s + RM of one source row y_s (lane l: vertical offset
+// l - RM); eight groups (one per source row of the tile) make a work-group of 8G threads. Each
+// source's table row is [d, e_0 .. e_R, 0 ...] (EW floats).
+template <int RM>
+struct RowsShape {
+    static constexpr int BH = 8 + 2 * RM, BW = 32 + 2 * RM;
+    static constexpr int G = 2 * RM + 1 <= 8 ? 8 : (2 * RM + 1 <= 16 ? 16 : (2 * RM + 1 <= 32 ? 32 : 64));
+    static constexpr int NT = 8 * G;
+    static constexpr int EW = pad4(RM + 3);
+    static constexpr int RP = 32 * EW + 4;  // source-row pitch: +4 floats puts the groups of a wave on other banks
+    static constexpr int BHp = BH | 1;      // block column pitch (odd: conflict-free both ways)
+    static constexpr int EBUF = 8 * RP;     // one layer's table
+    static constexpr int LDS = (2 * EBUF > BW * BHp ? 2 * EBUF : BW * BHp);
+    static constexpr int SPT = (256 + NT - 1) / NT;      // sources staged per thread
+    static constexpr int RLO = RM - 7 > 0 ? RM - 7 : 0;  // smallest compiled sweep radius
+};
+
+// One source row of one layer into this lane's output row: for every live source x_s (wave-
+// uniform skip), a = d e_v with v = |dy| clamped to R+1 (a zero entry), then 2R+1 FMAs along
+// the row, acc[x_s + dx + RM] += a e_|dx|. RP_ is the radius compiled; entries past the
+// layer's own radius are zero, so a smaller layer radius adds exact zeros.
+template <int RP_, int RM, int EW>
+__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v,
+                                           uint32_t wm) {
+    constexpr int NE4 = (RP_ + 2 + 3) / 4;
+#pragma unroll
+    for (int xs = 0; xs < 32; ++xs) {
+        if (!((wm >> xs) & 1u)) continue;
+        const float *E = rowE + xs * EW;
+        float e[4 * NE4];
+#pragma unroll
+        for (int q = 0; q < NE4; ++q) {
+            const float4 t4 = *reinterpret_cast<const float4 *>(E + 4 * q);
+            e[4 * q] = t4.x; e[4 * q + 1] = t4.y; e[4 * q + 2] = t4.z; e[4 * q + 3] = t4.w;
+        }
+        const float a = e[0] * E[1 + v];
+#pragma unroll
+        for (int dx = -RP_; dx <= RP_; ++dx) {
+            const int c = xs + dx + RM;
+            acc[c] = fmaf(a, e[1 + (dx < 0 ? -dx : dx)], acc[c]);
+        }
+    }
+}
+
+template <int RP_, int RM, int EW>
+__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v,
+                                              uint32_t wm) {
+    if constexpr (RP_ <= RM) {
+        if (rp == RP_)
+            rows_sweep<RP_, RM, EW>(acc, rowE, v, wm);
+        else
+            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v, wm);
+    }
+}
+
+// Work-group per work unit (step, tile, block) of class RM (complete_superposition.cu:280-330:
+// e_0 = erf(r/2), e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product d e_x e_y). Lane l of
+// group g owns block row g + l (tile row g + l - RM) for source row g and keeps the 32 + 2RM
+// columns of that row in registers across all layers of the unit. Per layer the work-group
+// stages the tile's 256 table rows once (double-buffered, one barrier per layer; the next
+// layer's pixels are loaded during the current sweep); every source is then applied by all
+// lanes of its group at once, so the horizontal weights are shared reads and the register
+// index is static. The groups are added into the block in source-row order: bitwise
+// repeatable.
+template <int RM>
+__global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
+    FanGeom g, const float2 *__restrict__ iddSigma, const int *__restrict__ tileRad, const int *__restrict__ effRad,
+    const LayerScalars *__restrict__ ls, const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
+    const int *__restrict__ classCount, const int2 *__restrict__ classList, int cls, int maxUnits,
+    const int *__restrict__ slotLayers, float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
+    using S = RowsShape<RM>;
+    constexpr int BH = S::BH, BW = S::BW, G = S::G, NT = S::NT, EW = S::EW, RP = S::RP, BHp = S::BHp, SPT = S::SPT;
+    __shared__ __align__(16) float s_lds[S::LDS];
+    __shared__ uint32_t s_live[2][8];
+    __shared__ int s_list[kMaxLayersDev];  // the unit's layers: l | R << 16
+    __shared__ int s_n;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int grp = tid / G, ql = tid % G;  // source row, output row offset + RM
+    const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
+    const int count = min(classCount[cls], maxUnits);
+    const int2 *list = classList + size_t(cls) * maxUnits;
+    const size_t planeXY = size_t(g.sy) * g.sx;
+    const int ady = ql > RM ? ql - RM : RM - ql;
+    for (int u = blockIdx.x; u < count; u += gridDim.x) {
+        const int2 unit = list[u];
+        const int slot = unit.x, b = unit.y;
+        const int k = slot / nt, t = slot % nt;
+        const int tx = t % ntx, ty = t / ntx;
+        // this block's layers in order, from the slot's list (wave 0, lane = list entry)
+        if (w == 0) {
+            const int *sl = slotLayers + size_t(slot) * g.numLayers;
+            int base = 0;
+            for (int j0 = 0; j0 < g.numLayers; j0 += 64) {
+                const int e = j0 + lane < g.numLayers ? sl[j0 + lane] : -1;
+                // entries after the list's terminator are not written for this beam: only the
+                // lanes before the first negative entry hold list entries
+                const uint64_t ends = __ballot(e < 0);
+                const int endLane = ends ? __builtin_ctzll(ends) : 64;
+                const bool in = lane < endLane && (e >> 16) == b + 1;
+                const uint64_t m = __ballot(in);
+                if (in) s_list[base + __popcll(m & ((uint64_t(1) << lane) - 1))] = (e & 0xff) | (((e >> 8) & 0xff) << 16);
+                base += __popcll(m);
+                if (ends) break;  // end of the list
+            }
+            if (lane == 0) s_n = base;
+        }
+        // sources this thread stages: src = tid + i NT
+        size_t pixOff[SPT];
+        bool pixIn[SPT];
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int src = tid + i * NT;
+            const int gx = tx * 32 + (src & 31), gy = ty * 8 + (src >> 5);
+            pixIn[i] = src < 256 && gx < g.rx && gy < g.ry;
+            pixOff[i] = size_t(gy) * g.sx + gx;
+        }
+        __syncthreads();
+        const int n = s_n;
+        float acc[BW];
+#pragma unroll
+        for (int c = 0; c < BW; ++c) acc[c] = 0.0f;
+        // first layer's pixels
+        float2 pre[SPT];
+        {
+            const int l0 = n > 0 ? (s_list[0] & 0xffff) : 0;
+            const float2 *plane = iddSigma + (size_t(l0) * g.K + k) * planeXY;
+#pragma unroll
+            for (int i = 0; i < SPT; ++i) pre[i] = (n > 0 && pixIn[i]) ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
+        }
+        for (int li = 0; li < n; ++li) {
+            const int R = s_list[li] >> 16;
+            const int buf = li & 1;
+            float2 cur[SPT];
+#pragma unroll
+            for (int i = 0; i < SPT; ++i) cur[i] = pre[i];
+            if (li + 1 < n) {
+                const int l1 = s_list[li + 1] & 0xffff;
+                const float2 *plane = iddSigma + (size_t(l1) * g.K + k) * planeXY;
+#pragma unroll
+                for (int i = 0; i < SPT; ++i) pre[i] = pixIn[i] ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
+            }
+            float *E = s_lds + buf * S::EBUF;
+#pragma unroll
+            for (int i = 0; i < SPT; ++i) {
+                const int src = tid + i * NT;
+                if (src >= 256) break;  // uniform per wave: NT and 256 are multiples of 64
+                const float d = cur[i].x, r = cur[i].y;
+                const uint64_t live = __ballot(d != 0.0f);
+                if (lane == 0) {
+                    s_live[buf][src >> 5] = uint32_t(live);
+                    s_live[buf][(src >> 5) + 1] = uint32_t(live >> 32);
+                }
+                float row[EW];
+                row[0] = d;
+                float en = erf_as(r * 0.5f), eo = -en;
+#pragma unroll
+                for (int m = 0; m < EW - 1; ++m) {
+                    float e = 0.0f;
+                    if (m <= R) {
+                        e = 0.5f * (en - eo);
+                        eo = en;
+                        if (m < R) en = erf_as(r * (float(m) + 1.5f));
+                    }
+                    row[1 + m] = e;
+                }
+                float *dst = E + (src >> 5) * RP + (src & 31) * EW;
+#pragma unroll
+                for (int m = 0; m < EW / 4; ++m)
+                    *reinterpret_cast<float4 *>(dst + 4 * m) =
+                        make_float4(row[4 * m], row[4 * m + 1], row[4 * m + 2], row[4 * m + 3]);
+            }
+            lds_barrier();
+            // skip a source only when no group of the wave has it live
+            uint32_t wmv = 0;
+#pragma unroll
+            for (int h = 0; h < 64 / G; ++h) wmv |= s_live[buf][(w * 64) / G + h];
+            const uint32_t wm = __builtin_amdgcn_readfirstlane(wmv);
+            if (wm) {
+                const int v = ady < R + 1 ? ady : R + 1;
+                const int rp = R > S::RLO ? R : S::RLO;
+                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v, wm);
+            }
+        }
+        __syncthreads();  // tables dead: the block reuses their LDS
+        float *s_B = s_lds;
+        for (int i = tid; i < BW * BHp; i += NT) s_B[i] = 0.0f;
+        __syncthreads();
+        // groups added in source-row order: within a wave by lane ranges, across waves by barriers
+        for (int ww = 0; ww < NT / 64; ++ww) {
+            if (w == ww) {
+#pragma unroll
+                for (int h = 0; h < 64 / G; ++h) {
+                    if (lane / G == h && ql <= 2 * RM) {
+                        const int row = grp + ql;
+#pragma unroll
+                        for (int c = 0; c < BW; ++c) s_B[c * BHp + row] += acc[c];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        float *out;
+        if (b < 0) {
+            out = mainSlots + size_t(slot) * kSlotFloats;
+        } else {
+            const int qq = gs[b].tailBase + (k - gs[b].tailStart);
+            out = (qq < bs->tailPlanes) ? tailSlots + (size_t(qq) * nt + t) * kSlotFloats : nullptr;
+        }
+        if (out)
+            for (int i = tid; i < BH * BW; i += NT) out[(i / BW) * kSlotPitch + (i % BW)] = s_B[(i % BW) * BHp + i / BW];
+        __syncthreads();
+    }
+}
+
 // Sum of the item blocks covering BEV pixel (X, Y) of one plane, in (ty, tx) order.
 // RMmax: the largest block layout radius of the plane, which bounds the tiles that can reach
 // the pixel (a tile's block spans 32 + 2RM columns and 8 + 2RM rows around it).
@@ -1361,9 +1691,13 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
                        int *tileCode, hipStream_t s) {
     const int nt = (g.sx / 32) * (g.sy / 8);
-    const size_t lds = sizeof(float) * (size_t(lut.nRrl) + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
-    hipLaunchKernelGGL(k_fill, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl, rayW, fi,
-                       fo, iddSigma, firstPassive, fc, hf, tileCode);
+    const size_t lds = sizeof(float) * (size_t(lut.nRrl) + 2 * size_t(g.K) + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
+    if (hf.rayToNuc)
+        hipLaunchKernelGGL(k_fill<true>, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl,
+                           rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
+    else
+        hipLaunchKernelGGL(k_fill<false>, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl,
+                           rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
     return hipGetLastError();
 }
 
@@ -1383,7 +1717,7 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
-                                     int *planeRM, hipStream_t s) {
+                                     int *planeRM, int *slotLayers, hipStream_t s) {
     hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap,
                        planeRM);
     const int nt = (g.sx / 32) * (g.sy / 8);
@@ -1394,36 +1728,39 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
                        overflowFlag);
     hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
-                       tailItemR, classCount, classList, maxUnits, planeRM);
+                       tailItemR, classCount, classList, maxUnits, planeRM, slotLayers);
     return hipGetLastError();
 }
 
 template <int RM>
 static void launch_class(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
-                         const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs, const int *classCount,
-                         const int2 *classList, int cls, int maxUnits, float *mainSlots, float *tailSlots, int grid,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, gs, bs,
-                       classCount, classList, cls, maxUnits, mainSlots, tailSlots);
+                         const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs, int *classCount,
+                         const int2 *classList, int cls, int maxUnits, const int *slotLayers, float *mainSlots,
+                         float *tailSlots, int grid, hipStream_t s) {
+    if constexpr (RM <= 15)
+        hipLaunchKernelGGL(k_superpose_rows<RM>, dim3(grid), dim3(RowsShape<RM>::NT), 0, s, g, iddSigma, tileRad,
+                           effRad, ls, gs, bs, classCount, classList, cls, maxUnits, slotLayers, mainSlots,
+                           tailSlots);
+    else
+        hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, gs, bs,
+                           classCount, classList, cls, maxUnits, mainSlots, tailSlots);
 }
 
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
                             const LayerScalars *ls, const GroupScalars *gs, const BeamScalars *bs,
-                            const int *classCount, const int2 *classList, int maxUnits, float *mainSlots,
-                            float *tailSlots, int gridCap, const hipStream_t streams[1 + kSideStreams]) {
+                            int *classCount, const int2 *classList, int maxUnits, const int *slotLayers,
+                            float *mainSlots, float *tailSlots, int gridCap,
+                            const hipStream_t streams[1 + kSideStreams]) {
     const int grid = maxUnits < gridCap ? maxUnits : gridCap;
     // the heavy middle classes on separate streams; light ones share
 #define PBS_CLASS(RMV, C, S)                                                                                       \
-    launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxUnits, mainSlots,    \
-                      tailSlots, grid, streams[S])
-    PBS_CLASS(12, 4, 0);
-    PBS_CLASS(16, 5, 1);
-    PBS_CLASS(8, 3, 2);
-    PBS_CLASS(6, 2, 3);
-    PBS_CLASS(4, 1, 1);
-    PBS_CLASS(2, 0, 2);
-    PBS_CLASS(24, 6, 3);
-    PBS_CLASS(32, 7, 3);
+    launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxUnits, slotLayers,   \
+                      mainSlots, tailSlots, grid, streams[S])
+    PBS_CLASS(11, 1, 0);
+    PBS_CLASS(7, 0, 1);
+    PBS_CLASS(15, 2, 2);
+    PBS_CLASS(24, 3, 3);
+    PBS_CLASS(32, 4, 3);
 #undef PBS_CLASS
     return hipGetLastError();
 }
