@@ -1302,6 +1302,7 @@ struct RowsShape {
     static constexpr int LDS = (2 * EBUF > BW * BHp ? 2 * EBUF : BW * BHp);
     static constexpr int SPT = (256 + NT - 1) / NT;      // sources staged per thread
     static constexpr int RLO = RM - 7 > 0 ? RM - 7 : 0;  // smallest compiled sweep radius
+    static_assert(2 * RM + 1 <= G - 1, "the last lane of a group must stay unused (pair combine)");
 };
 
 // One source row of one layer into this lane's output row: for every live source x_s (wave-
@@ -1474,13 +1475,17 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
         float *s_B = s_lds;
         for (int i = tid; i < BW * BHp; i += NT) s_B[i] = 0.0f;
         __syncthreads();
-        // groups added in source-row order: within a wave by lane ranges, across waves by barriers
+        // pairs of source-row groups first: the odd group's lane ql - 1 holds the even group's row
+        // ql (lane G - 1 of every group is beyond 2RM and holds zeros)
+#pragma unroll
+        for (int c = 0; c < BW; ++c) acc[c] += __shfl(acc[c], lane + G - 1, 64);
+        // then the pairs in source-row order: within a wave by lane ranges, across waves by barriers
         for (int ww = 0; ww < NT / 64; ++ww) {
             if (w == ww) {
 #pragma unroll
-                for (int h = 0; h < 64 / G; ++h) {
-                    if (lane / G == h && ql <= 2 * RM) {
-                        const int row = grp + ql;
+                for (int h = 0; h < 64 / G; h += 2) {
+                    const int row = grp + ql;
+                    if (lane / G == h && row < BH) {
 #pragma unroll
                         for (int c = 0; c < BW; ++c) s_B[c * BHp + row] += acc[c];
                     }
@@ -1501,16 +1506,23 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
     }
 }
 
-// Sum of the item blocks covering BEV pixel (X, Y) of one plane, in (ty, tx) order.
-// RMmax: the largest block layout radius of the plane, which bounds the tiles that can reach
-// the pixel (a tile's block spans 32 + 2RM columns and 8 + 2RM rows around it).
-__device__ __forceinline__ float gather_blocks(const FanGeom &g, int X, int Y, const int *__restrict__ layoutR,
-                                               const float *__restrict__ slots, int RMmax) {
+// Floor division by a positive divisor.
+__device__ __forceinline__ int floor_div(int a, int d) { return (a >= 0 ? a : a - d + 1) / d; }
+
+// Sum of the item blocks covering the BEV pixels of one 32x8 output tile of a plane, in
+// (ty, tx) order. The work-group's output tile (otx, oty) sits on the item tiling shifted by
+// the plane's 32-pixel margin, so the candidate item tiles are the same for all its pixels
+// (RMmax: the plane's largest block layout radius, a tile's block spans 32 + 2RM columns and
+// 8 + 2RM rows around it); each lane then adds the one element of each candidate block that
+// covers its pixel.
+__device__ __forceinline__ float gather_tile(const FanGeom &g, int otx, int oty, int lx, int ly,
+                                             const int *__restrict__ layoutR, const float *__restrict__ slots,
+                                             int RMmax) {
     const int ntx = g.sx / 32, nty = g.sy / 8;
-    const int xr = X - 32, yr = Y - 32;
-    // floor divisions of possibly negative values (xr, yr >= -32, RMmax <= 32)
-    const int tx0 = max(0, (xr - 31 - RMmax + 64 * 32) / 32 - 64), tx1 = min(ntx - 1, (xr + RMmax + 64 * 32) / 32 - 64);
-    const int ty0 = max(0, (yr - 7 - RMmax + 8 * 64) / 8 - 64), ty1 = min(nty - 1, (yr + RMmax + 8 * 64) / 8 - 64);
+    const int xr0 = 32 * otx - 32, yr0 = 8 * oty - 32;  // the output tile's first pixel, item-tile frame
+    const int tx0 = max(0, floor_div(xr0 - 31 - RMmax, 32)), tx1 = min(ntx - 1, floor_div(xr0 + 31 + RMmax, 32));
+    const int ty0 = max(0, floor_div(yr0 - 7 - RMmax, 8)), ty1 = min(nty - 1, floor_div(yr0 + 7 + RMmax, 8));
+    const int xr = xr0 + lx, yr = yr0 + ly;
     float sum = 0.0f;
     for (int ty = ty0; ty <= ty1; ++ty)
         for (int tx = tx0; tx <= tx1; ++tx) {
@@ -1524,23 +1536,25 @@ __device__ __forceinline__ float gather_blocks(const FanGeom &g, int X, int Y, c
     return sum;
 }
 
-// Main BEV [K][sy+64][sx+64] and tail BEV planes [tailCap][sy+64][sx+64].
+// Main BEV [K][sy+64][sx+64] and tail BEV planes [tailCap][sy+64][sx+64]; a work-group per
+// 32x8 output tile of a plane (one lane per pixel).
 __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__restrict__ bs,
                                                const int *__restrict__ itemR, const float *__restrict__ mainSlots,
                                                float *__restrict__ bev, const int *__restrict__ tailItemR,
                                                const float *__restrict__ tailSlots, float *__restrict__ tailBev,
                                                const int *__restrict__ planeRM) {
     const int bx = g.sx + 64, by = g.sy + 64;
-    const int X = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int otx = blockIdx.x, oty = blockIdx.y;
+    const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+    const int X = 32 * otx + lx, Y = 8 * oty + ly;
     const int z = blockIdx.z;
-    if (X >= bx || Y >= by) return;
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (z < g.K) {
         float sum = 0.0f;
         const int RMmax = planeRM[z];
         if (z >= bs->beamFirstInside && z < bs->mainEndMax && RMmax >= 0)
-            sum = gather_blocks(g, X, Y, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats, RMmax);
+            sum = gather_tile(g, otx, oty, lx, ly, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats,
+                              RMmax);
         bev[(size_t(z) * by + Y) * bx + X] = sum;
     } else {
         // tail planes, strided over the gridDim.z - K tail block rows
@@ -1548,8 +1562,8 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
         for (int q = z - g.K; q < bs->tailPlanes; q += tz) {
             const int RMmax = planeRM[g.K + q];
             const float sum = RMmax < 0 ? 0.0f
-                                        : gather_blocks(g, X, Y, tailItemR + size_t(q) * nt,
-                                                        tailSlots + size_t(q) * nt * kSlotFloats, RMmax);
+                                        : gather_tile(g, otx, oty, lx, ly, tailItemR + size_t(q) * nt,
+                                                      tailSlots + size_t(q) * nt * kSlotFloats, RMmax);
             tailBev[(size_t(q) * by + Y) * bx + X] = sum;
         }
     }
@@ -1770,8 +1784,8 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
                         hipStream_t s) {
     const int bx = g.sx + 64, by = g.sy + 64;
     const int tz = tailCap < 64 ? tailCap : 64;
-    hipLaunchKernelGGL(k_merge, dim3((bx + 63) / 64, (by + 3) / 4, g.K + tz), dim3(256), 0, s, g, bs, itemR,
-                       mainSlots, bev, tailItemR, tailSlots, tailBev, planeRM);
+    hipLaunchKernelGGL(k_merge, dim3(bx / 32, by / 8, g.K + tz), dim3(256), 0, s, g, bs, itemR, mainSlots, bev,
+                       tailItemR, tailSlots, tailBev, planeRM);
     return hipGetLastError();
 }
 
