@@ -113,6 +113,16 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int doseDims[3], const float d
 /* Accumulate one beam's dose into doseDevice ([nz][ny][nx] float32 device buffer) on the
  * engine's stream; returns after the work is queued (synchronize with pbs_engine_sync). */
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDevice);
+/* Adjoint with respect to the spot weights. No counterpart in the reference, which has no
+ * gradient path; it serves weight optimisation (the transposed superposition gather, then the
+ * transposed subspot convolution). For J(dose) with dJ/dDose = gradDoseDevice ([nz][ny][nx]
+ * float32 device buffer on this engine's dose grid) it writes gradSpotWeights[numSpots] (host)
+ * = dJ/dw at the beam's weights, holding live masks, tile radii and step ranges fixed (the dose
+ * is linear in the weights for fixed masks). Spot-lattice beams only (spot positions and
+ * weights given), halo off; returns after the gradient is on the host. The BEV gradient is
+ * accumulated with float atomics, so results agree run to run to rounding, not bitwise. */
+int pbs_engine_beam_adjoint(pbs_engine *e, const pbs_beam_settings *beam, const float *gradDoseDevice,
+                            float *gradSpotWeights);
 int pbs_engine_sync(pbs_engine *e);
 /* Raw stream handle (hipStream_t) the engine queues work on. */
 void *pbs_engine_stream(pbs_engine *e);

@@ -124,6 +124,9 @@ def lib():
         L.orc_transfer.argtypes = [C.POINTER(Plan), C.c_int, _d, _d, _i]
         L.orc_beam_dose.restype = C.c_int
         L.orc_beam_dose.argtypes = [_f, _i, _f, _f, _d, _i, _f, _f, C.POINTER(Beam), C.POINTER(Energy), C.POINTER(Plan)]
+        L.orc_beam_adjoint.restype = C.c_int
+        L.orc_beam_adjoint.argtypes = [_f, _i, _f, _f, _d, _i, _f, _f, C.POINTER(Beam), C.POINTER(Energy), _d,
+                                       C.POINTER(Plan)]
         _lib = L
     return _lib
 
@@ -270,6 +273,26 @@ def beam_dose(case: dict, beam_index: int = 0, dose: np.ndarray | None = None,
     if rc != 0:
         raise RuntimeError("oracle: " + p.error.decode())
     return dose, p
+
+
+def beam_adjoint(case: dict, g_dose: np.ndarray, beam_index: int = 0, use_subspots: bool = False,
+                 ct_hu_plus_1000: np.ndarray | None = None) -> np.ndarray:
+    """FP64 dJ/d(spot weights) of one beam for dJ/dDose = ``g_dose`` ([nz][ny][nx]), at the
+    beam's weights (spot-lattice path, halo off). Returns float64 [numSpots]."""
+    keep = _Keep()
+    en = make_energy(case["energy"], keep)
+    bm = make_beam(case["beams"][beam_index], keep, use_subspots)
+    ct_dims, ct_res, ct_corner, dose_dims, dose_res, dose_corner = case_args(case)
+    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
+    ct = np.ascontiguousarray(ct, dtype=np.float32)
+    g = np.ascontiguousarray(g_dose, dtype=np.float64).reshape(dose_dims[2], dose_dims[1], dose_dims[0])
+    out = np.zeros(int(bm.numSpots), np.float64)
+    p = Plan()
+    rc = lib().orc_beam_adjoint(_fp(ct), _ip(ct_dims), _fp(ct_res), _fp(ct_corner), _dp(g), _ip(dose_dims),
+                                _fp(dose_res), _fp(dose_corner), C.byref(bm), C.byref(en), _dp(out), C.byref(p))
+    if rc != 0:
+        raise RuntimeError("oracle: " + p.error.decode())
+    return out
 
 
 def ct_as_hu_plus_1000(ct: np.ndarray) -> np.ndarray:

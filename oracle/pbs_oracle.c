@@ -1306,3 +1306,276 @@ int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], co
     if (!planOut) free(p);
     return haloFail ? -1 : 0;
 }
+
+/* ------------------------------------------------------------------------------------
+ * adjoint with respect to the spot weights (lattice path, halo off)
+ *
+ * For fixed live masks, tile radii and step ranges (all of which depend on the CT, the
+ * sigmas and the sign of the weights, not on their size) the dose of orc_beam_dose is linear
+ * in the spot weights: dose = T S F C w (C: spot lattice -> ray weights, F: IDD = ray weight
+ * x unit IDD, S: superposition, T: BEV -> dose transfer). The adjoint applies the transposes
+ * in reverse order to dJ/dDose. Each transpose walks the same index pairs as its forward
+ * loop and scatters, so the pair sets are identical.
+ * ---------------------------------------------------------------------------------- */
+/* transpose of conv1d_gather: gIn[ci] += tap(o, ci) gOut[o] */
+static void conv1d_gather_adj(const double *gOut, int outW, int outStride, double *gIn, int inW, int inStride,
+                              double sigma, double pixelSp, double inOutOffset, double inOutDelta) {
+    const double sEff = sigma / pixelSp;
+    const double rS = 1.0 / (sqrt(2.0) * sEff);
+    const double reach = 3.0 * sEff + 0.5;
+    for (int o = 0; o < outW; ++o) {
+        const double go = gOut[(size_t)o * outStride];
+        int ci = (int)ceil(((double)o - reach - inOutOffset) / inOutDelta);
+        double dist = ci * inOutDelta + inOutOffset - (double)o;
+        while (dist < reach) {
+            if (ci >= 0 && ci < inW)
+                gIn[(size_t)ci * inStride] += 0.5 * (erf((dist + 0.5) * rS) - erf((dist - 0.5) * rS)) * go;
+            ++ci;
+            dist = ci * inOutDelta + inOutOffset - (double)o;
+        }
+    }
+}
+
+/* transpose of conv_plane: gSpot [sy][sx] += C^T (scale gRays) */
+static void conv_plane_adj(const double *gRays, int rx, int ry, double *gSpot, int sx, int sy, double sigX,
+                           double sigY, const orc_plan *p, float offX, float offY, double pxMultX, double pxMultY,
+                           double scale) {
+    double *gTmp = (double *)malloc(sizeof(double) * (size_t)rx * ry);
+    double *gInterm = (double *)calloc((size_t)rx * sy, sizeof(double));
+    float dX = p->spotDelta[0] / p->cpbRes[0], dY = p->spotDelta[1] / p->cpbRes[1];
+    float oX = (offX - p->cpbCorner[0]) / p->cpbRes[0], oY = (offY - p->cpbCorner[1]) / p->cpbRes[1];
+    for (size_t i = 0; i < (size_t)rx * ry; ++i) gTmp[i] = scale * gRays[i];
+    for (int x = 0; x < rx; ++x)
+        conv1d_gather_adj(gTmp + x, ry, rx, gInterm + x, sy, rx, sigY, (double)p->cpbRes[1] * pxMultY, oY, dY);
+    for (int y = 0; y < sy; ++y)
+        conv1d_gather_adj(gInterm + (size_t)y * rx, rx, 1, gSpot + (size_t)y * sx, sx, 1, sigX,
+                          (double)p->cpbRes[0] * pxMultX, oX, dX);
+    free(gTmp); free(gInterm);
+}
+
+/* transpose of orc_conv_lattice for one layer: gSpot [sy][sx] from gRays [ry][rx] */
+static void conv_lattice_adj(const orc_plan *p, const orc_beam *b, const orc_energy *en, int l, const double *gRays,
+                             double *gSpot) {
+    const int rx = p->rayDims[0], ry = p->rayDims[1], sx = p->spotDims[0], sy = p->spotDims[1];
+    const float entryZ = (float)p->beamFirstInside * p->fanDelta[2] + p->fanCorner[2];
+    const double pxX = 1.0f - entryZ / p->sourceDist[0], pxY = 1.0f - entryZ / p->sourceDist[1];
+    float eIdx = orc_find_decimal_ordered(en->energies, en->nEnergies, b->energies[l]);
+    float peak = orc_vector_interpolate(en->peaks, en->nEnergies, eIdx) * p->energyDepthToMm;
+    float ca, cb;
+    air_coefs(peak, &ca, &cb);
+    int done = 0;
+    if (b->useSubspotConvolution && b->maxSubspots > 0) {
+        for (int s = 0; s < b->maxSubspots; ++s) {
+            const float *ss = b->subspotData + ((size_t)l * b->maxSubspots + s) * 5;
+            if (!(ss[2] > 1.0e-8f)) continue;
+            float sgx = sigma_to_mm(p->lenToMm, ss[3]), sgy = sigma_to_mm(p->lenToMm, ss[4]);
+            if (!(sgx > 1.0e-8f) || !(sgy > 1.0e-8f)) continue;
+            float vx = ca * entryZ * entryZ + cb * entryZ + sgx * sgx;
+            float vy = ca * entryZ * entryZ + cb * entryZ + sgy * sgy;
+            double ex = sqrt((double)(vx > 0.f ? vx : 0.f)), ey = sqrt((double)(vy > 0.f ? vy : 0.f));
+            conv_plane_adj(gRays, rx, ry, gSpot, sx, sy, ex, ey, p, p->spotOffset[0] + ss[0] / p->lenToMm,
+                           p->spotOffset[1] + ss[1] / p->lenToMm, pxX, pxY, (double)ss[2]);
+            done = 1;
+        }
+    }
+    if (!done) {
+        float sgx = sigma_to_mm(p->lenToMm, b->spotSigmas[2 * l]);
+        float sgy = sigma_to_mm(p->lenToMm, b->spotSigmas[2 * l + 1]);
+        float vx = ca * entryZ * entryZ + cb * entryZ + sgx * sgx;
+        float vy = ca * entryZ * entryZ + cb * entryZ + sgy * sgy;
+        double ex = sqrt((double)(vx > 0.f ? vx : 0.f)), ey = sqrt((double)(vy > 0.f ? vy : 0.f));
+        conv_plane_adj(gRays, rx, ry, gSpot, sx, sy, ex, ey, p, p->spotOffset[0], p->spotOffset[1], pxX, pxY, 1.0);
+    }
+}
+
+/* transpose of sample3d_border_d: scatter g into the in-range corners */
+static void sample3d_border_adj(double *vol, const int dims[3], double x, double y, double z, double g) {
+    double fx = floor(x), fy = floor(y), fz = floor(z);
+    long x0 = (long)fx, y0 = (long)fy, z0 = (long)fz;
+    double tx = x - fx, ty = y - fy, tz = z - fz;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                long xi = x0 + dx, yi = y0 + dy, zi = z0 + dz;
+                if (xi < 0 || yi < 0 || zi < 0 || xi >= dims[0] || yi >= dims[1] || zi >= dims[2]) continue;
+                const double w = (dx ? tx : 1.0 - tx) * (dy ? ty : 1.0 - ty) * (dz ? tz : 1.0 - tz);
+                vol[((size_t)zi * dims[1] + yi) * dims[0] + xi] += w * g;
+            }
+}
+
+/* transpose of transfer_g: gBev += T^T gDose (the forward adds the sample when it is positive;
+ * samples of a non-negative field are non-negative, so the map is linear) */
+static void transfer_g_adj(orc_plan *p, const fan_grid *gr, int passive, const double *gDose, double *gBev,
+                           const int doseDims[3]) {
+    if (passive <= p->beamFirstInside) return;
+    int st[3], mxI[3];
+    transfer_box_g(p, gr, passive, st, mxI, doseDims);
+    if (!(st[0] <= mxI[0] && st[1] <= mxI[1] && st[2] <= mxI[2])) return;
+    int xEnd = st[0] + ((mxI[0] - st[0] + 1 + 31) / 32) * 32;
+    int yEnd = st[1] + ((mxI[1] - st[1] + 1 + 7) / 8) * 8;
+    if (xEnd > doseDims[0]) xEnd = doseDims[0];
+    if (yEnd > doseDims[1]) yEnd = doseDims[1];
+    aff g2d;
+    g2d.m = m3_from9(p->g2dose);
+    g2d.v = mk3(p->g2doseOff[0], p->g2doseOff[1], p->g2doseOff[2]);
+    aff d2g = aff_inv(g2d);
+    double M[9], O[3];
+    float t9[9];
+    m3_to9(d2g.m, t9);
+    for (int i = 0; i < 9; ++i) M[i] = t9[i];
+    O[0] = d2g.v.x; O[1] = d2g.v.y; O[2] = d2g.v.z;
+    const int bevDims[3] = {gr->sx + 2 * ORC_MAX_RAD, gr->sy + 2 * ORC_MAX_RAD, p->guaranteedPassive};
+    const double shift[3] = {ORC_MAX_RAD, ORC_MAX_RAD, -(double)p->beamFirstInside};
+    for (int y = st[1]; y < yEnd; ++y)
+        for (int x = st[0]; x < xEnd; ++x)
+            for (int z = st[2]; z <= mxI[2]; ++z) {
+                const double gv = gDose[((size_t)z * doseDims[1] + y) * doseDims[0] + x];
+                if (gv == 0.0) continue;
+                double g[3];
+                for (int r = 0; r < 3; ++r) g[r] = M[3 * r] * x + M[3 * r + 1] * y + M[3 * r + 2] * z + O[r];
+                double fx = g[0] / (1.0 - g[2] / p->sourceDist[0]);
+                double fy = g[1] / (1.0 - g[2] / p->sourceDist[1]);
+                double f[3] = {(fx - gr->fc[0]) / gr->fd[0] + shift[0], (fy - gr->fc[1]) / gr->fd[1] + shift[1],
+                               (g[2] - p->fanCorner[2]) / p->fanDelta[2] + shift[2]};
+                if (f[0] + 0.5 < 0.0 || f[1] + 0.5 < 0.0 || f[2] + 0.5 < 0.0) continue;
+                sample3d_border_adj(gBev, bevDims, f[0], f[1], f[2], gv);
+            }
+}
+
+/* transpose of superpose_grid (same radii and batching): gIdd[k][pix] = sum over the source's
+ * footprint of e_|j| e_|i| gBev, for the pixels with a non-zero IDD (the only ones the chain
+ * to the weights needs) */
+static void superpose_grid_adj(int k0, int k1, int sx, int sy, const double *idd, const double *rsigma,
+                               const double *gBev, double *gIdd) {
+    const int ntx = sx / ORC_TILE_X, nty = sy / ORC_TILE_Y;
+    const int bx = sx + 2 * ORC_MAX_RAD, by = sy + 2 * ORC_MAX_RAD;
+    (void)by;
+    if (k1 <= k0) return;
+    int hist[ORC_MAX_RAD + 2], batch[ORC_MAX_RAD + 1], effRad[ORC_MAX_RAD + 2];
+    memset(hist, 0, sizeof(hist));
+    const int nk = k1 - k0;
+    int *rad = (int *)malloc(sizeof(int) * (size_t)nk * ntx * nty);
+    for (int k = k0; k < k1; ++k)
+        for (int ty = 0; ty < nty; ++ty)
+            for (int tx = 0; tx < ntx; ++tx) {
+                double m = NAN;
+                for (int y = 0; y < ORC_TILE_Y; ++y)
+                    for (int x = 0; x < ORC_TILE_X; ++x)
+                        m = fmin(m, rsigma[((size_t)k * sy + ty * ORC_TILE_Y + y) * sx + tx * ORC_TILE_X + x]);
+                int r = orc_tile_radius_f32((float)m);
+                rad[((size_t)(k - k0) * nty + ty) * ntx + tx] = r;
+                hist[r]++;
+            }
+    orc_batch_bins(hist, batch, effRad);
+    double e[ORC_MAX_RAD + 1];
+    for (int k = k0; k < k1; ++k) {
+        const double *plane = gBev + (size_t)k * bx * (sy + 2 * ORC_MAX_RAD);
+        for (int ty = 0; ty < nty; ++ty)
+            for (int tx = 0; tx < ntx; ++tx) {
+                const int R = effRad[rad[((size_t)(k - k0) * nty + ty) * ntx + tx]];
+                for (int y = 0; y < ORC_TILE_Y; ++y)
+                    for (int x = 0; x < ORC_TILE_X; ++x) {
+                        const size_t ii = ((size_t)k * sy + ty * ORC_TILE_Y + y) * sx + tx * ORC_TILE_X + x;
+                        if (idd[ii] == 0.0) { gIdd[ii] = 0.0; continue; }
+                        const double r = rsigma[ii];
+                        double en = erf(r * 0.5), eo = -en;
+                        for (int i = 0; i <= R; ++i) {
+                            e[i] = 0.5 * (en - eo);
+                            eo = en;
+                            en = erf(r * ((double)i + 1.5));
+                        }
+                        const int X = tx * ORC_TILE_X + x + ORC_MAX_RAD;
+                        const int Y = ty * ORC_TILE_Y + y + ORC_MAX_RAD;
+                        double acc = 0.0;
+                        for (int j = -R; j <= R; ++j) {
+                            const double *row = plane + (size_t)(Y + j) * bx + X;
+                            double rsum = 0.0;
+                            for (int i = -R; i <= R; ++i) rsum += e[i < 0 ? -i : i] * row[i];
+                            acc += e[j < 0 ? -j : j] * rsum;
+                        }
+                        gIdd[ii] = acc;
+                    }
+            }
+    }
+    free(rad);
+}
+
+int orc_beam_adjoint(const float *ct, const int ctDims[3], const float ctRes[3], const float ctCorner[3],
+                     const double *gDose, const int doseDims[3], const float doseRes[3], const float doseCorner[3],
+                     const orc_beam *b, const orc_energy *en, double *gSpots, orc_plan *planOut) {
+    orc_plan *p = planOut ? planOut : (orc_plan *)malloc(sizeof(orc_plan));
+    if (orc_plan_beam(ctDims, ctRes, ctCorner, doseDims, doseRes, doseCorner, b, en, p) != 0) {
+        if (!planOut) free(p);
+        return -1;
+    }
+    if (p->legacyPath || b->nuclear) {
+        snprintf(p->error, sizeof(p->error), "the adjoint covers the spot-lattice path without the halo branch");
+        if (!planOut) free(p);
+        return -1;
+    }
+    const int rx = p->rayDims[0], ry = p->rayDims[1], K = p->steps, L = b->numLayers;
+    const int sx = p->superpDims[0], sy = p->superpDims[1];
+    const int spx = p->spotDims[0], spy = p->spotDims[1];
+    const size_t rplane = (size_t)rx * ry;
+    double *density = (double *)malloc(sizeof(double) * rplane * K);
+    double *wepl = (double *)malloc(sizeof(double) * rplane * K);
+    int *fi = (int *)malloc(sizeof(int) * rplane), *fo = (int *)malloc(sizeof(int) * rplane);
+    double *weplMin = (double *)malloc(sizeof(double) * K);
+    orc_trace(p, ct, ctDims, en, density, wepl, fi, fo);
+    orc_beam_cutoffs(p, b, en, wepl, fi, fo, weplMin);
+    double *rw = (double *)calloc(rplane * L, sizeof(double));
+    orc_conv_lattice(p, b, en, rw);
+    orc_layer_params(p, b, en, weplMin);
+    double *idd = (double *)malloc(sizeof(double) * rplane * K);
+    double *rs = (double *)malloc(sizeof(double) * rplane * K);
+    double *gIdd = (double *)malloc(sizeof(double) * rplane * K);
+    double *gW = (double *)malloc(sizeof(double) * rplane);
+    double *gSpot = (double *)calloc((size_t)spx * spy * L, sizeof(double));
+    int *fp = (int *)malloc(sizeof(int) * rplane);
+    const size_t bevN = (size_t)(sx + 64) * (sy + 64) * (p->guaranteedPassive > 0 ? p->guaranteedPassive : 1);
+    double *gBev = (double *)malloc(sizeof(double) * bevN);
+    fan_grid gr = primary_grid(p);
+    for (int l = 0; l < L; ++l) {
+        orc_fill(p, l, en, density, wepl, rw + (size_t)l * rplane, fi, fo, idd, rs, fp);
+        int mxp = INT32_MIN;
+        for (size_t r = 0; r < rplane; ++r) if (fp[r] > mxp) mxp = fp[r];
+        int pas = mxp;
+        if (pas < p->beamFirstInside + 1) pas = p->beamFirstInside + 1;
+        if (pas < 0) pas = p->beamFirstInside + 1;
+        if (pas > p->afterLast[l]) pas = p->afterLast[l];
+        if (pas > K) pas = K;
+        p->layerPassive[l] = pas;
+        memset(gBev, 0, sizeof(double) * bevN);
+        transfer_g_adj(p, &gr, pas, gDose, gBev, doseDims);
+        superpose_grid_adj(p->beamFirstInside, pas, sx, sy, idd, rs, gBev, gIdd);
+        /* IDD = W x (unit IDD) on live rays: dJ/dW = sum_k dJ/dIDD IDD / W */
+        const double *W = rw + (size_t)l * rplane;
+        for (size_t r = 0; r < rplane; ++r) {
+            double acc = 0.0;
+            if (W[r] >= 1e-6)
+                for (int k = p->beamFirstInside; k < pas; ++k) {
+                    const size_t ii = (size_t)k * rplane + r;
+                    if (idd[ii] != 0.0) acc += gIdd[ii] * idd[ii];
+                }
+            gW[r] = W[r] >= 1e-6 ? acc / W[r] : 0.0;
+        }
+        conv_lattice_adj(p, b, en, l, gW, gSpot + (size_t)l * spx * spy);
+    }
+    /* spots -> their lattice cells (build_lattice's binning) */
+    int off = 0;
+    for (int l = 0; l < L; ++l) {
+        for (int i = 0; i < b->layerSpotCounts[l]; ++i) {
+            const int s = off + i;
+            const float x = b->spotPositions[2 * s], y = b->spotPositions[2 * s + 1];
+            const float fx = p->spotDelta[0] != 0.0f ? (x - p->spotOffset[0]) / p->spotDelta[0] : 0.0f;
+            const float fy = p->spotDelta[1] != 0.0f ? (y - p->spotOffset[1]) / p->spotDelta[1] : 0.0f;
+            const int ix = (int)lroundf(fx), iy = (int)lroundf(fy);
+            gSpots[s] = (ix >= 0 && ix < spx && iy >= 0 && iy < spy) ? gSpot[((size_t)l * spy + iy) * spx + ix] : 0.0;
+        }
+        off += b->layerSpotCounts[l];
+    }
+    free(density); free(wepl); free(fi); free(fo); free(weplMin); free(rw);
+    free(idd); free(rs); free(gIdd); free(gW); free(gSpot); free(fp); free(gBev);
+    if (!planOut) free(p);
+    return 0;
+}

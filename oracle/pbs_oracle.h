@@ -196,6 +196,14 @@ int orc_beam_dose(const float *ctHuPlus1000, const int ctDims[3], const float ct
  * (bev_ray_tracing.cu:17-62 after its -0.5). Exposed for tests. */
 double orc_sample3d_border(const float *vol, const int dims[3], double x, double y, double z);
 
+/* Adjoint with respect to the spot weights (spot-lattice path, halo off): gSpots[numSpots] =
+ * dJ/dw for J with dJ/dDose = gDose ([nz][ny][nx]), at the weights of `beam`. The transposes of
+ * the transfer, superposition, IDD scaling and separable spot convolution, in FP64.
+ * Pinned: <gDose, dose(w)> = <w, gSpots> (the dose is linear in w for fixed masks) and central
+ * finite differences of orc_beam_dose. */
+int orc_beam_adjoint(const float *ctHuPlus1000, const int ctDims[3], const float ctRes[3], const float ctCorner[3],
+                     const double *gDose, const int doseDims[3], const float doseRes[3], const float doseCorner[3],
+                     const orc_beam *beam, const orc_energy *en, double *gSpots, orc_plan *plan);
 #ifdef __cplusplus
 }
 #endif

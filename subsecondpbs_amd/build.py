@@ -48,7 +48,7 @@ def build(verbose: bool = False) -> str:
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-c",
           os.path.join(CSRC, "plan.cpp"), "-o", plan_o])
     objs = [plan_o]
-    for src in ("kernels.hip", "engine.hip", "compat.hip"):
+    for src in ("kernels.hip", "adjoint.hip", "engine.hip", "compat.hip"):
         o = os.path.join(obj_dir, src.replace(".hip", ".o"))
         _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)

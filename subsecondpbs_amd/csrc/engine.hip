@@ -152,6 +152,11 @@ struct pbs_engine {
     // final-dose binding
     std::vector<float> fdEnergies, fdSigmas;
     float fdRaySpacing = 0.5f;
+    // adjoint with respect to the spot weights: dJ/dIDD x IDD / W per (layer, step, ray), dJ/dW,
+    // the transposed convolution's intermediate and the spot-lattice gradient
+    DevBuf<float> adjI, adjW, adjInterm, adjSpot;
+    ConvArgs convArgs{};
+    std::vector<float> hostAdjSpot;
 };
 
 namespace {
@@ -449,10 +454,16 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int dims[3], const float res[3
 }
 
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
-                            const float *cpbWorldGeom);
+                            const float *cpbWorldGeom, const float *gradDose = nullptr, float *gradSpots = nullptr);
 
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
+}
+
+int pbs_engine_beam_adjoint(pbs_engine *e, const pbs_beam_settings *beam, const float *gradDoseDev,
+                            float *gradSpotWeights) {
+    if (!gradDoseDev || !gradSpotWeights) return set_error(PBS_E_INVALID, "null argument");
+    return engine_beam_dose(e, beam, nullptr, nullptr, nullptr, gradDoseDev, gradSpotWeights);
 }
 
 // One grid's stages after the fill: layer groups and boxes, tile radii, bins, work units;
@@ -498,11 +509,43 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     return PBS_OK;
 }
 
+// The adjoint of one grid's stages after the fill: the same layer groups, tile radii, bins and
+// work units as the forward, then the transposed transfer (into the BEV buffers, which here hold
+// dJ/dBEV), the transposed superposition and IDD scaling into dJ/dW (e->adjW).
+static int run_grid_adjoint(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork &w, const float2 *iddSigma,
+                            LayerScalars *ls, BeamScalars *bs, const float *gradDose, hipStream_t s) {
+    HIP_TRY(launch_finalize_and_tiles(g, iddSigma, ls, w.groupScal.p, bs, w.tailPlaneGroup.p, w.tailPlaneK.p,
+                                      w.tailCap, w.tileRad.p, w.hist.p, w.effRad.p, w.itemR.p, w.tailItemR.p,
+                                      w.classCount.p, w.classList.p, w.maxUnits, nullptr, 1, w.planeRM.p,
+                                      w.slotLayers.p, s));
+    const size_t bevPlane = size_t(g.sx + 64) * (g.sy + 64);
+    HIP_TRY(launch_adj_zero(w.bev.p, size_t(g.K) * bevPlane, s));
+    HIP_TRY(launch_adj_zero(w.tailBev.p, size_t(w.tailCap) * bevPlane, s));
+    TransferArgs ta{};
+    fill_transfer_args(p, g, ta);
+    int gx = 1, gy = 1, gz = 1;
+    host_transfer_grid(p, g, e->doseDims, gx, gy, gz);
+    int depthAxis = 0;
+    for (int a = 1; a < 3; ++a)
+        if (std::fabs(g.d2g[6 + a]) > std::fabs(g.d2g[6 + depthAxis])) depthAxis = a;
+    HIP_TRY(launch_adj_transfer(g, ta, bs, w.groupScal.p, gradDose, w.bev.p, w.tailBev.p, gx * 32, gy * 8, gz,
+                                depthAxis, s));
+    const size_t nray = size_t(g.rx) * g.ry;
+    HIP_TRY(e->adjI.ensure(size_t(g.numLayers) * g.K * nray));
+    HIP_TRY(e->adjW.ensure(size_t(g.numLayers) * nray));
+    HIP_TRY(launch_adj_zero(e->adjI.p, size_t(g.numLayers) * g.K * nray, s));
+    HIP_TRY(launch_adj_superpose(g, iddSigma, e->rayW.p, w.groupScal.p, bs, w.classCount.p, w.classList.p,
+                                 w.maxUnits, w.slotLayers.p, w.bev.p, w.tailBev.p, e->adjI.p, s));
+    HIP_TRY(launch_adj_reduce(g, ls, bs, e->adjI.p, e->adjW.p, s));
+    return PBS_OK;
+}
+
 // cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
-                            const float *cpbWorldGeom) {
-    if (!e || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+                            const float *cpbWorldGeom, const float *gradDose, float *gradSpots) {
+    const bool adjoint = gradDose != nullptr;
+    if (!e || !beam || (!doseDev && !adjoint)) return set_error(PBS_E_INVALID, "null argument");
     if (!e->ctPtr) return set_error(PBS_E_INVALID, "no CT bound to the engine");
     if (e->doseDims[0] <= 0) return set_error(PBS_E_INVALID, "no dose grid bound to the engine");
     int rc = validate_beam(*beam, 0);
@@ -522,6 +565,9 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     const bool subMode = p.subspotConv && !p.legacyPath;
     const int nSub = subMode ? beam->maxSubspotsPerLayer : 1;
     const bool halo = e->nuclearModel != 0;
+    if (adjoint && (halo || p.legacyPath))
+        return set_error(PBS_E_INVALID, halo ? "the adjoint does not cover the nuclear halo branch"
+                                             : "the adjoint needs spot positions and weights (spot-lattice path)");
     if (halo) {
         if (!e->nucW.p || !e->nucS2.p)
             return set_error(PBS_E_GEOMETRY,
@@ -622,6 +668,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         a.lenToMm = p.lenToMm;
         HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
                             e->subEntry.p, L, s));
+        e->convArgs = a;
     } else {
         // subspot -> CPB in the gantry frame (wrapper:3480-3511); the ray grid is the CPB grid
         CpbArgs a{};
@@ -654,6 +701,26 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (e->nuclearModel == 3) fc = FillConsts{216.09f, 0.08f, 0, 0};  // FLUKA
     HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
                         e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p, s));
+    if (adjoint) {
+        // 4-6 transposed, then the transposed convolution and the spots' lattice cells
+        int rc3 = run_grid_adjoint(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, gradDose, s);
+        if (rc3 != PBS_OK) return rc3;
+        const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
+        HIP_TRY(e->adjInterm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
+        HIP_TRY(e->adjSpot.ensure(splane * L));
+        HIP_TRY(launch_adj_conv(e->convArgs, e->adjW.p, e->adjInterm.p, e->adjSpot.p, e->layerScal.p, e->beamScal.p,
+                                e->subspots.p, e->subEntry.p, L, s));
+        e->hostAdjSpot.resize(splane * L);
+        HIP_TRY(hipMemcpyAsync(e->hostAdjSpot.data(), e->adjSpot.p, sizeof(float) * splane * L,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < beam->numSpots; ++i) {
+            const int c = i < int(p.spotCell.size()) ? p.spotCell[i] : -1;
+            gradSpots[i] = c >= 0 ? e->hostAdjSpot[size_t(c)] : 0.0f;
+        }
+        e->haveLast = true;
+        return PBS_OK;
+    }
     // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
     for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];

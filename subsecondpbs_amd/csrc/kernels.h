@@ -76,4 +76,19 @@ hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamS
                            const float *bev, const float *tailBev, float *dose, int extentX, int extentY, int extentZ,
                            int depthAxis, hipStream_t s);
 
+// adjoint with respect to the spot weights (adjoint.hip)
+hipError_t launch_adj_zero(float *v, size_t n, hipStream_t s);
+hipError_t launch_adj_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs,
+                               const GroupScalars *gs, const float *gDose, float *gBev, float *gTailBev, int extentX,
+                               int extentY, int extentZ, int depthAxis, hipStream_t s);
+hipError_t launch_adj_superpose(const FanGeom &g, const float2 *iddSigma, const float *rayW, const GroupScalars *gs,
+                                const BeamScalars *bs, const int *classCount, const int2 *classList, int maxUnits,
+                                const int *slotLayers, const float *gBev, const float *gTailBev, float *gI,
+                                hipStream_t s);
+hipError_t launch_adj_reduce(const FanGeom &g, const LayerScalars *ls, const BeamScalars *bs, const float *gI,
+                             float *gW, hipStream_t s);
+hipError_t launch_adj_conv(const ConvArgs &a, const float *gW, float *gInterm, float *gSpot, const LayerScalars *ls,
+                           const BeamScalars *bs, const float *subspots, const float *subEntry, int numLayers,
+                           hipStream_t s);
+
 }  // namespace pbs

@@ -161,6 +161,7 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
     const int nx = std::max(1, int(std::lround((ux.back() - ox) / dx)) + 1);
     const int ny = std::max(1, int(std::lround((uy.back() - oy) / dy)) + 1);
     std::vector<float> dense(size_t(nx) * ny * b.numLayers, 0.0f);
+    std::vector<int> cells(size_t(total), -1);
     int off = 0;
     for (int l = 0; l < b.numLayers; ++l) {
         const int cnt = b.layerSpotCounts[l];
@@ -169,16 +170,19 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
             const int s = off + i;
             const float x = b.spotPositions[2 * s], y = b.spotPositions[2 * s + 1], w = b.spotWeights[s];
             if (!std::isfinite(w) || w < 0.0f) return false;
-            if (w == 0.0f) continue;
             const float fx = dx != 0.0f ? (x - ox) / dx : 0.0f;
             const float fy = dy != 0.0f ? (y - oy) / dy : 0.0f;
             const int ix = int(std::lround(fx)), iy = int(std::lround(fy));
-            if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) return false;
+            const bool inLattice = ix >= 0 && ix < nx && iy >= 0 && iy < ny;
+            if (inLattice) cells[s] = int((size_t(l) * ny + iy) * nx + ix);
+            if (w == 0.0f) continue;
+            if (!inLattice) return false;
             if (!(std::fabs(ox + ix * dx - x) <= 2e-2f) || !(std::fabs(oy + iy * dy - y) <= 2e-2f)) return false;
             dense[(size_t(l) * ny + iy) * nx + ix] += w;
         }
         off += cnt;
     }
+    p.spotCell.swap(cells);
     p.spotDims[0] = nx; p.spotDims[1] = ny;
     p.spotOffset[0] = ox; p.spotOffset[1] = oy;
     p.spotDelta[0] = dx; p.spotDelta[1] = dy;

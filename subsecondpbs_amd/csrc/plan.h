@@ -55,6 +55,7 @@ struct BeamPlan {
     int spotDims[2] = {0, 0};
     float spotOffset[2] = {0, 0}, spotDelta[2] = {0, 0};
     std::vector<float> denseWeights;  // [L][sy][sx]
+    std::vector<int> spotCell;        // per spot: its cell (l * sy + iy) * sx + ix in denseWeights, or -1
     // grids
     float cpbCorner[3] = {0, 0, 0}, cpbRes[3] = {0, 0, 0};
     int rayDims[2] = {0, 0};

@@ -65,6 +65,17 @@ class Engine:
         rc = _abi.lib().pbs_engine_beam_dose(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()))
         _abi.check(rc, "pbs_engine_beam_dose")
 
+    def beam_adjoint(self, beam: dict, grad_dose, use_subspots: bool | None = None) -> np.ndarray:
+        """dJ/d(spot weights) of one beam for dJ/dDose = ``grad_dose`` (device tensor
+        [nz][ny][nx] float32), at the beam's weights; float32 numpy [numSpots]. Synchronous."""
+        keep = _abi.Keep()
+        bs = _abi.beam_settings(beam, keep, use_subspots)
+        out = np.zeros(int(bs.numSpots), np.float32)
+        rc = _abi.lib().pbs_engine_beam_adjoint(self.handle, C.byref(bs), C.c_void_p(grad_dose.data_ptr()),
+                                                C.c_void_p(out.ctypes.data))
+        _abi.check(rc, "pbs_engine_beam_adjoint")
+        return out
+
     def sync(self):
         _abi.check(_abi.lib().pbs_engine_sync(self.handle), "pbs_engine_sync")
 

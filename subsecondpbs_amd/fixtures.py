@@ -227,6 +227,33 @@ def head_beam(seed: int = 2, gantry_deg: float = 90.0, n_layers: int = 30,
                      beam_dir=bdir, bm_x=bmx, bm_y=bmy, sad=2000.0)
 
 
+def make_config4(seed: int = 1) -> dict:
+    """Config-1 beam with a weight-optimisation objective (differentiation case).
+
+    Target: the 60 mm cube centred on the iso; OAR: an ellipsoid at the lateral edge of the
+    field (centre (35, 0, 0) mm, semi-axes 12, 18, 25 mm). Objective
+    J = sum_T (D - 2)^2 + 0.1 sum_OAR D^2, with the weights first scaled so that the mean target
+    dose is 2 (the dose is linear in the weights). ``fd_spots``: 64 seeded spot indices for the
+    central finite-difference check (h = 1e-3 w).
+    """
+    case = make_config1(seed)
+    nx, ny, nz = case["dose_dims"]
+    cx, cy, cz = case["dose_corner"]
+    rx_, ry_, rz_ = case["dose_resolution"]
+    z, y, x = np.meshgrid(cz + rz_ * np.arange(nz), cy + ry_ * np.arange(ny), cx + rx_ * np.arange(nx),
+                          indexing="ij")
+    case["target_mask"] = (np.abs(x) <= 30.0) & (np.abs(y) <= 30.0) & (np.abs(z) <= 30.0)
+    case["oar_mask"] = ((x - 35.0) / 12.0) ** 2 + (y / 18.0) ** 2 + (z / 25.0) ** 2 <= 1.0
+    n = int(np.asarray(case["beams"][0]["spot_weights"]).size)
+    rng = Stream(seed, "config4.fd_spots")
+    order = np.argsort(rng.uniform(n))
+    case["fd_spots"] = np.sort(order[:64]).astype(np.int64)
+    case["prescription"] = 2.0
+    case["oar_factor"] = 0.1
+    case["name"] = "config4-objective-64"
+    return case
+
+
 def make_config2(seed: int = 2, gantry_deg: float = 90.0) -> dict:
     """Head-size phantom with one 30-layer proton beam at gantry 90 deg (along +x)."""
     ct, dims, res, corner = head_phantom(seed)
