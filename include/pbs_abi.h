@@ -217,6 +217,45 @@ int pbs_final_dose_roi(const vec3i *roiIndicesHost, int numROIIndices, vec3f dos
                        vec3f doseGridCorner, vec3f beamDirection, vec3f sourcePosition, int numLayers,
                        vec3f *roiMinCorner, vec3f *roiMaxCorner, vec3i *roiDims, CPBGrid *cpbGrid);
 
+/* ---- served form -------------------------------------------------------------------
+ * No counterpart in the reference (its entry is synchronous, single request, not reentrant).
+ * A request server for dose services: resident CTs, a pool of engine slots (each with its own
+ * streams), and a worker thread that admits queued requests onto free slots while the device
+ * memory they need (dose buffer, resident beam state, growth of the slot's work buffers) fits
+ * the budget, so requests that arrive while others run overlap on the device (continuous
+ * batching). A full request may keep its beam resident (keepState): its unit-weight IDD /
+ * rSigmaEff field, geometry and per-layer values stay in HBM, and weight-only updates of that
+ * beam re-run only the convolution, the rescale, the superposition, the merge and the transfer.
+ * Every request computes on its own engine and buffers: its result does not depend on what
+ * else is in flight. Results of resident beams use IDD = W x unit IDD, which can differ from the
+ * plain form's single product by one rounding (float ulps of the dose).
+ */
+typedef struct pbs_server pbs_server;
+/* slots 1..16; memoryBudgetBytes <= 0: 80 % of the free device memory at creation. */
+pbs_server *pbs_server_create(int gpuId, const pbs_energy_struct *energy, int slots, double memoryBudgetBytes);
+void pbs_server_destroy(pbs_server *s);
+/* A resident CT (HU or HU+1000, host or device pointer, copied); *ctId names it in requests. */
+int pbs_server_add_ct(pbs_server *s, const float *ct, const int ctDims[3], const float ctRes[3],
+                      const float ctCorner[3], int ctIsDevice, int *ctId);
+int pbs_server_remove_ct(pbs_server *s, int ctId);
+/* Queue a full beam on a resident CT and dose grid; the beam settings are copied. keepState:
+ * the beam stays resident for pbs_server_submit_weights, under the handle *ticket. */
+int pbs_server_submit(pbs_server *s, int ctId, const int doseDims[3], const float doseRes[3],
+                      const float doseCorner[3], const pbs_beam_settings *beam, int keepState, long long *ticket);
+/* Queue a weight-only update of a resident beam (numSpots weights in the beam's spot order). */
+int pbs_server_submit_weights(pbs_server *s, long long stateTicket, const float *spotWeights, int numSpots,
+                              long long *ticket);
+/* 1 done, 0 queued or running, < 0 the request's error (pbs_last_error has the message). */
+int pbs_server_query(pbs_server *s, long long ticket);
+int pbs_server_wait(pbs_server *s, long long ticket);
+/* Waits, copies the dose ([nz][ny][nx] float32) to host and frees the request's dose buffer. */
+int pbs_server_result(pbs_server *s, long long ticket, float *doseHost);
+/* Milliseconds from submit to admission and from submit to completion (-1 while pending). */
+int pbs_server_timing(pbs_server *s, long long ticket, double *queueMs, double *totalMs);
+/* Frees a resident beam (no update of it may be queued or running). */
+int pbs_server_release_state(pbs_server *s, long long stateTicket);
+int pbs_server_memory(pbs_server *s, double *usedBytes, double *budgetBytes);
+
 #ifdef __cplusplus
 }
 #endif

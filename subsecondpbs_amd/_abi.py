@@ -85,6 +85,9 @@ EXPORTED = [
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
+    "pbs_server_create", "pbs_server_destroy", "pbs_server_add_ct", "pbs_server_remove_ct", "pbs_server_submit",
+    "pbs_server_submit_weights", "pbs_server_query", "pbs_server_wait", "pbs_server_result", "pbs_server_timing",
+    "pbs_server_release_state", "pbs_server_memory",
 ]
 
 _lib = None
@@ -148,6 +151,28 @@ def lib():
         L.pbs_batch_sync.argtypes = [C.c_void_p]
         L.pbs_batch_lanes.restype = C.c_int
         L.pbs_batch_lanes.argtypes = [C.c_void_p]
+        LL = C.POINTER(C.c_longlong)
+        L.pbs_server_create.restype = C.c_void_p
+        L.pbs_server_create.argtypes = [C.c_int, C.POINTER(EnergyStruct), C.c_int, C.c_double]
+        L.pbs_server_destroy.restype = None
+        L.pbs_server_destroy.argtypes = [C.c_void_p]
+        L.pbs_server_add_ct.restype = C.c_int
+        L.pbs_server_add_ct.argtypes = [C.c_void_p, C.c_void_p, _i, _f, _f, C.c_int, C.POINTER(C.c_int)]
+        L.pbs_server_remove_ct.restype = C.c_int
+        L.pbs_server_remove_ct.argtypes = [C.c_void_p, C.c_int]
+        L.pbs_server_submit.restype = C.c_int
+        L.pbs_server_submit.argtypes = [C.c_void_p, C.c_int, _i, _f, _f, C.POINTER(BeamSettings), C.c_int, LL]
+        L.pbs_server_submit_weights.restype = C.c_int
+        L.pbs_server_submit_weights.argtypes = [C.c_void_p, C.c_longlong, _f, C.c_int, LL]
+        for fn in ("pbs_server_query", "pbs_server_wait", "pbs_server_release_state"):
+            getattr(L, fn).restype = C.c_int
+            getattr(L, fn).argtypes = [C.c_void_p, C.c_longlong]
+        L.pbs_server_result.restype = C.c_int
+        L.pbs_server_result.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p]
+        L.pbs_server_timing.restype = C.c_int
+        L.pbs_server_timing.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.pbs_server_memory.restype = C.c_int
+        L.pbs_server_memory.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pbs_final_dose_bind.restype = C.c_int
         L.pbs_final_dose_bind.argtypes = [C.c_void_p, _f, _f, C.c_int, C.c_float]
         L.cudaFinalDose.restype = C.c_int

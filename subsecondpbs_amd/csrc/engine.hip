@@ -8,11 +8,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pbs_abi.h"
@@ -159,6 +165,28 @@ struct pbs_engine {
     std::vector<float> hostAdjSpot;
 };
 
+// Resident state of a served beam: everything of its first run that the spot weights do not
+// change (host plan, fan geometry, per-layer host values, the post-trace beam scalars and the
+// per-step minimum WEPL, the subspot table, and the unit-weight IDD / rSigmaEff field with its
+// passive steps). A weight-only update then re-runs the convolution, the rescale of the unit
+// field, the superposition, the merge and the transfer; the trace and the fill recurrence are
+// not repeated.
+struct BeamState {
+    BeamPlan plan;
+    FanGeom geom{};
+    std::vector<LayerHost> layerHost;
+    int nSub = 1, maxSub = 0, numSpots = 0;
+    bool subMode = false;
+    DevBuf<float2> unitIdd;
+    DevBuf<int> unitFp;
+    DevBuf<BeamScalars> bs0;
+    DevBuf<float> weplMin, subspots;
+    size_t bytes() const {
+        return unitIdd.n * sizeof(float2) + unitFp.n * sizeof(int) + bs0.n * sizeof(BeamScalars) +
+               (weplMin.n + subspots.n) * sizeof(float);
+    }
+};
+
 namespace {
 
 constexpr int kStages = 8;  // trace, conv, fill, tiles, superpose, merge, transfer, halo
@@ -196,6 +224,22 @@ void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
     g.maxCutoffWepl = p.maxCutoffWepl;
     g.numLayers = p.numLayers;
     g.legacy = p.legacyPath ? 1 : 0;
+}
+
+// Spot-lattice convolution arguments of a planned beam (convolution.cu:477-513).
+ConvArgs make_conv_args(const BeamPlan &p, int rx, int ry, int nSub, bool subMode) {
+    ConvArgs a{};
+    a.rx = rx; a.ry = ry; a.spotX = p.spotDims[0]; a.spotY = p.spotDims[1];
+    a.nSub = nSub; a.subspotMode = subMode ? 1 : 0;
+    a.cpbResX = p.cpbRes[0]; a.cpbResY = p.cpbRes[1];
+    a.deltaX = p.spotDelta[0] / p.cpbRes[0];
+    a.deltaY = p.spotDelta[1] / p.cpbRes[1];
+    a.offX = (p.spotOffset[0] - p.cpbCorner[0]) / p.cpbRes[0];
+    a.offY = (p.spotOffset[1] - p.cpbCorner[1]) / p.cpbRes[1];
+    a.spotOffX = p.spotOffset[0]; a.spotOffY = p.spotOffset[1];
+    a.cpbCornerX = p.cpbCorner[0]; a.cpbCornerY = p.cpbCorner[1];
+    a.lenToMm = p.lenToMm;
+    return a;
 }
 
 // Host worst-case transfer launch extents: the box for steps [0, K-1] contains every
@@ -454,7 +498,8 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int dims[3], const float res[3
 }
 
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
-                            const float *cpbWorldGeom, const float *gradDose = nullptr, float *gradSpots = nullptr);
+                            const float *cpbWorldGeom, const float *gradDose = nullptr, float *gradSpots = nullptr,
+                            BeamState *keep = nullptr);
 
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
@@ -543,7 +588,7 @@ static int run_grid_adjoint(pbs_engine *e, const BeamPlan &p, const FanGeom &g, 
 // cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
-                            const float *cpbWorldGeom, const float *gradDose, float *gradSpots) {
+                            const float *cpbWorldGeom, const float *gradDose, float *gradSpots, BeamState *keep) {
     const bool adjoint = gradDose != nullptr;
     if (!e || !beam || (!doseDev && !adjoint)) return set_error(PBS_E_INVALID, "null argument");
     if (!e->ctPtr) return set_error(PBS_E_INVALID, "no CT bound to the engine");
@@ -568,6 +613,9 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (adjoint && (halo || p.legacyPath))
         return set_error(PBS_E_INVALID, halo ? "the adjoint does not cover the nuclear halo branch"
                                              : "the adjoint needs spot positions and weights (spot-lattice path)");
+    if (keep && (halo || p.legacyPath))
+        return set_error(PBS_E_INVALID, halo ? "resident beams do not cover the nuclear halo branch"
+                                             : "resident beams need spot positions and weights (spot-lattice path)");
     if (halo) {
         if (!e->nucW.p || !e->nucS2.p)
             return set_error(PBS_E_GEOMETRY,
@@ -648,6 +696,24 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
                               e->prim.classCount.p, e->prim.hist.p, s));
+    if (keep) {
+        HIP_TRY(keep->bs0.ensure(1));
+        HIP_TRY(keep->weplMin.ensure(K));
+        HIP_TRY(keep->subspots.ensure(size_t(L) * beam->maxSubspotsPerLayer * 5));
+        HIP_TRY(keep->unitIdd.ensure(nsup * K * L));
+        HIP_TRY(keep->unitFp.ensure(nray * L));
+        HIP_TRY(hipMemcpyAsync(keep->bs0.p, e->beamScal.p, sizeof(BeamScalars), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(keep->weplMin.p, e->weplMin.p, sizeof(float) * K, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(keep->subspots.p, e->subspots.p, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
+                               hipMemcpyDeviceToDevice, s));
+        keep->plan = p;
+        keep->geom = g;
+        keep->layerHost = lh;
+        keep->nSub = nSub;
+        keep->maxSub = beam->maxSubspotsPerLayer;
+        keep->subMode = subMode;
+        keep->numSpots = beam->numSpots;
+    }
     // 2. ray weights
     HIP_TRY(mark());
     if (!p.legacyPath) {
@@ -655,17 +721,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(e->spot.ensure(splane * L));
         HIP_TRY(e->interm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
         HIP_TRY(hipMemcpyAsync(e->spot.p, p.denseWeights.data(), sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
-        ConvArgs a{};
-        a.rx = rx; a.ry = ry; a.spotX = p.spotDims[0]; a.spotY = p.spotDims[1];
-        a.nSub = nSub; a.subspotMode = subMode ? 1 : 0;
-        a.cpbResX = p.cpbRes[0]; a.cpbResY = p.cpbRes[1];
-        a.deltaX = p.spotDelta[0] / p.cpbRes[0];
-        a.deltaY = p.spotDelta[1] / p.cpbRes[1];
-        a.offX = (p.spotOffset[0] - p.cpbCorner[0]) / p.cpbRes[0];
-        a.offY = (p.spotOffset[1] - p.cpbCorner[1]) / p.cpbRes[1];
-        a.spotOffX = p.spotOffset[0]; a.spotOffY = p.spotOffset[1];
-        a.cpbCornerX = p.cpbCorner[0]; a.cpbCornerY = p.cpbCorner[1];
-        a.lenToMm = p.lenToMm;
+        const ConvArgs a = make_conv_args(p, rx, ry, nSub, subMode);
         HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
                             e->subEntry.p, L, s));
         e->convArgs = a;
@@ -699,8 +755,23 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (e->nuclearModel == 1) fc = FillConsts{169.00f, 0.06f, 1, 0};  // GAUSS_FIT
     if (e->nuclearModel == 2) fc = FillConsts{190.44f, 0.0f, 0, 0};   // SOUKUP
     if (e->nuclearModel == 3) fc = FillConsts{216.09f, 0.08f, 0, 0};  // FLUKA
-    HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p, e->rayW.p,
-                        e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p, s));
+    if (keep) {
+        // the weight-independent field (every ray at weight 1) into the resident state, then the
+        // layer scalars reset and the rescale to this request's weights
+        fc.unitWeight = 1;
+        HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p,
+                            e->rayW.p, e->fi.p, e->fo.p, keep->unitIdd.p, keep->unitFp.p, fc, hf, e->prim.tileRad.p,
+                            s));
+        HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
+                                  subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, 1.0f,
+                                  e->prim.classCount.p, e->prim.hist.p, s));
+        HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, keep->unitIdd.p,
+                               keep->unitFp.p, e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
+    } else {
+        HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p,
+                            e->rayW.p, e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p,
+                            s));
+    }
     if (adjoint) {
         // 4-6 transposed, then the transposed convolution and the spots' lattice cells
         int rc3 = run_grid_adjoint(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, gradDose, s);
@@ -740,6 +811,75 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     e->haloUsed |= halo;
     HIP_TRY(mark());
     if (e->profiling) e->evBeams.push_back(std::move(marks));
+    e->haveLast = true;
+    return PBS_OK;
+}
+
+// Weight-only update of a resident beam on engine e: the lattice re-binned with the new
+// weights, the convolution, the rescale of the resident unit-weight field, then the grid
+// stages into doseDev (accumulated). The CT, dose grid and nuclear model of e must be those of
+// the state's first run.
+static int engine_weights_update(pbs_engine *e, const BeamState &st, const float *weights, int numSpots,
+                                 float *doseDev) {
+    if (!e || !weights || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (numSpots != st.numSpots) return set_error(PBS_E_INVALID, "weight count differs from the resident beam's");
+    if (e->nuclearModel != 0) return set_error(PBS_E_INVALID, "resident beams do not cover the nuclear halo branch");
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->uploadPending) {
+        HIP_TRY(hipEventSynchronize(e->uploadDone));
+        e->uploadPending = false;
+    }
+    BeamPlan &p = e->plan;
+    p = st.plan;
+    std::fill(p.denseWeights.begin(), p.denseWeights.end(), 0.0f);
+    for (int i = 0; i < numSpots; ++i) {
+        const float w = weights[i];
+        if (!std::isfinite(w) || w < 0.0f) return set_error(PBS_E_INVALID, "spot weights must be finite and >= 0");
+        if (w == 0.0f) continue;
+        const int c = p.spotCell[size_t(i)];
+        if (c < 0) return set_error(PBS_E_GEOMETRY, "a weighted spot lies outside the resident beam's lattice");
+        p.denseWeights[size_t(c)] += w;
+    }
+    FanGeom &g = e->geom;
+    g = st.geom;
+    for (int i = 0; i < 3; ++i) g.doseDims[i] = e->doseDims[i];
+    const int L = p.numLayers, K = p.steps, rx = g.rx, ry = g.ry;
+    const size_t nray = size_t(rx) * ry, nsup = size_t(g.sx) * g.sy;
+    const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
+    HIP_TRY(e->weplMin.ensure(K));
+    HIP_TRY(e->rayW.ensure(nray * L));
+    HIP_TRY(e->iddSigma.ensure(nsup * K * L));
+    HIP_TRY(e->firstPassive.ensure(nray * L));
+    HIP_TRY(e->prim.ensure(L, K, g.sx, g.sy));
+    HIP_TRY(e->layerHost.ensure(L));
+    HIP_TRY(e->layerScal.ensure(L));
+    HIP_TRY(e->beamScal.ensure(1));
+    HIP_TRY(e->subEntry.ensure(size_t(L) * std::max(st.maxSub, 1) * 2));
+    HIP_TRY(e->spot.ensure(splane * L));
+    HIP_TRY(e->interm.ensure(size_t(L) * st.nSub * p.spotDims[1] * rx));
+    hipStream_t s = e->stream;
+    e->hostLayers = st.layerHost;
+    HIP_TRY(hipMemcpyAsync(e->layerHost.p, e->hostLayers.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->spot.p, p.denseWeights.data(), sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->beamScal.p, st.bs0.p, sizeof(BeamScalars), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->weplMin.p, st.weplMin.p, sizeof(float) * K, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
+                              st.subMode ? st.subspots.p : nullptr, st.subMode ? st.nSub : 0, e->subEntry.p, 1.0f,
+                              e->prim.classCount.p, e->prim.hist.p, s));
+    const ConvArgs a = make_conv_args(p, rx, ry, st.nSub, st.subMode);
+    HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, st.subspots.p,
+                        e->subEntry.p, L, s));
+    e->convArgs = a;
+    HIP_TRY(hipEventRecord(e->uploadDone, s));
+    e->uploadPending = true;
+    HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, st.unitIdd.p, st.unitFp.p,
+                           e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
+    hipStream_t streams[1 + kSideStreams] = {s};
+    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];
+    int rc = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev, streams,
+                      nullptr);
+    if (rc != PBS_OK) return rc;
+    e->lastHalo = false;
     e->haveLast = true;
     return PBS_OK;
 }
@@ -1210,3 +1350,552 @@ int pbs_batch_sync(pbs_batch *b) {
 }
 
 }  // extern "C\"
+
+// ---------------------------------------------------------------------------------------
+// served form: a request server with resident CTs and beams, admission by device-memory
+// footprint and continuous batching over engine slots
+// ---------------------------------------------------------------------------------------
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Deep copy of a pbs_beam_settings (the caller's arrays may go away after submit).
+struct BeamCopy {
+    pbs_beam_settings b{};
+    std::vector<float> energies, sigmas, subspots, positions, weights, dirs, cutoffs;
+    std::vector<int> counts, roi;
+    void set(const pbs_beam_settings &in) {
+        b = in;
+        const int L = in.numLayers;
+        auto cp = [](std::vector<float> &v, const float *p, size_t n) -> const float * {
+            if (!p) return nullptr;
+            v.assign(p, p + n);
+            return v.data();
+        };
+        b.energies = cp(energies, in.energies, size_t(L));
+        b.spotSigmas = cp(sigmas, in.spotSigmas, size_t(L) * 2);
+        b.subspotData = cp(subspots, in.subspotData, size_t(L) * std::max(in.maxSubspotsPerLayer, 0) * 5);
+        b.spotPositions = cp(positions, in.spotPositions, size_t(std::max(in.numSpots, 0)) * 2);
+        b.spotWeights = cp(weights, in.spotWeights, size_t(std::max(in.numSpots, 0)));
+        b.spotBeamDirections = cp(dirs, in.spotBeamDirections, size_t(std::max(in.numSpots, 0)) * 3);
+        b.layerLongitudinalCutoffs = cp(cutoffs, in.layerLongitudinalCutoffs, size_t(L));
+        if (in.layerSpotCounts) {
+            counts.assign(in.layerSpotCounts, in.layerSpotCounts + L);
+            b.layerSpotCounts = counts.data();
+        }
+        if (in.roiLinearIndices && in.numRoiLinearIndices > 0) {
+            roi.assign(in.roiLinearIndices, in.roiLinearIndices + in.numRoiLinearIndices);
+            b.roiLinearIndices = roi.data();
+        }
+    }
+};
+
+struct ServerCt {
+    DevBuf<float> ct;  // HU+1000, resident
+    int dims[3] = {0, 0, 0};
+    float res[3] = {0, 0, 0}, corner[3] = {0, 0, 0};
+    int users = 0;     // queued or running requests and resident beams on this CT
+};
+
+struct ServerReq {
+    long long ticket = 0;
+    int kind = 0;  // 0 full beam, 1 weight-only update of a resident beam
+    int ctId = -1;
+    int doseDims[3] = {0, 0, 0};
+    float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
+    BeamCopy beam;
+    bool keep = false;                 // full beam whose state stays resident (handle = ticket)
+    long long stateTicket = 0;         // weight-only: the resident beam
+    std::vector<float> weights;
+    int status = 0;                    // 0 queued, 1 running, 2 done, < 0 error code
+    std::string error;
+    int slot = -1;
+    float *dose = nullptr;
+    size_t doseN = 0;
+    size_t doseCharge = 0, stateCharge = 0;  // budget bytes held for the dose buffer / resident state
+    hipEvent_t done = nullptr;
+    double tSubmit = 0, tStart = 0, tDone = 0;
+};
+
+}  // namespace
+
+struct pbs_server {
+    int device = 0;
+    std::vector<pbs_engine *> slots;
+    std::vector<long long> slotTicket;   // running request per slot, 0 = free
+    std::vector<size_t> slotWork;        // device bytes each slot's engine holds for its work buffers
+    std::map<int, std::unique_ptr<ServerCt>> cts;
+    int nextCt = 1;
+    std::map<long long, std::unique_ptr<BeamState>> states;  // resident beams by their first ticket
+    std::map<long long, int> stateCt;
+    std::map<long long, std::unique_ptr<ServerReq>> reqs;
+    std::deque<long long> queue;
+    long long tickets = 0;
+    size_t budget = 0, used = 0;  // admission budget and bytes charged (work buffers, doses, states)
+    hipStream_t copyStream = nullptr;
+    std::vector<hipEvent_t> freeEvents;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv, doneCv;
+    bool stop = false;
+};
+
+namespace {
+
+size_t dose_bytes(const int d[3]) { return size_t(d[0]) * d[1] * d[2] * sizeof(float); }
+
+// Device bytes an engine's work buffers need for one beam (the GridWork and per-beam arrays of
+// engine_beam_dose), from the host plan.
+size_t work_bytes(const BeamPlan &p, int nSub) {
+    const size_t L = size_t(p.numLayers), K = size_t(p.steps);
+    const size_t nray = size_t(p.rayDims[0]) * p.rayDims[1], nsup = size_t(p.superpDims[0]) * p.superpDims[1];
+    const size_t nt = nsup / 256, maxItems = K * nt;
+    const size_t tailCap = std::min(L, K) * std::max<size_t>(1, (K + 1) / 2);
+    const size_t bev = size_t(p.superpDims[0] + 64) * (p.superpDims[1] + 64);
+    size_t b = 0;
+    b += (2 * nray * K + nray * L * 2 + K) * 4;                    // density, wepl, rayW, firstPassive, weplMin
+    b += nsup * K * L * 8;                                         // iddSigma
+    b += L * K * nt * 4 * 2;                                       // tile codes, slot layer lists
+    b += (maxItems + tailCap * nt) * kSlotFloats * 4;              // item blocks
+    b += (K + tailCap) * bev * 4;                                  // BEV planes
+    b += size_t(kNumClasses) * (maxItems + tailCap * nt) * 8;      // work-unit lists
+    b += size_t(p.spotDims[0]) * p.spotDims[1] * L * 4 * (1 + size_t(nSub));  // lattice, convolution
+    return b;
+}
+
+size_t state_bytes(const BeamPlan &p, int maxSub) {
+    const size_t L = size_t(p.numLayers), K = size_t(p.steps);
+    const size_t nray = size_t(p.rayDims[0]) * p.rayDims[1], nsup = size_t(p.superpDims[0]) * p.superpDims[1];
+    return nsup * K * L * 8 + nray * L * 4 + K * 4 + L * size_t(std::max(maxSub, 0)) * 20 + sizeof(BeamScalars);
+}
+
+void finish_error(pbs_server *sv, ServerReq &r, int code, const std::string &msg) {
+    if (r.kind == 0) sv->cts[r.ctId]->users--;
+    r.status = code;
+    r.error = msg;
+    r.tDone = now_ms();
+    sv->doneCv.notify_all();
+}
+
+// Admits the next queued request that fits the budget onto a free slot (called with mu held;
+// releases it while the beam is planned and queued). Returns true if one was started.
+bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
+    int slot = -1;
+    for (size_t i = 0; i < sv->slots.size(); ++i)
+        if (sv->slotTicket[i] == 0) { slot = int(i); break; }
+    if (slot < 0) return false;
+    for (size_t qi = 0; qi < sv->queue.size(); ++qi) {
+        ServerReq &r = *sv->reqs[sv->queue[qi]];
+        // footprint: the dose buffer, the resident state of a kept beam, and the growth of the
+        // slot's work buffers
+        BeamPlan plan;
+        int nSub = 1, maxSub = 0;
+        const BeamState *st = nullptr;
+        if (r.kind == 1) {
+            auto it = sv->states.find(r.stateTicket);
+            if (it == sv->states.end()) {
+                sv->queue.erase(sv->queue.begin() + long(qi));
+                finish_error(sv, r, PBS_E_INVALID, "unknown or released resident beam");
+                return true;
+            }
+            st = it->second.get();
+            plan = st->plan;
+            nSub = st->nSub;
+            maxSub = st->maxSub;
+        } else {
+            ServerCt &ct = *sv->cts[r.ctId];
+            if (!plan_beam(ct.dims, ct.res, ct.corner, r.doseDims, r.doseRes, r.doseCorner, r.beam.b,
+                           sv->slots[size_t(slot)]->energy, plan, nullptr)) {
+                sv->queue.erase(sv->queue.begin() + long(qi));
+                finish_error(sv, r, PBS_E_GEOMETRY, plan.error);
+                return true;
+            }
+            maxSub = r.beam.b.maxSubspotsPerLayer;
+            nSub = (plan.subspotConv && !plan.legacyPath) ? maxSub : 1;
+        }
+        const size_t need = work_bytes(plan, nSub);
+        const size_t grow = need > sv->slotWork[size_t(slot)] ? need - sv->slotWork[size_t(slot)] : 0;
+        const size_t doseB = dose_bytes(r.doseDims), stateB = r.keep ? state_bytes(plan, maxSub) : 0;
+        const size_t fp = doseB + stateB + grow;
+        if (sv->used + fp > sv->budget) {
+            bool anyRunning = false;
+            for (long long t : sv->slotTicket) anyRunning |= t != 0;
+            if (anyRunning) continue;  // wait for memory; a smaller request further back may fit
+            sv->queue.erase(sv->queue.begin() + long(qi));
+            finish_error(sv, r, PBS_E_NOMEM, "request exceeds the server's device-memory budget");
+            return true;
+        }
+        sv->used += fp;
+        sv->slotWork[size_t(slot)] += grow;  // work buffers stay with the slot
+        r.doseCharge = doseB;
+        r.stateCharge = stateB;
+        r.slot = slot;
+        r.status = 1;
+        r.tStart = now_ms();
+        sv->slotTicket[size_t(slot)] = r.ticket;
+        sv->queue.erase(sv->queue.begin() + long(qi));
+        ServerReq *rp = &r;
+        std::unique_ptr<BeamState> keepState;
+        if (r.keep) keepState.reset(new BeamState());
+        lk.unlock();
+        // plan and queue the beam on the slot's engine
+        pbs_engine *e = sv->slots[size_t(slot)];
+        int rc = PBS_OK;
+        std::string err;
+        hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&rp->dose), dose_bytes(rp->doseDims), e->stream);
+        if (he == hipSuccess) he = hipMemsetAsync(rp->dose, 0, dose_bytes(rp->doseDims), e->stream);
+        if (he != hipSuccess) {
+            rc = PBS_E_DEVICE;
+            err = hipGetErrorString(he);
+        }
+        rp->doseN = size_t(rp->doseDims[0]) * rp->doseDims[1] * rp->doseDims[2];
+        if (rc == PBS_OK) {
+            if (rp->kind == 1) {
+                (void)pbs_engine_set_dose_grid(e, rp->doseDims, rp->doseRes, rp->doseCorner);
+                ServerCt &ct = *sv->cts[rp->ctId];
+                (void)pbs_engine_set_ct(e, ct.ct.p, ct.dims, ct.res, ct.corner, 2);
+                rc = engine_weights_update(e, *st, rp->weights.data(), int(rp->weights.size()), rp->dose);
+            } else {
+                ServerCt &ct = *sv->cts[rp->ctId];
+                (void)pbs_engine_set_ct(e, ct.ct.p, ct.dims, ct.res, ct.corner, 2);
+                (void)pbs_engine_set_dose_grid(e, rp->doseDims, rp->doseRes, rp->doseCorner);
+                rc = engine_beam_dose(e, &rp->beam.b, rp->dose, nullptr, nullptr, nullptr, nullptr, keepState.get());
+            }
+            if (rc != PBS_OK) err = g_lastError;
+        }
+        if (rc == PBS_OK) {
+            if (!sv->freeEvents.empty()) {
+                rp->done = sv->freeEvents.back();
+                sv->freeEvents.pop_back();
+            } else if (hipEventCreateWithFlags(&rp->done, hipEventDisableTiming) != hipSuccess) {
+                rp->done = nullptr;
+            }
+            if (!rp->done || hipEventRecord(rp->done, e->stream) != hipSuccess) {
+                rc = PBS_E_DEVICE;
+                err = "hipEventRecord failed";
+            }
+        }
+        // the engine has planned and queued the beam: the request's host copies are no longer needed
+        rp->beam = BeamCopy();
+        std::vector<float>().swap(rp->weights);
+        lk.lock();
+        if (rc != PBS_OK) {
+            (void)hipStreamSynchronize(e->stream);
+            sv->slotTicket[size_t(slot)] = 0;
+            if (rp->dose) (void)hipFreeAsync(rp->dose, e->stream);
+            rp->dose = nullptr;
+            sv->used -= rp->doseCharge + rp->stateCharge;
+            rp->doseCharge = rp->stateCharge = 0;
+            finish_error(sv, *rp, rc, err);
+        } else if (rp->keep) {
+            sv->states[rp->ticket] = std::move(keepState);
+            sv->stateCt[rp->ticket] = rp->ctId;
+            sv->cts[rp->ctId]->users++;
+        }
+        return true;
+    }
+    return false;
+}
+
+// Collects finished requests (mu held): frees their slot; the dose stays until fetched.
+bool collect(pbs_server *sv) {
+    bool any = false;
+    for (size_t i = 0; i < sv->slots.size(); ++i) {
+        const long long t = sv->slotTicket[i];
+        if (!t) continue;
+        ServerReq &r = *sv->reqs[t];
+        const hipError_t q = hipEventQuery(r.done);
+        if (q == hipErrorNotReady) continue;
+        r.tDone = now_ms();
+        if (q != hipSuccess) {
+            r.status = PBS_E_DEVICE;
+            r.error = hipGetErrorString(q);
+        } else {
+            r.status = 2;
+        }
+        sv->freeEvents.push_back(r.done);
+        r.done = nullptr;
+        sv->slotTicket[i] = 0;
+        if (r.kind == 0) sv->cts[r.ctId]->users--;
+        any = true;
+    }
+    if (any) sv->doneCv.notify_all();
+    return any;
+}
+
+void worker_loop(pbs_server *sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (!sv->stop) {
+        bool progress = collect(sv);
+        while (admit_one(sv, lk)) progress = true;
+        if (!progress) {
+            bool running = false;
+            for (long long t : sv->slotTicket) running |= t != 0;
+            // poll the running requests' events; sleep until a submit when idle
+            if (running)
+                sv->cv.wait_for(lk, std::chrono::microseconds(20));
+            else
+                sv->cv.wait_for(lk, std::chrono::milliseconds(50));
+        }
+    }
+}
+
+int server_request_status(pbs_server *sv, long long ticket, ServerReq **out) {
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return set_error(PBS_E_INVALID, "unknown ticket");
+    *out = it->second.get();
+    return PBS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pbs_server *pbs_server_create(int gpuId, const pbs_energy_struct *energy, int slots, double memoryBudgetBytes) {
+    if (slots < 1 || slots > 16) {
+        set_error(PBS_E_INVALID, "slots must be 1..16");
+        return nullptr;
+    }
+    if (hipSetDevice(gpuId) != hipSuccess) {
+        set_error(PBS_E_DEVICE, "hipSetDevice failed");
+        return nullptr;
+    }
+    std::unique_ptr<pbs_server> sv(new pbs_server());
+    sv->device = gpuId;
+    for (int i = 0; i < slots; ++i) {
+        pbs_engine *e = pbs_engine_create(gpuId, energy);
+        if (!e) {
+            for (pbs_engine *x : sv->slots) pbs_engine_destroy(x);
+            return nullptr;
+        }
+        sv->slots.push_back(e);
+    }
+    sv->slotTicket.assign(size_t(slots), 0);
+    sv->slotWork.assign(size_t(slots), 0);
+    size_t freeB = 0, totalB = 0;
+    (void)hipMemGetInfo(&freeB, &totalB);
+    sv->budget = memoryBudgetBytes > 0 ? size_t(memoryBudgetBytes) : size_t(0.8 * double(freeB));
+    if (hipStreamCreateWithFlags(&sv->copyStream, hipStreamNonBlocking) != hipSuccess) {
+        for (pbs_engine *x : sv->slots) pbs_engine_destroy(x);
+        set_error(PBS_E_DEVICE, "hipStreamCreate failed");
+        return nullptr;
+    }
+    pbs_server *raw = sv.release();
+    raw->worker = std::thread(worker_loop, raw);
+    return raw;
+}
+
+void pbs_server_destroy(pbs_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->device);
+    for (pbs_engine *e : sv->slots) (void)hipStreamSynchronize(e->stream);
+    for (auto &kv : sv->reqs) {
+        if (kv.second->dose) (void)hipFree(kv.second->dose);
+        if (kv.second->done) (void)hipEventDestroy(kv.second->done);
+    }
+    for (hipEvent_t ev : sv->freeEvents) (void)hipEventDestroy(ev);
+    sv->states.clear();
+    for (pbs_engine *e : sv->slots) pbs_engine_destroy(e);
+    if (sv->copyStream) (void)hipStreamDestroy(sv->copyStream);
+    delete sv;
+}
+
+int pbs_server_add_ct(pbs_server *sv, const float *ct, const int ctDims[3], const float ctRes[3],
+                      const float ctCorner[3], int ctIsDevice, int *ctId) {
+    if (!sv || !ct || !ctDims || !ctRes || !ctCorner || !ctId) return set_error(PBS_E_INVALID, "null argument");
+    // the CT adapter of the engine API, then a resident copy owned by the server
+    std::unique_ptr<ServerCt> c(new ServerCt());
+    const size_t n = size_t(ctDims[0]) * ctDims[1] * ctDims[2];
+    std::vector<float> host;
+    const float *src = ct;
+    HIP_TRY(hipSetDevice(sv->device));
+    if (ctIsDevice) {
+        host.resize(n);
+        HIP_TRY(hipMemcpy(host.data(), ct, n * sizeof(float), hipMemcpyDeviceToHost));
+        src = host.data();
+    }
+    const int type = detect_ct_type(src, n);
+    if (type == 2) return set_error(PBS_E_UNSUPPORTED, "density-valued CT input: pass HU or HU+1000");
+    std::vector<float> conv;
+    if (type == 0) {
+        conv.resize(n);
+        for (size_t i = 0; i < n; ++i) conv[i] = src[i] + 1000.0f;
+        src = conv.data();
+    }
+    HIP_TRY(c->ct.ensure(n));
+    HIP_TRY(hipMemcpy(c->ct.p, src, n * sizeof(float), hipMemcpyHostToDevice));
+    for (int i = 0; i < 3; ++i) { c->dims[i] = ctDims[i]; c->res[i] = ctRes[i]; c->corner[i] = ctCorner[i]; }
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->used += n * sizeof(float);
+    *ctId = sv->nextCt++;
+    sv->cts[*ctId] = std::move(c);
+    return PBS_OK;
+}
+
+int pbs_server_remove_ct(pbs_server *sv, int ctId) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->cts.find(ctId);
+    if (it == sv->cts.end()) return set_error(PBS_E_INVALID, "unknown CT");
+    if (it->second->users > 0) return set_error(PBS_E_INVALID, "CT in use by requests or resident beams");
+    sv->used -= it->second->ct.n * sizeof(float);
+    sv->cts.erase(it);
+    return PBS_OK;
+}
+
+int pbs_server_submit(pbs_server *sv, int ctId, const int doseDims[3], const float doseRes[3],
+                      const float doseCorner[3], const pbs_beam_settings *beam, int keepState, long long *ticket) {
+    if (!sv || !doseDims || !doseRes || !doseCorner || !beam || !ticket) return set_error(PBS_E_INVALID, "null argument");
+    int rc = validate_beam(*beam, 0);
+    if (rc != PBS_OK) return rc;
+    if (doseDims[0] <= 0 || doseDims[1] <= 0 || doseDims[2] <= 0) return set_error(PBS_E_INVALID, "doseDims must be positive");
+    std::unique_ptr<ServerReq> r(new ServerReq());
+    r->kind = 0;
+    r->ctId = ctId;
+    for (int i = 0; i < 3; ++i) { r->doseDims[i] = doseDims[i]; r->doseRes[i] = doseRes[i]; r->doseCorner[i] = doseCorner[i]; }
+    r->beam.set(*beam);
+    r->keep = keepState != 0;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto it = sv->cts.find(ctId);
+        if (it == sv->cts.end()) return set_error(PBS_E_INVALID, "unknown CT");
+        it->second->users++;
+        r->ticket = ++sv->tickets;
+        r->tSubmit = now_ms();
+        *ticket = r->ticket;
+        sv->queue.push_back(r->ticket);
+        sv->reqs[r->ticket] = std::move(r);
+    }
+    sv->cv.notify_all();
+    return PBS_OK;
+}
+
+int pbs_server_submit_weights(pbs_server *sv, long long stateTicket, const float *spotWeights, int numSpots,
+                              long long *ticket) {
+    if (!sv || !spotWeights || !ticket) return set_error(PBS_E_INVALID, "null argument");
+    std::unique_ptr<ServerReq> r(new ServerReq());
+    r->kind = 1;
+    r->stateTicket = stateTicket;
+    r->weights.assign(spotWeights, spotWeights + std::max(numSpots, 0));
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        auto st = sv->states.find(stateTicket);
+        if (st == sv->states.end()) return set_error(PBS_E_INVALID, "unknown or released resident beam");
+        if (numSpots != st->second->numSpots) return set_error(PBS_E_INVALID, "weight count differs from the resident beam's");
+        const ServerReq &first = *sv->reqs[stateTicket];
+        r->ctId = sv->stateCt[stateTicket];
+        for (int i = 0; i < 3; ++i) {
+            r->doseDims[i] = first.doseDims[i];
+            r->doseRes[i] = first.doseRes[i];
+            r->doseCorner[i] = first.doseCorner[i];
+        }
+        r->ticket = ++sv->tickets;
+        r->tSubmit = now_ms();
+        *ticket = r->ticket;
+        sv->queue.push_back(r->ticket);
+        sv->reqs[r->ticket] = std::move(r);
+    }
+    sv->cv.notify_all();
+    return PBS_OK;
+}
+
+int pbs_server_query(pbs_server *sv, long long ticket) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    ServerReq *r = nullptr;
+    int rc = server_request_status(sv, ticket, &r);
+    if (rc != PBS_OK) return rc;
+    if (r->status < 0) return set_error(r->status, r->error);
+    return r->status == 2 ? 1 : 0;
+}
+
+int pbs_server_wait(pbs_server *sv, long long ticket) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    ServerReq *r = nullptr;
+    int rc = server_request_status(sv, ticket, &r);
+    if (rc != PBS_OK) return rc;
+    sv->doneCv.wait(lk, [&] { return r->status == 2 || r->status < 0; });
+    if (r->status < 0) return set_error(r->status, r->error);
+    return PBS_OK;
+}
+
+int pbs_server_result(pbs_server *sv, long long ticket, float *doseHost) {
+    if (!sv || !doseHost) return set_error(PBS_E_INVALID, "null argument");
+    int rc = pbs_server_wait(sv, ticket);
+    if (rc != PBS_OK) return rc;
+    float *dose = nullptr;
+    size_t n = 0, fp = 0;
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        ServerReq &r = *sv->reqs[ticket];
+        if (!r.dose) return set_error(PBS_E_INVALID, "result already fetched");
+        dose = r.dose;
+        n = r.doseN;
+        fp = r.doseCharge;
+        r.dose = nullptr;
+        r.doseCharge = 0;
+    }
+    HIP_TRY(hipMemcpyAsync(doseHost, dose, n * sizeof(float), hipMemcpyDeviceToHost, sv->copyStream));
+    HIP_TRY(hipFreeAsync(dose, sv->copyStream));
+    HIP_TRY(hipStreamSynchronize(sv->copyStream));
+    {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->used -= std::min(fp, sv->used);
+    }
+    sv->cv.notify_all();
+    return PBS_OK;
+}
+
+int pbs_server_timing(pbs_server *sv, long long ticket, double *queueMs, double *totalMs) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    ServerReq *r = nullptr;
+    int rc = server_request_status(sv, ticket, &r);
+    if (rc != PBS_OK) return rc;
+    if (queueMs) *queueMs = r->tStart > 0 ? r->tStart - r->tSubmit : -1.0;
+    if (totalMs) *totalMs = r->tDone > 0 ? r->tDone - r->tSubmit : -1.0;
+    return PBS_OK;
+}
+
+int pbs_server_release_state(pbs_server *sv, long long stateTicket) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    auto it = sv->states.find(stateTicket);
+    if (it == sv->states.end()) return set_error(PBS_E_INVALID, "unknown resident beam");
+    for (long long q : sv->queue)
+        if (sv->reqs[q]->kind == 1 && sv->reqs[q]->stateTicket == stateTicket)
+            return set_error(PBS_E_INVALID, "updates of this beam are still queued");
+    for (long long t : sv->slotTicket)
+        if (t && sv->reqs[t]->kind == 1 && sv->reqs[t]->stateTicket == stateTicket)
+            return set_error(PBS_E_INVALID, "an update of this beam is running");
+    for (pbs_engine *e : sv->slots) (void)hipStreamSynchronize(e->stream);
+    auto first = sv->reqs.find(stateTicket);
+    const size_t b = first != sv->reqs.end() ? first->second->stateCharge : 0;
+    if (first != sv->reqs.end()) first->second->stateCharge = 0;
+    sv->used -= std::min(b, sv->used);
+    sv->cts[sv->stateCt[stateTicket]]->users--;
+    sv->states.erase(it);
+    sv->stateCt.erase(stateTicket);
+    return PBS_OK;
+}
+
+int pbs_server_memory(pbs_server *sv, double *usedBytes, double *budgetBytes) {
+    if (!sv) return set_error(PBS_E_INVALID, "null server");
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (usedBytes) *usedBytes = double(sv->used);
+    if (budgetBytes) *budgetBytes = double(sv->budget);
+    return PBS_OK;
+}
+
+}  // extern "C"

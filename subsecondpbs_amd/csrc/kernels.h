@@ -30,7 +30,8 @@ struct CpbArgs {
 
 struct FillConsts {
     float eRefSq, sigmaDelta;
-    int suppressDistalDip, pad;
+    int suppressDistalDip;
+    int unitWeight;  // 1: every ray weight is taken as 1 (the weight-independent field of a resident beam)
 };
 
 struct TransferArgs {
@@ -56,6 +57,12 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
                        const BeamScalars *bs, const float *density, const float *wepl, const float *rayW, const int *fi,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
                        int *tileCode, hipStream_t s);
+// Weight rescale of a resident beam's unit-weight field: IDD = W x unit IDD on rays with
+// W >= 1e-6 (else 0 / inf), firstPassive, the tile codes and the layer passive step, as the fill
+// writes them for weights W.
+hipError_t launch_rescale(const FanGeom &g, LayerScalars *ls, const BeamScalars *bs,
+                          const float *rayW, const float2 *unitIdd, const int *unitFp, float2 *iddSigma,
+                          int *firstPassive, int *tileCode, hipStream_t s);
 hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
                             BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s);
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s);

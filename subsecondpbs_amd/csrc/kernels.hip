@@ -531,7 +531,7 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
     const unsigned pAfter = unsigned(ls[l].afterLast);
     const unsigned firstIn = rayIn ? unsigned(fi[ray]) : 0u;
     unsigned afterLast = rayIn ? unsigned(min(fo[ray], int(pAfter))) : 0u;
-    const float w0 = rayIn ? rayW[size_t(l) * nray + ray] : 0.0f;
+    const float w0 = rayIn ? (fc.unitWeight ? 1.0f : rayW[size_t(l) * nray + ray]) : 0.0f;
     bool live = rayIn;
     if (w0 < 1e-6f || afterLast < first) { live = false; afterLast = 0; }
     float spOld = 0.f, cdOld = 0.f;
@@ -696,6 +696,68 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
         passive = int(afterLast);
     }
     // layer passive step = max over rays (wrapper:4950-4979): one atomic per wave
+    const int mx = wave_max_i(passive);
+    if (lane == 0) atomicMax(&ls[l].passiveRaw, mx);
+}
+
+// Weight rescale of a resident beam (the served form's weight-only update): the fill's
+// recurrence does not depend on the ray weight, which only scales the IDD and decides the live
+// mask (w >= 1e-6, idd_sigma.cu:78). From the unit-weight field it writes, per ray and step,
+// IDD = W x unit IDD and rSigmaEff (inf on dead rays), firstPassive, the tile codes and the
+// layer's passive step, as the fill does for weights W. One block per (32x8 tile, layer).
+__global__ __launch_bounds__(256) void k_rescale(FanGeom g, LayerScalars *__restrict__ ls,
+                                                 const BeamScalars *__restrict__ bs, const float *__restrict__ rayW,
+                                                 const float2 *__restrict__ unitIdd, const int *__restrict__ unitFp,
+                                                 float2 *__restrict__ iddSigma, int *__restrict__ firstPassive,
+                                                 int *__restrict__ tileCode) {
+    __shared__ float s_min[2][kFillChunk][4];
+    __shared__ int s_live[2][kFillChunk][4];
+    const int ntx = g.sx / 32, nt = ntx * (g.sy / 8);
+    const int t = blockIdx.x, l = blockIdx.y;
+    const int x = (t % ntx) * 32 + (threadIdx.x & 31), y = (t / ntx) * 8 + (threadIdx.x >> 5);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool rayIn = x < g.rx && y < g.ry;
+    const int nray = g.rx * g.ry;
+    const int ray = rayIn ? y * g.rx + x : 0;
+    const unsigned first = unsigned(bs->beamFirstInside);
+    const unsigned pAfter = unsigned(ls[l].afterLast);
+    const float W = rayIn ? rayW[size_t(l) * nray + ray] : 0.0f;
+    const bool live = rayIn && W >= 1e-6f;
+    const size_t outStep = size_t(g.sx) * g.sy;
+    const size_t base = (size_t(l) * g.K * g.sy + y) * g.sx + x;
+    int *codes = tileCode + size_t(l) * g.K * nt + t;
+    for (unsigned kc = first, cb = 0; kc < pAfter; kc += kFillChunk, cb ^= 1u) {
+        const int n = min(int(pAfter - kc), kFillChunk);
+#pragma unroll
+        for (int u = 0; u < kFillChunk; ++u) {
+            if (u < n) {
+                const size_t i = base + size_t(kc + u) * outStep;
+                const float2 v = unitIdd[i];
+                const float res = live ? W * v.x : 0.0f;
+                const float rs = live ? v.y : PBS_INF;
+                iddSigma[i] = make_float2(res, rs);
+                const float m = wave_min(rs);
+                const bool any = __any(res > 0.0f);
+                if (lane == 0) {
+                    s_min[cb][u][w] = m;
+                    s_live[cb][u][w] = any ? 1 : 0;
+                }
+            }
+        }
+        __syncthreads();
+        if (int(threadIdx.x) < n) {
+            const int u = threadIdx.x;
+            const float mm = fminf(fminf(s_min[cb][u][0], s_min[cb][u][1]), fminf(s_min[cb][u][2], s_min[cb][u][3]));
+            const int lv = s_live[cb][u][0] | s_live[cb][u][1] | s_live[cb][u][2] | s_live[cb][u][3];
+            codes[size_t(kc + u) * nt] = tile_radius(mm) | (lv ? 256 : 0);
+        }
+    }
+    int passive = INT_MIN;
+    if (rayIn) {
+        const int fp = live ? unitFp[size_t(l) * nray + ray] : 0;
+        firstPassive[size_t(l) * nray + ray] = fp;
+        passive = fp;
+    }
     const int mx = wave_max_i(passive);
     if (lane == 0) atomicMax(&ls[l].passiveRaw, mx);
 }
@@ -1712,6 +1774,15 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
     else
         hipLaunchKernelGGL(k_fill<false>, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl,
                            rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescale(const FanGeom &g, LayerScalars *ls, const BeamScalars *bs,
+                          const float *rayW, const float2 *unitIdd, const int *unitFp, float2 *iddSigma,
+                          int *firstPassive, int *tileCode, hipStream_t s) {
+    const int nt = (g.sx / 32) * (g.sy / 8);
+    hipLaunchKernelGGL(k_rescale, dim3(nt, g.numLayers), dim3(256), 0, s, g, ls, bs, rayW, unitIdd, unitFp, iddSigma,
+                       firstPassive, tileCode);
     return hipGetLastError();
 }
 
