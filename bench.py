@@ -109,8 +109,153 @@ def cpu_baseline(case: dict, runs: int) -> dict:
                       f"median {med:.2f} s per beam"}
 
 
+def served_mode(args) -> int:
+    """Served vs plain on the serving mix (config 5): the same request list through the plain
+    engine one request at a time (one engine per CT, doses left on the device) and through the
+    request server (all requests submitted at once; resident beams for the ones that later get
+    weight-only updates), then an open-loop Poisson stream for latency percentiles."""
+    import torch
+
+    from subsecondpbs_amd import fixtures
+    from subsecondpbs_amd.engine import Engine
+    from subsecondpbs_amd.server import Server
+
+    cfg = fixtures.make_config5(n_requests=args.requests)
+    reqs, cts = cfg["requests"], cfg["cts"]
+    engines = []
+    for c in cts:
+        e = Engine(cfg["energy"])
+        e.set_ct(c["ct"], c["ct_dims"], c["ct_resolution"], c["ct_corner"])
+        e.set_dose_grid(c["dose_dims"], c["dose_resolution"], c["dose_corner"])
+        engines.append(e)
+
+    def plain_pass(keep_results):
+        lat, out = [], []
+        t0 = time.perf_counter()
+        for r in reqs:
+            c = cts[r["ct"]]
+            nx, ny, nz = c["dose_dims"]
+            ts = time.perf_counter()
+            d = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+            engines[r["ct"]].beam_dose(r["beam"], d)
+            engines[r["ct"]].sync()
+            lat.append((time.perf_counter() - ts) * 1e3)
+            if keep_results:
+                out.append(d)
+        return time.perf_counter() - t0, lat, out
+
+    plain_pass(False)  # warm-up: work buffers sized
+    for e in engines:
+        e.set_profiling(True)
+        e.stage_times()
+    plain_s, plain_lat, plain_res = plain_pass(True)
+    gpu_ms = 0.0
+    stage_sum = {}
+    for e in engines:
+        st, _ = e.stage_times()
+        for k, v in st.items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+        gpu_ms += sum(st.values())
+        e.close()
+
+    sv = Server(cfg["energy"], slots=args.slots)
+    cids = [sv.add_ct(c["ct"], c["ct_dims"], c["ct_resolution"], c["ct_corner"]) for c in cts]
+    kept = {r["update_of"] for r in reqs if r["update_of"] >= 0}
+
+    def submit(i, handles):
+        r = reqs[i]
+        c = cts[r["ct"]]
+        if r["update_of"] >= 0:
+            return sv.submit_weights(handles[r["update_of"]], r["beam"]["spot_weights"])
+        t = sv.submit(cids[r["ct"]], c["dose_dims"], c["dose_resolution"], c["dose_corner"], r["beam"],
+                      keep=i in kept)
+        if i in kept:
+            handles[i] = t
+        return t
+
+    def served_pass():
+        handles = {}
+        t0 = time.perf_counter()
+        tickets = [submit(i, handles) for i in range(len(reqs))]
+        res = [sv.result(t, device=True) for t in tickets]
+        total = time.perf_counter() - t0
+        lat = [sv.timing(t)[1] for t in tickets]
+        for h in handles.values():
+            sv.release(h)
+        return total, lat, res
+
+    served_pass()  # warm-up: slot work buffers sized
+    served_s, served_lat, served_res = served_pass()
+    err = max(float((a - b).abs().max().item()) / max(float(b.max().item()), 1e-30)
+              for a, b in zip(served_res, plain_res))
+    del served_res, plain_res
+
+    # open-loop Poisson arrivals at `rate` requests/s
+    rng = np.random.default_rng(5)
+    gaps = rng.exponential(1.0 / args.rate, len(reqs))
+    handles = {}
+    t0 = time.perf_counter()
+    tickets = []
+    for i in range(len(reqs)):
+        due = t0 + float(np.sum(gaps[: i + 1]))
+        while time.perf_counter() < due:
+            time.sleep(min(1e-4, max(0.0, due - time.perf_counter())))
+        tickets.append(submit(i, handles))
+    for t in tickets:
+        sv.result(t, device=True)
+    open_lat = [sv.timing(t)[1] for t in tickets]
+    for h in handles.values():
+        sv.release(h)
+    used, budget = sv.memory()
+    sv.close()
+
+    n = len(reqs)
+    plain_bps, served_bps = n / plain_s, n / served_s
+    out = {
+        "metric": METRIC,
+        "value": served_bps,
+        "unit": "beams/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": n,
+        "ms_per_step": 1e3 * served_s / n,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"serving mix: {n} requests over CTs 128^3, 256x256x160, 512x512x200; "
+                               "500-8000 spots, 10-40 layers, random gantry; "
+                               f"{len(kept)} resident beams, {sum(r['update_of'] >= 0 for r in reqs)} weight-only updates",
+                   "form": "served", "slots": args.slots},
+        "plain_beams_per_s": plain_bps,
+        "served_over_plain": served_bps / plain_bps,
+        "plain_p50_ms": float(np.percentile(plain_lat, 50)),
+        "plain_p99_ms": float(np.percentile(plain_lat, 99)),
+        "served_closed_loop_p50_ms": float(np.percentile(served_lat, 50)),
+        "served_closed_loop_p99_ms": float(np.percentile(served_lat, 99)),
+        "open_loop_rate_per_s": args.rate,
+        "open_loop_p50_ms": float(np.percentile(open_lat, 50)),
+        "open_loop_p99_ms": float(np.percentile(open_lat, 99)),
+        "max_dose_err_served_vs_plain": err,
+        "plain_gpu_stage_ms_total": gpu_ms,
+        "plain_gpu_stage_ms": {k: round(v, 3) for k, v in stage_sum.items()},
+        "served_gpu_busy_fraction_upper_bound": gpu_ms / (1e3 * served_s),
+        "server_memory_charged_gb": used / 1e9,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=("plain", "served", "optimize"), default="plain",
+                    help="plain: the head-size beam one request at a time (default); served: the serving "
+                         "mix through the request server vs the plain engine; optimize: the config-4 "
+                         "spot-weight optimisation loop")
+    ap.add_argument("--requests", type=int, default=48, help="served mode: requests in the mix")
+    ap.add_argument("--slots", type=int, default=4, help="served mode: engine slots of the server")
+    ap.add_argument("--rate", type=float, default=100.0, help="served mode: open-loop arrival rate (req/s)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -124,6 +269,16 @@ def main() -> int:
         print("bench.py: this engine runs on one GPU per process; multi-GPU runs are not part of "
               "this build (beams would shard across GPUs with one dose reduction).", file=sys.stderr)
         return 2
+
+    if args.mode == "served":
+        return served_mode(args)
+    if args.mode == "optimize":
+        from subsecondpbs_amd.optimize import run
+
+        out = run(steps=args.steps)
+        out["objective"] = [round(v, 6) for v in out["objective"]]
+        print(json.dumps(out))
+        return 0
 
     import torch
 

@@ -150,7 +150,8 @@ int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float 
  * [ry][rx]; weplMin [K]; rayWeights [L][ry][rx]; iddSigma [L][K][sy][sx][2] (IDD, rSigmaEff;
  * only steps [beamFirstInside, afterLast_l) are defined); firstPassive [L][ry][rx];
  * tileRad [L][K][sy/8][sx/32] (-1 where not computed); radHist [L][34]; bev
- * [bevZ][sy+64][sx+64]; ints: see pbs_debug_ints. */
+ * [bevZ][sy+64][sx+64] (main BEV; only planes [beamFirstInside, mainEndMax) are defined, the
+ * transfer reads the others as 0); ints: see pbs_debug_ints. */
 typedef struct pbs_debug_ints {
     int rayDims[2], superpDims[2], steps, numLayers, legacyPath;
     int beamFirstInside, beamFirstOutside, firstPastCutoffAll, guaranteedPassive;
@@ -250,6 +251,8 @@ int pbs_server_query(pbs_server *s, long long ticket);
 int pbs_server_wait(pbs_server *s, long long ticket);
 /* Waits, copies the dose ([nz][ny][nx] float32) to host and frees the request's dose buffer. */
 int pbs_server_result(pbs_server *s, long long ticket, float *doseHost);
+/* The same into a device buffer of the server's GPU (no host transfer). */
+int pbs_server_result_device(pbs_server *s, long long ticket, float *doseDevice);
 /* Milliseconds from submit to admission and from submit to completion (-1 while pending). */
 int pbs_server_timing(pbs_server *s, long long ticket, double *queueMs, double *totalMs);
 /* Frees a resident beam (no update of it may be queued or running). */

@@ -86,7 +86,8 @@ EXPORTED = [
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
     "pbs_server_create", "pbs_server_destroy", "pbs_server_add_ct", "pbs_server_remove_ct", "pbs_server_submit",
-    "pbs_server_submit_weights", "pbs_server_query", "pbs_server_wait", "pbs_server_result", "pbs_server_timing",
+    "pbs_server_submit_weights", "pbs_server_query", "pbs_server_wait", "pbs_server_result",
+    "pbs_server_result_device", "pbs_server_timing",
     "pbs_server_release_state", "pbs_server_memory",
 ]
 
@@ -169,6 +170,8 @@ def lib():
             getattr(L, fn).argtypes = [C.c_void_p, C.c_longlong]
         L.pbs_server_result.restype = C.c_int
         L.pbs_server_result.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p]
+        L.pbs_server_result_device.restype = C.c_int
+        L.pbs_server_result_device.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p]
         L.pbs_server_timing.restype = C.c_int
         L.pbs_server_timing.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.pbs_server_memory.restype = C.c_int

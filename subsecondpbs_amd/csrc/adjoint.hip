@@ -19,16 +19,16 @@
 namespace pbs {
 
 // Scatter g into the in-range corners of a trilinear sample (the transpose of the border-0
-// trilinear gather of the transfer).
-__device__ __forceinline__ void trilinear_border_adj(float *__restrict__ v, int nx, int ny, int nz, float x, float y,
-                                                     float z, float g) {
+// trilinear gather of the transfer; planes outside [zlo, zhi) read as 0 there).
+__device__ __forceinline__ void trilinear_border_adj(float *__restrict__ v, int nx, int ny, int zlo, int zhi, float x,
+                                                     float y, float z, float g) {
     const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
     const float tx = x - fx, ty = y - fy, tz = z - fz;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int xi = x0 + (q & 1), yi = y0 + ((q >> 1) & 1), zi = z0 + (q >> 2);
-        if (unsigned(xi) < unsigned(nx) && unsigned(yi) < unsigned(ny) && unsigned(zi) < unsigned(nz)) {
+        if (unsigned(xi) < unsigned(nx) && unsigned(yi) < unsigned(ny) && zi >= zlo && zi < zhi) {
             const float w = ((q & 1) ? tx : 1.0f - tx) * (((q >> 1) & 1) ? ty : 1.0f - ty) * ((q >> 2) ? tz : 1.0f - tz);
             unsafeAtomicAdd(v + (size_t(zi) * ny + yi) * nx + xi, w * g);
         }
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta
     const float fzI = pz + (ta.globalOffset[2] - float(inside));
     if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return;
     if (fxI > float(bx) || fyI > float(by)) return;
-    if (fzI <= float(g.K)) trilinear_border_adj(gBev, bx, by, g.K, fxI, fyI, fzI, gv);
+    if (fzI <= float(g.K)) trilinear_border_adj(gBev, bx, by, inside, min(bs->mainEndMax, g.K), fxI, fyI, fzI, gv);
     for (int j = 0; j < m; ++j) {
         const GroupScalars &G = gs[j];
         if (G.tailCount <= 0) continue;
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta
             z >= G.boxEnd[2])
             continue;
         const int nz = min(G.tailCount, bs->tailPlanes - G.tailBase);  // planes that exist
-        if (nz > 0) trilinear_border_adj(gTailBev + size_t(G.tailBase) * bx * by, bx, by, nz, fxI, fyI, zl, gv);
+        if (nz > 0) trilinear_border_adj(gTailBev + size_t(G.tailBase) * bx * by, bx, by, 0, nz, fxI, fyI, zl, gv);
     }
 }
 

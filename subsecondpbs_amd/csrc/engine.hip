@@ -151,6 +151,9 @@ struct pbs_engine {
     hipStream_t side[kSideStreams] = {};
     hipEvent_t forkEv = nullptr, joinEv[kSideStreams] = {};
     bool uploadPending = false;
+    // all work on the engine stream (no side-stream fork for the radius classes): the request
+    // server's slots overlap whole requests instead
+    bool oneStream = false;
     // optional per-stage timing: kStages + 1 events per beam, resolved by pbs_engine_stage_times
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -529,13 +532,19 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
                                       w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, codesFromFill ? 1 : 0,
                                       w.planeRM.p, w.slotLayers.p, s));
     HIP_TRY(mk());
-    HIP_TRY(hipEventRecord(e->forkEv, s));
-    for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
-    HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
-                             w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, streams));
-    for (int i = 0; i < kSideStreams; ++i) {
-        HIP_TRY(hipEventRecord(e->joinEv[i], streams[1 + i]));
-        HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
+    if (e->oneStream) {
+        const hipStream_t one[1 + kSideStreams] = {s, s, s, s};
+        HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
+                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, one));
+    } else {
+        HIP_TRY(hipEventRecord(e->forkEv, s));
+        for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
+        HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
+                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, streams));
+        for (int i = 0; i < kSideStreams; ++i) {
+            HIP_TRY(hipEventRecord(e->joinEv[i], streams[1 + i]));
+            HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
+        }
     }
     HIP_TRY(mk());
     HIP_TRY(launch_merge(g, bs, w.itemR.p, w.slots.p, w.bev.p, w.tailItemR.p, w.tailSlots.p, w.tailBev.p, w.tailCap,
@@ -1408,6 +1417,7 @@ struct ServerReq {
     float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
     BeamCopy beam;
     bool keep = false;                 // full beam whose state stays resident (handle = ticket)
+    int numSpots = 0;
     long long stateTicket = 0;         // weight-only: the resident beam
     std::vector<float> weights;
     int status = 0;                    // 0 queued, 1 running, 2 done, < 0 error code
@@ -1495,6 +1505,9 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
         int nSub = 1, maxSub = 0;
         const BeamState *st = nullptr;
         if (r.kind == 1) {
+            // an update reads the resident field its beam's first run writes: it starts after that run
+            auto base = sv->reqs.find(r.stateTicket);
+            if (base != sv->reqs.end() && base->second->status >= 0 && base->second->status != 2) continue;
             auto it = sv->states.find(r.stateTicket);
             if (it == sv->states.end()) {
                 sv->queue.erase(sv->queue.begin() + long(qi));
@@ -1672,6 +1685,7 @@ pbs_server *pbs_server_create(int gpuId, const pbs_energy_struct *energy, int sl
             for (pbs_engine *x : sv->slots) pbs_engine_destroy(x);
             return nullptr;
         }
+        e->oneStream = true;
         sv->slots.push_back(e);
     }
     sv->slotTicket.assign(size_t(slots), 0);
@@ -1765,6 +1779,7 @@ int pbs_server_submit(pbs_server *sv, int ctId, const int doseDims[3], const flo
     for (int i = 0; i < 3; ++i) { r->doseDims[i] = doseDims[i]; r->doseRes[i] = doseRes[i]; r->doseCorner[i] = doseCorner[i]; }
     r->beam.set(*beam);
     r->keep = keepState != 0;
+    r->numSpots = beam->numSpots;
     {
         std::lock_guard<std::mutex> g(sv->mu);
         auto it = sv->cts.find(ctId);
@@ -1789,11 +1804,16 @@ int pbs_server_submit_weights(pbs_server *sv, long long stateTicket, const float
     r->weights.assign(spotWeights, spotWeights + std::max(numSpots, 0));
     {
         std::lock_guard<std::mutex> g(sv->mu);
-        auto st = sv->states.find(stateTicket);
-        if (st == sv->states.end()) return set_error(PBS_E_INVALID, "unknown or released resident beam");
-        if (numSpots != st->second->numSpots) return set_error(PBS_E_INVALID, "weight count differs from the resident beam's");
-        const ServerReq &first = *sv->reqs[stateTicket];
-        r->ctId = sv->stateCt[stateTicket];
+        // the resident beam may still be queued or running: its first request holds the handle
+        auto fr = sv->reqs.find(stateTicket);
+        if (fr == sv->reqs.end() || !fr->second->keep || fr->second->status < 0)
+            return set_error(PBS_E_INVALID, "unknown or released resident beam");
+        if (fr->second->status == 2 && !sv->states.count(stateTicket))
+            return set_error(PBS_E_INVALID, "unknown or released resident beam");
+        if (numSpots != fr->second->numSpots)
+            return set_error(PBS_E_INVALID, "weight count differs from the resident beam's");
+        const ServerReq &first = *fr->second;
+        r->ctId = first.ctId;
         for (int i = 0; i < 3; ++i) {
             r->doseDims[i] = first.doseDims[i];
             r->doseRes[i] = first.doseRes[i];
@@ -1830,8 +1850,7 @@ int pbs_server_wait(pbs_server *sv, long long ticket) {
     return PBS_OK;
 }
 
-int pbs_server_result(pbs_server *sv, long long ticket, float *doseHost) {
-    if (!sv || !doseHost) return set_error(PBS_E_INVALID, "null argument");
+static int server_result(pbs_server *sv, long long ticket, float *dst, hipMemcpyKind kind) {
     int rc = pbs_server_wait(sv, ticket);
     if (rc != PBS_OK) return rc;
     float *dose = nullptr;
@@ -1846,7 +1865,7 @@ int pbs_server_result(pbs_server *sv, long long ticket, float *doseHost) {
         r.dose = nullptr;
         r.doseCharge = 0;
     }
-    HIP_TRY(hipMemcpyAsync(doseHost, dose, n * sizeof(float), hipMemcpyDeviceToHost, sv->copyStream));
+    HIP_TRY(hipMemcpyAsync(dst, dose, n * sizeof(float), kind, sv->copyStream));
     HIP_TRY(hipFreeAsync(dose, sv->copyStream));
     HIP_TRY(hipStreamSynchronize(sv->copyStream));
     {
@@ -1855,6 +1874,16 @@ int pbs_server_result(pbs_server *sv, long long ticket, float *doseHost) {
     }
     sv->cv.notify_all();
     return PBS_OK;
+}
+
+int pbs_server_result(pbs_server *sv, long long ticket, float *doseHost) {
+    if (!sv || !doseHost) return set_error(PBS_E_INVALID, "null argument");
+    return server_result(sv, ticket, doseHost, hipMemcpyDeviceToHost);
+}
+
+int pbs_server_result_device(pbs_server *sv, long long ticket, float *doseDevice) {
+    if (!sv || !doseDevice) return set_error(PBS_E_INVALID, "null argument");
+    return server_result(sv, ticket, doseDevice, hipMemcpyDeviceToDevice);
 }
 
 int pbs_server_timing(pbs_server *sv, long long ticket, double *queueMs, double *totalMs) {

@@ -81,6 +81,29 @@ __device__ __forceinline__ float trilinear_border(const T *__restrict__ v, int n
     return c0 + (c1 - c0) * tz;
 }
 
+// Trilinear sample of a stack of planes with border 0, where only planes [zlo, zhi) hold data
+// (the others read as 0 and are never written).
+__device__ __forceinline__ float trilinear_planes(const float *__restrict__ v, int nx, int ny, int zlo, int zhi,
+                                                  float x, float y, float z) {
+    const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+    const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
+    const float tx = x - fx, ty = y - fy, tz = z - fz;
+    float c[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int xi = x0 + (q & 1), yi = y0 + ((q >> 1) & 1), zi = z0 + (q >> 2);
+        const bool in = unsigned(xi) < unsigned(nx) && unsigned(yi) < unsigned(ny) && zi >= zlo && zi < zhi;
+        c[q] = in ? v[(size_t(zi) * ny + yi) * nx + xi] : 0.0f;
+    }
+    const float c00 = c[0] + (c[1] - c[0]) * tx;
+    const float c10 = c[2] + (c[3] - c[2]) * tx;
+    const float c01 = c[4] + (c[5] - c[4]) * tx;
+    const float c11 = c[6] + (c[7] - c[6]) * tx;
+    const float c0 = c00 + (c10 - c00) * ty;
+    const float c1 = c01 + (c11 - c01) * ty;
+    return c0 + (c1 - c0) * tz;
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
@@ -1612,9 +1635,12 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
     const int z = blockIdx.z;
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (z < g.K) {
+        // only the planes [beamFirstInside, mainEndMax) hold main items; the transfer reads the
+        // others as 0, so they are not written
+        if (z < bs->beamFirstInside || z >= bs->mainEndMax) return;
         float sum = 0.0f;
         const int RMmax = planeRM[z];
-        if (z >= bs->beamFirstInside && z < bs->mainEndMax && RMmax >= 0)
+        if (RMmax >= 0)
             sum = gather_tile(g, otx, oty, lx, ly, itemR + size_t(z) * nt, mainSlots + size_t(z) * nt * kSlotFloats,
                               RMmax);
         bev[(size_t(z) * by + Y) * bx + X] = sum;
@@ -1664,7 +1690,8 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
     if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return;
     if (fxI > float(bx) || fyI > float(by)) return;  // wholly outside the BEV: border value 0
     float v = 0.0f;
-    if (fzI <= float(g.K)) v = trilinear_border(bev, bx, by, g.K, fxI, fyI, fzI);
+    if (fzI <= float(g.K))
+        v = trilinear_planes(bev, bx, by, inside, min(bs->mainEndMax, g.K), fxI, fyI, fzI);
     for (int j = 0; j < m; ++j) {
         const GroupScalars &G = gs[j];
         if (G.tailCount <= 0) continue;

@@ -254,6 +254,79 @@ def make_config4(seed: int = 1) -> dict:
     return case
 
 
+def scaled_head_phantom(dims, res, seed: int):
+    """Head-like phantom on any grid: the config-2 body, bones and air cavity scaled to the
+    grid's field of view (HU). Noise from numpy's PCG64 (large grids)."""
+    corner = tuple(-0.5 * res[i] * (dims[i] - 1) for i in range(3))
+    xs, ys, zs = _grid_centres(dims, res, corner)
+    fov = [res[i] * dims[i] for i in range(3)]
+    s = min(fov[0] / 256.0, fov[1] / 256.0, fov[2] / 320.0)
+    rng = np.random.default_rng(seed)
+    ct = np.full((dims[2], dims[1], dims[0]), -1000.0, dtype=np.float32)
+    for k, z in enumerate(zs):
+        Y, X = np.meshgrid(ys, xs, indexing="ij")
+        body = (X / (80.0 * s)) ** 2 + (Y / (100.0 * s)) ** 2 + (z / (70.0 * s)) ** 2 <= 1.0
+        sl = ct[k]
+        sl[body] = (rng.standard_normal(int(body.sum())) * 20.0).astype(np.float32)
+        for cx in (-40.0 * s, 40.0 * s):
+            bone = ((X - cx) / (12.0 * s)) ** 2 + ((Y - 20.0 * s) / (15.0 * s)) ** 2 + (z / (25.0 * s)) ** 2 <= 1.0
+            sl[bone] = (700.0 + 500.0 * rng.uniform(size=int(bone.sum()))).astype(np.float32)
+        air = X ** 2 + (Y + 35.0 * s) ** 2 + (z - 20.0 * s) ** 2 <= (12.0 * s) ** 2
+        sl[air] = -1000.0
+    return ct, tuple(dims), tuple(res), corner
+
+
+def make_config5(seed: int = 5, n_requests: int = 48, weight_only_fraction: float = 0.3) -> dict:
+    """Serving mix: CTs 128^3 (2 mm), 256x256x160 (1x1x2 mm) and 512x512x200 (0.5x0.5x1.25 mm,
+    dose on a 1x1x2.5 mm grid); each request one proton beam at a random gantry angle with
+    500-8000 spots over 10-40 layers (5 mm hexagonal lattice), 70-180 MeV; a fraction of the
+    requests are weight-only updates of an earlier request's beam (log-normal re-weighting).
+
+    ``requests``: dicts with ``ct`` (index into ``cts``), ``beam``, and ``update_of`` (index of
+    the request whose beam is re-weighted, or -1).
+    """
+    grids = [((128, 128, 128), (2.0, 2.0, 2.0), None), ((256, 256, 160), (1.0, 1.0, 2.0), None),
+             ((512, 512, 200), (0.5, 0.5, 1.25), ((256, 256, 100), (1.0, 1.0, 2.5)))]
+    cts = []
+    for i, (dims, res, dose) in enumerate(grids):
+        ct, dims, res, corner = scaled_head_phantom(dims, res, seed * 10 + i)
+        if dose is None:
+            dd, dr, dc = dims, res, corner
+        else:
+            dd, dr = dose
+            dc = tuple(-0.5 * dr[j] * (dd[j] - 1) for j in range(3))
+        cts.append({"ct": ct, "ct_dims": dims, "ct_resolution": res, "ct_corner": corner,
+                    "dose_dims": tuple(dd), "dose_resolution": tuple(dr), "dose_corner": tuple(dc)})
+    rng = Stream(seed, "config5.requests")
+    u = rng.uniform(8 * n_requests)
+    reqs = []
+    for i in range(n_requests):
+        a = u[8 * i: 8 * i + 8]
+        full = [j for j, r in enumerate(reqs) if r["update_of"] < 0]
+        if full and a[0] < weight_only_fraction:
+            base = full[int(a[1] * len(full)) % len(full)]
+            b = dict(reqs[base]["beam"])
+            w = np.asarray(b["spot_weights"], np.float64)
+            wr = Stream(seed, f"config5.reweight.{i}").normal(w.size)
+            b["spot_weights"] = (w * np.exp(0.3 * wr)).astype(np.float32)
+            reqs.append({"ct": reqs[base]["ct"], "beam": b, "update_of": base})
+            continue
+        n_spots = 500 + int(a[2] * 7501)
+        n_layers = 10 + int(a[3] * 31)
+        per_layer = max(1, n_spots // n_layers)
+        radius = math.sqrt(per_layer * 25.0 * math.sqrt(3.0) / 2.0 / math.pi)
+        xy = hex_lattice(5.0, radius)
+        energies = np.linspace(70.0, 180.0, n_layers).astype(np.float32)
+        wr = Stream(seed, f"config5.weights.{i}")
+        weights = [np.exp(0.5 * wr.normal(xy.shape[0])) for _ in energies]
+        sg = (6.0 - 3.0 * (energies.astype(np.float64) - 70.0) / 110.0).astype(np.float32)
+        bdir, bmx, bmy = beam_basis(360.0 * float(a[4]))
+        beam = make_beam(energies, [xy] * n_layers, weights, np.stack([sg, sg], axis=1), beam_dir=bdir, bm_x=bmx,
+                         bm_y=bmy, sad=2000.0)
+        reqs.append({"ct": int(a[5] * 3) % 3, "beam": beam, "update_of": -1})
+    return {"name": "config5-serving-mix", "cts": cts, "requests": reqs, "energy": load_energy_tables()}
+
+
 def make_config2(seed: int = 2, gantry_deg: float = 90.0) -> dict:
     """Head-size phantom with one 30-layer proton beam at gantry 90 deg (along +x)."""
     ct, dims, res, corner = head_phantom(seed)

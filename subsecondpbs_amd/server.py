@@ -90,8 +90,17 @@ class Server:
     def wait(self, ticket: int):
         _abi.check(_abi.lib().pbs_server_wait(self.handle, int(ticket)), "pbs_server_wait")
 
-    def result(self, ticket: int) -> np.ndarray:
+    def result(self, ticket: int, device: bool = False):
+        """The dose as float32 (Z, Y, X): numpy on the host, or (device=True) a torch tensor on
+        the GPU (no host transfer)."""
         nx, ny, nz = self._dims.pop(int(ticket))
+        if device:
+            import torch
+
+            out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+            rc = _abi.lib().pbs_server_result_device(self.handle, int(ticket), C.c_void_p(out.data_ptr()))
+            _abi.check(rc, "pbs_server_result_device")
+            return out
         out = np.empty((nz, ny, nx), np.float32)
         rc = _abi.lib().pbs_server_result(self.handle, int(ticket), C.c_void_p(out.ctypes.data))
         _abi.check(rc, "pbs_server_result")
