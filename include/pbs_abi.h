@@ -72,6 +72,11 @@ typedef struct pbs_energy_struct {
     int nRRlSamples; float rRlScaleFact; const float *rRlVector;
     const float *nucWeightMatrix;   /* [nEnergies][nEnergySamples] or NULL */
     const float *nucSqSigmaMatrix;  /* [nEnergies][nEnergySamples] or NULL */
+    /* Ion of the table set (not in RTDEnergyStruct, which is proton only): the fill's residual
+     * energy E = ionRangeCoef (peak - depth)^ionRangeExp in MeV/u, the rest mass per nucleon in
+     * MeV, and the factor (Z/A)^2 on the multiple-scattering constant. 0 = the proton values
+     * of idd_sigma.cu:172-181 (8.639415, 0.5649718, 938.3, 1). */
+    float ionRangeCoef, ionRangeExp, ionRestMass, ionScatterScale;
 } pbs_energy_struct;
 
 /*

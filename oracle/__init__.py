@@ -30,6 +30,8 @@ class Energy(C.Structure):
         ("nSp", C.c_int), ("spScale", C.c_float), ("sp", _f),
         ("nRrl", C.c_int), ("rrlScale", C.c_float), ("rrl", _f),
         ("nucWeight", _f), ("nucSqSigma", _f),
+        ("ionRangeCoef", C.c_float), ("ionRangeExp", C.c_float), ("ionRestMass", C.c_float),
+        ("ionScatterScale", C.c_float),
     ]
 
 
@@ -180,6 +182,12 @@ def make_energy(energy: dict, keep: _Keep) -> Energy:
     if "nuclear_weight_matrix" in energy:
         e.nucWeight = keep.f32(energy["nuclear_weight_matrix"])
         e.nucSqSigma = keep.f32(energy["nuclear_sq_sigma_matrix"])
+    ion = energy.get("ion")
+    if ion:
+        e.ionRangeCoef = float(ion["range_coef"])
+        e.ionRangeExp = float(ion["range_exp"])
+        e.ionRestMass = float(ion["rest_mass"])
+        e.ionScatterScale = float(ion["scatter_scale"])
     return e
 
 

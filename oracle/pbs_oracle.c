@@ -865,6 +865,11 @@ void orc_fill_halo(const orc_plan *p, int l, const orc_energy *en, const double 
     double eRefSq, sigmaDelta;
     int suppressDip;
     halo_model_consts(model, &eRefSq, &sigmaDelta, &suppressDip);
+    /* the table set's ion (proton defaults: idd_sigma.cu:172-181) */
+    const double rc = en->ionRangeCoef > 0.0f ? (double)en->ionRangeCoef : 8.639415;
+    const double rq = en->ionRangeExp > 0.0f ? (double)en->ionRangeExp : 0.5649718;
+    const double M = en->ionRestMass > 0.0f ? (double)en->ionRestMass : 938.3;
+    if (en->ionScatterScale > 0.0f) eRefSq *= (double)en->ionScatterScale;
     const int halo = model != 0 && rayToNuc && nucW && nucIdd && nucRs && en->nucWeight && en->nucSqSigma &&
                      nucPlane > 0 && p->spotDist > 0.0f;
     /* entry sigma^2 of the halo: sigmaSqAirCoefs(peak) at the entry plane + spot sigma_x^2
@@ -900,8 +905,8 @@ void orc_fill_halo(const orc_plan *p, int l, const orc_energy *en, const double 
                 cumulDose = lut2d_clamp(en->cidd, en->nSamples, en->nEnergies, cumulSp * scale, eIdx);
                 const double rho = density[idx];
                 if (cumulSp < peak) {
-                    double resE = 8.639415 * pow(peak - 0.5 * (cumulSp + cumulSpOld), 0.5649718);
-                    double betaP = resE + 938.3 - 938.3 * 938.3 / (resE + 938.3);
+                    double resE = rc * pow(peak - 0.5 * (cumulSp + cumulSpOld), rq);
+                    double betaP = resE + M - M * M / (resE + M);
                     double rRl = rho * lut1d_clamp(en->rrl, en->nRrl, rho * en->rrlScale);
                     double thetaSq = eRefSq / (betaP * betaP) * stepLength * rRl;
                     sigmaSq += incScat + incDiv;

@@ -39,6 +39,9 @@ typedef struct {
     int nSp; float spScale; const float *sp;
     int nRrl; float rrlScale; const float *rrl;
     const float *nucWeight, *nucSqSigma; /* optional [nEnergies][nSamples] */
+    /* ion of the table set: residual energy c (peak - depth)^q MeV/u, rest mass per nucleon,
+     * (Z/A)^2 on the scattering constant; 0 = proton (idd_sigma.cu:172-181) */
+    float ionRangeCoef, ionRangeExp, ionRestMass, ionScatterScale;
 } orc_energy;
 
 typedef struct {

@@ -44,6 +44,8 @@ class EnergyStruct(C.Structure):
         ("nSpSamples", C.c_int), ("spScaleFact", C.c_float), ("spVector", _f),
         ("nRRlSamples", C.c_int), ("rRlScaleFact", C.c_float), ("rRlVector", _f),
         ("nucWeightMatrix", _f), ("nucSqSigmaMatrix", _f),
+        ("ionRangeCoef", C.c_float), ("ionRangeExp", C.c_float), ("ionRestMass", C.c_float),
+        ("ionScatterScale", C.c_float),
     ]
 
 
@@ -241,6 +243,12 @@ def energy_struct(energy: dict, keep: Keep) -> EnergyStruct:
     if energy.get("nuclear_weight_matrix") is not None:
         e.nucWeightMatrix = keep.f32(energy["nuclear_weight_matrix"])
         e.nucSqSigmaMatrix = keep.f32(energy["nuclear_sq_sigma_matrix"])
+    ion = energy.get("ion")
+    if ion:
+        e.ionRangeCoef = float(ion["range_coef"])
+        e.ionRangeExp = float(ion["range_exp"])
+        e.ionRestMass = float(ion["rest_mass"])
+        e.ionScatterScale = float(ion["scatter_scale"])
     return e
 
 

@@ -229,6 +229,21 @@ void fill_geom(const BeamPlan &p, const pbs_engine &e, FanGeom &g) {
     g.legacy = p.legacyPath ? 1 : 0;
 }
 
+// Scattering and range-energy constants of the fill: the nuclear model's eRefSq / sigmaDelta
+// (idd_sigma.cu:108-121) and the table set's ion (proton defaults, idd_sigma.cu:172-181).
+FillConsts fill_consts(const pbs_engine &e) {
+    FillConsts fc{198.81f, 0.21f, 0, 0, 8.639415f, 0.5649718f, 938.3f, 0.0f};
+    if (e.nuclearModel == 1) { fc.eRefSq = 169.00f; fc.sigmaDelta = 0.06f; fc.suppressDistalDip = 1; }  // GAUSS_FIT
+    if (e.nuclearModel == 2) { fc.eRefSq = 190.44f; fc.sigmaDelta = 0.0f; }                              // SOUKUP
+    if (e.nuclearModel == 3) { fc.eRefSq = 216.09f; fc.sigmaDelta = 0.08f; }                             // FLUKA
+    const pbs_energy_struct &en = e.energy;
+    if (en.ionRangeCoef > 0.0f) fc.rangeCoef = en.ionRangeCoef;
+    if (en.ionRangeExp > 0.0f) fc.rangeExp = en.ionRangeExp;
+    if (en.ionRestMass > 0.0f) fc.restMass = en.ionRestMass;
+    if (en.ionScatterScale > 0.0f) fc.eRefSq *= en.ionScatterScale;
+    return fc;
+}
+
 // Spot-lattice convolution arguments of a planned beam (convolution.cu:477-513).
 ConvArgs make_conv_args(const BeamPlan &p, int rx, int ry, int nSub, bool subMode) {
     ConvArgs a{};
@@ -760,10 +775,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     e->uploadPending = true;
     // 3. IDD / sigma (+ the halo's share when the halo branch is on; idd_sigma.cu:108-121)
     HIP_TRY(mark());
-    FillConsts fc{198.81f, 0.21f, 0, 0};
-    if (e->nuclearModel == 1) fc = FillConsts{169.00f, 0.06f, 1, 0};  // GAUSS_FIT
-    if (e->nuclearModel == 2) fc = FillConsts{190.44f, 0.0f, 0, 0};   // SOUKUP
-    if (e->nuclearModel == 3) fc = FillConsts{216.09f, 0.08f, 0, 0};  // FLUKA
+    FillConsts fc = fill_consts(*e);
     if (keep) {
         // the weight-independent field (every ray at weight 1) into the resident state, then the
         // layer scalars reset and the rescale to this request's weights

@@ -29,9 +29,11 @@ struct CpbArgs {
 };
 
 struct FillConsts {
-    float eRefSq, sigmaDelta;
+    float eRefSq, sigmaDelta;  // eRefSq includes the ion's (Z/A)^2 scattering factor
     int suppressDistalDip;
     int unitWeight;  // 1: every ray weight is taken as 1 (the weight-independent field of a resident beam)
+    float rangeCoef, rangeExp, restMass;  // residual energy c (peak - depth)^q (MeV/u), rest mass / nucleon
+    float pad;
 };
 
 struct TransferArgs {

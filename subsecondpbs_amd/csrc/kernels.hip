@@ -588,8 +588,9 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
         const float cd = lut_rows(c0, c1, nS, ay, sp * L.scaleFact);
         const float mid = 0.5f * (sp + spOld);
         const bool pre = sp < L.peakDepth;
-        const float resE = 8.639415f * pow_hw(L.peakDepth - mid, 0.5649718f);
-        const float betaP = resE + 938.3f - 938.3f * 938.3f * rcp_hw(resE + 938.3f);
+        const float resE = fc.rangeCoef * pow_hw(L.peakDepth - mid, fc.rangeExp);
+        const float M = fc.restMass;
+        const float betaP = resE + M - M * M * rcp_hw(resE + M);
         const float rRl = rho * lut1d_clamp(s_rrl, nRrl, rho * lut.rrlScale);
         const float thetaSq = fc.eRefSq * rcp_hw(betaP * betaP) * stepLength * rRl;
         const float incSum = incScat + incDiv;

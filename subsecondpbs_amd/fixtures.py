@@ -327,6 +327,65 @@ def make_config5(seed: int = 5, n_requests: int = 48, weight_only_fraction: floa
     return {"name": "config5-serving-mix", "cts": cts, "requests": reqs, "energy": load_energy_tables()}
 
 
+def _mixed_beam(rng_u, seed: int, tag: str, n_spots: int, n_layers: int, e_lo: float, e_hi: float,
+                sigma_hi: float, sigma_lo: float, gantry_deg: float) -> dict:
+    """A beam of ~n_spots spots over n_layers layers (hexagonal lattice, 5 mm pitch, tightened so the
+    field stays within a 45 mm radius), log-normal(0, 0.5) weights, sigma falling linearly from
+    sigma_hi at e_lo to sigma_lo at e_hi."""
+    per_layer = max(1, n_spots // n_layers)
+    radius = math.sqrt(per_layer * 25.0 * math.sqrt(3.0) / 2.0 / math.pi)
+    pitch = 5.0 if radius <= 45.0 else 5.0 * 45.0 / radius
+    xy = hex_lattice(pitch, min(radius, 45.0))
+    energies = np.linspace(e_lo, e_hi, n_layers).astype(np.float32)
+    wr = Stream(seed, tag)
+    weights = [np.exp(0.5 * wr.normal(xy.shape[0])) for _ in energies]
+    t = (energies.astype(np.float64) - e_lo) / max(e_hi - e_lo, 1e-9)
+    sg = (sigma_hi + (sigma_lo - sigma_hi) * t).astype(np.float32)
+    bdir, bmx, bmy = beam_basis(gantry_deg)
+    return make_beam(energies, [xy] * n_layers, weights, np.stack([sg, sg], axis=1), beam_dir=bdir, bm_x=bmx,
+                     bm_y=bmy, sad=2000.0)
+
+
+def make_config3(seed: int = 3, n_proton: int = 12, n_carbon: int = 4) -> dict:
+    """Batching case on the config-1 CT: proton beams at gantry angles uniform in [0, 360) with
+    500-8000 spots over 10-40 layers (70-150 MeV, sigma 6 -> 3 mm) and carbon-ion beams of
+    20 layers within 100-400 MeV/u (500-2000 spots, sigma 4 -> 2.5 mm) on the project-defined
+    carbon tables (``subsecondpbs_amd.carbon``, not reference data) with the halo branch.
+
+    Keys: ``beams`` (protons then carbons), ``kinds`` ("proton" / "carbon" per beam), ``energy``
+    (proton tables), ``carbon_energy`` (carbon tables with halo matrices), ``carbon_nuclear``
+    (halo model of the carbon beams: 3, FLUKA constants).
+    """
+    from .carbon import build_carbon_tables
+
+    case = make_config1(seed)
+    rng = Stream(seed, "config3.beams")
+    beams, kinds = [], []
+    for i in range(n_proton):
+        a = rng.uniform(4)
+        beams.append(_mixed_beam(a, seed, f"config3.proton.{i}", 500 + int(a[0] * 7501), 10 + int(a[1] * 31),
+                                 70.0, 150.0, 6.0, 3.0, 360.0 * float(a[2])))
+        kinds.append("proton")
+    for i in range(n_carbon):
+        a = rng.uniform(4)
+        e_lo = 100.0 + 100.0 * float(a[1])
+        beams.append(_mixed_beam(a, seed, f"config3.carbon.{i}", 500 + int(a[0] * 1501), 20,
+                                 e_lo, e_lo + 200.0 * (0.5 + 0.5 * float(a[3])), 4.0, 2.5, 360.0 * float(a[2])))
+        kinds.append("carbon")
+    case.update({"name": "config3-batch-16-mixed", "beams": beams, "kinds": kinds,
+                 "carbon_energy": build_carbon_tables(), "carbon_nuclear": 3})
+    return case
+
+
+def case_for_beam(case: dict, i: int) -> dict:
+    """A one-beam case (CT, grids, tables) for beam ``i`` of a mixed-ion case."""
+    out = dict(case)
+    out["beams"] = [case["beams"][i]]
+    if case.get("kinds") and case["kinds"][i] == "carbon":
+        out["energy"] = case["carbon_energy"]
+    return out
+
+
 def make_config2(seed: int = 2, gantry_deg: float = 90.0) -> dict:
     """Head-size phantom with one 30-layer proton beam at gantry 90 deg (along +x)."""
     ct, dims, res, corner = head_phantom(seed)
