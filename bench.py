@@ -247,12 +247,143 @@ def served_mode(args) -> int:
     return 0
 
 
+def plan_4field(eng, reps: int = 10) -> dict:
+    """Wall time of the 4-field head plan (gantry 0/90/180/270, 4 x 3270 spots) on the engine that
+    already holds the head CT: the four beams queued on one engine stream, and through a 4-lane
+    Batch (one beam per lane, lanes added in order). Median over ``reps`` after one warm-up."""
+    import torch
+
+    from subsecondpbs_amd import fixtures
+    from subsecondpbs_amd.batch import Batch
+
+    case = fixtures.make_plan_4field()
+    nx, ny, nz = case["dose_dims"]
+    dose = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+    bt = Batch(case["energy"], lanes=4)
+    bt.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    bt.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    def one_stream():
+        for b in case["beams"]:
+            eng.beam_dose(b, dose)
+        eng.sync()
+
+    def lanes():
+        bt.dose(case["beams"], dose)
+        bt.sync()
+
+    out = {}
+    for name, run in (("engine_one_stream", one_stream), ("batch_4_lanes", lanes)):
+        run()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out[name] = float(np.median(ts))
+    bt.close()
+    out["beams"] = 4
+    out["spots"] = int(sum(np.asarray(b["spot_weights"]).size for b in case["beams"]))
+    return out
+
+
+def batch_mode(args) -> int:
+    """Batching case (config 3): 12 proton + 4 carbon beams of uneven size on the config-1 CT.
+    Plain: one engine per ion, one beam at a time (queue, wait). Batched: one 4-lane Batch per ion,
+    all beams queued, one wait. Both with results left on the device."""
+    import torch
+
+    from subsecondpbs_amd import fixtures
+    from subsecondpbs_amd.batch import Batch
+    from subsecondpbs_amd.engine import Engine
+
+    case = fixtures.make_config3()
+    nx, ny, nz = case["dose_dims"]
+    sets = []
+    for kind, energy, model in (("proton", case["energy"], 0),
+                                ("carbon", case["carbon_energy"], case["carbon_nuclear"])):
+        beams = [b for b, k in zip(case["beams"], case["kinds"]) if k == kind]
+        e = Engine(energy)
+        bt = Batch(energy, lanes=args.slots)
+        for o in (e, bt):
+            o.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+            o.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+            o.set_nuclear(model)
+        sets.append((beams, e, bt))
+    plain_dose = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+    batch_dose = torch.zeros_like(plain_dose)
+
+    def plain_pass():
+        plain_dose.zero_()
+        lat = []
+        t0 = time.perf_counter()
+        for beams, e, _ in sets:
+            for b in beams:
+                ts = time.perf_counter()
+                e.beam_dose(b, plain_dose)
+                e.sync()
+                lat.append((time.perf_counter() - ts) * 1e3)
+        return time.perf_counter() - t0, lat
+
+    def batch_pass():
+        batch_dose.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for beams, _, bt in sets:
+            bt.dose(beams, batch_dose)
+        for _, _, bt in sets:
+            bt.sync()
+        return time.perf_counter() - t0
+
+    plain_pass()
+    batch_pass()
+    for _, e, _ in sets:
+        e.set_profiling(True)
+        e.stage_times()
+    plain_pass()
+    stage_sum = {}
+    for _, e, _ in sets:
+        st, _ = e.stage_times()
+        e.set_profiling(False)
+        for k, v in st.items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    reps = max(args.steps // 10, 3)
+    plain_t, lat = [], []
+    batch_t = []
+    for _ in range(reps):
+        t, l = plain_pass()
+        plain_t.append(t)
+        lat += l
+        batch_t.append(batch_pass())
+    n = len(case["beams"])
+    pm, bm = float(np.median(plain_t)), float(np.median(batch_t))
+    err = float((batch_dose - plain_dose).abs().max().item() / plain_dose.abs().max().item())
+    for _, e, bt in sets:
+        e.close()
+        bt.close()
+    out = {
+        "metric": METRIC, "value": n / bm, "unit": "beams/s", "n_gpus": 1, "steps": reps * n, "warmup": n,
+        "ms_per_step": 1e3 * bm / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "batching case: 12 proton beams (500-8000 spots, 10-40 layers, random gantry) + 4 "
+                               "carbon beams (20 layers, project-defined 12C tables with the halo branch) on the "
+                               "64^3 water/slab CT", "form": "batched", "lanes": args.slots},
+        "plain_beams_per_s": n / pm, "batched_over_plain": pm / bm,
+        "plain_p50_ms_per_beam": float(np.percentile(lat, 50)), "plain_p99_ms_per_beam": float(np.percentile(lat, 99)),
+        "max_dose_err_batched_vs_plain": err,
+        "plain_gpu_stage_ms_per_pass": {k: round(v, 4) for k, v in stage_sum.items()},
+        "plain_gpu_busy_fraction": sum(stage_sum.values()) / (1e3 * pm),
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=("plain", "served", "optimize"), default="plain",
+    ap.add_argument("--mode", choices=("plain", "served", "optimize", "batch"), default="plain",
                     help="plain: the head-size beam one request at a time (default); served: the serving "
                          "mix through the request server vs the plain engine; optimize: the config-4 "
-                         "spot-weight optimisation loop")
+                         "spot-weight optimisation loop; batch: the 16-beam mixed proton/carbon batch")
     ap.add_argument("--requests", type=int, default=48, help="served mode: requests in the mix")
     ap.add_argument("--slots", type=int, default=4, help="served mode: engine slots of the server")
     ap.add_argument("--rate", type=float, default=100.0, help="served mode: open-loop arrival rate (req/s)")
@@ -261,6 +392,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plan4", action="store_true", help="skip the 4-field plan timing")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB 5 times (a known-size kernel "
                          "for scaling HBM counters)")
@@ -272,6 +404,8 @@ def main() -> int:
 
     if args.mode == "served":
         return served_mode(args)
+    if args.mode == "batch":
+        return batch_mode(args)
     if args.mode == "optimize":
         from subsecondpbs_amd.optimize import run
 
@@ -328,6 +462,7 @@ def main() -> int:
     stages, nb = eng.stage_times()
     eng.set_profiling(False)
     per_beam = {k: v / max(nb, 1) for k, v in stages.items()}
+    plan4 = plan_4field(eng) if not args.no_plan4 else None
 
     # roofline of the dominant stage
     dom = max(per_beam, key=per_beam.get)
@@ -338,7 +473,8 @@ def main() -> int:
         roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / FP32_PEAK_TFLOPS,
                 "traffic": traffic.get("superpose_stage_bytes") if traffic else None,
-                "kernel": "k_superpose<RM>: 8 radius classes on 4 streams, one stage",
+                "kernel": "superposition stage: k_superpose_rows<7/11/15> and k_superpose<24/32>, 5 radius "
+                          "classes on 4 streams (stage wall time from HIP events)",
                 "compute": "fp32 VALU FMA (no matrix cores; peak is the fp32 dense figure)"}
     else:
         b = work.get(dom, {}).get("bytes", 0.0)
@@ -371,6 +507,8 @@ def main() -> int:
                         "fill_mb": work["fill"]["bytes"] / 1e6, "trace_mb": work["trace"]["bytes"] / 1e6},
         "roofline": roof,
     }
+    if plan4 is not None:
+        out["plan_4field_ms"] = plan4
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, max(args.cpu_runs, 1))
     else:

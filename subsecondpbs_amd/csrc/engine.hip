@@ -349,11 +349,25 @@ int pbs_device_count(void) {
     return n;
 }
 
-pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
+}  // extern "C"
+
+// An engine with one stream (oneStream) creates no side streams: the runtime maps streams onto
+// a few hardware queues, and idle side streams of many engines would put their main streams on
+// the same queue, where unrelated work serialises.
+static pbs_engine *engine_create(int gpuId, const pbs_energy_struct *en, bool oneStream);
+
+extern "C" {
+
+pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) { return engine_create(gpuId, en, false); }
+
+}  // extern "C"
+
+static pbs_engine *engine_create(int gpuId, const pbs_energy_struct *en, bool oneStream) {
     if (validate_energy(en) != PBS_OK) return nullptr;
     if (hipSetDevice(gpuId) != hipSuccess) { set_error(PBS_E_DEVICE, "hipSetDevice failed"); return nullptr; }
     std::unique_ptr<pbs_engine> e(new pbs_engine());
     e->device = gpuId;
+    e->oneStream = oneStream;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&e->forkEv, hipEventDisableTiming) != hipSuccess) {
@@ -364,7 +378,7 @@ pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
         set_error(PBS_E_NOMEM, "hipMalloc failed");
         return nullptr;
     }
-    for (int i = 0; i < kSideStreams; ++i) {
+    for (int i = 0; i < (oneStream ? 0 : kSideStreams); ++i) {
         if (hipStreamCreateWithFlags(&e->side[i], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&e->joinEv[i], hipEventDisableTiming) != hipSuccess) {
             set_error(PBS_E_DEVICE, "hipStreamCreate failed");
@@ -399,6 +413,8 @@ pbs_engine *pbs_engine_create(int gpuId, const pbs_energy_struct *en) {
                     e->sp.p, en->nSpSamples, en->spScaleFact, e->rrl.p, en->nRRlSamples, en->rRlScaleFact};
     return e.release();
 }
+
+extern "C" {
 
 void pbs_engine_destroy(pbs_engine *e) {
     if (!e) return;
@@ -815,7 +831,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     }
     // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
-    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];
+    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
     int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev,
                        streams, &marks);
     if (rc2 != PBS_OK) return rc2;
@@ -896,7 +912,7 @@ static int engine_weights_update(pbs_engine *e, const BeamState &st, const float
     HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, st.unitIdd.p, st.unitFp.p,
                            e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
     hipStream_t streams[1 + kSideStreams] = {s};
-    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->side[i];
+    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
     int rc = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev, streams,
                       nullptr);
     if (rc != PBS_OK) return rc;
@@ -1215,10 +1231,10 @@ pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lane
     }
     std::unique_ptr<pbs_batch> b(new pbs_batch());
     b->device = gpuId;
-    b->owner = pbs_engine_create(gpuId, energy);
+    b->owner = engine_create(gpuId, energy, true);
     if (!b->owner) return nullptr;
     for (int i = 0; i < lanes; ++i) {
-        pbs_engine *e = pbs_engine_create(gpuId, energy);
+        pbs_engine *e = engine_create(gpuId, energy, true);
         if (!e) {
             for (pbs_engine *x : b->lanes) pbs_engine_destroy(x);
             pbs_engine_destroy(b->owner);
@@ -1692,12 +1708,11 @@ pbs_server *pbs_server_create(int gpuId, const pbs_energy_struct *energy, int sl
     std::unique_ptr<pbs_server> sv(new pbs_server());
     sv->device = gpuId;
     for (int i = 0; i < slots; ++i) {
-        pbs_engine *e = pbs_engine_create(gpuId, energy);
+        pbs_engine *e = engine_create(gpuId, energy, true);
         if (!e) {
             for (pbs_engine *x : sv->slots) pbs_engine_destroy(x);
             return nullptr;
         }
-        e->oneStream = true;
         sv->slots.push_back(e);
     }
     sv->slotTicket.assign(size_t(slots), 0);
