@@ -226,7 +226,7 @@ int pbs_final_dose_roi(const vec3i *roiIndicesHost, int numROIIndices, vec3f dos
 /* ---- served form -------------------------------------------------------------------
  * No counterpart in the reference (its entry is synchronous, single request, not reentrant).
  * A request server for dose services: resident CTs, a pool of engine slots (each with its own
- * streams), and a worker thread that admits queued requests onto free slots while the device
+ * stream), and a worker thread that admits queued requests onto free slots while the device
  * memory they need (dose buffer, resident beam state, growth of the slot's work buffers) fits
  * the budget, so requests that arrive while others run overlap on the device (continuous
  * batching). A full request may keep its beam resident (keepState): its unit-weight IDD /
