@@ -393,6 +393,9 @@ def main() -> int:
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plan4", action="store_true", help="skip the 4-field plan timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tuning", default="", help="fixed launch shapes instead of the autotuner, e.g. "
+                                                 "superpose_grid=2048,transfer_shape=3,one_stream=0")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB 5 times (a known-size kernel "
                          "for scaling HBM counters)")
@@ -441,6 +444,19 @@ def main() -> int:
     oracle_s = time.perf_counter() - t0
     err = float(np.abs(one - ref).max() / ref.max())
     work = algorithmic_work(dbg, case)
+
+    # launch shapes picked by the autotuner on this beam (outside the timed region; every shape
+    # gives the same dose bitwise)
+    tuning = None
+    if args.tuning:
+        fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        eng.set_tuning(**fixed)
+        tuning = {"best": eng.tuning(), "fixed": True}
+    elif not args.no_autotune:
+        from subsecondpbs_amd import autotune
+
+        res = autotune.tune(eng, beam, dose, reps=5)
+        tuning = {"best": res["best"], "default_ms": res["default_ms"], "best_ms": res["best_ms"]}
 
     for _ in range(args.warmup):
         eng.beam_dose(beam, dose)
@@ -509,6 +525,8 @@ def main() -> int:
     }
     if plan4 is not None:
         out["plan_4field_ms"] = plan4
+    if tuning is not None:
+        out["autotune"] = tuning
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, max(args.cpu_runs, 1))
     else:

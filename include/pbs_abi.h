@@ -137,6 +137,19 @@ int pbs_engine_set_profiling(pbs_engine *e, int enable);
 /* Device time per stage summed over the beams queued since the last call (profiling on):
  * msOut[8] = trace, conv, fill, tiles, superpose, merge, transfer, halo. */
 int pbs_engine_stage_times(pbs_engine *e, float *msOut, int *beams);
+/* Launch shapes of this engine, chosen by the autotuner (subsecondpbs_amd/autotune.py); no
+ * counterpart in the reference. None of them changes a result: every unit of work is computed
+ * the same way whatever block runs it. Keys:
+ *   PBS_TUNE_SUPERPOSE_GRID  work-groups per superposition radius class (64..65536, default 4096)
+ *   PBS_TUNE_TRANSFER_SHAPE  transfer voxel brick, 0..4: (8,8,4) (16,16,1) (16,8,2) (32,8,1)
+ *                            (8,4,8) along (cross, cross, depth) axes of the dose grid (default 0)
+ *   PBS_TUNE_ONE_STREAM      1: radius classes on the engine stream only; 0: on side streams too
+ *                            (engines of pbs_batch / pbs_server have no side streams) */
+#define PBS_TUNE_SUPERPOSE_GRID 1
+#define PBS_TUNE_TRANSFER_SHAPE 2
+#define PBS_TUNE_ONE_STREAM 3
+int pbs_engine_set_tuning(pbs_engine *e, int key, int value);
+int pbs_engine_get_tuning(pbs_engine *e, int key, int *value);
 
 /* Nuclear (halo) branch for the following beams: 0 off, 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA
  * (the reference's NUCLEAR_CORR build modes, idd_sigma.cu:108-121). Requires the energy

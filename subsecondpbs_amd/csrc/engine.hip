@@ -154,6 +154,10 @@ struct pbs_engine {
     // all work on the engine stream (no side-stream fork for the radius classes): the request
     // server's slots overlap whole requests instead
     bool oneStream = false;
+    // launch shapes (pbs_engine_set_tuning): superposition grid cap per radius class and the
+    // transfer's voxel brick (index into kTransferShapes)
+    int tuneSuperposeGrid = 4096;
+    int tuneTransferShape = 0;
     // optional per-stage timing: kStages + 1 events per beam, resolved by pbs_engine_stage_times
     bool profiling = false;
     std::vector<hipEvent_t> evPool;
@@ -566,12 +570,14 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     if (e->oneStream) {
         const hipStream_t one[1 + kSideStreams] = {s, s, s, s};
         HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
-                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, one));
+                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, e->tuneSuperposeGrid,
+                                 one));
     } else {
         HIP_TRY(hipEventRecord(e->forkEv, s));
         for (int i = 0; i < kSideStreams; ++i) HIP_TRY(hipStreamWaitEvent(streams[1 + i], e->forkEv, 0));
         HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
-                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, 4096, streams));
+                                 w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, e->tuneSuperposeGrid,
+                                 streams));
         for (int i = 0; i < kSideStreams; ++i) {
             HIP_TRY(hipEventRecord(e->joinEv[i], streams[1 + i]));
             HIP_TRY(hipStreamWaitEvent(s, e->joinEv[i], 0));
@@ -590,7 +596,7 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
     for (int a = 1; a < 3; ++a)
         if (std::fabs(g.d2g[6 + a]) > std::fabs(g.d2g[6 + depthAxis])) depthAxis = a;
     HIP_TRY(launch_transfer(g, ta, bs, w.groupScal.p, w.bev.p, w.tailBev.p, doseDev, gx * 32, gy * 8, gz, depthAxis,
-                            s));
+                            e->tuneTransferShape, s));
     return PBS_OK;
 }
 
@@ -925,6 +931,36 @@ int pbs_engine_set_profiling(pbs_engine *e, int enable) {
     if (!e) return set_error(PBS_E_INVALID, "null engine");
     e->profiling = enable != 0;
     return PBS_OK;
+}
+
+int pbs_engine_set_tuning(pbs_engine *e, int key, int value) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    switch (key) {
+    case PBS_TUNE_SUPERPOSE_GRID:
+        if (value < 64 || value > 65536) return set_error(PBS_E_INVALID, "superposition grid must be 64..65536");
+        e->tuneSuperposeGrid = value;
+        return PBS_OK;
+    case PBS_TUNE_TRANSFER_SHAPE:
+        if (value < 0 || value >= kNumTransferShapes) return set_error(PBS_E_INVALID, "unknown transfer shape");
+        e->tuneTransferShape = value;
+        return PBS_OK;
+    case PBS_TUNE_ONE_STREAM:
+        if (value == 0 && !e->side[0]) return set_error(PBS_E_INVALID, "this engine was created without side streams");
+        e->oneStream = value != 0;
+        return PBS_OK;
+    default:
+        return set_error(PBS_E_INVALID, "unknown tuning key");
+    }
+}
+
+int pbs_engine_get_tuning(pbs_engine *e, int key, int *value) {
+    if (!e || !value) return set_error(PBS_E_INVALID, "null argument");
+    switch (key) {
+    case PBS_TUNE_SUPERPOSE_GRID: *value = e->tuneSuperposeGrid; return PBS_OK;
+    case PBS_TUNE_TRANSFER_SHAPE: *value = e->tuneTransferShape; return PBS_OK;
+    case PBS_TUNE_ONE_STREAM: *value = e->oneStream ? 1 : 0; return PBS_OK;
+    default: return set_error(PBS_E_INVALID, "unknown tuning key");
+    }
 }
 
 int pbs_engine_stage_times(pbs_engine *e, float *msOut, int *beams) {

@@ -13,6 +13,8 @@ constexpr int kSlotPitch = 96;                    // 32 + 2 * 32 columns
 constexpr int kSlotFloats = kSlotPitch * 72;      // (8 + 2 * 32) rows
 constexpr int kNumClasses = 5;
 constexpr int kMaxLayersDev = 256;
+// transfer voxel bricks (x, y along the dose grid's cross-beam axes, z along its depth axis)
+constexpr int kNumTransferShapes = 5;
 constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
 
 struct ConvArgs {
@@ -83,7 +85,7 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
                         hipStream_t s);
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
                            const float *bev, const float *tailBev, float *dose, int extentX, int extentY, int extentZ,
-                           int depthAxis, hipStream_t s);
+                           int depthAxis, int shape, hipStream_t s);
 
 // adjoint with respect to the spot weights (adjoint.hip)
 hipError_t launch_adj_zero(float *v, size_t n, hipStream_t s);

@@ -1890,9 +1890,14 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
 
 hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs, const GroupScalars *gs,
                            const float *bev, const float *tailBev, float *dose, int extentX, int extentY, int extentZ,
-                           int depthAxis, hipStream_t s) {
-    int t[3] = {8, 8, 8};
-    t[depthAxis] = 4;
+                           int depthAxis, int shape, hipStream_t s) {
+    // 256-voxel bricks: (first cross axis, second cross axis, depth axis)
+    static const int kShapes[kNumTransferShapes][3] = {{8, 8, 4}, {16, 16, 1}, {16, 8, 2}, {32, 8, 1}, {8, 4, 8}};
+    const int *sh = kShapes[shape >= 0 && shape < kNumTransferShapes ? shape : 0];
+    int t[3];
+    t[depthAxis] = sh[2];
+    t[depthAxis == 0 ? 1 : 0] = sh[0];
+    t[depthAxis == 2 ? 1 : 2] = sh[1];
     const int e[3] = {extentX, extentY, extentZ};
     dim3 grid;
     grid.x = unsigned((e[0] + t[0] - 1) / t[0]);

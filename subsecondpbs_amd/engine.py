@@ -107,6 +107,23 @@ class Engine:
         return {"nucDims": (nx, ny), "spotDist": float(sd.value), "iddSigma": f, "radHist": h,
                 "overflow": int(ov.value)}
 
+    TUNING_KEYS = {"superpose_grid": 1, "transfer_shape": 2, "one_stream": 3}
+
+    def set_tuning(self, **kw):
+        """Launch shapes (see ``subsecondpbs_amd.autotune``): superpose_grid, transfer_shape,
+        one_stream. They change timing only, never a result."""
+        for k, v in kw.items():
+            _abi.check(_abi.lib().pbs_engine_set_tuning(self.handle, self.TUNING_KEYS[k], int(v)),
+                       "pbs_engine_set_tuning")
+
+    def tuning(self) -> dict:
+        out = {}
+        for k, key in self.TUNING_KEYS.items():
+            v = C.c_int(0)
+            _abi.check(_abi.lib().pbs_engine_get_tuning(self.handle, key, C.byref(v)), "pbs_engine_get_tuning")
+            out[k] = int(v.value)
+        return out
+
     def set_profiling(self, on: bool = True):
         _abi.check(_abi.lib().pbs_engine_set_profiling(self.handle, int(bool(on))), "pbs_engine_set_profiling")
 
