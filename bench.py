@@ -412,8 +412,11 @@ def main() -> int:
     if args.mode == "optimize":
         from subsecondpbs_amd.optimize import run
 
-        out = run(steps=args.steps)
+        runs = {m: run(steps=args.steps, mode=m) for m in ("full", "resident", "graph")}
+        out = runs["graph"]
         out["objective"] = [round(v, 6) for v in out["objective"]]
+        out["ms_per_step_by_mode"] = {m: r["ms_per_step"] for m, r in runs.items()}
+        out["objective_last_by_mode"] = {m: r["objective_last"] for m, r in runs.items()}
         print(json.dumps(out))
         return 0
 

@@ -128,6 +128,28 @@ int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *do
  * accumulated with float atomics, so results agree run to run to rounding, not bitwise. */
 int pbs_engine_beam_adjoint(pbs_engine *e, const pbs_beam_settings *beam, const float *gradDoseDevice,
                             float *gradSpotWeights);
+/* Resident beams on a plain engine: no counterpart in the reference; they serve callers that
+ * re-evaluate one beam at many spot weights (weight optimisation, the served form's updates).
+ * pbs_engine_keep_beam runs the beam into doseDevice (accumulated) and keeps its weight-
+ * independent state in HBM: plan, fan geometry, post-trace scalars and the unit-weight IDD /
+ * rSigmaEff field. pbs_engine_update_beam adds the beam's dose at new weights (numSpots, the
+ * beam's spot order) to doseDevice, re-running only the convolution, the rescale of the unit
+ * field, the superposition, the merge and the transfer. pbs_engine_update_adjoint gives
+ * dJ/dw at the new weights from the same state (no trace, no fill). A resident beam is used
+ * with the engine that made it, whose CT, dose grid and nuclear model (off) must stay as they
+ * were. Spot-lattice beams only, halo off. Results use IDD = W x unit IDD, one rounding from the
+ * plain form's dose.
+ * pbs_resident_beam_use_graph(rb, 1): updates are replayed as one captured HIP graph while the
+ * engine, dose buffer, dose grid, launch shapes and device buffers are those captured (the
+ * first call, and any call after a change, runs eagerly and captures again). */
+typedef struct pbs_resident_beam pbs_resident_beam;
+int pbs_engine_keep_beam(pbs_engine *e, const pbs_beam_settings *beam, float *doseDevice, pbs_resident_beam **out);
+int pbs_engine_update_beam(pbs_engine *e, pbs_resident_beam *rb, const float *spotWeights, int numSpots,
+                           float *doseDevice);
+int pbs_engine_update_adjoint(pbs_engine *e, pbs_resident_beam *rb, const float *spotWeights, int numSpots,
+                              const float *gradDoseDevice, float *gradSpotWeights);
+int pbs_resident_beam_use_graph(pbs_resident_beam *rb, int enable);
+void pbs_resident_beam_destroy(pbs_resident_beam *rb);
 int pbs_engine_sync(pbs_engine *e);
 /* Raw stream handle (hipStream_t) the engine queues work on. */
 void *pbs_engine_stream(pbs_engine *e);

@@ -82,7 +82,8 @@ class CPBGrid(C.Structure):
 EXPORTED = [
     "pbs_subsecond_wrapper", "pbs_last_error", "pbs_device_count",
     "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
-    "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
+    "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
+    "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
@@ -119,6 +120,16 @@ def lib():
         L.pbs_engine_beam_dose.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p]
         L.pbs_engine_beam_adjoint.restype = C.c_int
         L.pbs_engine_beam_adjoint.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p, C.c_void_p]
+        L.pbs_engine_keep_beam.restype = C.c_int
+        L.pbs_engine_keep_beam.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p, C.POINTER(C.c_void_p)]
+        L.pbs_engine_update_beam.restype = C.c_int
+        L.pbs_engine_update_beam.argtypes = [C.c_void_p, C.c_void_p, _f, C.c_int, C.c_void_p]
+        L.pbs_engine_update_adjoint.restype = C.c_int
+        L.pbs_engine_update_adjoint.argtypes = [C.c_void_p, C.c_void_p, _f, C.c_int, C.c_void_p, C.c_void_p]
+        L.pbs_resident_beam_use_graph.restype = C.c_int
+        L.pbs_resident_beam_use_graph.argtypes = [C.c_void_p, C.c_int]
+        L.pbs_resident_beam_destroy.restype = None
+        L.pbs_resident_beam_destroy.argtypes = [C.c_void_p]
         L.pbs_engine_sync.restype = C.c_int
         L.pbs_engine_sync.argtypes = [C.c_void_p]
         L.pbs_engine_stream.restype = C.c_void_p

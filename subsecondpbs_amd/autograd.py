@@ -9,6 +9,10 @@ scaling, then the transposed subspot convolution), evaluated at the forward's we
 The dose is linear in the weights for fixed live masks, tile radii and step ranges (these do
 not depend on the size of the weights), so the gradient is exact up to float rounding wherever
 the masks do not change.
+
+``ResidentSpotWeightDose.apply(weights, resident)`` is the same on a resident beam
+(``Engine.keep_beam``): the forward is a weight-only update (optionally a HIP graph replay) and
+the backward the adjoint from the resident state, so neither repeats the trace or the fill.
 """
 from __future__ import annotations
 
@@ -40,8 +44,28 @@ class SpotWeightDose(torch.autograd.Function):
         return torch.from_numpy(np.asarray(g)).to(ctx.wdev, ctx.wdtype), None, None
 
 
-def beam_dose(weights: torch.Tensor, engine: Engine, beam: dict) -> torch.Tensor:
-    """Differentiable dose of ``beam`` at ``weights`` (see SpotWeightDose)."""
+class ResidentSpotWeightDose(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, weights: torch.Tensor, resident):
+        if weights.dim() != 1:
+            raise ValueError("weights must be a 1-D tensor, one entry per spot")
+        w = weights.detach().to("cpu", torch.float32).numpy().copy()
+        dose = resident.dose(w)
+        ctx.resident, ctx.w = resident, w
+        ctx.wdev, ctx.wdtype = weights.device, weights.dtype
+        return dose
+
+    @staticmethod
+    def backward(ctx, grad_dose: torch.Tensor):
+        g = ctx.resident.adjoint(ctx.w, grad_dose.detach().to(torch.float32).contiguous())
+        return torch.from_numpy(np.asarray(g)).to(ctx.wdev, ctx.wdtype), None
+
+
+def beam_dose(weights: torch.Tensor, engine: Engine, beam: dict, resident=None) -> torch.Tensor:
+    """Differentiable dose of ``beam`` at ``weights`` (see SpotWeightDose); with ``resident``
+    (the beam kept on ``engine``), through ResidentSpotWeightDose."""
+    if resident is not None:
+        return ResidentSpotWeightDose.apply(weights, resident)
     return SpotWeightDose.apply(weights, engine, beam)
 
 

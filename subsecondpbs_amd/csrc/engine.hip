@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -42,6 +43,10 @@ int set_error(int code, const std::string &msg) {
             return set_error(PBS_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(_e));            \
     } while (0)
 
+// Bumped whenever any device buffer of the library is freed or moved: a captured HIP graph
+// holds raw addresses and is replayed only while this count is the one it was captured at.
+std::atomic<unsigned long long> g_bufGen{0};
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -51,7 +56,10 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_bufGen.fetch_add(1);
+        }
         p = nullptr;
         n = 0;
     }
@@ -188,6 +196,40 @@ struct BeamState {
     DevBuf<int> unitFp;
     DevBuf<BeamScalars> bs0;
     DevBuf<float> weplMin, subspots;
+    // HIP graph of the weight-only update chain (engine-level resident beams with graphs on).
+    // It is replayed only on the engine, dose buffer, launch shapes and device buffers it was
+    // captured with; its uploads read the pinned copies below, which the host refills before
+    // each replay once the previous replay has read them (graphDone).
+    bool useGraph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *graphEngine = nullptr;
+    float *graphDose = nullptr;
+    int graphDoseDims[3] = {0, 0, 0};
+    int graphTune[2] = {0, 0};
+    unsigned long long graphGen = 0;
+    hipEvent_t graphDone = nullptr;
+    float *pinWeights = nullptr;
+    LayerHost *pinLayers = nullptr;
+    size_t pinWeightsN = 0, pinLayersN = 0;
+    void drop_graph() {
+        if (graphDone) (void)hipEventSynchronize(graphDone);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graphEngine = nullptr;
+        graphDose = nullptr;
+    }
+    BeamState() = default;
+    BeamState(const BeamState &) = delete;
+    BeamState &operator=(const BeamState &) = delete;
+    ~BeamState() {
+        drop_graph();
+        if (graphDone) (void)hipEventDestroy(graphDone);
+        if (pinWeights) (void)hipHostFree(pinWeights);
+        if (pinLayers) (void)hipHostFree(pinLayers);
+    }
     size_t bytes() const {
         return unitIdd.n * sizeof(float2) + unitFp.n * sizeof(int) + bs0.n * sizeof(BeamScalars) +
                (weplMin.n + subspots.n) * sizeof(float);
@@ -862,9 +904,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
 // weights, the convolution, the rescale of the resident unit-weight field, then the grid
 // stages into doseDev (accumulated). The CT, dose grid and nuclear model of e must be those of
 // the state's first run.
-static int engine_weights_update(pbs_engine *e, const BeamState &st, const float *weights, int numSpots,
-                                 float *doseDev) {
-    if (!e || !weights || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+// Host part of a weight-only update: the resident plan with the lattice re-binned at the new
+// weights (e->plan.denseWeights), the fan geometry, and every buffer the chain needs sized.
+static int update_prepare(pbs_engine *e, const BeamState &st, const float *weights, int numSpots) {
+    if (!e || !weights) return set_error(PBS_E_INVALID, "null argument");
     if (numSpots != st.numSpots) return set_error(PBS_E_INVALID, "weight count differs from the resident beam's");
     if (e->nuclearModel != 0) return set_error(PBS_E_INVALID, "resident beams do not cover the nuclear halo branch");
     HIP_TRY(hipSetDevice(e->device));
@@ -900,10 +943,23 @@ static int engine_weights_update(pbs_engine *e, const BeamState &st, const float
     HIP_TRY(e->subEntry.ensure(size_t(L) * std::max(st.maxSub, 1) * 2));
     HIP_TRY(e->spot.ensure(splane * L));
     HIP_TRY(e->interm.ensure(size_t(L) * st.nSub * p.spotDims[1] * rx));
+    return PBS_OK;
+}
+
+// Device part of a weight-only update (after update_prepare), all on the engine stream: the
+// per-layer values and the re-binned lattice uploaded from wPlane / layers, the post-trace
+// scalars restored, cut-offs, convolution, the rescale of the unit-weight field, then either the
+// grid stages into doseDev or (gradDose given) the adjoint into gradSpots. markUpload: record
+// the engine's upload event (eager calls; a captured chain reads pinned copies instead).
+static int update_enqueue(pbs_engine *e, const BeamState &st, const float *wPlane, const LayerHost *layers,
+                          float *doseDev, bool markUpload, const float *gradDose, float *gradSpots) {
+    const BeamPlan &p = e->plan;
+    const FanGeom &g = e->geom;
+    const int L = p.numLayers, K = p.steps, rx = g.rx, ry = g.ry;
+    const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
     hipStream_t s = e->stream;
-    e->hostLayers = st.layerHost;
-    HIP_TRY(hipMemcpyAsync(e->layerHost.p, e->hostLayers.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(e->spot.p, p.denseWeights.data(), sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->layerHost.p, layers, sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->spot.p, wPlane, sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(e->beamScal.p, st.bs0.p, sizeof(BeamScalars), hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipMemcpyAsync(e->weplMin.p, st.weplMin.p, sizeof(float) * K, hipMemcpyDeviceToDevice, s));
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
@@ -913,18 +969,175 @@ static int engine_weights_update(pbs_engine *e, const BeamState &st, const float
     HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, st.subspots.p,
                         e->subEntry.p, L, s));
     e->convArgs = a;
-    HIP_TRY(hipEventRecord(e->uploadDone, s));
-    e->uploadPending = true;
+    if (markUpload) {
+        HIP_TRY(hipEventRecord(e->uploadDone, s));
+        e->uploadPending = true;
+    }
     HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, st.unitIdd.p, st.unitFp.p,
                            e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
-    hipStream_t streams[1 + kSideStreams] = {s};
-    for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
-    int rc = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev, streams,
-                      nullptr);
-    if (rc != PBS_OK) return rc;
+    if (gradDose) {
+        int rc = run_grid_adjoint(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, gradDose, s);
+        if (rc != PBS_OK) return rc;
+        HIP_TRY(e->adjInterm.ensure(size_t(L) * st.nSub * p.spotDims[1] * rx));
+        HIP_TRY(e->adjSpot.ensure(splane * L));
+        HIP_TRY(launch_adj_conv(e->convArgs, e->adjW.p, e->adjInterm.p, e->adjSpot.p, e->layerScal.p, e->beamScal.p,
+                                st.subspots.p, e->subEntry.p, L, s));
+        e->hostAdjSpot.resize(splane * L);
+        HIP_TRY(hipMemcpyAsync(e->hostAdjSpot.data(), e->adjSpot.p, sizeof(float) * splane * L,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        for (int i = 0; i < st.numSpots; ++i) {
+            const int c = i < int(p.spotCell.size()) ? p.spotCell[size_t(i)] : -1;
+            gradSpots[i] = c >= 0 ? e->hostAdjSpot[size_t(c)] : 0.0f;
+        }
+    } else {
+        hipStream_t streams[1 + kSideStreams] = {s};
+        for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
+        int rc = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev,
+                          streams, nullptr);
+        if (rc != PBS_OK) return rc;
+    }
     e->lastHalo = false;
     e->haveLast = true;
     return PBS_OK;
+}
+
+// Weight-only update of a resident beam on engine e: the lattice re-binned with the new
+// weights, the convolution, the rescale of the resident unit-weight field, then the grid
+// stages into doseDev (accumulated), or the adjoint into gradSpots when gradDose is given. The
+// CT, dose grid and nuclear model of e must be those of the state's first run.
+static int engine_weights_update(pbs_engine *e, const BeamState &st, const float *weights, int numSpots,
+                                 float *doseDev, const float *gradDose = nullptr, float *gradSpots = nullptr) {
+    if (!doseDev && !gradDose) return set_error(PBS_E_INVALID, "null argument");
+    int rc = update_prepare(e, st, weights, numSpots);
+    if (rc != PBS_OK) return rc;
+    e->hostLayers = st.layerHost;
+    return update_enqueue(e, st, e->plan.denseWeights.data(), e->hostLayers.data(), doseDev, true, gradDose,
+                          gradSpots);
+}
+
+// The same update through a HIP graph of the whole chain (state.useGraph): replayed while the
+// engine, dose buffer, dose grid, launch shapes and every device buffer are the ones captured;
+// otherwise the update runs eagerly and the chain is captured again for the next call.
+static int engine_weights_update_graph(pbs_engine *e, BeamState &st, const float *weights, int numSpots,
+                                       float *doseDev) {
+    if (!st.useGraph || e->profiling) return engine_weights_update(e, st, weights, numSpots, doseDev);
+    if (!doseDev) return set_error(PBS_E_INVALID, "null argument");
+    const bool valid = st.exec && st.graphEngine == e && st.graphDose == doseDev &&
+                       st.graphDoseDims[0] == e->doseDims[0] && st.graphDoseDims[1] == e->doseDims[1] &&
+                       st.graphDoseDims[2] == e->doseDims[2] && st.graphTune[0] == e->tuneSuperposeGrid &&
+                       st.graphTune[1] == e->tuneTransferShape && st.graphGen == g_bufGen.load();
+    if (valid) {
+        int rc = update_prepare(e, st, weights, numSpots);
+        if (rc != PBS_OK) return rc;
+        // the previous replay's uploads have read the pinned weights before they are rewritten
+        HIP_TRY(hipEventSynchronize(st.graphDone));
+        std::memcpy(st.pinWeights, e->plan.denseWeights.data(), sizeof(float) * e->plan.denseWeights.size());
+        HIP_TRY(hipGraphLaunch(st.exec, e->stream));
+        HIP_TRY(hipEventRecord(st.graphDone, e->stream));
+        e->lastHalo = false;
+        e->haveLast = true;
+        return PBS_OK;
+    }
+    // eager update now (sizes every buffer), then capture the chain for the next calls
+    int rc = engine_weights_update(e, st, weights, numSpots, doseDev);
+    if (rc != PBS_OK) return rc;
+    st.drop_graph();
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->uploadPending = false;
+    const size_t nw = e->plan.denseWeights.size(), nl = st.layerHost.size();
+    if (st.pinWeightsN < nw) {
+        if (st.pinWeights) HIP_TRY(hipHostFree(st.pinWeights));
+        st.pinWeights = nullptr;
+        st.pinWeightsN = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&st.pinWeights), sizeof(float) * std::max<size_t>(nw, 1)));
+        st.pinWeightsN = nw;
+    }
+    if (st.pinLayersN < nl) {
+        if (st.pinLayers) HIP_TRY(hipHostFree(st.pinLayers));
+        st.pinLayers = nullptr;
+        st.pinLayersN = 0;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&st.pinLayers), sizeof(LayerHost) * std::max<size_t>(nl, 1)));
+        st.pinLayersN = nl;
+    }
+    if (!st.graphDone) HIP_TRY(hipEventCreateWithFlags(&st.graphDone, hipEventDisableTiming));
+    std::memcpy(st.pinWeights, e->plan.denseWeights.data(), sizeof(float) * nw);
+    std::memcpy(st.pinLayers, st.layerHost.data(), sizeof(LayerHost) * nl);
+    const unsigned long long gen = g_bufGen.load();
+    const bool one = e->oneStream;
+    e->oneStream = true;  // the radius classes on the engine stream only: one linear chain
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    rc = update_enqueue(e, st, st.pinWeights, st.pinLayers, doseDev, false, nullptr, nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    e->oneStream = one;
+    if (rc != PBS_OK || ce != hipSuccess || g_bufGen.load() != gen) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != PBS_OK) return rc;
+        if (ce != hipSuccess) return set_error(PBS_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        return PBS_OK;  // a buffer moved during capture: stay eager, capture again next call
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(PBS_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    st.graph = graph;
+    st.exec = exec;
+    st.graphEngine = e;
+    st.graphDose = doseDev;
+    for (int i = 0; i < 3; ++i) st.graphDoseDims[i] = e->doseDims[i];
+    st.graphTune[0] = e->tuneSuperposeGrid;
+    st.graphTune[1] = e->tuneTransferShape;
+    st.graphGen = gen;
+    HIP_TRY(hipEventRecord(st.graphDone, e->stream));
+    return PBS_OK;
+}
+
+struct pbs_resident_beam {
+    BeamState st;
+    pbs_engine *engine = nullptr;
+};
+
+int pbs_engine_keep_beam(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, pbs_resident_beam **out) {
+    if (!e || !beam || !doseDev || !out) return set_error(PBS_E_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<pbs_resident_beam> rb(new pbs_resident_beam());
+    rb->engine = e;
+    int rc = engine_beam_dose(e, beam, doseDev, nullptr, nullptr, nullptr, nullptr, &rb->st);
+    if (rc != PBS_OK) return rc;
+    *out = rb.release();
+    return PBS_OK;
+}
+
+int pbs_engine_update_beam(pbs_engine *e, pbs_resident_beam *rb, const float *spotWeights, int numSpots,
+                           float *doseDev) {
+    if (!e || !rb || !spotWeights || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (rb->engine != e) return set_error(PBS_E_INVALID, "the resident beam belongs to another engine");
+    return engine_weights_update_graph(e, rb->st, spotWeights, numSpots, doseDev);
+}
+
+int pbs_engine_update_adjoint(pbs_engine *e, pbs_resident_beam *rb, const float *spotWeights, int numSpots,
+                              const float *gradDoseDev, float *gradSpotWeights) {
+    if (!e || !rb || !spotWeights || !gradDoseDev || !gradSpotWeights) return set_error(PBS_E_INVALID, "null argument");
+    if (rb->engine != e) return set_error(PBS_E_INVALID, "the resident beam belongs to another engine");
+    return engine_weights_update(e, rb->st, spotWeights, numSpots, nullptr, gradDoseDev, gradSpotWeights);
+}
+
+int pbs_resident_beam_use_graph(pbs_resident_beam *rb, int enable) {
+    if (!rb) return set_error(PBS_E_INVALID, "null argument");
+    (void)hipSetDevice(rb->engine->device);
+    rb->st.useGraph = enable != 0;
+    if (!rb->st.useGraph) rb->st.drop_graph();
+    return PBS_OK;
+}
+
+void pbs_resident_beam_destroy(pbs_resident_beam *rb) {
+    if (!rb) return;
+    (void)hipSetDevice(rb->engine->device);
+    (void)hipStreamSynchronize(rb->engine->stream);
+    delete rb;
 }
 
 int pbs_engine_set_profiling(pbs_engine *e, int enable) {

@@ -7,6 +7,7 @@ callers synchronise with :meth:`Engine.sync` before reading results.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -24,9 +25,12 @@ class Engine:
         self.device = device
         self.ct_dims = None
         self.dose_dims = None
+        self._resident = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "handle", None):
+            for rb in list(getattr(self, "_resident", ())):
+                rb.close()  # resident beams hold device state of this engine
             _abi.lib().pbs_engine_destroy(self.handle)
             self.handle = None
 
@@ -75,6 +79,18 @@ class Engine:
                                                 C.c_void_p(out.ctypes.data))
         _abi.check(rc, "pbs_engine_beam_adjoint")
         return out
+
+    def keep_beam(self, beam: dict, dose, use_subspots: bool | None = None) -> "ResidentBeam":
+        """Queue the beam into ``dose`` and keep its weight-independent state resident on this
+        engine; weight-only updates and gradients then skip the trace and the fill."""
+        keep = _abi.Keep()
+        bs = _abi.beam_settings(beam, keep, use_subspots)
+        h = C.c_void_p(0)
+        rc = _abi.lib().pbs_engine_keep_beam(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()), C.byref(h))
+        _abi.check(rc, "pbs_engine_keep_beam")
+        rb = ResidentBeam(self, h.value, int(bs.numSpots))
+        self._resident.add(rb)
+        return rb
 
     def sync(self):
         _abi.check(_abi.lib().pbs_engine_sync(self.handle), "pbs_engine_sync")
@@ -187,3 +203,60 @@ class Engine:
             "groupTailCount": np.array(ints.groupTailCount[:ints.numGroups], np.int32),
         }
         return out
+
+
+class ResidentBeam:
+    """A beam kept resident on an Engine (``Engine.keep_beam``): its unit-weight IDD / sigma
+    field, plan and geometry stay in HBM. ``update`` adds the dose at new spot weights to a device
+    buffer; ``adjoint`` gives dJ/dw at new weights; ``use_graph(True)`` replays updates as one
+    captured HIP graph (while the engine, dose buffer and device buffers stay the same)."""
+
+    def __init__(self, engine, handle, num_spots: int):
+        self.engine = engine
+        self.handle = handle
+        self.num_spots = num_spots
+
+    def update(self, weights, dose):
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        rc = _abi.lib().pbs_engine_update_beam(self.engine.handle, self.handle, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                               int(w.size), C.c_void_p(dose.data_ptr()))
+        _abi.check(rc, "pbs_engine_update_beam")
+
+    def dose(self, weights):
+        """The beam's dose at ``weights`` as a new device tensor (Z, Y, X). Computed into a buffer
+        kept by this object, so a graph captured for that buffer is replayed call after call."""
+        import torch
+
+        nx, ny, nz = self.engine.dose_dims
+        buf = getattr(self, "_buf", None)
+        if buf is None or tuple(buf.shape) != (nz, ny, nx):
+            buf = self._buf = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+        else:
+            buf.zero_()
+        torch.cuda.current_stream().synchronize()  # the engine stream reads it next
+        self.update(weights, buf)
+        self.engine.sync()
+        return buf.clone()
+
+    def adjoint(self, weights, grad_dose) -> np.ndarray:
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        out = np.zeros(self.num_spots, np.float32)
+        rc = _abi.lib().pbs_engine_update_adjoint(self.engine.handle, self.handle,
+                                                  w.ctypes.data_as(C.POINTER(C.c_float)), int(w.size),
+                                                  C.c_void_p(grad_dose.data_ptr()), C.c_void_p(out.ctypes.data))
+        _abi.check(rc, "pbs_engine_update_adjoint")
+        return out
+
+    def use_graph(self, on: bool = True):
+        _abi.check(_abi.lib().pbs_resident_beam_use_graph(self.handle, int(bool(on))), "pbs_resident_beam_use_graph")
+
+    def close(self):
+        if self.handle:
+            _abi.lib().pbs_resident_beam_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

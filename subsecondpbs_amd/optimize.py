@@ -36,9 +36,21 @@ def setup(case: dict):
     return eng, beam, w0 * scale, t, o
 
 
-def run(steps: int = 50, lr_frac: float = 0.05, seed: int = 1) -> dict:
+def run(steps: int = 50, lr_frac: float = 0.05, seed: int = 1, mode: str = "graph") -> dict:
+    """mode: "full" (every step runs the whole chain: trace, fill, ... and the full adjoint),
+    "resident" (the beam kept resident: weight-only forward and adjoint from the state) or
+    "graph" (resident, with the forward replayed as a captured HIP graph)."""
     case = F.make_config4(seed)
     eng, beam, w, t, o = setup(case)
+    resident = None
+    if mode in ("resident", "graph"):
+        nx, ny, nz = case["dose_dims"]
+        b0 = dict(beam)
+        b0["spot_weights"] = w.numpy().copy()
+        tmp = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+        resident = eng.keep_beam(b0, tmp)
+        eng.sync()
+        resident.use_graph(mode == "graph")
     w = w.clone().requires_grad_(True)
     opt = torch.optim.Adam([w], lr=lr_frac * float(w.detach().mean()))
     pres, of = case["prescription"], case["oar_factor"]
@@ -47,7 +59,7 @@ def run(steps: int = 50, lr_frac: float = 0.05, seed: int = 1) -> dict:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         opt.zero_grad()
-        J = objective(beam_dose(w, eng, beam), t, o, pres, of)
+        J = objective(beam_dose(w, eng, beam, resident), t, o, pres, of)
         J.backward()
         opt.step()
         with torch.no_grad():
@@ -56,20 +68,23 @@ def run(steps: int = 50, lr_frac: float = 0.05, seed: int = 1) -> dict:
         times.append((time.perf_counter() - t0) * 1e3)
         hist.append(float(J.item()))
     with torch.no_grad():
-        Jf = float(objective(beam_dose(w, eng, beam), t, o, pres, of).item())
+        Jf = float(objective(beam_dose(w, eng, beam, resident), t, o, pres, of).item())
+    if resident is not None:
+        resident.close()
     eng.close()
     return {"case": case["name"], "steps": steps, "spots": int(w.numel()), "objective_first": hist[0],
             "objective_last": Jf, "objective": hist, "ms_per_step": float(np.mean(times[1:] if steps > 1 else times)),
             "ms_per_step_p50": float(np.percentile(times, 50)), "total_ms": float(np.sum(times)),
-            "step": "projected Adam; forward dose + hand-written adjoint per step"}
+            "mode": mode, "step": "projected Adam; forward dose + hand-written adjoint per step"}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--lr-frac", type=float, default=0.05)
+    ap.add_argument("--mode", choices=("full", "resident", "graph"), default="graph")
     a = ap.parse_args()
-    out = run(a.steps, a.lr_frac)
+    out = run(a.steps, a.lr_frac, mode=a.mode)
     out["objective"] = [round(v, 6) for v in out["objective"]]
     print(json.dumps(out))
     return 0
