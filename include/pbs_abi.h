@@ -57,6 +57,19 @@ typedef struct pbs_beam_settings {
     int numRoiLinearIndices;
     int useSubspotConvolution;    /* 1: per-subspot lattice convolution (reference branch
                                      wrapper:3770-3822, compiled in but switched off there) */
+    /* Water-equivalent input of the CarbonPBS contract (RTDBeamSettings::waterEquivalence and
+     * rayWeqHeader, raytracedicom_integration.h:63-70): a 9-value header [depth_start,
+     * depth_step, depth_n, y_start, y_step, y_n, x_start, x_step, x_n] optionally followed by
+     * x_n * y_n * depth_n water-equivalent depths at index (x * y_n + y) * depth_n + k (values past
+     * that count are ignored, wrapper:3566-3590). With samples, they replace the CT tracer
+     * (fillBevFromWeqVolumeKernel, wrapper:1923-1988), the step length and count come from the
+     * header and the start plane is aligned to depth_start (wrapper:3305-3370). NULL / 0: CT
+     * tracer. These members sit at the end so that zero-initialised older callers keep working. */
+    const float *waterEquivalence;
+    size_t waterEquivalenceCount;
+    int spotPositionsAreIndices;  /* 1: spotPositions are WEQ-lattice texel coordinates (+0.5 at
+                                     texel centres), rasterised bilinearly onto that lattice
+                                     (wrapper:931-968, 1295-1364); needs the header */
 } pbs_beam_settings;
 
 /* Energy/LUT payload: POD mirror of RTDEnergyStruct (raytracedicom_integration.h:97-128). */
@@ -93,6 +106,9 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
 
 /* Message of the last failing call on this thread (empty string if none). */
 const char *pbs_last_error(void);
+/* sizeof of the ABI structs as compiled into the library (bindings check their mirrors against
+ * it): 0 pbs_beam_settings, 1 pbs_energy_struct; -1 for another value. No GPU needed. */
+long pbs_abi_sizeof(int which);
 
 /* Number of visible HIP devices (0 when none; never fails). */
 int pbs_device_count(void);

@@ -45,6 +45,7 @@ class Beam(C.Structure):
         ("beamDirection", C.c_float * 3), ("bmXDirection", C.c_float * 3),
         ("bmYDirection", C.c_float * 3), ("sourcePosition", C.c_float * 3),
         ("sad", C.c_float), ("useSubspotConvolution", C.c_int), ("nuclear", C.c_int),
+        ("weq", _f), ("weqCount", C.c_size_t), ("spotIndices", C.c_int),
     ]
 
 
@@ -76,6 +77,9 @@ class Plan(C.Structure):
         ("numLayers", C.c_int), ("error", C.c_char * 256),
         ("nucDims", C.c_int * 2), ("spotDist", C.c_float), ("nucRadHist", (C.c_int * 34) * 256),
         ("entrySpotSigmaX", C.c_float * 256),
+        ("weqVolume", C.c_int), ("spotIndices", C.c_int), ("weqDims", C.c_int * 3),
+        ("weqStepMm", C.c_float), ("weqX0", C.c_float), ("weqY0", C.c_float), ("weqDx", C.c_float),
+        ("weqDy", C.c_float), ("weqBody", _f),
     ]
 
 
@@ -95,6 +99,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.orc_sizeof.restype = C.c_long
+        L.orc_sizeof.argtypes = [C.c_int]
         L.orc_find_decimal_ordered.restype = C.c_float
         L.orc_find_decimal_ordered.argtypes = [_f, C.c_int, C.c_float]
         L.orc_vector_interpolate.restype = C.c_float
@@ -216,6 +222,14 @@ def make_beam(beam: dict, keep: _Keep, use_subspots: bool = False) -> Beam:
     b.sourcePosition[:] = [float(v) for v in beam["source_position"]]
     b.sad = float(beam["sad"])
     b.useSubspotConvolution = 1 if use_subspots else 0
+    for key in ("water_equivalence", "ray_weq_header"):  # the WEQ header in use, then its samples
+        v = beam.get(key)
+        if v is not None and np.asarray(v).size >= 9:
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).ravel())
+            b.weq = keep.f32(a)
+            b.weqCount = int(a.size)
+            break
+    b.spotIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
     return b
 
 
@@ -259,6 +273,7 @@ def plan_beam(case: dict, beam_index: int = 0, use_subspots: bool = False) -> Pl
     p = Plan()
     lib().orc_plan_beam(_ip(ct_dims), _fp(ct_res), _fp(ct_corner), _ip(dose_dims), _fp(dose_res),
                         _fp(dose_corner), C.byref(bm), C.byref(en), C.byref(p))
+    p._keep = keep  # the plan points at the beam's WEQ samples
     return p
 
 

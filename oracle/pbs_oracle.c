@@ -297,6 +297,44 @@ static float *build_lattice(const orc_beam *b, orc_plan *p) {
     int total = 0;
     for (int l = 0; l < b->numLayers; ++l) total += b->layerSpotCounts[l];
     if (total <= 0 || total != b->numSpots) return NULL;
+    if (b->spotIndices) {
+        /* index mode (wrapper:1295-1364): the WEQ ray lattice of the header; each spot split
+         * bilinearly at its texel coordinate - 1/2 over the four cells around it (wrapper:931-968) */
+        if (!b->weq || b->weqCount < 9) return NULL;
+        const float *h = b->weq;
+        int nx = (int)lroundf(h[8]), ny = (int)lroundf(h[5]);
+        if (nx < 1) nx = 1;
+        if (ny < 1) ny = 1;
+        float *dense = (float *)calloc((size_t)nx * ny * b->numLayers, sizeof(float));
+        int off = 0;
+        for (int l = 0; l < b->numLayers; ++l) {
+            int cnt = b->layerSpotCounts[l];
+            if (cnt <= 0) { free(dense); return NULL; }
+            for (int i = 0; i < cnt; ++i) {
+                int s = off + i;
+                float w = b->spotWeights[s];
+                if (!isfinite(w) || w < 0.0f) { free(dense); return NULL; }
+                if (w == 0.0f) continue;
+                float lx = b->spotPositions[2 * s] - 0.5f, ly = b->spotPositions[2 * s + 1] - 0.5f;
+                int x0 = (int)floorf(lx), y0 = (int)floorf(ly);
+                float fx = lx - (float)x0, fy = ly - (float)y0;
+                float frac[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+                int any = 0;
+                for (int q = 0; q < 4; ++q) {
+                    int cx = x0 + (q & 1), cy = y0 + (q >> 1);
+                    if (!(frac[q] > 0.0f) || cx < 0 || cx >= nx || cy < 0 || cy >= ny) continue;
+                    dense[((size_t)l * ny + cy) * nx + cx] += w * frac[q];
+                    any = 1;
+                }
+                if (!any) { free(dense); return NULL; }
+            }
+            off += cnt;
+        }
+        p->spotDims[0] = nx; p->spotDims[1] = ny;
+        p->spotOffset[0] = h[6]; p->spotOffset[1] = h[3];
+        p->spotDelta[0] = h[7]; p->spotDelta[1] = h[4];
+        return dense;
+    }
     float *xs = (float *)malloc(sizeof(float) * total), *ys = (float *)malloc(sizeof(float) * total);
     for (int i = 0; i < total; ++i) { xs[i] = b->spotPositions[2 * i]; ys[i] = b->spotPositions[2 * i + 1]; }
     qsort(xs, total, sizeof(float), cmp_float);
@@ -349,8 +387,13 @@ static int infer_source_dist(const orc_beam *b, f3 bmX, f3 bmY, f3 bmZ, float sa
     if (!(sad > 1.0e-6f) || !b->spotBeamDirections || !b->spotPositions || b->numSpots <= 0) return 0;
     float *ax = (float *)malloc(sizeof(float) * b->numSpots), *ay = (float *)malloc(sizeof(float) * b->numSpots);
     int nx = 0, ny = 0;
+    if (b->spotIndices && (!b->weq || b->weqCount < 9)) { free(ax); free(ay); return 0; }
     for (int i = 0; i < b->numSpots; ++i) {
         float px = b->spotPositions[2 * i], py = b->spotPositions[2 * i + 1];
+        if (b->spotIndices) { /* texel coordinate -> reference plane (wrapper:1881-1888) */
+            px = b->weq[6] + (px - 0.5f) * b->weq[7];
+            py = b->weq[3] + (py - 0.5f) * b->weq[4];
+        }
         f3 d = mk3(b->spotBeamDirections[3 * i], b->spotBeamDirections[3 * i + 1], b->spotBeamDirections[3 * i + 2]);
         float len = sqrtf(dot3(d, d));
         if (!(len > 1.0e-6f)) continue;
@@ -373,6 +416,12 @@ static int infer_source_dist(const orc_beam *b, f3 bmX, f3 bmY, f3 bmZ, float sa
     *sy = ny ? ay[ny / 2] : 0.0f;
     free(ax); free(ay);
     return (*sx > 0.0f || *sy > 0.0f);
+}
+
+/* sizeof of the structs as compiled (the Python mirrors are checked against it) */
+long orc_sizeof(int which) {
+    return which == 0 ? (long)sizeof(orc_beam) : which == 1 ? (long)sizeof(orc_plan)
+                      : which == 2 ? (long)sizeof(orc_energy) : -1;
 }
 
 int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3],
@@ -527,11 +576,41 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
                 minGZ = fminf(minGZ, g.z); maxGZ = fmaxf(maxGZ, g.z);
             }
     float stepLen = ctRes[2] * lenToMm;
-    int steps = (int)ceilf((maxGZ - minGZ) / fabsf(stepLen)) + 1;
-    p->steps = steps > 1 ? steps : 1;
+    float startZ = maxGZ;
+    p->weqVolume = b->weq && b->weqCount > 9;
+    p->spotIndices = b->spotIndices != 0;
+    if (p->weqVolume) {
+        /* WEQ volume (wrapper:3305-3370): step length and count from the header; the start plane
+         * moves to (mean source distance - depth_start) when that is within the volume's depth
+         * span, or 50 mm, of the CT's */
+        const float *h = b->weq;
+        if (h[1] * lenToMm > 0.0f) stepLen = h[1] * lenToMm;
+        float proj = h[0] * lenToMm;
+        float aligned = 0.5f * (p->sourceDist[0] + p->sourceDist[1]) - proj;
+        long nh = lroundf(h[2]);
+        float span = fabsf(h[1] * lenToMm) * (float)(nh > 0 ? nh : 0);
+        float allowed = span > 50.0f ? span : 50.0f;
+        if (isfinite(aligned) && isfinite(proj) && proj > 0.0f && fabsf(aligned - maxGZ) <= allowed) startZ = aligned;
+        p->steps = nh > 1 ? (int)nh : 1;
+        int nx = (int)lroundf(h[8]), ny = (int)lroundf(h[5]);
+        if (nx < 1) nx = 1;
+        if (ny < 1) ny = 1;
+        if (b->weqCount - 9 < (size_t)nx * ny * (size_t)p->steps) {
+            snprintf(p->error, sizeof(p->error), "water_equivalence payload is smaller than header-declared nx*ny*nStep");
+            return -1;
+        }
+        p->weqDims[0] = nx; p->weqDims[1] = ny; p->weqDims[2] = p->steps;
+        p->weqStepMm = fabsf(h[1] * lenToMm);
+        p->weqX0 = h[6] / lenToMm; p->weqY0 = h[3] / lenToMm;
+        p->weqDx = h[7] / lenToMm; p->weqDy = h[4] / lenToMm;
+        p->weqBody = h + 9;
+    } else {
+        int steps = (int)ceilf((maxGZ - minGZ) / fabsf(stepLen)) + 1;
+        p->steps = steps > 1 ? steps : 1;
+    }
     p->fanCorner[0] = p->cpbCorner[0] * lenToMm;
     p->fanCorner[1] = p->cpbCorner[1] * lenToMm;
-    p->fanCorner[2] = maxGZ;
+    p->fanCorner[2] = startZ;
     p->fanDelta[0] = p->cpbRes[0] * lenToMm;
     p->fanDelta[1] = p->cpbRes[1] * lenToMm;
     p->fanDelta[2] = -fabsf(stepLen);
@@ -568,8 +647,47 @@ static void fan_to_index(const fan_geom *g, double fi, double fj, double fk, dou
     for (int r = 0; r < 3; ++r) out[r] = g->m[3 * r] * x + g->m[3 * r + 1] * y + g->m[3 * r + 2] * z + g->o[r];
 }
 
+/* fillBevFromWeqVolumeKernel (wrapper:1923-1988): per ray, the nearest WEQ column (float index
+ * arithmetic, rounded half away from zero as lroundf); WEPL(k) = sample k (the last value read
+ * past the column or off the lattice), density = max(0, WEPL(k) - WEPL(k-1)) / step, entry / exit
+ * steps from the first / last density > 1e-4 (+1; none: K and 0). */
+static void weq_trace(const orc_plan *p, double *density, double *wepl, int *firstInside, int *firstOutside) {
+    const int rx = p->rayDims[0], ry = p->rayDims[1], K = p->steps;
+    const size_t plane = (size_t)rx * ry;
+    const int nx = p->weqDims[0], ny = p->weqDims[1], ns = p->weqDims[2];
+    const int active = K < ns ? K : ns;
+    const double step = p->weqStepMm > 0.0f ? (double)p->weqStepMm : 1.0;
+    for (int j = 0; j < ry; ++j) {
+        for (int i = 0; i < rx; ++i) {
+            const size_t r = (size_t)j * rx + i;
+            float rayX = p->cpbCorner[0] + (float)i * p->cpbRes[0], rayY = p->cpbCorner[1] + (float)j * p->cpbRes[1];
+            long qx = fabsf(p->weqDx) > 0.0f ? lroundf((rayX - p->weqX0) / p->weqDx) : -1;
+            long qy = fabsf(p->weqDy) > 0.0f ? lroundf((rayY - p->weqY0) / p->weqDy) : -1;
+            int in = qx >= 0 && qx < nx && qy >= 0 && qy < ny;
+            const float *col = in ? p->weqBody + ((size_t)qx * ny + (size_t)qy) * ns : NULL;
+            double prev = 0.0, last = 0.0;
+            int first = -1, lastMat = -1;
+            for (int k = 0; k < K; ++k) {
+                double w = last;
+                if (in && k < active) { w = (double)col[k]; last = w; }
+                double d = (w - prev > 0.0 ? w - prev : 0.0) / step;
+                wepl[(size_t)k * plane + r] = w;
+                density[(size_t)k * plane + r] = d;
+                if (d > 1.0e-4) { if (first < 0) first = k; lastMat = k; }
+                prev = w;
+            }
+            firstInside[r] = first >= 0 ? first + 1 : K;
+            firstOutside[r] = lastMat >= 0 ? lastMat + 1 : 0;
+        }
+    }
+}
+
 void orc_trace(const orc_plan *p, const float *ct, const int ctDims[3], const orc_energy *en,
                double *density, double *wepl, int *firstInside, int *firstOutside) {
+    if (p->weqVolume) {
+        weq_trace(p, density, wepl, firstInside, firstOutside);
+        return;
+    }
     const int rx = p->rayDims[0], ry = p->rayDims[1], K = p->steps;
     const size_t plane = (size_t)rx * ry;
     fan_geom g;
@@ -959,6 +1077,10 @@ void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *dens
 }
 
 int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeights) {
+    if (b->spotIndices) {
+        snprintf(p->error, sizeof(p->error), "the halo branch with WEQ-index spot positions is not supported");
+        return -1;
+    }
     float *dense = build_lattice(b, p);
     if (!dense) {
         snprintf(p->error, sizeof(p->error), "nuclear_correction=true requires raw spot lattice inputs");

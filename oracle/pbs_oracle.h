@@ -62,6 +62,11 @@ typedef struct {
     float sad;
     int useSubspotConvolution; /* 0 = per-layer spot sigma (reference default) */
     int nuclear;               /* halo branch: 0 off, 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA constants */
+    /* WEQ input (raytracedicom_integration.h:63-70): 9-value header [depth_start, depth_step,
+     * depth_n, y_start, y_step, y_n, x_start, x_step, x_n] + samples [x][y][k], or NULL */
+    const float *weq;
+    size_t weqCount;
+    int spotIndices;           /* spot positions are WEQ-lattice texel coordinates */
 } orc_beam;
 
 /* Beam plan: everything the host computes before and between kernels. */
@@ -105,7 +110,14 @@ typedef struct {
     float spotDist;
     int nucRadHist[ORC_MAX_LAYERS][ORC_MAX_RAD + 2];
     float entrySpotSigmaX[ORC_MAX_LAYERS];  /* spot sigma_x in mm (halo entry sigma) */
+    /* WEQ-volume input: samples replace the CT trace (wrapper:1923-1988, 3305-3370) */
+    int weqVolume, spotIndices;
+    int weqDims[3];                          /* x_n, y_n, depth_n */
+    float weqStepMm, weqX0, weqY0, weqDx, weqDy;
+    const float *weqBody;                    /* [x_n][y_n][depth_n] */
 } orc_plan;
+
+long orc_sizeof(int which);  /* 0 orc_beam, 1 orc_plan, 2 orc_energy */
 
 /* ---- helpers exposed for known-answer tests ---- */
 /* findDecimalOrdered (include/utils/vector_find.h:130-145). Pinned: KAT peak depths. */

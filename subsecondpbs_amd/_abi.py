@@ -33,6 +33,7 @@ class BeamSettings(C.Structure):
         ("sad", C.c_float), ("refPlaneZ", C.c_float),
         ("roiLinearIndices", _i), ("numRoiLinearIndices", C.c_int),
         ("useSubspotConvolution", C.c_int),
+        ("waterEquivalence", _f), ("waterEquivalenceCount", C.c_size_t), ("spotPositionsAreIndices", C.c_int),
     ]
 
 
@@ -82,7 +83,7 @@ class CPBGrid(C.Structure):
 EXPORTED = [
     "pbs_subsecond_wrapper", "pbs_last_error", "pbs_device_count",
     "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
-    "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
+    "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
     "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
@@ -105,6 +106,8 @@ def lib():
             raise RuntimeError(f"HIP engine library not built: {LIB_PATH} (run __graft_entry__.build())")
         L = C.CDLL(LIB_PATH)
         L.pbs_last_error.restype = C.c_char_p
+        L.pbs_abi_sizeof.restype = C.c_long
+        L.pbs_abi_sizeof.argtypes = [C.c_int]
         L.pbs_device_count.restype = C.c_int
         L.pbs_subsecond_wrapper.restype = C.c_int
         L.pbs_subsecond_wrapper.argtypes = [_f, _i, _f, _f, _f, _i, _f, _f, C.POINTER(BeamSettings), C.c_size_t,
@@ -324,7 +327,24 @@ def beam_settings(beam: dict, keep: Keep, use_subspots: bool | None = None) -> B
     if use_subspots is None:
         use_subspots = bool(beam.get("use_subspot_convolution", False))
     b.useSubspotConvolution = 1 if use_subspots else 0
+    weq = weq_vector(beam)
+    if weq is not None:
+        b.waterEquivalence = keep.f32(weq)
+        b.waterEquivalenceCount = int(weq.size)
+    b.spotPositionsAreIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
     return b
+
+
+def weq_vector(beam: dict):
+    """The beam's WEQ header (+ samples): ``water_equivalence``, else ``ray_weq_header``
+    (raytracedicom_pybind.cpp:1767-1781); None when neither holds the 9-value header."""
+    for key in ("water_equivalence", "ray_weq_header"):
+        v = beam.get(key)
+        if v is not None:
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).ravel())
+            if a.size >= 9:
+                return a
+    return None
 
 
 def i3(v):

@@ -57,10 +57,11 @@ extern "C" void subsecondWrapper(const float *ctData, const int3 &ctDims, const 
         b.refPlaneZ = s.refPlaneZ;
         b.roiLinearIndices = ptr_or_null(s.roiLinearIndices);
         b.numRoiLinearIndices = int(s.roiLinearIndices.size());
-        if (s.spotPositionsAreIndices || s.waterEquivalence.size() > 9) {
-            std::fprintf(stderr, "[RTD_ENTRY_ASSERT] beamIdx=%zu WEQ-indexed spots and WEQ volumes are not supported\n", i);
-            return;
-        }
+        // the WEQ header in use: waterEquivalence, else its header-only alias (wrapper:999-1002)
+        const std::vector<float> &weq = s.waterEquivalence.size() >= 9 ? s.waterEquivalence : s.rayWeqHeader;
+        b.waterEquivalence = weq.size() >= 9 ? weq.data() : nullptr;
+        b.waterEquivalenceCount = weq.size() >= 9 ? weq.size() : 0;
+        b.spotPositionsAreIndices = s.spotPositionsAreIndices ? 1 : 0;
         if (s.energies.size() != s.spotSigmas.size()) {
             std::fprintf(stderr, "[RTD_ENTRY_ASSERT] beamIdx=%zu spotSigmas size must match energies\n", i);
             return;

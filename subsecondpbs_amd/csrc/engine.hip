@@ -129,7 +129,7 @@ struct pbs_engine {
     int doseDims[3] = {0, 0, 0};
     float doseRes[3] = {0, 0, 0}, doseCorner[3] = {0, 0, 0};
     // per-beam work buffers
-    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, cpb, traceScratch;
+    DevBuf<float> density, wepl, weplMin, spot, interm, rayW, subspots, subEntry, cpb, traceScratch, weq;
     DevBuf<float2> iddSigma;
     DevBuf<int> fi, fo, firstPassive;
     // superposition -> merge -> transfer state of the primary (ray) grid and the halo grid
@@ -380,6 +380,10 @@ int validate_beam(const pbs_beam_settings &b, size_t idx) {
         for (int l = 0; l < b.numLayers; ++l) total += b.layerSpotCounts[l];
         if (b.spotWeights && total != b.numSpots) return set_error(PBS_E_INVALID, pre + "sum(layerSpotCounts) must match spotWeights size");
     }
+    if (b.waterEquivalenceCount > 0 && (!b.waterEquivalence || b.waterEquivalenceCount < 9))
+        return set_error(PBS_E_INVALID, pre + "water_equivalence needs its 9-value header");
+    if (b.spotPositionsAreIndices && (!b.waterEquivalence || b.waterEquivalenceCount < 9))
+        return set_error(PBS_E_INVALID, pre + "spotPositionsAreIndices requires a 9-value WEQ header");
     return PBS_OK;
 }
 
@@ -388,6 +392,12 @@ int validate_beam(const pbs_beam_settings &b, size_t idx) {
 extern "C" {
 
 const char *pbs_last_error(void) { return g_lastError.c_str(); }
+
+long pbs_abi_sizeof(int which) {
+    if (which == 0) return long(sizeof(pbs_beam_settings));
+    if (which == 1) return long(sizeof(pbs_energy_struct));
+    return -1;
+}
 
 int pbs_device_count(void) {
     int n = 0;
@@ -701,6 +711,8 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (adjoint && (halo || p.legacyPath))
         return set_error(PBS_E_INVALID, halo ? "the adjoint does not cover the nuclear halo branch"
                                              : "the adjoint needs spot positions and weights (spot-lattice path)");
+    if ((adjoint || keep) && p.spotIndices)
+        return set_error(PBS_E_INVALID, "the adjoint and resident beams need physical spot positions, not WEQ indices");
     if (keep && (halo || p.legacyPath))
         return set_error(PBS_E_INVALID, halo ? "resident beams do not cover the nuclear halo branch"
                                              : "resident beams need spot positions and weights (spot-lattice path)");
@@ -778,7 +790,17 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     };
     // 1. trace and per-beam reductions
     HIP_TRY(mark());
-    HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p, s));
+    if (p.weqVolume) {
+        // the beam's water-equivalent samples replace the CT trace (wrapper:3566-3640)
+        const size_t nb = size_t(p.weqDims[0]) * p.weqDims[1] * p.weqDims[2];
+        HIP_TRY(e->weq.ensure(nb));
+        HIP_TRY(hipMemcpyAsync(e->weq.p, beam->waterEquivalence + 9, sizeof(float) * nb, hipMemcpyHostToDevice, s));
+        WeqArgs wa{e->weq.p, p.weqDims[0], p.weqDims[1], p.weqDims[2], std::min(K, p.weqDims[2]), p.weqStepMm,
+                   p.cpbCorner[0], p.cpbCorner[1], p.cpbRes[0], p.cpbRes[1], p.weqX0, p.weqY0, p.weqDx, p.weqDy};
+        HIP_TRY(launch_weq_fill(g, wa, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
+    } else {
+        HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p, s));
+    }
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,

@@ -45,6 +45,15 @@ struct TransferArgs {
 
 // Scratch floats the trace needs (the segment prefixes and carries), 8-byte aligned start.
 size_t trace_scratch_floats(const FanGeom &g);
+// WEQ-volume input of the trace stage: samples [nx][ny][nSteps], lattice origin / spacing, the
+// ray grid's origin / spacing (same units), active steps and the step length in mm.
+struct WeqArgs {
+    const float *body;
+    int nx, ny, nSteps, active;
+    float stepMm, rayX0, rayY0, rayDx, rayDy, x0, y0, dx, dy;
+};
+hipError_t launch_weq_fill(const FanGeom &g, const WeqArgs &a, float *density, float *wepl, int *fi, int *fo,
+                          hipStream_t s);
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
                         int *fo, float *scratch, hipStream_t s);
 hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,

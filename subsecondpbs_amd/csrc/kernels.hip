@@ -251,6 +251,45 @@ __global__ __launch_bounds__(256) void k_trace_fix(FanGeom g, const double *__re
     wepl[idx] = float(segCarry[size_t(k / kTraceSeg) * nray + ray] + double(wepl[idx]));
 }
 
+// WEQ-volume input in place of the trace (fillBevFromWeqVolumeKernel, wrapper:1923-1988), thread
+// per ray: the ray's WEQ column is the one nearest its gantry (x, y) (rounded half away from
+// zero); WEPL(k) = the column's sample k (the last one read past the column or off the lattice),
+// density = max(0, WEPL(k) - WEPL(k-1)) / step; entry / exit steps from the first / last step
+// with density > 1e-4 (+1; none: K and 0).
+__global__ __launch_bounds__(256) void k_weq_fill(FanGeom g, WeqArgs a, float *__restrict__ density,
+                                                  float *__restrict__ wepl, int *__restrict__ fi,
+                                                  int *__restrict__ fo) {
+    const int r = int(blockIdx.x) * 256 + int(threadIdx.x);
+    const int plane = g.rx * g.ry;
+    if (r >= plane) return;
+    const int x = r % g.rx, y = r / g.rx;
+    const float rayX = a.rayX0 + float(x) * a.rayDx, rayY = a.rayY0 + float(y) * a.rayDy;
+    const float qx = fabsf(a.dx) > 0.0f ? roundf((rayX - a.x0) / a.dx) : -1.0f;
+    const float qy = fabsf(a.dy) > 0.0f ? roundf((rayY - a.y0) / a.dy) : -1.0f;
+    const bool in = qx >= 0.0f && qx < float(a.nx) && qy >= 0.0f && qy < float(a.ny);
+    const float *col = in ? a.body + (size_t(int(qx)) * a.ny + size_t(int(qy))) * a.nSteps : nullptr;
+    const float step = a.stepMm > 0.0f ? a.stepMm : 1.0f;
+    float prev = 0.0f, last = 0.0f;
+    int first = -1, lastMat = -1;
+    for (int k = 0; k < g.K; ++k) {
+        float w = last;
+        if (in && k < a.active) {
+            w = col[k];
+            last = w;
+        }
+        const float d = fmaxf(0.0f, w - prev) / step;
+        wepl[size_t(k) * plane + r] = w;
+        density[size_t(k) * plane + r] = d;
+        if (d > 1.0e-4f) {
+            if (first < 0) first = k;
+            lastMat = k;
+        }
+        prev = w;
+    }
+    fi[r] = first >= 0 ? first + 1 : g.K;
+    fo[r] = lastMat >= 0 ? lastMat + 1 : 0;
+}
+
 // blocks [0, K): min WEPL over rays that leave material; block K: entry/exit step extremes.
 __global__ __launch_bounds__(256) void k_trace_reduce(FanGeom g, const float *__restrict__ wepl,
                                                       const int *__restrict__ fi, const int *__restrict__ fo,
@@ -1754,6 +1793,13 @@ hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, fl
                        segCarry, fi, fo);
     if (g.K > kTraceSeg)
         hipLaunchKernelGGL(k_trace_fix, dim3((n + 255) / 256, g.K - kTraceSeg), dim3(256), 0, s, g, segCarry, wepl);
+    return hipGetLastError();
+}
+
+hipError_t launch_weq_fill(const FanGeom &g, const WeqArgs &a, float *density, float *wepl, int *fi, int *fo,
+                          hipStream_t s) {
+    const int n = g.rx * g.ry;
+    hipLaunchKernelGGL(k_weq_fill, dim3((n + 255) / 256), dim3(256), 0, s, g, a, density, wepl, fi, fo);
     return hipGetLastError();
 }
 

@@ -140,6 +140,47 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
     int total = 0;
     for (int l = 0; l < b.numLayers; ++l) total += b.layerSpotCounts[l];
     if (total <= 0 || total != b.numSpots) return false;
+    if (b.spotPositionsAreIndices) {
+        // index-coded spots (buildRawSpotLattice index mode, wrapper:1295-1364): the lattice is
+        // the WEQ ray lattice of the header; each spot is rasterised bilinearly at its texel
+        // coordinate - 1/2 (rasterizeContinuousSpotToDensePlane, wrapper:931-968)
+        if (!b.waterEquivalence || b.waterEquivalenceCount < 9) return false;
+        const float *h = b.waterEquivalence;
+        const int nx = std::max(1, int(std::lround(h[8]))), ny = std::max(1, int(std::lround(h[5])));
+        std::vector<float> dense(size_t(nx) * ny * b.numLayers, 0.0f);
+        int off = 0;
+        for (int l = 0; l < b.numLayers; ++l) {
+            const int cnt = b.layerSpotCounts[l];
+            if (cnt <= 0) return false;
+            for (int i = 0; i < cnt; ++i) {
+                const int s = off + i;
+                const float w = b.spotWeights[s];
+                if (!std::isfinite(w) || w < 0.0f) return false;
+                if (w == 0.0f) continue;
+                const float lx = b.spotPositions[2 * s] - 0.5f, ly = b.spotPositions[2 * s + 1] - 0.5f;
+                const int ix0 = int(std::floor(lx)), iy0 = int(std::floor(ly));
+                const float tx = lx - float(ix0), ty = ly - float(iy0);
+                bool wrote = false;
+                auto put = [&](int ix, int iy, float frac) {
+                    if (!(frac > 0.0f) || ix < 0 || ix >= nx || iy < 0 || iy >= ny) return;
+                    dense[(size_t(l) * ny + iy) * nx + ix] += w * frac;
+                    wrote = true;
+                };
+                put(ix0, iy0, (1.0f - tx) * (1.0f - ty));
+                put(ix0 + 1, iy0, tx * (1.0f - ty));
+                put(ix0, iy0 + 1, (1.0f - tx) * ty);
+                put(ix0 + 1, iy0 + 1, tx * ty);
+                if (!wrote) return false;
+            }
+            off += cnt;
+        }
+        p.spotCell.assign(size_t(total), -1);  // a spot spreads over up to four cells
+        p.spotDims[0] = nx; p.spotDims[1] = ny;
+        p.spotOffset[0] = h[6]; p.spotOffset[1] = h[3];
+        p.spotDelta[0] = h[7]; p.spotDelta[1] = h[4];
+        p.denseWeights.swap(dense);
+        return true;
+    }
     std::vector<float> xs(total), ys(total);
     for (int i = 0; i < total; ++i) { xs[i] = b.spotPositions[2 * i]; ys[i] = b.spotPositions[2 * i + 1]; }
     auto unique_sorted = [](std::vector<float> v) {
@@ -208,8 +249,15 @@ static bool align_grid(float minC, float maxC, float ref, float step, int multip
 static bool infer_source_dist(const pbs_beam_settings &b, F3 bmX, F3 bmY, F3 bmZ, float sad, float &sx, float &sy) {
     if (!(sad > 1.0e-6f) || !b.spotBeamDirections || !b.spotPositions || b.numSpots <= 0) return false;
     std::vector<float> ax, ay;
+    const bool idx = b.spotPositionsAreIndices != 0;
+    if (idx && (!b.waterEquivalence || b.waterEquivalenceCount < 9)) return false;
     for (int i = 0; i < b.numSpots; ++i) {
-        const float px = b.spotPositions[2 * i], py = b.spotPositions[2 * i + 1];
+        float px = b.spotPositions[2 * i], py = b.spotPositions[2 * i + 1];
+        if (idx) {  // texel coordinate -> reference-plane coordinate (wrapper:1881-1888)
+            const float *h = b.waterEquivalence;
+            px = h[6] + (px - 0.5f) * h[7];
+            py = h[3] + (py - 0.5f) * h[4];
+        }
         F3 d = f3(b.spotBeamDirections[3 * i], b.spotBeamDirections[3 * i + 1], b.spotBeamDirections[3 * i + 2]);
         const float len = std::sqrt(dot(d, d));
         if (!(len > 1.0e-6f)) continue;
@@ -376,11 +424,38 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
                 const F3 g = apply(w2g, add(ctCorner_mm, f3(float(cx) * csz.x, float(cy) * csz.y, float(cz) * csz.z)));
                 minGZ = std::fmin(minGZ, g.z); maxGZ = std::fmax(maxGZ, g.z);
             }
-    const float stepLen = ctRes[2] * lenToMm;
-    p.steps = std::max(1, int(std::ceil((maxGZ - minGZ) / std::fabs(stepLen))) + 1);
+    // a WEQ volume sets the step length and count and aligns the start plane to its depth_start
+    // when that lands within the volume's depth span (or 50 mm) of the CT's (wrapper:3305-3370)
+    p.weqVolume = b.waterEquivalence && b.waterEquivalenceCount > 9;
+    float stepLen = ctRes[2] * lenToMm;
+    float startZ = maxGZ;
+    if (p.weqVolume) {
+        const float *h = b.waterEquivalence;
+        const float sl = h[1] * lenToMm;
+        if (sl > 0.0f) stepLen = sl;
+        const float proj = h[0] * lenToMm;
+        const float aligned = 0.5f * (p.sourceDist[0] + p.sourceDist[1]) - proj;
+        const float span = std::fabs(h[1] * lenToMm) * float(std::max(0, int(std::lround(h[2]))));
+        if (std::isfinite(aligned) && std::isfinite(proj) && proj > 0.0f &&
+            std::fabs(aligned - maxGZ) <= std::max(span, 50.0f))
+            startZ = aligned;
+        p.steps = std::max(1, int(std::lround(h[2])));
+        const int nx = std::max(1, int(std::lround(h[8]))), ny = std::max(1, int(std::lround(h[5])));
+        if (b.waterEquivalenceCount - 9 < size_t(nx) * ny * size_t(p.steps)) {
+            p.error = "water_equivalence payload is smaller than header-declared nx*ny*nStep";
+            return false;
+        }
+        p.weqDims[0] = nx; p.weqDims[1] = ny; p.weqDims[2] = p.steps;
+        p.weqStepMm = std::fabs(h[1] * lenToMm);
+        p.weqX0 = h[6] / lenToMm; p.weqY0 = h[3] / lenToMm;
+        p.weqDx = h[7] / lenToMm; p.weqDy = h[4] / lenToMm;
+    } else {
+        p.steps = std::max(1, int(std::ceil((maxGZ - minGZ) / std::fabs(stepLen))) + 1);
+    }
+    p.spotIndices = b.spotPositionsAreIndices != 0;
     p.fanCorner[0] = p.cpbCorner[0] * lenToMm;
     p.fanCorner[1] = p.cpbCorner[1] * lenToMm;
-    p.fanCorner[2] = maxGZ;
+    p.fanCorner[2] = startZ;
     p.fanDelta[0] = p.cpbRes[0] * lenToMm;
     p.fanDelta[1] = p.cpbRes[1] * lenToMm;
     p.fanDelta[2] = -std::fabs(stepLen);
@@ -430,6 +505,10 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
 bool plan_halo(const BeamPlan &p, const pbs_beam_settings &b, HaloPlan &h, std::string &error) {
     if (p.legacyPath || p.spotDims[0] <= 0 || p.spotDims[1] <= 0) {
         error = "nuclear_correction=true requires raw spot lattice inputs; legacy CPB fallback cannot drive halo mode";
+        return false;
+    }
+    if (p.spotIndices) {
+        error = "the halo branch with WEQ-index spot positions is not supported by this build";
         return false;
     }
     const int rawX = p.spotDims[0], rawY = p.spotDims[1];

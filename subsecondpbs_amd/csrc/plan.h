@@ -68,6 +68,12 @@ struct BeamPlan {
     std::vector<LayerParams> layers;
     bool nuclear = false;
     bool subspotConv = false;
+    // WEQ-volume input (wrapper:1923-1988, 3305-3370, 3566-3640): the tracer is replaced by the
+    // beam's water-equivalent samples [x][y][k]; lattice of the samples in ray-grid units
+    bool weqVolume = false;
+    bool spotIndices = false;  // spot positions are WEQ-lattice texel coordinates
+    int weqDims[3] = {0, 0, 0};  // x_n, y_n, depth_n
+    float weqStepMm = 0.f, weqX0 = 0.f, weqY0 = 0.f, weqDx = 0.f, weqDy = 0.f;
 };
 
 // Explicit CPB grid in the gantry frame (used by cudaFinalDose, whose CPB grid comes from

@@ -386,6 +386,48 @@ def case_for_beam(case: dict, i: int) -> dict:
     return out
 
 
+def weq_volume(header, density_fn) -> np.ndarray:
+    """WEQ vector (header + samples [x][y][k], raytracedicom_integration.h:63-70) with
+    WEPL(x, y, k) = step x sum_{j <= k} density_fn(ix, iy, j)."""
+    h = np.asarray(header, dtype=np.float64)
+    nk, ny, nx = int(round(h[2])), int(round(h[5])), int(round(h[8]))
+    ix, iy, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nk), indexing="ij")
+    d = density_fn(ix, iy, k)
+    body = np.cumsum(d, axis=2) * h[1]
+    return np.concatenate([h, body.ravel()]).astype(np.float32)
+
+
+def make_weq_case(seed: int = 6, index_spots: bool = False, start_shift_mm: float = 4.0) -> dict:
+    """WEQ-volume input (the CarbonPBS contract): the config-1 CT bounds the geometry, but the
+    beam carries its own water-equivalent depths on an 81 x 71 lattice at 1 mm, 64 steps of 2 mm:
+    water-like density 1 + 0.01 ix (a lateral wedge along x only, so a transposed [x][y] index
+    shows), plus 0.5 in steps 20-29 for y >= 35 (a half slab). depth_start is set so the start
+    plane moves by ``start_shift_mm`` from the CT's. Spots: the config-1 11 x 11 field, 5 layers,
+    at physical positions or (``index_spots``) as WEQ texel coordinates (+0.5 at texel centres),
+    the corner spot of every layer a quarter texel off so that it splits over four cells."""
+    case = make_config1(seed)
+    beam = dict(case["beams"][0])
+    # gantry z of a world point is -z (origin at the isocentre), so the CT's start plane is at
+    # gantry z = 63; depth_start = (mean source distance) - (the start plane wanted)
+    depth_start = 2000.0 - (63.0 + start_shift_mm)
+    header = [depth_start, 2.0, 64.0, -35.0, 1.0, 71.0, -40.0, 1.0, 81.0]
+    slab = lambda ix, iy, k: 0.5 * ((k >= 20) & (k < 30) & (iy >= 35))
+    beam["water_equivalence"] = weq_volume(header, lambda ix, iy, k: 1.0 + 0.01 * ix + slab(ix, iy, k))
+    if index_spots:
+        pos = np.asarray(beam["spot_positions"], np.float32).reshape(-1, 2).copy()
+        raw = np.empty_like(pos)
+        raw[:, 0] = (pos[:, 0] - header[6]) / header[7] + 0.5
+        raw[:, 1] = (pos[:, 1] - header[3]) / header[4] + 0.5
+        per = len(pos) // len(beam["energies"])
+        raw[::per] += 0.25  # the first spot of each layer: between texel centres
+        beam["spot_positions"] = raw.ravel()
+        beam["spot_positions_are_indices"] = True
+    case["beams"] = [beam]
+    case["name"] = "weq-volume-64" + ("-index-spots" if index_spots else "")
+    case["weq_header"] = header
+    return case
+
+
 def make_config2(seed: int = 2, gantry_deg: float = 90.0) -> dict:
     """Head-size phantom with one 30-layer proton beam at gantry 90 deg (along +x)."""
     ct, dims, res, corner = head_phantom(seed)

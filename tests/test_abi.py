@@ -67,3 +67,15 @@ def test_cxx_dropin_symbols_exported():
     for name in ("subsecondWrapper", "createRTDBeamSettings", "destroyRTDBeamSettings",
                  "createRTDEnergyStruct", "destroyRTDEnergyStruct"):
         assert hasattr(lib, name), name
+
+
+def test_struct_mirrors_match_the_library():
+    """The ctypes mirrors of pbs_beam_settings / pbs_energy_struct have the compiled sizes."""
+    import ctypes as C
+
+    from subsecondpbs_amd import _abi
+
+    L = _abi.lib()
+    assert L.pbs_abi_sizeof(0) == C.sizeof(_abi.BeamSettings)
+    assert L.pbs_abi_sizeof(1) == C.sizeof(_abi.EnergyStruct)
+    assert L.pbs_abi_sizeof(7) == -1
