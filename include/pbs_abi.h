@@ -200,6 +200,11 @@ int pbs_engine_set_nuclear(pbs_engine *e, int model);
  * radius histograms [L][34], sticky overflow flag. Any pointer may be NULL. */
 int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float *haloIddSigma,
                           int *haloRadHist, int *radOverflow);
+/* The last beam's batched launch radius of every radius bin, [numLayers][34] (the batching of
+ * complete_superposition.cu:482-490 applied on the device: bin r's tiles run at radius
+ * effRad[r]); haloEffRad likewise for the halo grid (NULL to skip; the last beam must have
+ * used the halo branch). For the parity tests. */
+int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad);
 
 /* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
  * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside

@@ -85,7 +85,7 @@ EXPORTED = [
     "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
     "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
     "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
-    "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo",
+    "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo", "pbs_engine_debug_bins",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
@@ -149,6 +149,8 @@ def lib():
         L.pbs_engine_get_tuning.argtypes = [C.c_void_p, C.c_int, _i]
         L.pbs_engine_set_nuclear.restype = C.c_int
         L.pbs_engine_set_nuclear.argtypes = [C.c_void_p, C.c_int]
+        L.pbs_engine_debug_bins.restype = C.c_int
+        L.pbs_engine_debug_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.pbs_engine_debug_halo.restype = C.c_int
         L.pbs_engine_debug_halo.argtypes = [C.c_void_p, _i, _f, C.c_void_p, C.c_void_p, _i]
         L.pbs_batch_create.restype = C.c_void_p

@@ -541,6 +541,17 @@ int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float 
     return PBS_OK;
 }
 
+int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad) {
+    if (!e || !e->haveLast) return set_error(PBS_E_INVALID, "no beam has been computed");
+    if (haloEffRad && !e->lastHalo) return set_error(PBS_E_INVALID, "the last beam ran without the halo branch");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    const size_t n = size_t(e->plan.numLayers) * (kMaxRadDev + 2);
+    if (effRad) HIP_TRY(hipMemcpy(effRad, e->prim.effRad.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (haloEffRad) HIP_TRY(hipMemcpy(haloEffRad, e->halo.effRad.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    return PBS_OK;
+}
+
 int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const float res[3], const float corner[3],
                       int ctIsDevice) {
     if (!e || !ct || !dims || !res || !corner) return set_error(PBS_E_INVALID, "ctData must not be null");
@@ -772,16 +783,6 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                           q.convA, q.convB, q.sigmaX, q.sigmaY, 0.0f, q.airA, q.airC, 0.0f, 0.0f};
     }
     hipStream_t s = e->stream;
-    HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
-    if (halo) {
-        const HaloPlan &hp = e->haloPlan;
-        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, hp.rayToNuc.data(), sizeof(int) * nray, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(e->haloW.p, hp.weights.data(), sizeof(float) * hp.weights.size(),
-                               hipMemcpyHostToDevice, s));
-        HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
-    }
-    HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
-                           hipMemcpyHostToDevice, s));
     std::vector<hipEvent_t> marks;
     auto mark = [&]() {
         if (!e->profiling) return hipSuccess;
@@ -802,6 +803,18 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p, s));
     }
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
+    // the per-layer values and the subspot table go up behind the trace (copies from pageable
+    // memory hold the host, so the trace is already queued while they run)
+    HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    if (halo) {
+        const HaloPlan &hp = e->haloPlan;
+        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, hp.rayToNuc.data(), sizeof(int) * nray, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(e->haloW.p, hp.weights.data(), sizeof(float) * hp.weights.size(),
+                               hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
+    }
+    HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
+                           hipMemcpyHostToDevice, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,

@@ -183,10 +183,32 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
     }
     std::vector<float> xs(total), ys(total);
     for (int i = 0; i < total; ++i) { xs[i] = b.spotPositions[2 * i]; ys[i] = b.spotPositions[2 * i + 1]; }
-    auto unique_sorted = [](std::vector<float> v) {
-        std::sort(v.begin(), v.end());
+    // sorted values with neighbours within 1e-4 of the last kept one merged (the unique-x /
+    // unique-y step inference, wrapper:1366-1411). Exact repeats (a lattice repeats its positions
+    // in every layer) are dropped first through a small hash set, then only the distinct values
+    // are sorted: removing exact repeats changes neither the sorted order nor which value heads
+    // a merged run
+    auto unique_sorted = [](const std::vector<float> &v) {
+        size_t cap = 64;
+        while (cap < 2 * v.size()) cap <<= 1;
+        std::vector<uint32_t> slot(cap, 0u);
+        std::vector<char> used(cap, 0);
+        std::vector<float> d;
+        for (float x : v) {
+            const float f = x + 0.0f;  // -0 and +0 are one value
+            uint32_t bits;
+            std::memcpy(&bits, &f, 4);
+            size_t h = size_t(bits * 2654435761u) & (cap - 1);
+            while (used[h] && slot[h] != bits) h = (h + 1) & (cap - 1);
+            if (!used[h]) {
+                used[h] = 1;
+                slot[h] = bits;
+                d.push_back(f);
+            }
+        }
+        std::sort(d.begin(), d.end());
         std::vector<float> u;
-        for (float x : v) if (u.empty() || !(std::fabs(u.back() - x) <= 1e-4f)) u.push_back(x);
+        for (float x : d) if (u.empty() || !(std::fabs(u.back() - x) <= 1e-4f)) u.push_back(x);
         return u;
     };
     const std::vector<float> ux = unique_sorted(xs), uy = unique_sorted(ys);

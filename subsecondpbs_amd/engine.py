@@ -140,6 +140,16 @@ class Engine:
             out[k] = int(v.value)
         return out
 
+    def debug_bins(self, halo: bool = False) -> np.ndarray:
+        """The last beam's batched launch radius per radius bin, [numLayers][34] (halo grid if
+        ``halo``)."""
+        L = self.debug_fetch(())["ints"]["numLayers"]
+        out = np.zeros((L, 34), np.int32)
+        ptr = C.c_void_p(out.ctypes.data)
+        rc = _abi.lib().pbs_engine_debug_bins(self.handle, None if halo else ptr, ptr if halo else None)
+        _abi.check(rc, "pbs_engine_debug_bins")
+        return out
+
     def set_profiling(self, on: bool = True):
         _abi.check(_abi.lib().pbs_engine_set_profiling(self.handle, int(bool(on))), "pbs_engine_set_profiling")
 

@@ -255,3 +255,36 @@ def test_config2_dose_within_half_percent(config2_gpu):
     print(f"config2 max|dD|/Dmax = {err:.3e}  Dmax={d0.max():.4e}")
     assert err <= DOSE_TOL
     assert abs(dose.sum() / d0.sum() - 1.0) < 1e-3
+
+
+def _rule_bins(hist):
+    """The oracle's batching rule (complete_superposition.cu:413-490) on a radius histogram."""
+    eff = np.zeros(34, np.int32)
+    bt = np.zeros(33, np.int32)
+    h = np.ascontiguousarray(hist, np.int32)
+    O.lib().orc_batch_bins(O._ip(h), O._ip(bt), O._ip(eff))
+    return eff
+
+
+@gpu
+@pytest.mark.parametrize("which", ["config1", "config2"])
+def test_batch_bins_bit_exact(which, request):
+    """Batched launch radii from the device: the oracle's rule on the GPU's own radius histograms
+    (both configs), and on the oracle's histograms (config 1, whose histograms match exactly)."""
+    torch = _torch()
+    from subsecondpbs_amd.engine import Engine
+    case, ref = request.getfixturevalue(which)
+    eng = Engine(case["energy"])
+    eng.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    eng.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    nx, ny, nz = case["dose_dims"]
+    dose = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+    eng.beam_dose(case["beams"][0], dose)
+    eng.sync()
+    hist = eng.debug_fetch(("radHist",))["radHist"]
+    eff = eng.debug_bins()
+    eng.close()
+    for l in range(hist.shape[0]):
+        np.testing.assert_array_equal(eff[l], _rule_bins(hist[l]), err_msg=f"layer {l}")
+        if which == "config1":
+            np.testing.assert_array_equal(eff[l], _rule_bins(np.array(ref["plan"]["radHist"][l], np.int32)))
