@@ -153,6 +153,10 @@ struct pbs_engine {
     bool haveLast = false;
     // host staging of the last beam's uploads; the next beam waits for uploadDone before reuse
     std::vector<LayerHost> hostLayers;
+    // pinned staging of a beam's uploads (per-layer values, subspot table, spot lattice, halo
+    // arrays): copies from it are plain DMA, queued without holding the host
+    char *pin = nullptr;
+    size_t pinBytes = 0;
     hipEvent_t uploadDone = nullptr;
     // radius classes of the superposition run concurrently on the engine stream and these
     // side streams (fork after the work lists, join before the merge)
@@ -489,6 +493,7 @@ void pbs_engine_destroy(pbs_engine *e) {
     for (auto &m : e->evBeams)
         for (hipEvent_t ev : m) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : e->evPool) (void)hipEventDestroy(ev);
+    if (e->pin) (void)hipHostFree(e->pin);
     delete e;
 }
 
@@ -694,6 +699,21 @@ static int run_grid_adjoint(pbs_engine *e, const BeamPlan &p, const FanGeom &g, 
     return PBS_OK;
 }
 
+// Pinned staging of at least `bytes` (the engine's earlier uploads from it have completed: callers
+// wait for uploadDone first).
+static hipError_t ensure_pin(pbs_engine *e, size_t bytes) {
+    if (bytes <= e->pinBytes && e->pin) return hipSuccess;
+    if (e->pin) (void)hipHostFree(e->pin);
+    e->pin = nullptr;
+    e->pinBytes = 0;
+    const size_t n = std::max<size_t>(bytes, size_t(1) << 20);
+    const hipError_t r = hipHostMalloc(reinterpret_cast<void **>(&e->pin), n);
+    if (r == hipSuccess) e->pinBytes = n;
+    return r;
+}
+
+static size_t pin_align(size_t n) { return (n + 255) & ~size_t(255); }
+
 // cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
@@ -805,16 +825,25 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     // the per-layer values and the subspot table go up behind the trace (copies from pageable
     // memory hold the host, so the trace is already queued while they run)
-    HIP_TRY(hipMemcpyAsync(e->layerHost.p, lh.data(), sizeof(LayerHost) * L, hipMemcpyHostToDevice, s));
+    const size_t bLayers = sizeof(LayerHost) * L, bSub = sizeof(float) * L * beam->maxSubspotsPerLayer * 5;
+    const size_t bDense = p.legacyPath ? 0 : sizeof(float) * p.denseWeights.size();
+    const size_t bRay = halo ? sizeof(int) * nray : 0, bHaloW = halo ? sizeof(float) * e->haloPlan.weights.size() : 0;
+    const size_t oSub = pin_align(bLayers), oDense = oSub + pin_align(bSub), oRay = oDense + pin_align(bDense);
+    const size_t oHaloW = oRay + pin_align(bRay);
+    HIP_TRY(ensure_pin(e, oHaloW + pin_align(bHaloW)));
+    std::memcpy(e->pin, lh.data(), bLayers);
+    std::memcpy(e->pin + oSub, beam->subspotData, bSub);
+    if (bDense) std::memcpy(e->pin + oDense, p.denseWeights.data(), bDense);
+    HIP_TRY(hipMemcpyAsync(e->layerHost.p, e->pin, bLayers, hipMemcpyHostToDevice, s));
     if (halo) {
         const HaloPlan &hp = e->haloPlan;
-        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, hp.rayToNuc.data(), sizeof(int) * nray, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(e->haloW.p, hp.weights.data(), sizeof(float) * hp.weights.size(),
-                               hipMemcpyHostToDevice, s));
+        std::memcpy(e->pin + oRay, hp.rayToNuc.data(), bRay);
+        std::memcpy(e->pin + oHaloW, hp.weights.data(), bHaloW);
+        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, e->pin + oRay, bRay, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(e->haloW.p, e->pin + oHaloW, bHaloW, hipMemcpyHostToDevice, s));
         HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
     }
-    HIP_TRY(hipMemcpyAsync(e->subspots.p, beam->subspotData, sizeof(float) * L * beam->maxSubspotsPerLayer * 5,
-                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(e->subspots.p, e->pin + oSub, bSub, hipMemcpyHostToDevice, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
@@ -843,7 +872,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
         HIP_TRY(e->spot.ensure(splane * L));
         HIP_TRY(e->interm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
-        HIP_TRY(hipMemcpyAsync(e->spot.p, p.denseWeights.data(), sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(e->spot.p, e->pin + oDense, sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
         const ConvArgs a = make_conv_args(p, rx, ry, nSub, subMode);
         HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
                             e->subEntry.p, L, s));
