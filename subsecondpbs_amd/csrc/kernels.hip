@@ -1430,17 +1430,16 @@ struct RowsShape {
     static_assert(2 * RM + 1 <= G - 1, "the last lane of a group must stay unused (pair combine)");
 };
 
-// One source row of one layer into this lane's output row: for every live source x_s (wave-
-// uniform skip), a = d e_v with v = |dy| clamped to R+1 (a zero entry), then 2R+1 FMAs along
-// the row, acc[x_s + dx + RM] += a e_|dx|. RP_ is the radius compiled; entries past the
+// One source row of one layer into this lane's output row: for every source x_s, a = d e_v with
+// v = |dy| clamped to R+1 (a zero entry), then 2R+1 FMAs along the row, acc[x_s + dx + RM] +=
+// a e_|dx|. Dead sources (d = 0) add exact zeros; running them keeps the sweep branch-free,
+// which measured faster than skipping them. RP_ is the radius compiled; entries past the
 // layer's own radius are zero, so a smaller layer radius adds exact zeros.
 template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v,
-                                           uint32_t wm) {
+__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v) {
     constexpr int NE4 = (RP_ + 2 + 3) / 4;
 #pragma unroll
     for (int xs = 0; xs < 32; ++xs) {
-        if (!((wm >> xs) & 1u)) continue;
         const float *E = rowE + xs * EW;
         float e[4 * NE4];
 #pragma unroll
@@ -1458,13 +1457,12 @@ __device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const floa
 }
 
 template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v,
-                                              uint32_t wm) {
+__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v) {
     if constexpr (RP_ <= RM) {
         if (rp == RP_)
-            rows_sweep<RP_, RM, EW>(acc, rowE, v, wm);
+            rows_sweep<RP_, RM, EW>(acc, rowE, v);
         else
-            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v, wm);
+            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v);
     }
 }
 
@@ -1585,7 +1583,7 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
                         make_float4(row[4 * m], row[4 * m + 1], row[4 * m + 2], row[4 * m + 3]);
             }
             lds_barrier();
-            // skip a source only when no group of the wave has it live
+            // skip the layer when no source row of the wave has a live pixel
             uint32_t wmv = 0;
 #pragma unroll
             for (int h = 0; h < 64 / G; ++h) wmv |= s_live[buf][(w * 64) / G + h];
@@ -1593,7 +1591,7 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
             if (wm) {
                 const int v = ady < R + 1 ? ady : R + 1;
                 const int rp = R > S::RLO ? R : S::RLO;
-                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v, wm);
+                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v);
             }
         }
         __syncthreads();  // tables dead: the block reuses their LDS
