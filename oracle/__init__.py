@@ -132,6 +132,10 @@ def lib():
         L.orc_transfer.argtypes = [C.POINTER(Plan), C.c_int, _d, _d, _i]
         L.orc_beam_dose.restype = C.c_int
         L.orc_beam_dose.argtypes = [_f, _i, _f, _f, _d, _i, _f, _f, C.POINTER(Beam), C.POINTER(Energy), C.POINTER(Plan)]
+        L.orc_beam_dose_entry.restype = C.c_int
+        L.orc_beam_dose_entry.argtypes = [_f, _i, _f, _f, _d, _i, _f, _f, C.POINTER(Beam), C.POINTER(Energy), _i,
+                                          C.POINTER(Plan)]
+        L.orc_trace_entry_margin.argtypes = [C.POINTER(Plan), _f, _i, _d]
         L.orc_beam_adjoint.restype = C.c_int
         L.orc_beam_adjoint.argtypes = [_f, _i, _f, _f, _d, _i, _f, _f, C.POINTER(Beam), C.POINTER(Energy), _d,
                                        C.POINTER(Plan)]
@@ -278,9 +282,11 @@ def plan_beam(case: dict, beam_index: int = 0, use_subspots: bool = False) -> Pl
 
 
 def beam_dose(case: dict, beam_index: int = 0, dose: np.ndarray | None = None,
-              use_subspots: bool = False, ct_hu_plus_1000: np.ndarray | None = None, nuclear: int = 0):
+              use_subspots: bool = False, ct_hu_plus_1000: np.ndarray | None = None, nuclear: int = 0,
+              entry_override: np.ndarray | None = None):
     """FP64 dose of one beam accumulated into ``dose`` ([nz][ny][nx] float64); returns (dose, plan).
-    ``nuclear``: halo branch off (0) or its model: 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA."""
+    ``nuclear``: halo branch off (0) or its model: 1 GAUSS_FIT, 2 SOUKUP, 3 FLUKA.
+    ``entry_override`` ([ry][rx] int32): a ray's entry step where >= 0 in place of the trace's."""
     keep = _Keep()
     en = make_energy(case["energy"], keep)
     bm = make_beam(case["beams"][beam_index], keep, use_subspots)
@@ -291,11 +297,25 @@ def beam_dose(case: dict, beam_index: int = 0, dose: np.ndarray | None = None,
     if dose is None:
         dose = np.zeros((dose_dims[2], dose_dims[1], dose_dims[0]), np.float64)
     p = Plan()
-    rc = lib().orc_beam_dose(_fp(ct), _ip(ct_dims), _fp(ct_res), _fp(ct_corner), _dp(dose), _ip(dose_dims),
-                             _fp(dose_res), _fp(dose_corner), C.byref(bm), C.byref(en), C.byref(p))
+    ov = None if entry_override is None else np.ascontiguousarray(entry_override, np.int32)
+    rc = lib().orc_beam_dose_entry(_fp(ct), _ip(ct_dims), _fp(ct_res), _fp(ct_corner), _dp(dose), _ip(dose_dims),
+                                   _fp(dose_res), _fp(dose_corner), C.byref(bm), C.byref(en),
+                                   None if ov is None else _ip(ov), C.byref(p))
     if rc != 0:
         raise RuntimeError("oracle: " + p.error.decode())
     return dose, p
+
+
+def entry_margin(case: dict, beam_index: int = 0, ct_hu_plus_1000: np.ndarray | None = None) -> np.ndarray:
+    """Per ray ([ry][rx]), how far the cumulative HU+1000 around the entry step lies from the
+    150 threshold (orc_trace_entry_margin)."""
+    p = plan_beam(case, beam_index)
+    ct_dims = case_args(case)[0]
+    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
+    ct = np.ascontiguousarray(ct, dtype=np.float32)
+    m = np.zeros((p.rayDims[1], p.rayDims[0]))
+    lib().orc_trace_entry_margin(C.byref(p), _fp(ct), _ip(ct_dims), _dp(m))
+    return m
 
 
 def beam_adjoint(case: dict, g_dose: np.ndarray, beam_index: int = 0, use_subspots: bool = False,

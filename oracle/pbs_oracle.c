@@ -1307,9 +1307,46 @@ void orc_transfer(orc_plan *p, int l, const double *bev, double *dose, const int
 /* ------------------------------------------------------------------------------------
  * whole beam
  * ---------------------------------------------------------------------------------- */
+void orc_trace_entry_margin(const orc_plan *p, const float *ct, const int ctDims[3], double *margin) {
+    const int rx = p->rayDims[0], ry = p->rayDims[1], K = p->steps;
+    if (p->weqVolume) {
+        for (size_t r = 0; r < (size_t)rx * ry; ++r) margin[r] = INFINITY;
+        return;
+    }
+    fan_geom g;
+    fan_geom_from(p->g2im, p->g2imOff, p, &g);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ry; ++j) {
+        for (int i = 0; i < rx; ++i) {
+            double a[3], bpt[3];
+            fan_to_index(&g, i, j, 0, a);
+            fan_to_index(&g, i, j, 1, bpt);
+            double inc[3] = {bpt[0] - a[0], bpt[1] - a[1], bpt[2] - a[2]};
+            /* the cumulative sum never decreases: the decision is fixed by the last step below the
+             * threshold and the first at or above it */
+            double c = 0.0, below = -INFINITY, m = INFINITY;
+            for (int k = 0; k < K; ++k) {
+                c += orc_sample3d_border(ct, ctDims, a[0] + k * inc[0], a[1] + k * inc[1], a[2] + k * inc[2]);
+                if (c < 150.0) { below = c; continue; }
+                m = c - 150.0;
+                break;
+            }
+            if (below > -INFINITY && 150.0 - below < m) m = 150.0 - below;
+            margin[(size_t)j * rx + i] = m;
+        }
+    }
+}
+
 int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], const float ctCorner[3],
                   double *dose, const int doseDims[3], const float doseRes[3], const float doseCorner[3],
                   const orc_beam *b, const orc_energy *en, orc_plan *planOut) {
+    return orc_beam_dose_entry(ct, ctDims, ctRes, ctCorner, dose, doseDims, doseRes, doseCorner, b, en, NULL,
+                               planOut);
+}
+
+int orc_beam_dose_entry(const float *ct, const int ctDims[3], const float ctRes[3], const float ctCorner[3],
+                        double *dose, const int doseDims[3], const float doseRes[3], const float doseCorner[3],
+                        const orc_beam *b, const orc_energy *en, const int *entryOverride, orc_plan *planOut) {
     orc_plan *p = planOut ? planOut : (orc_plan *)malloc(sizeof(orc_plan));
     if (orc_plan_beam(ctDims, ctRes, ctCorner, doseDims, doseRes, doseCorner, b, en, p) != 0) {
         if (!planOut) free(p);
@@ -1323,6 +1360,9 @@ int orc_beam_dose(const float *ct, const int ctDims[3], const float ctRes[3], co
     int *fi = (int *)malloc(sizeof(int) * rplane), *fo = (int *)malloc(sizeof(int) * rplane);
     double *weplMin = (double *)malloc(sizeof(double) * K);
     orc_trace(p, ct, ctDims, en, density, wepl, fi, fo);
+    if (entryOverride)
+        for (size_t r = 0; r < rplane; ++r)
+            if (entryOverride[r] >= 0) fi[r] = entryOverride[r];
     orc_beam_cutoffs(p, b, en, wepl, fi, fo, weplMin);
     double *rw = (double *)calloc(rplane * L, sizeof(double));
     if (!p->legacyPath) {

@@ -148,6 +148,14 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
 void orc_trace(const orc_plan *plan, const float *ctHuPlus1000, const int ctDims[3],
                const orc_energy *en, double *density, double *wepl, int *firstInside, int *firstOutside);
 
+/* Distance of each ray's entry decision from the threshold (bev_ray_tracing.cu:129-141): with
+ * c(k) the cumulative HU+1000 after step k (non-decreasing) and f = firstInside, the smaller of
+ * 150 - c(f-1) (when c(f-1) < 150) and c(f) - 150 (when f < K); +inf when neither exists or for a
+ * WEQ volume. A ray whose margin is below the trace's sample rounding can enter one step earlier
+ * or later in a float trace. [ry][rx]. Pinned: a slab CT whose cumulative HU lands on 150 +- e. */
+void orc_trace_entry_margin(const orc_plan *plan, const float *ctHuPlus1000, const int ctDims[3],
+                            double *margin);
+
 /* Per-beam reductions and cutoffs (wrapper:3651-3700). Fills plan->beamFirstInside.. */
 void orc_beam_cutoffs(orc_plan *plan, const orc_beam *beam, const orc_energy *en,
                       const double *wepl, const int *firstInside, const int *firstOutside,
@@ -206,6 +214,14 @@ void orc_transfer(orc_plan *plan, int layer, const double *bev, double *dose,
 int orc_beam_dose(const float *ctHuPlus1000, const int ctDims[3], const float ctRes[3], const float ctCorner[3],
                   double *dose, const int doseDims[3], const float doseRes[3], const float doseCorner[3],
                   const orc_beam *beam, const orc_energy *en, orc_plan *planOut);
+
+/* orc_beam_dose with the trace's firstInside replaced by entryOverride[r] wherever that is >= 0
+ * ([ry][rx]; NULL: none). Lets a test follow a float trace's entry step on rays whose decision
+ * sits on the threshold (orc_trace_entry_margin). */
+int orc_beam_dose_entry(const float *ctHuPlus1000, const int ctDims[3], const float ctRes[3],
+                        const float ctCorner[3], double *dose, const int doseDims[3], const float doseRes[3],
+                        const float doseCorner[3], const orc_beam *beam, const orc_energy *en,
+                        const int *entryOverride, orc_plan *planOut);
 
 /* Trilinear sample with border 0 at continuous index (x,y,z), voxel centres at integers
  * (bev_ray_tracing.cu:17-62 after its -0.5). Exposed for tests. */

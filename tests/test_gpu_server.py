@@ -5,7 +5,10 @@ Checks: a full request equals the plain engine's dose bitwise; a resident beam's
 and its weight-only updates agree with the plain engine at the new weights (one rounding of
 IDD = W x unit IDD apart) and with the FP64 oracle within 0.5 % of Dmax; every result is the same
 alone and in a mix of overlapping requests (bitwise); a request larger than the budget fails
-with an out-of-memory error; the halo branch is refused for resident beams.
+with an out-of-memory error; the halo branch is refused for resident beams; requests of the
+serving mix on each CT size agree with the FP64 oracle within 0.5 % of Dmax (the oracle takes
+the float trace's entry step on rays whose cumulative HU+1000 passes within 1 of the 150 entry
+threshold, where the two traces may round to different steps).
 """
 import copy
 
@@ -15,11 +18,14 @@ import pytest
 import oracle as O
 from subsecondpbs_amd import fixtures as F
 
+from .oracle_stages import dose_following_entry
+
 torch = pytest.importorskip("torch")
 gpu = pytest.mark.gpu
 
 
-def _plain(case, beam):
+def _plain(case, beam, entry=False):
+    """The plain engine's dose of ``beam``; with ``entry``, also its per-ray entry steps."""
     from subsecondpbs_amd.engine import Engine
 
     eng = Engine(case["energy"])
@@ -30,8 +36,9 @@ def _plain(case, beam):
     eng.beam_dose(beam, d)
     eng.sync()
     out = d.cpu().numpy()
+    fi = eng.debug_fetch(("fi",))["fi"] if entry else None
     eng.close()
-    return out
+    return (out, fi) if entry else out
 
 
 def _grid(case):
@@ -134,3 +141,32 @@ def test_budget_too_small_fails_cleanly():
     with pytest.raises(RuntimeError, match="budget"):
         sv.result(t)
     sv.close()
+
+
+@gpu
+@pytest.mark.parametrize("ri", [0, 4, 7])
+def test_serving_mix_requests_against_oracle(ri):
+    """Requests of the serving mix on each CT size (512x512x200 with a 256x256x100 dose grid,
+    128^3, 256x256x160) through the server, against the FP64 oracle within 0.5 % of Dmax. Request
+    7 has rays whose cumulative HU+1000 passes within 2e-4 of the entry threshold."""
+    from subsecondpbs_amd.server import Server
+
+    cfg = F.make_config5()
+    r = cfg["requests"][ri]
+    c = cfg["cts"][r["ct"]]
+    print(f"\n[gpu-test] serving-mix request {ri}: CT {c['ct_dims']}, {len(r['beam']['energies'])} layers, "
+          f"{np.asarray(r['beam']['spot_weights']).size} spots", flush=True)
+    sv = Server(cfg["energy"], slots=2)
+    cid = sv.add_ct(c["ct"], c["ct_dims"], c["ct_resolution"], c["ct_corner"])
+    got = sv.result(sv.submit(cid, c["dose_dims"], c["dose_resolution"], c["dose_corner"], r["beam"]))
+    sv.close()
+    case = dict(c)
+    case["beams"] = [r["beam"]]
+    case["energy"] = cfg["energy"]
+    # the same beam through a plain engine: bitwise the server's dose, and its entry steps
+    plain, fi = _plain(case, r["beam"], entry=True)
+    assert np.array_equal(plain, got)
+    ref, n_followed = dose_following_entry(case, fi)
+    err = float(np.abs(got.astype(np.float64) - ref).max() / ref.max())
+    print(f"request {ri}: max|dD|/Dmax = {err:.3e} ({n_followed} rays on the entry threshold followed)")
+    assert err <= 5e-3

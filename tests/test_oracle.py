@@ -127,6 +127,13 @@ def _trace(case):
     return p, dens, wepl, fi, fo, keep, en, bm
 
 
+def _trace_hu(case):
+    """_trace of a case whose CT is in HU (converted to HU+1000 as the beam dose does)."""
+    c = dict(case)
+    c["ct"] = O.ct_as_hu_plus_1000(case["ct"])
+    return _trace(c)
+
+
 def test_trace_water_closed_form():
     case = _water_case()
     p, dens, wepl, fi, fo, *_ = _trace(case)
@@ -497,6 +504,38 @@ def test_trace_ray_that_never_enters():
     p, dens, wepl, fi, fo, *_ = _trace(case)
     assert fi[0, 0] == p.steps and fo[0, 0] == 1
     assert dens[:, 0, 0].max() < 0.01
+
+
+@pytest.mark.parametrize("e,first", [(0.25, 11), (-0.3, 12)])
+def test_entry_margin_around_the_threshold(e, first):
+    """On the central ray (samples on voxel centres) the cumulative HU+1000 is 100 after step 10
+    and 150 + e after step 11: the entry step is 11 for e >= 0 (c(11) reaches 150) and 12 below,
+    and the margin is |e| either way (150 - c(fi-1) or c(fi) - 150, whichever is smaller)."""
+    case = _water_case(n=(16, 16, 40))
+    case["ct"][:10] = 0.0
+    case["ct"][10] = 100.0
+    case["ct"][11] = 50.0 + e
+    p, dens, wepl, fi, fo, *_ = _trace(case)
+    i0 = int(round(-p.cpbCorner[0]))
+    j0 = int(round(-p.cpbCorner[1]))
+    assert fi[j0, i0] == first
+    m = O.entry_margin(case, ct_hu_plus_1000=case["ct"])
+    assert m[j0, i0] == pytest.approx(abs(e), abs=1e-5)
+    # margins are distances: never negative, and at most 150 below the threshold
+    assert (m >= 0.0).all() and np.isfinite(m).all()
+
+
+def test_entry_override_follows_the_given_steps():
+    case = F.make_config1()
+    d0, _ = O.beam_dose(case)
+    *_, fi, _fo, _k, _e, _b = _trace_hu(case)
+    same, _ = O.beam_dose(case, entry_override=fi)
+    assert np.array_equal(same, d0)
+    ov = np.full_like(fi, -1)
+    j, i = fi.shape[0] // 2, fi.shape[1] // 2   # the central ray carries spot weight
+    ov[j, i] = fi[j, i] + 3
+    moved, _ = O.beam_dose(case, entry_override=ov)
+    assert not np.array_equal(moved, d0)
 
 
 # ---------------------------------------------------------------- halo (nuclear) branch ---
