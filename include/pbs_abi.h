@@ -107,7 +107,8 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
 /* Message of the last failing call on this thread (empty string if none). */
 const char *pbs_last_error(void);
 /* sizeof of the ABI structs as compiled into the library (bindings check their mirrors against
- * it): 0 pbs_beam_settings, 1 pbs_energy_struct; -1 for another value. No GPU needed. */
+ * it): 0 pbs_beam_settings, 1 pbs_energy_struct, 2 pbs_plan_info; -1 for another value. No GPU
+ * needed. */
 long pbs_abi_sizeof(int which);
 
 /* Number of visible HIP devices (0 when none; never fails). */
@@ -205,6 +206,25 @@ int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float 
  * effRad[r]); haloEffRad likewise for the halo grid (NULL to skip; the last beam must have
  * used the halo branch). For the parity tests. */
 int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad);
+
+/* Host planning of one beam, without a device (no reference counterpart: it exposes the
+ * planner that pbs_engine_beam_dose runs first, so that tests on a machine without a GPU can
+ * check it against the FP64 oracle's plan). The grids and depths follow the wrapper's host code
+ * (raytracedicom_wrapper.cu:2728-3370, 3746-4329). */
+typedef struct pbs_plan_info {
+    int legacyPath, spotIndices, weqVolume, numLayers;
+    int rayDims[2], superpDims[2], spotDims[2], steps;
+    float spotOffset[2], spotDelta[2], cpbCorner[2], cpbRes[2];
+    float fanCorner[3], fanDelta[3], sourceDist[2];
+    float volConst, volLin, volSq;
+} pbs_plan_info;
+/* layerEnergyIdx / layerPeakDepth [numLayers], denseWeights [numLayers][spotDims1][spotDims0]
+ * (denseCap floats available) and spotCells [numSpots] (the dense cell of each spot, -1 off
+ * the lattice) may be NULL. */
+int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
+                   const float doseRes[3], const float doseCorner[3], const pbs_beam_settings *beam,
+                   const pbs_energy_struct *energy, pbs_plan_info *info, float *layerEnergyIdx,
+                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells);
 
 /* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
  * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside

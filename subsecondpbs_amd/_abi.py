@@ -67,6 +67,17 @@ class DebugInts(C.Structure):
     ]
 
 
+class PlanInfo(C.Structure):
+    """pbs_plan_info: the host planner's grids (pbs_plan_debug)."""
+    _fields_ = [
+        ("legacyPath", C.c_int), ("spotIndices", C.c_int), ("weqVolume", C.c_int), ("numLayers", C.c_int),
+        ("rayDims", C.c_int * 2), ("superpDims", C.c_int * 2), ("spotDims", C.c_int * 2), ("steps", C.c_int),
+        ("spotOffset", C.c_float * 2), ("spotDelta", C.c_float * 2), ("cpbCorner", C.c_float * 2),
+        ("cpbRes", C.c_float * 2), ("fanCorner", C.c_float * 3), ("fanDelta", C.c_float * 3),
+        ("sourceDist", C.c_float * 2), ("volConst", C.c_float), ("volLin", C.c_float), ("volSq", C.c_float),
+    ]
+
+
 class Vec3f(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
 
@@ -86,6 +97,7 @@ EXPORTED = [
     "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
     "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo", "pbs_engine_debug_bins",
+    "pbs_plan_debug",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
@@ -151,6 +163,10 @@ def lib():
         L.pbs_engine_set_nuclear.argtypes = [C.c_void_p, C.c_int]
         L.pbs_engine_debug_bins.restype = C.c_int
         L.pbs_engine_debug_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pbs_plan_debug.restype = C.c_int
+        L.pbs_plan_debug.argtypes = [_i, _f, _f, _i, _f, _f, C.POINTER(BeamSettings), C.POINTER(EnergyStruct),
+                                     C.POINTER(PlanInfo), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                     C.c_void_p]
         L.pbs_engine_debug_halo.restype = C.c_int
         L.pbs_engine_debug_halo.argtypes = [C.c_void_p, _i, _f, C.c_void_p, C.c_void_p, _i]
         L.pbs_batch_create.restype = C.c_void_p
@@ -225,19 +241,23 @@ class Keep:
     def __init__(self):
         self.bufs = []
 
+    def _ptr(self, a, ptype):
+        # a pointer object viewing a c_void_p that holds the buffer's address (several times
+        # cheaper than ndarray.ctypes.data_as); the c_void_p is kept with the buffer
+        self.bufs.append(a)
+        holder = C.c_void_p(a.__array_interface__["data"][0])
+        self.bufs.append(holder)
+        return ptype.from_address(C.addressof(holder))
+
     def f32(self, a):
         if a is None:
             return None
-        a = np.ascontiguousarray(a, dtype=np.float32)
-        self.bufs.append(a)
-        return a.ctypes.data_as(_f)
+        return self._ptr(np.ascontiguousarray(a, dtype=np.float32), _f)
 
     def i32(self, a):
         if a is None:
             return None
-        a = np.ascontiguousarray(a, dtype=np.int32)
-        self.bufs.append(a)
-        return a.ctypes.data_as(_i)
+        return self._ptr(np.ascontiguousarray(a, dtype=np.int32), _i)
 
 
 def energy_struct(energy: dict, keep: Keep) -> EnergyStruct:
@@ -277,7 +297,7 @@ def _vec(beam, key, n, default):
     v = np.asarray(v, dtype=np.float32).ravel()
     if v.size != n:
         raise ValueError(f"beam['{key}'] must have {n} values")
-    return [float(x) for x in v]
+    return v.tolist()
 
 
 def beam_settings(beam: dict, keep: Keep, use_subspots: bool | None = None) -> BeamSettings:
@@ -335,6 +355,40 @@ def beam_settings(beam: dict, keep: Keep, use_subspots: bool | None = None) -> B
         b.waterEquivalenceCount = int(weq.size)
     b.spotPositionsAreIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
     return b
+
+
+def plan_debug(case: dict, beam: dict, use_subspots: bool | None = None) -> dict:
+    """The host planner's result for ``beam`` on the case's CT and dose grid (pbs_plan_debug; no
+    device needed): the PlanInfo fields, per-layer energy index and peak depth, the dense spot
+    lattice [L][ny][nx] and each spot's cell."""
+    keep = Keep()
+    es = energy_struct(case["energy"], keep)
+    bs = beam_settings(beam, keep, use_subspots)
+    d, dp = i3(case["ct_dims"])
+    r, rp = f3(case["ct_resolution"])
+    c, cp = f3(case["ct_corner"])
+    dd, ddp = i3(case["dose_dims"])
+    dr, drp = f3(case["dose_resolution"])
+    dc, dcp = f3(case["dose_corner"])
+    info = PlanInfo()
+    L = int(bs.numLayers)
+    rc = lib().pbs_plan_debug(dp, rp, cp, ddp, drp, dcp, C.byref(bs), C.byref(es), C.byref(info), None, None,
+                              None, 0, None)
+    check(rc, "pbs_plan_debug")
+    eidx = np.zeros(L, np.float32)
+    peak = np.zeros(L, np.float32)
+    nx, ny = info.spotDims
+    dense = np.zeros(max(L * nx * ny, 1), np.float32)
+    cells = np.zeros(max(int(bs.numSpots), 1), np.int32)
+    rc = lib().pbs_plan_debug(dp, rp, cp, ddp, drp, dcp, C.byref(bs), C.byref(es), C.byref(info), eidx.ctypes.data,
+                              peak.ctypes.data, dense.ctypes.data, dense.size, cells.ctypes.data)
+    check(rc, "pbs_plan_debug")
+    out = {k: (list(getattr(info, k)) if not isinstance(getattr(info, k), (int, float)) else getattr(info, k))
+           for k, _ in PlanInfo._fields_}
+    out["energyIdx"], out["peakDepth"] = eidx, peak
+    out["dense"] = dense[:L * nx * ny].reshape(L, ny, nx) if not info.legacyPath else None
+    out["spotCell"] = cells[:int(bs.numSpots)]
+    return out
 
 
 def weq_vector(beam: dict):

@@ -156,6 +156,7 @@ struct pbs_engine {
     // pinned staging of a beam's uploads (per-layer values, subspot table, spot lattice, halo
     // arrays): copies from it are plain DMA, queued without holding the host
     char *pin = nullptr;
+    char *pinDev = nullptr;  // the staging's device-visible address (read by k_upload)
     size_t pinBytes = 0;
     hipEvent_t uploadDone = nullptr;
     // radius classes of the superposition run concurrently on the engine stream and these
@@ -400,6 +401,7 @@ const char *pbs_last_error(void) { return g_lastError.c_str(); }
 long pbs_abi_sizeof(int which) {
     if (which == 0) return long(sizeof(pbs_beam_settings));
     if (which == 1) return long(sizeof(pbs_energy_struct));
+    if (which == 2) return long(sizeof(pbs_plan_info));
     return -1;
 }
 
@@ -543,6 +545,53 @@ int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float 
     if (haloRadHist)
         HIP_TRY(hipMemcpy(haloRadHist, e->halo.hist.p, size_t(L) * (kMaxRadDev + 2) * 4, hipMemcpyDeviceToHost));
     if (radOverflow) HIP_TRY(hipMemcpy(radOverflow, e->haloErr.p, sizeof(int), hipMemcpyDeviceToHost));
+    return PBS_OK;
+}
+
+int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
+                   const float doseRes[3], const float doseCorner[3], const pbs_beam_settings *beam,
+                   const pbs_energy_struct *energy, pbs_plan_info *info, float *layerEnergyIdx,
+                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells) {
+    if (!ctDims || !ctRes || !ctCorner || !doseDims || !doseRes || !doseCorner || !beam || !energy || !info)
+        return set_error(PBS_E_INVALID, "null argument");
+    int rc = validate_beam(*beam, 0);
+    if (rc != PBS_OK) return rc;
+    BeamPlan p;
+    if (!plan_beam(ctDims, ctRes, ctCorner, doseDims, doseRes, doseCorner, *beam, *energy, p, nullptr))
+        return set_error(PBS_E_GEOMETRY, p.error);
+    pbs_plan_info &o = *info;
+    o.legacyPath = p.legacyPath ? 1 : 0;
+    o.spotIndices = p.spotIndices ? 1 : 0;
+    o.weqVolume = p.weqVolume ? 1 : 0;
+    o.numLayers = p.numLayers;
+    o.steps = p.steps;
+    for (int i = 0; i < 2; ++i) {
+        o.rayDims[i] = p.rayDims[i];
+        o.superpDims[i] = p.superpDims[i];
+        o.spotDims[i] = p.spotDims[i];
+        o.spotOffset[i] = p.spotOffset[i];
+        o.spotDelta[i] = p.spotDelta[i];
+        o.cpbCorner[i] = p.cpbCorner[i];
+        o.cpbRes[i] = p.cpbRes[i];
+        o.sourceDist[i] = p.sourceDist[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        o.fanCorner[i] = p.fanCorner[i];
+        o.fanDelta[i] = p.fanDelta[i];
+    }
+    o.volConst = p.volConst;
+    o.volLin = p.volLin;
+    o.volSq = p.volSq;
+    for (int l = 0; l < p.numLayers; ++l) {
+        if (layerEnergyIdx) layerEnergyIdx[l] = p.layers[l].energyIdx;
+        if (layerPeakDepth) layerPeakDepth[l] = p.layers[l].peakDepth;
+    }
+    if (denseWeights) {
+        if (denseCap < p.denseWeights.size()) return set_error(PBS_E_INVALID, "denseCap is smaller than the lattice");
+        std::copy(p.denseWeights.begin(), p.denseWeights.end(), denseWeights);
+    }
+    if (spotCells)
+        for (int i = 0; i < beam->numSpots; ++i) spotCells[i] = i < int(p.spotCell.size()) ? p.spotCell[i] : -1;
     return PBS_OK;
 }
 
@@ -705,11 +754,29 @@ static hipError_t ensure_pin(pbs_engine *e, size_t bytes) {
     if (bytes <= e->pinBytes && e->pin) return hipSuccess;
     if (e->pin) (void)hipHostFree(e->pin);
     e->pin = nullptr;
+    e->pinDev = nullptr;
     e->pinBytes = 0;
     const size_t n = std::max<size_t>(bytes, size_t(1) << 20);
-    const hipError_t r = hipHostMalloc(reinterpret_cast<void **>(&e->pin), n);
-    if (r == hipSuccess) e->pinBytes = n;
-    return r;
+    hipError_t r = hipHostMalloc(reinterpret_cast<void **>(&e->pin), n);
+    if (r != hipSuccess) return r;
+    r = hipHostGetDevicePointer(reinterpret_cast<void **>(&e->pinDev), e->pin, 0);
+    if (r != hipSuccess) {
+        (void)hipHostFree(e->pin);
+        e->pin = nullptr;
+        e->pinDev = nullptr;
+        return r;
+    }
+    e->pinBytes = n;
+    return hipSuccess;
+}
+
+// One segment of the beam's upload: `bytes` (a multiple of 4) from the staging at `off`.
+static void add_upload(UploadArgs &a, const pbs_engine *e, void *dst, size_t off, size_t bytes) {
+    if (bytes == 0) return;
+    UploadSeg &u = a.seg[a.n++];
+    u.src = reinterpret_cast<const uint32_t *>(e->pinDev + off);
+    u.dst = static_cast<uint32_t *>(dst);
+    u.words = uint32_t(bytes / 4);
 }
 
 static size_t pin_align(size_t n) { return (n + 255) & ~size_t(255); }
@@ -834,16 +901,22 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     std::memcpy(e->pin, lh.data(), bLayers);
     std::memcpy(e->pin + oSub, beam->subspotData, bSub);
     if (bDense) std::memcpy(e->pin + oDense, p.denseWeights.data(), bDense);
-    HIP_TRY(hipMemcpyAsync(e->layerHost.p, e->pin, bLayers, hipMemcpyHostToDevice, s));
+    UploadArgs up{};
+    add_upload(up, e, e->layerHost.p, 0, bLayers);
+    add_upload(up, e, e->subspots.p, oSub, bSub);
+    if (bDense) {
+        HIP_TRY(e->spot.ensure(p.denseWeights.size()));
+        add_upload(up, e, e->spot.p, oDense, bDense);
+    }
     if (halo) {
         const HaloPlan &hp = e->haloPlan;
         std::memcpy(e->pin + oRay, hp.rayToNuc.data(), bRay);
         std::memcpy(e->pin + oHaloW, hp.weights.data(), bHaloW);
-        HIP_TRY(hipMemcpyAsync(e->rayToNuc.p, e->pin + oRay, bRay, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(e->haloW.p, e->pin + oHaloW, bHaloW, hipMemcpyHostToDevice, s));
+        add_upload(up, e, e->rayToNuc.p, oRay, bRay);
+        add_upload(up, e, e->haloW.p, oHaloW, bHaloW);
         HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
     }
-    HIP_TRY(hipMemcpyAsync(e->subspots.p, e->pin + oSub, bSub, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_upload(up, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
@@ -869,10 +942,8 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     // 2. ray weights
     HIP_TRY(mark());
     if (!p.legacyPath) {
-        const size_t splane = size_t(p.spotDims[0]) * p.spotDims[1];
-        HIP_TRY(e->spot.ensure(splane * L));
+        // the dense spot lattice went up with the beam's other arrays (k_upload above)
         HIP_TRY(e->interm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
-        HIP_TRY(hipMemcpyAsync(e->spot.p, e->pin + oDense, sizeof(float) * splane * L, hipMemcpyHostToDevice, s));
         const ConvArgs a = make_conv_args(p, rx, ry, nSub, subMode);
         HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
                             e->subEntry.p, L, s));

@@ -79,6 +79,20 @@ hipError_t launch_rescale(const FanGeom &g, LayerScalars *ls, const BeamScalars 
 hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
                             BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s);
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s);
+
+// Uploads of a beam's small host arrays in one kernel that reads the engine's mapped pinned
+// staging directly (a DMA-engine copy adds ~15 us of latency to the stream per call).
+constexpr int kMaxUploadSegs = 5;
+struct UploadSeg {
+    const uint32_t *src;  // device-visible address in mapped pinned host memory
+    uint32_t *dst;
+    uint32_t words;
+};
+struct UploadArgs {
+    UploadSeg seg[kMaxUploadSegs];
+    int n;
+};
+hipError_t launch_upload(const UploadArgs &a, hipStream_t s);
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,
                                      BeamScalars *bs, int *tailPlaneGroup, int *tailPlaneK, int tailCap, int *tileRad,
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,

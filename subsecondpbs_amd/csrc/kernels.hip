@@ -1746,12 +1746,30 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
     }
 }
 
+// Segment blockIdx.y of the upload, 32-bit words strided over the blocks of blockIdx.x (the
+// source is host memory: every word is read once).
+__global__ __launch_bounds__(256) void k_upload(UploadArgs a) {
+    const int sg = blockIdx.y;
+    if (sg >= a.n) return;
+    const UploadSeg u = a.seg[sg];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < u.words; i += gridDim.x * 256u) u.dst[i] = u.src[i];
+}
+
 }  // namespace pbs
 
 // ---------------------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------------------
 namespace pbs {
+
+hipError_t launch_upload(const UploadArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    uint32_t most = 0;
+    for (int i = 0; i < a.n; ++i) most = most > a.seg[i].words ? most : a.seg[i].words;
+    const unsigned bx = std::max(1u, std::min(64u, (most + 255u) / 256u));
+    hipLaunchKernelGGL(k_upload, dim3(bx, unsigned(a.n)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 // Sub-buffer offsets of the trace scratch in floats, each rounded up to 64 floats (256 B) so
 // the float2 and double arrays are aligned whatever nray * K is.

@@ -55,6 +55,17 @@ Aff concat(const Aff &first, const Aff &second) {
     return r;
 }
 
+// std::lround for the per-spot lattice cells without the library call: int(v) truncates toward
+// zero and v - trunc(v) is exact below 2^23 (above it v is an integer and the remainder is 0), so
+// the half cases go away from zero as lround's do. Values beyond 1e9 (and NaN) take lround.
+static inline int round_half_away(float v) {
+    if (!(std::fabs(v) < 1.0e9f)) return int(std::lround(v));
+    int i = int(v);
+    const float r = v - float(i);
+    if (r >= 0.5f) ++i; else if (r <= -0.5f) --i;
+    return i;
+}
+
 static int last_smaller_or_eq(const float *v, int n, float value) {
     int upper = n - 1, lower = 0;
     if (v[n - 1] <= value) return upper;
@@ -190,7 +201,8 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
     // a merged run
     auto unique_sorted = [](const std::vector<float> &v) {
         size_t cap = 64;
-        while (cap < 2 * v.size()) cap <<= 1;
+        int capBits = 6;
+        while (cap < 2 * v.size()) { cap <<= 1; ++capBits; }
         std::vector<uint32_t> slot(cap, 0u);
         std::vector<char> used(cap, 0);
         std::vector<float> d;
@@ -198,7 +210,9 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
             const float f = x + 0.0f;  // -0 and +0 are one value
             uint32_t bits;
             std::memcpy(&bits, &f, 4);
-            size_t h = size_t(bits * 2654435761u) & (cap - 1);
+            // Fibonacci hashing: the product's high bits (lattice coordinates such as 2.5 or 5.0
+            // have all-zero low mantissa bits, so the low bits of the product would all collide)
+            size_t h = size_t((bits * 2654435761u) >> (32 - capBits));
             while (used[h] && slot[h] != bits) h = (h + 1) & (cap - 1);
             if (!used[h]) {
                 used[h] = 1;
@@ -235,7 +249,7 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
             if (!std::isfinite(w) || w < 0.0f) return false;
             const float fx = dx != 0.0f ? (x - ox) / dx : 0.0f;
             const float fy = dy != 0.0f ? (y - oy) / dy : 0.0f;
-            const int ix = int(std::lround(fx)), iy = int(std::lround(fy));
+            const int ix = round_half_away(fx), iy = round_half_away(fy);
             const bool inLattice = ix >= 0 && ix < nx && iy >= 0 && iy < ny;
             if (inLattice) cells[s] = int((size_t(l) * ny + iy) * nx + ix);
             if (w == 0.0f) continue;

@@ -14,6 +14,20 @@ import numpy as np
 from . import _abi
 
 
+class PreparedBeam:
+    """A beam's ``pbs_beam_settings`` struct, built once from the beam dict.
+
+    This is the struct a C or C++ caller of ``pbs_engine_beam_dose`` holds itself. Every call with
+    it still plans, uploads and computes the beam; only the dict -> struct conversion is not
+    repeated. The struct views the beam's arrays where they are contiguous and of the struct's
+    type (and copies otherwise), so prepare again after changing the beam."""
+
+    def __init__(self, beam: dict, use_subspots: bool | None = None):
+        self._keep = _abi.Keep()
+        self.settings = _abi.beam_settings(beam, self._keep, use_subspots)
+        self.num_spots = int(self.settings.numSpots)
+
+
 class Engine:
     def __init__(self, energy: dict, device: int = 0):
         self._keep = _abi.Keep()
@@ -62,10 +76,19 @@ class Engine:
         self.dose_dims = tuple(int(v) for v in dims)
 
     # ------------------------------------------------------------------ compute
-    def beam_dose(self, beam: dict, dose, use_subspots: bool | None = None):
-        """Queue one beam; its dose is added into ``dose`` (device tensor [nz][ny][nx] float32)."""
-        keep = _abi.Keep()
-        bs = _abi.beam_settings(beam, keep, use_subspots)
+    @staticmethod
+    def prepare(beam: dict, use_subspots: bool | None = None) -> PreparedBeam:
+        """The beam's C struct, for repeated :meth:`beam_dose` calls (see PreparedBeam)."""
+        return PreparedBeam(beam, use_subspots)
+
+    def beam_dose(self, beam, dose, use_subspots: bool | None = None):
+        """Queue one beam (a dict or a PreparedBeam); its dose is added into ``dose`` (device
+        tensor [nz][ny][nx] float32)."""
+        if isinstance(beam, PreparedBeam):
+            bs = beam.settings
+        else:
+            keep = _abi.Keep()
+            bs = _abi.beam_settings(beam, keep, use_subspots)
         rc = _abi.lib().pbs_engine_beam_dose(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()))
         _abi.check(rc, "pbs_engine_beam_dose")
 

@@ -78,4 +78,5 @@ def test_struct_mirrors_match_the_library():
     L = _abi.lib()
     assert L.pbs_abi_sizeof(0) == C.sizeof(_abi.BeamSettings)
     assert L.pbs_abi_sizeof(1) == C.sizeof(_abi.EnergyStruct)
+    assert L.pbs_abi_sizeof(2) == C.sizeof(_abi.PlanInfo)
     assert L.pbs_abi_sizeof(7) == -1
