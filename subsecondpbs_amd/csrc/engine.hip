@@ -886,10 +886,11 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         WeqArgs wa{e->weq.p, p.weqDims[0], p.weqDims[1], p.weqDims[2], std::min(K, p.weqDims[2]), p.weqStepMm,
                    p.cpbCorner[0], p.cpbCorner[1], p.cpbRes[0], p.cpbRes[1], p.weqX0, p.weqY0, p.weqDx, p.weqDy};
         HIP_TRY(launch_weq_fill(g, wa, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
+        HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     } else {
-        HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p, s));
+        HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p,
+                             e->weplMin.p, e->beamScal.p, s));
     }
-    HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     // the per-layer values and the subspot table go up behind the trace (copies from pageable
     // memory hold the host, so the trace is already queued while they run)
     const size_t bLayers = sizeof(LayerHost) * L, bSub = sizeof(float) * L * beam->maxSubspotsPerLayer * 5;

@@ -54,8 +54,10 @@ struct WeqArgs {
 };
 hipError_t launch_weq_fill(const FanGeom &g, const WeqArgs &a, float *density, float *wepl, int *fi, int *fo,
                           hipStream_t s);
+// The CT trace, with the per-beam reductions of launch_trace_reduce folded into its carry and fix
+// passes (weplMin per step, beamFirstInside / beamFirstOutside of bs).
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
-                        int *fo, float *scratch, hipStream_t s);
+                        int *fo, float *scratch, float *weplMin, BeamScalars *bs, hipStream_t s);
 hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,
                                BeamScalars *bs, hipStream_t s);
 hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,

@@ -147,7 +147,8 @@ constexpr int kTraceSeg = 16;
 __global__ __launch_bounds__(256) void k_trace_seg(FanGeom g, const float *__restrict__ ct, LutDev lut,
                                                    float *__restrict__ density, float *__restrict__ wepl,
                                                    float *__restrict__ huPrefix, float2 *__restrict__ segSp,
-                                                   float *__restrict__ segHu, int *__restrict__ segLast) {
+                                                   float *__restrict__ segHu, int *__restrict__ segLast,
+                                                   BeamScalars *__restrict__ bs) {
     extern __shared__ float s_lut[];  // density LUT, then stopping-power LUT
     for (int i = threadIdx.x; i < lut.nDensity; i += blockDim.x) s_lut[i] = lut.density[i];
     for (int i = threadIdx.x; i < lut.nSp; i += blockDim.x) s_lut[lut.nDensity + i] = lut.sp[i];
@@ -156,6 +157,11 @@ __global__ __launch_bounds__(256) void k_trace_seg(FanGeom g, const float *__res
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
     const int seg = blockIdx.y;
     if (ray >= nray) return;
+    if (ray == 0 && seg == 0) {
+        // the carry pass reduces the beam's entry / exit steps into these
+        bs->beamFirstInside = INT_MAX;
+        bs->beamFirstOutside = INT_MIN;
+    }
     const float *s_dens = s_lut, *s_sp = s_lut + lut.nDensity;
     const int i = ray % g.rx, j = ray / g.rx;
     const float fx = g.fc[0] + float(i) * g.fd[0];
@@ -212,43 +218,77 @@ __global__ __launch_bounds__(256) void k_trace_carry(FanGeom g, const float *__r
                                                      const float2 *__restrict__ segSp,
                                                      const float *__restrict__ segHu, const int *__restrict__ segLast,
                                                      double *__restrict__ segCarry, int *__restrict__ firstInside,
-                                                     int *__restrict__ firstOutside) {
+                                                     int *__restrict__ firstOutside, unsigned *__restrict__ weplMinBits,
+                                                     BeamScalars *__restrict__ bs) {
     const int nray = g.rx * g.ry;
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray >= nray) return;
-    const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
-    double carrySp = 0.0, carryHu = 0.0;
-    int count = -1, last = -1;
-    for (int sg = 0; sg < S; ++sg) {
-        const size_t o = size_t(sg) * nray + ray;
-        segCarry[o] = carrySp;
-        const float2 sp = segSp[o];
-        const double hu = double(segHu[o]);
-        if (count < 0 && carryHu + hu >= 150.0) {
-            // the threshold is crossed in this segment: find the step
-            const int k0 = sg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
-            count = k1;
-            for (int k = k0; k < k1; ++k)
-                if (carryHu + double(huPrefix[size_t(k) * nray + ray]) >= 150.0) { count = k; break; }
+    // the per-step minimum that k_trace_fix reduces into starts at 1e20 (Macro.cuh INF)
+    if (ray < g.K) weplMinBits[ray] = __float_as_uint(1.0e20f);
+    // whole waves past the last ray leave; lanes past it take neutral values in the reductions
+    if (ray - int(threadIdx.x & 63) >= nray) return;
+    int fiv = INT_MAX, fov = INT_MIN;
+    if (ray < nray) {
+        const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
+        double carrySp = 0.0, carryHu = 0.0;
+        int count = -1, last = -1;
+        for (int sg = 0; sg < S; ++sg) {
+            const size_t o = size_t(sg) * nray + ray;
+            segCarry[o] = carrySp;
+            const float2 sp = segSp[o];
+            const double hu = double(segHu[o]);
+            if (count < 0 && carryHu + hu >= 150.0) {
+                // the threshold is crossed in this segment: find the step
+                const int k0 = sg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
+                count = k1;
+                for (int k = k0; k < k1; ++k)
+                    if (carryHu + double(huPrefix[size_t(k) * nray + ray]) >= 150.0) { count = k; break; }
+            }
+            carrySp += double(sp.x) - double(sp.y);
+            carryHu += hu;
+            last = max(last, segLast[o]);
         }
-        carrySp += double(sp.x) - double(sp.y);
-        carryHu += hu;
-        last = max(last, segLast[o]);
+        if (count < 0) count = g.K;
+        fiv = max(count, 1);
+        fov = max(last, 0) + 1;
+        firstInside[ray] = fiv;
+        firstOutside[ray] = fov;
     }
-    if (count < 0) count = g.K;
-    firstInside[ray] = max(count, 1);
-    firstOutside[ray] = max(last, 0) + 1;
+    // beamFirstInside / beamFirstOutside (k_trace_seg set them to INT_MAX / INT_MIN)
+    const int mn = wave_min_i(fiv), mx = wave_max_i(fov);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&bs->beamFirstInside, mn);
+        atomicMax(&bs->beamFirstOutside, mx);
+    }
 }
 
-// Pass 3, thread per (step, ray): the segment's carry added to its local WEPL prefix.
+// Pass 3, thread per (step, ray): the segment's carry added to its local WEPL prefix (segment 0
+// has none), and the step's minimum WEPL over the rays that leave material (fo > 0), the
+// weplMin of the wrapper's per-beam reduction (wrapper:3651-3700): one atomic per block on the
+// float's bits (WEPL >= 0, so the bit patterns order as the values).
 __global__ __launch_bounds__(256) void k_trace_fix(FanGeom g, const double *__restrict__ segCarry,
-                                                   float *__restrict__ wepl) {
+                                                   float *__restrict__ wepl, const int *__restrict__ fo,
+                                                   unsigned *__restrict__ weplMinBits) {
+    __shared__ float s_m[4];
     const int nray = g.rx * g.ry;
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y + kTraceSeg;  // segment 0 has no carry
-    if (ray >= nray || k >= g.K) return;
-    const size_t idx = size_t(k) * nray + ray;
-    wepl[idx] = float(segCarry[size_t(k / kTraceSeg) * nray + ray] + double(wepl[idx]));
+    const int k = blockIdx.y;
+    float m = 1.0e20f;
+    if (ray < nray) {
+        const size_t idx = size_t(k) * nray + ray;
+        float v = wepl[idx];
+        if (k >= kTraceSeg) {
+            v = float(segCarry[size_t(k / kTraceSeg) * nray + ray] + double(v));
+            wepl[idx] = v;
+        }
+        if (fo[ray] > 0) m = v + 0.0f;  // -0 -> +0
+    }
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float bm = fminf(fminf(s_m[0], s_m[1]), fminf(s_m[2], s_m[3]));
+        if (bm < 1.0e20f) atomicMin(&weplMinBits[k], __float_as_uint(bm));
+    }
 }
 
 // WEQ-volume input in place of the trace (fillBevFromWeqVolumeKernel, wrapper:1923-1988), thread
@@ -1130,6 +1170,8 @@ __device__ __forceinline__ void append_unit(int *__restrict__ classCount, int2 *
 // One thread per (step, tile): the item's launch class from the largest launch radius of its
 // live layers, the block layout radius of its main block and group tail blocks (-1 when a
 // block is empty), and one work unit (slot, block) per non-empty block in its class list.
+// The layer scalars, launch radii and group tail ranges are staged in LDS and the tile codes
+// are loaded eight layers at a time, so the loop's global loads do not wait on each other.
 __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict__ tileRad,
                                                const int *__restrict__ effRad, const LayerScalars *__restrict__ ls,
                                                const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
@@ -1137,10 +1179,29 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
                                                int *__restrict__ classCount, int2 *__restrict__ classList,
                                                int maxUnits, int *__restrict__ planeRM,
                                                int *__restrict__ slotLayers) {
+    extern __shared__ int s_items[];
+    const int L = g.numLayers, m = bs->numGroups;
+    int *s_pass = s_items, *s_group = s_pass + L, *s_mainEnd = s_group + L;
+    int *s_eff = s_mainEnd + L;                 // [L][kMaxRadDev + 2]
+    int *s_tail = s_eff + L * (kMaxRadDev + 2);  // [m][2]: tailStart, tailCount
+    for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        s_pass[i] = ls[i].passive;
+        s_group[i] = ls[i].group;
+        s_mainEnd[i] = ls[i].mainEnd;
+    }
+    for (int i = threadIdx.x; i < L * (kMaxRadDev + 2); i += blockDim.x) s_eff[i] = effRad[i];
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        s_tail[2 * i] = gs[i].tailStart;
+        s_tail[2 * i + 1] = gs[i].tailCount;
+    }
+    __syncthreads();
     const int nt = (g.sx / 32) * (g.sy / 8);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int k = blockIdx.y;
-    if (t >= nt) return;
+    // waves past the last tile leave; lanes past it stay (inactive) for the wave reductions
+    if (t - int(threadIdx.x & 63) >= nt) return;
+    const bool tileIn = t < nt;
+    auto inTail = [&](int j) { return k >= s_tail[2 * j] && k < s_tail[2 * j] + s_tail[2 * j + 1]; };
     int Rmax = -1;
     bool mainLive = false;
     uint64_t tailLive[kMaxLayersDev / 64] = {0, 0, 0, 0};  // groups with a live tail layer here
@@ -1148,41 +1209,48 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
     const int slot = k * nt + t;
     // the slot's live layers in layer order, each l | R << 8 | (block + 1) << 16 (block -1 =
     // main BEV, j = layer group j's tail), terminated by -1 when fewer than numLayers
-    int *list = slotLayers + size_t(slot) * g.numLayers;
+    int *list = slotLayers + size_t(slot) * L;
     int n = 0;
-    for (int l = 0; l < g.numLayers; ++l) {
-        if (k < inside || k >= ls[l].passive) continue;  // steps the layer does not superpose
-        const int code = tileRad[(size_t(l) * g.K + k) * nt + t];
-        if (code < 0 || !(code & 256)) continue;
-        const int j = ls[l].group;
-        const bool isMain = k < ls[l].mainEnd;
-        if (!isMain && !(j >= 0 && in_tail(gs[j], k))) continue;
-        mainLive |= isMain;
-        if (!isMain) tailLive[j >> 6] |= uint64_t(1) << (j & 63);
-        const int R = effRad[l * (kMaxRadDev + 2) + (code & 255)];
-        Rmax = max(Rmax, R);
-        list[n++] = l | (R << 8) | ((isMain ? 0 : j + 1) << 16);
+    for (int l0 = 0; l0 < L; l0 += 8) {
+        int code[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            code[u] = (tileIn && l0 + u < L) ? tileRad[(size_t(l0 + u) * g.K + k) * nt + t] : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int l = l0 + u;
+            if (l >= L || k < inside || k >= s_pass[l]) continue;  // steps the layer does not superpose
+            if (code[u] < 0 || !(code[u] & 256)) continue;
+            const int j = s_group[l];
+            const bool isMain = k < s_mainEnd[l];
+            if (!isMain && !(j >= 0 && inTail(j))) continue;
+            mainLive |= isMain;
+            if (!isMain) tailLive[j >> 6] |= uint64_t(1) << (j & 63);
+            const int R = s_eff[l * (kMaxRadDev + 2) + (code[u] & 255)];
+            Rmax = max(Rmax, R);
+            list[n++] = l | (R << 8) | ((isMain ? 0 : j + 1) << 16);
+        }
     }
-    if (n < g.numLayers) list[n] = -1;
+    if (tileIn && n < L) list[n] = -1;
     const int c = Rmax >= 0 ? radius_class(Rmax) : 0;
     const int RM = Rmax >= 0 ? class_radius(c) : -1;
-    itemR[slot] = mainLive ? RM : -1;
-    if (mainLive) {
-        append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
-        atomicMax(&planeRM[k], RM);
+    // the plane's largest block layout radius: one atomic per wave
+    const int lane = threadIdx.x & 63;
+    if (tileIn) itemR[slot] = mainLive ? RM : -1;
+    if (tileIn && mainLive) append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
+    {
+        const int pm = wave_max_i(mainLive ? RM : -1);
+        if (lane == 0 && pm >= 0) atomicMax(&planeRM[k], pm);
     }
-    const int m = bs->numGroups;
     for (int j = 0; j < m; ++j) {
-        const GroupScalars G = gs[j];
-        if (!in_tail(G, k)) continue;
-        const int q = G.tailBase + (k - G.tailStart);
+        if (!inTail(j)) continue;  // uniform: k is the block's step
+        const int q = gs[j].tailBase + (k - s_tail[2 * j]);
         if (q >= bs->tailPlanes) continue;
-        const bool live = (tailLive[j >> 6] >> (j & 63)) & 1;
-        tailItemR[size_t(q) * nt + t] = live ? RM : -1;
-        if (live) {
-            append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
-            atomicMax(&planeRM[g.K + q], RM);
-        }
+        const bool live = tileIn && ((tailLive[j >> 6] >> (j & 63)) & 1);
+        if (tileIn) tailItemR[size_t(q) * nt + t] = live ? RM : -1;
+        if (live) append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
+        const int pm = wave_max_i(live ? RM : -1);
+        if (lane == 0 && pm >= 0) atomicMax(&planeRM[g.K + q], pm);
     }
 }
 
@@ -1793,7 +1861,7 @@ static TraceScratch trace_layout(const FanGeom &g) {
 size_t trace_scratch_floats(const FanGeom &g) { return trace_layout(g).total; }
 
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
-                        int *fo, float *scratch, hipStream_t s) {
+                        int *fo, float *scratch, float *weplMin, BeamScalars *bs, hipStream_t s) {
     const int n = g.rx * g.ry;
     const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
     const TraceScratch L = trace_layout(g);
@@ -1803,12 +1871,13 @@ hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, fl
     int *segLast = reinterpret_cast<int *>(scratch + L.last);
     double *segCarry = reinterpret_cast<double *>(scratch + L.carry);
     const size_t shm = sizeof(float) * size_t(lut.nDensity + lut.nSp);
+    unsigned *minBits = reinterpret_cast<unsigned *>(weplMin);
     hipLaunchKernelGGL(k_trace_seg, dim3((n + 255) / 256, S), dim3(256), shm, s, g, ct, lut, density, wepl, huPrefix,
-                       segSp, segHu, segLast);
-    hipLaunchKernelGGL(k_trace_carry, dim3((n + 255) / 256), dim3(256), 0, s, g, huPrefix, segSp, segHu, segLast,
-                       segCarry, fi, fo);
-    if (g.K > kTraceSeg)
-        hipLaunchKernelGGL(k_trace_fix, dim3((n + 255) / 256, g.K - kTraceSeg), dim3(256), 0, s, g, segCarry, wepl);
+                       segSp, segHu, segLast, bs);
+    // the carry grid also covers the K steps whose minimum it initialises
+    hipLaunchKernelGGL(k_trace_carry, dim3((std::max(n, g.K) + 255) / 256), dim3(256), 0, s, g, huPrefix, segSp,
+                       segHu, segLast, segCarry, fi, fo, minBits, bs);
+    hipLaunchKernelGGL(k_trace_fix, dim3((n + 255) / 256, g.K), dim3(256), 0, s, g, segCarry, wepl, fo, minBits);
     return hipGetLastError();
 }
 
@@ -1902,8 +1971,10 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
         hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
     hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
                        overflowFlag);
-    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), 0, s, g, tileRad, effRad, ls, gs, bs, itemR,
-                       tailItemR, classCount, classList, maxUnits, planeRM, slotLayers);
+    const size_t itemsLds =
+        sizeof(int) * (size_t(g.numLayers) * (3 + kMaxRadDev + 2) + 2 * size_t(std::max(g.numLayers, 1)));
+    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), itemsLds, s, g, tileRad, effRad, ls, gs, bs,
+                       itemR, tailItemR, classCount, classList, maxUnits, planeRM, slotLayers);
     return hipGetLastError();
 }
 
