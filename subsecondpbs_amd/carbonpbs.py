@@ -365,9 +365,17 @@ def cuCalDose3(out_cscdata, out_cscptr, out_cscrowind, weq_data, roi_index, sour
                beam_xdir, beam_ydir, corner, resolution, dims, longitudal_cutoff, enelist, idddata, iddsetting,
                profiledata, profilesetting, beamparadata, subspot_data, layer_info, layer_energy, outNNZ, idbeamxy,
                num_particles_per_beam, cutoff, sad, beam_para_pos, python_nnz_size, gpu_id: int = 0,
-               nuclear_correction: bool = False, verbose: int = 0, tables_dir: str | None = None) -> dict:
-    """CSC outputs zeroed (no spot matrix is formed) and the final dose in ``dose_grid``
-    (Nx, Ny, Nz) (pybind:1955-2050)."""
+               nuclear_correction: bool = False, verbose: int = 0, tables_dir: str | None = None,
+               operator_mode: bool = False) -> dict:
+    """The final dose in ``dose_grid`` (Nx, Ny, Nz) and the CSC outputs (pybind:1955-2050).
+
+    By default the CSC outputs are zeroed, as the reference does (pybind:1995-1998). With
+    ``operator_mode`` they hold the spot matrix (spot_matrix.spot_dose_matrix), in the CarbonPBS
+    engine's layout (carbonPBS/cudaCalDose.cpp:930-1019): column j = dose per unit weight of spot j
+    at the ROI voxels, row indices = positions in the ROI list (every voxel, x-major, when the ROI
+    is empty), entries <= ``cutoff`` (0 when None) left out. Columns are written while they fit
+    ``python_nnz_size`` (and the arrays); ``outNNZ[0]`` = entries written, ``status`` = -1 when a
+    column did not fit, else 0."""
     for a in (out_cscdata, out_cscptr, out_cscrowind, outNNZ):
         a[...] = 0
     nx, ny, nz = (int(v) for v in np.asarray(dims).ravel())
@@ -376,5 +384,44 @@ def cuCalDose3(out_cscdata, out_cscptr, out_cscrowind, weq_data, roi_index, sour
                   resolution, dims, longitudal_cutoff, enelist, idddata, iddsetting, profiledata, profilesetting,
                   beamparadata, subspot_data, layer_info, layer_energy, idbeamxy, num_particles_per_beam, sad,
                   cutoff, beam_para_pos, gpu_id, nuclear_correction, verbose, False, tables_dir)
-    return {"dose_grid": dose, "out_cscdata": out_cscdata, "out_cscptr": out_cscptr,
-            "out_cscrowind": out_cscrowind, "outNNZ": outNNZ}
+    out = {"dose_grid": dose, "out_cscdata": out_cscdata, "out_cscptr": out_cscptr,
+           "out_cscrowind": out_cscrowind, "outNNZ": outNNZ, "status": 0}
+    if not operator_mode:
+        return out
+    if nuclear_correction:
+        raise RuntimeError("cuCalDose3 operator_mode: the halo branch has no resident (weight-only) form")
+    from .engine import Engine
+    from .spot_matrix import roi_from_xyz, spot_dose_matrix
+
+    ctx = build_carbonpbs_context(weq_data, roi_index, source_energies, source, beam_dir, beam_xdir, beam_ydir,
+                                  corner, resolution, dims, longitudal_cutoff, enelist, idddata, iddsetting,
+                                  profiledata, profilesetting, beamparadata, subspot_data, layer_info, layer_energy,
+                                  idbeamxy, num_particles_per_beam, sad, beam_para_pos, tables_dir)
+    # ROI rows in the caller's order: CarbonPBS linear indices (x ny nz + y nz + z) -> engine order
+    lin = to_roi_linear_indices(roi_index, (nx, ny, nz)) if np.asarray(roi_index).size else \
+        np.arange(nx * ny * nz, dtype=np.int64)
+    lin = np.asarray(lin, np.int64)
+    roi = roi_from_xyz(np.stack([lin // (ny * nz), (lin // nz) % ny, lin % nz], axis=1), (nx, ny, nz))
+    eng = Engine(ctx["energy"], device=gpu_id)
+    try:
+        eng.set_ct(placeholder_ct(ctx["dims"], ctx["beam"]["roi_linear_indices"]), ctx["dims"], ctx["resolution"],
+                   ctx["corner"])
+        eng.set_dose_grid(ctx["dims"], ctx["resolution"], ctx["corner"])
+        data, rows, indptr, _ = spot_dose_matrix(eng, ctx["beam"], roi, 0.0 if cutoff is None else float(cutoff))
+    finally:
+        eng.close()
+    cap = min(int(python_nnz_size) if python_nnz_size is not None else data.size, out_cscdata.size,
+              out_cscrowind.size)
+    S = indptr.size - 1
+    if out_cscptr.size < S + 1:
+        raise RuntimeError("out_cscptr must hold numSpots + 1 entries")
+    done = S
+    while done > 0 and indptr[done] > cap:
+        done -= 1
+    n = int(indptr[done])
+    out_cscdata.reshape(-1)[:n] = data[:n]
+    out_cscrowind.reshape(-1)[:n] = rows[:n]
+    out_cscptr.reshape(-1)[:done + 1] = indptr[:done + 1]
+    outNNZ.reshape(-1)[0] = n
+    out["status"] = 0 if done == S else -1
+    return out
