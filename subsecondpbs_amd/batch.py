@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
+from .engine import torch_ready
 
 
 class Batch:
@@ -43,6 +44,7 @@ class Batch:
         if isinstance(ct, torch.Tensor):
             if not (ct.is_cuda and ct.dtype == torch.float32 and ct.is_contiguous()):
                 raise ValueError("device CT must be a contiguous float32 tensor")
+            torch_ready()  # the lanes copy the CT from it
             ptr, mode = C.c_void_p(ct.data_ptr()), 1
         else:
             a = np.ascontiguousarray(ct, dtype=np.float32)
@@ -68,6 +70,7 @@ class Batch:
         arr = (_abi.BeamSettings * len(beams))()
         for i, b in enumerate(beams):
             arr[i] = _abi.beam_settings(b, keep)
+        torch_ready()  # the lanes' streams read and write it next
         _abi.check(_abi.lib().pbs_batch_dose(self.handle, arr, len(beams), C.c_void_p(dose.data_ptr())),
                    "pbs_batch_dose")
 
@@ -76,6 +79,7 @@ class Batch:
         keep = _abi.Keep()
         bs = _abi.beam_settings(beam, keep)
         t = C.c_longlong(0)
+        torch_ready()  # the lanes' streams read and write it next
         _abi.check(_abi.lib().pbs_batch_submit(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()), C.byref(t)),
                    "pbs_batch_submit")
         return int(t.value)

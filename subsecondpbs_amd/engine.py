@@ -14,6 +14,14 @@ import numpy as np
 from . import _abi
 
 
+def torch_ready():
+    """Block until the work queued so far on torch's current stream has run. For device tensors
+    handed to native streams other than an engine's (batch lanes, server copies)."""
+    import torch
+
+    torch.cuda.current_stream().synchronize()
+
+
 class PreparedBeam:
     """A beam's ``pbs_beam_settings`` struct, built once from the beam dict.
 
@@ -40,6 +48,23 @@ class Engine:
         self.ct_dims = None
         self.dose_dims = None
         self._resident = weakref.WeakSet()
+        self._torch_ev = None
+        self._torch_stream = None
+
+    def after_torch(self):
+        """Order this engine's stream after the work queued so far on torch's current stream, on
+        the device (an event, no host wait): a dose tensor that torch has just zeroed or written
+        is then safe to accumulate into. Every call below that takes a torch tensor does this."""
+        import torch
+
+        cur = torch.cuda.current_stream()
+        if cur.query():
+            return  # everything torch queued has run: no device-side wait needed
+        if self._torch_ev is None:
+            self._torch_ev = torch.cuda.Event()
+            self._torch_stream = torch.cuda.ExternalStream(self.stream_ptr)
+        self._torch_ev.record(cur)
+        self._torch_stream.wait_event(self._torch_ev)
 
     def close(self):
         if getattr(self, "handle", None):
@@ -64,6 +89,7 @@ class Engine:
             a = np.ascontiguousarray(ct, dtype=np.float32)
             rc = _abi.lib().pbs_engine_set_ct(self.handle, a.ctypes.data, dp, rp, cp, 0)
         else:
+            torch_ready()  # the engine copies the CT synchronously
             rc = _abi.lib().pbs_engine_set_ct(self.handle, C.c_void_p(ct.data_ptr()), dp, rp, cp, 1)
         _abi.check(rc, "pbs_engine_set_ct")
         self.ct_dims = tuple(int(v) for v in dims)
@@ -89,6 +115,7 @@ class Engine:
         else:
             keep = _abi.Keep()
             bs = _abi.beam_settings(beam, keep, use_subspots)
+        self.after_torch()
         rc = _abi.lib().pbs_engine_beam_dose(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()))
         _abi.check(rc, "pbs_engine_beam_dose")
 
@@ -98,6 +125,7 @@ class Engine:
         keep = _abi.Keep()
         bs = _abi.beam_settings(beam, keep, use_subspots)
         out = np.zeros(int(bs.numSpots), np.float32)
+        self.after_torch()
         rc = _abi.lib().pbs_engine_beam_adjoint(self.handle, C.byref(bs), C.c_void_p(grad_dose.data_ptr()),
                                                 C.c_void_p(out.ctypes.data))
         _abi.check(rc, "pbs_engine_beam_adjoint")
@@ -109,6 +137,7 @@ class Engine:
         keep = _abi.Keep()
         bs = _abi.beam_settings(beam, keep, use_subspots)
         h = C.c_void_p(0)
+        self.after_torch()
         rc = _abi.lib().pbs_engine_keep_beam(self.handle, C.byref(bs), C.c_void_p(dose.data_ptr()), C.byref(h))
         _abi.check(rc, "pbs_engine_keep_beam")
         rb = ResidentBeam(self, h.value, int(bs.numSpots))
@@ -251,6 +280,7 @@ class ResidentBeam:
 
     def update(self, weights, dose):
         w = np.ascontiguousarray(weights, dtype=np.float32)
+        self.engine.after_torch()
         rc = _abi.lib().pbs_engine_update_beam(self.engine.handle, self.handle, w.ctypes.data_as(C.POINTER(C.c_float)),
                                                int(w.size), C.c_void_p(dose.data_ptr()))
         _abi.check(rc, "pbs_engine_update_beam")
@@ -274,6 +304,7 @@ class ResidentBeam:
     def adjoint(self, weights, grad_dose) -> np.ndarray:
         w = np.ascontiguousarray(weights, dtype=np.float32)
         out = np.zeros(self.num_spots, np.float32)
+        self.engine.after_torch()
         rc = _abi.lib().pbs_engine_update_adjoint(self.engine.handle, self.handle,
                                                   w.ctypes.data_as(C.POINTER(C.c_float)), int(w.size),
                                                   C.c_void_p(grad_dose.data_ptr()), C.c_void_p(out.ctypes.data))

@@ -15,6 +15,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
+from .engine import torch_ready
 
 
 class Server:
@@ -49,6 +50,7 @@ class Server:
             a = np.ascontiguousarray(ct, dtype=np.float32)
             rc = _abi.lib().pbs_server_add_ct(self.handle, C.c_void_p(a.ctypes.data), dp, rp, cp, 0, C.byref(cid))
         else:
+            torch_ready()  # the server copies the CT from it
             rc = _abi.lib().pbs_server_add_ct(self.handle, C.c_void_p(ct.data_ptr()), dp, rp, cp, 1, C.byref(cid))
         _abi.check(rc, "pbs_server_add_ct")
         return int(cid.value)
@@ -98,6 +100,7 @@ class Server:
             import torch
 
             out = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+            torch_ready()  # the server's copy stream writes it next
             rc = _abi.lib().pbs_server_result_device(self.handle, int(ticket), C.c_void_p(out.data_ptr()))
             _abi.check(rc, "pbs_server_result_device")
             return out
