@@ -80,6 +80,7 @@ class Plan(C.Structure):
         ("weqVolume", C.c_int), ("spotIndices", C.c_int), ("weqDims", C.c_int * 3),
         ("weqStepMm", C.c_float), ("weqX0", C.c_float), ("weqY0", C.c_float), ("weqDx", C.c_float),
         ("weqDy", C.c_float), ("weqBody", _f),
+        ("nucRaw", C.c_int * 2), ("nucFallback", C.c_int), ("nucOffset", C.c_float * 2), ("nucDelta", C.c_float * 2),
     ]
 
 

@@ -1076,17 +1076,140 @@ void orc_fill(const orc_plan *p, int l, const orc_energy *en, const double *dens
     orc_fill_halo(p, l, en, density, wepl, rayWeights, fi, fo, idd, rsigma, firstPassive, 0, NULL, NULL, NULL, NULL);
 }
 
+/* One axis of the physical PB lattice (inferAxis, wrapper:1555-1634). The distinct values are
+ * the first seen of each 0.05 mm cluster, sorted. The step is the median gap; each gap must be
+ * k steps with k in [1, 32] and |gap/step - k| <= 1/4. Every value must then lie within a
+ * quarter cell of a point of the count = round(extent / step) + 1 lattice. Returns 1 on success. */
+static int pb_axis_infer(const float *v, int n, float *delta, float *offset, int *count) {
+    if (n <= 0) return 0;
+    float *u = (float *)malloc(sizeof(float) * (size_t)n);
+    int nu = 0;
+    for (int i = 0; i < n; ++i) {
+        int j = 0;
+        while (j < nu && !(fabsf(v[i] - u[j]) <= 0.05f)) ++j;
+        if (j == nu) u[nu++] = v[i];
+    }
+    qsort(u, (size_t)nu, sizeof(float), cmp_float);
+    *offset = u[0];
+    if (nu == 1) { *delta = 1.0f; *count = 1; free(u); return 1; }
+    const size_t ng = (size_t)(nu > 1 ? nu - 1 : 1);  /* nu >= 2 here */
+    float *g = (float *)malloc(sizeof(float) * ng);
+    float *gs = (float *)malloc(sizeof(float) * ng);
+    for (int i = 0; i + 1 < nu; ++i) g[i] = gs[i] = u[i + 1] - u[i];
+    qsort(gs, (size_t)(nu - 1), sizeof(float), cmp_float);
+    const float med = gs[(nu - 1) / 2];
+    int ok = med > 0.0f;
+    for (int i = 0; ok && i + 1 < nu; ++i) {
+        const float r = g[i] / med, k = roundf(r);
+        if (k < 1.0f || k > 32.0f || fabsf(r - k) > 0.25f) ok = 0;
+    }
+    if (ok) {
+        *delta = med;
+        *count = (int)lroundf((u[nu - 1] - u[0]) / med) + 1;
+        if (*count < 1) *count = 1;
+        for (int i = 0; ok && i < n; ++i) {
+            const float f = (v[i] - *offset) / *delta, k = roundf(f);
+            if (k < 0.0f || k > (float)(*count - 1) || fabsf(f - k) > 0.25f) ok = 0;
+        }
+    }
+    free(u); free(g); free(gs);
+    return ok;
+}
+
+/* The physical PB lattice of index-coded spots (buildPhysicalPBLatticeView, wrapper:1478-1820):
+ * texel coordinates decoded to mm, both axes inferred over all spots; failing that, over the
+ * first layer with the most spots (>= 3), with origin and count from all spots' extent; failing
+ * that, the WEQ header's lattice, where spots off it are clamped to the edge. Weights go to the
+ * nearest cell. Returns the dense [L][ny][nx] lattice and fills p->nuc{Raw,Offset,Delta}. */
+static float *pb_lattice_view(const orc_beam *b, orc_plan *p) {
+    const float *h = b->weq;
+    if (!h || b->weqCount < 9) return NULL;
+    const int n = b->numSpots;
+    float *x = (float *)malloc(sizeof(float) * (size_t)n), *y = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        x[i] = h[6] + (b->spotPositions[2 * i] - 0.5f) * h[7];
+        y[i] = h[3] + (b->spotPositions[2 * i + 1] - 0.5f) * h[4];
+    }
+    float dx = 0, dy = 0, ox = 0, oy = 0;
+    int nx = 0, ny = 0, fallback = 0;
+    if (!(pb_axis_infer(x, n, &dx, &ox, &nx) && pb_axis_infer(y, n, &dy, &oy, &ny))) {
+        int best = -1, bestN = 0, first = 0, bestFirst = 0;
+        for (int l = 0; l < b->numLayers; ++l) {
+            if (b->layerSpotCounts[l] > bestN) { bestN = b->layerSpotCounts[l]; best = l; bestFirst = first; }
+            first += b->layerSpotCounts[l];
+        }
+        int ok = 0;
+        float ldx, ldy, lox, loy;
+        int lnx, lny;
+        if (best >= 0 && bestN >= 3 && pb_axis_infer(x + bestFirst, bestN, &ldx, &lox, &lnx) &&
+            pb_axis_infer(y + bestFirst, bestN, &ldy, &loy, &lny)) {
+            float x0 = x[0], x1 = x[0], y0 = y[0], y1 = y[0];
+            for (int i = 1; i < n; ++i) {
+                x0 = fminf(x0, x[i]); x1 = fmaxf(x1, x[i]);
+                y0 = fminf(y0, y[i]); y1 = fmaxf(y1, y[i]);
+            }
+            dx = ldx; dy = ldy; ox = x0; oy = y0;
+            nx = (int)lroundf((x1 - x0) / dx) + 1;
+            ny = (int)lroundf((y1 - y0) / dy) + 1;
+            if (nx < 1) nx = 1;
+            if (ny < 1) ny = 1;
+            ok = 1;
+        }
+        if (!ok) {
+            nx = (int)lroundf(h[8]); if (nx < 1) nx = 1;
+            ny = (int)lroundf(h[5]); if (ny < 1) ny = 1;
+            dx = h[7]; dy = h[4]; ox = h[6]; oy = h[3];
+            fallback = 1;
+        }
+    }
+    float *dense = NULL;
+    if (dx > 0.0f && dy > 0.0f && nx > 0 && ny > 0) {
+        dense = (float *)calloc((size_t)nx * ny * b->numLayers, sizeof(float));
+        int s = 0;
+        for (int l = 0; l < b->numLayers && dense; ++l)
+            for (int i = 0; i < b->layerSpotCounts[l]; ++i, ++s) {
+                const float w = b->spotWeights[s];
+                if (!isfinite(w) || w < 0.0f) { free(dense); dense = NULL; break; }
+                if (w == 0.0f) continue;
+                int ix = (int)lroundf((x[s] - ox) / dx), iy = (int)lroundf((y[s] - oy) / dy);
+                if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) {
+                    if (!fallback) { free(dense); dense = NULL; break; }
+                    ix = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);
+                    iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
+                }
+                dense[((size_t)l * ny + iy) * nx + ix] += w;
+            }
+    }
+    free(x); free(y);
+    if (dense) {
+        p->nucRaw[0] = nx; p->nucRaw[1] = ny;
+        p->nucOffset[0] = ox; p->nucOffset[1] = oy;
+        p->nucDelta[0] = dx; p->nucDelta[1] = dy;
+        p->nucFallback = fallback;
+    }
+    return dense;
+}
+
 int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeights) {
+    float *dense;
     if (b->spotIndices) {
-        snprintf(p->error, sizeof(p->error), "the halo branch with WEQ-index spot positions is not supported");
-        return -1;
+        dense = pb_lattice_view(b, p);
+        if (!dense) {
+            snprintf(p->error, sizeof(p->error), "nuclear_correction=true requires a physical PB lattice view");
+            return -1;
+        }
+    } else {
+        dense = build_lattice(b, p);
+        if (!dense) {
+            snprintf(p->error, sizeof(p->error), "nuclear_correction=true requires raw spot lattice inputs");
+            return -1;
+        }
+        p->nucRaw[0] = p->spotDims[0]; p->nucRaw[1] = p->spotDims[1];
+        p->nucOffset[0] = p->spotOffset[0]; p->nucOffset[1] = p->spotOffset[1];
+        p->nucDelta[0] = p->spotDelta[0]; p->nucDelta[1] = p->spotDelta[1];
+        p->nucFallback = 0;
     }
-    float *dense = build_lattice(b, p);
-    if (!dense) {
-        snprintf(p->error, sizeof(p->error), "nuclear_correction=true requires raw spot lattice inputs");
-        return -1;
-    }
-    const int rawX = p->spotDims[0], rawY = p->spotDims[1];
+    const int rawX = p->nucRaw[0], rawY = p->nucRaw[1];
     const int nucX = (rawX + ORC_TILE_X - 1) / ORC_TILE_X * ORC_TILE_X;
     const int nucY = (rawY + ORC_TILE_Y - 1) / ORC_TILE_Y * ORC_TILE_Y;
     p->nucDims[0] = nucX;
@@ -1094,7 +1217,7 @@ int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeig
     const int rx = p->rayDims[0], ry = p->rayDims[1];
     for (int i = 0; i < rx * ry; ++i) rayToNuc[i] = -1;
     for (int j = 0; j < rawY; ++j) {
-        const float gy = p->spotOffset[1] + (float)j * p->spotDelta[1];
+        const float gy = p->nucOffset[1] + (float)j * p->nucDelta[1];
         const int rayY = (int)lroundf((gy - p->cpbCorner[1]) / p->cpbRes[1]);
         if (rayY < 0 || rayY >= ry) {
             snprintf(p->error, sizeof(p->error), "dense halo lattice row mapped outside active ray grid");
@@ -1102,7 +1225,7 @@ int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeig
             return -1;
         }
         for (int i = 0; i < rawX; ++i) {
-            const float gx = p->spotOffset[0] + (float)i * p->spotDelta[0];
+            const float gx = p->nucOffset[0] + (float)i * p->nucDelta[0];
             const int rayX = (int)lroundf((gx - p->cpbCorner[0]) / p->cpbRes[0]);
             if (rayX < 0 || rayX >= rx) {
                 snprintf(p->error, sizeof(p->error), "dense halo lattice column mapped outside active ray grid");
@@ -1120,9 +1243,10 @@ int orc_halo_plan(orc_plan *p, const orc_beam *b, int *rayToNuc, double *nucWeig
                         (i < rawX && j < rawY) ? dense[((size_t)l * rawY + j) * rawX + i] : 0.0;
     }
     free(dense);
-    /* spot spacing: the beam's spotDelta when set (> 1.5 rays), else the lattice's */
+    /* spot spacing: the beam's spotDelta when set (> 1.5 rays), else the halo lattice's
+     * (wrapper:4149-4171) */
     const int beamDelta = b->spotDelta[0] > b->raySpacing[0] * 1.5f;
-    const float spacing = beamDelta ? b->spotDelta[0] : p->spotDelta[0];
+    const float spacing = beamDelta ? b->spotDelta[0] : p->nucDelta[0];
     if (b->raySpacing[0] > 0.0f && spacing / b->raySpacing[0] < 1.5f) {
         snprintf(p->error, sizeof(p->error), "nuclear superposition gate: spotDistInRays < 1.5");
         return -1;
@@ -1412,8 +1536,8 @@ int orc_beam_dose_entry(const float *ct, const int ctDims[3], const float ctRes[
             nucBev = (double *)malloc(sizeof(double) * nucBevN);
         }
     }
-    fan_grid nucGrid = {p->nucDims[0], p->nucDims[1], {p->spotOffset[0], p->spotOffset[1]},
-                        {p->spotDelta[0], p->spotDelta[1]}};
+    fan_grid nucGrid = {p->nucDims[0], p->nucDims[1], {p->nucOffset[0], p->nucOffset[1]},
+                        {p->nucDelta[0], p->nucDelta[1]}};
     for (int l = 0; l < L && !haloFail; ++l) {
         if (b->nuclear) {
             for (size_t i = 0; i < nucPlane * K; ++i) { nucIdd[i] = 0.0; nucRs[i] = INFINITY; }

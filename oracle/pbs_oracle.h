@@ -115,6 +115,10 @@ typedef struct {
     int weqDims[3];                          /* x_n, y_n, depth_n */
     float weqStepMm, weqX0, weqY0, weqDx, weqDy;
     const float *weqBody;                    /* [x_n][y_n][depth_n] */
+    /* halo lattice (cells = spots): the spot lattice for physical positions, the physical PB
+     * view of index-coded spots otherwise (wrapper:1478-1820, 3421-3437) */
+    int nucRaw[2], nucFallback;
+    float nucOffset[2], nucDelta[2];
 } orc_plan;
 
 long orc_sizeof(int which);  /* 0 orc_beam, 1 orc_plan, 2 orc_energy */

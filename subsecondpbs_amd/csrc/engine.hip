@@ -844,8 +844,8 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         gh = g;  // the halo grid: cells are spots (nucIdxToFan, wrapper:5527-5535)
         gh.sx = gh.rx = hp.nucDims[0];
         gh.sy = gh.ry = hp.nucDims[1];
-        gh.fc[0] = p.spotOffset[0]; gh.fc[1] = p.spotOffset[1];
-        gh.fd[0] = p.spotDelta[0]; gh.fd[1] = p.spotDelta[1];
+        gh.fc[0] = hp.offset[0]; gh.fc[1] = hp.offset[1];
+        gh.fd[0] = hp.delta[0]; gh.fd[1] = hp.delta[1];
         const size_t nucPlane = size_t(hp.nucDims[0]) * hp.nucDims[1];
         HIP_TRY(e->halo.ensure(L, K, gh.sx, gh.sy));
         HIP_TRY(e->rayToNuc.ensure(nray));

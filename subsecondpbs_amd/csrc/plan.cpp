@@ -538,30 +538,167 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
     return true;
 }
 
+// One axis of the physical PB lattice of index-coded spots (buildPhysicalPBLatticeView's
+// inferAxis, wrapper:1555-1634): distinct values (0.05 mm merge, first seen kept), sorted; the
+// step is the median gap, every gap must be 1-32 steps within a quarter step, and every value a
+// lattice point within a quarter cell.
+static bool pb_axis(const std::vector<float> &vals, float &delta, float &offset, int &count) {
+    if (vals.empty()) return false;
+    std::vector<float> u;
+    for (float v : vals) {
+        bool seen = false;
+        for (float w : u)
+            if (std::fabs(v - w) <= 0.05f) { seen = true; break; }
+        if (!seen) u.push_back(v);
+    }
+    std::sort(u.begin(), u.end());
+    offset = u.front();
+    if (u.size() == 1) {
+        delta = 1.0f;
+        count = 1;
+        return true;
+    }
+    std::vector<float> gaps;
+    for (size_t i = 1; i < u.size(); ++i) gaps.push_back(u[i] - u[i - 1]);
+    std::vector<float> sorted = gaps;
+    std::sort(sorted.begin(), sorted.end());
+    const float med = sorted[sorted.size() / 2];
+    if (!(med > 0.0f)) return false;
+    for (float d : gaps) {
+        const float ratio = d / med, k = std::round(ratio);
+        if (k < 1.0f || k > 32.0f || std::fabs(ratio - k) > 0.25f) return false;
+    }
+    delta = med;
+    count = std::max(1, int(std::lround((u.back() - u.front()) / med)) + 1);
+    for (float v : vals) {
+        const float f = (v - offset) / delta, k = std::round(f);
+        if (k < 0.0f || k > float(count - 1) || std::fabs(f - k) > 0.25f) return false;
+    }
+    return true;
+}
+
+// The physical PB lattice of index-coded spots (buildPhysicalPBLatticeView, wrapper:1478-1820):
+// spots decoded to mm (x = h6 + (i - 1/2) h7, y = h3 + (j - 1/2) h4), axes inferred over all
+// spots, else over the layer with the most spots (offset and extent from all spots), else the WEQ
+// header lattice (out-of-range spots then clamped to the edge); each weight goes to its nearest
+// cell.
+static bool pb_lattice_view(const pbs_beam_settings &b, HaloPlan &h, std::vector<float> &dense, std::string &error) {
+    const float *hd = b.waterEquivalence;
+    if (!hd || b.waterEquivalenceCount < 9) {
+        error = "spotPositionsAreIndices requires a 9-value WEQ header";
+        return false;
+    }
+    const int n = b.numSpots;
+    std::vector<float> xs(n), ys(n);
+    for (int i = 0; i < n; ++i) {
+        xs[i] = hd[6] + (b.spotPositions[2 * i] - 0.5f) * hd[7];
+        ys[i] = hd[3] + (b.spotPositions[2 * i + 1] - 0.5f) * hd[4];
+    }
+    int nx = 0, ny = 0;
+    float dx = 0.f, dy = 0.f, ox = 0.f, oy = 0.f;
+    bool fallback = false;
+    if (!(pb_axis(xs, dx, ox, nx) && pb_axis(ys, dy, oy, ny))) {
+        int best = -1, bestCount = 0, start = 0, bestStart = 0;
+        for (int l = 0; l < b.numLayers; ++l) {
+            if (b.layerSpotCounts[l] > bestCount) { bestCount = b.layerSpotCounts[l]; best = l; bestStart = start; }
+            start += b.layerSpotCounts[l];
+        }
+        bool ok = false;
+        if (best >= 0 && bestCount >= 3) {
+            std::vector<float> lx(xs.begin() + bestStart, xs.begin() + bestStart + bestCount);
+            std::vector<float> ly(ys.begin() + bestStart, ys.begin() + bestStart + bestCount);
+            float ldx, ldy, lox, loy;
+            int lnx, lny;
+            if (pb_axis(lx, ldx, lox, lnx) && pb_axis(ly, ldy, loy, lny)) {
+                dx = ldx;
+                dy = ldy;
+                const float minX = *std::min_element(xs.begin(), xs.end()), maxX = *std::max_element(xs.begin(), xs.end());
+                const float minY = *std::min_element(ys.begin(), ys.end()), maxY = *std::max_element(ys.begin(), ys.end());
+                nx = std::max(1, int(std::lround((maxX - minX) / dx)) + 1);
+                ny = std::max(1, int(std::lround((maxY - minY) / dy)) + 1);
+                ox = minX;
+                oy = minY;
+                ok = true;
+            }
+        }
+        if (!ok) {
+            nx = std::max(1, int(std::lround(hd[8])));
+            ny = std::max(1, int(std::lround(hd[5])));
+            dx = hd[7];
+            dy = hd[4];
+            ox = hd[6];
+            oy = hd[3];
+            fallback = true;
+        }
+    }
+    if (!(dx > 0.0f) || !(dy > 0.0f) || nx <= 0 || ny <= 0) {
+        error = "inferred lattice has invalid cardinality or step";
+        return false;
+    }
+    dense.assign(size_t(nx) * ny * b.numLayers, 0.0f);
+    int off = 0;
+    for (int l = 0; l < b.numLayers; ++l) {
+        for (int i = 0; i < b.layerSpotCounts[l]; ++i) {
+            const int s = off + i;
+            const float w = b.spotWeights[s];
+            if (!std::isfinite(w) || w < 0.0f) {
+                error = "spot weights must be finite and >= 0";
+                return false;
+            }
+            if (w == 0.0f) continue;
+            int ix = int(std::lround((xs[s] - ox) / dx)), iy = int(std::lround((ys[s] - oy) / dy));
+            if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) {
+                if (!fallback) {
+                    error = "decoded spot is outside inferred PB lattice bounds";
+                    return false;
+                }
+                ix = std::max(0, std::min(nx - 1, ix));
+                iy = std::max(0, std::min(ny - 1, iy));
+            }
+            dense[(size_t(l) * ny + iy) * nx + ix] += w;
+        }
+        off += b.layerSpotCounts[l];
+    }
+    h.rawDims[0] = nx; h.rawDims[1] = ny;
+    h.offset[0] = ox; h.offset[1] = oy;
+    h.delta[0] = dx; h.delta[1] = dy;
+    h.latticeFallback = fallback;
+    return true;
+}
+
 bool plan_halo(const BeamPlan &p, const pbs_beam_settings &b, HaloPlan &h, std::string &error) {
     if (p.legacyPath || p.spotDims[0] <= 0 || p.spotDims[1] <= 0) {
         error = "nuclear_correction=true requires raw spot lattice inputs; legacy CPB fallback cannot drive halo mode";
         return false;
     }
+    // the halo cells are the spots' lattice: the planner's own for physical positions, the
+    // physical PB view for index-coded spots (wrapper:3421-3437)
+    std::vector<float> view;
+    const std::vector<float> *dense = &p.denseWeights;
     if (p.spotIndices) {
-        error = "the halo branch with WEQ-index spot positions is not supported by this build";
-        return false;
+        if (!pb_lattice_view(b, h, view, error)) return false;
+        dense = &view;
+    } else {
+        h.rawDims[0] = p.spotDims[0]; h.rawDims[1] = p.spotDims[1];
+        h.offset[0] = p.spotOffset[0]; h.offset[1] = p.spotOffset[1];
+        h.delta[0] = p.spotDelta[0]; h.delta[1] = p.spotDelta[1];
+        h.latticeFallback = false;
     }
-    const int rawX = p.spotDims[0], rawY = p.spotDims[1];
+    const int rawX = h.rawDims[0], rawY = h.rawDims[1];
     const int nucX = (rawX + 31) / 32 * 32, nucY = (rawY + 7) / 8 * 8;
     h.nucDims[0] = nucX;
     h.nucDims[1] = nucY;
     const int rx = p.rayDims[0], ry = p.rayDims[1];
     h.rayToNuc.assign(size_t(rx) * ry, -1);
     for (int j = 0; j < rawY; ++j) {
-        const float gy = p.spotOffset[1] + float(j) * p.spotDelta[1];
+        const float gy = h.offset[1] + float(j) * h.delta[1];
         const int rayY = int(std::lround((gy - p.cpbCorner[1]) / p.cpbRes[1]));
         if (rayY < 0 || rayY >= ry) {
             error = "dense halo lattice row mapped outside active ray grid";
             return false;
         }
         for (int i = 0; i < rawX; ++i) {
-            const float gx = p.spotOffset[0] + float(i) * p.spotDelta[0];
+            const float gx = h.offset[0] + float(i) * h.delta[0];
             const int rayX = int(std::lround((gx - p.cpbCorner[0]) / p.cpbRes[0]));
             if (rayX < 0 || rayX >= rx) {
                 error = "dense halo lattice column mapped outside active ray grid";
@@ -575,9 +712,11 @@ bool plan_halo(const BeamPlan &p, const pbs_beam_settings &b, HaloPlan &h, std::
     for (int l = 0; l < L; ++l)
         for (int j = 0; j < rawY; ++j)
             for (int i = 0; i < rawX; ++i)
-                h.weights[(size_t(l) * nucY + j) * nucX + i] = p.denseWeights[(size_t(l) * rawY + j) * rawX + i];
+                h.weights[(size_t(l) * nucY + j) * nucX + i] = (*dense)[(size_t(l) * rawY + j) * rawX + i];
+    // spot spacing: the beam's spotDelta when set (> 1.5 rays), else the halo lattice's
+    // (wrapper:4149-4171)
     const bool beamDelta = b.spotDelta[0] > b.raySpacing[0] * 1.5f;
-    const float spacing = beamDelta ? b.spotDelta[0] : p.spotDelta[0];
+    const float spacing = beamDelta ? b.spotDelta[0] : h.delta[0];
     if (b.raySpacing[0] > 0.0f && spacing / b.raySpacing[0] < 1.5f) {
         error = "nuclear superposition gate: spotDistInRays < 1.5 (the halo radius would exceed 32)";
         return false;

@@ -88,6 +88,10 @@ struct CpbOverride {
 // spot's gantry position carries that spot's halo; spotDist = spot spacing / fan ray spacing.
 struct HaloPlan {
     int nucDims[2] = {0, 0};
+    int rawDims[2] = {0, 0};       // the halo lattice (spot cells) before 32x8 padding
+    float offset[2] = {0.f, 0.f};  // gantry position of cell (0, 0), mm
+    float delta[2] = {0.f, 0.f};   // cell spacing, mm
+    bool latticeFallback = false;  // index spots: the WEQ header lattice stood in for the PB grid
     float spotDist = 0.f;
     std::vector<int> rayToNuc;   // [ry][rx], -1 = no spot centred on the ray
     std::vector<float> weights;  // [L][nucY][nucX]

@@ -174,3 +174,28 @@ def test_halo_config1_fill_fields(model):
     for l in range(len(ref)):
         assert rep[f"idd{l}"] < 2e-4 and rep[f"rs{l}"] < 1e-4, l
         assert rep[f"nidd{l}"] < 2e-4 and rep[f"nrs{l}"] < 1e-4, l
+
+
+@gpu
+@pytest.mark.parametrize("fallback", [False, True])
+def test_halo_index_spots(fallback):
+    """The halo branch with WEQ-index spots: the halo cells come from the physical PB lattice
+    view (canonical 11 x 11 at 5 mm for spots on texel centres; the WEQ header's 81 x 71 lattice
+    when a shifted spot breaks the inference and the beam states its 5 mm spacing), against the
+    oracle's dose within 0.5 % of Dmax."""
+    from .test_weq import _texel_centre_index_case
+
+    if fallback:
+        case = F.make_weq_case(index_spots=True)
+        case["beams"][0]["spot_delta"] = np.array([5.0, 5.0, 0.0], np.float32)
+    else:
+        case = _texel_centre_index_case()
+    case["energy"] = EN
+    print(f"\n[gpu-test] halo with index spots, fallback={fallback}", flush=True)
+    ref, p = O.beam_dose(case, nuclear=1)
+    dose, halo, ints = _run(case, 1)
+    assert halo["nucDims"] == tuple(p.nucDims) and halo["spotDist"] == pytest.approx(p.spotDist)
+    assert halo["overflow"] == 0
+    err = _rel(dose, ref)
+    print(f"halo index spots (fallback={fallback}) max|dD|/Dmax = {err:.3e}; nucDims {tuple(p.nucDims)}")
+    assert err <= DOSE_TOL

@@ -148,3 +148,56 @@ def test_index_spot_split_is_bilinear():
     # a plausible slip (no -1/2 texel offset) puts the mass one texel over
     wrong = ray_weights(x0 + 0.75, y0 + 1.25)
     assert np.abs(wrong - want).max() > 1e-2 * want.max()
+
+
+def _texel_centre_index_case(**kw):
+    case = F.make_weq_case(index_spots=True, **kw)
+    b = case["beams"][0]
+    raw = np.asarray(b["spot_positions"], np.float32).reshape(-1, 2).copy()
+    raw[::len(raw) // len(b["energies"])] -= 0.25  # every spot on a texel centre
+    b["spot_positions"] = raw.ravel()
+    return case
+
+
+def _halo_plan(case):
+    keep = O._Keep()
+    bm = O.make_beam(case["beams"][0], keep)
+    p = O.plan_beam(case)
+    rx, ry = p.rayDims[0], p.rayDims[1]
+    L = bm.numLayers
+    nx = (max(p.spotDims[0], 1) + 31) // 32 * 32
+    ny = (max(p.spotDims[1], 1) + 7) // 8 * 8
+    ray_to_nuc = np.zeros(rx * ry, np.int32)
+    w = np.zeros(L * nx * ny)
+    rc = O.lib().orc_halo_plan(C.byref(p), C.byref(bm), O._ip(ray_to_nuc), O._dp(w))
+    return rc, p, ray_to_nuc, w
+
+
+def test_halo_lattice_view_of_index_spots():
+    """Index-coded spots at texel centres decode to the physical beam's 5 mm lattice
+    (buildPhysicalPBLatticeView, wrapper:1478-1820): the halo cells, their weights, the spot
+    spacing and the number of rays that centre a spot equal the physical-position beam's."""
+    rc_p, pp, r2n_p, w_p = _halo_plan(F.make_weq_case())
+    rc_i, pi_, r2n_i, w_i = _halo_plan(_texel_centre_index_case())
+    assert rc_p == 0 and rc_i == 0, pi_.error
+    assert tuple(pi_.nucRaw) == tuple(pp.spotDims) == (11, 11) and pi_.nucFallback == 0
+    np.testing.assert_allclose(pi_.nucDelta, pp.spotDelta[:2], rtol=1e-6)
+    np.testing.assert_allclose(pi_.nucOffset, pp.spotOffset[:2], atol=1e-4)
+    assert tuple(pi_.nucDims) == tuple(pp.nucDims) and pi_.spotDist == pytest.approx(pp.spotDist)
+    n = pp.nucDims[0] * pp.nucDims[1] * 5
+    np.testing.assert_array_equal(w_i[:n], w_p[:n])
+    assert (r2n_i >= 0).sum() == (r2n_p >= 0).sum() == 121
+
+
+def test_halo_lattice_view_fallback_and_gate():
+    """A spot a quarter texel off its centre breaks the canonical axes (a 0.25 mm gap is not a
+    whole number of 5 mm steps) on all spots and on the densest layer, so the WEQ header's 1 mm
+    lattice stands in: its spacing fails the spotDistInRays >= 1.5 gate unless the beam states its
+    spot spacing, and then the halo grid is the 81 x 71 texel lattice."""
+    case = F.make_weq_case(index_spots=True)
+    rc, p, _, _ = _halo_plan(case)
+    assert rc != 0 and b"spotDistInRays" in p.error
+    case["beams"][0]["spot_delta"] = np.array([5.0, 5.0, 0.0], np.float32)
+    rc, p, r2n, w = _halo_plan(case)
+    assert rc == 0 and p.nucFallback == 1 and tuple(p.nucRaw) == (81, 71)
+    assert abs(w.sum() - float(np.asarray(case["beams"][0]["spot_weights"]).sum())) < 1e-3
