@@ -177,19 +177,26 @@ def test_halo_config1_fill_fields(model):
 
 
 @gpu
-@pytest.mark.parametrize("fallback", [False, True])
+@pytest.mark.parametrize("fallback", ["canonical", "densest-layer", "weq-header"])
 def test_halo_index_spots(fallback):
     """The halo branch with WEQ-index spots: the halo cells come from the physical PB lattice
-    view (canonical 11 x 11 at 5 mm for spots on texel centres; the WEQ header's 81 x 71 lattice
-    when a shifted spot breaks the inference and the beam states its 5 mm spacing), against the
-    oracle's dose within 0.5 % of Dmax."""
+    view (canonical 11 x 11 at 5 mm for spots on texel centres; the densest layer's 5 mm step when
+    odd layers are shifted by 2 mm; the WEQ header's 81 x 71 lattice when a quarter-texel shift
+    breaks the inference and the beam states its 5 mm spacing), against the oracle's dose within
+    0.5 % of Dmax."""
     from .test_weq import _texel_centre_index_case
 
-    if fallback:
+    if fallback == "weq-header":
         case = F.make_weq_case(index_spots=True)
         case["beams"][0]["spot_delta"] = np.array([5.0, 5.0, 0.0], np.float32)
     else:
         case = _texel_centre_index_case()
+        if fallback == "densest-layer":
+            b = case["beams"][0]
+            raw = np.asarray(b["spot_positions"], np.float32).reshape(-1, 2).copy()
+            layer = np.repeat(np.arange(len(b["energies"])), np.asarray(b["layer_spot_counts"]))
+            raw[layer % 2 == 1, 0] += 2.0
+            b["spot_positions"] = raw.ravel()
     case["energy"] = EN
     print(f"\n[gpu-test] halo with index spots, fallback={fallback}", flush=True)
     ref, p = O.beam_dose(case, nuclear=1)

@@ -201,3 +201,27 @@ def test_halo_lattice_view_fallback_and_gate():
     rc, p, r2n, w = _halo_plan(case)
     assert rc == 0 and p.nucFallback == 1 and tuple(p.nucRaw) == (81, 71)
     assert abs(w.sum() - float(np.asarray(case["beams"][0]["spot_weights"]).sum())) < 1e-3
+
+
+def test_halo_lattice_view_densest_layer():
+    """Odd layers shifted by 2 mm (two texels): over all spots the gaps alternate 2 and 3 mm (median
+    2, a 3 mm gap is 1.5 steps), so no single step fits, but each layer alone is the 5 mm lattice.
+    (A 1 mm shift would not do: gaps of 1 and 4 mm are whole multiples of their 1 mm median.) The view then takes the step of
+    the first layer with the most spots and the origin and extent of all spots
+    (wrapper:1647-1707): 5 mm cells from the smallest decoded x, one column more to reach the
+    shifted layers, and every spot binned to its nearest cell."""
+    case = _texel_centre_index_case()
+    b = case["beams"][0]
+    raw = np.asarray(b["spot_positions"], np.float32).reshape(-1, 2).copy()
+    counts = np.asarray(b["layer_spot_counts"])
+    layer = np.repeat(np.arange(counts.size), counts)
+    raw[layer % 2 == 1, 0] += 2.0
+    b["spot_positions"] = raw.ravel()
+    rc, p, r2n, w = _halo_plan(case)
+    assert rc == 0, p.error
+    assert p.nucFallback == 0 and p.nucDelta[0] == pytest.approx(5.0) and p.nucDelta[1] == pytest.approx(5.0)
+    h = case["weq_header"]
+    xs = h[6] + (raw[:, 0] - 0.5) * h[7]
+    assert p.nucOffset[0] == pytest.approx(float(xs.min()), abs=1e-5)
+    assert p.nucRaw[0] == int(round((xs.max() - xs.min()) / 5.0)) + 1 == 11 and p.nucRaw[1] == 11
+    assert abs(w.sum() - float(np.asarray(b["spot_weights"]).sum())) < 1e-3
