@@ -138,7 +138,7 @@ __device__ int first_larger_ordered(const float *v, int n, float value) {
 // The trace runs as (ray, 16-step segment) threads so that every ray's K steps are spread
 // over many waves: pass 1 samples the CT and keeps segment-local prefixes, pass 2 turns the
 // segment totals into carries and the entry / exit steps, pass 3 adds the carries.
-constexpr int kTraceSeg = 16;
+constexpr int kTraceSeg = 8;
 
 // Pass 1 (bev_ray_tracing.cu:65-159 per segment): explicit trilinear CT sample at fan step k
 // (texel centre -1/2, border 0), density and SP from the LUTs in LDS, the segment's Kahan
