@@ -1113,22 +1113,32 @@ __global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restr
                        int *__restrict__ overflowFlag) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= numLayers) return;
+    // every loop has a fixed trip count and is unrolled, so h and eff stay in registers (a loop
+    // bound by layerMax would index them at run time and put them in scratch memory)
     int h[kMaxRadDev + 2];
+#pragma unroll
     for (int i = 0; i < kMaxRadDev + 2; ++i) h[i] = hist[l * (kMaxRadDev + 2) + i];
     if (overflowFlag && h[kMaxRadDev + 1] > 0) *overflowFlag = 1;
     h[kMaxRadDev] += h[kMaxRadDev + 1];  // primary overflow clamps into the largest radius
     h[kMaxRadDev + 1] = 0;
     int layerMax = 0;
-    for (int i = 0; i < kMaxRadDev + 2; ++i) if (h[i] > 0) layerMax = i;
+#pragma unroll
+    for (int i = 0; i < kMaxRadDev + 2; ++i) layerMax = h[i] > 0 ? i : layerMax;
     int eff[kMaxRadDev + 2];
+#pragma unroll
     for (int i = 0; i < kMaxRadDev + 2; ++i) eff[i] = i;
     eff[kMaxRadDev + 1] = kMaxRadDev;
+    // from the largest radius down: bins are batched upwards until kMinTilesDev tiles
     int batched = 0, rec = layerMax;
-    for (int r = layerMax; r > 0; --r) {
-        batched += h[r];
-        eff[r] = rec;
-        if (batched >= kMinTilesDev) { rec = r - 1; batched = 0; }
+#pragma unroll
+    for (int r = kMaxRadDev; r > 0; --r) {
+        if (r <= layerMax) {
+            batched += h[r];
+            eff[r] = rec;
+            if (batched >= kMinTilesDev) { rec = r - 1; batched = 0; }
+        }
     }
+#pragma unroll
     for (int i = 0; i < kMaxRadDev + 2; ++i) effRad[l * (kMaxRadDev + 2) + i] = eff[i];
 }
 
