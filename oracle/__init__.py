@@ -46,6 +46,8 @@ class Beam(C.Structure):
         ("bmYDirection", C.c_float * 3), ("sourcePosition", C.c_float * 3),
         ("sad", C.c_float), ("useSubspotConvolution", C.c_int), ("nuclear", C.c_int),
         ("weq", _f), ("weqCount", C.c_size_t), ("spotIndices", C.c_int),
+        ("cpbOverride", C.c_int), ("cpbOvCorner", C.c_float * 2), ("cpbOvRes", C.c_float * 2),
+        ("cpbOvDims", C.c_int * 2),
     ]
 
 
@@ -235,6 +237,12 @@ def make_beam(beam: dict, keep: _Keep, use_subspots: bool = False) -> Beam:
             b.weqCount = int(a.size)
             break
     b.spotIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
+    ov = beam.get("cpb_override")  # (corner[2], res[2], dims[2]) in the gantry frame
+    if ov is not None:
+        b.cpbOverride = 1
+        b.cpbOvCorner[:] = [float(v) for v in ov[0]]
+        b.cpbOvRes[:] = [float(v) for v in ov[1]]
+        b.cpbOvDims[:] = [int(v) for v in ov[2]]
     return b
 
 

@@ -477,7 +477,7 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
 
     /* spot lattice and CPB / ray grid (wrapper:2951-3212) */
     float *lattice = build_lattice(b, p);
-    const int haveLattice = lattice != NULL;
+    const int haveLattice = lattice != NULL && !b->cpbOverride;
     free(lattice);
     f3 doseRes_mm = mk3(doseRes[0] * lenToMm, doseRes[1] * lenToMm, doseRes[2] * lenToMm);
     f3 doseCorner_mm = mk3(doseCorner[0] * lenToMm, doseCorner[1] * lenToMm, doseCorner[2] * lenToMm);
@@ -501,6 +501,13 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
         p->cpbRes[0] = b->raySpacing[0]; p->cpbRes[1] = b->raySpacing[1]; p->cpbRes[2] = doseRes[2];
         p->rayDims[0] = nx; p->rayDims[1] = ny;
         p->legacyPath = 0;
+    } else if (b->cpbOverride) {
+        /* cudaFinalDose: the CPB grid is the ROI box (cuda_final_dose.cu:302-308) */
+        p->cpbCorner[0] = b->cpbOvCorner[0]; p->cpbCorner[1] = b->cpbOvCorner[1]; p->cpbCorner[2] = 0.0f;
+        p->cpbRes[0] = b->cpbOvRes[0]; p->cpbRes[1] = b->cpbOvRes[1]; p->cpbRes[2] = doseRes[2];
+        p->rayDims[0] = b->cpbOvDims[0] > 1 ? b->cpbOvDims[0] : 1;
+        p->rayDims[1] = b->cpbOvDims[1] > 1 ? b->cpbOvDims[1] : 1;
+        p->legacyPath = 1;
     } else {
         /* legacy CPB path (wrapper:3175-3212): grid covers the projected dose box and subspots */
         if (!(b->raySpacing[0] > 0.0f) || !(b->raySpacing[1] > 0.0f))

@@ -67,6 +67,11 @@ typedef struct {
     const float *weq;
     size_t weqCount;
     int spotIndices;           /* spot positions are WEQ-lattice texel coordinates */
+    /* explicit CPB grid in the gantry frame (cudaFinalDose: the ROI box at 0.5 mm,
+     * cuda_final_dose.cu:302-308); 0 = the wrapper's own grid */
+    int cpbOverride;
+    float cpbOvCorner[2], cpbOvRes[2];
+    int cpbOvDims[2];
 } orc_beam;
 
 /* Beam plan: everything the host computes before and between kernels. */

@@ -117,8 +117,9 @@ struct pbs_engine {
     hipStream_t stream = nullptr;
     // LUTs (device) and host copies used by planning
     DevBuf<float> cidd, dens, sp, rrl, nucW, nucS2;
-    std::vector<float> hEnergies, hPeaks, hScales;
+    std::vector<float> hEnergies, hPeaks, hScales, hDens, hSp;
     pbs_energy_struct energy{};
+    PlanEnv env{};  // the environment's unit / input-type overrides, read when the engine is created
     LutDev lut{};
     // CT
     DevBuf<float> ct;
@@ -413,6 +414,13 @@ int pbs_device_count(void) {
 
 }  // extern "C"
 
+// The engine bound by pbs_final_dose_bind for cudaFinalDose (whose reference signature carries
+// no context). g_finalDoseMutex guards the pointer and is held for a whole cudaFinalDose call,
+// so pbs_engine_destroy can clear the binding without racing a call that uses it; calls of
+// cudaFinalDose are serialised, as the reference's blocking, global-device call was.
+static std::mutex g_finalDoseMutex;
+static pbs_engine *g_finalDoseEngine = nullptr;
+
 // An engine with one stream (oneStream) creates no side streams: the runtime maps streams onto
 // a few hardware queues, and idle side streams of many engines would put their main streams on
 // the same queue, where unrelated work serialises.
@@ -466,10 +474,15 @@ static pbs_engine *engine_create(int gpuId, const pbs_energy_struct *en, bool on
     e->hEnergies.assign(en->energiesPerU, en->energiesPerU + en->nEnergies);
     e->hPeaks.assign(en->peakDepths, en->peakDepths + en->nEnergies);
     e->hScales.assign(en->scaleFacts, en->scaleFacts + en->nEnergies);
+    e->hDens.assign(en->densityVector, en->densityVector + en->nDensitySamples);
+    e->hSp.assign(en->spVector, en->spVector + en->nSpSamples);
+    e->env = read_plan_env();
     e->energy = *en;
     e->energy.energiesPerU = e->hEnergies.data();
     e->energy.peakDepths = e->hPeaks.data();
     e->energy.scaleFacts = e->hScales.data();
+    e->energy.densityVector = e->hDens.data();
+    e->energy.spVector = e->hSp.data();
     e->energy.ciddMatrix = nullptr;  // device copy only
     e->lut = LutDev{e->cidd.p, en->nEnergies, en->nEnergySamples, e->dens.p, en->nDensitySamples, en->densityScaleFact,
                     e->sp.p, en->nSpSamples, en->spScaleFact, e->rrl.p, en->nRRlSamples, en->rRlScaleFact};
@@ -480,6 +493,10 @@ extern "C" {
 
 void pbs_engine_destroy(pbs_engine *e) {
     if (!e) return;
+    {
+        std::lock_guard<std::mutex> lk(g_finalDoseMutex);
+        if (g_finalDoseEngine == e) g_finalDoseEngine = nullptr;
+    }
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -557,7 +574,8 @@ int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorn
     int rc = validate_beam(*beam, 0);
     if (rc != PBS_OK) return rc;
     BeamPlan p;
-    if (!plan_beam(ctDims, ctRes, ctCorner, doseDims, doseRes, doseCorner, *beam, *energy, p, nullptr))
+    const PlanEnv env = read_plan_env();
+    if (!plan_beam(ctDims, ctRes, ctCorner, doseDims, doseRes, doseCorner, *beam, *energy, p, nullptr, &env))
         return set_error(PBS_E_GEOMETRY, p.error);
     pbs_plan_info &o = *info;
     o.legacyPath = p.legacyPath ? 1 : 0;
@@ -606,6 +624,28 @@ int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad) {
     return PBS_OK;
 }
 
+}  // extern "C"
+
+// CT adapter (wrapper:2497-2521, 2378-2441): the input type forced by RTD_CT_INPUT_TYPE or detected
+// from sampled values; HU, density and SPR volumes are converted to HU+1000 on the host (density and
+// SPR through the engine's density / SP table), an HU+1000 volume is used as given.
+static int adapt_ct(const pbs_engine *e, const float *src, size_t n, std::vector<float> &conv, const float *&out) {
+    const int type = e->env.ctInputType >= 0 ? e->env.ctInputType : detect_ct_type(src, n);
+    out = src;
+    if (type == 1) return PBS_OK;
+    conv.resize(n);
+    const bool dens = type == 2;
+    const std::vector<float> &lut = dens ? e->hDens : e->hSp;
+    if (!ct_to_hu_plus_1000(src, n, type, lut.data(), int(lut.size()),
+                            dens ? e->energy.densityScaleFact : e->energy.spScaleFact, conv.data()))
+        return set_error(PBS_E_INVALID, "CT conversion to HU+1000 failed: the density / SP table is empty or its "
+                                        "scale is 0 (RTD_CT_INPUT_TYPE=HU|HU+1000|DENSITY|SPR overrides the detection)");
+    out = conv.data();
+    return PBS_OK;
+}
+
+extern "C" {
+
 int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const float res[3], const float corner[3],
                       int ctIsDevice) {
     if (!e || !ct || !dims || !res || !corner) return set_error(PBS_E_INVALID, "ctData must not be null");
@@ -625,18 +665,9 @@ int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const f
         HIP_TRY(hipMemcpy(host.data(), ct, n * sizeof(float), hipMemcpyDeviceToHost));
         src = host.data();
     }
-    // CT adapter (wrapper:2290-2441): HU -> HU+1000; density/SPR are inverted through the LUT
-    const int type = detect_ct_type(src, n);
     std::vector<float> conv;
-    if (type == 0) {
-        conv.resize(n);
-        for (size_t i = 0; i < n; ++i) conv[i] = src[i] + 1000.0f;
-        src = conv.data();
-    } else if (type == 2) {
-        const float *lut = e->energy.densityVector;
-        (void)lut;
-        return set_error(PBS_E_UNSUPPORTED, "density-valued CT input: pass HU or HU+1000");
-    }
+    int rc = adapt_ct(e, src, n, conv, src);
+    if (rc != PBS_OK) return rc;
     HIP_TRY(e->ct.ensure(n));
     HIP_TRY(hipMemcpy(e->ct.p, src, n * sizeof(float), hipMemcpyHostToDevice));
     e->ctPtr = e->ct.p;
@@ -797,7 +828,8 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         e->uploadPending = false;
     }
     BeamPlan &p = e->plan;
-    if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p, ov))
+    if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p, ov,
+                   &e->env))
         return set_error(PBS_E_GEOMETRY, p.error);
     FanGeom &g = e->geom;
     fill_geom(p, *e, g);
@@ -1437,7 +1469,11 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
         return set_error(PBS_E_INVALID, "nuclearCorrection must be 0, 1 (GAUSS_FIT), 2 (SOUKUP) or 3 (FLUKA)");
     std::unique_ptr<pbs_engine, void (*)(pbs_engine *)> e(pbs_engine_create(gpuId, energy), pbs_engine_destroy);
     if (!e) return PBS_E_DEVICE;
-    rc = pbs_engine_set_ct(e.get(), ctData, ctDims, ctResolution, ctCorner, 0);
+    // the CT may be a host or a device buffer (texture_ultra_optimized.cu:43-62)
+    hipPointerAttribute_t ctAttr{};
+    const bool ctOnDevice = hipPointerGetAttributes(&ctAttr, ctData) == hipSuccess && ctAttr.type == hipMemoryTypeDevice;
+    (void)hipGetLastError();
+    rc = pbs_engine_set_ct(e.get(), ctData, ctDims, ctResolution, ctCorner, ctOnDevice ? 1 : 0);
     if (rc != PBS_OK) return rc;
     rc = pbs_engine_set_dose_grid(e.get(), doseDims, doseResolution, doseCorner);
     if (rc != PBS_OK) return rc;
@@ -1460,8 +1496,6 @@ int pbs_subsecond_wrapper(const float *ctData, const int ctDims[3], const float 
 // ---------------------------------------------------------------------------------------
 // cudaFinalDose
 // ---------------------------------------------------------------------------------------
-static pbs_engine *g_finalDoseEngine = nullptr;
-
 int pbs_final_dose_bind(pbs_engine *e, const float *layerEnergies, const float *layerSigmas, int numLayers,
                         float raySpacing) {
     if (!e || !layerEnergies || !layerSigmas || numLayers <= 0)
@@ -1469,6 +1503,7 @@ int pbs_final_dose_bind(pbs_engine *e, const float *layerEnergies, const float *
     e->fdEnergies.assign(layerEnergies, layerEnergies + numLayers);
     e->fdSigmas.assign(layerSigmas, layerSigmas + 2 * numLayers);
     e->fdRaySpacing = raySpacing;
+    std::lock_guard<std::mutex> lk(g_finalDoseMutex);
     g_finalDoseEngine = e;
     return PBS_OK;
 }
@@ -1527,11 +1562,12 @@ int cudaFinalDose(const float *subspotData, const vec3i *roiIndices, int numROII
     const int hit = pbs_final_dose_roi(roi.data(), numROIIndices, doseGridResolution, doseGridCorner, beamDirection,
                                        sourcePosition, numLayers, &mn, &mx, &rd, &cpb);
     if (!hit) return PBS_OK;  // the reference returns before touching finalDose
-    const size_t nd = size_t(doseGridDims.x) * doseGridDims.y * doseGridDims.z;
-    HIP_TRY(hipMemset(finalDose, 0, nd * sizeof(float)));
+    std::lock_guard<std::mutex> lk(g_finalDoseMutex);
     pbs_engine *e = g_finalDoseEngine;
     if (!e || int(e->fdEnergies.size()) != numLayers)
         return set_error(PBS_E_INVALID, "cudaFinalDose: bind CT, LUTs and layer energies with pbs_final_dose_bind");
+    const size_t nd = size_t(doseGridDims.x) * doseGridDims.y * doseGridDims.z;
+    HIP_TRY(hipMemset(finalDose, 0, nd * sizeof(float)));
     std::vector<float> sub(size_t(numLayers) * maxSubspotsPerLayer * 5);
     {
         hipPointerAttribute_t at{};
@@ -1934,7 +1970,7 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
         } else {
             ServerCt &ct = *sv->cts[r.ctId];
             if (!plan_beam(ct.dims, ct.res, ct.corner, r.doseDims, r.doseRes, r.doseCorner, r.beam.b,
-                           sv->slots[size_t(slot)]->energy, plan, nullptr)) {
+                           sv->slots[size_t(slot)]->energy, plan, nullptr, &sv->slots[size_t(slot)]->env)) {
                 sv->queue.erase(sv->queue.begin() + long(qi));
                 finish_error(sv, r, PBS_E_GEOMETRY, plan.error);
                 return true;
@@ -2150,14 +2186,10 @@ int pbs_server_add_ct(pbs_server *sv, const float *ct, const int ctDims[3], cons
         HIP_TRY(hipMemcpy(host.data(), ct, n * sizeof(float), hipMemcpyDeviceToHost));
         src = host.data();
     }
-    const int type = detect_ct_type(src, n);
-    if (type == 2) return set_error(PBS_E_UNSUPPORTED, "density-valued CT input: pass HU or HU+1000");
     std::vector<float> conv;
-    if (type == 0) {
-        conv.resize(n);
-        for (size_t i = 0; i < n; ++i) conv[i] = src[i] + 1000.0f;
-        src = conv.data();
-    }
+    if (sv->slots.empty()) return set_error(PBS_E_INVALID, "server has no engine slots");
+    int rc = adapt_ct(sv->slots[0], src, n, conv, src);
+    if (rc != PBS_OK) return rc;
     HIP_TRY(c->ct.ensure(n));
     HIP_TRY(hipMemcpy(c->ct.p, src, n * sizeof(float), hipMemcpyHostToDevice));
     for (int i = 0; i < 3; ++i) { c->dims[i] = ctDims[i]; c->res[i] = ctRes[i]; c->corner[i] = ctCorner[i]; }

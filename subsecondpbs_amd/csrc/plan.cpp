@@ -3,8 +3,11 @@
 #include "plan.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 namespace pbs {
 
@@ -124,6 +127,70 @@ int detect_ct_type(const float *ct, size_t n) {
     if (mn < -200.0f) return 0;
     if (mx <= 20.0f && mn >= 0.0f) return 2;
     return 1;
+}
+
+static float fractional_index_monotonic(const float *v, int n, float value) {
+    // findFractionalIndexMonotonic (wrapper:2337-2375): clamp, bisection, linear fraction
+    if (n <= 1) return 0.0f;
+    const bool inc = v[n - 1] >= v[0];
+    if (inc) {
+        if (value <= v[0]) return 0.0f;
+        if (value >= v[n - 1]) return float(n - 1);
+    } else {
+        if (value >= v[0]) return 0.0f;
+        if (value <= v[n - 1]) return float(n - 1);
+    }
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (inc ? (v[mid] <= value) : (v[mid] >= value)) lo = mid; else hi = mid;
+    }
+    const float d = v[lo + 1] - v[lo];
+    float t = d != 0.0f ? (value - v[lo]) / d : 0.0f;
+    t = std::min(std::max(t, 0.0f), 1.0f);
+    return float(lo) + t;
+}
+
+bool ct_to_hu_plus_1000(const float *in, size_t n, int type, const float *lut, int lutN, float scale, float *out) {
+    if (type == 0) {
+        for (size_t i = 0; i < n; ++i) out[i] = in[i] + 1000.0f;
+        return true;
+    }
+    if (!lut || lutN <= 0 || scale == 0.0f) return false;
+    for (size_t i = 0; i < n; ++i) out[i] = fractional_index_monotonic(lut, lutN, in[i]) / scale;
+    return true;
+}
+
+PlanEnv read_plan_env() {
+    PlanEnv env;
+    auto lower = [](const char *v) {
+        std::string s(v);
+        for (char &c : s) c = char(std::tolower(static_cast<unsigned char>(c)));
+        return s;
+    };
+    if (const char *v = std::getenv("RTD_CT_INPUT_TYPE")) {
+        const std::string s = lower(v);
+        if (s == "hu") env.ctInputType = 0;
+        else if (s == "hu+1000" || s == "huplus1000" || s == "hup1000") env.ctInputType = 1;
+        else if (s == "density" || s == "rho") env.ctInputType = 2;
+        else if (s == "spr" || s == "stoppingpower" || s == "stopping_power") env.ctInputType = 3;
+    }
+    const char *scale = std::getenv("RTD_ENERGY_DEPTH_SCALE");
+    const char *unit = std::getenv("RTD_ENERGY_DEPTH_UNIT");
+    if (scale && scale[0] != '\0') {
+        const float f = std::max(0.0f, float(std::atof(scale)));
+        env.energyDepthMode = f == 0.0f ? 2 : 1;
+        env.energyDepthToMm = f;
+    } else if (unit && unit[0] != '\0') {
+        const std::string u = lower(unit);
+        env.energyDepthMode = (u == "cm" || u == "mm") ? 1 : 2;
+        env.energyDepthToMm = u == "cm" ? 10.0f : 1.0f;
+    }
+    if (const char *v = std::getenv("RTD_CPB_COVERAGE_CM")) {
+        const float f = float(std::atof(v));
+        if (f > 0.0f && std::isfinite(f)) env.cpbCoverage = f;
+    }
+    return env;
 }
 
 void batch_bins(const int *histIn, int *batched, int *effRad) {
@@ -321,8 +388,9 @@ static float sigma_to_mm(float lenToMm, float s) { return (lenToMm > 1.0f && s >
 
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
                const float doseRes[3], const float doseCorner[3], const pbs_beam_settings &b,
-               const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov) {
+               const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov, const PlanEnv *envIn) {
     p = BeamPlan{};
+    const PlanEnv env = envIn ? *envIn : PlanEnv{};
     if (b.numLayers <= 0 || b.numLayers > kMaxLayers) { p.error = "numLayers out of range"; return false; }
     p.numLayers = b.numLayers;
     p.subspotConv = b.useSubspotConvolution != 0;
@@ -331,7 +399,8 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
     p.energyDepthToMm = p.lenToMm;
     float peakMax = 0.0f;
     for (int i = 0; i < en.nEnergies; ++i) peakMax = std::max(peakMax, en.peakDepths[i]);
-    if (p.lenToMm == 10.0f && peakMax > 200.0f) p.energyDepthToMm = 1.0f;
+    if (env.energyDepthMode == 1) p.energyDepthToMm = env.energyDepthToMm;
+    else if (env.energyDepthMode == 0 && p.lenToMm == 10.0f && peakMax > 200.0f) p.energyDepthToMm = 1.0f;
     const float lenToMm = p.lenToMm;
 
     // beam basis (wrapper:2893-2936)
@@ -416,8 +485,9 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
                     minGY = std::fmin(minGY, sy); maxGY = std::fmax(maxGY, sy);
                 }
             }
-        const float covX = (maxGX - minGX) / lenToMm, covY = (maxGY - minGY) / lenToMm;
+        float covX = (maxGX - minGX) / lenToMm, covY = (maxGY - minGY) / lenToMm;
         if (!(covX > 0.0f) || !(covY > 0.0f)) { p.error = "Legacy CPB path produced non-positive coverage"; return false; }
+        if (env.cpbCoverage > 0.0f) covX = covY = env.cpbCoverage;
         p.cpbCorner[0] = minGX / lenToMm; p.cpbCorner[1] = minGY / lenToMm; p.cpbCorner[2] = 0.0f;
         p.cpbRes[0] = b.raySpacing[0]; p.cpbRes[1] = b.raySpacing[1]; p.cpbRes[2] = doseRes[2];
         p.rayDims[0] = std::max(1, int(std::ceil(covX / p.cpbRes[0])));

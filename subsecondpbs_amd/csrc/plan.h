@@ -99,11 +99,29 @@ struct HaloPlan {
 // Returns false with `error` set when the beam cannot drive the halo branch.
 bool plan_halo(const BeamPlan &plan, const pbs_beam_settings &beam, HaloPlan &halo, std::string &error);
 
+// The reference's environment overrides of its unit and input-type heuristics, read once by
+// read_plan_env() when an entry point starts (an engine reads them when it is created);
+// nothing here writes the environment.
+//  * RTD_CT_INPUT_TYPE = hu | hu+1000 (huplus1000, hup1000) | density (rho) | spr
+//    (stoppingpower, stopping_power), any case; unset or other = auto (wrapper:2288-2299);
+//  * RTD_ENERGY_DEPTH_SCALE = <float>: the LUT depth unit in mm (0 or negative: the CT's
+//    length unit); else RTD_ENERGY_DEPTH_UNIT = cm | mm (other values: the CT's length unit);
+//    with neither, the cm-CT / mm-table heuristic decides (wrapper:2742-2790);
+//  * RTD_CPB_COVERAGE_CM = <float > 0>: the legacy CPB grid's coverage in input units on both
+//    axes (wrapper:3193-3202).
+struct PlanEnv {
+    int ctInputType = -1;          // -1 auto, 0 HU, 1 HU+1000, 2 density, 3 SPR
+    int energyDepthMode = 0;       // 0 heuristic, 1 fixed energyDepthToMm, 2 the CT's length unit
+    float energyDepthToMm = 0.0f;
+    float cpbCoverage = 0.0f;      // > 0: legacy CPB coverage override
+};
+PlanEnv read_plan_env();
+
 // Returns false (and sets plan.error) on a contract failure.
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3],
                const int doseDims[3], const float doseRes[3], const float doseCorner[3],
                const pbs_beam_settings &beam, const pbs_energy_struct &energy, BeamPlan &plan,
-               const CpbOverride *cpbOverride = nullptr);
+               const CpbOverride *cpbOverride = nullptr, const PlanEnv *env = nullptr);
 
 // findDecimalOrdered (include/utils/vector_find.h:130-145) and the wrapper's own energy
 // bracket (wrapper:3926-3992).
@@ -112,6 +130,11 @@ float layer_energy_index(const pbs_energy_struct &e, float energy);
 
 // Reference CT auto-detection on a host buffer: 0 = HU, 1 = HU+1000, 2 = density/SPR.
 int detect_ct_type(const float *ct, size_t n);
+// convertCtToHUPlus1000 (wrapper:2337-2441) for type 0 (HU), 2 (density) or 3 (SPR): HU+1000 into
+// out[n]. Density and SPR are inverted through `lut` (the density or SP table, index =
+// HU+1000 x scale) by the reference's bisection, which picks one crossing where the table is not
+// monotonic. Returns false when the table is empty or its scale is 0.
+bool ct_to_hu_plus_1000(const float *in, size_t n, int type, const float *lut, int lutN, float scale, float *out);
 
 // Radius bins -> launch radii (complete_superposition.cu:413-490).
 void batch_bins(const int *hist34, int *batched33, int *effRad34);

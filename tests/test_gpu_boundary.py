@@ -111,10 +111,8 @@ def test_legacy_cpb_path_config1(config1):
     assert err <= DOSE_TOL
 
 
-@gpu
-def test_cuda_final_dose_roi_miss_and_hit(config1):
-    torch = _need_gpu()
-    case, _ = config1
+def _final_dose_setup(case):
+    """Engine bound for cudaFinalDose with the config-1 layers; a 20^3-voxel ROI in the phantom."""
     from subsecondpbs_amd.engine import Engine
     eng = Engine(case["energy"])
     eng.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
@@ -126,15 +124,51 @@ def test_cuda_final_dose_roi_miss_and_hit(config1):
     lib = _abi.lib()
     _abi.check(lib.pbs_final_dose_bind(C.c_void_p(eng.handle), energies.ctypes.data_as(_abi._f),
                                        sig.ctypes.data_as(_abi._f), L, C.c_float(0.5)), "pbs_final_dose_bind")
+    g = np.arange(22, 42)
+    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
+    roi = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.int32)
+    return eng, b, roi
+
+
+def _final_dose_oracle(case, b, roi, bdir, src, sad):
+    """FP64 dose of cudaFinalDose's chain: the legacy subspot -> CPB path on the ROI's 0.5 mm CPB
+    grid (cuda_final_dose.cu:66-129, 302-308), expressed in the gantry frame (y = -world y)."""
+    lib = _abi.lib()
+    mn, mx, rd, cpb = _abi.Vec3f(), _abi.Vec3f(), _abi.Vec3i(), _abi.CPBGrid()
+    roiv = (_abi.Vec3i * len(roi))(*[_abi.Vec3i(*map(int, r)) for r in roi])
+    hit = lib.pbs_final_dose_roi(roiv, len(roi), _abi.Vec3f(*case["dose_resolution"]), _abi.Vec3f(*case["dose_corner"]),
+                                 _abi.Vec3f(*bdir), _abi.Vec3f(*src), len(b["energies"]), C.byref(mn), C.byref(mx),
+                                 C.byref(rd), C.byref(cpb))
+    assert hit == 1
+    f32 = np.float32
+    ov = ((cpb.corner.x, float(-(f32(cpb.corner.y) + f32(cpb.dims.y) * f32(cpb.resolution.y)))),
+          (cpb.resolution.x, cpb.resolution.y), (cpb.dims.x, cpb.dims.y))
+    beam = {"energies": b["energies"], "spot_sigmas": b["spot_sigmas"], "subspot_data": b["subspot_data"],
+            "beam_direction": bdir, "beam_xdir": (1, 0, 0), "beam_ydir": (0, 1, 0), "source_position": src,
+            "sad": sad, "ray_spacing": (cpb.resolution.x, cpb.resolution.y), "cpb_override": ov}
+    c2 = dict(case)
+    c2["beams"] = [beam]
+    ref, plan = O.beam_dose(c2)
+    assert plan.legacyPath == 1 and tuple(plan.rayDims) == (cpb.dims.x, cpb.dims.y)
+    return ref
+
+
+@gpu
+def test_cuda_final_dose_full_chain_vs_oracle(config1):
+    """cudaFinalDose misses (dose untouched), then runs the whole chain on the ROI's CPB grid:
+    the dose is overwritten and agrees with the FP64 oracle's legacy CPB path on the same grid
+    within 0.5 % of Dmax."""
+    torch = _need_gpu()
+    case, _ = config1
+    eng, b, roi_np = _final_dose_setup(case)
+    L = len(b["energies"])
+    lib = _abi.lib()
     nx, ny, nz = case["dose_dims"]
     res = _abi.Vec3f(*case["dose_resolution"])
     corner = _abi.Vec3f(*case["dose_corner"])
     dims = _abi.Vec3i(nx, ny, nz)
     sub = torch.from_numpy(np.ascontiguousarray(b["subspot_data"])).cuda()
-    # ROI: a 20x20x20-voxel cube in the middle of the phantom
-    g = np.arange(22, 42)
-    zz, yy, xx = np.meshgrid(g, g, g, indexing="ij")
-    roi = torch.from_numpy(np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1).astype(np.int32)).cuda()
+    roi = torch.from_numpy(roi_np).cuda()
     dose = torch.full((nz, ny, nx), 7.0, dtype=torch.float32, device="cuda")
     # beam pointing away from the ROI: untouched
     rc = lib.cudaFinalDose(C.c_void_p(sub.data_ptr()), C.c_void_p(roi.data_ptr()), roi.shape[0], res, corner,
@@ -149,8 +183,33 @@ def test_cuda_final_dose_roi_miss_and_hit(config1):
     _abi.check(rc, "cudaFinalDose")
     d = dose.cpu().numpy()
     eng.close()
-    assert np.isfinite(d).all() and d.min() >= 0.0 and d.max() > 0.0
-    assert not (d == 7.0).any()
-    # dose is deposited along +z inside the lateral extent of the spots
-    prof = d.sum(axis=(1, 2))
-    assert prof[:5].sum() > 0 and prof.argmax() > 5
+    ref = _final_dose_oracle(case, b, roi_np, (0, 0, 1), (0, 0, -2000), 2000.0)
+    assert ref.max() > 0
+    err = _rel(d, ref)
+    print(f"cudaFinalDose config1 ROI chain max|dD|/Dmax = {err:.3e}")
+    assert err <= DOSE_TOL
+    assert float(d.sum()) == pytest.approx(float(ref.sum()), rel=1e-3)
+
+
+@gpu
+def test_cuda_final_dose_after_engine_destroy(config1):
+    """Destroying the bound engine clears the binding: a later call fails with PBS_E_INVALID
+    and leaves the dose untouched, instead of reading the freed engine."""
+    torch = _need_gpu()
+    case, _ = config1
+    eng, b, roi_np = _final_dose_setup(case)
+    eng.close()
+    lib = _abi.lib()
+    nx, ny, nz = case["dose_dims"]
+    sub = torch.from_numpy(np.ascontiguousarray(b["subspot_data"])).cuda()
+    roi = torch.from_numpy(roi_np).cuda()
+    dose = torch.full((nz, ny, nx), 7.0, dtype=torch.float32, device="cuda")
+    rc = lib.cudaFinalDose(C.c_void_p(sub.data_ptr()), C.c_void_p(roi.data_ptr()), roi.shape[0],
+                           _abi.Vec3f(*case["dose_resolution"]), _abi.Vec3f(*case["dose_corner"]),
+                           _abi.Vec3f(0, 0, 1), _abi.Vec3f(0, 0, -2000), C.c_float(2000.0), C.c_float(0.0),
+                           len(b["energies"]), int(sub.shape[1]), C.c_void_p(dose.data_ptr()),
+                           _abi.Vec3i(nx, ny, nz))
+    assert rc == -1  # PBS_E_INVALID
+    assert b"pbs_final_dose_bind" in lib.pbs_last_error()
+    torch.cuda.synchronize()
+    assert bool((dose == 7.0).all())
