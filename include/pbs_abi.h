@@ -217,6 +217,7 @@ typedef struct pbs_plan_info {
     float spotOffset[2], spotDelta[2], cpbCorner[2], cpbRes[2];
     float fanCorner[3], fanDelta[3], sourceDist[2];
     float volConst, volLin, volSq;
+    float lenToMm, energyDepthToMm; /* input length unit and LUT depth unit, in mm */
 } pbs_plan_info;
 /* layerEnergyIdx / layerPeakDepth [numLayers], denseWeights [numLayers][spotDims1][spotDims0]
  * (denseCap floats available) and spotCells [numSpots] (the dense cell of each spot, -1 off
@@ -225,6 +226,12 @@ int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorn
                    const float doseRes[3], const float doseCorner[3], const pbs_beam_settings *beam,
                    const pbs_energy_struct *energy, pbs_plan_info *info, float *layerEnergyIdx,
                    float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells);
+/* The engine's CT adapter on a host buffer, without a device (no reference counterpart: it exposes
+ * what pbs_engine_set_ct runs first, for tests on a machine without a GPU). The input type is
+ * RTD_CT_INPUT_TYPE's, else detected from sampled values (raytracedicom_wrapper.cu:2288-2335);
+ * HU, density and SPR are converted to HU+1000 (wrapper:2337-2441), HU+1000 is copied. out[n];
+ * *typeUsed (may be NULL) = 0 HU, 1 HU+1000, 2 density, 3 SPR. */
+int pbs_ct_to_hu_plus_1000(const float *ct, size_t n, const pbs_energy_struct *energy, float *out, int *typeUsed);
 
 /* Intermediate arrays of the last pbs_engine_beam_dose call copied to host (parity tests).
  * Any pointer may be NULL. Shapes: density/wepl [K][ry][rx]; firstInside/firstOutside

@@ -48,6 +48,7 @@ class Beam(C.Structure):
         ("weq", _f), ("weqCount", C.c_size_t), ("spotIndices", C.c_int),
         ("cpbOverride", C.c_int), ("cpbOvCorner", C.c_float * 2), ("cpbOvRes", C.c_float * 2),
         ("cpbOvDims", C.c_int * 2),
+        ("energyDepthMode", C.c_int), ("energyDepthToMm", C.c_float), ("cpbCoverage", C.c_float),
     ]
 
 
@@ -102,6 +103,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.orc_ct_adapter.restype = C.c_int
+        L.orc_ct_adapter.argtypes = [_f, C.c_size_t, C.c_int, C.POINTER(Energy), _f]
         L.orc_sizeof.restype = C.c_long
         L.orc_sizeof.argtypes = [C.c_int]
         L.orc_find_decimal_ordered.restype = C.c_float
@@ -237,6 +240,10 @@ def make_beam(beam: dict, keep: _Keep, use_subspots: bool = False) -> Beam:
             b.weqCount = int(a.size)
             break
     b.spotIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
+    # unit overrides as values (what the RTD_ENERGY_DEPTH_* / RTD_CPB_COVERAGE_CM variables select)
+    b.energyDepthMode = int(beam.get("energy_depth_mode", 0))
+    b.energyDepthToMm = float(beam.get("energy_depth_to_mm", 0.0))
+    b.cpbCoverage = float(beam.get("cpb_coverage", 0.0))
     ov = beam.get("cpb_override")  # (corner[2], res[2], dims[2]) in the gantry frame
     if ov is not None:
         b.cpbOverride = 1
@@ -301,7 +308,7 @@ def beam_dose(case: dict, beam_index: int = 0, dose: np.ndarray | None = None,
     bm = make_beam(case["beams"][beam_index], keep, use_subspots)
     bm.nuclear = int(nuclear)
     ct_dims, ct_res, ct_corner, dose_dims, dose_res, dose_corner = case_args(case)
-    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
+    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"], case["energy"])
     ct = np.ascontiguousarray(ct, dtype=np.float32)
     if dose is None:
         dose = np.zeros((dose_dims[2], dose_dims[1], dose_dims[0]), np.float64)
@@ -320,7 +327,7 @@ def entry_margin(case: dict, beam_index: int = 0, ct_hu_plus_1000: np.ndarray | 
     150 threshold (orc_trace_entry_margin)."""
     p = plan_beam(case, beam_index)
     ct_dims = case_args(case)[0]
-    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
+    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"], case["energy"])
     ct = np.ascontiguousarray(ct, dtype=np.float32)
     m = np.zeros((p.rayDims[1], p.rayDims[0]))
     lib().orc_trace_entry_margin(C.byref(p), _fp(ct), _ip(ct_dims), _dp(m))
@@ -335,7 +342,7 @@ def beam_adjoint(case: dict, g_dose: np.ndarray, beam_index: int = 0, use_subspo
     en = make_energy(case["energy"], keep)
     bm = make_beam(case["beams"][beam_index], keep, use_subspots)
     ct_dims, ct_res, ct_corner, dose_dims, dose_res, dose_corner = case_args(case)
-    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"])
+    ct = ct_hu_plus_1000 if ct_hu_plus_1000 is not None else ct_as_hu_plus_1000(case["ct"], case["energy"])
     ct = np.ascontiguousarray(ct, dtype=np.float32)
     g = np.ascontiguousarray(g_dose, dtype=np.float64).reshape(dose_dims[2], dose_dims[1], dose_dims[0])
     out = np.zeros(int(bm.numSpots), np.float64)
@@ -347,16 +354,15 @@ def beam_adjoint(case: dict, g_dose: np.ndarray, beam_index: int = 0, use_subspo
     return out
 
 
-def ct_as_hu_plus_1000(ct: np.ndarray) -> np.ndarray:
-    """CT adapter of the reference (wrapper:2307-2441): sampled min/max decide the unit;
-    HU is shifted by +1000, HU+1000 is kept. Density/SPR inputs are not handled here."""
+def ct_as_hu_plus_1000(ct: np.ndarray, energy: dict | None = None, forced_type: int = -1) -> np.ndarray:
+    """CT adapter of the reference (orc_ct_adapter; wrapper:2307-2441): the sampled min/max decide
+    the input type unless ``forced_type`` (0 HU, 1 HU+1000, 2 density, 3 SPR) is given; HU is
+    shifted by +1000, density / SPR are inverted through the tables of ``energy``."""
     flat = np.ascontiguousarray(ct, dtype=np.float32).ravel()
-    n = flat.size
-    stride = n // 100000 if n > 100000 else 1
-    s = flat[::stride]
-    mn, mx = float(s.min()), float(s.max())
-    if mn < -200.0:
-        return (flat + np.float32(1000.0)).reshape(ct.shape)
-    if mx <= 20.0 and mn >= 0.0:
-        raise ValueError("density-valued CT: the oracle does not invert LUTs")
-    return flat.reshape(ct.shape)
+    out = np.empty_like(flat)
+    keep = _Keep()
+    en = make_energy(energy, keep) if energy is not None else Energy()
+    t = lib().orc_ct_adapter(_fp(flat), flat.size, int(forced_type), C.byref(en), _fp(out))
+    if t < 0 or (t >= 2 and energy is None):
+        raise ValueError("density / SPR CT: the adapter needs the energy dict's tables")
+    return out.reshape(np.shape(ct))

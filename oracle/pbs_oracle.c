@@ -419,6 +419,53 @@ static int infer_source_dist(const orc_beam *b, f3 bmX, f3 bmY, f3 bmZ, float sa
 }
 
 /* sizeof of the structs as compiled (the Python mirrors are checked against it) */
+static float frac_index_monotonic(const float *v, int n, float value) {
+    /* findFractionalIndexMonotonic (wrapper:2337-2375) */
+    if (n <= 1) return 0.0f;
+    int inc = v[n - 1] >= v[0];
+    if (inc ? value <= v[0] : value >= v[0]) return 0.0f;
+    if (inc ? value >= v[n - 1] : value <= v[n - 1]) return (float)(n - 1);
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        int keepLo = inc ? v[mid] <= value : v[mid] >= value;
+        if (keepLo) lo = mid; else hi = mid;
+    }
+    float den = v[lo + 1] - v[lo];
+    float t = den != 0.0f ? (value - v[lo]) / den : 0.0f;
+    if (t < 0.0f) t = 0.0f;
+    if (t > 1.0f) t = 1.0f;
+    return (float)lo + t;
+}
+
+int orc_ct_adapter(const float *in, size_t n, int forcedType, const orc_energy *en, float *out) {
+    int type = forcedType;
+    if (type < 0) {
+        /* sampleCtStats / autoDetectCtType (wrapper:2307-2335): every (n / 1e5)-th value */
+        size_t stride = n > 100000 ? n / 100000 : 1;
+        float mn = in[0], mx = in[0];
+        for (size_t i = 0; i < n; i += stride) {
+            if (in[i] < mn) mn = in[i];
+            if (in[i] > mx) mx = in[i];
+        }
+        type = mn < -200.0f ? 0 : (mx <= 20.0f && mn >= 0.0f) ? 2 : 1;
+    }
+    if (type == 1) {
+        memcpy(out, in, n * sizeof(float));
+        return 1;
+    }
+    if (type == 0) {
+        for (size_t i = 0; i < n; ++i) out[i] = in[i] + 1000.0f;
+        return 0;
+    }
+    const float *lut = type == 2 ? en->density : en->sp;
+    int nl = type == 2 ? en->nDensity : en->nSp;
+    float scale = type == 2 ? en->densityScale : en->spScale;
+    if (!lut || nl <= 0 || scale == 0.0f) return -1;
+    for (size_t i = 0; i < n; ++i) out[i] = frac_index_monotonic(lut, nl, in[i]) / scale;
+    return type;
+}
+
 long orc_sizeof(int which) {
     return which == 0 ? (long)sizeof(orc_beam) : which == 1 ? (long)sizeof(orc_plan)
                       : which == 2 ? (long)sizeof(orc_energy) : -1;
@@ -436,7 +483,8 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
     p->energyDepthToMm = p->lenToMm;
     float peakMax = 0.0f;
     for (int i = 0; i < en->nEnergies; ++i) if (en->peaks[i] > peakMax) peakMax = en->peaks[i];
-    if (p->lenToMm == 10.0f && peakMax > 200.0f) p->energyDepthToMm = 1.0f;
+    if (b->energyDepthMode == 1) p->energyDepthToMm = b->energyDepthToMm;
+    else if (b->energyDepthMode == 0 && p->lenToMm == 10.0f && peakMax > 200.0f) p->energyDepthToMm = 1.0f;
     const float lenToMm = p->lenToMm;
 
     /* beam basis (wrapper:2893-2936) */
@@ -535,6 +583,7 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
             }
         float covX = (maxGX - minGX) / lenToMm, covY = (maxGY - minGY) / lenToMm;
         if (!(covX > 0.0f) || !(covY > 0.0f)) return fail(p, "Legacy CPB path produced non-positive coverage");
+        if (b->cpbCoverage > 0.0f) covX = covY = b->cpbCoverage;
         p->cpbCorner[0] = minGX / lenToMm; p->cpbCorner[1] = minGY / lenToMm; p->cpbCorner[2] = 0.0f;
         p->cpbRes[0] = b->raySpacing[0]; p->cpbRes[1] = b->raySpacing[1]; p->cpbRes[2] = doseRes[2];
         int nx = (int)ceilf(covX / p->cpbRes[0]), ny = (int)ceilf(covY / p->cpbRes[1]);

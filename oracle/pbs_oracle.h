@@ -72,6 +72,12 @@ typedef struct {
     int cpbOverride;
     float cpbOvCorner[2], cpbOvRes[2];
     int cpbOvDims[2];
+    /* the environment overrides of the units (wrapper:2742-2790, 3193-3202), as values:
+     * energyDepthMode 0 = the cm-CT / mm-table heuristic, 1 = energyDepthToMm as given,
+     * 2 = the CT's length unit; cpbCoverage > 0 replaces the legacy CPB coverage */
+    int energyDepthMode;
+    float energyDepthToMm;
+    float cpbCoverage;
 } orc_beam;
 
 /* Beam plan: everything the host computes before and between kernels. */
@@ -145,6 +151,14 @@ void orc_batch_bins(const int *hist /*[34]*/, int *batched /*[33]*/, int *effRad
 /* Tile radii for a [K][ry][rx] float field over steps [k0,k1): rad[(k-k0)][ty][tx].
  * NaN entries are ignored (fminf semantics). Pinned: brute-force min + integer formula. */
 void orc_tile_radii_f32(const float *rsigma, int rx, int ry, int k0, int k1, int *rad);
+
+/* CT adapter (wrapper:2307-2441): sampled min/max decide the input type unless forcedType >= 0
+ * (0 HU, 1 HU+1000, 2 density, 3 SPR; the RTD_CT_INPUT_TYPE override); HU gets +1000, density and
+ * SPR are inverted through the density / SP table by findFractionalIndexMonotonic's bisection
+ * (wrapper:2337-2375). out[n]. Returns the type used, or -1 when the table is empty.
+ * Pinned: HU shift; table round trip (the table at the result equals the input) on every value
+ * class and the bisection's crossing among all crossings on the non-monotonic density stretch. */
+int orc_ct_adapter(const float *in, size_t n, int forcedType, const orc_energy *en, float *out);
 
 /* ---- stages ---- */
 /* Host planning up to (but excluding) the trace. Pinned: closed-form grids of the configs. */

@@ -75,6 +75,7 @@ class PlanInfo(C.Structure):
         ("spotOffset", C.c_float * 2), ("spotDelta", C.c_float * 2), ("cpbCorner", C.c_float * 2),
         ("cpbRes", C.c_float * 2), ("fanCorner", C.c_float * 3), ("fanDelta", C.c_float * 3),
         ("sourceDist", C.c_float * 2), ("volConst", C.c_float), ("volLin", C.c_float), ("volSq", C.c_float),
+        ("lenToMm", C.c_float), ("energyDepthToMm", C.c_float),
     ]
 
 
@@ -97,7 +98,7 @@ EXPORTED = [
     "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
     "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo", "pbs_engine_debug_bins",
-    "pbs_plan_debug",
+    "pbs_plan_debug", "pbs_ct_to_hu_plus_1000",
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
@@ -163,6 +164,8 @@ def lib():
         L.pbs_engine_set_nuclear.argtypes = [C.c_void_p, C.c_int]
         L.pbs_engine_debug_bins.restype = C.c_int
         L.pbs_engine_debug_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pbs_ct_to_hu_plus_1000.restype = C.c_int
+        L.pbs_ct_to_hu_plus_1000.argtypes = [_f, C.c_size_t, C.POINTER(EnergyStruct), _f, _i]
         L.pbs_plan_debug.restype = C.c_int
         L.pbs_plan_debug.argtypes = [_i, _f, _f, _i, _f, _f, C.POINTER(BeamSettings), C.POINTER(EnergyStruct),
                                      C.POINTER(PlanInfo), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
@@ -389,6 +392,19 @@ def plan_debug(case: dict, beam: dict, use_subspots: bool | None = None) -> dict
     out["dense"] = dense[:L * nx * ny].reshape(L, ny, nx) if not info.legacyPath else None
     out["spotCell"] = cells[:int(bs.numSpots)]
     return out
+
+
+def ct_to_hu_plus_1000(ct, energy: dict):
+    """The engine's CT adapter on the host (pbs_ct_to_hu_plus_1000): (HU+1000 volume, input type
+    used: 0 HU, 1 HU+1000, 2 density, 3 SPR). RTD_CT_INPUT_TYPE forces the type."""
+    keep = Keep()
+    es = energy_struct(energy, keep)
+    a = np.ascontiguousarray(ct, dtype=np.float32)
+    out = np.empty_like(a)
+    t = C.c_int(-1)
+    check(lib().pbs_ct_to_hu_plus_1000(a.ctypes.data_as(_f), a.size, C.byref(es), out.ctypes.data_as(_f),
+                                       C.byref(t)), "pbs_ct_to_hu_plus_1000")
+    return out, t.value
 
 
 def weq_vector(beam: dict):

@@ -489,6 +489,32 @@ static pbs_engine *engine_create(int gpuId, const pbs_energy_struct *en, bool on
     return e.release();
 }
 
+// CT adapter (wrapper:2497-2521, 2378-2441): the input type forced by RTD_CT_INPUT_TYPE or detected
+// from sampled values; HU, density and SPR volumes are converted to HU+1000 on the host (density and
+// SPR through the engine's density / SP table), an HU+1000 volume is used as given.
+static int adapt_ct_with(const PlanEnv &env, const pbs_energy_struct &en, const float *src, size_t n, float *conv,
+                         int *typeUsed) {
+    const int type = env.ctInputType >= 0 ? env.ctInputType : detect_ct_type(src, n);
+    if (typeUsed) *typeUsed = type;
+    if (type == 1) return PBS_OK;
+    const bool dens = type == 2;
+    if (!ct_to_hu_plus_1000(src, n, type, dens ? en.densityVector : en.spVector,
+                            dens ? en.nDensitySamples : en.nSpSamples, dens ? en.densityScaleFact : en.spScaleFact, conv))
+        return set_error(PBS_E_INVALID, "CT conversion to HU+1000 failed: the density / SP table is empty or its "
+                                        "scale is 0 (RTD_CT_INPUT_TYPE=HU|HU+1000|DENSITY|SPR overrides the detection)");
+    return PBS_OK;
+}
+
+static int adapt_ct(const pbs_engine *e, const float *src, size_t n, std::vector<float> &conv, const float *&out) {
+    conv.resize(n);
+    int type = 1;
+    const int rc = adapt_ct_with(e->env, e->energy, src, n, conv.data(), &type);
+    if (rc != PBS_OK) return rc;
+    out = type == 1 ? src : conv.data();
+    if (type == 1) conv.clear();
+    return PBS_OK;
+}
+
 extern "C" {
 
 void pbs_engine_destroy(pbs_engine *e) {
@@ -600,6 +626,8 @@ int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorn
     o.volConst = p.volConst;
     o.volLin = p.volLin;
     o.volSq = p.volSq;
+    o.lenToMm = p.lenToMm;
+    o.energyDepthToMm = p.energyDepthToMm;
     for (int l = 0; l < p.numLayers; ++l) {
         if (layerEnergyIdx) layerEnergyIdx[l] = p.layers[l].energyIdx;
         if (layerPeakDepth) layerPeakDepth[l] = p.layers[l].peakDepth;
@@ -613,6 +641,16 @@ int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorn
     return PBS_OK;
 }
 
+int pbs_ct_to_hu_plus_1000(const float *ct, size_t n, const pbs_energy_struct *energy, float *out, int *typeUsed) {
+    if (!ct || !energy || !out || n == 0) return set_error(PBS_E_INVALID, "null or empty argument");
+    int type = 1;
+    const int rc = adapt_ct_with(read_plan_env(), *energy, ct, n, out, &type);
+    if (rc != PBS_OK) return rc;
+    if (type == 1) std::memcpy(out, ct, n * sizeof(float));
+    if (typeUsed) *typeUsed = type;
+    return PBS_OK;
+}
+
 int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad) {
     if (!e || !e->haveLast) return set_error(PBS_E_INVALID, "no beam has been computed");
     if (haloEffRad && !e->lastHalo) return set_error(PBS_E_INVALID, "the last beam ran without the halo branch");
@@ -623,28 +661,6 @@ int pbs_engine_debug_bins(pbs_engine *e, int *effRad, int *haloEffRad) {
     if (haloEffRad) HIP_TRY(hipMemcpy(haloEffRad, e->halo.effRad.p, n * sizeof(int), hipMemcpyDeviceToHost));
     return PBS_OK;
 }
-
-}  // extern "C"
-
-// CT adapter (wrapper:2497-2521, 2378-2441): the input type forced by RTD_CT_INPUT_TYPE or detected
-// from sampled values; HU, density and SPR volumes are converted to HU+1000 on the host (density and
-// SPR through the engine's density / SP table), an HU+1000 volume is used as given.
-static int adapt_ct(const pbs_engine *e, const float *src, size_t n, std::vector<float> &conv, const float *&out) {
-    const int type = e->env.ctInputType >= 0 ? e->env.ctInputType : detect_ct_type(src, n);
-    out = src;
-    if (type == 1) return PBS_OK;
-    conv.resize(n);
-    const bool dens = type == 2;
-    const std::vector<float> &lut = dens ? e->hDens : e->hSp;
-    if (!ct_to_hu_plus_1000(src, n, type, lut.data(), int(lut.size()),
-                            dens ? e->energy.densityScaleFact : e->energy.spScaleFact, conv.data()))
-        return set_error(PBS_E_INVALID, "CT conversion to HU+1000 failed: the density / SP table is empty or its "
-                                        "scale is 0 (RTD_CT_INPUT_TYPE=HU|HU+1000|DENSITY|SPR overrides the detection)");
-    out = conv.data();
-    return PBS_OK;
-}
-
-extern "C" {
 
 int pbs_engine_set_ct(pbs_engine *e, const float *ct, const int dims[3], const float res[3], const float corner[3],
                       int ctIsDevice) {

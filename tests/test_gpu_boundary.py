@@ -213,3 +213,63 @@ def test_cuda_final_dose_after_engine_destroy(config1):
     assert b"pbs_final_dose_bind" in lib.pbs_last_error()
     torch.cuda.synchronize()
     assert bool((dose == 7.0).all())
+
+
+@gpu
+def test_density_ct_input_and_forced_type(config1, monkeypatch):
+    """A density-valued CT (the table's values at the config-1 HU+1000 texels) is detected as
+    density and inverted through the table back to the same HU+1000 volume, so the dose equals
+    the HU+1000 CT's bitwise; RTD_CT_INPUT_TYPE=hu on the HU+1000 volume shifts it by 1000 and
+    changes the dose."""
+    torch = _need_gpu()
+    case, ref = config1
+    from subsecondpbs_amd.engine import Engine
+    hp = O.ct_as_hu_plus_1000(case["ct"], case["energy"])
+    dens = np.asarray(case["energy"]["density_vector"], np.float32)[np.rint(hp).astype(np.int64)]
+    assert np.array_equal(np.rint(hp), hp)
+    nx, ny, nz = case["dose_dims"]
+    out = []
+    for ct, env in ((hp, None), (dens, None), (hp, "hu")):
+        if env:
+            monkeypatch.setenv("RTD_CT_INPUT_TYPE", env)
+        else:
+            monkeypatch.delenv("RTD_CT_INPUT_TYPE", raising=False)
+        eng = Engine(case["energy"])  # the engine reads the environment when it is created
+        eng.set_ct(ct, case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+        eng.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+        d = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+        eng.beam_dose(case["beams"][0], d)
+        eng.sync()
+        eng.close()
+        out.append(d.cpu().numpy())
+    assert _rel(out[0], ref) <= DOSE_TOL
+    assert np.array_equal(out[1], out[0])
+    assert not np.array_equal(out[2], out[0])
+
+
+@gpu
+def test_subsecond_wrapper_device_ct_pointer(config1):
+    """ctData may be a device buffer (the reference detects the pointer kind,
+    texture_ultra_optimized.cu:43-62): the dose equals the host-CT call's bitwise."""
+    torch = _need_gpu()
+    case, ref = config1
+    keep = _abi.Keep()
+    es = _abi.energy_struct(case["energy"], keep)
+    arr = (_abi.BeamSettings * 1)()
+    arr[0] = _abi.beam_settings(case["beams"][0], keep)
+    ct = np.ascontiguousarray(case["ct"], np.float32)
+    ct_dev = torch.from_numpy(ct).cuda()
+    torch.cuda.synchronize()
+    args = []
+    for v, f in ((case["ct_dims"], _abi.i3), (case["ct_resolution"], _abi.f3), (case["ct_corner"], _abi.f3),
+                 (case["dose_dims"], _abi.i3), (case["dose_resolution"], _abi.f3), (case["dose_corner"], _abi.f3)):
+        args.append(f(v))  # (array, pointer): the array stays alive in args
+    doses = []
+    for ptr in (ct.ctypes.data_as(_abi._f), C.cast(C.c_void_p(ct_dev.data_ptr()), _abi._f)):
+        dose = np.zeros(tuple(reversed(case["dose_dims"])), np.float32)
+        rc = _abi.lib().pbs_subsecond_wrapper(ptr, args[0][1], args[1][1], args[2][1], dose.ctypes.data_as(_abi._f),
+                                              args[3][1], args[4][1], args[5][1], arr, 1, es, 0, 0, 0)
+        _abi.check(rc, "pbs_subsecond_wrapper")
+        doses.append(dose)
+    assert _rel(doses[0], ref) <= DOSE_TOL
+    assert np.array_equal(doses[0], doses[1])
