@@ -2,8 +2,8 @@
 (``subsecondpbs_amd.carbon``; not reference data, every result here uses that project LUT).
 
 The four carbon beams of the batching case (config-1 CT, 20 layers within 100-400 MeV/u):
-plan integers, transfer box and tile radii (the oracle's float rule applied to the GPU's sigma)
-bit-exact; dose within 0.5 % of Dmax with the halo branch off and on (FLUKA constants; halo
+plan integers, transfer box, tile radii (the oracle's float rule applied to the GPU's sigma),
+radius histograms and batched launch radii (the oracle's own plan) bit-exact; dose within 0.5 % of Dmax with the halo branch off and on (FLUKA constants; halo
 radius histograms bit-exact under the same rule); and the whole 16-beam mixed batch (12 proton +
 4 carbon beams, each set on its own ``Batch``) against the sum of the oracle's per-beam doses.
 """
@@ -47,6 +47,7 @@ def _engine_dose(one, model):
     eng.beam_dose(one["beams"][0], dose)
     eng.sync()
     ints = eng.debug_fetch(("radHist", "iddSigma", "tileRad"))
+    ints["effRad"] = eng.debug_bins()
     halo = eng.debug_halo() if model else None
     out = dose.cpu().numpy()
     eng.close()
@@ -77,11 +78,18 @@ def test_carbon_beam_plain(config3, i):
         want = np.zeros((k1 - first, sy // 8, sx // 32), np.int32)
         O.lib().orc_tile_radii_f32(O._fp(rs), sx, sy, first, k1, O._ip(want))
         np.testing.assert_array_equal(dbg["tileRad"][l, first:k1] & 255, want)
-    moved = int(np.abs(dbg["radHist"] - np.array([list(p.radHist[l]) for l in range(L)], np.int32)).sum())
+    # radius histograms and batched launch radii: the oracle's own plan (its FP64 sigma)
+    np.testing.assert_array_equal(dbg["radHist"], np.array([list(p.radHist[l]) for l in range(L)], np.int32))
+    eff = dbg["effRad"]
+    for l in range(L):
+        e = np.zeros(34, np.int32)
+        bt = np.zeros(33, np.int32)
+        O.lib().orc_batch_bins(O._ip(np.ascontiguousarray(np.array(p.radHist[l]), np.int32)), O._ip(bt), O._ip(e))
+        np.testing.assert_array_equal(eff[l], e, err_msg=f"layer {l}")
+        np.testing.assert_array_equal(bt, np.array(p.batchRad[l]), err_msg=f"layer {l}")
     assert ints["transferStart"] == tuple(p.transferStart) and ints["transferMax"] == tuple(p.transferMax)
     err = _rel(dose, ref)
-    print(f"carbon beam {i}: max|dD|/Dmax = {err:.3e}; radius-histogram entries off the FP64 "
-          f"oracle's: {moved} (project-defined 12C tables)")
+    print(f"carbon beam {i}: max|dD|/Dmax = {err:.3e} (project-defined 12C tables)")
     assert err <= DOSE_TOL
 
 
