@@ -135,6 +135,18 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int doseDims[3], const float d
 /* Accumulate one beam's dose into doseDevice ([nz][ny][nx] float32 device buffer) on the
  * engine's stream; returns after the work is queued (synchronize with pbs_engine_sync). */
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDevice);
+/* pbs_engine_beam_dose without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Every per-beam work buffer is carved from a static 256 MiB device scratch
+ * (one per device, held from the library's load until the process ends) and the beam's small
+ * arrays go up through it when the engine's pinned staging is too small; no hipMalloc, hipFree or
+ * host-pinned allocation is made. Buffers the engine already holds are used when they are large
+ * enough. A beam whose work buffers exceed the scratch fails with PBS_E_NOMEM and a message (the
+ * config-1-size beams fit: 64^3 CT, 96x80 rays, 64 steps, 5 layers; a head-size beam does not).
+ * Every call on one device shares that device's scratch, which keeps the last call's contents:
+ * calls that overlap on one GPU (from several threads or engines) race, so callers serialise
+ * them. Results equal pbs_engine_beam_dose's bitwise. Stage profiling must be off. */
+int pbs_engine_beam_dose_noalloc(pbs_engine *e, const pbs_beam_settings *beam, float *doseDevice);
 /* Adjoint with respect to the spot weights. No counterpart in the reference, which has no
  * gradient path; it serves weight optimisation (the transposed superposition gather, then the
  * transposed subspot convolution). For J(dose) with dJ/dDose = gradDoseDevice ([nz][ny][nx]

@@ -95,7 +95,7 @@ class CPBGrid(C.Structure):
 EXPORTED = [
     "pbs_subsecond_wrapper", "pbs_last_error", "pbs_device_count",
     "pbs_engine_create", "pbs_engine_destroy", "pbs_engine_set_ct", "pbs_engine_set_dose_grid",
-    "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
+    "pbs_abi_sizeof", "pbs_engine_beam_dose", "pbs_engine_beam_dose_noalloc", "pbs_engine_beam_adjoint", "pbs_engine_keep_beam", "pbs_engine_update_beam",
     "pbs_engine_update_adjoint", "pbs_resident_beam_use_graph", "pbs_resident_beam_destroy", "pbs_engine_sync", "pbs_engine_stream", "pbs_engine_debug_fetch",
     "pbs_engine_set_profiling", "pbs_engine_stage_times", "pbs_engine_set_tuning", "pbs_engine_get_tuning", "pbs_engine_set_nuclear", "pbs_engine_debug_halo", "pbs_engine_debug_bins",
     "pbs_plan_debug", "pbs_ct_to_hu_plus_1000",
@@ -134,6 +134,8 @@ def lib():
         L.pbs_engine_set_dose_grid.argtypes = [C.c_void_p, _i, _f, _f]
         L.pbs_engine_beam_dose.restype = C.c_int
         L.pbs_engine_beam_dose.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p]
+        L.pbs_engine_beam_dose_noalloc.restype = C.c_int
+        L.pbs_engine_beam_dose_noalloc.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p]
         L.pbs_engine_beam_adjoint.restype = C.c_int
         L.pbs_engine_beam_adjoint.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p, C.c_void_p]
         L.pbs_engine_keep_beam.restype = C.c_int

@@ -47,27 +47,66 @@ int set_error(int code, const std::string &msg) {
 // holds raw addresses and is replayed only while this count is the one it was captured at.
 std::atomic<unsigned long long> g_bufGen{0};
 
+// Bump allocator over a device scratch region, active on the calling thread for the duration of
+// a no-allocation entry point: DevBuf::ensure then carves from it instead of calling hipMalloc.
+struct Arena {
+    unsigned char *base = nullptr;  // device address of the scratch
+    size_t cap = 0, used = 0;
+    bool overflow = false;
+};
+thread_local Arena *t_arena = nullptr;
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;  // p lies in a no-allocation call's scratch (never freed)
+    T *parked = nullptr;    // a too-small own allocation replaced during such a call, freed later
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) {
+        if (p && !borrowed) {
             (void)hipFree(p);
             g_bufGen.fetch_add(1);
         }
+        if (parked) {
+            (void)hipFree(parked);
+            g_bufGen.fetch_add(1);
+        }
         p = nullptr;
+        parked = nullptr;
+        borrowed = false;
         n = 0;
     }
     hipError_t ensure(size_t count) {
-        if (count <= n && p) return hipSuccess;
+        count = std::max<size_t>(count, 1);
+        if (Arena *a = t_arena) {
+            // an own allocation that is large enough is used as it is; anything else is carved
+            // from the scratch (a carve of an earlier call is stale: the scratch is re-carved on
+            // every call)
+            if (p && !borrowed && count <= n) return hipSuccess;
+            const size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+            if (a->used + bytes > a->cap) {
+                a->overflow = true;
+                return hipErrorOutOfMemory;
+            }
+            if (p && !borrowed) {
+                if (parked) return hipErrorOutOfMemory;  // cannot happen: a parked buffer's p is borrowed
+                parked = p;
+            }
+            p = reinterpret_cast<T *>(a->base + a->used);
+            a->used += bytes;
+            n = count;
+            borrowed = true;
+            g_bufGen.fetch_add(1);
+            return hipSuccess;
+        }
+        if (count <= n && p && !borrowed) return hipSuccess;
         release();
-        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(count, 1) * sizeof(T));
-        if (e == hipSuccess) n = std::max<size_t>(count, 1);
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        if (e == hipSuccess) n = count;
         return e;
     }
 };
@@ -75,6 +114,34 @@ struct DevBuf {
 }  // namespace
 
 using namespace pbs;
+
+// Device scratch of pbs_engine_beam_dose_noalloc: every per-beam buffer of such a call is carved
+// from it, so the call allocates nothing. Each device holds its own copy from the library's load
+// until the process ends. Host code never uses the array's name, only the device address that
+// hipGetSymbolAddress returns for the current device (g_scratchDev).
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMu;
+static unsigned char *g_scratchDev[kMaxDevices] = {};
+static bool g_scratchTried[kMaxDevices] = {};
+
+// The scratch's device address on `dev`, looked up once per device with that device current
+// (engine creation); nullptr when the lookup failed.
+static unsigned char *scratch_on_device(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchTried[dev]) {
+        g_scratchTried[dev] = true;
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess)
+            g_scratchDev[dev] = static_cast<unsigned char *>(ptr);
+        else
+            (void)hipGetLastError();
+    }
+    return g_scratchDev[dev];
+}
 
 // Buffers of one superposition grid (tiles, work lists, item blocks, BEV planes, groups).
 struct GridWork {
@@ -184,6 +251,10 @@ struct pbs_engine {
     DevBuf<float> adjI, adjW, adjInterm, adjSpot;
     ConvArgs convArgs{};
     std::vector<float> hostAdjSpot;
+    // staging of a no-allocation call whose pinned staging is too small: host bytes and their
+    // device copy (carved from the scratch) that k_upload reads
+    std::vector<char> hostStage;
+    DevBuf<char> devStage;
 };
 
 // Resident state of a served beam: everything of its first run that the spot weights do not
@@ -438,6 +509,7 @@ static pbs_engine *engine_create(int gpuId, const pbs_energy_struct *en, bool on
     std::unique_ptr<pbs_engine> e(new pbs_engine());
     e->device = gpuId;
     e->oneStream = oneStream;
+    (void)scratch_on_device(gpuId);  // set-up of the no-allocation entry point on this device
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->uploadDone, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&e->forkEv, hipEventDisableTiming) != hipSuccess) {
@@ -707,6 +779,23 @@ int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *do
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
 }
 
+int pbs_engine_beam_dose_noalloc(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
+    if (!e) return set_error(PBS_E_INVALID, "null engine");
+    if (e->profiling)
+        return set_error(PBS_E_INVALID, "pbs_engine_beam_dose_noalloc: stage profiling creates events; turn it off");
+    unsigned char *base = scratch_on_device(e->device);
+    if (!base) return set_error(PBS_E_DEVICE, "pbs_engine_beam_dose_noalloc: the device scratch is not available");
+    Arena a{base, kNoAllocScratchBytes, 0, false};
+    t_arena = &a;
+    const int rc = engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
+    t_arena = nullptr;
+    if (a.overflow)
+        return set_error(PBS_E_NOMEM, "pbs_engine_beam_dose_noalloc: the beam's work buffers exceed the " +
+                                          std::to_string(kNoAllocScratchBytes >> 20) +
+                                          " MiB no-allocation scratch; use pbs_engine_beam_dose");
+    return rc;
+}
+
 int pbs_engine_beam_adjoint(pbs_engine *e, const pbs_beam_settings *beam, const float *gradDoseDev,
                             float *gradSpotWeights) {
     if (!gradDoseDev || !gradSpotWeights) return set_error(PBS_E_INVALID, "null argument");
@@ -818,10 +907,10 @@ static hipError_t ensure_pin(pbs_engine *e, size_t bytes) {
 }
 
 // One segment of the beam's upload: `bytes` (a multiple of 4) from the staging at `off`.
-static void add_upload(UploadArgs &a, const pbs_engine *e, void *dst, size_t off, size_t bytes) {
+static void add_upload(UploadArgs &a, const char *stageDev, void *dst, size_t off, size_t bytes) {
     if (bytes == 0) return;
     UploadSeg &u = a.seg[a.n++];
-    u.src = reinterpret_cast<const uint32_t *>(e->pinDev + off);
+    u.src = reinterpret_cast<const uint32_t *>(stageDev + off);
     u.dst = static_cast<uint32_t *>(dst);
     u.words = uint32_t(bytes / 4);
 }
@@ -946,23 +1035,40 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     const size_t bRay = halo ? sizeof(int) * nray : 0, bHaloW = halo ? sizeof(float) * e->haloPlan.weights.size() : 0;
     const size_t oSub = pin_align(bLayers), oDense = oSub + pin_align(bSub), oRay = oDense + pin_align(bDense);
     const size_t oHaloW = oRay + pin_align(bRay);
-    HIP_TRY(ensure_pin(e, oHaloW + pin_align(bHaloW)));
-    std::memcpy(e->pin, lh.data(), bLayers);
-    std::memcpy(e->pin + oSub, beam->subspotData, bSub);
-    if (bDense) std::memcpy(e->pin + oDense, p.denseWeights.data(), bDense);
+    const size_t stageBytes = oHaloW + pin_align(bHaloW);
+    // the staging k_upload reads: the mapped pinned buffer, or (a no-allocation call whose pinned
+    // staging is too small) a device copy of host bytes carved from the scratch
+    const bool devStage = t_arena && (!e->pin || e->pinBytes < stageBytes);
+    char *stageHost = nullptr;
+    const char *stageSrc = nullptr;
+    if (devStage) {
+        e->hostStage.resize(stageBytes);
+        HIP_TRY(e->devStage.ensure(stageBytes));
+        stageHost = e->hostStage.data();
+        stageSrc = e->devStage.p;
+    } else {
+        HIP_TRY(ensure_pin(e, stageBytes));
+        stageHost = e->pin;
+        stageSrc = e->pinDev;
+    }
+    std::memcpy(stageHost, lh.data(), bLayers);
+    std::memcpy(stageHost + oSub, beam->subspotData, bSub);
+    if (bDense) std::memcpy(stageHost + oDense, p.denseWeights.data(), bDense);
+    if (halo) {
+        std::memcpy(stageHost + oRay, e->haloPlan.rayToNuc.data(), bRay);
+        std::memcpy(stageHost + oHaloW, e->haloPlan.weights.data(), bHaloW);
+    }
+    if (devStage) HIP_TRY(hipMemcpyAsync(e->devStage.p, stageHost, stageBytes, hipMemcpyHostToDevice, s));
     UploadArgs up{};
-    add_upload(up, e, e->layerHost.p, 0, bLayers);
-    add_upload(up, e, e->subspots.p, oSub, bSub);
+    add_upload(up, stageSrc, e->layerHost.p, 0, bLayers);
+    add_upload(up, stageSrc, e->subspots.p, oSub, bSub);
     if (bDense) {
         HIP_TRY(e->spot.ensure(p.denseWeights.size()));
-        add_upload(up, e, e->spot.p, oDense, bDense);
+        add_upload(up, stageSrc, e->spot.p, oDense, bDense);
     }
     if (halo) {
-        const HaloPlan &hp = e->haloPlan;
-        std::memcpy(e->pin + oRay, hp.rayToNuc.data(), bRay);
-        std::memcpy(e->pin + oHaloW, hp.weights.data(), bHaloW);
-        add_upload(up, e, e->rayToNuc.p, oRay, bRay);
-        add_upload(up, e, e->haloW.p, oHaloW, bHaloW);
+        add_upload(up, stageSrc, e->rayToNuc.p, oRay, bRay);
+        add_upload(up, stageSrc, e->haloW.p, oHaloW, bHaloW);
         HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
     }
     HIP_TRY(launch_upload(up, s));
