@@ -2018,6 +2018,11 @@ struct pbs_server {
     size_t budget = 0, used = 0;  // admission budget and bytes charged (work buffers, doses, states)
     hipStream_t copyStream = nullptr;
     std::vector<hipEvent_t> freeEvents;
+    // dose buffers of fetched results, by size, for the next requests: a buffer returns here only
+    // after its result copy has completed on the host, so reuse needs no stream-ordered
+    // allocator; their bytes stay charged to the budget until freed
+    std::multimap<size_t, float *> idleDose;
+    size_t idleBytes = 0;
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv, doneCv;
@@ -2103,7 +2108,19 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
         const size_t need = work_bytes(plan, nSub);
         const size_t grow = need > sv->slotWork[size_t(slot)] ? need - sv->slotWork[size_t(slot)] : 0;
         const size_t doseB = dose_bytes(r.doseDims), stateB = r.keep ? state_bytes(plan, maxSub) : 0;
-        const size_t fp = doseB + stateB + grow;
+        auto idle = sv->idleDose.find(doseB);
+        size_t fp = (idle != sv->idleDose.end() ? 0 : doseB) + stateB + grow;
+        // idle dose buffers of other sizes give their memory back before the request waits for it
+        for (auto it = sv->idleDose.begin(); sv->used + fp > sv->budget && it != sv->idleDose.end();) {
+            if (it == idle) {
+                ++it;
+                continue;
+            }
+            (void)hipFree(it->second);
+            sv->used -= std::min(it->first, sv->used);
+            sv->idleBytes -= it->first;
+            it = sv->idleDose.erase(it);
+        }
         if (sv->used + fp > sv->budget) {
             bool anyRunning = false;
             for (long long t : sv->slotTicket) anyRunning |= t != 0;
@@ -2113,6 +2130,11 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
             return true;
         }
         sv->used += fp;
+        if (idle != sv->idleDose.end()) {
+            r.dose = idle->second;
+            sv->idleBytes -= idle->first;
+            sv->idleDose.erase(idle);
+        }
         sv->slotWork[size_t(slot)] += grow;  // work buffers stay with the slot
         r.doseCharge = doseB;
         r.stateCharge = stateB;
@@ -2129,7 +2151,8 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
         pbs_engine *e = sv->slots[size_t(slot)];
         int rc = PBS_OK;
         std::string err;
-        hipError_t he = hipMallocAsync(reinterpret_cast<void **>(&rp->dose), dose_bytes(rp->doseDims), e->stream);
+        hipError_t he = hipSuccess;
+        if (!rp->dose) he = hipMalloc(reinterpret_cast<void **>(&rp->dose), dose_bytes(rp->doseDims));
         if (he == hipSuccess) he = hipMemsetAsync(rp->dose, 0, dose_bytes(rp->doseDims), e->stream);
         if (he != hipSuccess) {
             rc = PBS_E_DEVICE;
@@ -2169,9 +2192,14 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
         if (rc != PBS_OK) {
             (void)hipStreamSynchronize(e->stream);
             sv->slotTicket[size_t(slot)] = 0;
-            if (rp->dose) (void)hipFreeAsync(rp->dose, e->stream);
+            if (rp->dose) {
+                sv->idleDose.emplace(rp->doseCharge, rp->dose);  // the stream is idle: reusable at once
+                sv->idleBytes += rp->doseCharge;
+            } else {
+                sv->used -= std::min(rp->doseCharge, sv->used);
+            }
             rp->dose = nullptr;
-            sv->used -= rp->doseCharge + rp->stateCharge;
+            sv->used -= std::min(rp->stateCharge, sv->used);
             rp->doseCharge = rp->stateCharge = 0;
             finish_error(sv, *rp, rc, err);
         } else if (rp->keep) {
@@ -2288,6 +2316,7 @@ void pbs_server_destroy(pbs_server *sv) {
         if (kv.second->done) (void)hipEventDestroy(kv.second->done);
     }
     for (hipEvent_t ev : sv->freeEvents) (void)hipEventDestroy(ev);
+    for (auto &kv : sv->idleDose) (void)hipFree(kv.second);
     sv->states.clear();
     for (pbs_engine *e : sv->slots) pbs_engine_destroy(e);
     if (sv->copyStream) (void)hipStreamDestroy(sv->copyStream);
@@ -2431,13 +2460,22 @@ static int server_result(pbs_server *sv, long long ticket, float *dst, hipMemcpy
         r.dose = nullptr;
         r.doseCharge = 0;
     }
-    HIP_TRY(hipMemcpyAsync(dst, dose, n * sizeof(float), kind, sv->copyStream));
-    HIP_TRY(hipFreeAsync(dose, sv->copyStream));
-    HIP_TRY(hipStreamSynchronize(sv->copyStream));
+    const hipError_t ce = hipMemcpyAsync(dst, dose, n * sizeof(float), kind, sv->copyStream);
+    const hipError_t se = hipStreamSynchronize(sv->copyStream);
     {
+        // the copy has completed: the buffer serves a later request of the same size; beyond an
+        // eighth of the budget of idle buffers it is freed
         std::lock_guard<std::mutex> g(sv->mu);
-        sv->used -= std::min(fp, sv->used);
+        if (sv->idleBytes + fp <= sv->budget / 8) {
+            sv->idleDose.emplace(fp, dose);
+            sv->idleBytes += fp;
+        } else {
+            (void)hipFree(dose);
+            sv->used -= std::min(fp, sv->used);
+        }
     }
+    HIP_TRY(ce);
+    HIP_TRY(se);
     sv->cv.notify_all();
     return PBS_OK;
 }

@@ -170,3 +170,56 @@ def test_serving_mix_requests_against_oracle(ri):
     err = float(np.abs(got.astype(np.float64) - ref).max() / ref.max())
     print(f"request {ri}: max|dD|/Dmax = {err:.3e} ({n_followed} rays on the entry threshold followed)")
     assert err <= 5e-3
+
+
+@gpu
+def test_serving_mix_device_results_every_pass():
+    """The whole serving mix (48 requests on three CT sizes, 9 resident beams, 11 weight-only
+    updates) through 4 slots, three times, with every result handed over on the device at once:
+    each equals the plain engine's dose (full requests bitwise; updates within one rounding of
+    IDD = W x unit IDD). Regression test for dose buffers recycled while a result copy of the
+    same memory could still be pending."""
+    from subsecondpbs_amd.engine import Engine
+    from subsecondpbs_amd.server import Server
+
+    cfg = F.make_config5(n_requests=48)
+    reqs, cts = cfg["requests"], cfg["cts"]
+    plain = []
+    engines = []
+    for c in cts:
+        e = Engine(cfg["energy"])
+        e.set_ct(c["ct"], c["ct_dims"], c["ct_resolution"], c["ct_corner"])
+        e.set_dose_grid(c["dose_dims"], c["dose_resolution"], c["dose_corner"])
+        engines.append(e)
+    for r in reqs:
+        nx, ny, nz = cts[r["ct"]]["dose_dims"]
+        d = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+        engines[r["ct"]].beam_dose(r["beam"], d)
+        engines[r["ct"]].sync()
+        plain.append(d)
+    for e in engines:
+        e.close()
+    kept = {r["update_of"] for r in reqs if r["update_of"] >= 0}
+    sv = Server(cfg["energy"], slots=4)
+    cids = [sv.add_ct(c["ct"], c["ct_dims"], c["ct_resolution"], c["ct_corner"]) for c in cts]
+    for rep in range(3):
+        handles, tickets = {}, []
+        for i, r in enumerate(reqs):
+            c = cts[r["ct"]]
+            if r["update_of"] >= 0:
+                tickets.append(sv.submit_weights(handles[r["update_of"]], r["beam"]["spot_weights"]))
+            else:
+                t = sv.submit(cids[r["ct"]], *_grid(c), r["beam"], keep=i in kept)
+                tickets.append(t)
+                if i in kept:
+                    handles[i] = t
+        res = [sv.result(t, device=True) for t in tickets]
+        for h in handles.values():
+            sv.release(h)
+        for i, (a, b) in enumerate(zip(res, plain)):
+            err = float((a - b).abs().max().item()) / float(b.abs().max().item())
+            if reqs[i]["update_of"] < 0 and i not in kept:
+                assert err == 0.0, f"pass {rep} request {i}: {err:.3e}"
+            else:
+                assert err < 1e-5, f"pass {rep} request {i} (update of {reqs[i]['update_of']}): {err:.3e}"
+    sv.close()
