@@ -12,8 +12,10 @@ timing (as a C or C++ caller holds it); each step plans, uploads and computes th
 Prints one JSON line. Extra fields: p50/p99 ms per beam, the max dose error of one beam
 against the FP64 oracle, per-stage device times (HIP events on the engine stream, from a
 pass of their own after the timed region), the ms per beam when every call also converts
-the Python beam dict, the roofline of the dominant stage and the CPU baseline (the FP64
-oracle on the host cores).
+the Python beam dict, the roofline of the dominant stage and of each pass, the served and
+batched forms against the plain form on their own workloads (the serving mix with open-loop
+arrivals at 20/50/100/200 requests/s, and the 16-beam proton/carbon batch), and the CPU
+baseline (the FP64 oracle on the host cores: all threads and one thread).
 """
 from __future__ import annotations
 
@@ -31,7 +33,9 @@ if ROOT not in sys.path:
 
 METRIC = "beams/sec & p50/p99 ms per beam at ≤0.5% Dmax err vs FP64; served vs plain"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3           # MI355X dense FP32 vector peak (packed FP32 instructions)
+FP32_UNPACKED_PEAK_TFLOPS = 78.6   # the same without packed FP32, which this build does not emit
+OPEN_LOOP_RATES = (20.0, 50.0, 100.0, 200.0)
 
 
 def algorithmic_work(dbg: dict, case: dict) -> dict:
@@ -92,31 +96,70 @@ def measured_traffic() -> dict | None:
     path = os.path.join(ROOT, "profiles", "head_beam_hbm_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    # per-stage sums of the kernels' scaled read + write bytes
+    stage_of = (("k_trace", "trace"), ("k_fill", "fill"), ("k_transfer", "transfer"), ("k_merge", "merge"),
+                ("k_superpose", "superpose"))
+    sums = {}
+    for k, v in d.get("per_kernel_bytes", {}).items():
+        name = k.split("<")[0].split()[-1]
+        for prefix, stage in stage_of:
+            if name.startswith(prefix):
+                sums[stage] = sums.get(stage, 0.0) + v["read_bytes"] + v["write_bytes"]
+    for stage, b in sums.items():
+        d.setdefault(f"{stage}_stage_bytes", b)
+    return d
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def cpu_baseline(case: dict, runs: int) -> dict:
+    """The FP64 oracle on the host: one warm-up, then the median of ``runs`` beams on all threads,
+    and the median of 3 on one thread."""
     import oracle
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    ts = []
-    for _ in range(runs):
-        t0 = time.perf_counter()
-        oracle.beam_dose(case)
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
+
+    def timed(n):
+        oracle.set_threads(n)
+        out = []
+        for _ in range(n_runs[n]):
+            t0 = time.perf_counter()
+            oracle.beam_dose(case)
+            out.append(time.perf_counter() - t0)
+        return float(np.median(out))
+
+    n_runs = {threads: runs, 1: 3}
+    oracle.set_threads(threads)
+    oracle.beam_dose(case)  # warm-up
+    med = timed(threads)
+    one = timed(1) if threads != 1 else med
+    oracle.set_threads(threads)
     return {"value": 1.0 / med, "unit": "beams/s", "cores": threads, "kind": "port",
-            "sample": f"{runs} full head-size beams (30 layers, 3270 spots) through the FP64 oracle, "
-                      f"median {med:.2f} s per beam"}
+            "sample": f"full head-size beams (30 layers, 3270 spots) through the FP64 oracle: median of {runs} "
+                      f"on {threads} threads ({med:.2f} s per beam) after one warm-up; median of 3 on 1 thread",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "single_thread": {"value": 1.0 / one, "unit": "beams/s", "s_per_beam": one}}
 
 
-def served_mode(args) -> int:
+def served_summary(args, rates=OPEN_LOOP_RATES) -> dict:
     """Served vs plain on the serving mix (config 5): the same request list through the plain
     engine one request at a time (one engine per CT, doses left on the device) and through the
     request server (all requests submitted at once; resident beams for the ones that later get
-    weight-only updates), then an open-loop Poisson stream for latency percentiles."""
+    weight-only updates), then open-loop Poisson streams at each of ``rates`` (requests/s) for
+    latency percentiles."""
     import torch
 
     from subsecondpbs_amd import fixtures
@@ -193,59 +236,59 @@ def served_mode(args) -> int:
               for a, b in zip(served_res, plain_res))
     del served_res, plain_res
 
-    # open-loop Poisson arrivals at `rate` requests/s
-    rng = np.random.default_rng(5)
-    gaps = rng.exponential(1.0 / args.rate, len(reqs))
-    handles = {}
-    t0 = time.perf_counter()
-    tickets = []
-    for i in range(len(reqs)):
-        due = t0 + float(np.sum(gaps[: i + 1]))
-        while time.perf_counter() < due:
-            time.sleep(min(1e-4, max(0.0, due - time.perf_counter())))
-        tickets.append(submit(i, handles))
-    for t in tickets:
-        sv.result(t, device=True)
-    open_lat = [sv.timing(t)[1] for t in tickets]
-    for h in handles.values():
-        sv.release(h)
+    # open-loop Poisson arrivals at each rate (requests/s)
+    open_loop = {}
+    for rate in rates:
+        rng = np.random.default_rng(5)
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        handles = {}
+        t0 = time.perf_counter()
+        tickets = []
+        for i in range(len(reqs)):
+            due = t0 + float(np.sum(gaps[: i + 1]))
+            while time.perf_counter() < due:
+                time.sleep(min(1e-4, max(0.0, due - time.perf_counter())))
+            tickets.append(submit(i, handles))
+        for t in tickets:
+            sv.result(t, device=True)
+        lat_o = [sv.timing(t)[1] for t in tickets]
+        for h in handles.values():
+            sv.release(h)
+        open_loop[f"{rate:g}"] = {"p50_ms": float(np.percentile(lat_o, 50)), "p99_ms": float(np.percentile(lat_o, 99))}
     used, budget = sv.memory()
     sv.close()
 
     n = len(reqs)
     plain_bps, served_bps = n / plain_s, n / served_s
-    out = {
-        "metric": METRIC,
-        "value": served_bps,
-        "unit": "beams/s",
-        "n_gpus": 1,
-        "steps": n,
-        "warmup": n,
-        "ms_per_step": 1e3 * served_s / n,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": f"serving mix: {n} requests over CTs 128^3, 256x256x160, 512x512x200; "
-                               "500-8000 spots, 10-40 layers, random gantry; "
-                               f"{len(kept)} resident beams, {sum(r['update_of'] >= 0 for r in reqs)} weight-only updates",
-                   "form": "served", "slots": args.slots},
+    return {
+        "workload": f"serving mix: {n} requests over CTs 128^3, 256x256x160, 512x512x200; 500-8000 spots, "
+                    f"10-40 layers, random gantry; {len(kept)} resident beams, "
+                    f"{sum(r['update_of'] >= 0 for r in reqs)} weight-only updates; {args.slots} server slots",
+        "requests": n,
+        "served_beams_per_s": served_bps,
         "plain_beams_per_s": plain_bps,
         "served_over_plain": served_bps / plain_bps,
         "plain_p50_ms": float(np.percentile(plain_lat, 50)),
         "plain_p99_ms": float(np.percentile(plain_lat, 99)),
         "served_closed_loop_p50_ms": float(np.percentile(served_lat, 50)),
         "served_closed_loop_p99_ms": float(np.percentile(served_lat, 99)),
-        "open_loop_rate_per_s": args.rate,
-        "open_loop_p50_ms": float(np.percentile(open_lat, 50)),
-        "open_loop_p99_ms": float(np.percentile(open_lat, 99)),
+        "open_loop_ms_by_rate_per_s": open_loop,
         "max_dose_err_served_vs_plain": err,
         "plain_gpu_stage_ms_total": gpu_ms,
         "plain_gpu_stage_ms": {k: round(v, 3) for k, v in stage_sum.items()},
         "served_gpu_busy_fraction_upper_bound": gpu_ms / (1e3 * served_s),
         "server_memory_charged_gb": used / 1e9,
     }
+
+
+def served_mode(args) -> int:
+    sm = served_summary(args, rates=(args.rate,) if args.rate else OPEN_LOOP_RATES)
+    n = sm["requests"]
+    out = {"metric": METRIC, "value": sm["served_beams_per_s"], "unit": "beams/s", "n_gpus": 1, "steps": n,
+           "warmup": n, "ms_per_step": 1e3 / sm["served_beams_per_s"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": sm["workload"], "form": "served", "slots": args.slots}}
+    out.update(sm)
     print(json.dumps(out))
     return 0
 
@@ -290,7 +333,7 @@ def plan_4field(eng, reps: int = 10) -> dict:
     return out
 
 
-def batch_mode(args) -> int:
+def batch_summary(args) -> dict:
     """Batching case (config 3): 12 proton + 4 carbon beams of uneven size on the config-1 CT.
     Plain: one engine per ion, one beam at a time (queue, wait). Batched: one 4-lane Batch per ion,
     all beams queued, one wait. Both with results left on the device."""
@@ -364,19 +407,28 @@ def batch_mode(args) -> int:
     for _, e, bt in sets:
         e.close()
         bt.close()
-    out = {
-        "metric": METRIC, "value": n / bm, "unit": "beams/s", "n_gpus": 1, "steps": reps * n, "warmup": n,
-        "ms_per_step": 1e3 * bm / n, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "batching case: 12 proton beams (500-8000 spots, 10-40 layers, random gantry) + 4 "
-                               "carbon beams (20 layers, project-defined 12C tables with the halo branch) on the "
-                               "64^3 water/slab CT", "form": "batched", "lanes": args.slots},
+    return {
+        "workload": "batching case: 12 proton beams (500-8000 spots, 10-40 layers, random gantry) + 4 carbon "
+                    "beams (20 layers, project-defined 12C tables with the halo branch) on the 64^3 water/slab CT; "
+                    f"{args.slots} lanes",
+        "beams": n, "passes": reps,
+        "batched_beams_per_s": n / bm,
         "plain_beams_per_s": n / pm, "batched_over_plain": pm / bm,
         "plain_p50_ms_per_beam": float(np.percentile(lat, 50)), "plain_p99_ms_per_beam": float(np.percentile(lat, 99)),
         "max_dose_err_batched_vs_plain": err,
         "plain_gpu_stage_ms_per_pass": {k: round(v, 4) for k, v in stage_sum.items()},
         "plain_gpu_busy_fraction": sum(stage_sum.values()) / (1e3 * pm),
     }
+
+
+def batch_mode(args) -> int:
+    bm = batch_summary(args)
+    n = bm["beams"]
+    out = {"metric": METRIC, "value": bm["batched_beams_per_s"], "unit": "beams/s", "n_gpus": 1,
+           "steps": bm["passes"] * n, "warmup": n, "ms_per_step": 1e3 / bm["batched_beams_per_s"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": bm["workload"], "form": "batched", "lanes": args.slots}}
+    out.update(bm)
     print(json.dumps(out))
     return 0
 
@@ -389,14 +441,17 @@ def main() -> int:
                          "spot-weight optimisation loop; batch: the 16-beam mixed proton/carbon batch")
     ap.add_argument("--requests", type=int, default=48, help="served mode: requests in the mix")
     ap.add_argument("--slots", type=int, default=4, help="served mode: engine slots of the server")
-    ap.add_argument("--rate", type=float, default=100.0, help="served mode: open-loop arrival rate (req/s)")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="served mode: one open-loop arrival rate (req/s); default: 20, 50, 100 and 200")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plan4", action="store_true", help="skip the 4-field plan timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-served", action="store_true", help="plain mode: skip the served-vs-plain summary")
+    ap.add_argument("--no-batched", action="store_true", help="plain mode: skip the batched-vs-plain summary")
     ap.add_argument("--tuning", default="", help="fixed launch shapes instead of the autotuner, e.g. "
                                                  "superpose_grid=2048,transfer_shape=3,one_stream=0")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -498,24 +553,28 @@ def main() -> int:
     per_beam = {k: v / max(nb, 1) for k, v in stages.items()}
     plan4 = plan_4field(eng) if not args.no_plan4 else None
 
-    # roofline of the dominant stage
+    # roofline of every pass with an algorithmic measure, and of the dominant one
+    traffic = measured_traffic() or {}
+    passes = {}
+    for name, t_ms in per_beam.items():
+        if name == "superpose":
+            ach = work["superpose"]["flops"] / (t_ms * 1e-3) / 1e12
+            passes[name] = {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": ach / FP32_PEAK_TFLOPS, "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
+                            "frac_unpacked": ach / FP32_UNPACKED_PEAK_TFLOPS,
+                            "traffic": traffic.get("superpose_stage_bytes"),
+                            "kernel": "superposition stage: k_superpose_rows<7/11/15> and k_superpose<24/32>, 5 "
+                                      "radius classes on 4 streams (stage wall time from HIP events)",
+                            "compute": "fp32 VALU FMA (no matrix cores: no dense contraction); peak = the dense "
+                                       "fp32 figure, peak_unpacked = without packed fp32", "stage_ms": t_ms}
+        elif work.get(name, {}).get("bytes"):
+            ach = work[name]["bytes"] / (t_ms * 1e-3) / 1e9
+            passes[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(f"{name}_stage_bytes"),
+                            "algorithmic_bytes": work[name]["bytes"], "stage_ms": t_ms}
     dom = max(per_beam, key=per_beam.get)
-    t_dom = per_beam[dom] * 1e-3
-    traffic = measured_traffic()
-    if dom == "superpose":
-        ach = work["superpose"]["flops"] / t_dom / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP32_PEAK_TFLOPS,
-                "traffic": traffic.get("superpose_stage_bytes") if traffic else None,
-                "kernel": "superposition stage: k_superpose_rows<7/11/15> and k_superpose<24/32>, 5 radius "
-                          "classes on 4 streams (stage wall time from HIP events)",
-                "compute": "fp32 VALU FMA (no matrix cores; peak is the fp32 dense figure)"}
-    else:
-        b = work.get(dom, {}).get("bytes", 0.0)
-        ach = b / t_dom / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": dom}
-    roof["stage_ms"] = per_beam[dom]
+    roof = dict(passes.get(dom) or {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": None, "traffic": None, "kernel": dom, "stage_ms": per_beam[dom]})
 
     out = {
         "metric": METRIC,
@@ -541,11 +600,20 @@ def main() -> int:
         "algorithmic": {"superpose_gflop": work["superpose"]["flops"] / 1e9,
                         "fill_mb": work["fill"]["bytes"] / 1e6, "trace_mb": work["trace"]["bytes"] / 1e6},
         "roofline": roof,
+        "roofline_passes": passes,
     }
     if plan4 is not None:
         out["plan_4field_ms"] = plan4
     if tuning is not None:
         out["autotune"] = tuning
+    if not args.no_served:
+        sm = served_summary(args)
+        out["served_over_plain"] = sm["served_over_plain"]
+        out["served"] = sm
+    if not args.no_batched:
+        bm = batch_summary(args)
+        out["batched_over_plain"] = bm["batched_over_plain"]
+        out["batched"] = bm
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(case, max(args.cpu_runs, 1))
     else:

@@ -103,6 +103,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_ct_adapter.restype = C.c_int
         L.orc_ct_adapter.argtypes = [_f, C.c_size_t, C.c_int, C.POINTER(Energy), _f]
         L.orc_sizeof.restype = C.c_long
@@ -352,6 +354,11 @@ def beam_adjoint(case: dict, g_dose: np.ndarray, beam_index: int = 0, use_subspo
     if rc != 0:
         raise RuntimeError("oracle: " + p.error.decode())
     return out
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's following calls."""
+    lib().orc_set_threads(int(n))
 
 
 def ct_as_hu_plus_1000(ct: np.ndarray, energy: dict | None = None, forced_type: int = -1) -> np.ndarray:

@@ -8,6 +8,9 @@
  * dose arithmetic is evaluated in double. Citations are to files under the reference
  * repository (src/..., include/...).
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "pbs_oracle.h"
 
 #include <float.h>
@@ -464,6 +467,14 @@ int orc_ct_adapter(const float *in, size_t n, int forcedType, const orc_energy *
     if (!lut || nl <= 0 || scale == 0.0f) return -1;
     for (size_t i = 0; i < n; ++i) out[i] = frac_index_monotonic(lut, nl, in[i]) / scale;
     return type;
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }
 
 long orc_sizeof(int which) {

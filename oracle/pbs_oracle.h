@@ -12,8 +12,7 @@
  * integers it produces (grid sizes, corners, step ranges) are defined by that float
  * arithmetic. Every per-ray, per-step, per-pixel and per-voxel quantity (trilinear CT
  * samples, WEPL, convolution taps, IDD, sigma, superposition, BEV->dose transfer) is
- * evaluated in double. The build flag -DORC_REAL=float gives an FP32 port with the same
- * algorithm, used only as a timed CPU baseline.
+ * evaluated in double. bench.py times this library on the host cores as the CPU baseline.
  *
  * Parity pinning (which check pins which function) is listed next to each declaration.
  */
@@ -133,6 +132,8 @@ typedef struct {
 } orc_plan;
 
 long orc_sizeof(int which);  /* 0 orc_beam, 1 orc_plan, 2 orc_energy */
+/* OpenMP threads of the following calls (the CPU baseline's 1-thread and all-core timings). */
+void orc_set_threads(int n);
 
 /* ---- helpers exposed for known-answer tests ---- */
 /* findDecimalOrdered (include/utils/vector_find.h:130-145). Pinned: KAT peak depths. */
