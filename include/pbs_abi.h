@@ -70,6 +70,24 @@ typedef struct pbs_beam_settings {
     int spotPositionsAreIndices;  /* 1: spotPositions are WEQ-lattice texel coordinates (+0.5 at
                                      texel centres), rasterised bilinearly onto that lattice
                                      (wrapper:931-968, 1295-1364); needs the header */
+    /* CarbonPBS lateral-profile model (RTDBeamSettings::profileEnergies, profileData,
+     * profileSetting, beamParaData, beamParaPos; raytracedicom_integration.h:53-58) and its two
+     * hooks on each layer's IDD / sigma field (wrapper:2080-2188, 4604-4832). profileData is
+     * [rows][depth][channels] (channels = nGauss weights, nGauss sigmas, 1 overall weight),
+     * profileSetting = [depth0, depthStep, depthN], beamParaData = [rows][3]. The reference
+     * compiles both hooks but switches them off (wrapper:2812-2814); carbonProfileHooks turns
+     * them on: bit 0 = sigma from the profile model, bit 1 = the overall-weight channel on the
+     * IDD. 0 (the default) keeps the reference's behaviour and ignores the arrays. */
+    const float *profileEnergies;
+    int numProfileEnergies;
+    const float *profileData;
+    size_t profileDataCount;
+    const float *profileSetting;
+    int numProfileSetting;
+    const float *beamParaData;
+    int numBeamParaData;
+    float beamParaPos;
+    int carbonProfileHooks;
 } pbs_beam_settings;
 
 /* Energy/LUT payload: POD mirror of RTDEnergyStruct (raytracedicom_integration.h:97-128). */
@@ -230,14 +248,18 @@ typedef struct pbs_plan_info {
     float fanCorner[3], fanDelta[3], sourceDist[2];
     float volConst, volLin, volSq;
     float lenToMm, energyDepthToMm; /* input length unit and LUT depth unit, in mm */
+    int profileHooks;               /* hooks that apply (bit 0 sigma, bit 1 weight) */
+    int profileRows, profileDepthN, profileChannels;
 } pbs_plan_info;
 /* layerEnergyIdx / layerPeakDepth [numLayers], denseWeights [numLayers][spotDims1][spotDims0]
- * (denseCap floats available) and spotCells [numSpots] (the dense cell of each spot, -1 off
- * the lattice) may be NULL. */
+ * (denseCap floats available), spotCells [numSpots] (the dense cell of each spot, -1 off
+ * the lattice) and layerProfile [numLayers][4] (profile row index and interpolated beam
+ * parameters of each layer) may be NULL. */
 int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
                    const float doseRes[3], const float doseCorner[3], const pbs_beam_settings *beam,
                    const pbs_energy_struct *energy, pbs_plan_info *info, float *layerEnergyIdx,
-                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells);
+                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells,
+                   float *layerProfile);
 /* The engine's CT adapter on a host buffer, without a device (no reference counterpart: it exposes
  * what pbs_engine_set_ct runs first, for tests on a machine without a GPU). The input type is
  * RTD_CT_INPUT_TYPE's, else detected from sampled values (raytracedicom_wrapper.cu:2288-2335);

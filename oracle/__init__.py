@@ -49,6 +49,10 @@ class Beam(C.Structure):
         ("cpbOverride", C.c_int), ("cpbOvCorner", C.c_float * 2), ("cpbOvRes", C.c_float * 2),
         ("cpbOvDims", C.c_int * 2),
         ("energyDepthMode", C.c_int), ("energyDepthToMm", C.c_float), ("cpbCoverage", C.c_float),
+        ("profileEnergies", _f), ("numProfileEnergies", C.c_int), ("profileData", _f),
+        ("profileDataCount", C.c_size_t), ("profileSetting", _f), ("numProfileSetting", C.c_int),
+        ("beamParaData", _f), ("numBeamParaData", C.c_int), ("beamParaPos", C.c_float),
+        ("carbonProfileHooks", C.c_int),
     ]
 
 
@@ -84,6 +88,9 @@ class Plan(C.Structure):
         ("weqStepMm", C.c_float), ("weqX0", C.c_float), ("weqY0", C.c_float), ("weqDx", C.c_float),
         ("weqDy", C.c_float), ("weqBody", _f),
         ("nucRaw", C.c_int * 2), ("nucFallback", C.c_int), ("nucOffset", C.c_float * 2), ("nucDelta", C.c_float * 2),
+        ("profileHooks", C.c_int), ("profileRows", C.c_int), ("profileDepthN", C.c_int),
+        ("profileChannels", C.c_int), ("profileDepth0", C.c_float), ("profileDepthStep", C.c_float),
+        ("beamParaPos", C.c_float), ("profileLayer", (C.c_float * 4) * MAX_LAYERS), ("profileTable", _f),
     ]
 
 
@@ -103,6 +110,8 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             build()
         L = C.CDLL(_LIB_PATH)
+        L.orc_profile_hooks.restype = None
+        L.orc_profile_hooks.argtypes = [C.POINTER(Plan), C.c_int, _d, _d, _d]
         L.orc_set_threads.restype = None
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_ct_adapter.restype = C.c_int
@@ -246,6 +255,18 @@ def make_beam(beam: dict, keep: _Keep, use_subspots: bool = False) -> Beam:
     b.energyDepthMode = int(beam.get("energy_depth_mode", 0))
     b.energyDepthToMm = float(beam.get("energy_depth_to_mm", 0.0))
     b.cpbCoverage = float(beam.get("cpb_coverage", 0.0))
+    for key, ptr, cnt in (("profile_energies", "profileEnergies", "numProfileEnergies"),
+                          ("profile_data", "profileData", "profileDataCount"),
+                          ("profile_setting", "profileSetting", "numProfileSetting"),
+                          ("beam_para_data", "beamParaData", "numBeamParaData")):
+        v = beam.get(key)
+        if v is not None:
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).ravel())
+            setattr(b, ptr, keep.f32(a))
+            setattr(b, cnt, int(a.size))
+    if beam.get("beam_para_pos") is not None:
+        b.beamParaPos = float(beam["beam_para_pos"])
+    b.carbonProfileHooks = int(beam.get("carbon_profile_hooks", 0))
     ov = beam.get("cpb_override")  # (corner[2], res[2], dims[2]) in the gantry frame
     if ov is not None:
         b.cpbOverride = 1

@@ -477,6 +477,135 @@ void orc_set_threads(int n) {
 #endif
 }
 
+static float layer_energy_index(const orc_energy *en, float energy);
+
+/* The profile model of a beam and the hooks that apply (wrapper:2816-2862 table shape; 4577-4603
+ * per-layer row and beam parameters; 4656-4660 and 4814-4819 the hooks' conditions). */
+static void plan_profile(const orc_beam *b, const orc_energy *en, orc_plan *p) {
+    p->profileHooks = 0;
+    const int want = b->carbonProfileHooks & 3;
+    const int L = b->numLayers;
+    const size_t size = b->profileData ? b->profileDataCount : 0;
+    if (!want || !b->profileSetting || b->numProfileSetting < 3 || lroundf(b->profileSetting[2]) <= 0 || size == 0)
+        return;
+    const int depthN = lroundf(b->profileSetting[2]) > 1 ? (int)lroundf(b->profileSetting[2]) : 1;
+    const size_t nPara = b->beamParaData ? (size_t)(b->numBeamParaData > 0 ? b->numBeamParaData : 0) : 0;
+    const size_t nEn = b->profileEnergies ? (size_t)(b->numProfileEnergies > 0 ? b->numProfileEnergies : 0) : 0;
+    const int paraRows = (int)(nPara / 3);
+    int rows = 0;
+    if (paraRows > 0 && nPara == (size_t)paraRows * 3 && size % ((size_t)paraRows * depthN) == 0) rows = paraRows;
+    else if (nEn > 0 && size % (nEn * depthN) == 0) rows = (int)nEn;
+    else if (L > 0 && size % ((size_t)L * depthN) == 0) rows = L;
+    else if (size % depthN == 0 && (size / depthN) % 11 == 0) rows = (int)(size / depthN / 11);
+    if (rows <= 0) return;
+    const size_t perRow = size / (size_t)rows;
+    if (perRow < (size_t)depthN) return;
+    const int channels = (int)(perRow / (size_t)depthN);
+    if (channels <= 0 || (size_t)rows * depthN * channels != size) return;
+    int hooks = 0;
+    if ((want & 1) && nPara >= (size_t)L * 3 && channels >= 3) hooks |= 1;
+    if (want & 2) hooks |= 2;
+    if (!hooks) return;
+    p->profileHooks = hooks;
+    p->profileRows = rows;
+    p->profileDepthN = depthN;
+    p->profileChannels = channels;
+    p->profileDepth0 = b->profileSetting[0];
+    p->profileDepthStep = b->profileSetting[1];
+    p->beamParaPos = b->beamParaPos;
+    p->profileTable = b->profileData;
+    const float rowMax = (float)(rows > 1 ? rows - 1 : 0);
+    for (int l = 0; l < L && l < ORC_MAX_LAYERS; ++l) {
+        float row;
+        if (rows == L) row = (float)l;
+        else if (nEn > 0 && rows == (int)nEn) row = frac_index_monotonic(b->profileEnergies, (int)nEn, b->energies[l]);
+        else if (rows == en->nEnergies && en->nEnergies > 0) {
+            row = layer_energy_index(en, b->energies[l]);
+            row = fminf(fmaxf(row, 0.0f), (float)(rows - 1));
+        } else row = fminf(fmaxf((float)l, 0.0f), rowMax);
+        row = fminf(fmaxf(row, 0.0f), rowMax);
+        p->profileLayer[l][0] = row;
+        for (int c = 0; c < 3; ++c) {
+            if (nPara < (size_t)rows * 3) { p->profileLayer[l][1 + c] = 0.0f; continue; }
+            const float cl = fminf(fmaxf(row, 0.0f), (float)(rows - 1));
+            int i0 = (int)floorf(cl);
+            i0 = i0 < 0 ? 0 : (i0 > rows - 1 ? rows - 1 : i0);
+            const int i1 = i0 + 1 > rows - 1 ? rows - 1 : i0 + 1;
+            const float t = cl - (float)i0;
+            const float v0 = b->beamParaData[(size_t)i0 * 3 + c], v1 = b->beamParaData[(size_t)i1 * 3 + c];
+            p->profileLayer[l][1 + c] = v0 + (v1 - v0) * t;
+        }
+    }
+}
+
+static double profile_at(const orc_plan *p, int c, double d, double r) {
+    const int d0 = (int)floor(d), r0 = (int)floor(r);
+    const int d1 = d0 + 1 < p->profileDepthN ? d0 + 1 : p->profileDepthN - 1;
+    const int r1 = r0 + 1 < p->profileRows ? r0 + 1 : p->profileRows - 1;
+    const double fd = d - d0, fr = r - r0;
+    const int ch = p->profileChannels, dn = p->profileDepthN;
+    const float *t = p->profileTable;
+#define PT(rr, dd) ((double)t[((size_t)(rr) * dn + (dd)) * ch + c])
+    const double v0 = PT(r0, d0) + (PT(r0, d1) - PT(r0, d0)) * fd;
+    const double v1 = PT(r1, d0) + (PT(r1, d1) - PT(r1, d0)) * fd;
+#undef PT
+    return v0 + (v1 - v0) * fr;
+}
+
+void orc_profile_hooks(const orc_plan *p, int l, const double *wepl, double *idd, double *rsigma) {
+    if (!p->profileHooks) return;
+    const int rx = p->rayDims[0], ry = p->rayDims[1];
+    const size_t plane = (size_t)rx * ry;
+    const int k0 = p->beamFirstInside > 0 ? p->beamFirstInside : 0;
+    const int k1 = p->afterLast[l] < p->steps ? p->afterLast[l] : p->steps;
+    const int nG = (p->profileChannels - 1) / 2;
+    const double row = p->profileLayer[l][0];
+    const double sad = 0.5 * ((double)p->sourceDist[0] + (double)p->sourceDist[1]);
+    const double dx = p->sourceDist[0], dy = p->sourceDist[1];
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < ry; ++y)
+        for (int x = 0; x < rx; ++x) {
+            const double fanX = (double)p->fanCorner[0] + x * (double)p->fanDelta[0];
+            const double fanY = (double)p->fanCorner[1] + y * (double)p->fanDelta[1];
+            const double slant = sqrt(1.0 + fanX * fanX / (dx * dx) + fanY * fanY / (dy * dy));
+            for (int k = k0; k < k1; ++k) {
+                const size_t i = (size_t)k * plane + (size_t)y * rx + x;
+                if (!(idd[i] > 0.0)) {
+                    if (p->profileHooks & 1) rsigma[i] = INFINITY;
+                    continue;
+                }
+                double d = (wepl[i] - p->profileDepth0) / p->profileDepthStep;
+                d = d < 0.0 ? 0.0 : (d > p->profileDepthN - 1 ? p->profileDepthN - 1 : d);
+                if (p->profileHooks & 1) {
+                    double sw = 0.0, s2 = 0.0;
+                    for (int q = 0; q < nG; ++q) {
+                        const double wq = profile_at(p, q, d, row), sq = profile_at(p, q + nG, d, row);
+                        sw += wq;
+                        s2 += wq * sq * sq;
+                    }
+                    if (!(sw > 1.0e-8)) {
+                        rsigma[i] = INFINITY;
+                    } else {
+                        /* geometric depth along the divergent ray from the virtual source */
+                        const double fanZ = (double)p->fanCorner[2] + k * (double)p->fanDelta[2];
+                        const double phy = (sad - fanZ) * slant - (sad - p->beamParaPos);
+                        const double r2 = p->profileLayer[l][1] + 2.0 * p->profileLayer[l][2] * phy +
+                                          p->profileLayer[l][3] * phy * phy;
+                        const double a2 = 0.5 * fmax(0.0, r2 + s2 / sw);
+                        const double axis = sqrt(a2);
+                        const double width = 0.5 * ((double)p->fanDelta[0] * (1.0 - fanZ / dx) +
+                                                    (double)p->fanDelta[1] * (1.0 - fanZ / dy));
+                        rsigma[i] = 0.5 * width / (sqrt(2.0) * (axis + 0.21));
+                    }
+                }
+                if (p->profileHooks & 2) {
+                    const double w = profile_at(p, p->profileChannels - 1, d, row);
+                    idd[i] *= w > 0.0 ? w : 0.0;
+                }
+            }
+        }
+}
+
 long orc_sizeof(int which) {
     return which == 0 ? (long)sizeof(orc_beam) : which == 1 ? (long)sizeof(orc_plan)
                       : which == 2 ? (long)sizeof(orc_energy) : -1;
@@ -691,6 +820,7 @@ int orc_plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorne
         p->volLin = dxyz * dZ * (-1.0f / dsx - 1.0f / dsy + 2.0f * cz / (dsx * dsy));
         p->volSq = dxyz * dZ * dZ / (dsx * dsy);
     }
+    plan_profile(b, en, p);
     p->ok = 1;
     return 0;
 }
@@ -1613,6 +1743,8 @@ int orc_beam_dose_entry(const float *ct, const int ctDims[3], const float ctRes[
         } else {
             orc_fill(p, l, en, density, wepl, rw + (size_t)l * rplane, fi, fo, idd, rs, fp);
         }
+        /* the CarbonPBS profile hooks rewrite the primary field (wrapper:4604-4832) */
+        orc_profile_hooks(p, l, wepl, idd, rs);
         int mxp = INT32_MIN;
         for (size_t r = 0; r < rplane; ++r) if (fp[r] > mxp) mxp = fp[r];
         /* wrapper:4984-4996 */

@@ -77,6 +77,18 @@ typedef struct {
     int energyDepthMode;
     float energyDepthToMm;
     float cpbCoverage;
+    /* CarbonPBS lateral-profile model and its hooks (bit 0 sigma, bit 1 overall weight; the
+     * reference has both off, wrapper:2812-2814) */
+    const float *profileEnergies;
+    int numProfileEnergies;
+    const float *profileData;   /* [rows][depthN][channels] */
+    size_t profileDataCount;
+    const float *profileSetting;  /* depth0, depthStep, depthN */
+    int numProfileSetting;
+    const float *beamParaData;  /* [rows][3] */
+    int numBeamParaData;
+    float beamParaPos;
+    int carbonProfileHooks;
 } orc_beam;
 
 /* Beam plan: everything the host computes before and between kernels. */
@@ -129,6 +141,12 @@ typedef struct {
      * view of index-coded spots otherwise (wrapper:1478-1820, 3421-3437) */
     int nucRaw[2], nucFallback;
     float nucOffset[2], nucDelta[2];
+    /* profile hooks that apply (wrapper:2816-2862, 4577-4603): table shape and depth axis, per
+     * layer the fractional table row and the beam parameters interpolated there */
+    int profileHooks, profileRows, profileDepthN, profileChannels;
+    float profileDepth0, profileDepthStep, beamParaPos;
+    float profileLayer[ORC_MAX_LAYERS][4];
+    const float *profileTable;
 } orc_plan;
 
 long orc_sizeof(int which);  /* 0 orc_beam, 1 orc_plan, 2 orc_energy */
@@ -207,6 +225,15 @@ void orc_layer_params(orc_plan *plan, const orc_beam *beam, const orc_energy *en
 void orc_fill(const orc_plan *plan, int layer, const orc_energy *en, const double *density,
               const double *wepl, const double *rayWeights, const int *firstInside,
               const int *firstOutside, double *idd, double *rsigma, int *firstPassive);
+
+/* CarbonPBS profile hooks on layer l's field after the fill (overrideRSigmaFromCarbonProfileKernel,
+ * applyCarbonProfileOverallWeightKernel; wrapper:2080-2188, 4604-4832), steps [beamFirstInside,
+ * afterLast) of the ray grid ([K][ry][rx]): sigma from the profile model where IDD > 0 (+inf
+ * elsewhere), then IDD times the overall-weight channel. The table is read by exact bilinear
+ * interpolation over depth and row (the reference's texture unit uses 8-bit weights: parity
+ * unpinned there). Pinned: closed forms of a one-Gaussian table (sigma override, divergence
+ * width, physical-depth polynomial) and of a depth-linear overall weight. */
+void orc_profile_hooks(const orc_plan *plan, int layer, const double *wepl, double *idd, double *rsigma);
 
 /* Halo lattice plan (wrapper:804-929; spot spacing wrapper:4062-4100, 4149-4172, 4338-4350):
  * plan->nucDims = spot lattice dims rounded up to 32x8, plan->spotDist = spot spacing / fan

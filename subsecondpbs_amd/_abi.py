@@ -34,6 +34,10 @@ class BeamSettings(C.Structure):
         ("roiLinearIndices", _i), ("numRoiLinearIndices", C.c_int),
         ("useSubspotConvolution", C.c_int),
         ("waterEquivalence", _f), ("waterEquivalenceCount", C.c_size_t), ("spotPositionsAreIndices", C.c_int),
+        ("profileEnergies", _f), ("numProfileEnergies", C.c_int), ("profileData", _f),
+        ("profileDataCount", C.c_size_t), ("profileSetting", _f), ("numProfileSetting", C.c_int),
+        ("beamParaData", _f), ("numBeamParaData", C.c_int), ("beamParaPos", C.c_float),
+        ("carbonProfileHooks", C.c_int),
     ]
 
 
@@ -76,6 +80,8 @@ class PlanInfo(C.Structure):
         ("cpbRes", C.c_float * 2), ("fanCorner", C.c_float * 3), ("fanDelta", C.c_float * 3),
         ("sourceDist", C.c_float * 2), ("volConst", C.c_float), ("volLin", C.c_float), ("volSq", C.c_float),
         ("lenToMm", C.c_float), ("energyDepthToMm", C.c_float),
+        ("profileHooks", C.c_int), ("profileRows", C.c_int), ("profileDepthN", C.c_int),
+        ("profileChannels", C.c_int),
     ]
 
 
@@ -171,7 +177,7 @@ def lib():
         L.pbs_plan_debug.restype = C.c_int
         L.pbs_plan_debug.argtypes = [_i, _f, _f, _i, _f, _f, C.POINTER(BeamSettings), C.POINTER(EnergyStruct),
                                      C.POINTER(PlanInfo), C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
-                                     C.c_void_p]
+                                     C.c_void_p, C.c_void_p]
         L.pbs_engine_debug_halo.restype = C.c_int
         L.pbs_engine_debug_halo.argtypes = [C.c_void_p, _i, _f, C.c_void_p, C.c_void_p, _i]
         L.pbs_batch_create.restype = C.c_void_p
@@ -359,6 +365,20 @@ def beam_settings(beam: dict, keep: Keep, use_subspots: bool | None = None) -> B
         b.waterEquivalence = keep.f32(weq)
         b.waterEquivalenceCount = int(weq.size)
     b.spotPositionsAreIndices = 1 if beam.get("spot_positions_are_indices", False) else 0
+    # CarbonPBS profile model (reference keys, raytracedicom_pybind.cpp:1782-1796) and the hooks
+    # switch of this build (0: off, as in the reference)
+    for key, ptr, cnt in (("profile_energies", "profileEnergies", "numProfileEnergies"),
+                          ("profile_data", "profileData", "profileDataCount"),
+                          ("profile_setting", "profileSetting", "numProfileSetting"),
+                          ("beam_para_data", "beamParaData", "numBeamParaData")):
+        v = beam.get(key)
+        if v is not None:
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float32).ravel())
+            setattr(b, ptr, keep.f32(a))
+            setattr(b, cnt, int(a.size))
+    if beam.get("beam_para_pos") is not None:
+        b.beamParaPos = float(beam["beam_para_pos"])
+    b.carbonProfileHooks = int(beam.get("carbon_profile_hooks", 0))
     return b
 
 
@@ -378,21 +398,23 @@ def plan_debug(case: dict, beam: dict, use_subspots: bool | None = None) -> dict
     info = PlanInfo()
     L = int(bs.numLayers)
     rc = lib().pbs_plan_debug(dp, rp, cp, ddp, drp, dcp, C.byref(bs), C.byref(es), C.byref(info), None, None,
-                              None, 0, None)
+                              None, 0, None, None)
     check(rc, "pbs_plan_debug")
     eidx = np.zeros(L, np.float32)
     peak = np.zeros(L, np.float32)
     nx, ny = info.spotDims
     dense = np.zeros(max(L * nx * ny, 1), np.float32)
     cells = np.zeros(max(int(bs.numSpots), 1), np.int32)
+    prof = np.zeros((L, 4), np.float32)
     rc = lib().pbs_plan_debug(dp, rp, cp, ddp, drp, dcp, C.byref(bs), C.byref(es), C.byref(info), eidx.ctypes.data,
-                              peak.ctypes.data, dense.ctypes.data, dense.size, cells.ctypes.data)
+                              peak.ctypes.data, dense.ctypes.data, dense.size, cells.ctypes.data, prof.ctypes.data)
     check(rc, "pbs_plan_debug")
     out = {k: (list(getattr(info, k)) if not isinstance(getattr(info, k), (int, float)) else getattr(info, k))
            for k, _ in PlanInfo._fields_}
     out["energyIdx"], out["peakDepth"] = eidx, peak
     out["dense"] = dense[:L * nx * ny].reshape(L, ny, nx) if not info.legacyPath else None
     out["spotCell"] = cells[:int(bs.numSpots)]
+    out["profileLayer"] = prof
     return out
 
 

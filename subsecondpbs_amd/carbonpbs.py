@@ -27,9 +27,10 @@ only bounds the geometry, since the WEQ volume replaces the trace; then runs the
 writes the dose into ``finalDose``: flat or C-contiguous (Nz, Ny, Nx) buffers accumulate,
 (Nx, Ny, Nz) buffers receive the dose in that order (pybind:980-1028; the (Nx, Ny, Nz) shape is
 tested first, so a cubic grid's 3-D buffer is filled x-major). The profile / beam-
-parameter arrays are accepted and checked but not used: the reference gates its profile-sigma
-and overall-weight hooks off (wrapper:2812-2814). ``cuCalDose3`` zeroes the CSC outputs (no
-spot matrix is formed, as in the reference) and returns the dose.
+parameter arrays travel with the beam; their profile-sigma and overall-weight hooks stay off
+unless ``carbon_profile_hooks`` turns them on (the reference gates them off,
+wrapper:2812-2814). ``cuCalDose3`` zeroes the CSC outputs (no spot matrix is formed, as in the
+reference) and returns the dose.
 """
 from __future__ import annotations
 
@@ -80,9 +81,14 @@ def normalized_mean_direction(dirs: np.ndarray) -> np.ndarray:
     return m / n if n > 0 else m
 
 
-def _slice_mode() -> str:
-    v = os.environ.get("RTD_IDD_SLICE_MODE", "").strip().lower()
+def _slice_mode(var: str = "RTD_IDD_SLICE_MODE") -> str:
+    v = os.environ.get(var, "").strip().lower()
     return "head" if v in ("head", "prefix") else "tail"
+
+
+def _lround(v: float) -> int:
+    """std::lround: halves away from zero."""
+    return int(np.sign(v) * np.floor(abs(v) + 0.5))
 
 
 def build_energy_from_carbonpbs(enelist, idddata, iddsetting, tables_dir: str | None = None) -> dict:
@@ -227,8 +233,11 @@ def to_roi_linear_indices(roi, dims) -> np.ndarray:
 def build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir, bmxdir, bmydir, corner,
                             resolution, dims, longitudalCutoff, enelist, idddata, iddsetting, profiledata,
                             profilesetting, beamparadata, subspotdata, layerInfo, layerEnergy, idbeamxy, nPar,
-                            sad, beamParaPos=None, tables_dir: str | None = None) -> dict:
-    """Beam / energy dicts and grid of a CarbonPBS call (pybind:1121-1373)."""
+                            sad, beamParaPos=None, tables_dir: str | None = None, carbon_profile_hooks: int = 0) -> dict:
+    """Beam / energy dicts and grid of a CarbonPBS call (pybind:1121-1373). The profile model
+    goes into the beam as the reference stores it (pybind:1269-1302: the depth step rescaled for a
+    compacted table, the LUT energy axis sliced to the table's rows by RTD_PROFILE_SLICE_MODE,
+    tail by default); ``carbon_profile_hooks`` (0, the reference's setting) switches its hooks."""
     d = [int(v) for v in np.asarray(dims).ravel()]
     if len(d) != 3:
         raise RuntimeError("dims must contain exactly 3 integers")
@@ -277,6 +286,30 @@ def build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir,
         beam["subspot_data"], beam["spot_sigmas"] = sub, derive_layer_sigmas_from_subspots(sub)
     else:
         beam["subspot_data"], beam["spot_sigmas"] = remap_subspot_lut_to_actual_layers(lut_e, sub, layer_e)
+    prof_rows = prof_depth = 0
+    if profiledata is not None:
+        pa = _f32(profiledata)
+        prof_rows = int(pa.shape[0]) if pa.ndim >= 1 else 0
+        prof_depth = int(pa.shape[1]) if pa.ndim >= 2 else 0
+        beam["profile_data"] = np.ascontiguousarray(pa).ravel()
+    if profilesetting is not None:
+        ps = to_float_vector(profilesetting).copy()
+        if ps.size >= 3 and prof_depth > 0:
+            logical = max(0, _lround(float(ps[2])))
+            if logical > prof_depth:
+                ps[1] = np.float32(ps[1] * (np.float32(logical) / np.float32(prof_depth)))
+            ps[2] = np.float32(prof_depth)
+        beam["profile_setting"] = ps
+    prof_e = lut_e
+    if prof_rows > 0 and lut_e.size and prof_rows != lut_e.size:
+        keep = min(lut_e.size, prof_rows)
+        prof_e = lut_e[:keep] if _slice_mode("RTD_PROFILE_SLICE_MODE") == "head" else lut_e[lut_e.size - keep:]
+    beam["profile_energies"] = prof_e
+    if beamparadata is not None:
+        beam["beam_para_data"] = np.ascontiguousarray(_f32(beamparadata)).ravel()
+    if beamParaPos is not None:
+        beam["beam_para_pos"] = float(beamParaPos)
+    beam["carbon_profile_hooks"] = int(carbon_profile_hooks)
     energy = build_energy_from_carbonpbs(enelist, idddata, iddsetting, tables_dir)
     return {"dims": tuple(d), "corner": tuple(float(v) for v in corner_v),
             "resolution": tuple(float(v) for v in res_v), "beam": beam, "energy": energy,
@@ -322,13 +355,15 @@ def cu_final_dose(finalDose, rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir
                   resolution, dims, longitudalCutoff, enelist, idddata, iddsetting, profiledata, profilesetting,
                   beamparadata, subspotdata, layerInfo, layerEnergy, idbeamxy, nPar, sad, cutoff=None,
                   beamParaPos=None, gpuId: int = 0, nuclear_correction: bool = False, verbose: int = 0,
-                  debug: bool = False, tables_dir: str | None = None):
+                  debug: bool = False, tables_dir: str | None = None, carbon_profile_hooks: int = 0):
     """Dose of a CarbonPBS beam written into ``finalDose`` (pybind:1873-1953). ``cutoff`` is
-    accepted for the contract and not used, as in the reference."""
+    accepted for the contract and not used, as in the reference. ``carbon_profile_hooks`` (not in
+    the reference's signature; 0 keeps its behaviour): bit 0 sigma from the profile model, bit 1
+    the overall-weight channel."""
     ctx = build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir, bmxdir, bmydir, corner,
                                   resolution, dims, longitudalCutoff, enelist, idddata, iddsetting, profiledata,
                                   profilesetting, beamparadata, subspotdata, layerInfo, layerEnergy, idbeamxy, nPar,
-                                  sad, beamParaPos, tables_dir)
+                                  sad, beamParaPos, tables_dir, carbon_profile_hooks)
     nx, ny, nz = ctx["dims"]
     out = finalDose
     if not isinstance(out, np.ndarray) or out.dtype != np.float32 or out.size != nx * ny * nz:

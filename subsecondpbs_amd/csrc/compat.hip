@@ -62,6 +62,17 @@ extern "C" void subsecondWrapper(const float *ctData, const int3 &ctDims, const 
         b.waterEquivalence = weq.size() >= 9 ? weq.data() : nullptr;
         b.waterEquivalenceCount = weq.size() >= 9 ? weq.size() : 0;
         b.spotPositionsAreIndices = s.spotPositionsAreIndices ? 1 : 0;
+        // the profile model travels along; its hooks stay off, as in the reference (wrapper:2812-2814)
+        b.profileEnergies = ptr_or_null(s.profileEnergies);
+        b.numProfileEnergies = int(s.profileEnergies.size());
+        b.profileData = ptr_or_null(s.profileData);
+        b.profileDataCount = s.profileData.size();
+        b.profileSetting = ptr_or_null(s.profileSetting);
+        b.numProfileSetting = int(s.profileSetting.size());
+        b.beamParaData = ptr_or_null(s.beamParaData);
+        b.numBeamParaData = int(s.beamParaData.size());
+        b.beamParaPos = s.beamParaPos;
+        b.carbonProfileHooks = 0;
         if (s.energies.size() != s.spotSigmas.size()) {
             std::fprintf(stderr, "[RTD_ENTRY_ASSERT] beamIdx=%zu spotSigmas size must match energies\n", i);
             return;

@@ -249,6 +249,9 @@ struct pbs_engine {
     // adjoint with respect to the spot weights: dJ/dIDD x IDD / W per (layer, step, ray), dJ/dW,
     // the transposed convolution's intermediate and the spot-lattice gradient
     DevBuf<float> adjI, adjW, adjInterm, adjSpot;
+    // CarbonPBS profile hooks: the profile table and the per-layer row / beam parameters
+    DevBuf<float> profTable;
+    DevBuf<float4> profLayer;
     ConvArgs convArgs{};
     std::vector<float> hostAdjSpot;
     // staging of a no-allocation call whose pinned staging is too small: host bytes and their
@@ -666,7 +669,8 @@ int pbs_engine_debug_halo(pbs_engine *e, int nucDims[2], float *spotDist, float 
 int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
                    const float doseRes[3], const float doseCorner[3], const pbs_beam_settings *beam,
                    const pbs_energy_struct *energy, pbs_plan_info *info, float *layerEnergyIdx,
-                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells) {
+                   float *layerPeakDepth, float *denseWeights, size_t denseCap, int *spotCells,
+                   float *layerProfile) {
     if (!ctDims || !ctRes || !ctCorner || !doseDims || !doseRes || !doseCorner || !beam || !energy || !info)
         return set_error(PBS_E_INVALID, "null argument");
     int rc = validate_beam(*beam, 0);
@@ -700,6 +704,13 @@ int pbs_plan_debug(const int ctDims[3], const float ctRes[3], const float ctCorn
     o.volSq = p.volSq;
     o.lenToMm = p.lenToMm;
     o.energyDepthToMm = p.energyDepthToMm;
+    o.profileHooks = p.profileHooks;
+    o.profileRows = p.profileRows;
+    o.profileDepthN = p.profileDepthN;
+    o.profileChannels = p.profileChannels;
+    if (layerProfile)
+        for (size_t i = 0; i < size_t(p.numLayers) * 4; ++i)
+            layerProfile[i] = i < p.profileLayer.size() ? p.profileLayer[i] : 0.0f;
     for (int l = 0; l < p.numLayers; ++l) {
         if (layerEnergyIdx) layerEnergyIdx[l] = p.layers[l].energyIdx;
         if (layerPeakDepth) layerPeakDepth[l] = p.layers[l].peakDepth;
@@ -951,6 +962,9 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (keep && (halo || p.legacyPath))
         return set_error(PBS_E_INVALID, halo ? "resident beams do not cover the nuclear halo branch"
                                              : "resident beams need spot positions and weights (spot-lattice path)");
+    const bool hooks = p.profileHooks != 0;
+    if (hooks && (adjoint || keep))
+        return set_error(PBS_E_INVALID, "the CarbonPBS profile hooks are not covered by the adjoint or resident beams");
     if (halo) {
         if (!e->nucW.p || !e->nucS2.p)
             return set_error(PBS_E_GEOMETRY,
@@ -1035,7 +1049,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     const size_t bRay = halo ? sizeof(int) * nray : 0, bHaloW = halo ? sizeof(float) * e->haloPlan.weights.size() : 0;
     const size_t oSub = pin_align(bLayers), oDense = oSub + pin_align(bSub), oRay = oDense + pin_align(bDense);
     const size_t oHaloW = oRay + pin_align(bRay);
-    const size_t stageBytes = oHaloW + pin_align(bHaloW);
+    const size_t bProf = hooks ? sizeof(float) * size_t(p.profileRows) * p.profileDepthN * p.profileChannels : 0;
+    const size_t bProfL = hooks ? sizeof(float) * p.profileLayer.size() : 0;
+    const size_t oProf = oHaloW + pin_align(bHaloW), oProfL = oProf + pin_align(bProf);
+    const size_t stageBytes = oProfL + pin_align(bProfL);
     // the staging k_upload reads: the mapped pinned buffer, or (a no-allocation call whose pinned
     // staging is too small) a device copy of host bytes carved from the scratch
     const bool devStage = t_arena && (!e->pin || e->pinBytes < stageBytes);
@@ -1058,6 +1075,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         std::memcpy(stageHost + oRay, e->haloPlan.rayToNuc.data(), bRay);
         std::memcpy(stageHost + oHaloW, e->haloPlan.weights.data(), bHaloW);
     }
+    if (hooks) {
+        std::memcpy(stageHost + oProf, beam->profileData, bProf);
+        std::memcpy(stageHost + oProfL, p.profileLayer.data(), bProfL);
+    }
     if (devStage) HIP_TRY(hipMemcpyAsync(e->devStage.p, stageHost, stageBytes, hipMemcpyHostToDevice, s));
     UploadArgs up{};
     add_upload(up, stageSrc, e->layerHost.p, 0, bLayers);
@@ -1070,6 +1091,12 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         add_upload(up, stageSrc, e->rayToNuc.p, oRay, bRay);
         add_upload(up, stageSrc, e->haloW.p, oHaloW, bHaloW);
         HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
+    }
+    if (hooks) {
+        HIP_TRY(e->profTable.ensure(bProf / sizeof(float)));
+        HIP_TRY(e->profLayer.ensure(size_t(L)));
+        add_upload(up, stageSrc, e->profTable.p, oProf, bProf);
+        add_upload(up, stageSrc, e->profLayer.p, oProfL, bProfL);
     }
     HIP_TRY(launch_upload(up, s));
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
@@ -1147,6 +1174,21 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                             e->rayW.p, e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p,
                             s));
     }
+    if (hooks) {
+        // the profile hooks rewrite the field after the fill (wrapper:4604-4832); the tile codes
+        // are then taken from the rewritten field instead of the fill's
+        ProfileArgs pa{};
+        pa.table = e->profTable.p;
+        pa.layer = e->profLayer.p;
+        pa.rows = p.profileRows;
+        pa.depthN = p.profileDepthN;
+        pa.channels = p.profileChannels;
+        pa.hooks = p.profileHooks;
+        pa.depth0 = p.profileDepth0;
+        pa.depthStep = p.profileDepthStep;
+        pa.beamParaPos = p.beamParaPos;
+        HIP_TRY(launch_profile_hooks(g, pa, e->wepl.p, e->iddSigma.p, e->layerScal.p, e->beamScal.p, s));
+    }
     if (adjoint) {
         // 4-6 transposed, then the transposed convolution and the spots' lattice cells
         int rc3 = run_grid_adjoint(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, gradDose, s);
@@ -1170,7 +1212,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
     for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
-    int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, true, doseDev,
+    int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, !hooks, doseDev,
                        streams, &marks);
     if (rc2 != PBS_OK) return rc2;
     // 7. halo grid through the same stages

@@ -78,13 +78,27 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
 hipError_t launch_rescale(const FanGeom &g, LayerScalars *ls, const BeamScalars *bs,
                           const float *rayW, const float2 *unitIdd, const int *unitFp, float2 *iddSigma,
                           int *firstPassive, int *tileCode, hipStream_t s);
+// CarbonPBS profile hooks on every layer's field after the fill (wrapper:2080-2188, 4604-4832):
+// bit 0 of `hooks` replaces rSigmaEff by the profile model's (multi-Gaussian sigma from the table
+// at the step's WEPL, plus the beam parameters' initial width at the step's physical depth; +inf
+// where IDD <= 0), bit 1 scales IDD by the table's overall-weight channel. The table is
+// [rows][depthN][channels], sampled as linear texture filtering at texel centres with clamped
+// addressing (bilinear over depth and row, channels exact).
+struct ProfileArgs {
+    const float *table;
+    const float4 *layer;  // [L]: table row (fractional), beam parameters 0..2
+    int rows, depthN, channels, hooks;
+    float depth0, depthStep, beamParaPos, pad;
+};
+hipError_t launch_profile_hooks(const FanGeom &g, const ProfileArgs &a, const float *wepl, float2 *iddSigma,
+                                const LayerScalars *ls, const BeamScalars *bs, hipStream_t s);
 hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
                             BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s);
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s);
 
 // Uploads of a beam's small host arrays in one kernel that reads the engine's mapped pinned
 // staging directly (a DMA-engine copy adds ~15 us of latency to the stream per call).
-constexpr int kMaxUploadSegs = 5;
+constexpr int kMaxUploadSegs = 7;
 struct UploadSeg {
     const uint32_t *src;  // device-visible address in mapped pinned host memory
     uint32_t *dst;

@@ -1961,6 +1961,81 @@ hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamSca
     return hipGetLastError();
 }
 
+// The profile table at (channel c, fractional depth d, fractional row r), d and r already clamped
+// to the table: linear texture filtering at texel centres with clamped addressing.
+__device__ __forceinline__ float profile_at(const ProfileArgs &a, int c, float d, float r) {
+    const int d0 = int(d), r0 = int(r);  // d, r >= 0
+    const int d1 = min(d0 + 1, a.depthN - 1), r1 = min(r0 + 1, a.rows - 1);
+    const float fd = d - float(d0), fr = r - float(r0);
+    const float *t0 = a.table + size_t(r0) * a.depthN * a.channels + c;
+    const float *t1 = a.table + size_t(r1) * a.depthN * a.channels + c;
+    const float a0 = t0[size_t(d0) * a.channels], b0 = t0[size_t(d1) * a.channels];
+    const float a1 = t1[size_t(d0) * a.channels], b1 = t1[size_t(d1) * a.channels];
+    const float v0 = a0 + (b0 - a0) * fd, v1 = a1 + (b1 - a1) * fd;
+    return v0 + (v1 - v0) * fr;
+}
+
+// Thread per (ray, layer), steps [beamFirstInside, afterLast): the superposition reads no other
+// step of the field.
+__global__ __launch_bounds__(256) void k_profile_hooks(FanGeom g, ProfileArgs a, const float *__restrict__ wepl,
+                                                       float2 *__restrict__ iddSigma,
+                                                       const LayerScalars *__restrict__ ls,
+                                                       const BeamScalars *__restrict__ bs) {
+    const int l = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= g.rx * g.ry) return;
+    const int x = r % g.rx, y = r / g.rx;
+    const float4 lp = a.layer[l];
+    const int k0 = max(bs->beamFirstInside, 0), k1 = min(ls[l].afterLast, g.K);
+    const size_t planeS = size_t(g.sx) * g.sy, planeR = size_t(g.rx) * g.ry;
+    float2 *f = iddSigma + size_t(l) * g.K * planeS + size_t(y) * g.sx + x;
+    const int nG = (a.channels - 1) / 2;
+    // distance from the virtual source along this divergent ray, per unit of axial distance
+    const float sad = 0.5f * (g.dist[0] + g.dist[1]);
+    const float fanX = g.fc[0] + float(x) * g.fd[0], fanY = g.fc[1] + float(y) * g.fd[1];
+    const float slant = sqrtf(1.0f + (fanX * fanX) / (g.dist[0] * g.dist[0]) + (fanY * fanY) / (g.dist[1] * g.dist[1]));
+    for (int k = k0; k < k1; ++k) {
+        float2 v = f[size_t(k) * planeS];
+        if (!(v.x > 0.0f)) {
+            if (a.hooks & 1) {
+                v.y = PBS_INF;
+                f[size_t(k) * planeS] = v;
+            }
+            continue;
+        }
+        const float w = wepl[size_t(k) * planeR + r];
+        const float di = fminf(fmaxf((w - a.depth0) / a.depthStep, 0.0f), float(a.depthN - 1));
+        if (a.hooks & 1) {
+            float sw = 0.0f, s2 = 0.0f;
+            for (int q = 0; q < nG; ++q) {
+                const float wq = profile_at(a, q, di, lp.x), sq = profile_at(a, q + nG, di, lp.x);
+                sw += wq;
+                s2 += wq * sq * sq;
+            }
+            if (!(sw > 1.0e-8f)) {
+                v.y = PBS_INF;
+            } else {
+                const float fanZ = g.fc[2] + float(k) * g.fd[2];
+                const float phy = (sad - fanZ) * slant - (sad - a.beamParaPos);
+                const float initR2 = lp.y + 2.0f * lp.z * phy + lp.w * phy * phy;
+                const float axis = sqrtf(fmaxf(0.0f, 0.5f * fmaxf(0.0f, initR2 + s2 / sw)));
+                const float width = 0.5f * (g.fd[0] * (1.0f - fanZ / g.dist[0]) + g.fd[1] * (1.0f - fanZ / g.dist[1]));
+                v.y = 0.5f * width / (1.41421356f * (axis + 0.21f));
+            }
+        }
+        if (a.hooks & 2) v.x *= fmaxf(0.0f, profile_at(a, a.channels - 1, di, lp.x));
+        f[size_t(k) * planeS] = v;
+    }
+}
+
+hipError_t launch_profile_hooks(const FanGeom &g, const ProfileArgs &a, const float *wepl, float2 *iddSigma,
+                                const LayerScalars *ls, const BeamScalars *bs, hipStream_t s) {
+    const int nray = g.rx * g.ry;
+    hipLaunchKernelGGL(k_profile_hooks, dim3(unsigned((nray + 255) / 256), unsigned(g.numLayers)), dim3(256), 0, s, g,
+                       a, wepl, iddSigma, ls, bs);
+    return hipGetLastError();
+}
+
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s) {
     const size_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL(k_clear_pairs, dim3(unsigned(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, v, n);

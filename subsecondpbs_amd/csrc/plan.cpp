@@ -386,6 +386,74 @@ static bool infer_source_dist(const pbs_beam_settings &b, F3 bmX, F3 bmY, F3 bmZ
 
 static float sigma_to_mm(float lenToMm, float s) { return (lenToMm > 1.0f && s > 0.0f && s < 1.0f) ? s * 10.0f : s; }
 
+// The CarbonPBS profile model of a beam and the hooks that apply (wrapper:2816-2862 table shape,
+// 4577-4603 per-layer row and beam parameters, 4656-4660 / 4814-4819 the hooks' conditions).
+static void plan_profile_hooks(const pbs_beam_settings &b, const pbs_energy_struct &en, BeamPlan &p) {
+    p.profileHooks = 0;
+    p.profileLayer.clear();
+    const int want = b.carbonProfileHooks & 3;
+    const int L = b.numLayers;
+    const size_t size = b.profileData ? b.profileDataCount : 0;
+    const bool model = b.profileSetting && b.numProfileSetting >= 3 && std::lround(b.profileSetting[2]) > 0 && size > 0;
+    if (!want || !model) return;
+    const int depthN = std::max(1, int(std::lround(b.profileSetting[2])));
+    const size_t nPara = b.beamParaData ? size_t(std::max(b.numBeamParaData, 0)) : 0;
+    const size_t nEn = b.profileEnergies ? size_t(std::max(b.numProfileEnergies, 0)) : 0;
+    const int paraRows = int(nPara / 3);
+    const size_t plane = size_t(depthN);
+    int rows = 0;
+    if (paraRows > 0 && nPara == size_t(paraRows) * 3 && size % (size_t(paraRows) * plane) == 0)
+        rows = paraRows;
+    else if (nEn > 0 && size % (nEn * plane) == 0)
+        rows = int(nEn);
+    else if (L > 0 && size % (size_t(L) * plane) == 0)
+        rows = L;
+    else if (size % plane == 0 && (size / plane) % 11 == 0)
+        rows = int(size / plane / 11);  // [row][depth][11 channels] exports (wrapper:2840-2849)
+    if (rows <= 0) return;
+    const size_t perRow = size / size_t(rows);
+    if (perRow < size_t(depthN)) return;
+    const int channels = int(perRow / size_t(depthN));
+    if (channels <= 0 || size_t(rows) * depthN * channels != size) return;
+    const bool haveBeamPara = nPara >= size_t(L) * 3;
+    int hooks = 0;
+    if ((want & 1) && haveBeamPara && channels >= 3) hooks |= 1;
+    if (want & 2) hooks |= 2;
+    if (!hooks) return;
+    p.profileHooks = hooks;
+    p.profileRows = rows;
+    p.profileDepthN = depthN;
+    p.profileChannels = channels;
+    p.profileDepth0 = b.profileSetting[0];
+    p.profileDepthStep = b.profileSetting[1];
+    p.beamParaPos = b.beamParaPos;
+    p.profileLayer.assign(size_t(L) * 4, 0.0f);
+    const float rowMax = float(std::max(0, rows - 1));
+    for (int l = 0; l < L; ++l) {
+        float row;
+        if (rows == L)
+            row = float(l);
+        else if (nEn > 0 && rows == int(nEn))
+            row = fractional_index_monotonic(b.profileEnergies, int(nEn), b.energies[l]);
+        else if (rows == en.nEnergies && en.nEnergies > 0)
+            row = std::min(std::max(p.layers[l].energyIdx, 0.0f), float(rows - 1));
+        else
+            row = std::min(std::max(float(l), 0.0f), rowMax);
+        row = std::min(std::max(row, 0.0f), rowMax);
+        float *o = &p.profileLayer[size_t(l) * 4];
+        o[0] = row;
+        for (int c = 0; c < 3; ++c) {
+            if (nPara < size_t(rows) * 3) { o[1 + c] = 0.0f; continue; }
+            const float cl = std::min(std::max(row, 0.0f), float(rows - 1));
+            const int i0 = std::max(0, std::min(rows - 1, int(std::floor(cl))));
+            const int i1 = std::max(0, std::min(rows - 1, i0 + 1));
+            const float t = cl - float(i0);
+            const float v0 = b.beamParaData[size_t(i0) * 3 + c], v1 = b.beamParaData[size_t(i1) * 3 + c];
+            o[1 + c] = v0 + (v1 - v0) * t;
+        }
+    }
+}
+
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
                const float doseRes[3], const float doseCorner[3], const pbs_beam_settings &b,
                const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov, const PlanEnv *envIn) {
@@ -605,6 +673,7 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
         L.airQuad = a * 1.0f * p.fanDelta[2] * p.fanDelta[2];
         L.airLin = 2.0f * a * 1.0f * p.fanDelta[2] * p.fanCorner[2] + c * p.fanDelta[2];
     }
+    plan_profile_hooks(b, en, p);
     return true;
 }
 

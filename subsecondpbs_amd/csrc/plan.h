@@ -74,6 +74,13 @@ struct BeamPlan {
     bool spotIndices = false;  // spot positions are WEQ-lattice texel coordinates
     int weqDims[3] = {0, 0, 0};  // x_n, y_n, depth_n
     float weqStepMm = 0.f, weqX0 = 0.f, weqY0 = 0.f, weqDx = 0.f, weqDy = 0.f;
+    // CarbonPBS profile hooks that apply to this beam (bit 0 sigma override, bit 1 overall
+    // weight; wrapper:2816-2862, 4577-4832): the table's shape, its depth axis and, per layer,
+    // the table row (fractional) and the beam parameters interpolated at it
+    int profileHooks = 0;
+    int profileRows = 0, profileDepthN = 0, profileChannels = 0;
+    float profileDepth0 = 0.f, profileDepthStep = 0.f, beamParaPos = 0.f;
+    std::vector<float> profileLayer;  // [L][4]: row index, beam parameters 0..2
 };
 
 // Explicit CPB grid in the gantry frame (used by cudaFinalDose, whose CPB grid comes from

@@ -148,3 +148,31 @@ def test_context_beam_equals_the_fixture_beam():
     d_ctx, _ = O.beam_dose(one)
     d_fix, _ = O.beam_dose(case)
     assert np.array_equal(d_ctx, d_fix)
+
+
+def test_context_profile_model(monkeypatch):
+    """The profile arrays travel with the beam as the reference stores them (pybind:1269-1302):
+    a table of 20 stored depths for 40 logical ones doubles the depth step and stores 20; the LUT
+    energy axis is cut to the table's rows (tail by default, head with RTD_PROFILE_SLICE_MODE);
+    the hooks stay off unless asked for."""
+    case = F.make_weq_case(index_spots=True)
+    args = carbonpbs_args(case)
+    E = np.asarray(args["enelist"], np.float32)
+    table = np.ones((6, 20, 5), np.float32)
+    args.update(profiledata=table, profilesetting=np.array([0.0, 2.0, 40.0], np.float32),
+                beamparadata=np.ones((6, 3), np.float32), beamParaPos=12.5)
+    b = CP.build_carbonpbs_context(**args)["beam"]
+    np.testing.assert_array_equal(b["profile_setting"], np.array([0.0, 4.0, 20.0], np.float32))
+    np.testing.assert_array_equal(b["profile_energies"], E[-6:])
+    assert b["profile_data"].size == table.size and b["beam_para_pos"] == 12.5
+    assert b["carbon_profile_hooks"] == 0
+    monkeypatch.setenv("RTD_PROFILE_SLICE_MODE", "head")
+    b = CP.build_carbonpbs_context(**args, carbon_profile_hooks=3)["beam"]
+    np.testing.assert_array_equal(b["profile_energies"], E[:6])
+    assert b["carbon_profile_hooks"] == 3
+    # with the hooks off the profile arrays change nothing
+    one = dict(case)
+    one["beams"] = [CP.build_carbonpbs_context(**args)["beam"]]
+    d_prof, _ = O.beam_dose(one)
+    d_fix, _ = O.beam_dose(case)
+    assert np.array_equal(d_prof, d_fix)
