@@ -1499,7 +1499,16 @@ struct RowsShape {
     static constexpr int G = 2 * RM + 1 <= 8 ? 8 : (2 * RM + 1 <= 16 ? 16 : (2 * RM + 1 <= 32 ? 32 : 64));
     static constexpr int NT = 8 * G;
     static constexpr int EW = pad4(RM + 3);
-    static constexpr int RP = 32 * EW + 4;  // source-row pitch: +4 floats puts the groups of a wave on other banks
+    // With EW a multiple of 16 floats, the 32 table rows of a source row would start on 4 bank
+    // groups only, and their 16-byte stores would conflict 8 ways: every fourth column then moves
+    // by 4 floats more (col_off), which spreads the starts over all 64 banks.
+    static constexpr bool SWZ = EW % 16 == 0;
+    static constexpr int SPAN = 32 * EW + (SWZ ? 32 : 0);
+    // source-row pitch: SPAN rounded up to G modulo 64 floats puts the 64 / G groups of a wave at
+    // bank offsets 0, G, 2G, ...: each group's per-lane reads E[1 + v] (v <= RM + 1 < G) then fall
+    // on banks no other group of the wave touches
+    static constexpr int RP = SPAN + ((G - SPAN % 64) % 64 + 64) % 64;
+    static __device__ __forceinline__ int col_off(int col) { return col * EW + (SWZ ? 4 * (col >> 2) : 0); }
     static constexpr int BHp = BH | 1;      // block column pitch (odd: conflict-free both ways)
     static constexpr int EBUF = 8 * RP;     // one layer's table
     static constexpr int LDS = (2 * EBUF > BW * BHp ? 2 * EBUF : BW * BHp);
@@ -1510,37 +1519,46 @@ struct RowsShape {
 
 // One source row of one layer into this lane's output row: for every source x_s, a = d e_v with
 // v = |dy| clamped to R+1 (a zero entry), then 2R+1 FMAs along the row, acc[x_s + dx + RM] +=
-// a e_|dx|. Dead sources (d = 0) add exact zeros; running them keeps the sweep branch-free,
-// which measured faster than skipping them. RP_ is the radius compiled; entries past the
-// layer's own radius are zero, so a smaller layer radius adds exact zeros.
+// a e_|dx|. The sources go in chunks of four; a chunk where no source row of the wave has a live
+// pixel (bit x_s of wm, the OR of the wave's live masks, uniform over the wave) is skipped by a
+// scalar branch: its sources would add exact zeros. Within a chunk the sweep is branch-free. RP_
+// is the radius compiled; entries past the layer's own radius are zero, so a smaller layer
+// radius adds exact zeros.
 template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v) {
+__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v,
+                                           uint32_t wm) {
     constexpr int NE4 = (RP_ + 2 + 3) / 4;
+    using S = RowsShape<RM>;
 #pragma unroll
-    for (int xs = 0; xs < 32; ++xs) {
-        const float *E = rowE + xs * EW;
-        float e[4 * NE4];
+    for (int x0 = 0; x0 < 32; x0 += 4) {
+        if (!((wm >> x0) & 0xfu)) continue;  // uniform: wm is a scalar
 #pragma unroll
-        for (int q = 0; q < NE4; ++q) {
-            const float4 t4 = *reinterpret_cast<const float4 *>(E + 4 * q);
-            e[4 * q] = t4.x; e[4 * q + 1] = t4.y; e[4 * q + 2] = t4.z; e[4 * q + 3] = t4.w;
-        }
-        const float a = e[0] * E[1 + v];
+        for (int xs = x0; xs < x0 + 4; ++xs) {
+            const float *E = rowE + S::col_off(xs);
+            float e[4 * NE4];
 #pragma unroll
-        for (int dx = -RP_; dx <= RP_; ++dx) {
-            const int c = xs + dx + RM;
-            acc[c] = fmaf(a, e[1 + (dx < 0 ? -dx : dx)], acc[c]);
+            for (int q = 0; q < NE4; ++q) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(E + 4 * q);
+                e[4 * q] = t4.x; e[4 * q + 1] = t4.y; e[4 * q + 2] = t4.z; e[4 * q + 3] = t4.w;
+            }
+            const float a = e[0] * E[1 + v];
+#pragma unroll
+            for (int dx = -RP_; dx <= RP_; ++dx) {
+                const int c = xs + dx + RM;
+                acc[c] = fmaf(a, e[1 + (dx < 0 ? -dx : dx)], acc[c]);
+            }
         }
     }
 }
 
 template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v) {
+__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v,
+                                              uint32_t wm) {
     if constexpr (RP_ <= RM) {
         if (rp == RP_)
-            rows_sweep<RP_, RM, EW>(acc, rowE, v);
+            rows_sweep<RP_, RM, EW>(acc, rowE, v, wm);
         else
-            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v);
+            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v, wm);
     }
 }
 
@@ -1654,7 +1672,7 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
                     }
                     row[1 + m] = e;
                 }
-                float *dst = E + (src >> 5) * RP + (src & 31) * EW;
+                float *dst = E + (src >> 5) * RP + S::col_off(src & 31);
 #pragma unroll
                 for (int m = 0; m < EW / 4; ++m)
                     *reinterpret_cast<float4 *>(dst + 4 * m) =
@@ -1669,7 +1687,7 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
             if (wm) {
                 const int v = ady < R + 1 ? ady : R + 1;
                 const int rp = R > S::RLO ? R : S::RLO;
-                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v);
+                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v, wm);
             }
         }
         __syncthreads();  // tables dead: the block reuses their LDS
