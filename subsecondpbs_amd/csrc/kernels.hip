@@ -135,7 +135,7 @@ __device__ int first_larger_ordered(const float *v, int n, float value) {
 // ---------------------------------------------------------------------------------------
 // trace
 // ---------------------------------------------------------------------------------------
-// The trace runs as (ray, 16-step segment) threads so that every ray's K steps are spread
+// The trace runs as (ray, 8-step segment) threads so that every ray's K steps are spread
 // over many waves: pass 1 samples the CT and keeps segment-local prefixes, pass 2 turns the
 // segment totals into carries and the entry / exit steps, pass 3 adds the carries.
 constexpr int kTraceSeg = 8;
