@@ -57,7 +57,27 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    caller = build_cpp_caller()
+    if verbose:
+        print("built", caller)
     return LIB
+
+
+def build_cpp_caller() -> str:
+    """The C++ caller of subsecondWrapper that the GPU tests run (tests/cpp/rtd_wrapper_main.cpp):
+    compiled with g++ against include/raytracedicom_integration.h and linked to the library built
+    above, found at run time through an rpath relative to the executable."""
+    root = os.path.dirname(PKG)
+    src = os.path.join(root, "tests", "cpp", "rtd_wrapper_main.cpp")
+    out_dir = os.path.join(root, "tests", "cpp", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "rtd_wrapper_main")
+    rocm_inc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "include")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", rocm_inc,
+          "-I", os.path.join(root, "include"), src, "-L", LIBDIR, "-lpbs_hip",
+          "-Wl,-rpath,$ORIGIN/../../../subsecondpbs_amd/_lib", "-o", exe + ".tmp"])
+    os.replace(exe + ".tmp", exe)
+    return exe
 
 
 if __name__ == "__main__":

@@ -5,8 +5,11 @@
 // transposes in reverse order to dJ/dDose:
 //
 //   k_adj_transfer ..... T^T: each dose voxel scatters its gradient into the 8 BEV cells its
-//                        forward sample reads (main BEV and the group tails it samples);
-//                        float atomics, so the BEV gradient is not bitwise repeatable
+//                        forward sample reads (main BEV and the group tails it samples), as
+//                        64-bit fixed-point integers: integer addition is associative, so the
+//                        sums do not depend on the order of the atomics and the BEV gradient
+//                        (and everything after it) is bitwise repeatable; k_adj_absmax fixes
+//                        the scale, k_adj_dequant turns the sums back into floats
 //   k_adj_superpose .... S^T and F^T per work unit (step, tile, block): the unit's BEV
 //                        window staged in LDS, each source pixel gathers its footprint
 //                        sum e_|j| e_|i| dJ/dBEV, times IDD / W (one writer per layer, step
@@ -18,28 +21,72 @@
 
 namespace pbs {
 
+// Fixed-point scale of the transposed transfer: 2^(40 - e) for max |dJ/dDose| = m 2^e
+// (m in [0.5, 1)). Every contribution w g (w <= 1) is then below 2^40 in magnitude, so up to
+// 2^22 contributions to one BEV cell stay inside the int64 range, and the quantum is 2^-40 of the
+// largest gradient (float keeps 2^-24 of each value).
+constexpr int kAdjFixBits = 40;
+__device__ __forceinline__ double adj_fix_scale(unsigned maxBits) {
+    int e = 0;
+    (void)frexpf(__uint_as_float(maxBits), &e);
+    return ldexp(1.0, kAdjFixBits - e);
+}
+
 // Scatter g into the in-range corners of a trilinear sample (the transpose of the border-0
-// trilinear gather of the transfer; planes outside [zlo, zhi) read as 0 there).
-__device__ __forceinline__ void trilinear_border_adj(float *__restrict__ v, int nx, int ny, int zlo, int zhi, float x,
-                                                     float y, float z, float g) {
+// trilinear gather of the transfer; planes outside [zlo, zhi) read as 0 there), in fixed point:
+// the float weight times g is exact in double, the power-of-two scale is exact, and one
+// rounding to an integer follows.
+__device__ __forceinline__ void trilinear_border_adj(unsigned long long *__restrict__ v, int nx, int ny, int zlo,
+                                                     int zhi, float x, float y, float z, float g, double scale) {
     const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
     const int x0 = int(fx), y0 = int(fy), z0 = int(fz);
     const float tx = x - fx, ty = y - fy, tz = z - fz;
+    const double gs = double(g) * scale;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int xi = x0 + (q & 1), yi = y0 + ((q >> 1) & 1), zi = z0 + (q >> 2);
         if (unsigned(xi) < unsigned(nx) && unsigned(yi) < unsigned(ny) && zi >= zlo && zi < zhi) {
             const float w = ((q & 1) ? tx : 1.0f - tx) * (((q >> 1) & 1) ? ty : 1.0f - ty) * ((q >> 2) ? tz : 1.0f - tz);
-            unsafeAtomicAdd(v + (size_t(zi) * ny + yi) * nx + xi, w * g);
+            const long long c = __double2ll_rn(double(w) * gs);
+            if (c) atomicAdd(v + (size_t(zi) * ny + yi) * nx + xi, static_cast<unsigned long long>(c));
         }
     }
+}
+
+// max |dJ/dDose| as float bits (non-negative floats order as their bit patterns); *maxBits was
+// cleared by k_adj_zero_fix.
+__global__ __launch_bounds__(256) void k_adj_absmax(const float *__restrict__ gDose, size_t n,
+                                                    unsigned *__restrict__ maxBits) {
+    float m = 0.0f;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        m = fmaxf(m, fabsf(gDose[i]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(maxBits, __float_as_uint(m));
+}
+
+__global__ void k_adj_zero_fix(unsigned long long *__restrict__ v, size_t n, unsigned *__restrict__ maxBits) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *maxBits = 0u;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        v[i] = 0ull;
+}
+
+// The fixed-point sums back to float dJ/dBEV (one rounding).
+__global__ void k_adj_dequant(const unsigned long long *__restrict__ q, size_t n, const unsigned *__restrict__ maxBits,
+                              float *__restrict__ out) {
+    const unsigned mb = *maxBits;
+    const double inv = mb ? 1.0 / adj_fix_scale(mb) : 0.0;  // a power of two: exact
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = float(double(static_cast<long long>(q[i])) * inv);
 }
 
 // T^T: the voxel -> fan map and sample selection of k_transfer (prim_transf_kernel.cu:11-44).
 __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
                                                       const GroupScalars *__restrict__ gs,
-                                                      const float *__restrict__ gDose, float *__restrict__ gBev,
-                                                      float *__restrict__ gTailBev, int tx, int ty, int tz) {
+                                                      const float *__restrict__ gDose,
+                                                      unsigned long long *__restrict__ gBev,
+                                                      unsigned long long *__restrict__ gTailBev,
+                                                      const unsigned *__restrict__ maxBits, int tx, int ty, int tz) {
     const int lx = int(threadIdx.x) % tx, ly = (int(threadIdx.x) / tx) % ty, lz = int(threadIdx.x) / (tx * ty);
     const int x = bs->boxStart[0] + int(blockIdx.x) * tx + lx;
     const int y = bs->boxStart[1] + int(blockIdx.y) * ty + ly;
@@ -47,6 +94,7 @@ __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta
     if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1] || z >= bs->boxEnd[2]) return;
     const float gv = gDose[(size_t(z) * g.doseDims[1] + y) * g.doseDims[0] + x];
     if (gv == 0.0f) return;
+    const double scale = adj_fix_scale(*maxBits);
     const int inside = bs->beamFirstInside;
     const int m = bs->numGroups;
     const int bx = g.sx + 64, by = g.sy + 64;
@@ -60,7 +108,8 @@ __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta
     const float fzI = pz + (ta.globalOffset[2] - float(inside));
     if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return;
     if (fxI > float(bx) || fyI > float(by)) return;
-    if (fzI <= float(g.K)) trilinear_border_adj(gBev, bx, by, inside, min(bs->mainEndMax, g.K), fxI, fyI, fzI, gv);
+    if (fzI <= float(g.K))
+        trilinear_border_adj(gBev, bx, by, inside, min(bs->mainEndMax, g.K), fxI, fyI, fzI, gv, scale);
     for (int j = 0; j < m; ++j) {
         const GroupScalars &G = gs[j];
         if (G.tailCount <= 0) continue;
@@ -70,7 +119,8 @@ __global__ __launch_bounds__(256) void k_adj_transfer(FanGeom g, TransferArgs ta
             z >= G.boxEnd[2])
             continue;
         const int nz = min(G.tailCount, bs->tailPlanes - G.tailBase);  // planes that exist
-        if (nz > 0) trilinear_border_adj(gTailBev + size_t(G.tailBase) * bx * by, bx, by, 0, nz, fxI, fyI, zl, gv);
+        if (nz > 0)
+            trilinear_border_adj(gTailBev + size_t(G.tailBase) * bx * by, bx, by, 0, nz, fxI, fyI, zl, gv, scale);
     }
 }
 
@@ -269,8 +319,14 @@ hipError_t launch_adj_zero(float *v, size_t n, hipStream_t s) {
 }
 
 hipError_t launch_adj_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs,
-                               const GroupScalars *gs, const float *gDose, float *gBev, float *gTailBev, int extentX,
-                               int extentY, int extentZ, int depthAxis, hipStream_t s) {
+                               const GroupScalars *gs, const float *gDose, size_t doseN, float *gBev,
+                               float *gTailBev, size_t mainN, size_t tailN, unsigned long long *fix,
+                               unsigned *maxBits, int extentX, int extentY, int extentZ, int depthAxis,
+                               hipStream_t s) {
+    // fixed-point accumulators [main planes | tail planes], cleared with the scale's maximum
+    const size_t nfix = mainN + tailN;
+    hipLaunchKernelGGL(k_adj_zero_fix, dim3(2048), dim3(256), 0, s, fix, nfix, maxBits);
+    hipLaunchKernelGGL(k_adj_absmax, dim3(1024), dim3(256), 0, s, gDose, doseN, maxBits);
     int t[3] = {8, 8, 8};
     t[depthAxis] = 4;
     const int e[3] = {extentX, extentY, extentZ};
@@ -278,7 +334,10 @@ hipError_t launch_adj_transfer(const FanGeom &g, const TransferArgs &ta, const B
     grid.x = unsigned((e[0] + t[0] - 1) / t[0]);
     grid.y = unsigned((e[1] + t[1] - 1) / t[1]);
     grid.z = unsigned((e[2] + t[2] - 1) / t[2]);
-    hipLaunchKernelGGL(k_adj_transfer, grid, dim3(256), 0, s, g, ta, bs, gs, gDose, gBev, gTailBev, t[0], t[1], t[2]);
+    hipLaunchKernelGGL(k_adj_transfer, grid, dim3(256), 0, s, g, ta, bs, gs, gDose, fix, fix + mainN, maxBits, t[0],
+                       t[1], t[2]);
+    hipLaunchKernelGGL(k_adj_dequant, dim3(2048), dim3(256), 0, s, fix, mainN, maxBits, gBev);
+    if (tailN) hipLaunchKernelGGL(k_adj_dequant, dim3(2048), dim3(256), 0, s, fix + mainN, tailN, maxBits, gTailBev);
     return hipGetLastError();
 }
 

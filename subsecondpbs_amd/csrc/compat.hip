@@ -12,6 +12,34 @@ namespace {
 template <typename T>
 const T *ptr_or_null(const std::vector<T> &v) { return v.empty() ? nullptr : v.data(); }
 
+// The array shapes a beam's vectors must agree on (wrapper:1113-1150); nullptr when they do.
+const char *shape_error(const RTDBeamSettings &s) {
+    const size_t L = s.energies.size();
+    if (!s.layerSpotCounts.empty()) {
+        if (s.layerSpotCounts.size() != L) return "layerSpotCounts needs one count per energy layer";
+        long total = 0;
+        for (int c : s.layerSpotCounts) total += c;
+        if (!s.spotWeights.empty() && size_t(total) != s.spotWeights.size())
+            return "the layer spot counts do not add up to the number of spot weights";
+        if (!s.spotPositions.empty()) {
+            if (s.spotPositions.size() % 2) return "spotPositions holds (x, y) pairs: its length must be even";
+            if (size_t(total) != s.spotPositions.size() / 2)
+                return "the layer spot counts do not add up to the number of spot positions";
+        }
+    }
+    if (!s.spotBeamDirections.empty()) {
+        if (s.spotBeamDirections.size() % 3) return "spotBeamDirections holds 3-vectors: its length must be a multiple of 3";
+        if (!s.spotPositions.empty() && s.spotBeamDirections.size() / 3 != s.spotPositions.size() / 2)
+            return "spotBeamDirections needs one direction per spot position";
+    }
+    if (!s.layerLongitudinalCutoffs.empty() && s.layerLongitudinalCutoffs.size() != L)
+        return "layerLongitudinalCutoffs needs one value per energy layer";
+    if (!s.profileData.empty() && s.profileSetting.size() < 3)
+        return "profileData needs profileSetting = (first depth, depth step, depth count)";
+    if (s.beamParaData.size() % 3) return "beamParaData holds rows of 3 values: its length must be a multiple of 3";
+    return nullptr;
+}
+
 }  // namespace
 
 extern "C" void subsecondWrapper(const float *ctData, const int3 &ctDims, const float3 &ctResolution,
@@ -79,6 +107,13 @@ extern "C" void subsecondWrapper(const float *ctData, const int3 &ctDims, const 
         }
         if (s.subspotData.size() != s.energies.size() * size_t(s.maxSubspotsPerLayer) * 5) {
             std::fprintf(stderr, "[RTD_ENTRY_ASSERT] beamIdx=%zu beam.subspotData size must equal numLayers*maxSubspotsPerLayer*5\n", i);
+            return;
+        }
+        // the vector-size checks of the entry validation (wrapper:1113-1150) that the POD form
+        // cannot make, as it carries no sizes for these arrays
+        const char *bad = shape_error(s);
+        if (bad) {
+            std::fprintf(stderr, "[RTD_ENTRY_ASSERT] beamIdx=%zu %s\n", i, bad);
             return;
         }
     }

@@ -249,6 +249,10 @@ struct pbs_engine {
     // adjoint with respect to the spot weights: dJ/dIDD x IDD / W per (layer, step, ray), dJ/dW,
     // the transposed convolution's intermediate and the spot-lattice gradient
     DevBuf<float> adjI, adjW, adjInterm, adjSpot;
+    // fixed-point dJ/dBEV of the transposed transfer (main planes, then tail planes) and the
+    // float bits of max |dJ/dDose| that fix its scale
+    DevBuf<unsigned long long> adjFix;
+    DevBuf<unsigned> adjMax;
     // CarbonPBS profile hooks: the profile table and the per-layer row / beam parameters
     DevBuf<float> profTable;
     DevBuf<float4> profLayer;
@@ -464,6 +468,11 @@ int validate_beam(const pbs_beam_settings &b, size_t idx) {
         return set_error(PBS_E_INVALID, pre + "water_equivalence needs its 9-value header");
     if (b.spotPositionsAreIndices && (!b.waterEquivalence || b.waterEquivalenceCount < 9))
         return set_error(PBS_E_INVALID, pre + "spotPositionsAreIndices requires a 9-value WEQ header");
+    // the profile model's shapes (wrapper:1143-1148)
+    if (b.profileData && b.profileDataCount > 0 && (!b.profileSetting || b.numProfileSetting < 3))
+        return set_error(PBS_E_INVALID, pre + "profileData needs profileSetting = (first depth, depth step, depth count)");
+    if (b.beamParaData && b.numBeamParaData % 3)
+        return set_error(PBS_E_INVALID, pre + "beamParaData holds rows of 3 values: its length must be a multiple of 3");
     return PBS_OK;
 }
 
@@ -874,8 +883,9 @@ static int run_grid_adjoint(pbs_engine *e, const BeamPlan &p, const FanGeom &g, 
                                       w.classCount.p, w.classList.p, w.maxUnits, nullptr, 1, w.planeRM.p,
                                       w.slotLayers.p, s));
     const size_t bevPlane = size_t(g.sx + 64) * (g.sy + 64);
-    HIP_TRY(launch_adj_zero(w.bev.p, size_t(g.K) * bevPlane, s));
-    HIP_TRY(launch_adj_zero(w.tailBev.p, size_t(w.tailCap) * bevPlane, s));
+    const size_t mainN = size_t(g.K) * bevPlane, tailN = size_t(w.tailCap) * bevPlane;
+    HIP_TRY(e->adjFix.ensure(mainN + tailN));
+    HIP_TRY(e->adjMax.ensure(1));
     TransferArgs ta{};
     fill_transfer_args(p, g, ta);
     int gx = 1, gy = 1, gz = 1;
@@ -883,8 +893,9 @@ static int run_grid_adjoint(pbs_engine *e, const BeamPlan &p, const FanGeom &g, 
     int depthAxis = 0;
     for (int a = 1; a < 3; ++a)
         if (std::fabs(g.d2g[6 + a]) > std::fabs(g.d2g[6 + depthAxis])) depthAxis = a;
-    HIP_TRY(launch_adj_transfer(g, ta, bs, w.groupScal.p, gradDose, w.bev.p, w.tailBev.p, gx * 32, gy * 8, gz,
-                                depthAxis, s));
+    const size_t doseN = size_t(e->doseDims[0]) * e->doseDims[1] * e->doseDims[2];
+    HIP_TRY(launch_adj_transfer(g, ta, bs, w.groupScal.p, gradDose, doseN, w.bev.p, w.tailBev.p, mainN, tailN,
+                                e->adjFix.p, e->adjMax.p, gx * 32, gy * 8, gz, depthAxis, s));
     const size_t nray = size_t(g.rx) * g.ry;
     HIP_TRY(e->adjI.ensure(size_t(g.numLayers) * g.K * nray));
     HIP_TRY(e->adjW.ensure(size_t(g.numLayers) * nray));

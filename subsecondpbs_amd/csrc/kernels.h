@@ -129,8 +129,10 @@ hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamS
 // adjoint with respect to the spot weights (adjoint.hip)
 hipError_t launch_adj_zero(float *v, size_t n, hipStream_t s);
 hipError_t launch_adj_transfer(const FanGeom &g, const TransferArgs &ta, const BeamScalars *bs,
-                               const GroupScalars *gs, const float *gDose, float *gBev, float *gTailBev, int extentX,
-                               int extentY, int extentZ, int depthAxis, hipStream_t s);
+                               const GroupScalars *gs, const float *gDose, size_t doseN, float *gBev,
+                               float *gTailBev, size_t mainN, size_t tailN, unsigned long long *fix,
+                               unsigned *maxBits, int extentX, int extentY, int extentZ, int depthAxis,
+                               hipStream_t s);
 hipError_t launch_adj_superpose(const FanGeom &g, const float2 *iddSigma, const float *rayW, const GroupScalars *gs,
                                 const BeamScalars *bs, const int *classCount, const int2 *classList, int maxUnits,
                                 const int *slotLayers, const float *gBev, const float *gTailBev, float *gI,

@@ -1,7 +1,8 @@
 """Adjoint of the HIP engine with respect to the spot weights (config-4 case: config-1 beam
 with a target / OAR objective).
 
-Checks: the GPU gradient against the FP64 oracle's adjoint; <g, dose(w)> = <w, A^T g> on the
+Checks: the GPU gradient against the FP64 oracle's adjoint; bitwise repeatability;
+<g, dose(w)> = <w, A^T g> on the
 GPU (the dose is linear in the weights for fixed masks); central finite differences of the GPU
 forward on 64 seeded spots with h = 1e-3 w; torch.autograd through SpotWeightDose; a few
 descent steps lower the objective.
@@ -59,6 +60,21 @@ def test_adjoint_matches_fp64_oracle(cfg4):
     err = np.abs(gw - ref).max() / np.abs(ref).max()
     print(f"adjoint vs FP64 oracle: max|dg|/max|g| = {err:.3e}")
     assert err < 2e-3
+
+
+@gpu
+def test_adjoint_bitwise_repeatable(cfg4):
+    """The transposed transfer sums in 64-bit fixed point, so the order of its atomics does
+    not reach the result: repeated adjoints are bitwise equal. A scaled gradient (another
+    fixed-point scale) is repeatable too, and scales the result."""
+    case, beam, eng, dose, g = cfg4
+    a = eng.beam_adjoint(beam, g)
+    for _ in range(3):
+        assert np.array_equal(eng.beam_adjoint(beam, g), a)
+    g4 = (4.0 * g).contiguous()
+    b = eng.beam_adjoint(beam, g4)
+    assert np.array_equal(eng.beam_adjoint(beam, g4), b)
+    np.testing.assert_allclose(b, 4.0 * a, rtol=1e-6, atol=1e-6 * float(np.abs(a).max()))
 
 
 @gpu

@@ -218,3 +218,21 @@ def test_hooks_change_the_oracle_dose():
         d, p = O.beam_dose(c)
         assert p.profileHooks == hooks
         assert np.array_equal(d, ref) == same, hooks
+
+
+@pytest.mark.parametrize("bad", ["setting", "para"])
+def test_profile_shapes_checked_at_entry(bad):
+    """The entry validation's profile-model shapes (raytracedicom_wrapper.cu:1143-1148): a
+    profile table needs the three-value depth setting, and beam parameters come in rows of 3."""
+    case = F.make_config1()
+    b = dict(case["beams"][0])
+    L = len(b["energies"])
+    b = profile_beam(b, L, hooks=0)
+    if bad == "setting":
+        b["profile_setting"] = np.asarray(b["profile_setting"], np.float32)[:2]
+        needle = "profileSetting"
+    else:
+        b["beam_para_data"] = np.asarray(b["beam_para_data"], np.float32).ravel()[:-1]
+        needle = "multiple of 3"
+    with pytest.raises(RuntimeError, match=needle):
+        _abi.plan_debug(case, b)
