@@ -305,11 +305,24 @@ int pbs_batch_set_ct(pbs_batch *b, const float *ct, const int ctDims[3], const f
 int pbs_batch_set_dose_grid(pbs_batch *b, const int doseDims[3], const float doseRes[3],
                             const float doseCorner[3]);
 int pbs_batch_set_nuclear(pbs_batch *b, int model);
-/* Sum of the doses of `numBeams` beams, accumulated into doseDevice: lane i takes beams
- * i, i + lanes, ...; each lane accumulates into its own buffer and the lanes are added in
- * lane order, so the result depends only on the beam list and the lane count. Returns when
- * the work is queued; pbs_batch_sync waits. */
+/* Sum of the doses of `numBeams` beams, accumulated into doseDevice. Spot-lattice beams on the
+ * CT run as one multi-beam pass per stage of the chain (trace, upload, setup, convolution, fill,
+ * tiles, each radius class, merge; the transfers last, adding the beams voxel by voxel in beam
+ * order), so the dose equals the one-beam-at-a-time sum bitwise. A batch holding any other beam
+ * (adjoint, resident, profile hooks, WEQ volume, legacy CPB) falls back to the lanes: lane i
+ * takes beams i, i + lanes, ... into its own buffer and the lanes are added in lane order.
+ * Returns when the work is queued; pbs_batch_sync waits. */
 int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDevice);
+/* Another energy-table set (e.g. carbon beside protons) with its nuclear model (0 off, 1
+ * GAUSS_FIT, 2 SOUKUP, 3 FLUKA); returns its id (>= 1; the creation's set is 0) or a negative
+ * status. */
+int pbs_batch_add_table(pbs_batch *b, const pbs_energy_struct *energy, int nuclearModel);
+/* pbs_batch_dose's multi-beam form with each beam's table set (tableIds NULL: all 0); beams of
+ * every table set share the passes. PBS_E_UNSUPPORTED for a beam the passes do not cover. */
+int pbs_batch_dose_tables(pbs_batch *b, const pbs_beam_settings *beams, const int *tableIds, size_t numBeams,
+                          float *doseDevice);
+/* Launches the last multi-beam call issued (0 when the lanes ran it). */
+int pbs_batch_last_launches(pbs_batch *b);
 /* Serving: queue one beam whose dose is accumulated into its own doseDevice buffer, on the
  * next lane (round robin). *ticket identifies the request for pbs_batch_query / wait. */
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDevice, long long *ticket);

@@ -1,10 +1,13 @@
 """Batched and served forms of the final-dose engine on one GPU.
 
-``Batch`` owns several engine lanes (own HIP stream and workspaces) sharing one resident CT
-and LUT set; the beams of a batch are dealt to the lanes and overlap on the device, and the
-lanes' doses are added in a fixed order (the result depends only on the beam list and the
-lane count). ``submit`` is the served form: each request gets its own dose buffer, so a
-request's result is the same alone and among others.
+``Batch.dose`` runs a batch of beams as one multi-beam launch per pass of the chain (trace,
+upload, setup, convolution, fill, tiles, each radius class, merge, and one transfer for all
+beams that adds them voxel by voxel in beam order): the dose equals the one-beam-at-a-time sum
+bitwise, and the launches do not grow with the beam count. Beams of several energy-table sets
+(``add_table``: carbon beside protons) share the passes. A batch holding beams the multi-beam
+passes do not cover runs on the lanes instead (own HIP stream and workspaces each, sharing the
+resident CT; lane doses added in a fixed order). ``submit`` is the served form: each request
+gets its own dose buffer, so a request's result is the same alone and among others.
 """
 from __future__ import annotations
 
@@ -64,15 +67,41 @@ class Batch:
     def set_nuclear(self, model: int):
         _abi.check(_abi.lib().pbs_batch_set_nuclear(self.handle, int(model)), "pbs_batch_set_nuclear")
 
-    def dose(self, beams: list[dict], dose):
-        """Queue the sum of the beams' doses into ``dose`` (device tensor [nz][ny][nx])."""
+    def add_table(self, energy: dict, nuclear: int = 0) -> int:
+        """Another energy-table set (e.g. carbon beside protons) and its nuclear model; returns
+        its id for ``dose(..., tables=...)`` (the set given at creation is 0)."""
+        es = _abi.energy_struct(energy, self._keep)
+        rc = _abi.lib().pbs_batch_add_table(self.handle, C.byref(es), int(nuclear))
+        if rc < 0:
+            _abi.check(rc, "pbs_batch_add_table")
+        return int(rc)
+
+    def prepare(self, beams: list[dict]):
+        """The beams' C structs, built once (a caller that repeats a batch keeps them)."""
         keep = _abi.Keep()
         arr = (_abi.BeamSettings * len(beams))()
         for i, b in enumerate(beams):
             arr[i] = _abi.beam_settings(b, keep)
-        torch_ready()  # the lanes' streams read and write it next
-        _abi.check(_abi.lib().pbs_batch_dose(self.handle, arr, len(beams), C.c_void_p(dose.data_ptr())),
-                   "pbs_batch_dose")
+        return arr, keep
+
+    def dose(self, beams, dose, tables=None):
+        """Queue the sum of the beams' doses into ``dose`` (device tensor [nz][ny][nx]).
+        ``beams``: beam dicts or the result of ``prepare``; ``tables``: each beam's table-set id."""
+        arr, keep = beams if isinstance(beams, tuple) else self.prepare(beams)
+        n = len(arr)
+        torch_ready()  # the batch's streams read and write it next
+        if tables is None:
+            rc = _abi.lib().pbs_batch_dose(self.handle, arr, n, C.c_void_p(dose.data_ptr()))
+            _abi.check(rc, "pbs_batch_dose")
+        else:
+            t = (C.c_int * n)(*[int(x) for x in tables])
+            rc = _abi.lib().pbs_batch_dose_tables(self.handle, arr, t, n, C.c_void_p(dose.data_ptr()))
+            _abi.check(rc, "pbs_batch_dose_tables")
+
+    @property
+    def last_launches(self) -> int:
+        """Kernel launches of the last multi-beam call (0 when the lanes ran it)."""
+        return int(_abi.lib().pbs_batch_last_launches(self.handle))
 
     def submit(self, beam: dict, dose) -> int:
         """Queue one beam into its own ``dose`` buffer; returns a ticket."""

@@ -504,6 +504,22 @@ int pbs_device_count(void) {
 static std::mutex g_finalDoseMutex;
 static pbs_engine *g_finalDoseEngine = nullptr;
 
+// Passes of one beam's chain in the multi-beam batch (the record key is stage * 64 + the
+// launch's index within the stage); a beam's records are in increasing key order.
+enum : int {
+    kStageTrace = 1,
+    kStageUpload = 2,  // and 3
+    kStageSetup = 4,
+    kStageConv = 5,
+    kStageFill = 6,
+    kStageGrid = 7,
+    kStageHaloSeed = 8,
+    kStageHaloGrid = 9,
+};
+static void rec_stage(int st) {
+    if (t_rec) t_rec->set_stage(st);
+}
+
 // An engine with one stream (oneStream) creates no side streams: the runtime maps streams onto
 // a few hardware queues, and idle side streams of many engines would put their main streams on
 // the same queue, where unrelated work serialises.
@@ -840,7 +856,7 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
                                       w.classCount.p, w.classList.p, w.maxUnits, overflowFlag, codesFromFill ? 1 : 0,
                                       w.planeRM.p, w.slotLayers.p, s));
     HIP_TRY(mk());
-    if (e->oneStream) {
+    if (e->oneStream || t_rec) {
         const hipStream_t one[1 + kSideStreams] = {s, s, s, s};
         HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
                                  w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, e->tuneSuperposeGrid,
@@ -974,6 +990,9 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         return set_error(PBS_E_INVALID, halo ? "resident beams do not cover the nuclear halo branch"
                                              : "resident beams need spot positions and weights (spot-lattice path)");
     const bool hooks = p.profileHooks != 0;
+    if (t_rec && (adjoint || keep || hooks || p.weqVolume || p.legacyPath || t_arena))
+        return set_error(PBS_E_UNSUPPORTED, "the multi-beam batch covers spot-lattice beams on a CT (no adjoint, "
+                                            "resident state, profile hooks, WEQ volume or legacy CPB path)");
     if (hooks && (adjoint || keep))
         return set_error(PBS_E_INVALID, "the CarbonPBS profile hooks are not covered by the adjoint or resident beams");
     if (halo) {
@@ -1050,6 +1069,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(launch_weq_fill(g, wa, e->density.p, e->wepl.p, e->fi.p, e->fo.p, s));
         HIP_TRY(launch_trace_reduce(g, e->wepl.p, e->fi.p, e->fo.p, e->weplMin.p, e->beamScal.p, s));
     } else {
+        rec_stage(kStageTrace);
         HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p,
                              e->weplMin.p, e->beamScal.p, s));
     }
@@ -1101,6 +1121,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     if (halo) {
         add_upload(up, stageSrc, e->rayToNuc.p, oRay, bRay);
         add_upload(up, stageSrc, e->haloW.p, oHaloW, bHaloW);
+        rec_stage(kStageUpload);
         HIP_TRY(launch_clear_pairs(e->haloIddSigma.p, size_t(hf.nucPlane) * K * L, s));
     }
     if (hooks) {
@@ -1109,7 +1130,9 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         add_upload(up, stageSrc, e->profTable.p, oProf, bProf);
         add_upload(up, stageSrc, e->profLayer.p, oProfL, bProfL);
     }
+    rec_stage(kStageUpload + 1);
     HIP_TRY(launch_upload(up, s));
+    rec_stage(kStageSetup);
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
@@ -1138,6 +1161,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         // the dense spot lattice went up with the beam's other arrays (k_upload above)
         HIP_TRY(e->interm.ensure(size_t(L) * nSub * p.spotDims[1] * rx));
         const ConvArgs a = make_conv_args(p, rx, ry, nSub, subMode);
+        rec_stage(kStageConv);
         HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, e->subspots.p,
                             e->subEntry.p, L, s));
         e->convArgs = a;
@@ -1163,8 +1187,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(launch_cpb(a, e->subspots.p, e->cpb.p, e->rayW.p, rx, ry, p.cpbCorner[0], p.cpbCorner[1], p.cpbRes[0],
                            p.cpbRes[1], L, s));
     }
-    HIP_TRY(hipEventRecord(e->uploadDone, s));
-    e->uploadPending = true;
+    if (!t_rec) {  // the batch executor records it after the batch's launches
+        HIP_TRY(hipEventRecord(e->uploadDone, s));
+        e->uploadPending = true;
+    }
     // 3. IDD / sigma (+ the halo's share when the halo branch is on; idd_sigma.cu:108-121)
     HIP_TRY(mark());
     FillConsts fc = fill_consts(*e);
@@ -1181,6 +1207,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, keep->unitIdd.p,
                                keep->unitFp.p, e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
     } else {
+        rec_stage(kStageFill);
         HIP_TRY(launch_fill(g, e->lut, e->layerHost.p, e->layerScal.p, e->beamScal.p, e->density.p, e->wepl.p,
                             e->rayW.p, e->fi.p, e->fo.p, e->iddSigma.p, e->firstPassive.p, fc, hf, e->prim.tileRad.p,
                             s));
@@ -1223,14 +1250,17 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     // 4-6. primary grid: passive steps, tiles, bins, items; superposition; merge; transfer
     hipStream_t streams[1 + kSideStreams] = {s};
     for (int i = 0; i < kSideStreams; ++i) streams[1 + i] = e->oneStream ? s : e->side[i];
+    rec_stage(kStageGrid);
     int rc2 = run_grid(e, p, g, e->prim, e->iddSigma.p, e->layerScal.p, e->beamScal.p, nullptr, !hooks, doseDev,
                        streams, &marks);
     if (rc2 != PBS_OK) return rc2;
     // 7. halo grid through the same stages
     HIP_TRY(mark());
     if (halo) {
+        rec_stage(kStageHaloSeed);
         HIP_TRY(launch_seed_grid(L, e->layerScal.p, e->beamScal.p, e->haloLayerScal.p, e->haloBeamScal.p,
                                  e->halo.hist.p, e->halo.classCount.p, s));
+        rec_stage(kStageHaloGrid);
         rc2 = run_grid(e, p, gh, e->halo, e->haloIddSigma.p, e->haloLayerScal.p, e->haloBeamScal.p, e->haloErr.p,
                        false, doseDev, streams, nullptr);
         if (rc2 != PBS_OK) return rc2;
@@ -1788,17 +1818,60 @@ int cudaFinalDose(const float *subspotData, const vec3i *roiIndices, int numROII
 // ---------------------------------------------------------------------------------------
 }  // extern "C"
 
+// Deep host copy of an energy table set: the multi-beam batch creates the engines of a table
+// set when a batch needs more of them, after the caller's arrays may have gone.
+struct HostEnergy {
+    pbs_energy_struct e{};
+    std::vector<float> energies, peaks, scales, cidd, dens, sp, rrl, nucW, nucS2;
+    explicit HostEnergy(const pbs_energy_struct &src) : e(src) {
+        const size_t nC = size_t(src.nEnergies) * size_t(src.nEnergySamples);
+        auto cp = [](std::vector<float> &v, const float *q, size_t n) -> const float * {
+            if (!q) return nullptr;
+            v.assign(q, q + n);
+            return v.data();
+        };
+        e.energiesPerU = cp(energies, src.energiesPerU, size_t(src.nEnergies));
+        e.peakDepths = cp(peaks, src.peakDepths, size_t(src.nEnergies));
+        e.scaleFacts = cp(scales, src.scaleFacts, size_t(src.nEnergies));
+        e.ciddMatrix = cp(cidd, src.ciddMatrix, nC);
+        e.densityVector = cp(dens, src.densityVector, size_t(src.nDensitySamples));
+        e.spVector = cp(sp, src.spVector, size_t(src.nSpSamples));
+        e.rRlVector = cp(rrl, src.rRlVector, size_t(src.nRRlSamples));
+        e.nucWeightMatrix = cp(nucW, src.nucWeightMatrix, nC);
+        e.nucSqSigmaMatrix = cp(nucS2, src.nucSqSigmaMatrix, nC);
+    }
+};
+
 struct pbs_batch {
     int device = 0;
     std::vector<pbs_engine *> lanes;
     pbs_engine *owner = nullptr;  // holds the CT every lane borrows
-    DevBuf<float> laneDose;  // [lanes][dose voxels] per-lane accumulators of pbs_batch_dose
+    DevBuf<float> laneDose;  // [lanes][dose voxels] per-lane accumulators of the lane form
     size_t doseN = 0;
     size_t next = 0;               // round-robin cursor of the served form
     long long tickets = 0;
     std::vector<std::pair<long long, hipEvent_t>> pending;
     std::vector<hipEvent_t> freeEvents;
     hipEvent_t laneDone[16] = {};
+    // multi-beam form: per energy-table set its host copy, nuclear model and the engines that
+    // hold one beam's buffers each; the batch stream and the launch table of its passes
+    std::vector<std::unique_ptr<HostEnergy>> tables;
+    std::vector<int> tableNuclear;
+    std::vector<std::vector<pbs_engine *>> slots;
+    hipStream_t stream = nullptr;
+    DevBuf<unsigned char> devTable;
+    unsigned char *hostTable = nullptr;  // pinned
+    size_t hostTableBytes = 0;
+    hipEvent_t tableDone = nullptr;  // the launch table's copy of the last multi-beam call
+    bool tablePending = false;
+    bool multiUsed = false;
+    int lastLaunches = 0;
+    bool haveCt = false;
+    int ctDims[3] = {};
+    float ctRes[3] = {}, ctCorner[3] = {};
+    bool haveDose = false;
+    int doseDims[3] = {};
+    float doseRes[3] = {}, doseCorner[3] = {};
 };
 
 namespace {
@@ -1829,6 +1902,15 @@ pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lane
     b->device = gpuId;
     b->owner = engine_create(gpuId, energy, true);
     if (!b->owner) return nullptr;
+    b->tables.emplace_back(new HostEnergy(*energy));
+    b->tableNuclear.push_back(0);
+    b->slots.emplace_back();
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->tableDone, hipEventDisableTiming) != hipSuccess) {
+        set_error(PBS_E_DEVICE, "hipStreamCreate failed");
+        pbs_engine_destroy(b->owner);
+        return nullptr;
+    }
     for (int i = 0; i < lanes; ++i) {
         pbs_engine *e = engine_create(gpuId, energy, true);
         if (!e) {
@@ -1848,8 +1930,14 @@ pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lane
 void pbs_batch_destroy(pbs_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (pbs_engine *e : b->lanes) pbs_engine_destroy(e);
+    for (auto &v : b->slots)
+        for (pbs_engine *e : v) pbs_engine_destroy(e);
     pbs_engine_destroy(b->owner);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->tableDone) (void)hipEventDestroy(b->tableDone);
+    if (b->hostTable) (void)hipHostFree(b->hostTable);
     for (auto &p : b->pending) (void)hipEventDestroy(p.second);
     for (hipEvent_t ev : b->freeEvents) (void)hipEventDestroy(ev);
     for (int i = 0; i < 16; ++i)
@@ -1868,6 +1956,13 @@ int pbs_batch_set_ct(pbs_batch *b, const float *ct, const int dims[3], const flo
         rc = pbs_engine_set_ct(e, b->owner->ctPtr, dims, res, corner, 2);
         if (rc != PBS_OK) return rc;
     }
+    for (auto &v : b->slots)
+        for (pbs_engine *e : v) {
+            rc = pbs_engine_set_ct(e, b->owner->ctPtr, dims, res, corner, 2);
+            if (rc != PBS_OK) return rc;
+        }
+    b->haveCt = true;
+    for (int i = 0; i < 3; ++i) { b->ctDims[i] = dims[i]; b->ctRes[i] = res[i]; b->ctCorner[i] = corner[i]; }
     return PBS_OK;
 }
 
@@ -1877,7 +1972,14 @@ int pbs_batch_set_dose_grid(pbs_batch *b, const int dims[3], const float res[3],
         const int rc = pbs_engine_set_dose_grid(e, dims, res, corner);
         if (rc != PBS_OK) return rc;
     }
+    for (auto &v : b->slots)
+        for (pbs_engine *e : v) {
+            const int rc = pbs_engine_set_dose_grid(e, dims, res, corner);
+            if (rc != PBS_OK) return rc;
+        }
     b->doseN = size_t(dims[0]) * dims[1] * dims[2];
+    b->haveDose = true;
+    for (int i = 0; i < 3; ++i) { b->doseDims[i] = dims[i]; b->doseRes[i] = res[i]; b->doseCorner[i] = corner[i]; }
     return PBS_OK;
 }
 
@@ -1887,11 +1989,15 @@ int pbs_batch_set_nuclear(pbs_batch *b, int model) {
         const int rc = pbs_engine_set_nuclear(e, model);
         if (rc != PBS_OK) return rc;
     }
+    for (pbs_engine *e : b->slots[0]) {
+        const int rc = pbs_engine_set_nuclear(e, model);
+        if (rc != PBS_OK) return rc;
+    }
+    b->tableNuclear[0] = model;
     return PBS_OK;
 }
 
-int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDev) {
-    if (!b || !beams || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+static int batch_dose_lanes(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDev) {
     if (b->doseN == 0) return set_error(PBS_E_INVALID, "no dose grid bound to the batch");
     HIP_TRY(hipSetDevice(b->device));
     const int L = int(b->lanes.size());
@@ -1922,6 +2028,111 @@ int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams
     HIP_TRY(hipGetLastError());
     return PBS_OK;
 }
+
+// Engine j of table set t for a multi-beam call (created on first use and bound to the batch's
+// CT, dose grid and the table's nuclear model).
+static pbs_engine *batch_slot(pbs_batch *b, int t, size_t j) {
+    std::vector<pbs_engine *> &v = b->slots[size_t(t)];
+    while (v.size() <= j) {
+        pbs_engine *e = engine_create(b->device, &b->tables[size_t(t)]->e, true);
+        if (!e) return nullptr;
+        v.push_back(e);
+        int rc = PBS_OK;
+        if (b->haveCt) rc = pbs_engine_set_ct(e, b->owner->ctPtr, b->ctDims, b->ctRes, b->ctCorner, 2);
+        if (rc == PBS_OK && b->haveDose) rc = pbs_engine_set_dose_grid(e, b->doseDims, b->doseRes, b->doseCorner);
+        if (rc == PBS_OK) rc = pbs_engine_set_nuclear(e, b->tableNuclear[size_t(t)]);
+        if (rc != PBS_OK) return nullptr;
+    }
+    return v[j];
+}
+
+// The multi-beam form: every beam's chain is recorded on its own engine (planning, buffers and
+// staging as for one beam, no launch), then the batch stream issues each pass once for all
+// beams (run_records): the launches of a batch do not grow with its beam count.
+static int batch_dose_multi(pbs_batch *b, const pbs_beam_settings *beams, const int *tableIds, size_t numBeams,
+                            float *doseDev) {
+    if (!b->haveCt) return set_error(PBS_E_INVALID, "no CT bound to the batch");
+    HIP_TRY(hipSetDevice(b->device));
+    Recorder rec;
+    std::vector<size_t> used(b->tables.size(), 0);
+    std::vector<pbs_engine *> engines;
+    for (size_t i = 0; i < numBeams; ++i) {
+        const int t = tableIds ? tableIds[i] : 0;
+        if (t < 0 || size_t(t) >= b->tables.size())
+            return set_error(PBS_E_INVALID, "beam " + std::to_string(i) + ": unknown table set " + std::to_string(t));
+        pbs_engine *e = batch_slot(b, t, used[size_t(t)]++);
+        if (!e) return PBS_E_DEVICE;
+        engines.push_back(e);
+        rec.beam = int(i);
+        rec.seq = 0;
+        rec.set_stage(0);
+        t_rec = &rec;
+        const int rc = engine_beam_dose(e, &beams[i], doseDev, nullptr, nullptr);
+        t_rec = nullptr;
+        if (rc != PBS_OK) return rc;
+    }
+    const size_t need = records_table_bytes(rec.recs);
+    if (b->tablePending) {
+        HIP_TRY(hipEventSynchronize(b->tableDone));
+        b->tablePending = false;
+    }
+    if (need > b->hostTableBytes) {
+        // the device table of earlier calls may still be read: grow only on an idle stream
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        if (b->hostTable) (void)hipHostFree(b->hostTable);
+        b->hostTable = nullptr;
+        b->hostTableBytes = 0;
+        const size_t cap = std::max(need + need / 2, size_t(1) << 16);
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&b->hostTable), cap));
+        b->hostTableBytes = cap;
+        HIP_TRY(b->devTable.ensure(cap));
+    }
+    int launches = 0;
+    HIP_TRY(run_records(rec.recs, b->hostTable, b->devTable.p, b->stream, &launches));
+    HIP_TRY(hipEventRecord(b->tableDone, b->stream));
+    b->tablePending = true;
+    // each engine's staging is read by this batch's upload pass: its next beam waits for it
+    for (pbs_engine *e : engines) {
+        HIP_TRY(hipEventRecord(e->uploadDone, b->stream));
+        e->uploadPending = true;
+    }
+    b->lastLaunches = launches;
+    b->multiUsed = true;
+    return PBS_OK;
+}
+
+int pbs_batch_dose(pbs_batch *b, const pbs_beam_settings *beams, size_t numBeams, float *doseDev) {
+    if (!b || !beams || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (b->doseN == 0) return set_error(PBS_E_INVALID, "no dose grid bound to the batch");
+    b->lastLaunches = 0;
+    const int rc = batch_dose_multi(b, beams, nullptr, numBeams, doseDev);
+    if (rc != PBS_E_UNSUPPORTED) return rc;
+    return batch_dose_lanes(b, beams, numBeams, doseDev);  // beams the multi-beam passes do not cover
+}
+
+int pbs_batch_add_table(pbs_batch *b, const pbs_energy_struct *energy, int nuclearModel) {
+    if (!b || !energy) return set_error(PBS_E_INVALID, "null argument");
+    const int rc = validate_energy(energy);
+    if (rc != PBS_OK) return rc;
+    if (nuclearModel < 0 || nuclearModel > 3)
+        return set_error(PBS_E_INVALID, "nuclearModel must be 0 (off), 1 (GAUSS_FIT), 2 (SOUKUP) or 3 (FLUKA)");
+    if (nuclearModel && !(energy->nucWeightMatrix && energy->nucSqSigmaMatrix))
+        return set_error(PBS_E_GEOMETRY, "a nuclear model needs the table set's nuclear tables");
+    b->tables.emplace_back(new HostEnergy(*energy));
+    b->tableNuclear.push_back(nuclearModel);
+    b->slots.emplace_back();
+    return int(b->tables.size()) - 1;
+}
+
+int pbs_batch_dose_tables(pbs_batch *b, const pbs_beam_settings *beams, const int *tableIds, size_t numBeams,
+                          float *doseDev) {
+    if (!b || !beams || !doseDev) return set_error(PBS_E_INVALID, "null argument");
+    if (b->doseN == 0) return set_error(PBS_E_INVALID, "no dose grid bound to the batch");
+    b->lastLaunches = 0;
+    return batch_dose_multi(b, beams, tableIds, numBeams, doseDev);
+}
+
+int pbs_batch_last_launches(pbs_batch *b) { return b ? b->lastLaunches : 0; }
 
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev, long long *ticket) {
     if (!b || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");
@@ -1976,6 +2187,17 @@ int pbs_batch_sync(pbs_batch *b) {
     for (pbs_engine *e : b->lanes) {
         const int r = pbs_engine_sync(e);
         if (r != PBS_OK && rc == PBS_OK) rc = r;
+    }
+    if (b->multiUsed) {
+        // the multi-beam passes ran on the batch stream; then each engine's halo check
+        HIP_TRY(hipSetDevice(b->device));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        b->tablePending = false;
+        for (auto &v : b->slots)
+            for (pbs_engine *e : v) {
+                const int r = pbs_engine_sync(e);
+                if (r != PBS_OK && rc == PBS_OK) rc = r;
+            }
     }
     for (auto &p : b->pending) b->freeEvents.push_back(p.second);
     b->pending.clear();

@@ -3,9 +3,61 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "device.h"
 
 namespace pbs {
+
+// ---------------------------------------------------------------------------------------
+// Multi-beam launches
+// ---------------------------------------------------------------------------------------
+// Every kernel of the dose chain is a body that runs one beam's grid. Called alone it is
+// launched as usual. In the batched form the engine runs with a Recorder installed: each launch
+// becomes a LaunchRec (the kernel's arguments, grid and LDS), and the batch executor
+// (run_records) issues the records of one pass for all beams of the batch as a single launch in
+// which every work-group finds its beam in a table of first work-groups.
+struct MultiTable {
+    const unsigned *base;  // [nb + 1]: first work-group of each beam's grid; base[nb] = total
+    const uint4 *dims;     // [nb]: the beam's own grid (x, y, z)
+    int nb;
+};
+using OneFn = void (*)(const void *args, dim3 grid, size_t lds, hipStream_t s);
+using ManyFn = void (*)(const void *args, const MultiTable &t, unsigned blocks, size_t lds, hipStream_t s);
+struct LaunchRec {
+    int key;   // pass position: stage * 64 + launch within the stage; equal keys of different beams merge
+    int beam;  // the beam's index in the batch
+    int seq;   // the record's index within its beam
+    OneFn one;
+    ManyFn many;
+    std::vector<unsigned char> args;  // the kernel's Args
+    size_t argAlign;
+    dim3 grid;
+    size_t lds;
+    unsigned manyBlocks;  // work-groups of the multi-beam launch; 0: the beams' grids concatenated
+    const void *target;   // records that share a launch share this too (the transfers' dose grid)
+};
+struct Recorder {
+    int beam = 0, stage = 0, sub = 0, seq = 0;
+    std::vector<LaunchRec> recs;
+    void set_stage(int st) {
+        stage = st;
+        sub = 0;
+    }
+};
+// Installed by the batch executor around the recording of one beam (nullptr: launch directly).
+extern thread_local Recorder *t_rec;
+// Passes that every beam's transfers share: all of them go into one launch after every other pass,
+// in beam order (main grid before halo grid), so each voxel adds the beams in the order the
+// one-beam-at-a-time form does.
+constexpr int kTransferKey = 1 << 24;
+// Bytes of the device table run_records needs for `recs` (arguments, first work-groups, grids).
+size_t records_table_bytes(const std::vector<LaunchRec> &recs);
+// Issues `recs` on `s`: records of equal key and kernel as one multi-beam launch, in key order.
+// The table is written to the pinned host buffer `hostTable` and copied to `devTable` (both at
+// least records_table_bytes). Returns the number of launches in *launches.
+hipError_t run_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, unsigned char *devTable,
+                       hipStream_t s, int *launches);
 
 constexpr int kMaxRadDev = 32;
 constexpr int kMinTilesDev = 16;

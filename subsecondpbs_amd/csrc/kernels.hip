@@ -28,8 +28,12 @@
 //                        transfer box also sample that group's tail planes
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <cstring>
+#include <functional>
+#include <type_traits>
 
 #include "device.h"
 #include "kernels.h"
@@ -144,7 +148,7 @@ constexpr int kTraceSeg = 8;
 // (texel centre -1/2, border 0), density and SP from the LUTs in LDS, the segment's Kahan
 // prefix of stepLen * SP (written to wepl), the segment's HU prefix, its totals and its last
 // step with HU+1000 > 150.
-__global__ __launch_bounds__(256) void k_trace_seg(FanGeom g, const float *__restrict__ ct, LutDev lut,
+__device__ __forceinline__ void trace_seg_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float *__restrict__ ct, LutDev lut,
                                                    float *__restrict__ density, float *__restrict__ wepl,
                                                    float *__restrict__ huPrefix, float2 *__restrict__ segSp,
                                                    float *__restrict__ segHu, int *__restrict__ segLast,
@@ -209,12 +213,29 @@ __global__ __launch_bounds__(256) void k_trace_seg(FanGeom g, const float *__res
     segHu[o] = cumHu;
     segLast[o] = lastIn;
 }
+struct TraceSegK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const float *ct;
+        LutDev lut;
+        float *density;
+        float *wepl;
+        float *huPrefix;
+        float2 *segSp;
+        float *segHu;
+        int *segLast;
+        BeamScalars *bs;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { trace_seg_body(b, d, a.g, a.ct, a.lut, a.density, a.wepl, a.huPrefix, a.segSp, a.segHu, a.segLast, a.bs); }
+};
+
 
 // Pass 2, thread per ray: segment carries (double), entry and exit steps. The cumulative HU
 // never decreases (HU+1000 >= 0), so the reference's flag loop (bev_ray_tracing.cu:141-158)
 // reduces to: firstInside = max(#steps with cumulative HU < 150, 1); firstOutside =
 // last step with HU+1000 > 150 (0 if none) + 1.
-__global__ __launch_bounds__(256) void k_trace_carry(FanGeom g, const float *__restrict__ huPrefix,
+__device__ __forceinline__ void trace_carry_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float *__restrict__ huPrefix,
                                                      const float2 *__restrict__ segSp,
                                                      const float *__restrict__ segHu, const int *__restrict__ segLast,
                                                      double *__restrict__ segCarry, int *__restrict__ firstInside,
@@ -260,12 +281,29 @@ __global__ __launch_bounds__(256) void k_trace_carry(FanGeom g, const float *__r
         atomicMax(&bs->beamFirstOutside, mx);
     }
 }
+struct TraceCarryK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const float *huPrefix;
+        const float2 *segSp;
+        const float *segHu;
+        const int *segLast;
+        double *segCarry;
+        int *firstInside;
+        int *firstOutside;
+        unsigned *weplMinBits;
+        BeamScalars *bs;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { trace_carry_body(b, d, a.g, a.huPrefix, a.segSp, a.segHu, a.segLast, a.segCarry, a.firstInside, a.firstOutside, a.weplMinBits, a.bs); }
+};
+
 
 // Pass 3, thread per (step, ray): the segment's carry added to its local WEPL prefix (segment 0
 // has none), and the step's minimum WEPL over the rays that leave material (fo > 0), the
 // weplMin of the wrapper's per-beam reduction (wrapper:3651-3700): one atomic per block on the
 // float's bits (WEPL >= 0, so the bit patterns order as the values).
-__global__ __launch_bounds__(256) void k_trace_fix(FanGeom g, const double *__restrict__ segCarry,
+__device__ __forceinline__ void trace_fix_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const double *__restrict__ segCarry,
                                                    float *__restrict__ wepl, const int *__restrict__ fo,
                                                    unsigned *__restrict__ weplMinBits) {
     __shared__ float s_m[4];
@@ -290,6 +328,18 @@ __global__ __launch_bounds__(256) void k_trace_fix(FanGeom g, const double *__re
         if (bm < 1.0e20f) atomicMin(&weplMinBits[k], __float_as_uint(bm));
     }
 }
+struct TraceFixK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const double *segCarry;
+        float *wepl;
+        const int *fo;
+        unsigned *weplMinBits;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { trace_fix_body(b, d, a.g, a.segCarry, a.wepl, a.fo, a.weplMinBits); }
+};
+
 
 // WEQ-volume input in place of the trace (fillBevFromWeqVolumeKernel, wrapper:1923-1988), thread
 // per ray: the ray's WEQ column is the one nearest its gantry (x, y) (rounded half away from
@@ -363,7 +413,7 @@ __global__ __launch_bounds__(256) void k_trace_reduce(FanGeom g, const float *__
 
 // Distal cut-offs, entry plane and entry sigmas (host formulas, evaluated without
 // contraction so the integers match the host's float arithmetic).
-__global__ __launch_bounds__(256) void k_beam_setup(FanGeom g, const float *__restrict__ weplMin,
+__device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float *__restrict__ weplMin,
                                                     const LayerHost *__restrict__ lh, BeamScalars *__restrict__ bs,
                                                     LayerScalars *__restrict__ ls, const float *__restrict__ subspots,
                                                     int maxSub, float *__restrict__ subEntry, float nucFactor,
@@ -406,6 +456,24 @@ __global__ __launch_bounds__(256) void k_beam_setup(FanGeom g, const float *__re
         }
     }
 }
+struct BeamSetupK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const float *weplMin;
+        const LayerHost *lh;
+        BeamScalars *bs;
+        LayerScalars *ls;
+        const float *subspots;
+        int maxSub;
+        float *subEntry;
+        float nucFactor;
+        int *classCount;
+        int *hist;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { beam_setup_body(b, d, a.g, a.weplMin, a.lh, a.bs, a.ls, a.subspots, a.maxSub, a.subEntry, a.nucFactor, a.classCount, a.hist); }
+};
+
 
 // ---------------------------------------------------------------------------------------
 // spot lattice -> ray weights
@@ -427,7 +495,7 @@ __device__ __forceinline__ float erf_gather(const float *__restrict__ in, int in
 }
 
 // grid: (ceil(rx/64), spotY, L * nSub); interm[(l*nSub + s)][spotY][rx]
-__global__ __launch_bounds__(64) void k_conv_x(ConvArgs a, const float *__restrict__ spot, float *__restrict__ interm,
+__device__ __forceinline__ void conv_x_body(const uint3 blockIdx, const uint3 gridDim, ConvArgs a, const float *__restrict__ spot, float *__restrict__ interm,
                                                const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                                const float *__restrict__ subspots, const float *__restrict__ subEntry) {
     const int o = blockIdx.x * 64 + threadIdx.x;
@@ -450,9 +518,23 @@ __global__ __launch_bounds__(64) void k_conv_x(ConvArgs a, const float *__restri
     }
     interm[((size_t(blockIdx.z) * a.spotY + y) * a.rx) + o] = r;
 }
+struct ConvXK {
+    static constexpr int NT = 64;
+    struct Args {
+        ConvArgs a;
+        const float *spot;
+        float *interm;
+        const LayerScalars *ls;
+        const BeamScalars *bs;
+        const float *subspots;
+        const float *subEntry;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { conv_x_body(b, d, a.a, a.spot, a.interm, a.ls, a.bs, a.subspots, a.subEntry); }
+};
+
 
 // grid: (ceil(rx/64), ry, L); sums the subspot planes in subspot order
-__global__ __launch_bounds__(64) void k_conv_y(ConvArgs a, const float *__restrict__ interm, float *__restrict__ rayW,
+__device__ __forceinline__ void conv_y_body(const uint3 blockIdx, const uint3 gridDim, ConvArgs a, const float *__restrict__ interm, float *__restrict__ rayW,
                                                const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                                const float *__restrict__ subspots, const float *__restrict__ subEntry) {
     const int x = blockIdx.x * 64 + threadIdx.x;
@@ -472,6 +554,20 @@ __global__ __launch_bounds__(64) void k_conv_y(ConvArgs a, const float *__restri
     }
     rayW[(size_t(l) * a.ry + o) * a.rx + x] = acc;
 }
+struct ConvYK {
+    static constexpr int NT = 64;
+    struct Args {
+        ConvArgs a;
+        const float *interm;
+        float *rayW;
+        const LayerScalars *ls;
+        const BeamScalars *bs;
+        const float *subspots;
+        const float *subEntry;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { conv_y_body(b, d, a.a, a.interm, a.rayW, a.ls, a.bs, a.subspots, a.subEntry); }
+};
+
 
 // CPB weights as a gather: each cell sums the subspots whose 3-sigma cell window covers it,
 // in subspot order. Window membership is the reference's floor/ceil rule.
@@ -580,7 +676,7 @@ __device__ __forceinline__ float lut_rows(const float *__restrict__ r0, const fl
 constexpr int kFillChunk = 4;
 
 template <bool HALO>
-__global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const LayerHost *__restrict__ lh,
+__device__ __forceinline__ void fill_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, LutDev lut, const LayerHost *__restrict__ lh,
                                               LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                               const float *__restrict__ density, const float *__restrict__ wepl,
                                               const float *__restrict__ rayW, const int *__restrict__ fi,
@@ -802,6 +898,29 @@ __global__ __launch_bounds__(256) void k_fill(FanGeom g, LutDev lut, const Layer
     const int mx = wave_max_i(passive);
     if (lane == 0) atomicMax(&ls[l].passiveRaw, mx);
 }
+template <bool HALO>
+struct FillK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        LutDev lut;
+        const LayerHost *lh;
+        LayerScalars *ls;
+        const BeamScalars *bs;
+        const float *density;
+        const float *wepl;
+        const float *rayW;
+        const int *fi;
+        const int *fo;
+        float2 *iddSigma;
+        int *firstPassive;
+        FillConsts fc;
+        HaloFill hf;
+        int *tileCode;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { fill_body<HALO>(b, d, a.g, a.lut, a.lh, a.ls, a.bs, a.density, a.wepl, a.rayW, a.fi, a.fo, a.iddSigma, a.firstPassive, a.fc, a.hf, a.tileCode); }
+};
+
 
 // Weight rescale of a resident beam (the served form's weight-only update): the fill's
 // recurrence does not depend on the ray weight, which only scales the IDD and decides the live
@@ -867,7 +986,7 @@ __global__ __launch_bounds__(256) void k_rescale(FanGeom g, LayerScalars *__rest
 
 // Radius histogram of one layer from its tile codes over the superposed steps
 // [beamFirstInside, passive) (complete_superposition.cu:197-255, 400-463); one block per layer.
-__global__ __launch_bounds__(256) void k_tile_hist(FanGeom g, const int *__restrict__ tileCode,
+__device__ __forceinline__ void tile_hist_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const int *__restrict__ tileCode,
                                                    const LayerScalars *__restrict__ ls,
                                                    const BeamScalars *__restrict__ bs, int *__restrict__ hist) {
     __shared__ int s_h[kMaxRadDev + 2];
@@ -882,6 +1001,18 @@ __global__ __launch_bounds__(256) void k_tile_hist(FanGeom g, const int *__restr
     __syncthreads();
     if (threadIdx.x < kMaxRadDev + 2) hist[l * (kMaxRadDev + 2) + threadIdx.x] = s_h[threadIdx.x];
 }
+struct TileHistK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const int *tileCode;
+        const LayerScalars *ls;
+        const BeamScalars *bs;
+        int *hist;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { tile_hist_body(b, d, a.g, a.tileCode, a.ls, a.bs, a.hist); }
+};
+
 
 // Transfer box of one passive step (wrapper:5343-5395): 8 fan corners -> dose index in float,
 // floor/ceil, x start rounded down to 32; the launch covers whole 32x8 thread blocks from the
@@ -923,7 +1054,7 @@ __device__ __forceinline__ float sampled_plane(const FanGeom &g, int inside, flo
 
 // Layer passive steps, layer groups, group boxes and tail planes (one block of 256 threads;
 // the groups are built in LDS and written out once).
-__global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars *__restrict__ ls,
+__device__ __forceinline__ void layer_finalize_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, LayerScalars *__restrict__ ls,
                                                         GroupScalars *__restrict__ gs, BeamScalars *__restrict__ bs,
                                                         int *__restrict__ tailPlaneGroup, int *__restrict__ tailPlaneK,
                                                         int tailCap, int *__restrict__ planeRM) {
@@ -1041,10 +1172,25 @@ __global__ __launch_bounds__(256) void k_layer_finalize(FanGeom g, LayerScalars 
         }
     }
 }
+struct LayerFinalizeK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        LayerScalars *ls;
+        GroupScalars *gs;
+        BeamScalars *bs;
+        int *tailPlaneGroup;
+        int *tailPlaneK;
+        int tailCap;
+        int *planeRM;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { layer_finalize_body(b, d, a.g, a.ls, a.gs, a.bs, a.tailPlaneGroup, a.tailPlaneK, a.tailCap, a.planeRM); }
+};
+
 
 // The halo grid reuses the beam's passive steps; its groups, boxes and tails are its own.
 // Copies the primary scalars and clears the grid's radius histograms and class counters.
-__global__ void k_seed_grid(int numLayers, const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+__device__ __forceinline__ void seed_grid_body(const uint3 blockIdx, const uint3 gridDim, int numLayers, const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                             LayerScalars *__restrict__ lsOut, BeamScalars *__restrict__ bsOut, int *__restrict__ hist,
                             int *__restrict__ classCount) {
     for (int l = threadIdx.x; l < numLayers; l += blockDim.x) lsOut[l] = ls[l];
@@ -1056,18 +1202,41 @@ __global__ void k_seed_grid(int numLayers, const LayerScalars *__restrict__ ls, 
         *bsOut = b;
     }
 }
+struct SeedGridK {
+    static constexpr int NT = 256;
+    struct Args {
+        int numLayers;
+        const LayerScalars *ls;
+        const BeamScalars *bs;
+        LayerScalars *lsOut;
+        BeamScalars *bsOut;
+        int *hist;
+        int *classCount;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { seed_grid_body(b, d, a.numLayers, a.ls, a.bs, a.lsOut, a.bsOut, a.hist, a.classCount); }
+};
+
 
 // (0, +inf) over a halo (IDD, rSigmaEff) field: cells no spot ray writes stay empty.
-__global__ void k_clear_pairs(float2 *__restrict__ v, size_t n) {
+__device__ __forceinline__ void clear_pairs_body(const uint3 blockIdx, const uint3 gridDim, float2 *__restrict__ v, size_t n) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
         v[i] = make_float2(0.0f, PBS_INF);
 }
+struct ClearPairsK {
+    static constexpr int NT = 256;
+    struct Args {
+        float2 *v;
+        size_t n;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { clear_pairs_body(b, d, a.v, a.n); }
+};
+
 
 // ---------------------------------------------------------------------------------------
 // tile radii, bins and work items
 // ---------------------------------------------------------------------------------------
 // one wave per (step, tile); loops over layers. code = rad | live << 8, -1 = not computed
-__global__ __launch_bounds__(256) void k_tiles(FanGeom g, const float2 *__restrict__ iddSigma,
+__device__ __forceinline__ void tiles_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float2 *__restrict__ iddSigma,
                                                const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
                                                int *__restrict__ tileRad, int *__restrict__ hist) {
     const int ntx = g.sx / 32, nty = g.sy / 8, nt = ntx * nty;
@@ -1105,11 +1274,24 @@ __global__ __launch_bounds__(256) void k_tiles(FanGeom g, const float2 *__restri
         }
     }
 }
+struct TilesK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const float2 *iddSigma;
+        const LayerScalars *ls;
+        const BeamScalars *bs;
+        int *tileRad;
+        int *hist;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { tiles_body(b, d, a.g, a.iddSigma, a.ls, a.bs, a.tileRad, a.hist); }
+};
+
 
 // overflowFlag (halo grid, complete_superposition.cu:558-620): a tile past the largest radius
 // sets the sticky flag the host reports as an error; the primary grid (nullptr) only clamps
 // it (413-463). Both clamp, so every launch stays within the largest radius.
-__global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restrict__ effRad,
+__device__ __forceinline__ void bins_body(const uint3 blockIdx, const uint3 gridDim, int numLayers, const int *__restrict__ hist, int *__restrict__ effRad,
                        int *__restrict__ overflowFlag) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= numLayers) return;
@@ -1141,6 +1323,17 @@ __global__ void k_bins(int numLayers, const int *__restrict__ hist, int *__restr
 #pragma unroll
     for (int i = 0; i < kMaxRadDev + 2; ++i) effRad[l * (kMaxRadDev + 2) + i] = eff[i];
 }
+struct BinsK {
+    static constexpr int NT = 64;
+    struct Args {
+        int numLayers;
+        const int *hist;
+        int *effRad;
+        int *overflowFlag;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { bins_body(b, d, a.numLayers, a.hist, a.effRad, a.overflowFlag); }
+};
+
 
 __device__ __forceinline__ int radius_class(int R) {
     // launch classes: accumulator layouts of radius 7, 11, 15 (2RM + 1 lanes fill a group of
@@ -1182,7 +1375,7 @@ __device__ __forceinline__ void append_unit(int *__restrict__ classCount, int2 *
 // block is empty), and one work unit (slot, block) per non-empty block in its class list.
 // The layer scalars, launch radii and group tail ranges are staged in LDS and the tile codes
 // are loaded eight layers at a time, so the loop's global loads do not wait on each other.
-__global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict__ tileRad,
+__device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const int *__restrict__ tileRad,
                                                const int *__restrict__ effRad, const LayerScalars *__restrict__ ls,
                                                const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
                                                int *__restrict__ itemR, int *__restrict__ tailItemR,
@@ -1263,6 +1456,26 @@ __global__ __launch_bounds__(256) void k_items(FanGeom g, const int *__restrict_
         if (lane == 0 && pm >= 0) atomicMax(&planeRM[g.K + q], pm);
     }
 }
+struct ItemsK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const int *tileRad;
+        const int *effRad;
+        const LayerScalars *ls;
+        const GroupScalars *gs;
+        const BeamScalars *bs;
+        int *itemR;
+        int *tailItemR;
+        int *classCount;
+        int2 *classList;
+        int maxUnits;
+        int *planeRM;
+        int *slotLayers;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { items_body(b, d, a.g, a.tileRad, a.effRad, a.ls, a.gs, a.bs, a.itemR, a.tailItemR, a.classCount, a.classList, a.maxUnits, a.planeRM, a.slotLayers); }
+};
+
 
 // ---------------------------------------------------------------------------------------
 // superposition
@@ -1351,7 +1564,7 @@ __device__ __forceinline__ void sweep_dispatch(int r, SliceAcc<RM> &acc, const f
 // order in LDS and written to the unit's slot as a (8 + 2RM) x (32 + 2RM) block. Fixed
 // orders throughout: bitwise repeatable.
 template <int RM>
-__global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__restrict__ iddSigma,
+__device__ __forceinline__ void superpose_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float2 *__restrict__ iddSigma,
                                                    const int *__restrict__ tileRad, const int *__restrict__ effRad,
                                                    const LayerScalars *__restrict__ ls,
                                                    const GroupScalars *__restrict__ gs,
@@ -1468,6 +1681,28 @@ __global__ __launch_bounds__(256) void k_superpose(FanGeom g, const float2 *__re
         __syncthreads();
     }
 }
+template <int RM>
+struct SuperposeK {
+    static constexpr int NT = 256;
+    static constexpr bool kUnitQueue = true;  // batched as a queue of the beams' units
+    struct Args {
+        FanGeom g;
+        const float2 *iddSigma;
+        const int *tileRad;
+        const int *effRad;
+        const LayerScalars *ls;
+        const GroupScalars *gs;
+        const BeamScalars *bs;
+        const int *classCount;
+        const int2 *classList;
+        int cls;
+        int maxUnits;
+        float *mainSlots;
+        float *tailSlots;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { superpose_body<RM>(b, d, a.g, a.iddSigma, a.tileRad, a.effRad, a.ls, a.gs, a.bs, a.classCount, a.classList, a.cls, a.maxUnits, a.mainSlots, a.tailSlots); }
+};
+
 
 // ---------------------------------------------------------------------------------------
 // Superposition with lanes on output rows (radius classes up to 16)
@@ -1572,8 +1807,7 @@ __device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM],
 // index is static. The groups are added into the block in source-row order: bitwise
 // repeatable.
 template <int RM>
-__global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
-    FanGeom g, const float2 *__restrict__ iddSigma, const int *__restrict__ tileRad, const int *__restrict__ effRad,
+__device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float2 *__restrict__ iddSigma, const int *__restrict__ tileRad, const int *__restrict__ effRad,
     const LayerScalars *__restrict__ ls, const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
     const int *__restrict__ classCount, const int2 *__restrict__ classList, int cls, int maxUnits,
     const int *__restrict__ slotLayers, float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
@@ -1724,6 +1958,29 @@ __global__ __launch_bounds__(RowsShape<RM>::NT) void k_superpose_rows(
         __syncthreads();
     }
 }
+template <int RM>
+struct SuperposeRowsK {
+    static constexpr int NT = RowsShape<RM>::NT;
+    static constexpr bool kUnitQueue = true;  // batched as a queue of the beams' units
+    struct Args {
+        FanGeom g;
+        const float2 *iddSigma;
+        const int *tileRad;
+        const int *effRad;
+        const LayerScalars *ls;
+        const GroupScalars *gs;
+        const BeamScalars *bs;
+        const int *classCount;
+        const int2 *classList;
+        int cls;
+        int maxUnits;
+        const int *slotLayers;
+        float *mainSlots;
+        float *tailSlots;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { superpose_rows_body<RM>(b, d, a.g, a.iddSigma, a.tileRad, a.effRad, a.ls, a.gs, a.bs, a.classCount, a.classList, a.cls, a.maxUnits, a.slotLayers, a.mainSlots, a.tailSlots); }
+};
+
 
 // Floor division by a positive divisor.
 __device__ __forceinline__ int floor_div(int a, int d) { return (a >= 0 ? a : a - d + 1) / d; }
@@ -1757,7 +2014,7 @@ __device__ __forceinline__ float gather_tile(const FanGeom &g, int otx, int oty,
 
 // Main BEV [K][sy+64][sx+64] and tail BEV planes [tailCap][sy+64][sx+64]; a work-group per
 // 32x8 output tile of a plane (one lane per pixel).
-__global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__restrict__ bs,
+__device__ __forceinline__ void merge_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const BeamScalars *__restrict__ bs,
                                                const int *__restrict__ itemR, const float *__restrict__ mainSlots,
                                                float *__restrict__ bev, const int *__restrict__ tailItemR,
                                                const float *__restrict__ tailSlots, float *__restrict__ tailBev,
@@ -1790,6 +2047,22 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
         }
     }
 }
+struct MergeK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        const BeamScalars *bs;
+        const int *itemR;
+        const float *mainSlots;
+        float *bev;
+        const int *tailItemR;
+        const float *tailSlots;
+        float *tailBev;
+        const int *planeRM;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { merge_body(b, d, a.g, a.bs, a.itemR, a.mainSlots, a.bev, a.tailItemR, a.tailSlots, a.tailBev, a.planeRM); }
+};
+
 
 // ---------------------------------------------------------------------------------------
 // BEV -> dose
@@ -1798,17 +2071,14 @@ __global__ __launch_bounds__(256) void k_merge(FanGeom g, const BeamScalars *__r
 // dose-index -> fan-index map is the reference's factored form
 // (transfer_param_struct_div3.cuh:34-44, transfer_param_struct_div3_impl.cu:14-34). Every
 // voxel samples the main BEV; a voxel inside a group's box also samples that group's tail.
-__global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, const BeamScalars *__restrict__ bs,
-                                                  const GroupScalars *__restrict__ gs, const float *__restrict__ bev,
-                                                  const float *__restrict__ tailBev, float *__restrict__ dose,
-                                                  int tx, int ty, int tz) {
-    // a block is a tx x ty x tz voxel brick (256 voxels), short along the beam's depth axis so
-    // its samples stay within a few BEV planes
-    const int lx = int(threadIdx.x) % tx, ly = (int(threadIdx.x) / tx) % ty, lz = int(threadIdx.x) / (tx * ty);
-    const int x = bs->boxStart[0] + int(blockIdx.x) * tx + lx;
-    const int y = bs->boxStart[1] + int(blockIdx.y) * ty + ly;
-    const int z = bs->boxStart[2] + int(blockIdx.z) * tz + lz;
-    if (x >= bs->boxEnd[0] || y >= bs->boxEnd[1] || z >= bs->boxEnd[2]) return;
+// The dose a beam adds at voxel (x, y, z), or 0 outside its union box.
+__device__ __forceinline__ float transfer_at(const FanGeom &g, const TransferArgs &ta,
+                                             const BeamScalars *__restrict__ bs, const GroupScalars *__restrict__ gs,
+                                             const float *__restrict__ bev, const float *__restrict__ tailBev, int x,
+                                             int y, int z) {
+    if (x < bs->boxStart[0] || y < bs->boxStart[1] || z < bs->boxStart[2] || x >= bs->boxEnd[0] ||
+        y >= bs->boxEnd[1] || z >= bs->boxEnd[2])
+        return 0.0f;
     const int inside = bs->beamFirstInside;
     const int m = bs->numGroups;
     const int bx = g.sx + 64, by = g.sy + 64;
@@ -1821,8 +2091,8 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
     const float fxI = px + ta.globalOffset[0], fyI = py + ta.globalOffset[1];
     const float fzI = pz + (ta.globalOffset[2] - float(inside));
     // prim_transf_kernel.cu:28-40: texel coordinate = fan index + 1/2; negatives skipped
-    if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return;
-    if (fxI > float(bx) || fyI > float(by)) return;  // wholly outside the BEV: border value 0
+    if (fxI + 0.5f < 0.0f || fyI + 0.5f < 0.0f || fzI + 0.5f < 0.0f) return 0.0f;
+    if (fxI > float(bx) || fyI > float(by)) return 0.0f;  // wholly outside the BEV: border value 0
     float v = 0.0f;
     if (fzI <= float(g.K))
         v = trilinear_planes(bev, bx, by, inside, min(bs->mainEndMax, g.K), fxI, fyI, fzI);
@@ -1836,19 +2106,127 @@ __global__ __launch_bounds__(256) void k_transfer(FanGeom g, TransferArgs ta, co
             continue;
         v += trilinear_border(tailBev + size_t(G.tailBase) * bx * by, bx, by, G.tailCount, fxI, fyI, zl);
     }
-    if (v > 0.0f) {
-        float *res = dose + (size_t(z) * g.doseDims[1] + y) * g.doseDims[0] + x;
-        *res += v;
+    return v;
+}
+
+// One beam: a thread per dose voxel inside its union box, a work-group per tx x ty x tz voxel
+// brick (256 voxels), short along the beam's depth axis so its samples stay within a few BEV
+// planes.
+struct TransferK {
+    static constexpr int NT = 256;
+    struct Args {
+        FanGeom g;
+        TransferArgs ta;
+        const BeamScalars *bs;
+        const GroupScalars *gs;
+        const float *bev;
+        const float *tailBev;
+        float *dose;
+        int tx, ty, tz;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) {
+        (void)d;
+        const int lx = int(threadIdx.x) % a.tx, ly = (int(threadIdx.x) / a.tx) % a.ty,
+                  lz = int(threadIdx.x) / (a.tx * a.ty);
+        const int x = a.bs->boxStart[0] + int(b.x) * a.tx + lx;
+        const int y = a.bs->boxStart[1] + int(b.y) * a.ty + ly;
+        const int z = a.bs->boxStart[2] + int(b.z) * a.tz + lz;
+        if (x >= a.bs->boxEnd[0] || y >= a.bs->boxEnd[1] || z >= a.bs->boxEnd[2]) return;
+        const float v = transfer_at(a.g, a.ta, a.bs, a.gs, a.bev, a.tailBev, x, y, z);
+        if (v > 0.0f) {
+            float *res = a.dose + (size_t(z) * a.g.doseDims[1] + y) * a.g.doseDims[0] + x;
+            *res += v;
+        }
     }
+};
+
+// Every transfer of a batch in one launch: a thread per voxel of the whole dose grid (8x8x4
+// bricks) adds the beams' values in list order (one owner per voxel, no atomics), so the sum is
+// the one the beams make one after another.
+__global__ __launch_bounds__(256) void k_transfer_many(const TransferK::Args *__restrict__ list, int n) {
+    float *__restrict__ dose = list[0].dose;
+    const int dx = list[0].g.doseDims[0], dy = list[0].g.doseDims[1], dz = list[0].g.doseDims[2];
+    const int nbx = (dx + 7) / 8, nby = (dy + 7) / 8;
+    const int x = int(blockIdx.x % unsigned(nbx)) * 8 + int(threadIdx.x & 7);
+    const int y = int((blockIdx.x / unsigned(nbx)) % unsigned(nby)) * 8 + int((threadIdx.x >> 3) & 7);
+    const int z = int(blockIdx.x / unsigned(nbx * nby)) * 4 + int(threadIdx.x >> 6);
+    if (x >= dx || y >= dy || z >= dz) return;
+    float acc = 0.0f;
+    bool any = false;
+    float *res = dose + (size_t(z) * dy + y) * dx + x;
+    for (int i = 0; i < n; ++i) {
+        const TransferK::Args &a = list[i];
+        const float v = transfer_at(a.g, a.ta, a.bs, a.gs, a.bev, a.tailBev, x, y, z);
+        if (v > 0.0f) {
+            if (!any) acc = *res;
+            any = true;
+            acc += v;
+        }
+    }
+    if (any) *res = acc;
 }
 
 // Segment blockIdx.y of the upload, 32-bit words strided over the blocks of blockIdx.x (the
 // source is host memory: every word is read once).
-__global__ __launch_bounds__(256) void k_upload(UploadArgs a) {
+__device__ __forceinline__ void upload_body(const uint3 blockIdx, const uint3 gridDim, const UploadArgs &a) {
     const int sg = blockIdx.y;
     if (sg >= a.n) return;
     const UploadSeg u = a.seg[sg];
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < u.words; i += gridDim.x * 256u) u.dst[i] = u.src[i];
+}
+struct UploadK {
+    static constexpr int NT = 256;
+    struct Args {
+        UploadArgs a;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { upload_body(b, d, a.a); }
+};
+
+// ---------------------------------------------------------------------------------------
+// one-beam and multi-beam launches of a kernel body K
+// ---------------------------------------------------------------------------------------
+template <typename K>
+__global__ __launch_bounds__(K::NT) void k_one(typename K::Args a) {
+    K::run(a, uint3{blockIdx.x, blockIdx.y, blockIdx.z}, uint3{gridDim.x, gridDim.y, gridDim.z});
+}
+
+// Work-group blockIdx.x of the concatenated grids: its beam is the last whose first work-group
+// is not after it (a scalar walk over at most a batch of entries), its index within that
+// beam's grid is unflattened x-fastest.
+template <typename K>
+__global__ __launch_bounds__(K::NT) void k_many(const typename K::Args *__restrict__ args, MultiTable t) {
+    const unsigned w = blockIdx.x;
+    int b = 0;
+    if (t.nb <= 64) {
+        // lane i reads the end of beam i: the beam is the number of ends at or before w
+        const int lane = int(threadIdx.x & 63);
+        const unsigned end = lane < t.nb ? t.base[lane + 1] : 0xffffffffu;
+        b = __builtin_amdgcn_readfirstlane(__popcll(__ballot(w >= end)));
+    } else {
+        while (b + 1 < t.nb && w >= t.base[b + 1]) ++b;
+    }
+    const uint4 d = t.dims[b];
+    const unsigned local = w - t.base[b];
+    const uint3 id{local % d.x, (local / d.x) % d.y, local / (d.x * d.y)};
+    K::run(args[b], id, uint3{d.x, d.y, d.z});
+}
+
+// The radius-class kernels over a batch: a fixed grid walks the beams in order, and each beam's
+// units (counted on the device) are spread over the whole grid, the beam's first unit on the
+// work-group after the previous beam's last, so the grid stays balanced whatever the counts.
+// The class bodies end every unit with a barrier, so a work-group may run them back to back.
+template <typename K>
+__global__ __launch_bounds__(K::NT) void k_many_units(const typename K::Args *__restrict__ args, int nb) {
+    const unsigned G = gridDim.x;
+    unsigned start = 0;
+    for (int b = 0; b < nb; ++b) {
+        const typename K::Args &a = args[b];
+        const int count = min(a.classCount[a.cls], a.maxUnits);
+        if (count <= 0) continue;
+        const unsigned id = (blockIdx.x + G - start % G) % G;
+        K::run(a, uint3{id, 0u, 0u}, uint3{G, 1u, 1u});
+        start += unsigned(count);
+    }
 }
 
 }  // namespace pbs
@@ -1858,12 +2236,197 @@ __global__ __launch_bounds__(256) void k_upload(UploadArgs a) {
 // ---------------------------------------------------------------------------------------
 namespace pbs {
 
+thread_local Recorder *t_rec = nullptr;
+
+template <typename K>
+static void launch_one(const void *a, dim3 grid, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_one<K>, grid, dim3(K::NT), lds, s, *static_cast<const typename K::Args *>(a));
+}
+
+template <typename K, typename = void>
+struct ClassKernel : std::false_type {};
+template <typename K>
+struct ClassKernel<K, std::void_t<decltype(K::kUnitQueue)>> : std::true_type {};
+
+// Work-groups of a radius class's batched launch: enough to fill the device several times over.
+constexpr unsigned kClassBatchGrid = 2048;
+
+template <typename K>
+static void launch_many(const void *a, const MultiTable &t, unsigned blocks, size_t lds, hipStream_t s) {
+    if constexpr (ClassKernel<K>::value) {
+        const unsigned g = blocks < kClassBatchGrid ? blocks : kClassBatchGrid;
+        hipLaunchKernelGGL(k_many_units<K>, dim3(g), dim3(K::NT), lds, s, static_cast<const typename K::Args *>(a),
+                           t.nb);
+    } else {
+        hipLaunchKernelGGL(k_many<K>, dim3(blocks), dim3(K::NT), lds, s, static_cast<const typename K::Args *>(a), t);
+    }
+}
+
+// The batch's transfers: one owner per voxel of the whole dose grid, t.nb entries in order
+// (`blocks` bricks of 8x8x4 voxels, from many_blocks).
+template <>
+void launch_many<TransferK>(const void *a, const MultiTable &t, unsigned blocks, size_t lds, hipStream_t s) {
+    (void)lds;
+    hipLaunchKernelGGL(k_transfer_many, dim3(blocks), dim3(256), 0, s, static_cast<const TransferK::Args *>(a), t.nb);
+}
+
+// Work-groups of a multi-beam launch of K: 0 means the beams' grids concatenated.
+template <typename K>
+static unsigned many_blocks(const typename K::Args &) { return 0; }
+template <>
+unsigned many_blocks<TransferK>(const TransferK::Args &a) {
+    const unsigned nx = unsigned(a.g.doseDims[0] + 7) / 8, ny = unsigned(a.g.doseDims[1] + 7) / 8;
+    return nx * ny * (unsigned(a.g.doseDims[2] + 3) / 4);
+}
+// Records that may share a launch also share this (the transfers: their dose grid).
+template <typename K>
+static const void *many_target(const typename K::Args &) { return nullptr; }
+template <>
+const void *many_target<TransferK>(const TransferK::Args &a) { return a.dose; }
+
+// Launches kernel K on `grid` now, or records it when a Recorder is installed (key: the current
+// stage and the launch's index within it, unless `key` is given).
+template <typename K>
+static void dispatch(const typename K::Args &a, dim3 grid, size_t lds, hipStream_t s, int key = -1) {
+    const bool empty = grid.x == 0 || grid.y == 0 || grid.z == 0;
+    if (!t_rec) {
+        if (!empty) launch_one<K>(&a, grid, lds, s);
+        return;
+    }
+    // an empty launch still takes its place in the stage, so the keys of later launches stay
+    // aligned with the other beams'
+    const int k = key >= 0 ? key : t_rec->stage * 64 + t_rec->sub++;
+    if (empty) return;
+    LaunchRec r;
+    r.key = k;
+    r.beam = t_rec->beam;
+    r.seq = t_rec->seq++;
+    r.one = &launch_one<K>;
+    r.many = &launch_many<K>;
+    r.args.resize(sizeof(typename K::Args));
+    std::memcpy(r.args.data(), &a, sizeof(typename K::Args));
+    r.argAlign = alignof(typename K::Args);
+    r.grid = grid;
+    r.lds = lds;
+    r.manyBlocks = many_blocks<K>(a);
+    r.target = many_target<K>(a);
+    t_rec->recs.push_back(std::move(r));
+}
+
+// ---------------------------------------------------------------------------------------
+// batch executor
+// ---------------------------------------------------------------------------------------
+namespace {
+
+struct RecGroup {
+    size_t first, count;  // a run of the sorted records
+};
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Records sorted by (key, kernel, target, beam, record index); a group is a run of equal key,
+// kernel and target. A beam has one record per key (its transfers excepted, which share the
+// kernel and go in beam order), so launching the groups in this order keeps every beam's own
+// order; records of one key but different kernels (beams of different table sets or branches)
+// do not depend on each other.
+void plan_groups(const std::vector<LaunchRec> &recs, std::vector<size_t> &order, std::vector<RecGroup> &groups) {
+    order.resize(recs.size());
+    for (size_t i = 0; i < recs.size(); ++i) order[i] = i;
+    const std::less<const void *> before;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const LaunchRec &x = recs[a], &y = recs[b];
+        if (x.key != y.key) return x.key < y.key;
+        const void *mx = reinterpret_cast<const void *>(x.many), *my = reinterpret_cast<const void *>(y.many);
+        if (mx != my) return before(mx, my);
+        if (x.target != y.target) return before(x.target, y.target);
+        if (x.beam != y.beam) return x.beam < y.beam;
+        return x.seq < y.seq;
+    });
+    groups.clear();
+    for (size_t i = 0; i < order.size(); ++i) {
+        const LaunchRec &r = recs[order[i]];
+        if (!groups.empty()) {
+            const LaunchRec &h = recs[order[groups.back().first]];
+            if (h.key == r.key && h.many == r.many && h.target == r.target) {
+                ++groups.back().count;
+                continue;
+            }
+        }
+        groups.push_back(RecGroup{i, 1});
+    }
+}
+
+// Table bytes of a group: the Args array, the first work-groups, the grids (256-byte aligned).
+size_t group_bytes(size_t argSize, size_t n) {
+    return align_up(align_up(argSize * n, 16) + align_up(sizeof(unsigned) * (n + 1), 16) + sizeof(uint4) * n, 256);
+}
+
+}  // namespace
+
+size_t records_table_bytes(const std::vector<LaunchRec> &recs) {
+    std::vector<size_t> order;
+    std::vector<RecGroup> groups;
+    plan_groups(recs, order, groups);
+    size_t b = 0;
+    for (const RecGroup &g : groups) b += group_bytes(recs[order[g.first]].args.size(), g.count);
+    return b;
+}
+
+hipError_t run_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, unsigned char *devTable,
+                       hipStream_t s, int *launches) {
+    std::vector<size_t> order;
+    std::vector<RecGroup> groups;
+    plan_groups(recs, order, groups);
+    std::vector<size_t> offs(groups.size());
+    std::vector<unsigned> total(groups.size());
+    size_t off = 0;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const RecGroup &g = groups[gi];
+        const size_t argSize = recs[order[g.first]].args.size();
+        unsigned char *h = hostTable + off;
+        offs[gi] = off;
+        unsigned *base = reinterpret_cast<unsigned *>(h + align_up(argSize * g.count, 16));
+        uint4 *dims = reinterpret_cast<uint4 *>(reinterpret_cast<unsigned char *>(base) +
+                                                align_up(sizeof(unsigned) * (g.count + 1), 16));
+        unsigned tot = 0;
+        for (size_t j = 0; j < g.count; ++j) {
+            const LaunchRec &r = recs[order[g.first + j]];
+            std::memcpy(h + j * argSize, r.args.data(), argSize);
+            base[j] = tot;
+            dims[j] = uint4{r.grid.x, r.grid.y, r.grid.z, 0u};
+            tot += r.grid.x * r.grid.y * r.grid.z;
+        }
+        base[g.count] = tot;
+        total[gi] = tot;
+        off += group_bytes(argSize, g.count);
+    }
+    if (off) {
+        const hipError_t e = hipMemcpyAsync(devTable, hostTable, off, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const RecGroup &g = groups[gi];
+        const LaunchRec &h = recs[order[g.first]];
+        const size_t argSize = h.args.size();
+        unsigned char *d = devTable + offs[gi];
+        const unsigned *base = reinterpret_cast<const unsigned *>(d + align_up(argSize * g.count, 16));
+        const uint4 *dims = reinterpret_cast<const uint4 *>(reinterpret_cast<const unsigned char *>(base) +
+                                                            align_up(sizeof(unsigned) * (g.count + 1), 16));
+        size_t lds = 0;
+        for (size_t j = 0; j < g.count; ++j) lds = std::max(lds, recs[order[g.first + j]].lds);
+        const MultiTable t{base, dims, int(g.count)};
+        h.many(d, t, h.manyBlocks ? h.manyBlocks : total[gi], lds, s);
+    }
+    if (launches) *launches = int(groups.size());
+    return hipGetLastError();
+}
+
 hipError_t launch_upload(const UploadArgs &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
     uint32_t most = 0;
     for (int i = 0; i < a.n; ++i) most = most > a.seg[i].words ? most : a.seg[i].words;
     const unsigned bx = std::max(1u, std::min(64u, (most + 255u) / 256u));
-    hipLaunchKernelGGL(k_upload, dim3(bx, unsigned(a.n)), dim3(256), 0, s, a);
+    dispatch<UploadK>({a}, dim3(bx, unsigned(a.n)), 0, s);
     return hipGetLastError();
 }
 
@@ -1900,12 +2463,10 @@ hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, fl
     double *segCarry = reinterpret_cast<double *>(scratch + L.carry);
     const size_t shm = sizeof(float) * size_t(lut.nDensity + lut.nSp);
     unsigned *minBits = reinterpret_cast<unsigned *>(weplMin);
-    hipLaunchKernelGGL(k_trace_seg, dim3((n + 255) / 256, S), dim3(256), shm, s, g, ct, lut, density, wepl, huPrefix,
-                       segSp, segHu, segLast, bs);
+    dispatch<TraceSegK>({g, ct, lut, density, wepl, huPrefix, segSp, segHu, segLast, bs}, dim3((n + 255) / 256, S), shm, s);
     // the carry grid also covers the K steps whose minimum it initialises
-    hipLaunchKernelGGL(k_trace_carry, dim3((std::max(n, g.K) + 255) / 256), dim3(256), 0, s, g, huPrefix, segSp,
-                       segHu, segLast, segCarry, fi, fo, minBits, bs);
-    hipLaunchKernelGGL(k_trace_fix, dim3((n + 255) / 256, g.K), dim3(256), 0, s, g, segCarry, wepl, fo, minBits);
+    dispatch<TraceCarryK>({g, huPrefix, segSp, segHu, segLast, segCarry, fi, fo, minBits, bs}, dim3((std::max(n, g.K) + 255) / 256), 0, s);
+    dispatch<TraceFixK>({g, segCarry, wepl, fo, minBits}, dim3((n + 255) / 256, g.K), 0, s);
     return hipGetLastError();
 }
 
@@ -1925,18 +2486,15 @@ hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *f
 hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
                              LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
                              int *classCount, int *hist, hipStream_t s) {
-    hipLaunchKernelGGL(k_beam_setup, dim3(1), dim3(256), 0, s, g, weplMin, lh, bs, ls, subspots, maxSub, subEntry,
-                       nucFactor, classCount, hist);
+    dispatch<BeamSetupK>({g, weplMin, lh, bs, ls, subspots, maxSub, subEntry, nucFactor, classCount, hist}, dim3(1), 0, s);
     return hipGetLastError();
 }
 
 hipError_t launch_conv(const ConvArgs &a, const float *spot, float *interm, float *rayW, const LayerScalars *ls,
                        const BeamScalars *bs, const float *subspots, const float *subEntry, int numLayers,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_conv_x, dim3((a.rx + 63) / 64, a.spotY, numLayers * a.nSub), dim3(64), 0, s, a, spot, interm,
-                       ls, bs, subspots, subEntry);
-    hipLaunchKernelGGL(k_conv_y, dim3((a.rx + 63) / 64, a.ry, numLayers), dim3(64), 0, s, a, interm, rayW, ls, bs,
-                       subspots, subEntry);
+    dispatch<ConvXK>({a, spot, interm, ls, bs, subspots, subEntry}, dim3((a.rx + 63) / 64, a.spotY, numLayers * a.nSub), 0, s);
+    dispatch<ConvYK>({a, interm, rayW, ls, bs, subspots, subEntry}, dim3((a.rx + 63) / 64, a.ry, numLayers), 0, s);
     return hipGetLastError();
 }
 
@@ -1956,11 +2514,9 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
     const int nt = (g.sx / 32) * (g.sy / 8);
     const size_t lds = sizeof(float) * (size_t(lut.nRrl) + 2 * size_t(g.K) + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
     if (hf.rayToNuc)
-        hipLaunchKernelGGL(k_fill<true>, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl,
-                           rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
+        dispatch<FillK<true>>({g, lut, lh, ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode}, dim3(nt, g.numLayers), lds, s);
     else
-        hipLaunchKernelGGL(k_fill<false>, dim3(nt, g.numLayers), dim3(256), lds, s, g, lut, lh, ls, bs, density, wepl,
-                           rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode);
+        dispatch<FillK<false>>({g, lut, lh, ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode}, dim3(nt, g.numLayers), lds, s);
     return hipGetLastError();
 }
 
@@ -1975,7 +2531,7 @@ hipError_t launch_rescale(const FanGeom &g, LayerScalars *ls, const BeamScalars 
 
 hipError_t launch_seed_grid(int numLayers, const LayerScalars *ls, const BeamScalars *bs, LayerScalars *lsOut,
                             BeamScalars *bsOut, int *hist, int *classCount, hipStream_t s) {
-    hipLaunchKernelGGL(k_seed_grid, dim3(1), dim3(256), 0, s, numLayers, ls, bs, lsOut, bsOut, hist, classCount);
+    dispatch<SeedGridK>({numLayers, ls, bs, lsOut, bsOut, hist, classCount}, dim3(1), 0, s);
     return hipGetLastError();
 }
 
@@ -2056,7 +2612,7 @@ hipError_t launch_profile_hooks(const FanGeom &g, const ProfileArgs &a, const fl
 
 hipError_t launch_clear_pairs(float2 *v, size_t n, hipStream_t s) {
     const size_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(k_clear_pairs, dim3(unsigned(blocks < 4096 ? blocks : 4096)), dim3(256), 0, s, v, n);
+    dispatch<ClearPairsK>({v, n}, dim3(unsigned(blocks < 4096 ? blocks : 4096)), 0, s);
     return hipGetLastError();
 }
 
@@ -2065,19 +2621,16 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
                                      int *planeRM, int *slotLayers, hipStream_t s) {
-    hipLaunchKernelGGL(k_layer_finalize, dim3(1), dim3(256), 0, s, g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap,
-                       planeRM);
+    dispatch<LayerFinalizeK>({g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap, planeRM}, dim3(1), 0, s);
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (codesFromFill)
-        hipLaunchKernelGGL(k_tile_hist, dim3(g.numLayers), dim3(256), 0, s, g, tileRad, ls, bs, hist);
+        dispatch<TileHistK>({g, tileRad, ls, bs, hist}, dim3(g.numLayers), 0, s);
     else
-        hipLaunchKernelGGL(k_tiles, dim3((nt + 3) / 4, g.K), dim3(256), 0, s, g, iddSigma, ls, bs, tileRad, hist);
-    hipLaunchKernelGGL(k_bins, dim3((g.numLayers + 63) / 64), dim3(64), 0, s, g.numLayers, hist, effRad,
-                       overflowFlag);
+        dispatch<TilesK>({g, iddSigma, ls, bs, tileRad, hist}, dim3((nt + 3) / 4, g.K), 0, s);
+    dispatch<BinsK>({g.numLayers, hist, effRad, overflowFlag}, dim3((g.numLayers + 63) / 64), 0, s);
     const size_t itemsLds =
         sizeof(int) * (size_t(g.numLayers) * (3 + kMaxRadDev + 2) + 2 * size_t(std::max(g.numLayers, 1)));
-    hipLaunchKernelGGL(k_items, dim3((nt + 255) / 256, g.K), dim3(256), itemsLds, s, g, tileRad, effRad, ls, gs, bs,
-                       itemR, tailItemR, classCount, classList, maxUnits, planeRM, slotLayers);
+    dispatch<ItemsK>({g, tileRad, effRad, ls, gs, bs, itemR, tailItemR, classCount, classList, maxUnits, planeRM, slotLayers}, dim3((nt + 255) / 256, g.K), itemsLds, s);
     return hipGetLastError();
 }
 
@@ -2087,12 +2640,9 @@ static void launch_class(const FanGeom &g, const float2 *iddSigma, const int *ti
                          const int2 *classList, int cls, int maxUnits, const int *slotLayers, float *mainSlots,
                          float *tailSlots, int grid, hipStream_t s) {
     if constexpr (RM <= 15)
-        hipLaunchKernelGGL(k_superpose_rows<RM>, dim3(grid), dim3(RowsShape<RM>::NT), 0, s, g, iddSigma, tileRad,
-                           effRad, ls, gs, bs, classCount, classList, cls, maxUnits, slotLayers, mainSlots,
-                           tailSlots);
+        dispatch<SuperposeRowsK<RM>>({g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, cls, maxUnits, slotLayers, mainSlots, tailSlots}, dim3(grid), 0, s);
     else
-        hipLaunchKernelGGL(k_superpose<RM>, dim3(grid), dim3(256), 0, s, g, iddSigma, tileRad, effRad, ls, gs, bs,
-                           classCount, classList, cls, maxUnits, mainSlots, tailSlots);
+        dispatch<SuperposeK<RM>>({g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, cls, maxUnits, mainSlots, tailSlots}, dim3(grid), 0, s);
 }
 
 hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int *tileRad, const int *effRad,
@@ -2119,8 +2669,7 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
                         hipStream_t s) {
     const int bx = g.sx + 64, by = g.sy + 64;
     const int tz = tailCap < 64 ? tailCap : 64;
-    hipLaunchKernelGGL(k_merge, dim3(bx / 32, by / 8, g.K + tz), dim3(256), 0, s, g, bs, itemR, mainSlots, bev,
-                       tailItemR, tailSlots, tailBev, planeRM);
+    dispatch<MergeK>({g, bs, itemR, mainSlots, bev, tailItemR, tailSlots, tailBev, planeRM}, dim3(bx / 32, by / 8, g.K + tz), 0, s);
     return hipGetLastError();
 }
 
@@ -2139,7 +2688,7 @@ hipError_t launch_transfer(const FanGeom &g, const TransferArgs &ta, const BeamS
     grid.x = unsigned((e[0] + t[0] - 1) / t[0]);
     grid.y = unsigned((e[1] + t[1] - 1) / t[1]);
     grid.z = unsigned((e[2] + t[2] - 1) / t[2]);
-    hipLaunchKernelGGL(k_transfer, grid, dim3(256), 0, s, g, ta, bs, gs, bev, tailBev, dose, t[0], t[1], t[2]);
+    dispatch<TransferK>({g, ta, bs, gs, bev, tailBev, dose, t[0], t[1], t[2]}, grid, 0, s, kTransferKey);
     return hipGetLastError();
 }
 

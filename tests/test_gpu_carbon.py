@@ -5,7 +5,8 @@ The four carbon beams of the batching case (config-1 CT, 20 layers within 100-40
 plan integers, transfer box, tile radii (the oracle's float rule applied to the GPU's sigma),
 radius histograms and batched launch radii (the oracle's own plan) bit-exact; dose within 0.5 % of Dmax with the halo branch off and on (FLUKA constants; halo
 radius histograms bit-exact under the same rule); and the whole 16-beam mixed batch (12 proton +
-4 carbon beams, each set on its own ``Batch``) against the sum of the oracle's per-beam doses.
+4 carbon beams in one ``Batch``, the carbon tables as a second table set) against the sum of the
+oracle's per-beam doses.
 """
 import numpy as np
 import pytest
@@ -124,16 +125,16 @@ def test_mixed_batch_config3(config3):
     case = config3
     nx, ny, nz = case["dose_dims"]
     total = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
-    for kind, energy, model in (("proton", case["energy"], 0),
-                                ("carbon", case["carbon_energy"], case["carbon_nuclear"])):
-        beams = [b for b, k in zip(case["beams"], case["kinds"]) if k == kind]
-        bt = Batch(energy, lanes=4)
-        bt.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
-        bt.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
-        bt.set_nuclear(model)
-        bt.dose(beams, total)
-        bt.sync()
-        bt.close()
+    bt = Batch(case["energy"], lanes=4)
+    carbon = bt.add_table(case["carbon_energy"], case["carbon_nuclear"])
+    bt.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    bt.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    bt.dose(case["beams"], total, tables=[carbon if k == "carbon" else 0 for k in case["kinds"]])
+    bt.sync()
+    launches = bt.last_launches
+    bt.close()
+    print(f"config3 mixed batch: {launches} launches for {len(case['beams'])} beams")
+    assert 0 < launches <= 40
     got = total.cpu().numpy()
     ref = np.zeros((nz, ny, nx))
     for i, kind in enumerate(case["kinds"]):
