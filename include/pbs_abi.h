@@ -321,8 +321,11 @@ int pbs_batch_add_table(pbs_batch *b, const pbs_energy_struct *energy, int nucle
  * every table set share the passes. PBS_E_UNSUPPORTED for a beam the passes do not cover. */
 int pbs_batch_dose_tables(pbs_batch *b, const pbs_beam_settings *beams, const int *tableIds, size_t numBeams,
                           float *doseDevice);
-/* Launches the last multi-beam call issued (0 when the lanes ran it). */
+/* Launches the last multi-beam call issued (0 when the lanes ran it). The launches of a
+ * multi-beam call form one HIP graph, captured when the call's launch list is new and replayed
+ * while later calls repeat it; pbs_batch_graph_captures counts the captures. */
 int pbs_batch_last_launches(pbs_batch *b);
+int pbs_batch_graph_captures(pbs_batch *b);
 /* Serving: queue one beam whose dose is accumulated into its own doseDevice buffer, on the
  * next lane (round robin). *ticket identifies the request for pbs_batch_query / wait. */
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDevice, long long *ticket);

@@ -108,7 +108,7 @@ EXPORTED = [
     "pbs_final_dose_bind", "cudaFinalDose", "pbs_final_dose_roi",
     "pbs_batch_create", "pbs_batch_destroy", "pbs_batch_set_ct", "pbs_batch_set_dose_grid", "pbs_batch_set_nuclear",
     "pbs_batch_dose", "pbs_batch_submit", "pbs_batch_query", "pbs_batch_wait", "pbs_batch_sync", "pbs_batch_lanes",
-    "pbs_batch_add_table", "pbs_batch_dose_tables", "pbs_batch_last_launches",
+    "pbs_batch_add_table", "pbs_batch_dose_tables", "pbs_batch_last_launches", "pbs_batch_graph_captures",
     "pbs_server_create", "pbs_server_destroy", "pbs_server_add_ct", "pbs_server_remove_ct", "pbs_server_submit",
     "pbs_server_submit_weights", "pbs_server_query", "pbs_server_wait", "pbs_server_result",
     "pbs_server_result_device", "pbs_server_timing",
@@ -199,6 +199,8 @@ def lib():
                                             C.c_void_p]
         L.pbs_batch_last_launches.restype = C.c_int
         L.pbs_batch_last_launches.argtypes = [C.c_void_p]
+        L.pbs_batch_graph_captures.restype = C.c_int
+        L.pbs_batch_graph_captures.argtypes = [C.c_void_p]
         L.pbs_batch_submit.restype = C.c_int
         L.pbs_batch_submit.argtypes = [C.c_void_p, C.POINTER(BeamSettings), C.c_void_p, C.POINTER(C.c_longlong)]
         L.pbs_batch_query.restype = C.c_int

@@ -99,6 +99,11 @@ class Batch:
             _abi.check(rc, "pbs_batch_dose_tables")
 
     @property
+    def graph_captures(self) -> int:
+        """HIP graphs captured so far (a repeated batch shape replays its graph)."""
+        return int(_abi.lib().pbs_batch_graph_captures(self.handle))
+
+    @property
     def last_launches(self) -> int:
         """Kernel launches of the last multi-beam call (0 when the lanes ran it)."""
         return int(_abi.lib().pbs_batch_last_launches(self.handle))

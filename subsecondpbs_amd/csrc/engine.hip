@@ -1866,6 +1866,11 @@ struct pbs_batch {
     bool tablePending = false;
     bool multiUsed = false;
     int lastLaunches = 0;
+    // the passes' HIP graph and the launch list and table it was captured with
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<RecLaunch> graphLaunches;
+    const unsigned char *graphTable = nullptr;
+    int graphCaptures = 0;
     bool haveCt = false;
     int ctDims[3] = {};
     float ctRes[3] = {}, ctCorner[3] = {};
@@ -1937,6 +1942,7 @@ void pbs_batch_destroy(pbs_batch *b) {
     pbs_engine_destroy(b->owner);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->tableDone) (void)hipEventDestroy(b->tableDone);
+    if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
     if (b->hostTable) (void)hipHostFree(b->hostTable);
     for (auto &p : b->pending) (void)hipEventDestroy(p.second);
     for (hipEvent_t ev : b->freeEvents) (void)hipEventDestroy(ev);
@@ -2048,7 +2054,7 @@ static pbs_engine *batch_slot(pbs_batch *b, int t, size_t j) {
 
 // The multi-beam form: every beam's chain is recorded on its own engine (planning, buffers and
 // staging as for one beam, no launch), then the batch stream issues each pass once for all
-// beams (run_records): the launches of a batch do not grow with its beam count.
+// beams (stage_records, issue_records): the launches of a batch do not grow with its beam count.
 static int batch_dose_multi(pbs_batch *b, const pbs_beam_settings *beams, const int *tableIds, size_t numBeams,
                             float *doseDev) {
     if (!b->haveCt) return set_error(PBS_E_INVALID, "no CT bound to the batch");
@@ -2087,8 +2093,29 @@ static int batch_dose_multi(pbs_batch *b, const pbs_beam_settings *beams, const 
         b->hostTableBytes = cap;
         HIP_TRY(b->devTable.ensure(cap));
     }
-    int launches = 0;
-    HIP_TRY(run_records(rec.recs, b->hostTable, b->devTable.p, b->stream, &launches));
+    std::vector<RecLaunch> launches;
+    const size_t bytes = stage_records(rec.recs, b->hostTable, launches);
+    if (bytes) HIP_TRY(hipMemcpyAsync(b->devTable.p, b->hostTable, bytes, hipMemcpyHostToDevice, b->stream));
+    // the passes as one HIP graph: captured when the launch list is new, replayed while it repeats
+    // (the per-beam arguments live in the table copied above, so a replay serves new beams of the
+    // same shape)
+    if (!(b->graphExec && launches == b->graphLaunches && b->devTable.p == b->graphTable)) {
+        if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+        b->graphExec = nullptr;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t ie = issue_records(launches, b->devTable.p, b->stream);
+        const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+        HIP_TRY(ie);
+        HIP_TRY(ce);
+        const hipError_t xe = hipGraphInstantiate(&b->graphExec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(xe);
+        b->graphLaunches = launches;
+        b->graphTable = b->devTable.p;
+        ++b->graphCaptures;
+    }
+    HIP_TRY(hipGraphLaunch(b->graphExec, b->stream));
     HIP_TRY(hipEventRecord(b->tableDone, b->stream));
     b->tablePending = true;
     // each engine's staging is read by this batch's upload pass: its next beam waits for it
@@ -2096,7 +2123,7 @@ static int batch_dose_multi(pbs_batch *b, const pbs_beam_settings *beams, const 
         HIP_TRY(hipEventRecord(e->uploadDone, b->stream));
         e->uploadPending = true;
     }
-    b->lastLaunches = launches;
+    b->lastLaunches = int(launches.size());
     b->multiUsed = true;
     return PBS_OK;
 }
@@ -2133,6 +2160,8 @@ int pbs_batch_dose_tables(pbs_batch *b, const pbs_beam_settings *beams, const in
 }
 
 int pbs_batch_last_launches(pbs_batch *b) { return b ? b->lastLaunches : 0; }
+
+int pbs_batch_graph_captures(pbs_batch *b) { return b ? b->graphCaptures : 0; }
 
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev, long long *ticket) {
     if (!b || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");

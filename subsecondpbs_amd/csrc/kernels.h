@@ -15,7 +15,7 @@ namespace pbs {
 // Every kernel of the dose chain is a body that runs one beam's grid. Called alone it is
 // launched as usual. In the batched form the engine runs with a Recorder installed: each launch
 // becomes a LaunchRec (the kernel's arguments, grid and LDS), and the batch executor
-// (run_records) issues the records of one pass for all beams of the batch as a single launch in
+// (stage_records, issue_records) issues the records of one pass for all beams of the batch as a single launch in
 // which every work-group finds its beam in a table of first work-groups.
 struct MultiTable {
     const unsigned *base;  // [nb + 1]: first work-group of each beam's grid; base[nb] = total
@@ -51,13 +51,27 @@ extern thread_local Recorder *t_rec;
 // in beam order (main grid before halo grid), so each voxel adds the beams in the order the
 // one-beam-at-a-time form does.
 constexpr int kTransferKey = 1 << 24;
-// Bytes of the device table run_records needs for `recs` (arguments, first work-groups, grids).
+// Bytes of the table stage_records writes for `recs` (arguments, first work-groups, grids).
 size_t records_table_bytes(const std::vector<LaunchRec> &recs);
-// Issues `recs` on `s`: records of equal key and kernel as one multi-beam launch, in key order.
-// The table is written to the pinned host buffer `hostTable` and copied to `devTable` (both at
-// least records_table_bytes). Returns the number of launches in *launches.
-hipError_t run_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, unsigned char *devTable,
-                       hipStream_t s, int *launches);
+// One multi-beam launch of a batch: its kernel, where its table lies (offsets from the table's
+// start: Args array, first work-groups, grids), beams, work-groups and LDS. Two batches whose
+// launch lists are equal differ only in their tables' contents.
+struct RecLaunch {
+    ManyFn many;
+    size_t off, baseOff, dimsOff;
+    int nb;
+    unsigned blocks;
+    size_t lds;
+    bool operator==(const RecLaunch &o) const {
+        return many == o.many && off == o.off && baseOff == o.baseOff && dimsOff == o.dimsOff && nb == o.nb &&
+               blocks == o.blocks && lds == o.lds;
+    }
+};
+// Writes the table of `recs` (at least records_table_bytes) to `hostTable` and lists the launches:
+// records of equal key and kernel as one multi-beam launch, in key order. Returns the bytes.
+size_t stage_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, std::vector<RecLaunch> &out);
+// Issues the launches on `s`, reading the table from `devTable` (a device copy of the host table).
+hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s);
 
 constexpr int kMaxRadDev = 32;
 constexpr int kMinTilesDev = 16;

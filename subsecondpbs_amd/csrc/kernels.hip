@@ -2372,52 +2372,46 @@ size_t records_table_bytes(const std::vector<LaunchRec> &recs) {
     return b;
 }
 
-hipError_t run_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, unsigned char *devTable,
-                       hipStream_t s, int *launches) {
+size_t stage_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, std::vector<RecLaunch> &out) {
     std::vector<size_t> order;
     std::vector<RecGroup> groups;
     plan_groups(recs, order, groups);
-    std::vector<size_t> offs(groups.size());
-    std::vector<unsigned> total(groups.size());
+    out.clear();
     size_t off = 0;
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         const RecGroup &g = groups[gi];
-        const size_t argSize = recs[order[g.first]].args.size();
+        const LaunchRec &head = recs[order[g.first]];
+        const size_t argSize = head.args.size();
         unsigned char *h = hostTable + off;
-        offs[gi] = off;
-        unsigned *base = reinterpret_cast<unsigned *>(h + align_up(argSize * g.count, 16));
-        uint4 *dims = reinterpret_cast<uint4 *>(reinterpret_cast<unsigned char *>(base) +
-                                                align_up(sizeof(unsigned) * (g.count + 1), 16));
+        const size_t baseOff = align_up(argSize * g.count, 16);
+        const size_t dimsOff = baseOff + align_up(sizeof(unsigned) * (g.count + 1), 16);
+        unsigned *base = reinterpret_cast<unsigned *>(h + baseOff);
+        uint4 *dims = reinterpret_cast<uint4 *>(h + dimsOff);
         unsigned tot = 0;
+        size_t lds = 0;
         for (size_t j = 0; j < g.count; ++j) {
             const LaunchRec &r = recs[order[g.first + j]];
             std::memcpy(h + j * argSize, r.args.data(), argSize);
             base[j] = tot;
             dims[j] = uint4{r.grid.x, r.grid.y, r.grid.z, 0u};
             tot += r.grid.x * r.grid.y * r.grid.z;
+            lds = std::max(lds, r.lds);
         }
         base[g.count] = tot;
-        total[gi] = tot;
+        out.push_back(RecLaunch{head.many, off, baseOff, dimsOff, int(g.count), head.manyBlocks ? head.manyBlocks : tot,
+                                lds});
         off += group_bytes(argSize, g.count);
     }
-    if (off) {
-        const hipError_t e = hipMemcpyAsync(devTable, hostTable, off, hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return e;
+    return off;
+}
+
+hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s) {
+    for (const RecLaunch &l : launches) {
+        unsigned char *d = devTable + l.off;
+        const MultiTable t{reinterpret_cast<const unsigned *>(d + l.baseOff), reinterpret_cast<const uint4 *>(d + l.dimsOff),
+                           l.nb};
+        l.many(d, t, l.blocks, l.lds, s);
     }
-    for (size_t gi = 0; gi < groups.size(); ++gi) {
-        const RecGroup &g = groups[gi];
-        const LaunchRec &h = recs[order[g.first]];
-        const size_t argSize = h.args.size();
-        unsigned char *d = devTable + offs[gi];
-        const unsigned *base = reinterpret_cast<const unsigned *>(d + align_up(argSize * g.count, 16));
-        const uint4 *dims = reinterpret_cast<const uint4 *>(reinterpret_cast<const unsigned char *>(base) +
-                                                            align_up(sizeof(unsigned) * (g.count + 1), 16));
-        size_t lds = 0;
-        for (size_t j = 0; j < g.count; ++j) lds = std::max(lds, recs[order[g.first + j]].lds);
-        const MultiTable t{base, dims, int(g.count)};
-        h.many(d, t, h.manyBlocks ? h.manyBlocks : total[gi], lds, s);
-    }
-    if (launches) *launches = int(groups.size());
     return hipGetLastError();
 }
 
