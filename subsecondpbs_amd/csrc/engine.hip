@@ -1871,6 +1871,7 @@ struct pbs_batch {
     std::vector<RecLaunch> graphLaunches;
     const unsigned char *graphTable = nullptr;
     int graphCaptures = 0;
+    ForkJoin fj{};  // side streams of the independent launches (radius classes)
     bool haveCt = false;
     int ctDims[3] = {};
     float ctRes[3] = {}, ctCorner[3] = {};
@@ -1910,11 +1911,16 @@ pbs_batch *pbs_batch_create(int gpuId, const pbs_energy_struct *energy, int lane
     b->tables.emplace_back(new HostEnergy(*energy));
     b->tableNuclear.push_back(0);
     b->slots.emplace_back();
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->tableDone, hipEventDisableTiming) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->tableDone, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->fj.fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kSideStreams; ++i)
+        ok = hipStreamCreateWithFlags(&b->fj.side[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->fj.join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         set_error(PBS_E_DEVICE, "hipStreamCreate failed");
         pbs_engine_destroy(b->owner);
-        return nullptr;
+        return nullptr;  // the streams and events made so far stay with the process
     }
     for (int i = 0; i < lanes; ++i) {
         pbs_engine *e = engine_create(gpuId, energy, true);
@@ -1943,6 +1949,11 @@ void pbs_batch_destroy(pbs_batch *b) {
     if (b->stream) (void)hipStreamDestroy(b->stream);
     if (b->tableDone) (void)hipEventDestroy(b->tableDone);
     if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+    for (int i = 0; i < kSideStreams; ++i) {
+        if (b->fj.side[i]) (void)hipStreamDestroy(b->fj.side[i]);
+        if (b->fj.join[i]) (void)hipEventDestroy(b->fj.join[i]);
+    }
+    if (b->fj.fork) (void)hipEventDestroy(b->fj.fork);
     if (b->hostTable) (void)hipHostFree(b->hostTable);
     for (auto &p : b->pending) (void)hipEventDestroy(p.second);
     for (hipEvent_t ev : b->freeEvents) (void)hipEventDestroy(ev);
@@ -2104,7 +2115,7 @@ static int batch_dose_multi(pbs_batch *b, const pbs_beam_settings *beams, const 
         b->graphExec = nullptr;
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const hipError_t ie = issue_records(launches, b->devTable.p, b->stream);
+        const hipError_t ie = issue_records(launches, b->devTable.p, b->stream, &b->fj);
         const hipError_t ce = hipStreamEndCapture(b->stream, &graph);
         HIP_TRY(ie);
         HIP_TRY(ce);

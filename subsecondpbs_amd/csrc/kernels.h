@@ -9,6 +9,8 @@
 
 namespace pbs {
 
+constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
+
 // ---------------------------------------------------------------------------------------
 // Multi-beam launches
 // ---------------------------------------------------------------------------------------
@@ -36,9 +38,11 @@ struct LaunchRec {
     size_t lds;
     unsigned manyBlocks;  // work-groups of the multi-beam launch; 0: the beams' grids concatenated
     const void *target;   // records that share a launch share this too (the transfers' dose grid)
+    bool par;             // one of a run of launches that do not depend on each other (radius classes)
 };
 struct Recorder {
     int beam = 0, stage = 0, sub = 0, seq = 0;
+    bool par = false;  // set around launches that do not depend on each other
     std::vector<LaunchRec> recs;
     void set_stage(int st) {
         stage = st;
@@ -62,16 +66,25 @@ struct RecLaunch {
     int nb;
     unsigned blocks;
     size_t lds;
+    bool par;  // may run beside the neighbouring launches marked par
     bool operator==(const RecLaunch &o) const {
         return many == o.many && off == o.off && baseOff == o.baseOff && dimsOff == o.dimsOff && nb == o.nb &&
-               blocks == o.blocks && lds == o.lds;
+               blocks == o.blocks && lds == o.lds && par == o.par;
     }
+};
+// Side streams and events for the independent launches of a batch: a run of par launches is
+// dealt round robin to the main stream and the sides (forked from and joined to the main one).
+struct ForkJoin {
+    hipStream_t side[kSideStreams];
+    hipEvent_t fork;
+    hipEvent_t join[kSideStreams];
 };
 // Writes the table of `recs` (at least records_table_bytes) to `hostTable` and lists the launches:
 // records of equal key and kernel as one multi-beam launch, in key order. Returns the bytes.
 size_t stage_records(const std::vector<LaunchRec> &recs, unsigned char *hostTable, std::vector<RecLaunch> &out);
 // Issues the launches on `s`, reading the table from `devTable` (a device copy of the host table).
-hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s);
+hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s,
+                         const ForkJoin *fj);
 
 constexpr int kMaxRadDev = 32;
 constexpr int kMinTilesDev = 16;
@@ -81,7 +94,6 @@ constexpr int kNumClasses = 5;
 constexpr int kMaxLayersDev = 256;
 // transfer voxel bricks (x, y along the dose grid's cross-beam axes, z along its depth axis)
 constexpr int kNumTransferShapes = 5;
-constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
 
 struct ConvArgs {
     int rx, ry, spotX, spotY, nSub, subspotMode;

@@ -2310,6 +2310,7 @@ static void dispatch(const typename K::Args &a, dim3 grid, size_t lds, hipStream
     r.lds = lds;
     r.manyBlocks = many_blocks<K>(a);
     r.target = many_target<K>(a);
+    r.par = t_rec->par;
     t_rec->recs.push_back(std::move(r));
 }
 
@@ -2399,18 +2400,44 @@ size_t stage_records(const std::vector<LaunchRec> &recs, unsigned char *hostTabl
         }
         base[g.count] = tot;
         out.push_back(RecLaunch{head.many, off, baseOff, dimsOff, int(g.count), head.manyBlocks ? head.manyBlocks : tot,
-                                lds});
+                                lds, head.par});
         off += group_bytes(argSize, g.count);
     }
     return off;
 }
 
-hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s) {
-    for (const RecLaunch &l : launches) {
+hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *devTable, hipStream_t s,
+                         const ForkJoin *fj) {
+    auto issue = [&](const RecLaunch &l, hipStream_t st) {
         unsigned char *d = devTable + l.off;
-        const MultiTable t{reinterpret_cast<const unsigned *>(d + l.baseOff), reinterpret_cast<const uint4 *>(d + l.dimsOff),
-                           l.nb};
-        l.many(d, t, l.blocks, l.lds, s);
+        const MultiTable t{reinterpret_cast<const unsigned *>(d + l.baseOff),
+                           reinterpret_cast<const uint4 *>(d + l.dimsOff), l.nb};
+        l.many(d, t, l.blocks, l.lds, st);
+    };
+    size_t i = 0;
+    while (i < launches.size()) {
+        size_t j = i + 1;
+        if (launches[i].par)
+            while (j < launches.size() && launches[j].par) ++j;
+        if (j - i == 1 || !fj) {
+            for (size_t q = i; q < j; ++q) issue(launches[q], s);
+        } else {
+            // fork: launch q of the run goes to stream (q - i) mod (1 + sides); join before the next
+            hipError_t e = hipEventRecord(fj->fork, s);
+            if (e != hipSuccess) return e;
+            const size_t lanes = std::min<size_t>(j - i, 1 + kSideStreams);
+            for (size_t k = 1; k < lanes; ++k)
+                if ((e = hipStreamWaitEvent(fj->side[k - 1], fj->fork, 0)) != hipSuccess) return e;
+            for (size_t q = i; q < j; ++q) {
+                const size_t k = (q - i) % lanes;
+                issue(launches[q], k ? fj->side[k - 1] : s);
+            }
+            for (size_t k = 1; k < lanes; ++k) {
+                if ((e = hipEventRecord(fj->join[k - 1], fj->side[k - 1])) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(s, fj->join[k - 1], 0)) != hipSuccess) return e;
+            }
+        }
+        i = j;
     }
     return hipGetLastError();
 }
@@ -2645,7 +2672,7 @@ hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int 
                             float *mainSlots, float *tailSlots, int gridCap,
                             const hipStream_t streams[1 + kSideStreams]) {
     const int grid = maxUnits < gridCap ? maxUnits : gridCap;
-    // the heavy middle classes on separate streams; light ones share
+    if (t_rec) t_rec->par = true;  // the classes write disjoint slots: recorded as independent
 #define PBS_CLASS(RMV, C, S)                                                                                       \
     launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxUnits, slotLayers,   \
                       mainSlots, tailSlots, grid, streams[S])
@@ -2655,6 +2682,7 @@ hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int 
     PBS_CLASS(24, 3, 3);
     PBS_CLASS(32, 4, 3);
 #undef PBS_CLASS
+    if (t_rec) t_rec->par = false;
     return hipGetLastError();
 }
 
