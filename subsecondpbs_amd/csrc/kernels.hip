@@ -1724,15 +1724,18 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("" ::: "memory");
 }
 
-// Launch shape of the row-lane superposition of class RM: a group of G >= 2RM + 1 lanes holds
-// the output rows y_s - RM .. y_s + RM of one source row y_s (lane l: vertical offset
-// l - RM); eight groups (one per source row of the tile) make a work-group of 8G threads. Each
-// source's table row is [d, e_0 .. e_R, 0 ...] (EW floats).
+// Launch shape of the row-lane superposition of class RM. The vertical weight of a source row
+// is e_|dy|, the same for the output rows y_s - q and y_s + q: a group of G >= RM + 1 lanes holds
+// one source row y_s and one half of its sources (x_s in [16h, 16h + 16)), lane q sweeping the
+// horizontal rows of |dy| = q once for both output rows. Sixteen groups (eight source rows times
+// two halves; a wave holds groups of one half) make a work-group of 16G threads. Each source's
+// table row is [d, e_0 .. e_R, 0 ...] (EW floats).
 template <int RM>
 struct RowsShape {
     static constexpr int BH = 8 + 2 * RM, BW = 32 + 2 * RM;
-    static constexpr int G = 2 * RM + 1 <= 8 ? 8 : (2 * RM + 1 <= 16 ? 16 : (2 * RM + 1 <= 32 ? 32 : 64));
-    static constexpr int NT = 8 * G;
+    static constexpr int HW = 16 + 2 * RM;  // the columns one half's sources reach
+    static constexpr int G = RM + 1 <= 8 ? 8 : (RM + 1 <= 16 ? 16 : (RM + 1 <= 32 ? 32 : 64));
+    static constexpr int NT = 16 * G;
     static constexpr int EW = pad4(RM + 3);
     // With EW a multiple of 16 floats, the 32 table rows of a source row would start on 4 bank
     // groups only, and their 16-byte stores would conflict 8 ways: every fourth column then moves
@@ -1740,32 +1743,37 @@ struct RowsShape {
     static constexpr bool SWZ = EW % 16 == 0;
     static constexpr int SPAN = 32 * EW + (SWZ ? 32 : 0);
     // source-row pitch: SPAN rounded up to G modulo 64 floats puts the 64 / G groups of a wave at
-    // bank offsets 0, G, 2G, ...: each group's per-lane reads E[1 + v] (v <= RM + 1 < G) then fall
+    // bank offsets 0, G, 2G, ...: each group's per-lane reads E[1 + v] (v <= RM < G) then fall
     // on banks no other group of the wave touches
     static constexpr int RP = SPAN + ((G - SPAN % 64) % 64 + 64) % 64;
     static __device__ __forceinline__ int col_off(int col) { return col * EW + (SWZ ? 4 * (col >> 2) : 0); }
-    static constexpr int BHp = BH | 1;      // block column pitch (odd: conflict-free both ways)
-    static constexpr int EBUF = 8 * RP;     // one layer's table
-    static constexpr int LDS = (2 * EBUF > BW * BHp ? 2 * EBUF : BW * BHp);
+    static constexpr int EBUF = 8 * RP;  // one layer's table
+    // after the last layer: four groups' rows at a time (odd pitch: the lanes of a wave write
+    // distinct banks)
+    static constexpr int HWp = HW | 1;
+    static constexpr int ACC = 4 * G * HWp;
+    static constexpr int OUT = BH * BW;
+    static constexpr int LDS = EBUF > ACC ? EBUF : ACC;
     static constexpr int SPT = (256 + NT - 1) / NT;      // sources staged per thread
+    static constexpr int OPT = (OUT + NT - 1) / NT;      // output pixels per thread
     static constexpr int RLO = RM - 7 > 0 ? RM - 7 : 0;  // smallest compiled sweep radius
-    static_assert(2 * RM + 1 <= G - 1, "the last lane of a group must stay unused (pair combine)");
+    static_assert(RM + 1 <= G, "a group holds the vertical offsets 0 .. RM");
+    static_assert(8 * G % 64 == 0, "a wave holds groups of one half only");
 };
 
-// One source row of one layer into this lane's output row: for every source x_s, a = d e_v with
-// v = |dy| clamped to R+1 (a zero entry), then 2R+1 FMAs along the row, acc[x_s + dx + RM] +=
-// a e_|dx|. The sources go in chunks of four; a chunk where no source row of the wave has a live
+// One source row of one layer into this lane's row: for every source x_s, a = d e_v with v = |dy|
+// clamped to R+1 (a zero entry), then 2R+1 FMAs along the row, acc[x_s + dx + RM] += a e_|dx|. The sources go in chunks of four; a chunk where no source row of the wave has a live
 // pixel (bit x_s of wm, the OR of the wave's live masks, uniform over the wave) is skipped by a
 // scalar branch: its sources would add exact zeros. Within a chunk the sweep is branch-free. RP_
 // is the radius compiled; entries past the layer's own radius are zero, so a smaller layer
 // radius adds exact zeros.
-template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const float *__restrict__ rowE, int v,
+template <int RP_, int RM, int EW, int H>
+__device__ __forceinline__ void rows_sweep(float (&acc)[16 + 2 * RM], const float *__restrict__ rowE, int v,
                                            uint32_t wm) {
     constexpr int NE4 = (RP_ + 2 + 3) / 4;
     using S = RowsShape<RM>;
 #pragma unroll
-    for (int x0 = 0; x0 < 32; x0 += 4) {
+    for (int x0 = 16 * H; x0 < 16 * H + 16; x0 += 4) {
         if (!((wm >> x0) & 0xfu)) continue;  // uniform: wm is a scalar
 #pragma unroll
         for (int xs = x0; xs < x0 + 4; ++xs) {
@@ -1779,51 +1787,60 @@ __device__ __forceinline__ void rows_sweep(float (&acc)[32 + 2 * RM], const floa
             const float a = e[0] * E[1 + v];
 #pragma unroll
             for (int dx = -RP_; dx <= RP_; ++dx) {
-                const int c = xs + dx + RM;
+                const int c = xs - 16 * H + dx + RM;
                 acc[c] = fmaf(a, e[1 + (dx < 0 ? -dx : dx)], acc[c]);
             }
         }
     }
 }
 
-template <int RP_, int RM, int EW>
-__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[32 + 2 * RM], const float *rowE, int v,
+template <int RP_, int RM, int EW, int H>
+__device__ __forceinline__ void rows_dispatch(int rp, float (&acc)[16 + 2 * RM], const float *rowE, int v,
                                               uint32_t wm) {
     if constexpr (RP_ <= RM) {
         if (rp == RP_)
-            rows_sweep<RP_, RM, EW>(acc, rowE, v, wm);
+            rows_sweep<RP_, RM, EW, H>(acc, rowE, v, wm);
         else
-            rows_dispatch<RP_ + 1, RM, EW>(rp, acc, rowE, v, wm);
+            rows_dispatch<RP_ + 1, RM, EW, H>(rp, acc, rowE, v, wm);
     }
 }
 
 // Work-group per work unit (step, tile, block) of class RM (complete_superposition.cu:280-330:
-// e_0 = erf(r/2), e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product d e_x e_y). Lane l of
-// group g owns block row g + l (tile row g + l - RM) for source row g and keeps the 32 + 2RM
-// columns of that row in registers across all layers of the unit. Per layer the work-group
-// stages the tile's 256 table rows once (double-buffered, one barrier per layer; the next
-// layer's pixels are loaded during the current sweep); every source is then applied by all
-// lanes of its group at once, so the horizontal weights are shared reads and the register
-// index is static. The groups are added into the block in source-row order: bitwise
-// repeatable.
+// e_0 = erf(r/2), e_i = (erf(r(i+1/2)) - erf(r(i-1/2)))/2, outer product d e_x e_y). Lane q of
+// group (h, g) keeps, across all layers of the unit, the 16 + 2RM columns that the sources
+// [16h, 16h + 16) of source row g reach, weighted by e_q: their contribution to block rows
+// g + RM - q and g + RM + q alike. Per layer the work-group stages the tile's 256 table rows once
+// (one barrier before the sweep, one after it; the next layer's pixels are loaded during the
+// sweep); every source is then applied by all lanes of its group at once, so the horizontal
+// weights are shared reads and the register index is static. At the end each block pixel adds
+// the sixteen groups' rows in group order: bitwise repeatable.
 template <int RM>
-__device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float2 *__restrict__ iddSigma, const int *__restrict__ tileRad, const int *__restrict__ effRad,
-    const LayerScalars *__restrict__ ls, const GroupScalars *__restrict__ gs, const BeamScalars *__restrict__ bs,
-    const int *__restrict__ classCount, const int2 *__restrict__ classList, int cls, int maxUnits,
-    const int *__restrict__ slotLayers, float *__restrict__ mainSlots, float *__restrict__ tailSlots) {
+__device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g,
+                                                    const float2 *__restrict__ iddSigma,
+                                                    const int *__restrict__ tileRad, const int *__restrict__ effRad,
+                                                    const LayerScalars *__restrict__ ls,
+                                                    const GroupScalars *__restrict__ gs,
+                                                    const BeamScalars *__restrict__ bs,
+                                                    const int *__restrict__ classCount,
+                                                    const int2 *__restrict__ classList, int cls, int maxUnits,
+                                                    const int *__restrict__ slotLayers, float *__restrict__ mainSlots,
+                                                    float *__restrict__ tailSlots) {
     using S = RowsShape<RM>;
-    constexpr int BH = S::BH, BW = S::BW, G = S::G, NT = S::NT, EW = S::EW, RP = S::RP, BHp = S::BHp, SPT = S::SPT;
+    constexpr int BW = S::BW, HW = S::HW, G = S::G, NT = S::NT, EW = S::EW, RP = S::RP, HWp = S::HWp;
+    constexpr int SPT = S::SPT, OPT = S::OPT;
     __shared__ __align__(16) float s_lds[S::LDS];
-    __shared__ uint32_t s_live[2][8];
+    __shared__ uint32_t s_live[8];
     __shared__ int s_list[kMaxLayersDev];  // the unit's layers: l | R << 16
     __shared__ int s_n;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int grp = tid / G, ql = tid % G;  // source row, output row offset + RM
+    const int gi = tid / G, q = tid % G;  // group, vertical offset |dy|
+    const int grp = gi % 8, half = gi / 8;  // source row, half of the sources (uniform per wave)
     const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
     const int count = min(classCount[cls], maxUnits);
     const int2 *list = classList + size_t(cls) * maxUnits;
     const size_t planeXY = size_t(g.sy) * g.sx;
-    const int ady = ql > RM ? ql - RM : RM - ql;
+    float *const E = s_lds;  // the layer's table
+    float *const A = s_lds;  // after the layers: four groups' rows [4][G][HWp]
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
         const int2 unit = list[u];
         const int slot = unit.x, b = unit.y;
@@ -1859,9 +1876,9 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
         }
         __syncthreads();
         const int n = s_n;
-        float acc[BW];
+        float acc[HW];
 #pragma unroll
-        for (int c = 0; c < BW; ++c) acc[c] = 0.0f;
+        for (int c = 0; c < HW; ++c) acc[c] = 0.0f;
         // first layer's pixels
         float2 pre[SPT];
         {
@@ -1872,7 +1889,6 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
         }
         for (int li = 0; li < n; ++li) {
             const int R = s_list[li] >> 16;
-            const int buf = li & 1;
             float2 cur[SPT];
 #pragma unroll
             for (int i = 0; i < SPT; ++i) cur[i] = pre[i];
@@ -1882,7 +1898,6 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
 #pragma unroll
                 for (int i = 0; i < SPT; ++i) pre[i] = pixIn[i] ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
             }
-            float *E = s_lds + buf * S::EBUF;
 #pragma unroll
             for (int i = 0; i < SPT; ++i) {
                 const int src = tid + i * NT;
@@ -1890,8 +1905,8 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
                 const float d = cur[i].x, r = cur[i].y;
                 const uint64_t live = __ballot(d != 0.0f);
                 if (lane == 0) {
-                    s_live[buf][src >> 5] = uint32_t(live);
-                    s_live[buf][(src >> 5) + 1] = uint32_t(live >> 32);
+                    s_live[src >> 5] = uint32_t(live);
+                    s_live[(src >> 5) + 1] = uint32_t(live >> 32);
                 }
                 float row[EW];
                 row[0] = d;
@@ -1916,33 +1931,48 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
             // skip the layer when no source row of the wave has a live pixel
             uint32_t wmv = 0;
 #pragma unroll
-            for (int h = 0; h < 64 / G; ++h) wmv |= s_live[buf][(w * 64) / G + h];
-            const uint32_t wm = __builtin_amdgcn_readfirstlane(wmv);
+            for (int h = 0; h < 64 / G; ++h) wmv |= s_live[((w * 64) / G + h) % 8];
+            const uint32_t wm = __builtin_amdgcn_readfirstlane(wmv) & (half ? 0xffff0000u : 0x0000ffffu);
             if (wm) {
-                const int v = ady < R + 1 ? ady : R + 1;
+                const int v = q < R + 1 ? q : R + 1;
                 const int rp = R > S::RLO ? R : S::RLO;
-                rows_dispatch<S::RLO, RM, EW>(rp, acc, E + grp * RP, v, wm);
+                if (half)
+                    rows_dispatch<S::RLO, RM, EW, 1>(rp, acc, E + grp * RP, v, wm);
+                else
+                    rows_dispatch<S::RLO, RM, EW, 0>(rp, acc, E + grp * RP, v, wm);
             }
+            lds_barrier();  // the table is read: the next layer may overwrite it
         }
-        __syncthreads();  // tables dead: the block reuses their LDS
-        float *s_B = s_lds;
-        for (int i = tid; i < BW * BHp; i += NT) s_B[i] = 0.0f;
-        __syncthreads();
-        // pairs of source-row groups first: the odd group's lane ql - 1 holds the even group's row
-        // ql (lane G - 1 of every group is beyond 2RM and holds zeros)
+        // block pixel (Y, X) = the sum over the groups (h, g), in group order, of row |Y - RM - g|
+        // of the group at column X - 16h, where |Y - RM - g| <= RM and 0 <= X - 16h < 16 + 2RM;
+        // four groups at a time through A
+        float part[OPT];
 #pragma unroll
-        for (int c = 0; c < BW; ++c) acc[c] += __shfl(acc[c], lane + G - 1, 64);
-        // then the pairs in source-row order: within a wave by lane ranges, across waves by barriers
-        for (int ww = 0; ww < NT / 64; ++ww) {
-            if (w == ww) {
+        for (int j = 0; j < OPT; ++j) part[j] = 0.0f;
+#pragma unroll 1
+        for (int ph = 0; ph < 4; ++ph) {
+            if (gi / 4 == ph && q <= RM) {
+                float *a = A + ((gi % 4) * G + q) * HWp;
 #pragma unroll
-                for (int h = 0; h < 64 / G; h += 2) {
-                    const int row = grp + ql;
-                    if (lane / G == h && row < BH) {
+                for (int c = 0; c < HW; ++c) a[c] = acc[c];
+            }
+            __syncthreads();
 #pragma unroll
-                        for (int c = 0; c < BW; ++c) s_B[c * BHp + row] += acc[c];
+            for (int j = 0; j < OPT; ++j) {
+                const int o = tid + j * NT;
+                if (o >= S::OUT) break;
+                const int Y = o / BW, X = o % BW;
+                const int hx = X - 16 * (ph / 2);  // the column within this phase's half
+                float sum = part[j];
+                if (hx >= 0 && hx < HW) {
+#pragma unroll
+                    for (int g4 = 0; g4 < 4; ++g4) {
+                        const int dy = Y - RM - ((4 * ph + g4) % 8);
+                        const int ady = dy < 0 ? -dy : dy;
+                        if (ady <= RM) sum += A[(g4 * G + ady) * HWp + hx];
                     }
                 }
+                part[j] = sum;
             }
             __syncthreads();
         }
@@ -1953,14 +1983,20 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
             const int qq = gs[b].tailBase + (k - gs[b].tailStart);
             out = (qq < bs->tailPlanes) ? tailSlots + (size_t(qq) * nt + t) * kSlotFloats : nullptr;
         }
-        if (out)
-            for (int i = tid; i < BH * BW; i += NT) out[(i / BW) * kSlotPitch + (i % BW)] = s_B[(i % BW) * BHp + i / BW];
-        __syncthreads();
+        if (out) {
+#pragma unroll
+            for (int j = 0; j < OPT; ++j) {
+                const int o = tid + j * NT;
+                if (o >= S::OUT) break;
+                out[(o / BW) * kSlotPitch + (o % BW)] = part[j];
+            }
+        }
     }
 }
 template <int RM>
 struct SuperposeRowsK {
     static constexpr int NT = RowsShape<RM>::NT;
+    static constexpr int WPE = RM <= 11 ? 3 : 1;  // 168 registers (three waves per SIMD) where it fits
     static constexpr bool kUnitQueue = true;  // batched as a queue of the beams' units
     struct Args {
         FanGeom g;
@@ -2185,8 +2221,19 @@ struct UploadK {
 // ---------------------------------------------------------------------------------------
 // one-beam and multi-beam launches of a kernel body K
 // ---------------------------------------------------------------------------------------
+// Waves per SIMD a kernel body is compiled for (its register budget): K::WPE, else no bound.
+template <typename K, typename = void>
+struct WavesPerEu {
+    static constexpr int value = 1;
+};
 template <typename K>
-__global__ __launch_bounds__(K::NT) void k_one(typename K::Args a) {
+struct WavesPerEu<K, std::void_t<decltype(K::WPE)>> {
+    static constexpr int value = K::WPE;
+};
+
+template <typename K>
+__global__ __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(WavesPerEu<K>::value))) void k_one(
+    typename K::Args a) {
     K::run(a, uint3{blockIdx.x, blockIdx.y, blockIdx.z}, uint3{gridDim.x, gridDim.y, gridDim.z});
 }
 
@@ -2194,7 +2241,8 @@ __global__ __launch_bounds__(K::NT) void k_one(typename K::Args a) {
 // is not after it (a scalar walk over at most a batch of entries), its index within that
 // beam's grid is unflattened x-fastest.
 template <typename K>
-__global__ __launch_bounds__(K::NT) void k_many(const typename K::Args *__restrict__ args, MultiTable t) {
+__global__ __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(WavesPerEu<K>::value))) void k_many(
+    const typename K::Args *__restrict__ args, MultiTable t) {
     const unsigned w = blockIdx.x;
     int b = 0;
     if (t.nb <= 64) {
@@ -2216,7 +2264,8 @@ __global__ __launch_bounds__(K::NT) void k_many(const typename K::Args *__restri
 // work-group after the previous beam's last, so the grid stays balanced whatever the counts.
 // The class bodies end every unit with a barrier, so a work-group may run them back to back.
 template <typename K>
-__global__ __launch_bounds__(K::NT) void k_many_units(const typename K::Args *__restrict__ args, int nb) {
+__global__ __launch_bounds__(K::NT) __attribute__((amdgpu_waves_per_eu(WavesPerEu<K>::value))) void k_many_units(
+    const typename K::Args *__restrict__ args, int nb) {
     const unsigned G = gridDim.x;
     unsigned start = 0;
     for (int b = 0; b < nb; ++b) {
