@@ -1839,8 +1839,15 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
     const int count = min(classCount[cls], maxUnits);
     const int2 *list = classList + size_t(cls) * maxUnits;
     const size_t planeXY = size_t(g.sy) * g.sx;
-    float *const E = s_lds;  // the layer's table
     float *const A = s_lds;  // after the layers: four groups' rows [4][G][HWp]
+    // this thread's block pixels (Y, X) in the final sum, X = pixel mod BW
+    int oy[OPT], ox[OPT];
+#pragma unroll
+    for (int j = 0; j < OPT; ++j) {
+        const int o = tid + j * NT;
+        oy[j] = o / BW;
+        ox[j] = o % BW;
+    }
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
         const int2 unit = list[u];
         const int slot = unit.x, b = unit.y;
@@ -1889,6 +1896,7 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
         }
         for (int li = 0; li < n; ++li) {
             const int R = s_list[li] >> 16;
+            float *E = s_lds;
             float2 cur[SPT];
 #pragma unroll
             for (int i = 0; i < SPT; ++i) cur[i] = pre[i];
@@ -1957,22 +1965,21 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
                 for (int c = 0; c < HW; ++c) a[c] = acc[c];
             }
             __syncthreads();
+            const int dx0 = 16 * (ph >> 1), dy0 = RM + 4 * (ph & 1);  // this phase's half and first row
 #pragma unroll
             for (int j = 0; j < OPT; ++j) {
-                const int o = tid + j * NT;
-                if (o >= S::OUT) break;
-                const int Y = o / BW, X = o % BW;
-                const int hx = X - 16 * (ph / 2);  // the column within this phase's half
-                float sum = part[j];
-                if (hx >= 0 && hx < HW) {
+                if (tid + j * NT >= S::OUT) break;
+                const int hx = ox[j] - dx0;  // the column within this phase's half
+                if (unsigned(hx) < unsigned(HW)) {
+                    const int y0 = oy[j] - dy0;
+                    float sum = part[j];
 #pragma unroll
                     for (int g4 = 0; g4 < 4; ++g4) {
-                        const int dy = Y - RM - ((4 * ph + g4) % 8);
-                        const int ady = dy < 0 ? -dy : dy;
+                        const int ady = abs(y0 - g4);
                         if (ady <= RM) sum += A[(g4 * G + ady) * HWp + hx];
                     }
+                    part[j] = sum;
                 }
-                part[j] = sum;
             }
             __syncthreads();
         }
