@@ -99,14 +99,14 @@ def measured_traffic() -> dict | None:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    # per-stage sums of the kernels' scaled read + write bytes
-    stage_of = (("k_trace", "trace"), ("k_fill", "fill"), ("k_transfer", "transfer"), ("k_merge", "merge"),
-                ("k_superpose", "superpose"))
+    # per-stage sums of the kernels' scaled read + write bytes (kernels named by their body:
+    # TraceSegK, FillK<false>, SuperposeRowsK<11>, ...)
+    stage_of = (("Trace", "trace"), ("Fill", "fill"), ("Transfer", "transfer"), ("Merge", "merge"),
+                ("Superpose", "superpose"))
     sums = {}
     for k, v in d.get("per_kernel_bytes", {}).items():
-        name = k.split("<")[0].split()[-1]
         for prefix, stage in stage_of:
-            if name.startswith(prefix):
+            if k.startswith(prefix):
                 sums[stage] = sums.get(stage, 0.0) + v["read_bytes"] + v["write_bytes"]
     for stage, b in sums.items():
         d.setdefault(f"{stage}_stage_bytes", b)
@@ -570,7 +570,7 @@ def main() -> int:
                             "frac": ach / FP32_PEAK_TFLOPS, "peak_unpacked": FP32_UNPACKED_PEAK_TFLOPS,
                             "frac_unpacked": ach / FP32_UNPACKED_PEAK_TFLOPS,
                             "traffic": traffic.get("superpose_stage_bytes"),
-                            "kernel": "superposition stage: k_superpose_rows<7/11/15> and k_superpose<24/32>, 5 "
+                            "kernel": "superposition stage: SuperposeRowsK<7/11/15> and SuperposeK<24/32>, 5 "
                                       "radius classes on 4 streams (stage wall time from HIP events)",
                             "compute": "fp32 VALU FMA (no matrix cores: no dense contraction); peak = the dense "
                                        "fp32 figure, peak_unpacked = without packed fp32", "stage_ms": t_ms}
