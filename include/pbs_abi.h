@@ -326,8 +326,9 @@ int pbs_batch_dose_tables(pbs_batch *b, const pbs_beam_settings *beams, const in
  * while later calls repeat it; pbs_batch_graph_captures counts the captures. */
 int pbs_batch_last_launches(pbs_batch *b);
 int pbs_batch_graph_captures(pbs_batch *b);
-/* Serving: queue one beam whose dose is accumulated into its own doseDevice buffer, on the
- * next lane (round robin). *ticket identifies the request for pbs_batch_query / wait. */
+/* Serving: queue one beam whose dose is accumulated into doseDevice, on the next lane (round
+ * robin); a doseDevice that a queued request still writes goes to that request's lane, so the
+ * two accumulate in order. *ticket identifies the request for pbs_batch_query / wait. */
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDevice, long long *ticket);
 /* 1 when the request's work has finished on the device, 0 while it runs. */
 int pbs_batch_query(pbs_batch *b, long long ticket);

@@ -6,8 +6,8 @@ beams that adds them voxel by voxel in beam order): the dose equals the one-beam
 bitwise, and the launches do not grow with the beam count. Beams of several energy-table sets
 (``add_table``: carbon beside protons) share the passes. A batch holding beams the multi-beam
 passes do not cover runs on the lanes instead (own HIP stream and workspaces each, sharing the
-resident CT; lane doses added in a fixed order). ``submit`` is the served form: each request
-gets its own dose buffer, so a request's result is the same alone and among others.
+resident CT; lane doses added in a fixed order). ``submit`` is the served form: a request with its own dose buffer gets the same result alone
+and among others; requests into one buffer are queued on one lane and add up in order.
 """
 from __future__ import annotations
 
@@ -109,7 +109,9 @@ class Batch:
         return int(_abi.lib().pbs_batch_last_launches(self.handle))
 
     def submit(self, beam: dict, dose) -> int:
-        """Queue one beam into its own ``dose`` buffer; returns a ticket."""
+        """Queue one beam whose dose is added into ``dose``; returns a ticket. A buffer that a
+        queued request still writes goes to that request's lane, so requests into one buffer
+        accumulate in submission order."""
         keep = _abi.Keep()
         bs = _abi.beam_settings(beam, keep)
         t = C.c_longlong(0)

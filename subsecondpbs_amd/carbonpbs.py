@@ -46,36 +46,43 @@ def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
 
 
-def to_float_vector(a) -> np.ndarray:
-    v = _f32(a)
-    if v.ndim > 1 and not (v.ndim == 2 and 1 in v.shape):
-        raise RuntimeError("Expected 1D float array or vector-like 2D float array")
-    return v.ravel()
+class ContractError(RuntimeError):
+    """A CarbonPBS argument of the wrong shape or size. A RuntimeError, like the exception the
+    reference's bound C++ raises for its argument checks."""
 
 
-def to_vec_columns(a, k: int, expected: int = 0) -> np.ndarray:
-    """(N, k) rows, (k, N) columns or a flat multiple of k -> flat [N][k] (pybind:141-223)."""
+def to_float_vector(a, name: str = "array") -> np.ndarray:
+    """``a`` as a flat float32 vector; a 2-D array counts as a vector when one extent is 1."""
     v = _f32(a)
-    if v.ndim == 1:
-        if v.size % k:
-            raise RuntimeError(f"Expected flattened {k}-vector array with length multiple of {k}")
-        out = v
+    if v.ndim <= 1 or (v.ndim == 2 and min(v.shape) == 1):
+        return v.ravel()
+    raise ContractError(f"{name}: got shape {v.shape}, a vector was expected")
+
+
+def to_vec_columns(a, k: int, expected: int = 0, name: str = "points") -> np.ndarray:
+    """k-component points as a flat [N][k] float32 array. Accepted forms, tried in this order
+    (pybind:141-223): a flat run of N*k values, rows (N, k), columns (k, N)."""
+    v = _f32(a)
+    if v.ndim == 1 and v.size % k == 0:
+        flat = v
     elif v.ndim == 2 and v.shape[1] == k:
-        out = v.ravel()
+        flat = v.ravel()
     elif v.ndim == 2 and v.shape[0] == k:
-        out = np.ascontiguousarray(v.T).ravel()
+        flat = np.ascontiguousarray(v.T).ravel()
     else:
-        raise RuntimeError(f"Expected shape (N,{k}) or ({k},N) for {k}-vector array")
-    if expected and out.size != expected * k:
-        raise RuntimeError(f"{k}-vector array length does not match expected count")
-    return out
+        raise ContractError(f"{name}: {k}-component points take shape (N, {k}), ({k}, N) or a flat length "
+                            f"divisible by {k}; got {v.shape}")
+    if expected and flat.size != expected * k:
+        raise ContractError(f"{name}: {flat.size // k} points given where {expected} are needed")
+    return flat
 
 
 def normalized_mean_direction(dirs: np.ndarray) -> np.ndarray:
-    """Mean of the per-spot directions (double sums), normalised in float (pybind:233-259)."""
+    """Unit mean of the per-spot directions: components summed in double, divided by the count,
+    then normalised in float (pybind:233-259)."""
     d = np.asarray(dirs, np.float32).reshape(-1, 3)
-    if d.size == 0:
-        raise RuntimeError("Direction array must contain one or more 3-vectors")
+    if not d.shape[0]:
+        raise ContractError("spot directions: none given")
     m = (d.astype(np.float64).sum(axis=0) / d.shape[0]).astype(np.float32)
     n = np.float32(np.sqrt(np.float32(m[0] * m[0] + m[1] * m[1] + m[2] * m[2])))
     return m / n if n > 0 else m
@@ -93,18 +100,18 @@ def _lround(v: float) -> int:
 
 def build_energy_from_carbonpbs(enelist, idddata, iddsetting, tables_dir: str | None = None) -> dict:
     """Energy dict from a CarbonPBS IDD table (pybind:521-638)."""
-    energies = to_float_vector(enelist)
-    if energies.size == 0:
-        raise RuntimeError("enelist must not be empty")
-    setting = to_float_vector(iddsetting)
+    energies = to_float_vector(enelist, "enelist")
+    if not energies.size:
+        raise ContractError("enelist: no energies given")
+    setting = to_float_vector(iddsetting, "iddsetting")
     if setting.size < 3:
-        raise RuntimeError("iddsetting must provide [start, step, nSamples]")
+        raise ContractError(f"iddsetting: {setting.size} values given; (start, step, samples) are needed")
     idd = _f32(idddata)
     if idd.ndim != 2:
-        raise RuntimeError("idddata must be a 2D array with shape (n_energies, n_samples)")
+        raise ContractError(f"idddata: got {idd.ndim} dimensions; a table (energies, depth samples) is needed")
     rows, cols = idd.shape
     if rows <= 0 or rows > energies.size:
-        raise RuntimeError("idddata rows must be in the range [1, enelist length]")
+        raise ContractError(f"idddata: {rows} rows for {energies.size} energies (1 .. {energies.size} rows allowed)")
     if rows != energies.size:
         energies = energies[:rows] if _slice_mode() == "head" else energies[energies.size - rows:]
     start_raw, step_raw = np.float32(setting[0]), np.float32(setting[1])
@@ -159,8 +166,8 @@ def derive_layer_longitudinal_cutoffs(per_spot, layer_counts) -> np.ndarray:
         cut = float(vals[0]) if vals.size else 0.0
         bad = np.nonzero(np.abs(vals - np.float32(cut)) > 1e-3)[0]
         if bad.size:
-            raise RuntimeError(f"longitudal_cutoff varies within layer {layer}: current RTD compatibility path "
-                               "only supports one longitudinal cutoff per layer")
+            raise ContractError(f"longitudal_cutoff varies within layer {layer} by more than 1e-3; "
+                                "one cut-off per layer is supported")
         out.append(cut)
         off += c
     return np.asarray(out, np.float32)
@@ -221,13 +228,13 @@ def to_roi_linear_indices(roi, dims) -> np.ndarray:
                 return (t[:, 0] * ny * nz + t[:, 1] * nz + t[:, 2]).astype(np.int32)
         return v.astype(np.int32)
     if a.ndim != 2:
-        raise RuntimeError("roi_index must be 1D xyz triplets or 2D (N,3)/(3,N)")
+        raise ContractError(f"roi_index: got {a.ndim} dimensions; flat triplets, (N, 3) or (3, N) are accepted")
     t = a.astype(np.int64)
     if t.shape[1] == 3:
         return (t[:, 0] * ny * nz + t[:, 1] * nz + t[:, 2]).astype(np.int32)
     if t.shape[0] == 3:
         return (t[0] * ny * nz + t[1] * nz + t[2]).astype(np.int32)
-    raise RuntimeError("roi_index must have shape (N,3) or (3,N)")
+    raise ContractError(f"roi_index: shape {a.shape} is neither (N, 3) nor (3, N)")
 
 
 def build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir, bmxdir, bmydir, corner,
@@ -240,48 +247,48 @@ def build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir,
     tail by default); ``carbon_profile_hooks`` (0, the reference's setting) switches its hooks."""
     d = [int(v) for v in np.asarray(dims).ravel()]
     if len(d) != 3:
-        raise RuntimeError("dims must contain exactly 3 integers")
-    corner_v, res_v = to_float_vector(corner), to_float_vector(resolution)
+        raise ContractError(f"dims: {len(d)} values given, 3 are needed")
+    corner_v, res_v = to_float_vector(corner, "corner"), to_float_vector(resolution, "resolution")
     if corner_v.size != 3 or res_v.size != 3:
-        raise RuntimeError("corner and resolution must contain exactly 3 floats")
+        raise ContractError(f"corner / resolution: {corner_v.size} / {res_v.size} values given, 3 each are needed")
     weq = _f32(rayweq).ravel()
     if weq.size < 9:
-        raise RuntimeError("rayweq must include the 9-value CarbonPBS header")
+        raise ContractError(f"rayweq: {weq.size} values, shorter than its 9-value header")
     h = weq[:9]
-    weights = to_float_vector(nPar)
+    weights = to_float_vector(nPar, "nPar")
     n = weights.size
-    spot_e = to_float_vector(all_energies)
+    spot_e = to_float_vector(all_energies, "all_energies")
     if spot_e.size and spot_e.size != n:
-        raise RuntimeError("all_energies length must match nPar length")
+        raise ContractError(f"all_energies: {spot_e.size} values for {n} spot weights")
     counts = np.asarray(layerInfo).astype(np.int32).ravel()
-    layer_e = to_float_vector(layerEnergy)
+    layer_e = to_float_vector(layerEnergy, "layerEnergy")
     if counts.size != layer_e.size:
-        raise RuntimeError("layerEnergy/layerInfo size mismatch")
-    dirs = to_vec_columns(tmpBeamDir, 3, n)
-    sad_v = to_float_vector(sad) if np.ndim(sad) else np.asarray([sad], np.float32)
-    if sad_v.size == 0:
-        raise RuntimeError("sad array must not be empty")
+        raise ContractError(f"layerEnergy has {layer_e.size} layers, layerInfo {counts.size}")
+    dirs = to_vec_columns(tmpBeamDir, 3, n, "tmpBeamDir")
+    sad_v = to_float_vector(sad, "sad") if np.ndim(sad) else np.asarray([sad], np.float32)
+    if not sad_v.size:
+        raise ContractError("sad: no value given")
     beam = {
         "water_equivalence": weq, "ray_spacing": np.array([h[7], h[4]], np.float32),
         "spot_delta": np.zeros(3, np.float32), "spot_offset": np.zeros(3, np.float32),
         "steps": max(1, int(np.round(float(h[2])))), "roi_linear_indices": to_roi_linear_indices(roiIdx, d),
         "energies": layer_e, "layer_spot_counts": counts, "spot_weights": weights,
-        "spot_positions": to_vec_columns(idbeamxy, 2, n), "spot_positions_are_indices": True,
+        "spot_positions": to_vec_columns(idbeamxy, 2, n, "idbeamxy"), "spot_positions_are_indices": True,
         "spot_beam_directions": dirs, "beam_direction": normalized_mean_direction(dirs),
-        "beam_xdir": to_vec_columns(bmxdir, 3)[:3], "beam_ydir": to_vec_columns(bmydir, 3)[:3],
-        "source_position": to_vec_columns(sourcePos, 3)[:3], "ref_plane_z": 0.0, "sad": float(sad_v[0]),
+        "beam_xdir": to_vec_columns(bmxdir, 3, name="bmxdir")[:3], "beam_ydir": to_vec_columns(bmydir, 3, name="bmydir")[:3],
+        "source_position": to_vec_columns(sourcePos, 3, name="sourcePos")[:3], "ref_plane_z": 0.0, "sad": float(sad_v[0]),
         "source_dist": np.zeros(2, np.float32),
     }
     if longitudalCutoff is not None:
-        per_spot = to_float_vector(longitudalCutoff)
+        per_spot = to_float_vector(longitudalCutoff, "longitudal_cutoff")
         if per_spot.size:
             if per_spot.size != int(counts.sum()):
-                raise RuntimeError("longitudal_cutoff length must match total spot count")
+                raise ContractError(f"longitudal_cutoff: {per_spot.size} values for {int(counts.sum())} spots")
             beam["layer_longitudinal_cutoffs"] = derive_layer_longitudinal_cutoffs(per_spot, counts)
     sub = _f32(subspotdata)
     if sub.ndim != 3 or sub.shape[2] != 5:
-        raise RuntimeError("subspotdata must have shape (num_layers, max_subspots_per_layer, 5)")
-    lut_e = to_float_vector(enelist)
+        raise ContractError(f"subspotdata: shape {sub.shape}; (layers, subspots per layer, 5) is needed")
+    lut_e = to_float_vector(enelist, "enelist")
     if sub.shape[0] == layer_e.size:
         beam["subspot_data"], beam["spot_sigmas"] = sub, derive_layer_sigmas_from_subspots(sub)
     else:
@@ -293,7 +300,7 @@ def build_carbonpbs_context(rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir,
         prof_depth = int(pa.shape[1]) if pa.ndim >= 2 else 0
         beam["profile_data"] = np.ascontiguousarray(pa).ravel()
     if profilesetting is not None:
-        ps = to_float_vector(profilesetting).copy()
+        ps = to_float_vector(profilesetting, "profilesetting").copy()
         if ps.size >= 3 and prof_depth > 0:
             logical = max(0, _lround(float(ps[2])))
             if logical > prof_depth:
@@ -367,7 +374,7 @@ def cu_final_dose(finalDose, rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir
     nx, ny, nz = ctx["dims"]
     out = finalDose
     if not isinstance(out, np.ndarray) or out.dtype != np.float32 or out.size != nx * ny * nz:
-        raise RuntimeError("out_final_dose size does not match dims")
+        raise ContractError(f"finalDose: a float32 array of {nx * ny * nz} voxels is needed for dims {ctx['dims']}")
     xyz = False
     if out.ndim == 1:
         native = out
@@ -378,8 +385,8 @@ def cu_final_dose(finalDose, rayweq, roiIdx, all_energies, sourcePos, tmpBeamDir
     elif out.ndim == 3 and out.flags.c_contiguous and out.shape == (nz, ny, nx):
         native = out.reshape(-1)
     else:
-        raise RuntimeError("out_final_dose must be either 1D flat, C-contiguous (Nz,Ny,Nx), or C-contiguous "
-                           "(Nx,Ny,Nz)")
+        raise ContractError(f"finalDose: shape {out.shape}; flat, or C-contiguous ({nz}, {ny}, {nx}) or "
+                            f"({nx}, {ny}, {nz}) is accepted")
     energy = _complete_nuclear(ctx["energy"], tables_dir) if nuclear_correction else ctx["energy"]
     ct = placeholder_ct(ctx["dims"], ctx["beam"]["roi_linear_indices"])
     subsecond_wrapper(ct, ctx["dims"], ctx["resolution"], ctx["corner"], native, ctx["dims"], ctx["resolution"],
@@ -449,7 +456,7 @@ def cuCalDose3(out_cscdata, out_cscptr, out_cscrowind, weq_data, roi_index, sour
               out_cscrowind.size)
     S = indptr.size - 1
     if out_cscptr.size < S + 1:
-        raise RuntimeError("out_cscptr must hold numSpots + 1 entries")
+        raise ContractError(f"out_cscptr: {out_cscptr.size} entries for {S} spots ({S + 1} needed)")
     done = S
     while done > 0 and indptr[done] > cap:
         done -= 1

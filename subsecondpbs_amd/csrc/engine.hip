@@ -1850,7 +1850,14 @@ struct pbs_batch {
     size_t doseN = 0;
     size_t next = 0;               // round-robin cursor of the served form
     long long tickets = 0;
-    std::vector<std::pair<long long, hipEvent_t>> pending;
+    // a submitted request: its ticket, the event after its work, its dose buffer and its lane
+    struct Pending {
+        long long ticket;
+        hipEvent_t done;
+        const float *dose;
+        int lane;
+    };
+    std::vector<Pending> pending;
     std::vector<hipEvent_t> freeEvents;
     hipEvent_t laneDone[16] = {};
     // multi-beam form: per energy-table set its host copy, nuclear model and the engines that
@@ -1955,7 +1962,7 @@ void pbs_batch_destroy(pbs_batch *b) {
     }
     if (b->fj.fork) (void)hipEventDestroy(b->fj.fork);
     if (b->hostTable) (void)hipHostFree(b->hostTable);
-    for (auto &p : b->pending) (void)hipEventDestroy(p.second);
+    for (auto &p : b->pending) (void)hipEventDestroy(p.done);
     for (hipEvent_t ev : b->freeEvents) (void)hipEventDestroy(ev);
     for (int i = 0; i < 16; ++i)
         if (b->laneDone[i]) (void)hipEventDestroy(b->laneDone[i]);
@@ -2177,7 +2184,18 @@ int pbs_batch_graph_captures(pbs_batch *b) { return b ? b->graphCaptures : 0; }
 int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev, long long *ticket) {
     if (!b || !beam || !doseDev) return set_error(PBS_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(b->device));
-    const int l = int(b->next++ % b->lanes.size());
+    // The transfer adds into the dose without atomics, so two requests into one buffer must not
+    // run on two lanes at once: a buffer with a request still queued goes to that request's lane,
+    // whose stream orders the two; other buffers take the lanes in turn.
+    int l = -1;
+    for (auto it = b->pending.rbegin(); it != b->pending.rend(); ++it) {
+        if (it->dose != doseDev) continue;
+        const hipError_t q = hipEventQuery(it->done);
+        if (q == hipErrorNotReady) l = it->lane;
+        else HIP_TRY(q);
+        break;
+    }
+    if (l < 0) l = int(b->next++ % b->lanes.size());
     pbs_engine *e = b->lanes[l];
     const int rc = pbs_engine_beam_dose(e, beam, doseDev);
     if (rc != PBS_OK) return rc;
@@ -2190,7 +2208,7 @@ int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev
     }
     HIP_TRY(hipEventRecord(ev, e->stream));
     const long long t = ++b->tickets;
-    b->pending.emplace_back(t, ev);
+    b->pending.push_back({t, ev, doseDev, l});
     if (ticket) *ticket = t;
     return PBS_OK;
 }
@@ -2198,11 +2216,11 @@ int pbs_batch_submit(pbs_batch *b, const pbs_beam_settings *beam, float *doseDev
 int pbs_batch_query(pbs_batch *b, long long ticket) {
     if (!b) return set_error(PBS_E_INVALID, "null batch");
     for (size_t i = 0; i < b->pending.size(); ++i) {
-        if (b->pending[i].first != ticket) continue;
-        const hipError_t q = hipEventQuery(b->pending[i].second);
+        if (b->pending[i].ticket != ticket) continue;
+        const hipError_t q = hipEventQuery(b->pending[i].done);
         if (q == hipErrorNotReady) return 0;
         HIP_TRY(q);
-        b->freeEvents.push_back(b->pending[i].second);
+        b->freeEvents.push_back(b->pending[i].done);
         b->pending.erase(b->pending.begin() + long(i));
         return 1;
     }
@@ -2212,9 +2230,9 @@ int pbs_batch_query(pbs_batch *b, long long ticket) {
 int pbs_batch_wait(pbs_batch *b, long long ticket) {
     if (!b) return set_error(PBS_E_INVALID, "null batch");
     for (size_t i = 0; i < b->pending.size(); ++i) {
-        if (b->pending[i].first != ticket) continue;
-        HIP_TRY(hipEventSynchronize(b->pending[i].second));
-        b->freeEvents.push_back(b->pending[i].second);
+        if (b->pending[i].ticket != ticket) continue;
+        HIP_TRY(hipEventSynchronize(b->pending[i].done));
+        b->freeEvents.push_back(b->pending[i].done);
         b->pending.erase(b->pending.begin() + long(i));
         break;
     }
@@ -2239,7 +2257,7 @@ int pbs_batch_sync(pbs_batch *b) {
                 if (r != PBS_OK && rc == PBS_OK) rc = r;
             }
     }
-    for (auto &p : b->pending) b->freeEvents.push_back(p.second);
+    for (auto &p : b->pending) b->freeEvents.push_back(p.done);
     b->pending.clear();
     return rc;
 }

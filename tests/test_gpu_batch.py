@@ -140,3 +140,32 @@ def test_batch_with_a_legacy_beam_runs_on_the_lanes(config3):
     e.close()
     got, want = dose.cpu().numpy(), ref.cpu().numpy()
     assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
+@gpu
+def test_submits_into_one_buffer_accumulate_in_order(config3):
+    """Three served requests into the same dose buffer, submitted back to back: they share one
+    lane (the transfer adds without atomics), so the buffer holds the three beams' sum in
+    submission order, bitwise the plain engine's sequential sum; a request into another buffer
+    meanwhile takes the next lane and still gets its own beam's dose."""
+    torch = _torch()
+    from subsecondpbs_amd.batch import Batch
+    case = config3
+    order = [0, 1, 2]
+    nx, ny, nz = case["dose_dims"]
+    bt = Batch(case["energy"], lanes=4)
+    bt.set_ct(case["ct"], case["ct_dims"], case["ct_resolution"], case["ct_corner"])
+    bt.set_dose_grid(case["dose_dims"], case["dose_resolution"], case["dose_corner"])
+    shared = torch.zeros((nz, ny, nx), dtype=torch.float32, device="cuda")
+    other = torch.zeros_like(shared)
+    tickets = [bt.submit(case["beams"][order[0]], shared), bt.submit(case["beams"][3], other)]
+    tickets += [bt.submit(case["beams"][i], shared) for i in order[1:]]
+    for t in tickets:
+        bt.wait(t)
+    got, got_other = shared.cpu().numpy(), other.cpu().numpy()
+    bt.close()
+    ref = _sequential(case, order)
+    ref_other = _sequential(case, [3])
+    print(f"3 requests into one buffer: max |d| = {np.abs(got - ref).max():.3e}")
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got_other, ref_other)
