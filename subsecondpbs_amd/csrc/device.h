@@ -61,6 +61,14 @@ struct BeamScalars {
     int boxMax[3], pad3;
 };
 
+// Per-work-group results of the one-pass trace, reduced by the beam setup (min == nullptr when
+// the beam's trace reduced them itself).
+struct TracePartials {
+    const float *min;    // [K][pitch]: minimum WEPL per step and work-group (1e20: none)
+    const int *fi, *fo;  // [n]: least entry step, greatest exit step per work-group
+    int pitch, n;
+};
+
 struct LayerScalars {
     int afterLast, passiveRaw, passive, mainEnd;  // mainEnd: superposed into the main BEV below this step
     float entrySigX, entrySigY;

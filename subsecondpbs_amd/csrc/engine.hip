@@ -1059,6 +1059,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     };
     // 1. trace and per-beam reductions
     HIP_TRY(mark());
+    TracePartials parts{nullptr, nullptr, nullptr, 0, 0};
     if (p.weqVolume) {
         // the beam's water-equivalent samples replace the CT trace (wrapper:3566-3640)
         const size_t nb = size_t(p.weqDims[0]) * p.weqDims[1] * p.weqDims[2];
@@ -1071,7 +1072,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     } else {
         rec_stage(kStageTrace);
         HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p,
-                             e->weplMin.p, e->beamScal.p, s));
+                             e->weplMin.p, e->beamScal.p, &parts, s));
     }
     // the per-layer values and the subspot table go up behind the trace (copies from pageable
     // memory hold the host, so the trace is already queued while they run)
@@ -1136,7 +1137,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
-                              e->prim.classCount.p, e->prim.hist.p, s));
+                              e->prim.classCount.p, e->prim.hist.p, parts, s));
     if (keep) {
         HIP_TRY(keep->bs0.ensure(1));
         HIP_TRY(keep->weplMin.ensure(K));
@@ -1203,7 +1204,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                             s));
         HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                                   subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, 1.0f,
-                                  e->prim.classCount.p, e->prim.hist.p, s));
+                                  e->prim.classCount.p, e->prim.hist.p, TracePartials{}, s));
         HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, keep->unitIdd.p,
                                keep->unitFp.p, e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
     } else {
@@ -1337,7 +1338,7 @@ static int update_enqueue(pbs_engine *e, const BeamState &st, const float *wPlan
     HIP_TRY(hipMemcpyAsync(e->weplMin.p, st.weplMin.p, sizeof(float) * K, hipMemcpyDeviceToDevice, s));
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               st.subMode ? st.subspots.p : nullptr, st.subMode ? st.nSub : 0, e->subEntry.p, 1.0f,
-                              e->prim.classCount.p, e->prim.hist.p, s));
+                              e->prim.classCount.p, e->prim.hist.p, TracePartials{}, s));
     const ConvArgs a = make_conv_args(p, rx, ry, st.nSub, st.subMode);
     HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, st.subspots.p,
                         e->subEntry.p, L, s));

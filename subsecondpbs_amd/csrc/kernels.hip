@@ -144,6 +144,58 @@ __device__ int first_larger_ordered(const float *v, int n, float value) {
 // segment totals into carries and the entry / exit steps, pass 3 adds the carries.
 constexpr int kTraceSeg = 8;
 
+// A ray's start and per-step increment in CT index space, and its step length. The increment
+// is the exact derivative of the divergent fan point (not a difference of two rounded points,
+// whose error would grow with the step index).
+struct RayGeom {
+    float p0[3], inc[3], stepLen;
+};
+__device__ __forceinline__ RayGeom ray_geom(const FanGeom &g, int ray) {
+    RayGeom rg;
+    const int i = ray % g.rx, j = ray / g.rx;
+    const float fx = g.fc[0] + float(i) * g.fd[0];
+    const float fy = g.fc[1] + float(j) * g.fd[1];
+    const float fz = g.fc[2];
+    const float gx = fx * (1.0f - fz / g.dist[0]);
+    const float gy = fy * (1.0f - fz / g.dist[1]);
+    const float dgx = -fx * (g.fd[2] / g.dist[0]);
+    const float dgy = -fy * (g.fd[2] / g.dist[1]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        rg.p0[r] = g.g2im[3 * r] * gx + g.g2im[3 * r + 1] * gy + g.g2im[3 * r + 2] * fz + g.g2imOff[r];
+        rg.inc[r] = g.g2im[3 * r] * dgx + g.g2im[3 * r + 1] * dgy + g.g2im[3 * r + 2] * g.fd[2];
+    }
+    const float dx = fx / g.dist[0], dy = fy / g.dist[1];
+    rg.stepLen = fabsf(g.fd[2]) * sqrtf(1.0f + dx * dx + dy * dy);
+    return rg;
+}
+
+// Running sums of a segment: the Kahan-compensated WEPL prefix (cum, comp), the HU prefix, and
+// the last step with HU+1000 > 150.
+struct TraceAcc {
+    float cum, comp, cumHu;
+    int lastIn = -1;
+};
+
+// Step k of a ray (bev_ray_tracing.cu:65-159): explicit trilinear CT sample (texel centre -1/2,
+// border 0), the running sums advanced by it; returns the step's density. Both trace forms
+// take their steps through this one function.
+__device__ __forceinline__ float trace_step(const FanGeom &g, const float *__restrict__ ct, const LutDev &lut,
+                                            const float *s_dens, const float *s_sp, const RayGeom &rg, int k,
+                                            TraceAcc &a) {
+    const float fk = float(k);
+    const float hu = trilinear_border(ct, g.ctDims[0], g.ctDims[1], g.ctDims[2], rg.p0[0] + fk * rg.inc[0],
+                                      rg.p0[1] + fk * rg.inc[1], rg.p0[2] + fk * rg.inc[2]);
+    a.cumHu += hu;
+    const float dens = lut1d_clamp(s_dens, lut.nDensity, hu * lut.densityScale);
+    const float y = rg.stepLen * lut1d_clamp(s_sp, lut.nSp, hu * lut.spScale) - a.comp;
+    const float t = a.cum + y;
+    a.comp = (t - a.cum) - y;
+    a.cum = t;
+    if (hu > 150.0f) a.lastIn = k;
+    return dens;
+}
+
 // Pass 1 (bev_ray_tracing.cu:65-159 per segment): explicit trilinear CT sample at fan step k
 // (texel centre -1/2, border 0), density and SP from the LUTs in LDS, the segment's Kahan
 // prefix of stepLen * SP (written to wepl), the segment's HU prefix, its totals and its last
@@ -167,47 +219,19 @@ __device__ __forceinline__ void trace_seg_body(const uint3 blockIdx, const uint3
         bs->beamFirstOutside = INT_MIN;
     }
     const float *s_dens = s_lut, *s_sp = s_lut + lut.nDensity;
-    const int i = ray % g.rx, j = ray / g.rx;
-    const float fx = g.fc[0] + float(i) * g.fd[0];
-    const float fy = g.fc[1] + float(j) * g.fd[1];
-    // ray start and per-step increment in CT index space; the increment is the exact
-    // derivative of the divergent fan point (not a difference of two rounded points, whose
-    // error would grow with the step index)
-    float p0[3], inc[3];
-    {
-        const float fz = g.fc[2];
-        const float gx = fx * (1.0f - fz / g.dist[0]);
-        const float gy = fy * (1.0f - fz / g.dist[1]);
-        const float dgx = -fx * (g.fd[2] / g.dist[0]);
-        const float dgy = -fy * (g.fd[2] / g.dist[1]);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            p0[r] = g.g2im[3 * r] * gx + g.g2im[3 * r + 1] * gy + g.g2im[3 * r + 2] * fz + g.g2imOff[r];
-            inc[r] = g.g2im[3 * r] * dgx + g.g2im[3 * r + 1] * dgy + g.g2im[3 * r + 2] * g.fd[2];
-        }
-    }
-    const float dx = fx / g.dist[0], dy = fy / g.dist[1];
-    const float stepLen = fabsf(g.fd[2]) * sqrtf(1.0f + dx * dx + dy * dy);
+    const RayGeom rg = ray_geom(g, ray);
     const int k0 = seg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
-    float cum = 0.0f, comp = 0.0f, cumHu = 0.0f;
-    int lastIn = -1;
+    TraceAcc a{};
     size_t idx = size_t(k0) * nray + ray;
 #pragma unroll 4
     for (int k = k0; k < k1; ++k) {
-        const float fk = float(k);
-        const float hu = trilinear_border(ct, g.ctDims[0], g.ctDims[1], g.ctDims[2], p0[0] + fk * inc[0],
-                                          p0[1] + fk * inc[1], p0[2] + fk * inc[2]);
-        cumHu += hu;
-        density[idx] = lut1d_clamp(s_dens, lut.nDensity, hu * lut.densityScale);
-        const float y = stepLen * lut1d_clamp(s_sp, lut.nSp, hu * lut.spScale) - comp;
-        const float t = cum + y;
-        comp = (t - cum) - y;
-        cum = t;
-        if (hu > 150.0f) lastIn = k;
-        wepl[idx] = cum;
-        huPrefix[idx] = cumHu;
+        density[idx] = trace_step(g, ct, lut, s_dens, s_sp, rg, k, a);
+        wepl[idx] = a.cum;
+        huPrefix[idx] = a.cumHu;
         idx += nray;
     }
+    const float cum = a.cum, comp = a.comp, cumHu = a.cumHu;
+    const int lastIn = a.lastIn;
     const size_t o = size_t(seg) * nray + ray;
     segSp[o] = make_float2(cum, comp);
     segHu[o] = cumHu;
@@ -341,6 +365,167 @@ struct TraceFixK {
 };
 
 
+// The trace in one pass, for beams of up to 16 x CH segments (K <= 128 CH steps): a work-group
+// of 512 threads holds 32 rays, 16 threads per ray, each thread a run of the ray's 8-step
+// segments [s0, s1) (balanced: one or two per thread for K = 129). The segments are the ones of
+// the three-pass form and each runs the same steps (trace_step); their totals meet in LDS, where
+// each thread sums the carries of the segments before its own in segment order, in double, as
+// the carry pass does (the two forms agree up to the compiler's choice of fused multiply-adds
+// in their inlined copies). The beam's reductions leave as per-work-group partials (the
+// per-step minimum over the group's rays that leave material, its least entry and greatest
+// exit step), which BeamSetupK reduces: no atomics, and nothing to initialise first.
+constexpr int kTraceRays = 32, kTraceThreadsPerRay = 16;
+
+template <int CH>
+__device__ __forceinline__ void trace_body(const uint3 blockIdx, FanGeom g, const float *__restrict__ ct, LutDev lut,
+                                           float *__restrict__ density, float *__restrict__ wepl, int *__restrict__ fi,
+                                           int *__restrict__ fo, float *__restrict__ partMin, int partPitch,
+                                           int *__restrict__ partFi, int *__restrict__ partFo) {
+    extern __shared__ float s_dyn[];
+    const int nLut = lut.nDensity + lut.nSp;
+    const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
+    float *s_dens = s_dyn, *s_sp = s_dyn + lut.nDensity;
+    float2 *s_seg = reinterpret_cast<float2 *>(s_dyn + (nLut + 3) / 4 * 4);  // [S][32] (cum, comp)
+    float *s_hu = reinterpret_cast<float *>(s_seg + S * kTraceRays);       // [S][32]
+    int *s_last = reinterpret_cast<int *>(s_hu + S * kTraceRays);          // [S][32]
+    int *s_cnt = s_last + S * kTraceRays;                                   // [32] entry step
+    for (int i = threadIdx.x; i < lut.nDensity; i += blockDim.x) s_dens[i] = lut.density[i];
+    for (int i = threadIdx.x; i < lut.nSp; i += blockDim.x) s_sp[i] = lut.sp[i];
+    const int rl = threadIdx.x & (kTraceRays - 1), t = threadIdx.x / kTraceRays;
+    const int nray = g.rx * g.ry;
+    const int ray = blockIdx.x * kTraceRays + rl;
+    const bool rayIn = ray < nray;
+    const int s0 = t * S / kTraceThreadsPerRay, s1 = (t + 1) * S / kTraceThreadsPerRay;
+    if (t == 0) s_cnt[rl] = g.K;  // no threshold crossing: K
+    __syncthreads();
+    float wv[CH * kTraceSeg], hv[CH * kTraceSeg];
+    if (rayIn) {
+        const RayGeom rg = ray_geom(g, ray);
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int sg = s0 + c;
+            if (sg >= s1) break;
+            TraceAcc a{};
+#pragma unroll
+            for (int j = 0; j < kTraceSeg; ++j) {
+                const int k = sg * kTraceSeg + j;
+                if (k >= g.K) break;
+                density[size_t(k) * nray + ray] = trace_step(g, ct, lut, s_dens, s_sp, rg, k, a);
+                wv[c * kTraceSeg + j] = a.cum;
+                hv[c * kTraceSeg + j] = a.cumHu;
+            }
+            s_seg[sg * kTraceRays + rl] = make_float2(a.cum, a.comp);
+            s_hu[sg * kTraceRays + rl] = a.cumHu;
+            s_last[sg * kTraceRays + rl] = a.lastIn;
+        }
+    }
+    __syncthreads();
+    int fov = INT_MIN;
+    if (rayIn) {
+        // the carries of the segments before s0, summed in segment order
+        double carrySp = 0.0, carryHu = 0.0;
+        int last = -1;
+        for (int sg = 0; sg < S; ++sg) {
+            if (sg < s0) {
+                const float2 sp = s_seg[sg * kTraceRays + rl];
+                carrySp += double(sp.x) - double(sp.y);
+                carryHu += double(s_hu[sg * kTraceRays + rl]);
+            }
+            last = max(last, s_last[sg * kTraceRays + rl]);
+        }
+        fov = max(last, 0) + 1;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int sg = s0 + c;
+            if (sg >= s1) break;
+            const double hu = double(s_hu[sg * kTraceRays + rl]);
+            const int k0 = sg * kTraceSeg, k1 = min(g.K, k0 + kTraceSeg);
+            // the cumulative HU first reaches 150 in this segment: its step (the running HU sum
+            // never decreases, so one segment at most sees the crossing)
+            if (carryHu < 150.0 && carryHu + hu >= 150.0) {
+                int count = k1;
+#pragma unroll
+                for (int j = kTraceSeg - 1; j >= 0; --j)
+                    if (k0 + j < k1 && carryHu + double(hv[c * kTraceSeg + j]) >= 150.0) count = k0 + j;
+                s_cnt[rl] = count;
+            }
+#pragma unroll
+            for (int j = 0; j < kTraceSeg; ++j) {
+                const int k = k0 + j;
+                if (k >= k1) break;
+                const float v = sg > 0 ? float(carrySp + double(wv[c * kTraceSeg + j])) : wv[c * kTraceSeg + j];
+                wepl[size_t(k) * nray + ray] = v;
+                wv[c * kTraceSeg + j] = v;
+            }
+            const float2 sp = s_seg[sg * kTraceRays + rl];
+            carrySp += double(sp.x) - double(sp.y);
+            carryHu += hu;
+        }
+    }
+    // per step: the minimum WEPL over this group's rays that leave material (fo > 0), from the
+    // 32 lanes of each half-wave (one thread per ray for the step's segment)
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const int sg = s0 + c;
+#pragma unroll
+        for (int j = 0; j < kTraceSeg; ++j) {
+            const int k = sg * kTraceSeg + j;
+            // uniform over the half-wave: its 32 lanes share t
+            if (sg >= s1 || k >= g.K) continue;
+            float m = (rayIn && fov > 0) ? wv[c * kTraceSeg + j] + 0.0f : 1.0e20f;  // -0 -> +0
+#pragma unroll
+            for (int o = kTraceRays / 2; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, kTraceRays));
+            if (rl == 0) partMin[size_t(k) * partPitch + blockIdx.x] = m;
+        }
+    }
+    // the last work-group also fills the pitch's pad columns
+    const int groups = (nray + kTraceRays - 1) / kTraceRays;
+    if (int(blockIdx.x) == groups - 1)
+        for (int i = threadIdx.x; i < g.K * (partPitch - groups); i += blockDim.x)
+            partMin[size_t(i / (partPitch - groups)) * partPitch + groups + i % (partPitch - groups)] = 1.0e20f;
+    __syncthreads();
+    if (t == 0) {
+        int fiv = INT_MAX;
+        if (rayIn) {
+            fiv = max(s_cnt[rl], 1);
+            fi[ray] = fiv;
+            fo[ray] = fov;
+        }
+        int mn = fiv, mx = fov;
+#pragma unroll
+        for (int o = kTraceRays / 2; o > 0; o >>= 1) {
+            mn = min(mn, __shfl_xor(mn, o, kTraceRays));
+            mx = max(mx, __shfl_xor(mx, o, kTraceRays));
+        }
+        if (rl == 0) {
+            partFi[blockIdx.x] = mn;
+            partFo[blockIdx.x] = mx;
+        }
+    }
+}
+template <int CH>
+struct TraceK {
+    static constexpr int NT = kTraceRays * kTraceThreadsPerRay;
+    struct Args {
+        FanGeom g;
+        const float *ct;
+        LutDev lut;
+        float *density;
+        float *wepl;
+        int *fi;
+        int *fo;
+        float *partMin;  // [K][partPitch]
+        int partPitch;
+        int *partFi;     // [work-groups]
+        int *partFo;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) {
+        (void)d;
+        trace_body<CH>(b, a.g, a.ct, a.lut, a.density, a.wepl, a.fi, a.fo, a.partMin, a.partPitch, a.partFi, a.partFo);
+    }
+};
+
+
 // WEQ-volume input in place of the trace (fillBevFromWeqVolumeKernel, wrapper:1923-1988), thread
 // per ray: the ray's WEQ column is the one nearest its gantry (x, y) (rounded half away from
 // zero); WEPL(k) = the column's sample k (the last one read past the column or off the lattice),
@@ -413,15 +598,55 @@ __global__ __launch_bounds__(256) void k_trace_reduce(FanGeom g, const float *__
 
 // Distal cut-offs, entry plane and entry sigmas (host formulas, evaluated without
 // contraction so the integers match the host's float arithmetic).
-__device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float *__restrict__ weplMin,
+__device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, float *__restrict__ weplMinOut,
                                                     const LayerHost *__restrict__ lh, BeamScalars *__restrict__ bs,
                                                     LayerScalars *__restrict__ ls, const float *__restrict__ subspots,
                                                     int maxSub, float *__restrict__ subEntry, float nucFactor,
-                                                    int *__restrict__ classCount, int *__restrict__ hist) {
+                                                    int *__restrict__ classCount, int *__restrict__ hist,
+                                                    const TracePartials tp) {
 #pragma clang fp contract(off)
-    const int inside = bs->beamFirstInside;
+    __shared__ float s_wm[kMaxTraceSteps];
+    __shared__ int s_io[2][4];
+    const float *weplMin = weplMinOut;
+    int inside, outside;
+    if (tp.min) {
+        // the one-pass trace's per-work-group partials: per-step minimum WEPL, least entry and
+        // greatest exit step, reduced here (and written out for the later readers)
+        for (int k = threadIdx.x; k < g.K; k += blockDim.x) {
+            const float4 *row = reinterpret_cast<const float4 *>(tp.min + size_t(k) * tp.pitch);
+            float m = 1.0e20f;
+            for (int q = 0; q < tp.pitch / 4; ++q) {
+                const float4 v = row[q];
+                m = fminf(m, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+            }
+            s_wm[k] = m;
+            weplMinOut[k] = m;
+        }
+        int mn = INT_MAX, mx = INT_MIN;
+        for (int i = threadIdx.x; i < tp.n; i += blockDim.x) {
+            mn = min(mn, tp.fi[i]);
+            mx = max(mx, tp.fo[i]);
+        }
+        mn = wave_min_i(mn);
+        mx = wave_max_i(mx);
+        if ((threadIdx.x & 63) == 0) {
+            s_io[0][threadIdx.x >> 6] = mn;
+            s_io[1][threadIdx.x >> 6] = mx;
+        }
+        __syncthreads();
+        inside = min(min(s_io[0][0], s_io[0][1]), min(s_io[0][2], s_io[0][3]));
+        outside = max(max(s_io[1][0], s_io[1][1]), max(s_io[1][2], s_io[1][3]));
+        if (threadIdx.x == 0) {
+            bs->beamFirstInside = inside;
+            bs->beamFirstOutside = outside;
+        }
+        weplMin = s_wm;
+    } else {
+        inside = bs->beamFirstInside;
+        outside = bs->beamFirstOutside;
+    }
     const int fpa = first_larger_ordered(weplMin, g.K, g.maxCutoffWepl);
-    const int gp = min(fpa, bs->beamFirstOutside);
+    const int gp = min(fpa, outside);
     const float entryZ = float(inside) * g.fd[2] + g.fc[2];
     const float pxX = 1.0f - entryZ / g.dist[0], pxY = 1.0f - entryZ / g.dist[1];
     if (threadIdx.x == 0) {
@@ -460,7 +685,7 @@ struct BeamSetupK {
     static constexpr int NT = 256;
     struct Args {
         FanGeom g;
-        const float *weplMin;
+        float *weplMin;
         const LayerHost *lh;
         BeamScalars *bs;
         LayerScalars *ls;
@@ -470,8 +695,9 @@ struct BeamSetupK {
         float nucFactor;
         int *classCount;
         int *hist;
+        TracePartials tp;
     };
-    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { beam_setup_body(b, d, a.g, a.weplMin, a.lh, a.bs, a.ls, a.subspots, a.maxSub, a.subEntry, a.nucFactor, a.classCount, a.hist); }
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { beam_setup_body(b, d, a.g, a.weplMin, a.lh, a.bs, a.ls, a.subspots, a.maxSub, a.subEntry, a.nucFactor, a.classCount, a.hist, a.tp); }
 };
 
 
@@ -1707,12 +1933,34 @@ struct SuperposeK {
 // ---------------------------------------------------------------------------------------
 // Superposition with lanes on output rows (radius classes up to 16)
 // ---------------------------------------------------------------------------------------
-// erf(x) for x >= 0 by Abramowitz & Stegun 7.1.26 (Hastings): |error| <= 1.5e-7.
-__device__ __forceinline__ float erf_as(float x) {
-    const float t = rcp_hw(1.0f + 0.3275911f * x);
-    const float p =
-        t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
-    return 1.0f - p * __builtin_amdgcn_exp2f(-1.44269504f * (x * x));
+// Table row of one source: row[1 + m] = e_m, the pixel weights of a Gaussian of reciprocal
+// width r (complete_superposition.cu:280-300: e_0 = erf(r/2), e_m = (erf(r(m+1/2)) - erf(r(m-1/2)))/2),
+// for m <= R and 0 past R. erf(x) = 1 - T(x) with the tail T(x) = t P(t) exp(-x^2), t = 1/(1 + p x)
+// (Abramowitz & Stegun 7.1.26, |error| <= 1.5e-7), so e_0 = 1 - T_0 and e_m = (T_{m-1} - T_m)/2: a
+// difference of tails, which keeps the small weights' relative accuracy. exp(-x_m^2) at x_m =
+// r(m + 1/2) follows x_{m+1}^2 - x_m^2 = 2 r^2 (m + 1): g_{m+1} = g_m h_m with h_m = c^(m+1),
+// c = exp(-2 r^2), so a row takes two exponentials and R + 1 reciprocals. r = inf (a dead pixel)
+// gives e_0 = 1 and zeros; r = 0 gives zeros.
+template <int EW>
+__device__ __forceinline__ void gauss_row(float (&row)[EW], float r, int R) {
+    const float r2 = r * r;
+    const float c = __builtin_amdgcn_exp2f(-2.88539008f * r2);  // exp(-2 r^2)
+    float g = __builtin_amdgcn_exp2f(-0.360673760f * r2);       // exp(-r^2 / 4)
+    float h = c, tPrev = 0.0f;
+#pragma unroll
+    for (int m = 0; m < EW - 1; ++m) {
+        float e = 0.0f;
+        if (m <= R) {
+            const float t = rcp_hw(1.0f + 0.3275911f * (r * (float(m) + 0.5f)));
+            const float T =
+                g * (t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f)))));
+            e = m == 0 ? 1.0f - T : 0.5f * (tPrev - T);
+            tPrev = T;
+            g *= h;
+            h *= c;
+        }
+        row[1 + m] = e;
+    }
 }
 
 // Work-group barrier for LDS hand-offs only: waits for this wave's LDS accesses, not for its
@@ -1834,7 +2082,11 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
     __shared__ int s_n;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int gi = tid / G, q = tid % G;  // group, vertical offset |dy|
-    const int grp = gi % 8, half = gi / 8;  // source row, half of the sources (uniform per wave)
+    const int grp = gi % 8;  // source row
+    // half of the sources: the same for the whole wave, and read as a scalar so that the chunk
+    // tests on the live mask below are scalar branches (a per-lane condition would make them
+    // divergent branches, with the accumulators copied around every chunk)
+    const int half = __builtin_amdgcn_readfirstlane(gi / 8);
     const int nt = (g.sx / 32) * (g.sy / 8), ntx = g.sx / 32;
     const int count = min(classCount[cls], maxUnits);
     const int2 *list = classList + size_t(cls) * maxUnits;
@@ -1886,31 +2138,22 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
         float acc[HW];
 #pragma unroll
         for (int c = 0; c < HW; ++c) acc[c] = 0.0f;
-        // first layer's pixels
-        float2 pre[SPT];
-        {
-            const int l0 = n > 0 ? (s_list[0] & 0xffff) : 0;
-            const float2 *plane = iddSigma + (size_t(l0) * g.K + k) * planeXY;
+        // the pixels (d, r) of list entry li
+        auto load = [&](int li, float2 (&px)[SPT]) {
+            const float2 *plane = iddSigma + (size_t(s_list[li] & 0xffff) * g.K + k) * planeXY;
 #pragma unroll
-            for (int i = 0; i < SPT; ++i) pre[i] = (n > 0 && pixIn[i]) ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
-        }
-        for (int li = 0; li < n; ++li) {
+            for (int i = 0; i < SPT; ++i) px[i] = pixIn[i] ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
+        };
+        // list entry li's table: each source's row [d, e_0 .. e_R, 0 ...] and the source rows' live
+        // masks. A wave whose 64 sources are all dead writes zero rows without evaluating the
+        // weights (the sweep may still read them, next to another row's live chunk)
+        auto stage = [&](int li, const float2 (&px)[SPT]) {
             const int R = s_list[li] >> 16;
-            float *E = s_lds;
-            float2 cur[SPT];
-#pragma unroll
-            for (int i = 0; i < SPT; ++i) cur[i] = pre[i];
-            if (li + 1 < n) {
-                const int l1 = s_list[li + 1] & 0xffff;
-                const float2 *plane = iddSigma + (size_t(l1) * g.K + k) * planeXY;
-#pragma unroll
-                for (int i = 0; i < SPT; ++i) pre[i] = pixIn[i] ? plane[pixOff[i]] : make_float2(0.0f, PBS_INF);
-            }
 #pragma unroll
             for (int i = 0; i < SPT; ++i) {
                 const int src = tid + i * NT;
                 if (src >= 256) break;  // uniform per wave: NT and 256 are multiples of 64
-                const float d = cur[i].x, r = cur[i].y;
+                const float d = px[i].x, r = px[i].y;
                 const uint64_t live = __ballot(d != 0.0f);
                 if (lane == 0) {
                     s_live[src >> 5] = uint32_t(live);
@@ -1918,24 +2161,30 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
                 }
                 float row[EW];
                 row[0] = d;
-                float en = erf_as(r * 0.5f), eo = -en;
+                if (live) {
+                    gauss_row<EW>(row, r, R);
+                } else {
 #pragma unroll
-                for (int m = 0; m < EW - 1; ++m) {
-                    float e = 0.0f;
-                    if (m <= R) {
-                        e = 0.5f * (en - eo);
-                        eo = en;
-                        if (m < R) en = erf_as(r * (float(m) + 1.5f));
-                    }
-                    row[1 + m] = e;
+                    for (int m = 1; m < EW; ++m) row[m] = 0.0f;
                 }
-                float *dst = E + (src >> 5) * RP + S::col_off(src & 31);
+                float *dst = s_lds + (src >> 5) * RP + S::col_off(src & 31);
 #pragma unroll
                 for (int m = 0; m < EW / 4; ++m)
                     *reinterpret_cast<float4 *>(dst + 4 * m) =
                         make_float4(row[4 * m], row[4 * m + 1], row[4 * m + 2], row[4 * m + 3]);
             }
+        };
+        // the next layer's pixels are loaded while this one is staged and swept
+        float2 pre[SPT];
+        if (n > 0) load(0, pre);
+        for (int li = 0; li < n; ++li) {
+            float2 cur[SPT];
+#pragma unroll
+            for (int i = 0; i < SPT; ++i) cur[i] = pre[i];
+            if (li + 1 < n) load(li + 1, pre);
+            stage(li, cur);
             lds_barrier();
+            const int R = s_list[li] >> 16;
             // skip the layer when no source row of the wave has a live pixel
             uint32_t wmv = 0;
 #pragma unroll
@@ -1945,9 +2194,9 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
                 const int v = q < R + 1 ? q : R + 1;
                 const int rp = R > S::RLO ? R : S::RLO;
                 if (half)
-                    rows_dispatch<S::RLO, RM, EW, 1>(rp, acc, E + grp * RP, v, wm);
+                    rows_dispatch<S::RLO, RM, EW, 1>(rp, acc, s_lds + grp * RP, v, wm);
                 else
-                    rows_dispatch<S::RLO, RM, EW, 0>(rp, acc, E + grp * RP, v, wm);
+                    rows_dispatch<S::RLO, RM, EW, 0>(rp, acc, s_lds + grp * RP, v, wm);
             }
             lds_barrier();  // the table is read: the next layer may overwrite it
         }
@@ -2526,10 +2775,42 @@ static TraceScratch trace_layout(const FanGeom &g) {
     return t;
 }
 
-size_t trace_scratch_floats(const FanGeom &g) { return trace_layout(g).total; }
+// The one-pass trace's partials in the scratch: the per-step minima [K][pitch] (pitch: the
+// work-groups rounded up to 4, the pad columns at 1e20), then the entry and exit steps.
+static int trace_groups(const FanGeom &g) { return (g.rx * g.ry + kTraceRays - 1) / kTraceRays; }
+static int trace_pitch(const FanGeom &g) { return (trace_groups(g) + 3) / 4 * 4; }
+static size_t trace_partial_floats(const FanGeom &g) {
+    return size_t(g.K) * trace_pitch(g) + 2 * size_t(trace_pitch(g));
+}
+
+size_t trace_scratch_floats(const FanGeom &g) { return std::max(trace_layout(g).total, trace_partial_floats(g)); }
+
+template <int CH>
+static void launch_trace_one_pass(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl,
+                                  int *fi, int *fo, float *scratch, TracePartials *parts, hipStream_t s) {
+    const int groups = trace_groups(g), pitch = trace_pitch(g);
+    float *partMin = scratch;
+    int *partFi = reinterpret_cast<int *>(scratch + size_t(g.K) * pitch);
+    int *partFo = partFi + pitch;
+    const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
+    const size_t shm = sizeof(float) * (size_t(lut.nDensity + lut.nSp + 3) / 4 * 4) +
+                       size_t(S) * kTraceRays * (sizeof(float2) + sizeof(float) + sizeof(int)) + sizeof(int) * kTraceRays;
+    dispatch<TraceK<CH>>({g, ct, lut, density, wepl, fi, fo, partMin, pitch, partFi, partFo}, dim3(groups), shm, s);
+    *parts = TracePartials{partMin, partFi, partFo, pitch, groups};
+}
 
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
-                        int *fo, float *scratch, float *weplMin, BeamScalars *bs, hipStream_t s) {
+                        int *fo, float *scratch, float *weplMin, BeamScalars *bs, TracePartials *parts,
+                        hipStream_t s) {
+    *parts = TracePartials{nullptr, nullptr, nullptr, 0, 0};
+    if (g.K <= 2 * kTraceSeg * kTraceThreadsPerRay) {
+        launch_trace_one_pass<2>(g, ct, lut, density, wepl, fi, fo, scratch, parts, s);
+        return hipGetLastError();
+    }
+    if (g.K <= kMaxTraceSteps) {
+        launch_trace_one_pass<4>(g, ct, lut, density, wepl, fi, fo, scratch, parts, s);
+        return hipGetLastError();
+    }
     const int n = g.rx * g.ry;
     const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
     const TraceScratch L = trace_layout(g);
@@ -2560,10 +2841,10 @@ hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *f
     return hipGetLastError();
 }
 
-hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
+hipError_t launch_beam_setup(const FanGeom &g, float *weplMin, const LayerHost *lh, BeamScalars *bs,
                              LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
-                             int *classCount, int *hist, hipStream_t s) {
-    dispatch<BeamSetupK>({g, weplMin, lh, bs, ls, subspots, maxSub, subEntry, nucFactor, classCount, hist}, dim3(1), 0, s);
+                             int *classCount, int *hist, const TracePartials &parts, hipStream_t s) {
+    dispatch<BeamSetupK>({g, weplMin, lh, bs, ls, subspots, maxSub, subEntry, nucFactor, classCount, hist, parts}, dim3(1), 0, s);
     return hipGetLastError();
 }
 
