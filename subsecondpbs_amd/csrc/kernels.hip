@@ -605,22 +605,23 @@ __device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint
                                                     int *__restrict__ classCount, int *__restrict__ hist,
                                                     const TracePartials tp) {
 #pragma clang fp contract(off)
-    __shared__ float s_wm[kMaxTraceSteps];
+    __shared__ unsigned s_wmb[kMaxTraceSteps];
     __shared__ int s_io[2][4];
     const float *weplMin = weplMinOut;
     int inside, outside;
     if (tp.min) {
         // the one-pass trace's per-work-group partials: per-step minimum WEPL, least entry and
-        // greatest exit step, reduced here (and written out for the later readers)
-        for (int k = threadIdx.x; k < g.K; k += blockDim.x) {
-            const float4 *row = reinterpret_cast<const float4 *>(tp.min + size_t(k) * tp.pitch);
-            float m = 1.0e20f;
-            for (int q = 0; q < tp.pitch / 4; ++q) {
-                const float4 v = row[q];
-                m = fminf(m, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
-            }
-            s_wm[k] = m;
-            weplMinOut[k] = m;
+        // greatest exit step, reduced here (and written out for the later readers). The minima
+        // are read as consecutive float4s by all threads and met by LDS atomic minima on the
+        // bits (WEPL >= 0, so the bit patterns order as the values)
+        for (int k = threadIdx.x; k < g.K; k += blockDim.x) s_wmb[k] = __float_as_uint(1.0e20f);
+        __syncthreads();
+        const int q4 = tp.pitch / 4;
+        const float4 *pm = reinterpret_cast<const float4 *>(tp.min);
+#pragma unroll 4
+        for (int i = threadIdx.x; i < g.K * q4; i += blockDim.x) {
+            const float4 v = pm[i];
+            atomicMin(&s_wmb[i / q4], __float_as_uint(fminf(fminf(v.x, v.y), fminf(v.z, v.w))));
         }
         int mn = INT_MAX, mx = INT_MIN;
         for (int i = threadIdx.x; i < tp.n; i += blockDim.x) {
@@ -640,7 +641,8 @@ __device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint
             bs->beamFirstInside = inside;
             bs->beamFirstOutside = outside;
         }
-        weplMin = s_wm;
+        for (int k = threadIdx.x; k < g.K; k += blockDim.x) weplMinOut[k] = __uint_as_float(s_wmb[k]);
+        weplMin = reinterpret_cast<const float *>(s_wmb);
     } else {
         inside = bs->beamFirstInside;
         outside = bs->beamFirstOutside;
