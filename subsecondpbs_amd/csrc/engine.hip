@@ -809,7 +809,7 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int dims[3], const float res[3
 
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
                             const float *cpbWorldGeom, const float *gradDose = nullptr, float *gradSpots = nullptr,
-                            BeamState *keep = nullptr);
+                            BeamState *keep = nullptr, BeamPlan *planned = nullptr);
 
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
@@ -957,8 +957,12 @@ static size_t pin_align(size_t n) { return (n + 255) & ~size_t(255); }
 
 // cpbWorldGeom (optional, 13 floats): beam direction, bmX, bmY, source, SAD of the subspot
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
+// `planned`: the beam's plan, made by the caller on this engine's CT, dose grid, tables and
+// environment (the server plans at admission to size the request), moved in instead of planned
+// again.
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
-                            const float *cpbWorldGeom, const float *gradDose, float *gradSpots, BeamState *keep) {
+                            const float *cpbWorldGeom, const float *gradDose, float *gradSpots, BeamState *keep,
+                            BeamPlan *planned) {
     const bool adjoint = gradDose != nullptr;
     if (!e || !beam || (!doseDev && !adjoint)) return set_error(PBS_E_INVALID, "null argument");
     if (!e->ctPtr) return set_error(PBS_E_INVALID, "no CT bound to the engine");
@@ -971,8 +975,10 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         e->uploadPending = false;
     }
     BeamPlan &p = e->plan;
-    if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p, ov,
-                   &e->env))
+    if (planned)
+        p = std::move(*planned);
+    else if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p,
+                        ov, &e->env))
         return set_error(PBS_E_GEOMETRY, p.error);
     FanGeom &g = e->geom;
     fill_geom(p, *e, g);
@@ -2503,7 +2509,9 @@ bool admit_one(pbs_server *sv, std::unique_lock<std::mutex> &lk) {
                 ServerCt &ct = *sv->cts[rp->ctId];
                 (void)pbs_engine_set_ct(e, ct.ct.p, ct.dims, ct.res, ct.corner, 2);
                 (void)pbs_engine_set_dose_grid(e, rp->doseDims, rp->doseRes, rp->doseCorner);
-                rc = engine_beam_dose(e, &rp->beam.b, rp->dose, nullptr, nullptr, nullptr, nullptr, keepState.get());
+                // the plan made at admission (same CT, dose grid, tables and environment)
+                rc = engine_beam_dose(e, &rp->beam.b, rp->dose, nullptr, nullptr, nullptr, nullptr, keepState.get(),
+                                      &plan);
             }
             if (rc != PBS_OK) err = g_lastError;
         }
