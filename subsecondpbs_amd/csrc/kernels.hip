@@ -1212,34 +1212,33 @@ __global__ __launch_bounds__(256) void k_rescale(FanGeom g, LayerScalars *__rest
     if (lane == 0) atomicMax(&ls[l].passiveRaw, mx);
 }
 
+// A layer's passive step from its first passive step (wrapper:4984-4996).
+__device__ __forceinline__ int layer_passive(const FanGeom &g, const LayerScalars &L, int inside) {
+    int pas = L.passiveRaw;
+    if (pas < inside + 1) pas = inside + 1;
+    if (pas < 0) pas = inside + 1;
+    if (pas > L.afterLast) pas = L.afterLast;
+    if (pas > g.K) pas = g.K;
+    return pas;
+}
+
 // Radius histogram of one layer from its tile codes over the superposed steps
 // [beamFirstInside, passive) (complete_superposition.cu:197-255, 400-463); one block per layer.
-__device__ __forceinline__ void tile_hist_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const int *__restrict__ tileCode,
-                                                   const LayerScalars *__restrict__ ls,
-                                                   const BeamScalars *__restrict__ bs, int *__restrict__ hist) {
-    __shared__ int s_h[kMaxRadDev + 2];
+// Layer blockIdx.x's radius histogram of its tile codes over [beamFirstInside, passive) into
+// s_h (complete at return: the caller's threads may read all of it).
+__device__ __forceinline__ void tile_hist_into(const uint3 blockIdx, const FanGeom &g, const int *__restrict__ tileCode,
+                                               const LayerScalars *__restrict__ ls, const BeamScalars *__restrict__ bs,
+                                               int *s_h) {
     const int l = blockIdx.x;
     const int nt = (g.sx / 32) * (g.sy / 8);
     if (threadIdx.x < kMaxRadDev + 2) s_h[threadIdx.x] = 0;
     __syncthreads();
-    const int k0 = bs->beamFirstInside, k1 = ls[l].passive;
+    const int k0 = bs->beamFirstInside, k1 = layer_passive(g, ls[l], k0);  // = ls[l].passive
     const int n = k1 > k0 ? (k1 - k0) * nt : 0;
     const int *codes = tileCode + (size_t(l) * g.K + k0) * nt;
     for (int i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&s_h[codes[i] & 255], 1);
     __syncthreads();
-    if (threadIdx.x < kMaxRadDev + 2) hist[l * (kMaxRadDev + 2) + threadIdx.x] = s_h[threadIdx.x];
 }
-struct TileHistK {
-    static constexpr int NT = 256;
-    struct Args {
-        FanGeom g;
-        const int *tileCode;
-        const LayerScalars *ls;
-        const BeamScalars *bs;
-        int *hist;
-    };
-    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { tile_hist_body(b, d, a.g, a.tileCode, a.ls, a.bs, a.hist); }
-};
 
 
 // Transfer box of one passive step (wrapper:5343-5395): 8 fan corners -> dose index in float,
@@ -1295,11 +1294,7 @@ __device__ __forceinline__ void layer_finalize_body(const uint3 blockIdx, const 
     const int inside = bs->beamFirstInside;
     const int L = g.numLayers;
     for (int l = threadIdx.x; l < L; l += blockDim.x) {
-        int pas = ls[l].passiveRaw;  // wrapper:4984-4996
-        if (pas < inside + 1) pas = inside + 1;
-        if (pas < 0) pas = inside + 1;
-        if (pas > ls[l].afterLast) pas = ls[l].afterLast;
-        if (pas > g.K) pas = g.K;
+        const int pas = layer_passive(g, ls[l], inside);
         ls[l].passive = pas;
         s_pas[l] = pas > inside ? pas : -1;  // layers with no superposed step take no part
     }
@@ -1519,15 +1514,13 @@ struct TilesK {
 // overflowFlag (halo grid, complete_superposition.cu:558-620): a tile past the largest radius
 // sets the sticky flag the host reports as an error; the primary grid (nullptr) only clamps
 // it (413-463). Both clamp, so every launch stays within the largest radius.
-__device__ __forceinline__ void bins_body(const uint3 blockIdx, const uint3 gridDim, int numLayers, const int *__restrict__ hist, int *__restrict__ effRad,
-                       int *__restrict__ overflowFlag) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= numLayers) return;
+// One layer's launch radii from its radius histogram `hl` (kMaxRadDev + 2 counts) into `effL`.
+__device__ __forceinline__ void layer_bins(const int *hl, int *__restrict__ effL, int *__restrict__ overflowFlag) {
     // every loop has a fixed trip count and is unrolled, so h and eff stay in registers (a loop
     // bound by layerMax would index them at run time and put them in scratch memory)
     int h[kMaxRadDev + 2];
 #pragma unroll
-    for (int i = 0; i < kMaxRadDev + 2; ++i) h[i] = hist[l * (kMaxRadDev + 2) + i];
+    for (int i = 0; i < kMaxRadDev + 2; ++i) h[i] = hl[i];
     if (overflowFlag && h[kMaxRadDev + 1] > 0) *overflowFlag = 1;
     h[kMaxRadDev] += h[kMaxRadDev + 1];  // primary overflow clamps into the largest radius
     h[kMaxRadDev + 1] = 0;
@@ -1549,7 +1542,13 @@ __device__ __forceinline__ void bins_body(const uint3 blockIdx, const uint3 grid
         }
     }
 #pragma unroll
-    for (int i = 0; i < kMaxRadDev + 2; ++i) effRad[l * (kMaxRadDev + 2) + i] = eff[i];
+    for (int i = 0; i < kMaxRadDev + 2; ++i) effL[i] = eff[i];
+}
+__device__ __forceinline__ void bins_body(const uint3 blockIdx, const uint3 gridDim, int numLayers, const int *__restrict__ hist, int *__restrict__ effRad,
+                       int *__restrict__ overflowFlag) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= numLayers) return;
+    layer_bins(hist + l * (kMaxRadDev + 2), effRad + l * (kMaxRadDev + 2), overflowFlag);
 }
 struct BinsK {
     static constexpr int NT = 64;
@@ -1560,6 +1559,34 @@ struct BinsK {
         int *overflowFlag;
     };
     static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { bins_body(b, d, a.numLayers, a.hist, a.effRad, a.overflowFlag); }
+};
+
+// The layer finalize, the per-layer radius histograms and the launch radii in one launch: work
+// group l < L histograms layer l's tile codes over [beamFirstInside, passive) (its passive step
+// computed as the finalize computes it) and bins them (thread 0); work-group L runs the finalize
+// (groups, boxes, tail planes). None of them reads what another writes.
+struct FinalizeTilesK {
+    static constexpr int NT = 256;
+    struct Args {
+        LayerFinalizeK::Args fin;
+        const int *tileCode;
+        int *hist;
+        int *effRad;
+        int *overflowFlag;
+    };
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) {
+        const int L = a.fin.g.numLayers;
+        if (int(b.x) == L) {
+            LayerFinalizeK::run(a.fin, uint3{0u, 0u, 0u}, uint3{1u, 1u, 1u});
+            return;
+        }
+        __shared__ int s_h[kMaxRadDev + 2];
+        tile_hist_into(b, a.fin.g, a.tileCode, a.fin.ls, a.fin.bs, s_h);
+        const int l = b.x;
+        if (threadIdx.x < kMaxRadDev + 2) a.hist[l * (kMaxRadDev + 2) + threadIdx.x] = s_h[threadIdx.x];
+        if (threadIdx.x == 0) layer_bins(s_h, a.effRad + l * (kMaxRadDev + 2), a.overflowFlag);
+        (void)d;
+    }
 };
 
 
@@ -1614,7 +1641,7 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
     const int L = g.numLayers, m = bs->numGroups;
     int *s_pass = s_items, *s_group = s_pass + L, *s_mainEnd = s_group + L;
     int *s_eff = s_mainEnd + L;                 // [L][kMaxRadDev + 2]
-    int *s_tail = s_eff + L * (kMaxRadDev + 2);  // [m][2]: tailStart, tailCount
+    int *s_tail = s_eff + L * (kMaxRadDev + 2);  // [m][3]: tailStart, tailCount, tailBase
     for (int i = threadIdx.x; i < L; i += blockDim.x) {
         s_pass[i] = ls[i].passive;
         s_group[i] = ls[i].group;
@@ -1622,8 +1649,9 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
     }
     for (int i = threadIdx.x; i < L * (kMaxRadDev + 2); i += blockDim.x) s_eff[i] = effRad[i];
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        s_tail[2 * i] = gs[i].tailStart;
-        s_tail[2 * i + 1] = gs[i].tailCount;
+        s_tail[3 * i] = gs[i].tailStart;
+        s_tail[3 * i + 1] = gs[i].tailCount;
+        s_tail[3 * i + 2] = gs[i].tailBase;
     }
     __syncthreads();
     const int nt = (g.sx / 32) * (g.sy / 8);
@@ -1632,7 +1660,7 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
     // waves past the last tile leave; lanes past it stay (inactive) for the wave reductions
     if (t - int(threadIdx.x & 63) >= nt) return;
     const bool tileIn = t < nt;
-    auto inTail = [&](int j) { return k >= s_tail[2 * j] && k < s_tail[2 * j] + s_tail[2 * j + 1]; };
+    auto inTail = [&](int j) { return k >= s_tail[3 * j] && k < s_tail[3 * j] + s_tail[3 * j + 1]; };
     int Rmax = -1;
     bool mainLive = false;
     uint64_t tailLive[kMaxLayersDev / 64] = {0, 0, 0, 0};  // groups with a live tail layer here
@@ -1675,7 +1703,7 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
     }
     for (int j = 0; j < m; ++j) {
         if (!inTail(j)) continue;  // uniform: k is the block's step
-        const int q = gs[j].tailBase + (k - s_tail[2 * j]);
+        const int q = s_tail[3 * j + 2] + (k - s_tail[3 * j]);
         if (q >= bs->tailPlanes) continue;
         const bool live = tileIn && ((tailLive[j >> 6] >> (j & 63)) & 1);
         if (tileIn) tailItemR[size_t(q) * nt + t] = live ? RM : -1;
@@ -2981,15 +3009,17 @@ hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, L
                                      int *hist, int *effRad, int *itemR, int *tailItemR, int *classCount,
                                      int2 *classList, int maxUnits, int *overflowFlag, int codesFromFill,
                                      int *planeRM, int *slotLayers, hipStream_t s) {
-    dispatch<LayerFinalizeK>({g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap, planeRM}, dim3(1), 0, s);
     const int nt = (g.sx / 32) * (g.sy / 8);
-    if (codesFromFill)
-        dispatch<TileHistK>({g, tileRad, ls, bs, hist}, dim3(g.numLayers), 0, s);
-    else
+    const LayerFinalizeK::Args fin{g, ls, gs, bs, tailPlaneGroup, tailPlaneK, tailCap, planeRM};
+    if (codesFromFill) {
+        dispatch<FinalizeTilesK>({fin, tileRad, hist, effRad, overflowFlag}, dim3(g.numLayers + 1), 0, s);
+    } else {
+        dispatch<LayerFinalizeK>(fin, dim3(1), 0, s);
         dispatch<TilesK>({g, iddSigma, ls, bs, tileRad, hist}, dim3((nt + 3) / 4, g.K), 0, s);
-    dispatch<BinsK>({g.numLayers, hist, effRad, overflowFlag}, dim3((g.numLayers + 63) / 64), 0, s);
+        dispatch<BinsK>({g.numLayers, hist, effRad, overflowFlag}, dim3((g.numLayers + 63) / 64), 0, s);
+    }
     const size_t itemsLds =
-        sizeof(int) * (size_t(g.numLayers) * (3 + kMaxRadDev + 2) + 2 * size_t(std::max(g.numLayers, 1)));
+        sizeof(int) * (size_t(g.numLayers) * (3 + kMaxRadDev + 2) + 3 * size_t(std::max(g.numLayers, 1)));
     dispatch<ItemsK>({g, tileRad, effRad, ls, gs, bs, itemR, tailItemR, classCount, classList, maxUnits, planeRM, slotLayers}, dim3((nt + 255) / 256, g.K), itemsLds, s);
     return hipGetLastError();
 }
