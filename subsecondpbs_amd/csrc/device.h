@@ -61,8 +61,8 @@ struct BeamScalars {
     int boxMax[3], pad3;
 };
 
-// Per-work-group results of the one-pass trace, reduced by the beam setup (min == nullptr when
-// the beam's trace reduced them itself).
+// Per-work-group results of the one-pass trace, reduced by the upload launch that follows the
+// trace (min == nullptr when the beam's trace reduced them itself).
 struct TracePartials {
     const float *min;    // [K][pitch]: minimum WEPL per step and work-group (1e20: none)
     const int *fi, *fo;  // [n]: least entry step, greatest exit step per work-group

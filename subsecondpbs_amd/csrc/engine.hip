@@ -1137,13 +1137,18 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         add_upload(up, stageSrc, e->profTable.p, oProf, bProf);
         add_upload(up, stageSrc, e->profLayer.p, oProfL, bProfL);
     }
+    // the trace's partials (one-pass trace) are reduced by the same launch
+    up.tp = parts;
+    up.weplMin = e->weplMin.p;
+    up.bs = e->beamScal.p;
+    up.K = K;
     rec_stage(kStageUpload + 1);
     HIP_TRY(launch_upload(up, s));
     rec_stage(kStageSetup);
     const float nucFactor = e->nuclearModel == 1 ? 0.97f : 1.0f;  // GAUSS_FIT entry sigma, wrapper:3788-3795
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, nucFactor,
-                              e->prim.classCount.p, e->prim.hist.p, parts, s));
+                              e->prim.classCount.p, e->prim.hist.p, s));
     if (keep) {
         HIP_TRY(keep->bs0.ensure(1));
         HIP_TRY(keep->weplMin.ensure(K));
@@ -1210,7 +1215,7 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
                             s));
         HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                                   subMode ? e->subspots.p : nullptr, subMode ? nSub : 0, e->subEntry.p, 1.0f,
-                                  e->prim.classCount.p, e->prim.hist.p, TracePartials{}, s));
+                                  e->prim.classCount.p, e->prim.hist.p, s));
         HIP_TRY(launch_rescale(g, e->layerScal.p, e->beamScal.p, e->rayW.p, keep->unitIdd.p,
                                keep->unitFp.p, e->iddSigma.p, e->firstPassive.p, e->prim.tileRad.p, s));
     } else {
@@ -1344,7 +1349,7 @@ static int update_enqueue(pbs_engine *e, const BeamState &st, const float *wPlan
     HIP_TRY(hipMemcpyAsync(e->weplMin.p, st.weplMin.p, sizeof(float) * K, hipMemcpyDeviceToDevice, s));
     HIP_TRY(launch_beam_setup(g, e->weplMin.p, e->layerHost.p, e->beamScal.p, e->layerScal.p,
                               st.subMode ? st.subspots.p : nullptr, st.subMode ? st.nSub : 0, e->subEntry.p, 1.0f,
-                              e->prim.classCount.p, e->prim.hist.p, TracePartials{}, s));
+                              e->prim.classCount.p, e->prim.hist.p, s));
     const ConvArgs a = make_conv_args(p, rx, ry, st.nSub, st.subMode);
     HIP_TRY(launch_conv(a, e->spot.p, e->interm.p, e->rayW.p, e->layerScal.p, e->beamScal.p, st.subspots.p,
                         e->subEntry.p, L, s));

@@ -134,19 +134,18 @@ hipError_t launch_weq_fill(const FanGeom &g, const WeqArgs &a, float *density, f
                           hipStream_t s);
 // The CT trace. Up to kMaxTraceSteps steps it runs in one pass whose per-beam reductions
 // (weplMin per step, beamFirstInside / beamFirstOutside of bs) are left as partials in the
-// scratch for the beam setup (*parts); past that in three passes that reduce them themselves
-// (parts->min == nullptr).
+// scratch (*parts), for the upload launch to reduce; past that in three passes that reduce
+// them themselves (parts->min == nullptr).
 constexpr int kMaxTraceSteps = 512;
 hipError_t launch_trace(const FanGeom &g, const float *ct, const LutDev &lut, float *density, float *wepl, int *fi,
                         int *fo, float *scratch, float *weplMin, BeamScalars *bs, TracePartials *parts,
                         hipStream_t s);
 hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *fi, const int *fo, float *weplMin,
                                BeamScalars *bs, hipStream_t s);
-// The beam's cut-offs and entry sigmas; with trace partials (parts.min != nullptr) it first
-// reduces them into weplMin and bs.
-hipError_t launch_beam_setup(const FanGeom &g, float *weplMin, const LayerHost *lh, BeamScalars *bs,
+// The beam's cut-offs and entry sigmas from weplMin and the beam's entry / exit steps.
+hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
                              LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
-                             int *classCount, int *hist, const TracePartials &parts, hipStream_t s);
+                             int *classCount, int *hist, hipStream_t s);
 hipError_t launch_conv(const ConvArgs &a, const float *spot, float *interm, float *rayW, const LayerScalars *ls,
                        const BeamScalars *bs, const float *subspots, const float *subEntry, int numLayers,
                        hipStream_t s);
@@ -191,6 +190,12 @@ struct UploadSeg {
 struct UploadArgs {
     UploadSeg seg[kMaxUploadSegs];
     int n;
+    // the one-pass trace's per-work-group partials, reduced by one more row of the launch into
+    // weplMin [K] and the beam's entry / exit steps (tp.min == nullptr: nothing to reduce)
+    TracePartials tp;
+    float *weplMin;
+    BeamScalars *bs;
+    int K;
 };
 hipError_t launch_upload(const UploadArgs &a, hipStream_t s);
 hipError_t launch_finalize_and_tiles(const FanGeom &g, const float2 *iddSigma, LayerScalars *ls, GroupScalars *gs,

@@ -373,7 +373,7 @@ struct TraceFixK {
 // the carry pass does (the two forms agree up to the compiler's choice of fused multiply-adds
 // in their inlined copies). The beam's reductions leave as per-work-group partials (the
 // per-step minimum over the group's rays that leave material, its least entry and greatest
-// exit step), which BeamSetupK reduces: no atomics, and nothing to initialise first.
+// exit step), which the upload launch reduces: no atomics, and nothing to initialise first.
 constexpr int kTraceRays = 32, kTraceThreadsPerRay = 16;
 
 template <int CH>
@@ -598,55 +598,13 @@ __global__ __launch_bounds__(256) void k_trace_reduce(FanGeom g, const float *__
 
 // Distal cut-offs, entry plane and entry sigmas (host formulas, evaluated without
 // contraction so the integers match the host's float arithmetic).
-__device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, float *__restrict__ weplMinOut,
+__device__ __forceinline__ void beam_setup_body(const uint3 blockIdx, const uint3 gridDim, FanGeom g, const float *__restrict__ weplMin,
                                                     const LayerHost *__restrict__ lh, BeamScalars *__restrict__ bs,
                                                     LayerScalars *__restrict__ ls, const float *__restrict__ subspots,
                                                     int maxSub, float *__restrict__ subEntry, float nucFactor,
-                                                    int *__restrict__ classCount, int *__restrict__ hist,
-                                                    const TracePartials tp) {
+                                                    int *__restrict__ classCount, int *__restrict__ hist) {
 #pragma clang fp contract(off)
-    __shared__ unsigned s_wmb[kMaxTraceSteps];
-    __shared__ int s_io[2][4];
-    const float *weplMin = weplMinOut;
-    int inside, outside;
-    if (tp.min) {
-        // the one-pass trace's per-work-group partials: per-step minimum WEPL, least entry and
-        // greatest exit step, reduced here (and written out for the later readers). The minima
-        // are read as consecutive float4s by all threads and met by LDS atomic minima on the
-        // bits (WEPL >= 0, so the bit patterns order as the values)
-        for (int k = threadIdx.x; k < g.K; k += blockDim.x) s_wmb[k] = __float_as_uint(1.0e20f);
-        __syncthreads();
-        const int q4 = tp.pitch / 4;
-        const float4 *pm = reinterpret_cast<const float4 *>(tp.min);
-#pragma unroll 4
-        for (int i = threadIdx.x; i < g.K * q4; i += blockDim.x) {
-            const float4 v = pm[i];
-            atomicMin(&s_wmb[i / q4], __float_as_uint(fminf(fminf(v.x, v.y), fminf(v.z, v.w))));
-        }
-        int mn = INT_MAX, mx = INT_MIN;
-        for (int i = threadIdx.x; i < tp.n; i += blockDim.x) {
-            mn = min(mn, tp.fi[i]);
-            mx = max(mx, tp.fo[i]);
-        }
-        mn = wave_min_i(mn);
-        mx = wave_max_i(mx);
-        if ((threadIdx.x & 63) == 0) {
-            s_io[0][threadIdx.x >> 6] = mn;
-            s_io[1][threadIdx.x >> 6] = mx;
-        }
-        __syncthreads();
-        inside = min(min(s_io[0][0], s_io[0][1]), min(s_io[0][2], s_io[0][3]));
-        outside = max(max(s_io[1][0], s_io[1][1]), max(s_io[1][2], s_io[1][3]));
-        if (threadIdx.x == 0) {
-            bs->beamFirstInside = inside;
-            bs->beamFirstOutside = outside;
-        }
-        for (int k = threadIdx.x; k < g.K; k += blockDim.x) weplMinOut[k] = __uint_as_float(s_wmb[k]);
-        weplMin = reinterpret_cast<const float *>(s_wmb);
-    } else {
-        inside = bs->beamFirstInside;
-        outside = bs->beamFirstOutside;
-    }
+    const int inside = bs->beamFirstInside, outside = bs->beamFirstOutside;
     const int fpa = first_larger_ordered(weplMin, g.K, g.maxCutoffWepl);
     const int gp = min(fpa, outside);
     const float entryZ = float(inside) * g.fd[2] + g.fc[2];
@@ -687,7 +645,7 @@ struct BeamSetupK {
     static constexpr int NT = 256;
     struct Args {
         FanGeom g;
-        float *weplMin;
+        const float *weplMin;
         const LayerHost *lh;
         BeamScalars *bs;
         LayerScalars *ls;
@@ -697,9 +655,8 @@ struct BeamSetupK {
         float nucFactor;
         int *classCount;
         int *hist;
-        TracePartials tp;
     };
-    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { beam_setup_body(b, d, a.g, a.weplMin, a.lh, a.bs, a.ls, a.subspots, a.maxSub, a.subEntry, a.nucFactor, a.classCount, a.hist, a.tp); }
+    static __device__ __forceinline__ void run(const Args &a, const uint3 b, const uint3 d) { beam_setup_body(b, d, a.g, a.weplMin, a.lh, a.bs, a.ls, a.subspots, a.maxSub, a.subEntry, a.nucFactor, a.classCount, a.hist); }
 };
 
 
@@ -2490,8 +2447,39 @@ __global__ __launch_bounds__(256) void k_transfer_many(const TransferK::Args *__
 
 // Segment blockIdx.y of the upload, 32-bit words strided over the blocks of blockIdx.x (the
 // source is host memory: every word is read once).
+// The one-pass trace's partials (row a.n of the upload launch): a wave per step takes the minimum
+// of the step's per-work-group minima, and wave 0 of work-group 0 the least entry and greatest
+// exit step. Minima and maxima are exact, whatever the order.
+__device__ __forceinline__ void trace_partials_body(const uint3 blockIdx, const uint3 gridDim, const UploadArgs &a) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = int(blockIdx.x) * 4 + w; k < a.K; k += int(gridDim.x) * 4) {
+        const float *row = a.tp.min + size_t(k) * a.tp.pitch;
+        float m = 1.0e20f;
+        for (int i = lane; i < a.tp.pitch; i += 64) m = fminf(m, row[i]);
+        m = wave_min(m);
+        if (lane == 0) a.weplMin[k] = m;
+    }
+    if (blockIdx.x == 0 && w == 0) {
+        int mn = INT_MAX, mx = INT_MIN;
+        for (int i = lane; i < a.tp.n; i += 64) {
+            mn = min(mn, a.tp.fi[i]);
+            mx = max(mx, a.tp.fo[i]);
+        }
+        mn = wave_min_i(mn);
+        mx = wave_max_i(mx);
+        if (lane == 0) {
+            a.bs->beamFirstInside = mn;
+            a.bs->beamFirstOutside = mx;
+        }
+    }
+}
+
 __device__ __forceinline__ void upload_body(const uint3 blockIdx, const uint3 gridDim, const UploadArgs &a) {
     const int sg = blockIdx.y;
+    if (sg == a.n && a.tp.min) {
+        trace_partials_body(blockIdx, gridDim, a);
+        return;
+    }
     if (sg >= a.n) return;
     const UploadSeg u = a.seg[sg];
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < u.words; i += gridDim.x * 256u) u.dst[i] = u.src[i];
@@ -2778,11 +2766,13 @@ hipError_t issue_records(const std::vector<RecLaunch> &launches, unsigned char *
 }
 
 hipError_t launch_upload(const UploadArgs &a, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
+    const bool reduce = a.tp.min != nullptr;
+    if (a.n <= 0 && !reduce) return hipSuccess;
     uint32_t most = 0;
     for (int i = 0; i < a.n; ++i) most = most > a.seg[i].words ? most : a.seg[i].words;
-    const unsigned bx = std::max(1u, std::min(64u, (most + 255u) / 256u));
-    dispatch<UploadK>({a}, dim3(bx, unsigned(a.n)), 0, s);
+    unsigned bx = std::max(1u, std::min(64u, (most + 255u) / 256u));
+    if (reduce) bx = std::max(bx, unsigned(a.K + 3) / 4u);  // a wave per step in the reduction row
+    dispatch<UploadK>({a}, dim3(bx, unsigned(a.n) + (reduce ? 1u : 0u)), 0, s);
     return hipGetLastError();
 }
 
@@ -2871,10 +2861,10 @@ hipError_t launch_trace_reduce(const FanGeom &g, const float *wepl, const int *f
     return hipGetLastError();
 }
 
-hipError_t launch_beam_setup(const FanGeom &g, float *weplMin, const LayerHost *lh, BeamScalars *bs,
+hipError_t launch_beam_setup(const FanGeom &g, const float *weplMin, const LayerHost *lh, BeamScalars *bs,
                              LayerScalars *ls, const float *subspots, int maxSub, float *subEntry, float nucFactor,
-                             int *classCount, int *hist, const TracePartials &parts, hipStream_t s) {
-    dispatch<BeamSetupK>({g, weplMin, lh, bs, ls, subspots, maxSub, subEntry, nucFactor, classCount, hist, parts}, dim3(1), 0, s);
+                             int *classCount, int *hist, hipStream_t s) {
+    dispatch<BeamSetupK>({g, weplMin, lh, bs, ls, subspots, maxSub, subEntry, nucFactor, classCount, hist}, dim3(1), 0, s);
     return hipGetLastError();
 }
 
