@@ -809,7 +809,7 @@ int pbs_engine_set_dose_grid(pbs_engine *e, const int dims[3], const float res[3
 
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
                             const float *cpbWorldGeom, const float *gradDose = nullptr, float *gradSpots = nullptr,
-                            BeamState *keep = nullptr, BeamPlan *planned = nullptr);
+                            BeamState *keep = nullptr, BeamPlan *planned = nullptr, bool mayDefer = true);
 
 int pbs_engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev) {
     return engine_beam_dose(e, beam, doseDev, nullptr, nullptr);
@@ -959,10 +959,11 @@ static size_t pin_align(size_t n) { return (n + 255) & ~size_t(255); }
 // positions when the CPB grid is given in world coordinates (cudaFinalDose).
 // `planned`: the beam's plan, made by the caller on this engine's CT, dose grid, tables and
 // environment (the server plans at admission to size the request), moved in instead of planned
-// again.
+// again. `mayDefer`: a plain lattice beam is planned from its lattice's axes, its trace queued,
+// and its spots placed while the trace runs (the placement is most of the planning time).
 static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float *doseDev, const CpbOverride *ov,
                             const float *cpbWorldGeom, const float *gradDose, float *gradSpots, BeamState *keep,
-                            BeamPlan *planned) {
+                            BeamPlan *planned, bool mayDefer) {
     const bool adjoint = gradDose != nullptr;
     if (!e || !beam || (!doseDev && !adjoint)) return set_error(PBS_E_INVALID, "null argument");
     if (!e->ctPtr) return set_error(PBS_E_INVALID, "no CT bound to the engine");
@@ -975,10 +976,12 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         e->uploadPending = false;
     }
     BeamPlan &p = e->plan;
+    const bool defer = mayDefer && !planned && !ov && !cpbWorldGeom && !adjoint && !keep && e->nuclearModel == 0 &&
+                       !t_rec && !t_arena;
     if (planned)
         p = std::move(*planned);
     else if (!plan_beam(e->ctDims, e->ctRes, e->ctCorner, e->doseDims, e->doseRes, e->doseCorner, *beam, e->energy, p,
-                        ov, &e->env))
+                        ov, &e->env, defer))
         return set_error(PBS_E_GEOMETRY, p.error);
     FanGeom &g = e->geom;
     fill_geom(p, *e, g);
@@ -1079,6 +1082,14 @@ static int engine_beam_dose(pbs_engine *e, const pbs_beam_settings *beam, float 
         rec_stage(kStageTrace);
         HIP_TRY(launch_trace(g, e->ctPtr, e->lut, e->density.p, e->wepl.p, e->fi.p, e->fo.p, e->traceScratch.p,
                              e->weplMin.p, e->beamScal.p, &parts, s));
+    }
+    // the lattice's spots, placed while the trace runs
+    if (p.placementPending && !place_plan_spots(*beam, p)) {
+        // a weighted spot off the lattice: not a lattice beam, so its grids are not the ones just
+        // traced. Once the trace is done, the beam is planned again in full (legacy CPB path)
+        HIP_TRY(hipStreamSynchronize(s));
+        for (hipEvent_t ev : marks) e->evPool.push_back(ev);
+        return engine_beam_dose(e, beam, doseDev, ov, cpbWorldGeom, gradDose, gradSpots, keep, nullptr, false);
     }
     // the per-layer values and the subspot table go up behind the trace (copies from pageable
     // memory hold the host, so the trace is already queued while they run)

@@ -213,7 +213,11 @@ void batch_bins(const int *histIn, int *batched, int *effRad) {
 }
 
 // buildRawSpotLattice, physical-position mode (wrapper:1259-1473).
-static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
+static bool place_spots(const pbs_beam_settings &b, BeamPlan &p);
+
+// The spot lattice of the beam (buildRawSpotLattice, wrapper:1366-1460): its axes, then (unless
+// `defer`) the placement of every spot on them.
+static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p, bool defer) {
     if (!b.layerSpotCounts || !b.spotPositions || !b.spotWeights || b.numSpots <= 0) return false;
     int total = 0;
     for (int l = 0; l < b.numLayers; ++l) total += b.layerSpotCounts[l];
@@ -304,6 +308,29 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
     const float dx = step_of(ux), dy = step_of(uy), ox = ux.front(), oy = uy.front();
     const int nx = std::max(1, int(std::lround((ux.back() - ox) / dx)) + 1);
     const int ny = std::max(1, int(std::lround((uy.back() - oy) / dy)) + 1);
+    for (int l = 0; l < b.numLayers; ++l)
+        if (b.layerSpotCounts[l] <= 0) return false;
+    p.spotDims[0] = nx; p.spotDims[1] = ny;
+    p.spotOffset[0] = ox; p.spotOffset[1] = oy;
+    p.spotDelta[0] = dx; p.spotDelta[1] = dy;
+    if (defer) {
+        p.placementPending = true;
+        return true;
+    }
+    if (place_spots(b, p)) return true;
+    p.spotDims[0] = p.spotDims[1] = 0;  // not a lattice beam: no lattice in the plan
+    p.spotOffset[0] = p.spotOffset[1] = 0.0f;
+    p.spotDelta[0] = p.spotDelta[1] = 0.0f;
+    return false;
+}
+
+// Every spot of a physical-position lattice onto the axes in p: its cell, and its weight added
+// to the dense lattice; false when a weighted spot is off the lattice.
+static bool place_spots(const pbs_beam_settings &b, BeamPlan &p) {
+    p.placementPending = false;
+    const int total = b.numSpots;
+    const int nx = p.spotDims[0], ny = p.spotDims[1];
+    const float ox = p.spotOffset[0], oy = p.spotOffset[1], dx = p.spotDelta[0], dy = p.spotDelta[1];
     std::vector<float> dense(size_t(nx) * ny * b.numLayers, 0.0f);
     std::vector<int> cells(size_t(total), -1);
     int off = 0;
@@ -327,12 +354,11 @@ static bool build_lattice(const pbs_beam_settings &b, BeamPlan &p) {
         off += cnt;
     }
     p.spotCell.swap(cells);
-    p.spotDims[0] = nx; p.spotDims[1] = ny;
-    p.spotOffset[0] = ox; p.spotOffset[1] = oy;
-    p.spotDelta[0] = dx; p.spotDelta[1] = dy;
     p.denseWeights.swap(dense);
     return true;
 }
+
+bool place_plan_spots(const pbs_beam_settings &b, BeamPlan &p) { return !p.placementPending || place_spots(b, p); }
 
 // alignGridToReferencePhase (wrapper:970-997).
 static bool align_grid(float minC, float maxC, float ref, float step, int multiple, float &origin, int &count) {
@@ -456,7 +482,8 @@ static void plan_profile_hooks(const pbs_beam_settings &b, const pbs_energy_stru
 
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3], const int doseDims[3],
                const float doseRes[3], const float doseCorner[3], const pbs_beam_settings &b,
-               const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov, const PlanEnv *envIn) {
+               const pbs_energy_struct &en, BeamPlan &p, const CpbOverride *ov, const PlanEnv *envIn,
+               bool deferPlacement) {
     p = BeamPlan{};
     const PlanEnv env = envIn ? *envIn : PlanEnv{};
     if (b.numLayers <= 0 || b.numLayers > kMaxLayers) { p.error = "numLayers out of range"; return false; }
@@ -498,7 +525,7 @@ bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3
     const Aff w2g = inverse(p.g2w);
 
     // spot lattice and CPB / ray grid (wrapper:2951-3212)
-    const bool lattice = ov ? false : build_lattice(b, p);
+    const bool lattice = ov ? false : build_lattice(b, p, deferPlacement && !b.spotPositionsAreIndices);
     const F3 doseRes_mm = f3(doseRes[0] * lenToMm, doseRes[1] * lenToMm, doseRes[2] * lenToMm);
     const F3 doseCorner_mm = f3(doseCorner[0] * lenToMm, doseCorner[1] * lenToMm, doseCorner[2] * lenToMm);
     if (lattice && b.raySpacing[0] > 0.0f && b.raySpacing[1] > 0.0f) {

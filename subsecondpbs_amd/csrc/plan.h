@@ -56,6 +56,7 @@ struct BeamPlan {
     float spotOffset[2] = {0, 0}, spotDelta[2] = {0, 0};
     std::vector<float> denseWeights;  // [L][sy][sx]
     std::vector<int> spotCell;        // per spot: its cell (l * sy + iy) * sx + ix in denseWeights, or -1
+    bool placementPending = false;    // the lattice's axes are set, its spots not placed yet
     // grids
     float cpbCorner[3] = {0, 0, 0}, cpbRes[3] = {0, 0, 0};
     int rayDims[2] = {0, 0};
@@ -124,11 +125,19 @@ struct PlanEnv {
 };
 PlanEnv read_plan_env();
 
-// Returns false (and sets plan.error) on a contract failure.
+// Returns false (and sets plan.error) on a contract failure. With deferPlacement, a beam of
+// physical spot positions is planned from its lattice's axes alone (the per-spot placement,
+// most of the planning time, is left for place_plan_spots; plan.placementPending): the grids
+// are the lattice path's, valid if every weighted spot then lands on a lattice point.
 bool plan_beam(const int ctDims[3], const float ctRes[3], const float ctCorner[3],
                const int doseDims[3], const float doseRes[3], const float doseCorner[3],
                const pbs_beam_settings &beam, const pbs_energy_struct &energy, BeamPlan &plan,
-               const CpbOverride *cpbOverride = nullptr, const PlanEnv *env = nullptr);
+               const CpbOverride *cpbOverride = nullptr, const PlanEnv *env = nullptr,
+               bool deferPlacement = false);
+// The deferred placement of a plan made with deferPlacement: the dense lattice weights and the
+// spots' cells. False when a weighted spot is off the lattice: the beam is then not a lattice
+// beam, and must be planned again without deferral (which takes the legacy CPB path).
+bool place_plan_spots(const pbs_beam_settings &beam, BeamPlan &plan);
 
 // findDecimalOrdered (include/utils/vector_find.h:130-145) and the wrapper's own energy
 // bracket (wrapper:3926-3992).
