@@ -1972,20 +1972,24 @@ struct RowsShape {
     static constexpr int G = RM + 1 <= 8 ? 8 : (RM + 1 <= 16 ? 16 : (RM + 1 <= 32 ? 32 : 64));
     static constexpr int NT = 16 * G;
     static constexpr int EW = pad4(RM + 3);
-    // With EW a multiple of 16 floats, the 32 table rows of a source row would start on 4 bank
-    // groups only, and their 16-byte stores would conflict 8 ways: every fourth column then moves
-    // by 4 floats more (col_off), which spreads the starts over all 64 banks.
-    static constexpr bool SWZ = EW % 16 == 0;
-    static constexpr int SPAN = 32 * EW + (SWZ ? 32 : 0);
+    // The table rows of a source row's 32 columns go out as 16-byte stores, which the LDS serves 8
+    // lanes (128 bytes) at a time: 8 consecutive columns must start on distinct 4-bank groups
+    // modulo 32 banks. A column stride that is an odd multiple of 4 floats does that (12, 20: the
+    // starts of any 8 aligned columns are distinct modulo 32 banks, of any 16 modulo 64). A stride
+    // of 16 would start them on 2 groups only (4-way conflicts): an EW of 16 gets a stride of 20.
+    static constexpr int CS = EW % 8 == 0 ? EW + 4 : EW;
+    static constexpr int SPAN = 32 * CS;
     // source-row pitch: SPAN rounded up to G modulo 64 floats puts the 64 / G groups of a wave at
     // bank offsets 0, G, 2G, ...: each group's per-lane reads E[1 + v] (v <= RM < G) then fall
     // on banks no other group of the wave touches
     static constexpr int RP = SPAN + ((G - SPAN % 64) % 64 + 64) % 64;
-    static __device__ __forceinline__ int col_off(int col) { return col * EW + (SWZ ? 4 * (col >> 2) : 0); }
+    static __device__ __forceinline__ int col_off(int col) { return col * CS; }
     static constexpr int EBUF = 8 * RP;  // one layer's table
-    // after the last layer: four groups' rows at a time (odd pitch: the lanes of a wave write
-    // distinct banks)
-    static constexpr int HWp = HW | 1;
+    // after the last layer: four groups' rows at a time. The pitch is odd, so the lanes of a wave
+    // write distinct banks. The block sum reads 32 lanes at a time along block rows, and a
+    // half-wave often spans two rows |dy| apart by one: at RM = 11 the pitch 39 puts such rows'
+    // columns on common banks (77 conflicted cycles per unit), 43 none (nor 31 and 47 at RM 7, 15)
+    static constexpr int HWp = RM == 11 ? 43 : (HW | 1);
     static constexpr int ACC = 4 * G * HWp;
     static constexpr int OUT = BH * BW;
     static constexpr int LDS = EBUF > ACC ? EBUF : ACC;
