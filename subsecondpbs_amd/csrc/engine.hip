@@ -864,7 +864,7 @@ static int run_grid(pbs_engine *e, const BeamPlan &p, const FanGeom &g, GridWork
                                       w.planeRM.p, w.slotLayers.p, s));
     HIP_TRY(mk());
     if (e->oneStream || t_rec) {
-        const hipStream_t one[1 + kSideStreams] = {s, s, s, s};
+        const hipStream_t one[1 + kSideStreams] = {s, s, s};
         HIP_TRY(launch_superpose(g, iddSigma, w.tileRad.p, w.effRad.p, ls, w.groupScal.p, bs, w.classCount.p,
                                  w.classList.p, w.maxUnits, w.slotLayers.p, w.slots.p, w.tailSlots.p, e->tuneSuperposeGrid,
                                  one));

@@ -9,7 +9,7 @@
 
 namespace pbs {
 
-constexpr int kSideStreams = 3;  // superposition radius classes run on 1 + kSideStreams streams
+constexpr int kSideStreams = 2;  // superposition radius classes run on 1 + kSideStreams streams
 
 // ---------------------------------------------------------------------------------------
 // Multi-beam launches

@@ -3039,11 +3039,14 @@ hipError_t launch_superpose(const FanGeom &g, const float2 *iddSigma, const int 
 #define PBS_CLASS(RMV, C, S)                                                                                       \
     launch_class<RMV>(g, iddSigma, tileRad, effRad, ls, gs, bs, classCount, classList, C, maxUnits, slotLayers,   \
                       mainSlots, tailSlots, grid, streams[S])
+    // class 11 on the engine stream, 7 on one side stream, 15 and the rare 24 / 32 on the other:
+    // each side stream costs a wait at the fork and another at the join (a third side stream for
+    // 24 / 32 cost more than it overlapped)
     PBS_CLASS(11, 1, 0);
     PBS_CLASS(7, 0, 1);
     PBS_CLASS(15, 2, 2);
-    PBS_CLASS(24, 3, 3);
-    PBS_CLASS(32, 4, 3);
+    PBS_CLASS(24, 3, 2);
+    PBS_CLASS(32, 4, 2);
 #undef PBS_CLASS
     if (t_rec) t_rec->par = false;
     return hipGetLastError();
