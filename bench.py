@@ -571,7 +571,7 @@ def main() -> int:
                             "frac_unpacked": ach / FP32_UNPACKED_PEAK_TFLOPS,
                             "traffic": traffic.get("superpose_stage_bytes"),
                             "kernel": "superposition stage: SuperposeRowsK<7/11/15> and SuperposeK<24/32>, 5 "
-                                      "radius classes on 4 streams (stage wall time from HIP events)",
+                                      "radius classes on 3 streams (stage wall time from HIP events)",
                             "compute": "fp32 VALU FMA (no matrix cores: no dense contraction); peak = the dense "
                                        "fp32 figure, peak_unpacked = without packed fp32", "stage_ms": t_ms}
         elif work.get(name, {}).get("bytes"):
