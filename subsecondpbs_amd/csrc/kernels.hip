@@ -113,6 +113,26 @@ __device__ __forceinline__ float wave_min(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
     return v;
 }
+// The same minimum as a uniform value, through DPP lane moves within each row of 16 lanes (the
+// quad swaps, then the half-row and row mirrors) and four lane reads across the rows: no LDS
+// round trips. All 64 lanes must be active. fminf ignores NaN, and a minimum does not depend on
+// the order it is taken in.
+__device__ __forceinline__ float wave_min_uniform(float v) {
+    auto dmin = [](float x, auto ctrl) {
+        const int y = __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), decltype(ctrl)::value,
+                                                  0xf, 0xf, false);
+        return fminf(x, __int_as_float(y));
+    };
+    v = dmin(v, std::integral_constant<int, 0xb1>{});   // quad_perm [1, 0, 3, 2]
+    v = dmin(v, std::integral_constant<int, 0x4e>{});   // quad_perm [2, 3, 0, 1]
+    v = dmin(v, std::integral_constant<int, 0x141>{});  // row_half_mirror
+    v = dmin(v, std::integral_constant<int, 0x140>{});  // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fminf(fminf(r0, r1), fminf(r2, r3));
+}
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -998,7 +1018,7 @@ __device__ __forceinline__ void fill_body(const uint3 blockIdx, const uint3 grid
     auto emit = [&](unsigned k, int u, int cb, float res, float rs, float nres, float nrs) {
         out[size_t(k) * outStep] = make_float2(res, rs);
         if (HALO && cell >= 0) nout[size_t(k) * hf.nucPlane] = make_float2(nres, nrs);
-        const float m = wave_min(rs);
+        const float m = wave_min_uniform(rs);  // every lane is active here
         const bool any = __any(res > 0.0f);
         if (lane == 0) {
             s_min[cb][u][w] = m;
