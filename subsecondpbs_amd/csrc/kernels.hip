@@ -117,7 +117,9 @@ __device__ __forceinline__ float wave_min(float v) {
 // quad swaps, then the half-row and row mirrors) and four lane reads across the rows: no LDS
 // round trips. All 64 lanes must be active. fminf ignores NaN, and a minimum does not depend on
 // the order it is taken in.
-__device__ __forceinline__ float wave_min_uniform(float v) {
+// The minimum over each row of 16 lanes, in every lane of the row (rows are whole: their lanes
+// are all active or all inactive).
+__device__ __forceinline__ float row_min(float v) {
     auto dmin = [](float x, auto ctrl) {
         const int y = __builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), decltype(ctrl)::value,
                                                   0xf, 0xf, false);
@@ -126,7 +128,20 @@ __device__ __forceinline__ float wave_min_uniform(float v) {
     v = dmin(v, std::integral_constant<int, 0xb1>{});   // quad_perm [1, 0, 3, 2]
     v = dmin(v, std::integral_constant<int, 0x4e>{});   // quad_perm [2, 3, 0, 1]
     v = dmin(v, std::integral_constant<int, 0x141>{});  // row_half_mirror
-    v = dmin(v, std::integral_constant<int, 0x140>{});  // row_mirror
+    return dmin(v, std::integral_constant<int, 0x140>{});  // row_mirror
+}
+// The minimum over each half-wave (lanes 0-31, 32-63), in every lane of the half: the rows'
+// minima and two lane reads per half. A half-wave may be inactive (its value is then unused).
+__device__ __forceinline__ float half_wave_min(float v) {
+    v = row_min(v);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (threadIdx.x & 32) ? fminf(r2, r3) : fminf(r0, r1);
+}
+__device__ __forceinline__ float wave_min_uniform(float v) {
+    v = row_min(v);
     const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
     const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
     const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
@@ -492,9 +507,7 @@ __device__ __forceinline__ void trace_body(const uint3 blockIdx, FanGeom g, cons
             const int k = sg * kTraceSeg + j;
             // uniform over the half-wave: its 32 lanes share t
             if (sg >= s1 || k >= g.K) continue;
-            float m = (rayIn && fov > 0) ? wv[c * kTraceSeg + j] + 0.0f : 1.0e20f;  // -0 -> +0
-#pragma unroll
-            for (int o = kTraceRays / 2; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, kTraceRays));
+            const float m = half_wave_min((rayIn && fov > 0) ? wv[c * kTraceSeg + j] + 0.0f : 1.0e20f);  // -0 -> +0
             if (rl == 0) partMin[size_t(k) * partPitch + blockIdx.x] = m;
         }
     }
@@ -1163,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_rescale(FanGeom g, LayerScalars *__rest
                 const float res = live ? W * v.x : 0.0f;
                 const float rs = live ? v.y : PBS_INF;
                 iddSigma[i] = make_float2(res, rs);
-                const float m = wave_min(rs);
+                const float m = wave_min_uniform(rs);  // every lane is active here
                 const bool any = __any(res > 0.0f);
                 if (lane == 0) {
                     s_min[cb][u][w] = m;
