@@ -2116,14 +2116,6 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
     const int2 *list = classList + size_t(cls) * maxUnits;
     const size_t planeXY = size_t(g.sy) * g.sx;
     float *const A = s_lds;  // after the layers: four groups' rows [4][G][HWp]
-    // this thread's block pixels (Y, X) in the final sum, X = pixel mod BW
-    int oy[OPT], ox[OPT];
-#pragma unroll
-    for (int j = 0; j < OPT; ++j) {
-        const int o = tid + j * NT;
-        oy[j] = o / BW;
-        ox[j] = o % BW;
-    }
     for (int u = blockIdx.x; u < count; u += gridDim.x) {
         const int2 unit = list[u];
         const int slot = unit.x, b = unit.y;
@@ -2227,6 +2219,12 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
         // block pixel (Y, X) = the sum over the groups (h, g), in group order, of row |Y - RM - g|
         // of the group at column X - 16h, where |Y - RM - g| <= RM and 0 <= X - 16h < 16 + 2RM;
         // four groups at a time through A
+        // this thread's block pixels (Y, X), X = pixel mod BW, from a thread index that an empty
+        // asm makes opaque here: the compiler would otherwise compute the pixels' coordinates and
+        // LDS addresses before the unit loop and hold them across every layer (~30 registers:
+        // 152 instead of 125 for RM 11, three waves per SIMD instead of four)
+        int tb = tid;
+        asm volatile("" : "+v"(tb));
         float part[OPT];
 #pragma unroll
         for (int j = 0; j < OPT; ++j) part[j] = 0.0f;
@@ -2241,10 +2239,11 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
             const int dx0 = 16 * (ph >> 1), dy0 = RM + 4 * (ph & 1);  // this phase's half and first row
 #pragma unroll
             for (int j = 0; j < OPT; ++j) {
-                if (tid + j * NT >= S::OUT) break;
-                const int hx = ox[j] - dx0;  // the column within this phase's half
+                const int o = tb + j * NT;
+                if (o >= S::OUT) break;
+                const int hx = o % BW - dx0;  // the column within this phase's half
                 if (unsigned(hx) < unsigned(HW)) {
-                    const int y0 = oy[j] - dy0;
+                    const int y0 = o / BW - dy0;
                     float sum = part[j];
 #pragma unroll
                     for (int g4 = 0; g4 < 4; ++g4) {
