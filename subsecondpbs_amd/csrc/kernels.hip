@@ -2275,7 +2275,10 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
 template <int RM>
 struct SuperposeRowsK {
     static constexpr int NT = RowsShape<RM>::NT;
-    static constexpr int WPE = RM <= 11 ? 3 : 1;  // 168 registers (three waves per SIMD) where it fits
+    // four waves per SIMD (128 registers) for RM 7 and 11: the one-beam kernels fit without the
+    // bound (115 / 125), the batch's unit queue needs it (137 otherwise; a few spilled dwords of
+    // its beam loop cost less than the fourth wave gains); three (168) for RM 15
+    static constexpr int WPE = RM <= 11 ? 4 : 3;
     static constexpr bool kUnitQueue = true;  // batched as a queue of the beams' units
     struct Args {
         FanGeom g;
