@@ -148,6 +148,23 @@ __device__ __forceinline__ float wave_min_uniform(float v) {
     const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
     return fminf(fminf(r0, r1), fminf(r2, r3));
 }
+// Integer maximum / minimum over the wave as uniform values, the same way (all 64 lanes active).
+template <bool MAX>
+__device__ __forceinline__ int wave_ext_uniform_i(int v) {
+    auto dext = [](int x, auto ctrl) {
+        const int y = __builtin_amdgcn_update_dpp(x, x, decltype(ctrl)::value, 0xf, 0xf, false);
+        return MAX ? max(x, y) : min(x, y);
+    };
+    v = dext(v, std::integral_constant<int, 0xb1>{});
+    v = dext(v, std::integral_constant<int, 0x4e>{});
+    v = dext(v, std::integral_constant<int, 0x141>{});
+    v = dext(v, std::integral_constant<int, 0x140>{});
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return MAX ? max(max(r0, r1), max(r2, r3)) : min(min(r0, r1), min(r2, r3));
+}
+__device__ __forceinline__ int wave_max_uniform_i(int v) { return wave_ext_uniform_i<true>(v); }
+__device__ __forceinline__ int wave_min_uniform_i(int v) { return wave_ext_uniform_i<false>(v); }
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
@@ -1367,10 +1384,10 @@ __device__ __forceinline__ void layer_finalize_body(const uint3 blockIdx, const 
             en0 = max(en0, G.boxEnd[0]); en1 = max(en1, G.boxEnd[1]); en2 = max(en2, G.boxEnd[2]);
             mi0 = max(mi0, G.boxMax[0]); mi1 = max(mi1, G.boxMax[1]); mi2 = max(mi2, G.boxMax[2]);
         }
-        mp = wave_max_i(mp); me = wave_max_i(me);
-        st0 = wave_min_i(st0); st1 = wave_min_i(st1); st2 = wave_min_i(st2);
-        en0 = wave_max_i(en0); en1 = wave_max_i(en1); en2 = wave_max_i(en2);
-        mi0 = wave_max_i(mi0); mi1 = wave_max_i(mi1); mi2 = wave_max_i(mi2);
+        mp = wave_max_uniform_i(mp); me = wave_max_uniform_i(me);  // wave 0, every lane active
+        st0 = wave_min_uniform_i(st0); st1 = wave_min_uniform_i(st1); st2 = wave_min_uniform_i(st2);
+        en0 = wave_max_uniform_i(en0); en1 = wave_max_uniform_i(en1); en2 = wave_max_uniform_i(en2);
+        mi0 = wave_max_uniform_i(mi0); mi1 = wave_max_uniform_i(mi1); mi2 = wave_max_uniform_i(mi2);
         if (lane == 0) {
             const int total = s_total;
             bs->numGroups = m;
@@ -1606,7 +1623,7 @@ __device__ __forceinline__ void append_unit(int *__restrict__ classCount, int2 *
         const int lane = __lane_id();
         int base = 0;
         if (lane == leader) base = atomicAdd(&classCount[cc], __popcll(peers));
-        base = __shfl(base, leader);
+        base = __builtin_amdgcn_readlane(base, leader);  // leader is uniform
         if (c == cc) {
             const int pos = base + __popcll(peers & ((uint64_t(1) << lane) - 1));
             if (pos < maxUnits) classList[size_t(cc) * maxUnits + pos] = unit;
@@ -1688,7 +1705,7 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
     if (tileIn) itemR[slot] = mainLive ? RM : -1;
     if (tileIn && mainLive) append_unit(classCount, classList, maxUnits, c, make_int2(slot, -1));
     {
-        const int pm = wave_max_i(mainLive ? RM : -1);
+        const int pm = wave_max_uniform_i(mainLive ? RM : -1);  // every lane is active here
         if (lane == 0 && pm >= 0) atomicMax(&planeRM[k], pm);
     }
     for (int j = 0; j < m; ++j) {
@@ -1698,7 +1715,7 @@ __device__ __forceinline__ void items_body(const uint3 blockIdx, const uint3 gri
         const bool live = tileIn && ((tailLive[j >> 6] >> (j & 63)) & 1);
         if (tileIn) tailItemR[size_t(q) * nt + t] = live ? RM : -1;
         if (live) append_unit(classCount, classList, maxUnits, c, make_int2(slot, j));
-        const int pm = wave_max_i(live ? RM : -1);
+        const int pm = wave_max_uniform_i(live ? RM : -1);
         if (lane == 0 && pm >= 0) atomicMax(&planeRM[g.K + q], pm);
     }
 }
