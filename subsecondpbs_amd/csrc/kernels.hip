@@ -2512,7 +2512,7 @@ __device__ __forceinline__ void trace_partials_body(const uint3 blockIdx, const 
         const float *row = a.tp.min + size_t(k) * a.tp.pitch;
         float m = 1.0e20f;
         for (int i = lane; i < a.tp.pitch; i += 64) m = fminf(m, row[i]);
-        m = wave_min(m);
+        m = wave_min_uniform(m);  // the wave is whole again after the loop
         if (lane == 0) a.weplMin[k] = m;
     }
     if (blockIdx.x == 0 && w == 0) {
@@ -2521,8 +2521,8 @@ __device__ __forceinline__ void trace_partials_body(const uint3 blockIdx, const 
             mn = min(mn, a.tp.fi[i]);
             mx = max(mx, a.tp.fo[i]);
         }
-        mn = wave_min_i(mn);
-        mx = wave_max_i(mx);
+        mn = wave_min_uniform_i(mn);
+        mx = wave_max_uniform_i(mx);
         if (lane == 0) {
             a.bs->beamFirstInside = mn;
             a.bs->beamFirstOutside = mx;
