@@ -3109,7 +3109,11 @@ hipError_t launch_merge(const FanGeom &g, const BeamScalars *bs, const int *item
                         const int *tailItemR, const float *tailSlots, float *tailBev, int tailCap, const int *planeRM,
                         hipStream_t s) {
     const int bx = g.sx + 64, by = g.sy + 64;
-    const int tz = tailCap < 64 ? tailCap : 64;
+    // tail planes over tz block rows, each looping over planes tz apart. tz follows the beam's
+    // steps, clamped to 64 .. 128: the head beam's 200 tail planes then take two passes (merge
+    // 64 -> 56 us against 64 rows), while shallower beams, with fewer tail planes, keep 64 rows
+    // (more would only add rows that find no plane)
+    const int tz = std::min(tailCap, std::max(64, std::min(g.K, 128)));
     dispatch<MergeK>({g, bs, itemR, mainSlots, bev, tailItemR, tailSlots, tailBev, planeRM}, dim3(bx / 32, by / 8, g.K + tz), 0, s);
     return hipGetLastError();
 }
