@@ -130,16 +130,6 @@ __device__ __forceinline__ float row_min(float v) {
     v = dmin(v, std::integral_constant<int, 0x141>{});  // row_half_mirror
     return dmin(v, std::integral_constant<int, 0x140>{});  // row_mirror
 }
-// The minimum over each half-wave (lanes 0-31, 32-63), in every lane of the half: the rows'
-// minima and two lane reads per half. A half-wave may be inactive (its value is then unused).
-__device__ __forceinline__ float half_wave_min(float v) {
-    v = row_min(v);
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return (threadIdx.x & 32) ? fminf(r2, r3) : fminf(r0, r1);
-}
 __device__ __forceinline__ float wave_min_uniform(float v) {
     v = row_min(v);
     const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
@@ -418,7 +408,7 @@ struct TraceFixK {
 
 
 // The trace in one pass, for beams of up to 16 x CH segments (K <= 128 CH steps): a work-group
-// of 512 threads holds 32 rays, 16 threads per ray, each thread a run of the ray's 8-step
+// of 256 threads holds 16 rays, 16 threads per ray, each thread a run of the ray's 8-step
 // segments [s0, s1) (balanced: one or two per thread for K = 129). The segments are the ones of
 // the three-pass form and each runs the same steps (trace_step); their totals meet in LDS, where
 // each thread sums the carries of the segments before its own in segment order, in double, as
@@ -426,7 +416,7 @@ struct TraceFixK {
 // in their inlined copies). The beam's reductions leave as per-work-group partials (the
 // per-step minimum over the group's rays that leave material, its least entry and greatest
 // exit step), which the upload launch reduces: no atomics, and nothing to initialise first.
-constexpr int kTraceRays = 32, kTraceThreadsPerRay = 16;
+constexpr int kTraceRays = 16, kTraceThreadsPerRay = 16;
 
 template <int CH>
 __device__ __forceinline__ void trace_body(const uint3 blockIdx, FanGeom g, const float *__restrict__ ct, LutDev lut,
@@ -437,10 +427,10 @@ __device__ __forceinline__ void trace_body(const uint3 blockIdx, FanGeom g, cons
     const int nLut = lut.nDensity + lut.nSp;
     const int S = (g.K + kTraceSeg - 1) / kTraceSeg;
     float *s_dens = s_dyn, *s_sp = s_dyn + lut.nDensity;
-    float2 *s_seg = reinterpret_cast<float2 *>(s_dyn + (nLut + 3) / 4 * 4);  // [S][32] (cum, comp)
-    float *s_hu = reinterpret_cast<float *>(s_seg + S * kTraceRays);       // [S][32]
-    int *s_last = reinterpret_cast<int *>(s_hu + S * kTraceRays);          // [S][32]
-    int *s_cnt = s_last + S * kTraceRays;                                   // [32] entry step
+    float2 *s_seg = reinterpret_cast<float2 *>(s_dyn + (nLut + 3) / 4 * 4);  // [S][rays] (cum, comp)
+    float *s_hu = reinterpret_cast<float *>(s_seg + S * kTraceRays);       // [S][rays]
+    int *s_last = reinterpret_cast<int *>(s_hu + S * kTraceRays);          // [S][rays]
+    int *s_cnt = s_last + S * kTraceRays;                                   // [rays] entry step
     for (int i = threadIdx.x; i < lut.nDensity; i += blockDim.x) s_dens[i] = lut.density[i];
     for (int i = threadIdx.x; i < lut.nSp; i += blockDim.x) s_sp[i] = lut.sp[i];
     const int rl = threadIdx.x & (kTraceRays - 1), t = threadIdx.x / kTraceRays;
@@ -515,16 +505,17 @@ __device__ __forceinline__ void trace_body(const uint3 blockIdx, FanGeom g, cons
         }
     }
     // per step: the minimum WEPL over this group's rays that leave material (fo > 0), from the
-    // 32 lanes of each half-wave (one thread per ray for the step's segment)
+    // 16 lanes of each row (one thread per ray for the step's segment; 16 rays per work-group:
+    // 576 work-groups for the head beam spread over the 256 CUs more evenly than 288 of 32 rays)
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
         const int sg = s0 + c;
 #pragma unroll
         for (int j = 0; j < kTraceSeg; ++j) {
             const int k = sg * kTraceSeg + j;
-            // uniform over the half-wave: its 32 lanes share t
+            // uniform over the row: its 16 lanes share t
             if (sg >= s1 || k >= g.K) continue;
-            const float m = half_wave_min((rayIn && fov > 0) ? wv[c * kTraceSeg + j] + 0.0f : 1.0e20f);  // -0 -> +0
+            const float m = row_min((rayIn && fov > 0) ? wv[c * kTraceSeg + j] + 0.0f : 1.0e20f);  // -0 -> +0
             if (rl == 0) partMin[size_t(k) * partPitch + blockIdx.x] = m;
         }
     }
