@@ -2501,9 +2501,13 @@ __global__ __launch_bounds__(256) void k_transfer_many(const TransferK::Args *__
 __device__ __forceinline__ void trace_partials_body(const uint3 blockIdx, const uint3 gridDim, const UploadArgs &a) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = int(blockIdx.x) * 4 + w; k < a.K; k += int(gridDim.x) * 4) {
-        const float *row = a.tp.min + size_t(k) * a.tp.pitch;
+        // the row as float4s: the pitch is a multiple of 4 and the partials start 256-byte aligned
+        const float4 *row = reinterpret_cast<const float4 *>(a.tp.min + size_t(k) * a.tp.pitch);
         float m = 1.0e20f;
-        for (int i = lane; i < a.tp.pitch; i += 64) m = fminf(m, row[i]);
+        for (int i = lane; i < a.tp.pitch / 4; i += 64) {
+            const float4 v = row[i];
+            m = fminf(m, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+        }
         m = wave_min_uniform(m);  // the wave is whole again after the loop
         if (lane == 0) a.weplMin[k] = m;
     }
