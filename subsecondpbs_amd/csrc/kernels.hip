@@ -2254,10 +2254,17 @@ __device__ __forceinline__ void superpose_rows_body(const uint3 blockIdx, const 
                 if (unsigned(hx) < unsigned(HW)) {
                     const int y0 = o / BW - dy0;
                     float sum = part[j];
+                    if (y0 >= 3 && y0 <= RM) {
+                        // rows y0 - g4 of the four groups, all within reach: a constant stride
+                        const float *col = A + y0 * HWp + hx;
 #pragma unroll
-                    for (int g4 = 0; g4 < 4; ++g4) {
-                        const int ady = abs(y0 - g4);
-                        if (ady <= RM) sum += A[(g4 * G + ady) * HWp + hx];
+                        for (int g4 = 0; g4 < 4; ++g4) sum += col[g4 * (G - 1) * HWp];
+                    } else {
+#pragma unroll
+                        for (int g4 = 0; g4 < 4; ++g4) {
+                            const int ady = abs(y0 - g4);
+                            if (ady <= RM) sum += A[(g4 * G + ady) * HWp + hx];
+                        }
                     }
                     part[j] = sum;
                 }
