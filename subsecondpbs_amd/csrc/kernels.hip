@@ -877,6 +877,18 @@ __device__ __forceinline__ float pow_hw(float x, float e) { return __builtin_amd
 
 // Bilinear lookup in a table row pair staged in LDS: the same arithmetic as lut2d_clamp for
 // a fixed row coordinate (rows r0 = row y0, r1 = row y1, fraction ay).
+// The same with the two rows interleaved, (r0[i], r1[i]) one float2: two 8-byte reads.
+__device__ __forceinline__ float lut_rows2(const float2 *__restrict__ r, int nx, float ay, float xb) {
+    const float fx = floorf(xb);
+    const float ax = xb - fx;
+    int x0 = int(fx);
+    const int x1 = min(max(x0 + 1, 0), nx - 1);
+    x0 = min(max(x0, 0), nx - 1);
+    const float2 a = r[x0], b = r[x1];
+    const float v0 = (1.0f - ax) * a.x + ax * b.x;
+    const float v1 = (1.0f - ax) * a.y + ax * b.y;
+    return (1.0f - ay) * v0 + ay * v1;
+}
 __device__ __forceinline__ float lut_rows(const float *__restrict__ r0, const float *__restrict__ r1, int nx,
                                           float ay, float xb) {
     const float fx = floorf(xb);
@@ -924,12 +936,12 @@ __device__ __forceinline__ void fill_body(const uint3 blockIdx, const uint3 grid
     const int nS = lut.nS, nRrl = lut.nRrl;
     float *s_rrl = s_lds;
     float *s_vw = s_rrl + nRrl, *s_mp = s_vw + g.K;
-    float *c0 = s_mp + g.K, *c1 = c0 + nS;
-    float *nw0 = c1 + nS, *nw1 = nw0 + nS, *ns0 = nw1 + nS, *ns1 = ns0 + nS;
+    // the layer's two CIDD rows interleaved, 8-byte aligned (one float of padding at most)
+    float2 *c01 = reinterpret_cast<float2 *>(s_mp + g.K + ((nRrl + 2 * g.K) & 1));
+    float *nw0 = reinterpret_cast<float *>(c01 + nS), *nw1 = nw0 + nS, *ns0 = nw1 + nS, *ns1 = ns0 + nS;
     for (int i = threadIdx.x; i < nRrl; i += blockDim.x) s_rrl[i] = lut.rrl[i];
     for (int i = threadIdx.x; i < nS; i += blockDim.x) {
-        c0[i] = lut.cidd[size_t(ey0) * nS + i];
-        c1[i] = lut.cidd[size_t(ey1) * nS + i];
+        c01[i] = make_float2(lut.cidd[size_t(ey0) * nS + i], lut.cidd[size_t(ey1) * nS + i]);
         if (HALO) {
             nw0[i] = hf.nucWeightLut[size_t(ey0) * nS + i];
             nw1[i] = hf.nucWeightLut[size_t(ey1) * nS + i];
@@ -987,7 +999,7 @@ __device__ __forceinline__ void fill_body(const uint3 blockIdx, const uint3 grid
     // one step of the recurrence for this ray (idd_sigma.cu:130-230); outputs are zero / inf
     // once the ray has stopped and before its first inside step
     auto step = [&](unsigned k, float sp, float rho, float &res, float &rs, float &nres, float &nrs) {
-        const float cd = lut_rows(c0, c1, nS, ay, sp * L.scaleFact);
+        const float cd = lut_rows2(c01, nS, ay, sp * L.scaleFact);
         const float mid = 0.5f * (sp + spOld);
         const bool pre = sp < L.peakDepth;
         const float resE = fc.rangeCoef * pow_hw(L.peakDepth - mid, fc.rangeExp);
@@ -2949,7 +2961,7 @@ hipError_t launch_fill(const FanGeom &g, const LutDev &lut, const LayerHost *lh,
                        const int *fo, float2 *iddSigma, int *firstPassive, const FillConsts &fc, const HaloFill &hf,
                        int *tileCode, hipStream_t s) {
     const int nt = (g.sx / 32) * (g.sy / 8);
-    const size_t lds = sizeof(float) * (size_t(lut.nRrl) + 2 * size_t(g.K) + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
+    const size_t lds = sizeof(float) * (size_t(lut.nRrl) + 2 * size_t(g.K) + 1 + size_t(hf.rayToNuc ? 6 : 2) * lut.nS);
     if (hf.rayToNuc)
         dispatch<FillK<true>>({g, lut, lh, ls, bs, density, wepl, rayW, fi, fo, iddSigma, firstPassive, fc, hf, tileCode}, dim3(nt, g.numLayers), lds, s);
     else
